@@ -1,0 +1,130 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this,
+and only as the checker / CPU baseline.  Keys follow the reference's 1-based
+convention: generators 1..n, boundary plane p is n+p (src/extended.jl:27-34).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+def build(force: bool = False) -> str:
+    src = os.path.join(_HERE, "hv_oracle.c")
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.orc_ctx_create.argtypes = [C.c_int, C.c_int64, P(C.c_double), C.c_int, P(C.c_double), P(C.c_double), P(C.c_void_p)]
+        L.orc_ctx_free.argtypes = [C.c_void_p]
+        L.orc_voronoi.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_uint64, P(C.c_void_p)]
+        L.orc_mesh_nvert.argtypes = [C.c_void_p]
+        L.orc_mesh_nvert.restype = C.c_int64
+        L.orc_mesh_maxsig.argtypes = [C.c_void_p]
+        L.orc_mesh_export.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_double)]
+        L.orc_mesh_stats.argtypes = [C.c_void_p, P(C.c_int64)]
+        L.orc_mesh_free.argtypes = [C.c_void_p]
+        L.orc_walkray.argtypes = [C.c_void_p, P(C.c_int64), C.c_int, P(C.c_double), C.c_int64,
+                                  P(C.c_int64), P(C.c_int), P(C.c_double), P(C.c_double)]
+        L.orc_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+STAT_NAMES = ["walkrays", "nn_calls", "inrange_calls", "descents", "tolerance_breach",
+              "irreparable", "duplicates", "fraud", "unbounded", "hp_full_mode"]
+
+
+class Oracle:
+    """FP64 CPU restatement of VoronoiGeometry's mesh step (RCNonGeneralHP, VI_KD)."""
+
+    def __init__(self, xs: np.ndarray, base: np.ndarray | None = None, normal: np.ndarray | None = None):
+        L = lib()
+        self.xs = np.ascontiguousarray(xs, dtype=np.float64)
+        self.n, self.d = self.xs.shape
+        if base is None:
+            base = np.zeros((0, self.d)); normal = np.zeros((0, self.d))
+        self.base = np.ascontiguousarray(base, dtype=np.float64)
+        self.normal = np.ascontiguousarray(normal, dtype=np.float64)
+        self.np = self.base.shape[0]
+        h = C.c_void_p()
+        rc = L.orc_ctx_create(self.d, self.n, _dp(self.xs), self.np, _dp(self.base), _dp(self.normal), C.byref(h))
+        if rc != 0:
+            raise ValueError(L.orc_last_error().decode())
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_ctx_free(self._h)
+            self._h = None
+
+    def voronoi(self, cells: tuple[int, int] | None = None, seed: int = 0):
+        """Returns (keys[V, d+1] 1-based sorted, coords[V, d], stats dict)."""
+        L = lib()
+        b, e = (0, self.n) if cells is None else cells
+        m = C.c_void_p()
+        rc = L.orc_voronoi(self._h, b, e, seed, C.byref(m))
+        if rc != 0:
+            raise RuntimeError(f"oracle walk failed ({rc}): {L.orc_last_error().decode()}")
+        try:
+            nv = L.orc_mesh_nvert(m)
+            ms = L.orc_mesh_maxsig(m)
+            keys = np.zeros((nv, ms), dtype=np.int64)
+            coords = np.zeros((nv, self.d), dtype=np.float64)
+            L.orc_mesh_export(m, _ip(keys), _dp(coords))
+            st = np.zeros(10, dtype=np.int64)
+            L.orc_mesh_stats(m, _ip(st))
+        finally:
+            L.orc_mesh_free(m)
+        return keys, coords, dict(zip(STAT_NAMES, st.tolist()))
+
+    def walkray(self, sig, drop: int, r, cell: int):
+        """One reference walkray; returns (new_key, coords, u) or None if unbounded."""
+        L = lib()
+        sig = np.ascontiguousarray(sig, dtype=np.int64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        out_sig = np.zeros(64, dtype=np.int64)
+        out_len = C.c_int(0)
+        out_r = np.zeros(self.d)
+        out_u = np.zeros(self.d)
+        rc = L.orc_walkray(self._h, _ip(sig), drop, _dp(r), cell, _ip(out_sig), C.byref(out_len), _dp(out_r), _dp(out_u))
+        if rc == 1:
+            return None
+        if rc != 0:
+            raise RuntimeError(L.orc_last_error().decode())
+        return out_sig[: out_len.value].copy(), out_r, out_u
+
+
+def sort_rows(keys: np.ndarray) -> np.ndarray:
+    """Canonical order of a key table: lexicographic over the (already sorted) rows."""
+    keys = np.asarray(keys)
+    if len(keys) == 0:
+        return keys
+    order = np.lexsort(keys.T[::-1])
+    return keys[order]
