@@ -1,0 +1,226 @@
+// hv_api.hip -- the extern "C" surface of libhv.so (declared in include/hv.h).
+#include "hv_internal.h"
+#include <cstring>
+#include <string>
+
+// ---------------------------------------------------------------- device pool
+DevPool::~DevPool() {
+    (void)hipSetDevice(device);
+    for (auto& kv : bufs)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+}
+
+void* DevPool::get(const std::string& name, size_t bytes, bool keep_contents, hipStream_t s) {
+    DBuf& b = bufs[name];
+    if (bytes <= b.n && b.p) return b.p;
+    if (bytes == 0) return b.p;
+    size_t want = bytes + bytes / 4 + 256;
+    void* p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (keep_contents && b.p && b.n) {
+        if (hipMemcpyAsync(p, b.p, b.n, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+            (void)hipFree(p);
+            return nullptr;
+        }
+        (void)hipStreamSynchronize(s);
+    }
+    if (b.p) {
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(b.p);
+    }
+    b.p = p;
+    b.n = want;
+    return p;
+}
+
+DBuf DevPool::take(const std::string& name) {
+    DBuf b;
+    auto it = bufs.find(name);
+    if (it != bufs.end()) {
+        b = it->second;
+        bufs.erase(it);
+    }
+    return b;
+}
+
+void DevPool::give(const std::string& name, DBuf b) {
+    if (!b.p) return;
+    DBuf& cur = bufs[name];
+    if (!cur.p) {
+        cur = b;
+    } else if (cur.n < b.n) {
+        (void)hipFree(cur.p);
+        cur = b;
+    } else {
+        (void)hipFree(b.p);
+    }
+}
+
+int hv_fail(hv_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------- dispatch helpers
+#define HV_DISPATCH(D, fn, ...)                               \
+    switch (D) {                                              \
+        case 2: return fn<2>(__VA_ARGS__);                    \
+        case 3: return fn<3>(__VA_ARGS__);                    \
+        case 4: return fn<4>(__VA_ARGS__);                    \
+        case 5: return fn<5>(__VA_ARGS__);                    \
+        case 6: return fn<6>(__VA_ARGS__);                    \
+        default: return HV_ERR_UNSUPPORTED;                   \
+    }
+
+extern "C" {
+
+int hv_create(hv_ctx** out, int device_ordinal) {
+    if (!out) return HV_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return HV_ERR_DEVICE;
+    if (device_ordinal < 0 || device_ordinal >= ndev) return HV_ERR_ARG;
+    if (hipSetDevice(device_ordinal) != hipSuccess) return HV_ERR_DEVICE;
+    hv_ctx* c = new hv_ctx();
+    c->device = device_ordinal;
+    c->pool = std::make_shared<DevPool>();
+    c->pool->device = device_ordinal;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return HV_ERR_DEVICE;
+    }
+    for (int i = 0; i < 4; ++i)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+            delete c;
+            return HV_ERR_DEVICE;
+        }
+    *out = c;
+    return HV_OK;
+}
+
+int hv_destroy(hv_ctx* c) {
+    if (!c) return HV_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 4; ++i) (void)hipEventDestroy(c->ev[i]);
+    (void)hipStreamDestroy(c->stream);
+    c->pool.reset();
+    delete c;
+    return HV_OK;
+}
+
+const char* hv_last_error(const hv_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+static int set_points_common(hv_ctx* c, int d, int64_t n, const double* xs, hipMemcpyKind kind) {
+    if (!c || !xs) return hv_fail(c, HV_ERR_ARG, "null argument");
+    if (d < 2 || d > 6) return hv_fail(c, HV_ERR_UNSUPPORTED, "dimension must be in 2..6");
+    if (n <= d) return hv_fail(c, HV_ERR_FEW_POINTS, "There are not enough points to create a Voronoi tessellation");
+    if (n >= (1ll << 30)) return hv_fail(c, HV_ERR_UNSUPPORTED, "too many generators for 32-bit keys");
+    (void)hipSetDevice(c->device);
+    void* p = c->pool->get("xin", sizeof(double) * d * n, false, c->stream);
+    if (!p) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+    if (hipMemcpyAsync(p, xs, sizeof(double) * d * n, kind, c->stream) != hipSuccess)
+        return hv_fail(c, HV_ERR_DEVICE, "copy of the points failed");
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return hv_fail(c, HV_ERR_DEVICE, "stream sync failed");
+    if (c->D != d) {
+        c->P = 0;
+        c->pbase.clear();
+        c->pnormal.clear();
+        c->pbc.clear();
+    }
+    c->D = d;
+    c->N = n;
+    return HV_OK;
+}
+
+int hv_set_points(hv_ctx* c, int d, int64_t n, const double* xs) {
+    return set_points_common(c, d, n, xs, hipMemcpyHostToDevice);
+}
+
+int hv_set_points_device(hv_ctx* c, int d, int64_t n, const double* xs_dev) {
+    return set_points_common(c, d, n, xs_dev, hipMemcpyDeviceToDevice);
+}
+
+int hv_set_boundary(hv_ctx* c, int nplanes, const double* base, const double* normal, const int16_t* bc) {
+    if (!c) return HV_ERR_ARG;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "hv_set_points must come first");
+    if (nplanes < 0 || (nplanes > 0 && (!base || !normal))) return hv_fail(c, HV_ERR_ARG, "bad planes");
+    const int D = c->D;
+    c->P = nplanes;
+    c->pbase.assign(base, base + (size_t)nplanes * D);
+    c->pnormal.assign(normal, normal + (size_t)nplanes * D);
+    c->pbc.assign(nplanes, 0);
+    if (bc)
+        for (int i = 0; i < nplanes; ++i) c->pbc[i] = bc[i];
+    return HV_OK;
+}
+
+int hv_build_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
+    if (!c || !out) return hv_fail(c, HV_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    (void)hipSetDevice(c->device);
+    HV_DISPATCH(c->D, hv_build_mesh_impl, c, o, out)
+}
+
+int hv_mesh_counts(const hv_mesh* m, int64_t* nvert, int32_t* max_sig, int64_t* nrays_unbounded) {
+    if (!m) return HV_ERR_ARG;
+    if (nvert) *nvert = m->nv;
+    if (max_sig) *max_sig = m->D + 1;
+    if (nrays_unbounded) *nrays_unbounded = m->stats.unbounded;
+    return HV_OK;
+}
+
+int hv_mesh_export(const hv_mesh* m, int64_t* sig, double* coords) {
+    if (!m || !sig || !coords) return HV_ERR_ARG;
+    (void)hipSetDevice(m->device);
+    HV_DISPATCH(m->D, hv_mesh_export_impl, m, sig, coords)
+}
+
+int hv_mesh_stats(const hv_mesh* m, hv_stats_t* out) {
+    if (!m || !out) return HV_ERR_ARG;
+    *out = m->stats;
+    return HV_OK;
+}
+
+int hv_mesh_free(hv_mesh* m) {
+    if (!m) return HV_ERR_ARG;
+    (void)hipSetDevice(m->device);
+    m->pool->give("vkey", m->key);
+    m->pool->give("vx", m->x);
+    m->pool->give("ids", m->ids);
+    delete m;
+    return HV_OK;
+}
+
+int hv_raycast_batch(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
+                     const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status) {
+    if (!c) return HV_ERR_ARG;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    if (nrays < 0 || (nrays > 0 && (!sig || !drop || !r || !out_sig || !out_r || !out_t || !out_status)))
+        return hv_fail(c, HV_ERR_ARG, "null argument");
+    (void)hipSetDevice(c->device);
+    HV_DISPATCH(c->D, hv_raycast_batch_impl, c, precision, nrays, sig, drop, r, out_sig, out_r, out_t, out_status)
+}
+
+int hv_nn_halfspace_batch(hv_ctx* c, int64_t nq, const double* q, const double* u, const double* cc,
+                          int64_t* idx, double* dist) {
+    if (!c) return HV_ERR_ARG;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    if (nq < 0 || (nq > 0 && (!q || !u || !cc || !idx || !dist))) return hv_fail(c, HV_ERR_ARG, "null argument");
+    (void)hipSetDevice(c->device);
+    HV_DISPATCH(c->D, hv_nn_halfspace_impl, c, nq, q, u, cc, idx, dist)
+}
+
+int hv_inrange_batch(hv_ctx* c, int64_t nq, const double* q, const double* radius, int64_t* ptr, int64_t* idx) {
+    if (!c) return HV_ERR_ARG;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    if (nq < 0 || (nq > 0 && (!q || !radius || !ptr))) return hv_fail(c, HV_ERR_ARG, "null argument");
+    (void)hipSetDevice(c->device);
+    HV_DISPATCH(c->D, hv_inrange_impl, c, nq, q, radius, ptr, idx)
+}
+
+}  // extern "C"

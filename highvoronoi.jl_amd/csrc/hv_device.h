@@ -1,0 +1,192 @@
+// hv_device.h -- device-side data layout and helpers of the gfx950 raycast walk.
+//
+// HBM layout (all tables live for one hv_build_mesh call):
+//   generators  xo[k*N + g]   SoA FP64, original index order (gathers by key element)
+//               xs[k*N + j]   SoA FP64, Morton-sorted order (leaf scans), xx[j] = |x_j|^2
+//               sidx[j]       sorted position -> original index
+//   tree        complete binary tree in heap order over leaves of L sorted points:
+//               node 1 is the root, node P2+l is leaf l; boxes blo/bhi[k*NN + node]
+//   vertices    key[v*(D+1) + j] sorted int32 (generator g or N+p for plane p),
+//               x[v*D + k] FP64; ids are provisional: one slot per ray of a level
+//   vertex hash int32 slots (-1 empty) holding vertex ids, probed linearly
+//   edge hash   uint64 slots: ((v << 4) | drop) + 1 of the first vertex that
+//               registered the edge, bit 62 set once its second end point is known
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HV_MAXD 6
+#define HV_EDGE_DONE (1ull << 62)
+
+struct DevGeo {
+    int N, P, L, nleaf, P2, NN;
+    const double* __restrict__ xo;
+    const double* __restrict__ xs;
+    const double* __restrict__ xx;
+    const int* __restrict__ sidx;
+    const double* __restrict__ pn;   // P x D normals (row per plane)
+    const double* __restrict__ ph;   // P offsets <n_p, b_p>
+    const double* __restrict__ blo;  // D x NN
+    const double* __restrict__ bhi;
+    double dlo[HV_MAXD], dhi[HV_MAXD];  // domain box (Morton frame)
+};
+
+struct DevRays {
+    int* v;          // source vertex id
+    int* drop;       // position in the source key of the generator walked away from
+    double* u;       // D per ray
+    double* c;       // half-space threshold (src/raycast.jl:802-805)
+    double* du;      // orthogonality defect of u (src/edgeiteratebase.jl:64-74)
+    double* tinit;   // certificate radius parameter for the first search
+    uint64_t* code;  // Morton code of the expected end point
+};
+
+__device__ __forceinline__ uint64_t hv_mix64(uint64_t z) {
+    z ^= z >> 33; z *= 0xff51afd7ed558ccdULL;
+    z ^= z >> 33; z *= 0xc4ceb9fe1a85ec53ULL;
+    z ^= z >> 33;
+    return z;
+}
+
+template <int K>
+__device__ __forceinline__ uint64_t hv_hash_ints(const int* k) {
+    uint64_t h = 0x9E3779B97F4A7C15ULL;
+#pragma unroll
+    for (int i = 0; i < K; ++i) h = hv_mix64(h ^ (uint64_t)(uint32_t)k[i]);
+    return h;
+}
+
+// edge of vertex key `key` (D+1 entries) without position `pos`
+// (static indices + selects: a runtime index into a register array would go to scratch)
+template <int D>
+__device__ __forceinline__ void hv_edge_of(const int* key, int pos, int* e) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) e[j] = (j < pos) ? key[j] : key[j + 1];
+}
+
+template <int D>
+__device__ __forceinline__ int hv_key_at(const int* key, int pos) {
+    int s = key[0];
+#pragma unroll
+    for (int j = 1; j <= D; ++j) s = (j == pos) ? key[j] : s;
+    return s;
+}
+
+// Row j (0..D-1) of the circumcentre system of a sorted key, relative to x0 = x_{key[0]}:
+//   generator e:  2 <x_e - x0, y> = |x_e - x0|^2        plane p:  <n_p, y> = h_p - <n_p, x0>
+template <int D>
+__device__ __forceinline__ void hv_vertex_row(const DevGeo& g, int e, const double* x0, double* row, double& rhs) {
+    const int N = g.N;
+    if (e < N) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double dk = g.xo[(int64_t)k * N + e] - x0[k];
+            row[k] = 2.0 * dk;
+            s = fma(dk, dk, s);
+        }
+        rhs = s;
+    } else {
+        const int p = e - N;
+        double s = g.ph[p];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double nk = g.pn[p * D + k];
+            row[k] = nk;
+            s = fma(-nk, x0[k], s);
+        }
+        rhs = s;
+    }
+}
+
+// Circumcentre of a sorted key (D+1 elements; generators first, planes N+p after).
+// The constraint rows a_j are orthonormalised (modified Gram-Schmidt, two passes) while
+// the right-hand side follows the same transformation, so y = sum_j beta_j q_j is the
+// point of the affine set {a_j . y = b_j} -- the projection the reference applies in
+// prepare_vertex_calculation / vertex_calculation_hp (src/raycast.jl:633-707).  One step
+// of iterative refinement uses the rows rebuilt from memory.  False if singular.
+template <int D>
+__device__ __forceinline__ bool hv_vertex_coords(const DevGeo& g, const int* key, double* out) {
+    const int N = g.N;
+    double x0[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) x0[k] = g.xo[(int64_t)k * N + key[0]];
+    double Q[D][D], Rl[D][D], inv[D], beta[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        double rhs;
+        hv_vertex_row<D>(g, key[j + 1], x0, Q[j], rhs);
+#pragma unroll
+        for (int i = 0; i < D; ++i) Rl[j][i] = 0.0;
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+            for (int i = 0; i < j; ++i) {
+                double a = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) a = fma(Q[j][k], Q[i][k], a);
+#pragma unroll
+                for (int k = 0; k < D; ++k) Q[j][k] = fma(-a, Q[i][k], Q[j][k]);
+                rhs = fma(-a, beta[i], rhs);
+                Rl[j][i] += a;
+            }
+        }
+        double nn = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) nn = fma(Q[j][k], Q[j][k], nn);
+        if (!(nn > 0.0)) return false;
+        const double s = 1.0 / sqrt(nn);
+        inv[j] = s;
+#pragma unroll
+        for (int k = 0; k < D; ++k) Q[j][k] *= s;
+        beta[j] = rhs * s;
+    }
+    double y[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) y[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+#pragma unroll
+        for (int k = 0; k < D; ++k) y[k] = fma(beta[j], Q[j][k], y[k]);
+    // refinement: residual r_j = b_j - a_j . y, transformed like b
+    double bet2[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        double row[D], rhs;
+        hv_vertex_row<D>(g, key[j + 1], x0, row, rhs);
+#pragma unroll
+        for (int k = 0; k < D; ++k) rhs = fma(-row[k], y[k], rhs);
+#pragma unroll
+        for (int i = 0; i < j; ++i) rhs = fma(-Rl[j][i], bet2[i], rhs);
+        bet2[j] = rhs * inv[j];
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+#pragma unroll
+        for (int k = 0; k < D; ++k) y[k] = fma(bet2[j], Q[j][k], y[k]);
+#pragma unroll
+    for (int k = 0; k < D; ++k) out[k] = x0[k] + y[k];
+    return true;
+}
+
+// Morton code of a point (bits interleaved, 60/D bits per coordinate) in the domain box.
+template <int D>
+__device__ __forceinline__ uint64_t hv_morton(const DevGeo& g, const double* p) {
+    constexpr int B = 60 / D;
+    constexpr double S = (double)((1ull << B) - 1);
+    uint32_t q[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        double w = g.dhi[k] - g.dlo[k];
+        double f = (p[k] - g.dlo[k]) / (w > 0 ? w : 1.0);
+        f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+        q[k] = (uint32_t)(f * S);
+    }
+    uint64_t code = 0;
+#pragma unroll
+    for (int b = B - 1; b >= 0; --b) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) code = (code << 1) | ((q[k] >> b) & 1u);
+    }
+    return code;
+}

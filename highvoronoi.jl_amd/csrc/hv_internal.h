@@ -1,0 +1,65 @@
+// hv_internal.h -- host-side structures shared by the C ABI and the walk driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+#include "../../include/hv.h"
+
+// Device buffer pool: tables survive between builds so a steady-state build does no
+// hipMalloc; a finished mesh takes ownership of its vertex tables and returns them.
+struct DBuf {
+    void* p = nullptr;
+    size_t n = 0;
+};
+
+struct DevPool {
+    int device = 0;
+    std::map<std::string, DBuf> bufs;
+    ~DevPool();
+    // returns nullptr (and keeps the old buffer) when hipMalloc fails
+    void* get(const std::string& name, size_t bytes, bool keep_contents = false, hipStream_t s = nullptr);
+    DBuf take(const std::string& name);
+    void give(const std::string& name, DBuf b);
+};
+
+struct hv_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int D = 0;
+    int64_t N = 0;
+    std::shared_ptr<DevPool> pool;
+    // boundary (host copies)
+    int P = 0;
+    std::vector<double> pbase, pnormal;
+    std::vector<int16_t> pbc;
+    std::string err;
+    hipEvent_t ev[4];
+};
+
+struct hv_mesh {
+    std::shared_ptr<DevPool> pool;
+    int device = 0;
+    int D = 0;
+    int64_t N = 0;
+    int64_t nv = 0;
+    DBuf key, x, ids;   // vertex table (provisional ids) and the list of live ids
+    hv_stats_t stats{};
+};
+
+int hv_fail(hv_ctx* c, int code, const std::string& msg);
+
+template <int D>
+int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out);
+template <int D>
+int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
+                          const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status);
+template <int D>
+int hv_nn_halfspace_impl(hv_ctx* c, int64_t nq, const double* q, const double* u, const double* cc,
+                         int64_t* idx, double* dist);
+template <int D>
+int hv_inrange_impl(hv_ctx* c, int64_t nq, const double* q, const double* radius, int64_t* ptr, int64_t* idx);
+template <int D>
+int hv_mesh_export_impl(const hv_mesh* m, int64_t* sig, double* coords);

@@ -1,0 +1,1302 @@
+// hv_walk.hip -- gfx950 kernels and host driver of the device-resident raycast walk.
+//
+// What the reference does one ray at a time on the CPU (src/sysvoronoi.jl:384-525,
+// src/raycast.jl:125-164,794-970), this file does as a breadth-first walk over the
+// whole vertex graph at once:
+//   level k:  register the d+1 edges of every vertex found in level k-1 in a device
+//             edge hash; an edge is walked only when exactly one end point is known
+//             (the cell1/cell2 rule of pushedge!, src/edgehashing.jl:66-115);
+//             emit one ray per such edge (direction of u_qr, src/tools.jl:773-790);
+//             sort rays by the Morton code of their expected end point;
+//             raycast in packets of 64 rays per wavefront over a kd-style tree of
+//             Morton-ordered generator leaves: argmin of the equidistance parameter t
+//             with the reference's winner rule (src/raycast.jl:877-916);
+//             insert the new keys into a device vertex hash; the winners are level k.
+// Seeds are the 2^d corners of the box: each is the vertex {nearest generator} u
+// {the d planes through it}.
+#include "hv_device.h"
+#include "hv_internal.h"
+#include <hipcub/hipcub.hpp>
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#define HVCK(call)                                                                 \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return hv_fail(c, HV_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ---------------------------------------------------------------------------------
+// points and tree
+
+template <int D>
+__global__ void k_points_in(const double* __restrict__ xin, int N, DevGeo g, double* __restrict__ xo,
+                            uint64_t* __restrict__ code, int* __restrict__ idx, int* __restrict__ bad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double p[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        p[k] = xin[(int64_t)i * D + k];
+        xo[(int64_t)k * N + i] = p[k];
+    }
+    // check_boundary (src/boundary.jl:427-441): <b_p - x, n_p> >= 0 for every plane
+    bool ok = true;
+    for (int q = 0; q < g.P; ++q) {
+        double s = g.ph[q];
+#pragma unroll
+        for (int k = 0; k < D; ++k) s -= g.pn[q * D + k] * p[k];
+        ok = ok && (s >= 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) ok = ok && isfinite(p[k]);
+    if (!ok) atomicAdd(bad, 1);
+    code[i] = hv_morton<D>(g, p);
+    idx[i] = i;
+}
+
+template <int D>
+__global__ void k_points_sorted(const double* __restrict__ xo, const int* __restrict__ idx, int N,
+                                double* __restrict__ xs, double* __restrict__ xx, int* __restrict__ sidx) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    int gi = idx[j];
+    sidx[j] = gi;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        double v = xo[(int64_t)k * N + gi];
+        xs[(int64_t)k * N + j] = v;
+        s = fma(v, v, s);
+    }
+    xx[j] = s;
+}
+
+template <int D>
+__global__ void k_leaf_boxes(DevGeo g, double* __restrict__ blo, double* __restrict__ bhi) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= g.P2) return;
+    int node = g.P2 + l;
+    int j0 = l * g.L, j1 = min(j0 + g.L, g.N);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        double lo = 1e300, hi = -1e300;
+        for (int j = j0; j < j1; ++j) {
+            double v = g.xs[(int64_t)k * g.N + j];
+            lo = fmin(lo, v);
+            hi = fmax(hi, v);
+        }
+        blo[(int64_t)k * g.NN + node] = lo;
+        bhi[(int64_t)k * g.NN + node] = hi;
+    }
+}
+
+template <int D>
+__global__ void k_node_boxes(int first, int count, int NN, double* __restrict__ blo, double* __restrict__ bhi) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int node = first + t;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        blo[(int64_t)k * NN + node] = fmin(blo[(int64_t)k * NN + 2 * node], blo[(int64_t)k * NN + 2 * node + 1]);
+        bhi[(int64_t)k * NN + node] = fmax(bhi[(int64_t)k * NN + 2 * node], bhi[(int64_t)k * NN + 2 * node + 1]);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// seeds: one block per box corner, brute-force nearest generator (ties -> smaller index)
+
+template <int D>
+__global__ __launch_bounds__(256) void k_seed_corners(DevGeo g, int* __restrict__ vkey, double* __restrict__ vx,
+                                                      int64_t base, int* __restrict__ status) {
+    __shared__ double sd[256];
+    __shared__ int si[256];
+    const int corner = blockIdx.x;
+    double cp[D];
+    int planes[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        bool hi = (corner >> k) & 1;
+        cp[k] = hi ? g.dhi[k] : g.dlo[k];
+        planes[k] = g.N + 2 * k + (hi ? 0 : 1);   // plane 2k: x_k = hi, 2k+1: x_k = lo
+    }
+    double best = 1e300;
+    int bi = 0x7fffffff;
+    for (int j = threadIdx.x; j < g.N; j += blockDim.x) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            double t = g.xo[(int64_t)k * g.N + j] - cp[k];
+            s = fma(t, t, s);
+        }
+        if (s < best || (s == best && j < bi)) { best = s; bi = j; }
+    }
+    sd[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            double o = sd[threadIdx.x + w];
+            int oi = si[threadIdx.x + w];
+            if (o < sd[threadIdx.x] || (o == sd[threadIdx.x] && oi < si[threadIdx.x])) {
+                sd[threadIdx.x] = o;
+                si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int64_t id = base + corner;
+        vkey[id * (D + 1)] = si[0];
+        // planes are already ascending in k
+#pragma unroll
+        for (int k = 0; k < D; ++k) vkey[id * (D + 1) + 1 + k] = planes[k];
+#pragma unroll
+        for (int k = 0; k < D; ++k) vx[id * D + k] = cp[k];
+        status[corner] = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// vertex hash insert: winners of a level (status > 0) become live vertices
+
+template <int D>
+__device__ __forceinline__ bool keys_equal(const int* __restrict__ a, const int* __restrict__ b) {
+    bool eq = true;
+#pragma unroll
+    for (int j = 0; j <= D; ++j) eq = eq && (a[j] == b[j]);
+    return eq;
+}
+
+template <int D>
+__global__ void k_insert(const int* __restrict__ status, int n, const int* __restrict__ vkey, int64_t base,
+                         int* __restrict__ htab, uint64_t hmask, uint8_t* __restrict__ win,
+                         unsigned long long* __restrict__ ctr) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    win[t] = 0;
+    if (status[t] <= 0) return;
+    const int id = (int)(base + t);
+    const int* key = vkey + (int64_t)id * (D + 1);
+    uint64_t h = hv_hash_ints<D + 1>(key) & hmask;
+    for (uint64_t probe = 0; probe <= hmask; ++probe) {
+        int cur = htab[h];
+        if (cur < 0) {
+            int old = atomicCAS(&htab[h], -1, id);
+            if (old == -1) { win[t] = 1; return; }
+            cur = old;
+        }
+        if (keys_equal<D>(key, vkey + (int64_t)cur * (D + 1))) { atomicAdd(&ctr[0], 1ull); return; }
+        h = (h + 1) & hmask;
+    }
+    atomicAdd(&ctr[1], 1ull);  // table full
+}
+
+template <int D>
+__global__ void k_rehash_vertices(const int* __restrict__ ids, int64_t n, const int* __restrict__ vkey,
+                                  int* __restrict__ htab, uint64_t hmask) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int id = ids[t];
+    uint64_t h = hv_hash_ints<D + 1>(vkey + (int64_t)id * (D + 1)) & hmask;
+    for (;;) {
+        if (atomicCAS(&htab[h], -1, id) == -1) return;
+        h = (h + 1) & hmask;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// edges: registration (pushedge! semantics) and ray emission
+
+template <int D>
+__global__ void k_register(const int* __restrict__ frontier, int nf, const int* __restrict__ vkey, int N,
+                           unsigned long long* __restrict__ etab, uint64_t emask, int64_t* __restrict__ reg,
+                           unsigned long long* __restrict__ ctr) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf * (D + 1)) return;
+    const int fi = t / (D + 1), pos = t % (D + 1);
+    const int v = frontier[fi];
+    const int* key = vkey + (int64_t)v * (D + 1);
+    int e[D];
+    hv_edge_of<D>(key, pos, e);
+    if (e[0] >= N) { reg[t] = -1; return; }   // an edge of planes only is a corner, not a line
+    const unsigned long long mine = ((((unsigned long long)v) << 4) | (unsigned)pos) + 1ull;
+    uint64_t h = hv_hash_ints<D>(e) & emask;
+    for (uint64_t probe = 0; probe <= emask; ++probe) {
+        unsigned long long cur = etab[h];
+        if (cur == 0ull) {
+            unsigned long long old = atomicCAS(&etab[h], 0ull, mine);
+            if (old == 0ull) { reg[t] = (int64_t)h; atomicAdd(&ctr[0], 1ull); return; }
+            cur = old;
+        }
+        const unsigned long long raw = (cur & ~HV_EDGE_DONE) - 1ull;
+        const int v2 = (int)(raw >> 4), p2 = (int)(raw & 15ull);
+        int e2[D];
+        hv_edge_of<D>(vkey + (int64_t)v2 * (D + 1), p2, e2);
+        bool eq = true;
+#pragma unroll
+        for (int j = 0; j < D; ++j) eq = eq && (e[j] == e2[j]);
+        if (eq) {
+            atomicOr(&etab[h], HV_EDGE_DONE);
+            reg[t] = (int64_t)h;
+            return;
+        }
+        h = (h + 1) & emask;
+    }
+    reg[t] = -2;
+    atomicAdd(&ctr[1], 1ull);
+}
+
+template <int D>
+__global__ void k_rehash_edges(const unsigned long long* __restrict__ old, uint64_t omask, const int* __restrict__ vkey,
+                               unsigned long long* __restrict__ etab, uint64_t emask) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > omask) return;
+    unsigned long long cur = old[t];
+    if (cur == 0ull) return;
+    const unsigned long long raw = (cur & ~HV_EDGE_DONE) - 1ull;
+    int e[D];
+    hv_edge_of<D>(vkey + (int64_t)(raw >> 4) * (D + 1), (int)(raw & 15ull), e);
+    uint64_t h = hv_hash_ints<D>(e) & emask;
+    for (;;) {
+        if (atomicCAS(&etab[h], 0ull, cur) == 0ull) return;
+        h = (h + 1) & emask;
+    }
+}
+
+// Direction of the walk along the edge key\{key[pos]} (u_qr, src/tools.jl:773-790):
+// u = -normalize(P_perp w), P_perp the projection onto the orthogonal complement of the
+// edge's affine hull; w = x_s - x0 for a dropped generator s, n_s for a dropped plane
+// (its mirror image of x0 lies on the +n_s side).  Planes enter the span via their
+// normals.  du is the orthogonality defect of src/edgeiteratebase.jl:64-74 and c the
+// half-space threshold of src/raycast.jl:802-805.
+template <int D>
+__device__ __forceinline__ void hv_span_vec(const DevGeo& g, int el, const double* x0, double* out) {
+    if (el < g.N) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) out[k] = g.xo[(int64_t)k * g.N + el] - x0[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < D; ++k) out[k] = g.pn[(el - g.N) * D + k];
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void hv_ray_setup(const DevGeo& g, const int* key, int pos, const double* r, double plane_tol,
+                             double* u, double& c, double& du, double& rho) {
+    const int N = g.N;
+    int e[D];
+    hv_edge_of<D>(key, pos, e);
+    const int s = hv_key_at<D>(key, pos);
+    double x0[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) x0[k] = g.xo[(int64_t)k * N + e[0]];
+    double q[D - 1][D];
+    // slots from the top so that plane normals (largest indices) are orthogonalised first
+#pragma unroll
+    for (int a = 0; a < D - 1; ++a) {
+        hv_span_vec<D>(g, e[D - 1 - a], x0, q[a]);
+#pragma unroll
+        for (int rep = 0; rep < 2; ++rep) {
+#pragma unroll
+            for (int b = 0; b < a; ++b) {
+                double dp = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) dp = fma(q[a][k], q[b][k], dp);
+#pragma unroll
+                for (int k = 0; k < D; ++k) q[a][k] = fma(-dp, q[b][k], q[a][k]);
+            }
+        }
+        double nn = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) nn = fma(q[a][k], q[a][k], nn);
+        nn = 1.0 / sqrt(nn);
+#pragma unroll
+        for (int k = 0; k < D; ++k) q[a][k] *= nn;
+    }
+    double w[D];
+    hv_span_vec<D>(g, s, x0, w);
+#pragma unroll
+    for (int rep = 0; rep < 2; ++rep) {
+#pragma unroll
+        for (int b = 0; b < D - 1; ++b) {
+            double dp = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) dp = fma(w[k], q[b][k], dp);
+#pragma unroll
+            for (int k = 0; k < D; ++k) w[k] = fma(-dp, q[b][k], w[k]);
+        }
+    }
+    double nw = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) nw = fma(w[k], w[k], nw);
+    nw = -1.0 / sqrt(nw);
+#pragma unroll
+    for (int k = 0; k < D; ++k) u[k] = w[k] * nw;
+    du = 0.0;
+#pragma unroll
+    for (int a = 0; a < D - 1; ++a) {
+        double raw[D];
+        hv_span_vec<D>(g, e[D - 1 - a], x0, raw);
+        double dp = 0.0, nn = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) { dp = fma(raw[k], u[k], dp); nn = fma(raw[k], raw[k], nn); }
+        du += fabs(dp) / sqrt(nn);
+    }
+    double ux0 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) ux0 = fma(u[k], x0[k], ux0);
+    double cmax = ux0;
+#pragma unroll
+    for (int j = 1; j < D; ++j) {
+        const int el = e[j];
+        double v;
+        if (el < N) {
+            v = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) v = fma(u[k], g.xo[(int64_t)k * N + el], v);
+        } else {
+            const int p = el - N;
+            double dist = g.ph[p], nu = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) { dist = fma(-g.pn[p * D + k], x0[k], dist); nu = fma(g.pn[p * D + k], u[k], nu); }
+            v = ux0 + 2.0 * dist * nu;   // <mirror of x0 across p, u>
+        }
+        cmax = fmax(cmax, v);
+    }
+    c = cmax + fabs(cmax) * plane_tol;
+    double rr = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) { double t = r[k] - x0[k]; rr = fma(t, t, rr); }
+    rho = sqrt(rr);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, int nf, const int* __restrict__ vkey,
+                       const double* __restrict__ vx, DevGeo g, const unsigned long long* __restrict__ etab,
+                       const int64_t* __restrict__ reg, DevRays R, int* __restrict__ nrays, double plane_tol) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf * (D + 1)) return;
+    const int64_t h = reg[t];
+    if (h < 0) return;
+    const unsigned long long cur = etab[h];
+    if (cur & HV_EDGE_DONE) return;
+    const int fi = t / (D + 1), pos = t % (D + 1);
+    const int v = frontier[fi];
+    const unsigned long long mine = ((((unsigned long long)v) << 4) | (unsigned)pos) + 1ull;
+    if (cur != mine) return;
+    int key[D + 1];
+#pragma unroll
+    for (int j = 0; j <= D; ++j) key[j] = vkey[(int64_t)v * (D + 1) + j];
+    double r[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) r[k] = vx[(int64_t)v * D + k];
+    double u[D], c, du, rho;
+    hv_ray_setup<D>(g, key, pos, r, plane_tol, u, c, du, rho);
+    const int idx = atomicAdd(nrays, 1);
+    R.v[idx] = v;
+    R.drop[idx] = pos;
+#pragma unroll
+    for (int k = 0; k < D; ++k) R.u[(int64_t)idx * D + k] = u[k];
+    R.c[idx] = c;
+    R.du[idx] = du;
+    double pe[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) pe[k] = r[k] + rho * u[k];
+    R.code[idx] = hv_morton<D>(g, pe);
+}
+
+// ---------------------------------------------------------------------------------
+// The candidate scan: one wavefront walks the generator tree for 64 rays at once.
+
+#define HV_ST_NONE 0
+#define HV_ST_FOUND 1
+#define HV_ST_TIE 2
+#define HV_ST_SINGULAR -2
+
+template <int D>
+__global__ __launch_bounds__(256) void k_raycast64(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
+                                                   const int* __restrict__ vkey, const double* __restrict__ vx,
+                                                   int* __restrict__ status, int* __restrict__ win,
+                                                   double* __restrict__ tout, double plane_tol,
+                                                   unsigned long long* __restrict__ ctr) {
+    __shared__ int stk[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rid = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = rid < nrays;
+    const int ray = act ? order[rid] : order[0];
+    const int N = g.N;
+    const int v = R.v[ray], pos = R.drop[ray];
+    int key[D + 1];
+#pragma unroll
+    for (int j = 0; j <= D; ++j) key[j] = vkey[(int64_t)v * (D + 1) + j];
+    int e[D];
+    hv_edge_of<D>(key, pos, e);
+    const int s = hv_key_at<D>(key, pos);
+    double u[D], r[D], x0[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        u[k] = R.u[(int64_t)ray * D + k];
+        r[k] = vx[(int64_t)v * D + k];
+        x0[k] = g.xo[(int64_t)k * N + e[0]];
+    }
+    const double cth = R.c[ray];
+    const double du = R.du[ray];
+    double ux0 = 0.0, rx0 = 0.0, x0x0 = 0.0, rho2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        ux0 = fma(u[k], x0[k], ux0);
+        rx0 = fma(r[k], x0[k], rx0);
+        x0x0 = fma(x0[k], x0[k], x0x0);
+        double t = r[k] - x0[k];
+        rho2 = fma(t, t, rho2);
+    }
+    const double K = 2.0 * rx0 - x0x0;
+    // best two candidates (t, index); generators in sorted index space, planes as N+p
+    double t1 = 1e300, t2 = 1e300;
+    int i1 = -1, i2 = -1;
+    // boundary planes (mirror pseudo-generators, src/raycast.jl:354-375, src/extended.jl:131-141)
+    for (int p = 0; p < g.P; ++p) {
+        const int id = N + p;
+        bool member = (id == s);
+#pragma unroll
+        for (int j = 0; j < D; ++j) member = member || (e[j] == id);
+        if (member) continue;
+        double nu = 0.0, dist = g.ph[p], nr = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double nk = g.pn[p * D + k];
+            nu = fma(nk, u[k], nu);
+            dist = fma(-nk, x0[k], dist);
+            nr = fma(nk, r[k], nr);
+        }
+        if (!(ux0 + 2.0 * dist * nu > cth)) continue;
+        const double tp = (g.ph[p] - nr) / nu;
+        if (!(tp >= plane_tol)) continue;
+        if (tp < t1) { t2 = t1; i2 = i1; t1 = tp; i1 = id; }
+        else if (tp < t2) { t2 = tp; i2 = id; }
+    }
+    constexpr double W = 1e-7;   // widest near-tie window of the reference (src/raycast.jl:902)
+    // packet centre: mean expected end point r + rho u over the active lanes
+    double pc[D];
+    {
+        const double rho = sqrt(rho2);
+        const double wgt = act ? 1.0 : 0.0;
+        double cnt = wgt;
+#pragma unroll
+        for (int k = 0; k < D; ++k) pc[k] = wgt * (r[k] + rho * u[k]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) pc[k] += __shfl_xor(pc[k], off);
+            cnt += __shfl_xor(cnt, off);
+        }
+        const double ic = cnt > 0 ? 1.0 / cnt : 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) pc[k] *= ic;
+    }
+    // certificate ball B(tau) = { x : |x - (r + tau u)| < |x0 - (r + tau u)| }
+    double bc[D], R2;
+    auto set_ball = [&](double tau) {
+        R2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            bc[k] = r[k] + tau * u[k];
+            double t = bc[k] - x0[k];
+            R2 = fma(t, t, R2);
+        }
+        R2 = R2 * (1.0 + 1e-9) + 1e-300;
+    };
+    set_ball(t1 + W);
+    auto box_hit = [&](int node) -> bool {
+        double d2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double lo = g.blo[(int64_t)k * g.NN + node], hi = g.bhi[(int64_t)k * g.NN + node];
+            const double a = fmax(fmax(lo - bc[k], bc[k] - hi), 0.0);
+            d2 = fma(a, a, d2);
+        }
+        return act && (d2 <= R2);
+    };
+    unsigned long long pairs = 0, leaves = 0;
+    int sp = 1;
+    stk[wv][0] = 1;
+    while (sp > 0) {
+        --sp;
+        const int node = __builtin_amdgcn_readfirstlane(stk[wv][sp]);
+        const bool hit = box_hit(node);
+        const unsigned long long m = __ballot(hit);
+        if (m == 0ull) continue;
+        if (node >= g.P2) {
+            const int leaf = node - g.P2;
+            const int j0 = leaf * g.L, j1 = min(j0 + g.L, N);
+            leaves += 1;
+            pairs += (unsigned long long)__popcll(m) * (unsigned long long)(j1 - j0);
+            if (hit) {
+                const double lim = t1 + W;
+                double tl = lim;
+                for (int j = j0; j < j1; ++j) {
+                    double ux = 0.0, rxj = 0.0;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) {
+                        const double xk = g.xs[(int64_t)k * N + j];
+                        ux = fma(u[k], xk, ux);
+                        rxj = fma(r[k], xk, rxj);
+                    }
+                    if (!(ux > cth)) continue;
+                    const double den2 = 2.0 * (ux - ux0);
+                    const double num = fma(-2.0, rxj, g.xx[j] + K);
+                    if (num < den2 * (t1 + W)) {
+                        const double t = num / den2;
+                        if (t >= plane_tol) {
+                            if (t < t1) { t2 = t1; i2 = i1; t1 = t; i1 = j; }
+                            else if (t < t2) { t2 = t; i2 = j; }
+                        }
+                    }
+                }
+                if (t1 + W < tl) set_ball(t1 + W);
+            }
+        } else {
+            // children, nearer (to the packet centre) popped first
+            const int a = 2 * node, b = 2 * node + 1;
+            double da = 0.0, db = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const double la = g.blo[(int64_t)k * g.NN + a], ha = g.bhi[(int64_t)k * g.NN + a];
+                const double lb = g.blo[(int64_t)k * g.NN + b], hb = g.bhi[(int64_t)k * g.NN + b];
+                const double xa = fmax(fmax(la - pc[k], pc[k] - ha), 0.0);
+                const double xb = fmax(fmax(lb - pc[k], pc[k] - hb), 0.0);
+                da = fma(xa, xa, da);
+                db = fma(xb, xb, db);
+            }
+            const int nearc = (da <= db) ? a : b, farc = (da <= db) ? b : a;
+            stk[wv][sp] = farc;
+            stk[wv][sp + 1] = nearc;
+            sp += 2;
+        }
+    }
+    // counters (one atomic per wavefront)
+    if (lane == 0) {
+        atomicAdd(&ctr[0], pairs);
+        atomicAdd(&ctr[1], leaves);
+    }
+    if (!act) return;
+    if (i1 < 0) { status[rid] = HV_ST_NONE; return; }
+    // original indices and the reference's form of t (src/raycast.jl:424-431)
+    auto orig = [&](int i) { return i < N ? g.sidx[i] : i; };
+    auto t_ref = [&](int w, double tscan, double& score, double& cosang) -> double {
+        if (w >= N) {
+            const int p = w - N;
+            double nu = 0.0, dist = g.ph[p];
+#pragma unroll
+            for (int k = 0; k < D; ++k) { nu = fma(g.pn[p * D + k], u[k], nu); dist = fma(-g.pn[p * D + k], x0[k], dist); }
+            score = 2.0 * dist * nu;
+            cosang = nu;
+            return tscan;
+        }
+        double num = 0.0, den = 0.0, dd = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double xk = g.xo[(int64_t)k * N + w];
+            const double dx = xk - x0[k];
+            num = fma(dx, x0[k] + xk - 2.0 * r[k], num);
+            den = fma(u[k], dx, den);
+            dd = fma(dx, dx, dd);
+        }
+        score = den;
+        cosang = den / sqrt(dd);
+        return num / (2.0 * den);
+    };
+    int w1 = orig(i1);
+    double s1, c1;
+    double tw = t_ref(w1, t1, s1, c1);
+    int st = HV_ST_FOUND;
+    if (i2 >= 0) {
+        const int w2 = orig(i2);
+        double s2, c2;
+        const double tw2 = t_ref(w2, t2, s2, c2);
+        double rn = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) rn = fma(r[k], r[k], rn);
+        const double full_error = (tw * du + sqrt(rn) * 1e-15) / c1;
+        const double window = fmin(1e-7, (10.0 + D) * full_error);
+        if (tw2 < tw || tw2 <= tw + window) {
+            // near tie: the larger <u, x - x0> wins (src/raycast.jl:903-916)
+            st = HV_ST_TIE;
+            if (s2 > s1) { w1 = w2; tw = tw2; }
+        }
+    }
+    win[rid] = w1;
+    status[rid] = st;
+    tout[rid] = tw;
+    if (st == HV_ST_TIE) atomicAdd(&ctr[2], 1ull);
+}
+
+// New key = edge + winner (sorted) and its circumcentre, written to the provisional slot.
+template <int D>
+__global__ __launch_bounds__(256) void k_finalize(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
+                           int* __restrict__ vkey, double* __restrict__ vx, int64_t base,
+                           int* __restrict__ status, const int* __restrict__ win) {
+    const int rid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rid >= nrays) return;
+    if (status[rid] <= 0) return;
+    const int ray = order[rid];
+    const int v = R.v[ray], pos = R.drop[ray];
+    int key[D + 1];
+#pragma unroll
+    for (int j = 0; j <= D; ++j) key[j] = vkey[(int64_t)v * (D + 1) + j];
+    int e[D];
+    hv_edge_of<D>(key, pos, e);
+    const int w1 = win[rid];
+    int ins = 0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) ins += (e[j] < w1) ? 1 : 0;
+    int nk[D + 1];
+#pragma unroll
+    for (int j = 0; j <= D; ++j) nk[j] = (j < ins) ? e[j] : (j == ins ? w1 : e[j > 0 ? j - 1 : 0]);
+    double xn[D];
+    if (!hv_vertex_coords<D>(g, nk, xn)) { status[rid] = HV_ST_SINGULAR; return; }
+    const int64_t id = base + rid;
+#pragma unroll
+    for (int j = 0; j <= D; ++j) vkey[id * (D + 1) + j] = nk[j];
+#pragma unroll
+    for (int k = 0; k < D; ++k) vx[id * D + k] = xn[k];
+}
+
+__global__ void k_iota(int* __restrict__ a, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+// ---------------------------------------------------------------------------------
+// host driver
+
+namespace {
+
+struct Tree {
+    DevGeo g{};
+};
+
+template <typename T>
+T* pget(hv_ctx* c, const char* name, size_t count, bool keep = false) {
+    return (T*)c->pool->get(name, count * sizeof(T), keep, c->stream);
+}
+
+// Points -> SoA + Morton order + leaf boxes + complete binary tree of boxes.
+template <int D>
+int build_tree(hv_ctx* c, int L, Tree& T) {
+    const int N = (int)c->N;
+    DevGeo& g = T.g;
+    memset(&g, 0, sizeof g);
+    g.N = N;
+    g.P = c->P;
+    g.L = L;
+    g.nleaf = (N + L - 1) / L;
+    g.P2 = 1;
+    while (g.P2 < g.nleaf) g.P2 <<= 1;
+    g.NN = 2 * g.P2;
+    // domain box from the axis-aligned planes
+    for (int k = 0; k < D; ++k) { g.dlo[k] = 0.0; g.dhi[k] = 1.0; }
+    for (int p = 0; p < c->P; ++p)
+        for (int k = 0; k < D; ++k) {
+            double nk = c->pnormal[p * D + k];
+            if (nk > 0.5) g.dhi[k] = c->pbase[p * D + k];
+            if (nk < -0.5) g.dlo[k] = c->pbase[p * D + k];
+        }
+    double* pn = pget<double>(c, "pn", std::max(1, c->P * D));
+    double* ph = pget<double>(c, "ph", std::max(1, c->P));
+    if (!pn || !ph) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (planes)");
+    std::vector<double> hph(std::max(1, c->P));
+    for (int p = 0; p < c->P; ++p) {
+        double s = 0;
+        for (int k = 0; k < D; ++k) s += c->pnormal[p * D + k] * c->pbase[p * D + k];
+        hph[p] = s;
+    }
+    if (c->P) {
+        HVCK(hipMemcpyAsync(pn, c->pnormal.data(), sizeof(double) * c->P * D, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipMemcpyAsync(ph, hph.data(), sizeof(double) * c->P, hipMemcpyHostToDevice, c->stream));
+    }
+    g.pn = pn;
+    g.ph = ph;
+    const double* xin = (const double*)c->pool->get("xin", 0);
+    double* xo = pget<double>(c, "xo", (size_t)N * D);
+    double* xs = pget<double>(c, "xs", (size_t)N * D);
+    double* xx = pget<double>(c, "xx", N);
+    int* sidx = pget<int>(c, "sidx", N);
+    uint64_t* code = pget<uint64_t>(c, "code", N);
+    uint64_t* code2 = pget<uint64_t>(c, "code2", N);
+    int* idx = pget<int>(c, "idx", N);
+    int* idx2 = pget<int>(c, "idx2", N);
+    int* bad = pget<int>(c, "bad", 1);
+    double* blo = pget<double>(c, "blo", (size_t)g.NN * D);
+    double* bhi = pget<double>(c, "bhi", (size_t)g.NN * D);
+    if (!xin || !xo || !xs || !xx || !sidx || !code || !code2 || !idx || !idx2 || !bad || !blo || !bhi)
+        return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tree)");
+    HVCK(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+    k_points_in<D><<<nblk(N, 256), 256, 0, c->stream>>>(xin, N, g, xo, code, idx, bad);
+    HVCK(hipGetLastError());
+    size_t tmpb = 0;
+    HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, code, code2, idx, idx2, N, 0, 64, c->stream));
+    void* tmp = c->pool->get("cubtmp", tmpb, false, c->stream);
+    if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
+    HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tmpb, code, code2, idx, idx2, N, 0, 64, c->stream));
+    k_points_sorted<D><<<nblk(N, 256), 256, 0, c->stream>>>(xo, idx2, N, xs, xx, sidx);
+    HVCK(hipGetLastError());
+    g.xo = xo;
+    g.xs = xs;
+    g.xx = xx;
+    g.sidx = sidx;
+    g.blo = blo;
+    g.bhi = bhi;
+    k_leaf_boxes<D><<<nblk(g.P2, 256), 256, 0, c->stream>>>(g, blo, bhi);
+    HVCK(hipGetLastError());
+    for (int first = g.P2 / 2; first >= 1; first /= 2) {
+        k_node_boxes<D><<<nblk(first, 256), 256, 0, c->stream>>>(first, first, g.NN, blo, bhi);
+        HVCK(hipGetLastError());
+    }
+    int hbad = 0;
+    HVCK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    if (hbad) return hv_fail(c, HV_ERR_DOMAIN, "a node does not lie in the domain given by the boundary");
+    return HV_OK;
+}
+
+// axis-aligned box check: planes 2k (x_k = hi, +e_k) and 2k+1 (x_k = lo, -e_k)
+int check_box(hv_ctx* c, int D) {
+    if (c->P != 2 * D) return hv_fail(c, HV_ERR_UNSUPPORTED, "the device walk needs the 2d planes of a cuboid boundary");
+    for (int k = 0; k < D; ++k)
+        for (int side = 0; side < 2; ++side) {
+            const int p = 2 * k + side;
+            for (int q = 0; q < D; ++q) {
+                double want = (q == k) ? (side == 0 ? 1.0 : -1.0) : 0.0;
+                if (c->pnormal[p * D + q] != want)
+                    return hv_fail(c, HV_ERR_UNSUPPORTED, "boundary planes must be those of cuboid(d, periodic=[]) in the reference's order");
+            }
+            if (c->pbc[p] > 0) return hv_fail(c, HV_ERR_UNSUPPORTED, "periodic planes are not handled by the device walk");
+        }
+    return HV_OK;
+}
+
+// vertex count per generator in the unit cube, measured with the CPU oracle (d = 2..6)
+double vertices_per_point(int D) {
+    switch (D) {
+        case 2: return 2.2;
+        case 3: return 7.0;
+        case 4: return 29.0;
+        case 5: return 140.0;
+        default: return 700.0;
+    }
+}
+
+}  // namespace
+
+template <int D>
+int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
+    hv_opts opt{};
+    if (o) opt = *o;
+    const int L = opt.leaf_size > 0 ? opt.leaf_size : 32;
+    const double plane_tol = opt.plane_tolerance > 0 ? opt.plane_tolerance : 1e-12;
+    if (opt.precision != HV_PREC_FP64) return hv_fail(c, HV_ERR_UNSUPPORTED, "precision mode not built yet");
+    int rc = check_box(c, D);
+    if (rc) return rc;
+    const int N = (int)c->N;
+    hipEvent_t e0 = c->ev[0], e1 = c->ev[1], r0 = c->ev[2], r1 = c->ev[3];
+    HVCK(hipEventRecord(e0, c->stream));
+    Tree T;
+    rc = build_tree<D>(c, L, T);
+    if (rc) return rc;
+    DevGeo& g = T.g;
+    // capacities
+    const int nseed = 1 << D;
+    int64_t vcap = opt.vertex_capacity > 0 ? opt.vertex_capacity
+                                           : (int64_t)(2.5 * vertices_per_point(D) * N) + 4 * nseed + 4096;
+    auto hpow = [](int64_t n) { uint64_t p = 1024; while ((int64_t)p < n) p <<= 1; return p; };
+    uint64_t hsize = hpow(2 * (int64_t)(1.2 * vertices_per_point(D) * N) + 4 * nseed);
+    uint64_t esize = hpow((int64_t)((D + 1) * 1.2 * vertices_per_point(D) * N) + 16 * nseed);
+    int* vkey = pget<int>(c, "vkey", (size_t)vcap * (D + 1));
+    double* vx = pget<double>(c, "vx", (size_t)vcap * D);
+    int* ids = pget<int>(c, "ids", (size_t)vcap);
+    int* htab = pget<int>(c, "htab", hsize);
+    unsigned long long* etab = pget<unsigned long long>(c, "etab", esize);
+    unsigned long long* ctr = pget<unsigned long long>(c, "ctr", 16);
+    int* dcount = pget<int>(c, "dcount", 4);
+    if (!vkey || !vx || !ids || !htab || !etab || !ctr || !dcount)
+        return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tables)");
+    HVCK(hipMemsetAsync(htab, 0xff, hsize * sizeof(int), c->stream));
+    HVCK(hipMemsetAsync(etab, 0, esize * sizeof(unsigned long long), c->stream));
+    HVCK(hipMemsetAsync(ctr, 0, 16 * sizeof(unsigned long long), c->stream));
+    hv_stats_t st{};
+    // seeds
+    int64_t vnext = 0;   // next provisional id
+    int64_t nlive = 0;   // live vertices (ids[0..nlive))
+    {
+        int* status = pget<int>(c, "status", nseed);
+        uint8_t* win = pget<uint8_t>(c, "win", nseed);
+        if (!status || !win) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+        k_seed_corners<D><<<nseed, 256, 0, c->stream>>>(g, vkey, vx, vnext, status);
+        HVCK(hipGetLastError());
+        k_insert<D><<<nblk(nseed, 256), 256, 0, c->stream>>>(status, nseed, vkey, vnext, htab, hsize - 1, win, ctr + 8);
+        HVCK(hipGetLastError());
+        size_t tb = 0;
+        hipcub::CountingInputIterator<int> it((int)vnext);
+        HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, win, ids + nlive, dcount, nseed, c->stream));
+        void* tmp = c->pool->get("cubtmp2", tb, false, c->stream);
+        HVCK(hipcub::DeviceSelect::Flagged(tmp, tb, it, win, ids + nlive, dcount, nseed, c->stream));
+        int nsel = 0;
+        HVCK(hipMemcpyAsync(&nsel, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        vnext += nseed;
+        nlive += nsel;
+    }
+    int64_t fbeg = 0, fcount = nlive;   // frontier = ids[fbeg .. fbeg+fcount)
+    double ms_ray = 0.0;
+    int64_t levels = 0;
+    unsigned long long hctr[16];
+    while (fcount > 0) {
+        ++levels;
+        const int64_t nslots = fcount * (D + 1);
+        // grow the edge hash when it could pass half load
+        HVCK(hipMemcpyAsync(hctr, ctr, sizeof hctr, hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        if ((int64_t)(hctr[4] + nslots) * 2 > (int64_t)esize) {
+            uint64_t nsz = esize;
+            while ((int64_t)(hctr[4] + nslots) * 2 > (int64_t)nsz) nsz <<= 1;
+            DBuf old = c->pool->take("etab");
+            unsigned long long* ne = pget<unsigned long long>(c, "etab", nsz);
+            if (!ne) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (edge hash)");
+            HVCK(hipMemsetAsync(ne, 0, nsz * sizeof(unsigned long long), c->stream));
+            k_rehash_edges<D><<<nblk(esize, 256), 256, 0, c->stream>>>((unsigned long long*)old.p, esize - 1, vkey, ne, nsz - 1);
+            HVCK(hipGetLastError());
+            HVCK(hipStreamSynchronize(c->stream));
+            HVCK(hipFree(old.p));
+            etab = ne;
+            esize = nsz;
+        }
+        int64_t* reg = pget<int64_t>(c, "reg", nslots);
+        if (!reg) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (reg)");
+        const int* frontier = ids + fbeg;
+        // ctr[4] counts unique edges, ctr[5] edge-table overflow
+        k_register<D><<<nblk(nslots, 256), 256, 0, c->stream>>>(frontier, (int)fcount, vkey, N, etab, esize - 1, reg, ctr + 4);
+        HVCK(hipGetLastError());
+        DevRays R;
+        R.v = pget<int>(c, "ray_v", nslots);
+        R.drop = pget<int>(c, "ray_drop", nslots);
+        R.u = pget<double>(c, "ray_u", nslots * D);
+        R.c = pget<double>(c, "ray_c", nslots);
+        R.du = pget<double>(c, "ray_du", nslots);
+        R.tinit = nullptr;
+        R.code = pget<uint64_t>(c, "ray_code", nslots);
+        uint64_t* code2 = pget<uint64_t>(c, "ray_code2", nslots);
+        int* ord0 = pget<int>(c, "ray_ord0", nslots);
+        int* ord = pget<int>(c, "ray_ord", nslots);
+        if (!R.v || !R.drop || !R.u || !R.c || !R.du || !R.code || !code2 || !ord0 || !ord)
+            return hv_fail(c, HV_ERR_DEVICE, "out of device memory (rays)");
+        HVCK(hipMemsetAsync(dcount, 0, sizeof(int), c->stream));
+        k_emit<D><<<nblk(nslots, 256), 256, 0, c->stream>>>(frontier, (int)fcount, vkey, vx, g, etab, reg, R, dcount, plane_tol);
+        HVCK(hipGetLastError());
+        int nr = 0;
+        HVCK(hipMemcpyAsync(&nr, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipMemcpyAsync(hctr, ctr, sizeof hctr, hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        if (hctr[5]) return hv_fail(c, HV_ERR_CAPACITY, "edge hash overflow");
+        if (nr == 0) break;
+        // vertex table / hash capacity for this level's candidates
+        if (vnext + nr > vcap) {
+            int64_t ncap = std::max<int64_t>(vcap * 3 / 2, vnext + nr + 1024);
+            int* nk2 = (int*)c->pool->get("vkey", (size_t)ncap * (D + 1) * sizeof(int), true, c->stream);
+            double* nx2 = (double*)c->pool->get("vx", (size_t)ncap * D * sizeof(double), true, c->stream);
+            int* ni2 = (int*)c->pool->get("ids", (size_t)ncap * sizeof(int), true, c->stream);
+            if (!nk2 || !nx2 || !ni2) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (vertex table)");
+            vkey = nk2; vx = nx2; ids = ni2; vcap = ncap;
+            frontier = ids + fbeg;
+        }
+        if ((nlive + nr) * 2 > (int64_t)hsize) {
+            uint64_t nsz = hsize;
+            while ((nlive + nr) * 2 > (int64_t)nsz) nsz <<= 1;
+            int* nh = (int*)c->pool->get("htab", nsz * sizeof(int), false, c->stream);
+            if (!nh) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (vertex hash)");
+            HVCK(hipMemsetAsync(nh, 0xff, nsz * sizeof(int), c->stream));
+            k_rehash_vertices<D><<<nblk(nlive, 256), 256, 0, c->stream>>>(ids, nlive, vkey, nh, nsz - 1);
+            HVCK(hipGetLastError());
+            htab = nh;
+            hsize = nsz;
+        }
+        // sort rays by the Morton code of their expected end point
+        {
+            size_t tb = 0;
+            HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, R.code, code2, ord0, ord, nr, 0, 64, c->stream));
+            void* tmp = c->pool->get("cubtmp", tb, false, c->stream);
+            if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
+            k_iota<<<nblk(nr, 256), 256, 0, c->stream>>>(ord0, nr);
+            HVCK(hipGetLastError());
+            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, nr, 0, 64, c->stream));
+        }
+        int* status = pget<int>(c, "status", nr);
+        uint8_t* win = pget<uint8_t>(c, "win", nr);
+        int* rwin = pget<int>(c, "ray_win", nr);
+        double* rt = pget<double>(c, "ray_t", nr);
+        if (!status || !win || !rwin || !rt) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (status)");
+        HVCK(hipEventRecord(r0, c->stream));
+        k_raycast64<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, ord, nr, vkey, vx, status, rwin, rt, plane_tol, ctr + 0);
+        HVCK(hipGetLastError());
+        HVCK(hipEventRecord(r1, c->stream));
+        k_finalize<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, ord, nr, vkey, vx, vnext, status, rwin);
+        HVCK(hipGetLastError());
+        k_insert<D><<<nblk(nr, 256), 256, 0, c->stream>>>(status, nr, vkey, vnext, htab, hsize - 1, win, ctr + 8);
+        HVCK(hipGetLastError());
+        size_t tb = 0;
+        hipcub::CountingInputIterator<int> it((int)vnext);
+        HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, win, ids + nlive, dcount, nr, c->stream));
+        void* tmp = c->pool->get("cubtmp2", tb, false, c->stream);
+        if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
+        HVCK(hipcub::DeviceSelect::Flagged(tmp, tb, it, win, ids + nlive, dcount, nr, c->stream));
+        int nsel = 0;
+        HVCK(hipMemcpyAsync(&nsel, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        HVCK(hipEventElapsedTime(&ms, r0, r1));
+        ms_ray += ms;
+        st.rays += nr;
+        fbeg = nlive;
+        fcount = nsel;
+        nlive += nsel;
+        vnext += nr;
+    }
+    HVCK(hipEventRecord(e1, c->stream));
+    HVCK(hipMemcpyAsync(hctr, ctr, sizeof hctr, hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    float ms_all = 0.f;
+    HVCK(hipEventElapsedTime(&ms_all, e0, e1));
+    if (hctr[9]) return hv_fail(c, HV_ERR_CAPACITY, "vertex hash overflow");
+    st.levels = levels;
+    st.vertices = nlive;
+    st.pairs_scanned = (int64_t)hctr[0];
+    st.leaves_visited = (int64_t)hctr[1];
+    st.near_ties = (int64_t)hctr[2];
+    st.duplicates = (int64_t)hctr[8];
+    st.ms_total = ms_all;
+    st.ms_raycast = ms_ray;
+    hv_mesh* m = new hv_mesh();
+    m->pool = c->pool;
+    m->device = c->device;
+    m->D = D;
+    m->N = N;
+    m->nv = nlive;
+    m->key = c->pool->take("vkey");
+    m->x = c->pool->take("vx");
+    m->ids = c->pool->take("ids");
+    m->stats = st;
+    *out = m;
+    return HV_OK;
+}
+
+template <int D>
+int hv_mesh_export_impl(const hv_mesh* m, int64_t* sig, double* coords) {
+    const int64_t nv = m->nv;
+    std::vector<int> ids(nv);
+    if (hipMemcpy(ids.data(), m->ids.p, sizeof(int) * nv, hipMemcpyDeviceToHost) != hipSuccess) return HV_ERR_DEVICE;
+    int64_t maxid = 0;
+    for (int v : ids) maxid = std::max<int64_t>(maxid, v);
+    std::vector<int> key((size_t)(maxid + 1) * (D + 1));
+    std::vector<double> x((size_t)(maxid + 1) * D);
+    if (hipMemcpy(key.data(), m->key.p, key.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return HV_ERR_DEVICE;
+    if (hipMemcpy(x.data(), m->x.p, x.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return HV_ERR_DEVICE;
+    std::vector<int64_t> order(nv);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        const int* ka = &key[(size_t)ids[a] * (D + 1)];
+        const int* kb = &key[(size_t)ids[b] * (D + 1)];
+        return std::lexicographical_compare(ka, ka + D + 1, kb, kb + D + 1);
+    });
+    for (int64_t i = 0; i < nv; ++i) {
+        const int v = ids[order[i]];
+        for (int j = 0; j <= D; ++j) sig[i * (D + 1) + j] = (int64_t)key[(size_t)v * (D + 1) + j] + 1;
+        for (int k = 0; k < D; ++k) coords[i * D + k] = x[(size_t)v * D + k];
+    }
+    return HV_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// ray-level and query entry points (hv_raycast_batch, hv_nn_halfspace_batch, hv_inrange_batch)
+
+template <int D>
+__global__ __launch_bounds__(256) void k_setup_rays(int n, const int* __restrict__ vkey, const double* __restrict__ vx,
+                             const int* __restrict__ drop, DevGeo g, DevRays R, double plane_tol) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int key[D + 1];
+#pragma unroll
+    for (int j = 0; j <= D; ++j) key[j] = vkey[(int64_t)t * (D + 1) + j];
+    double r[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) r[k] = vx[(int64_t)t * D + k];
+    double u[D], c, du, rho;
+    hv_ray_setup<D>(g, key, drop[t], r, plane_tol, u, c, du, rho);
+    R.v[t] = t;
+    R.drop[t] = drop[t];
+#pragma unroll
+    for (int k = 0; k < D; ++k) R.u[(int64_t)t * D + k] = u[k];
+    R.c[t] = c;
+    R.du[t] = du;
+    double pe[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) pe[k] = r[k] + rho * u[k];
+    R.code[t] = hv_morton<D>(g, pe);
+}
+
+// box distance helpers for the per-thread (stackless, heap-order) tree walks
+template <int D>
+__device__ __forceinline__ double hv_box_d2(const DevGeo& g, int node, const double* q) {
+    double d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double lo = g.blo[(int64_t)k * g.NN + node], hi = g.bhi[(int64_t)k * g.NN + node];
+        const double a = fmax(fmax(lo - q[k], q[k] - hi), 0.0);
+        d2 = fma(a, a, d2);
+    }
+    return d2;
+}
+
+__device__ __forceinline__ int hv_next_node(int i) {
+    while (i & 1) i >>= 1;
+    return i == 0 ? 0 : i + 1;
+}
+
+template <int D>
+__global__ void k_nn_halfspace(DevGeo g, int nq, const double* __restrict__ q, const double* __restrict__ u,
+                               const double* __restrict__ cc, int64_t* __restrict__ idx, double* __restrict__ dist) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    double p[D], uu[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) { p[k] = q[(int64_t)t * D + k]; uu[k] = u[(int64_t)t * D + k]; }
+    const double c = cc[t];
+    double best = INFINITY;
+    int bi = -1;
+    int node = 1;
+    while (node) {
+        if (hv_box_d2<D>(g, node, p) <= best) {
+            if (node < g.P2) { node = 2 * node; continue; }
+            const int leaf = node - g.P2;
+            const int j0 = leaf * g.L, j1 = min(j0 + g.L, g.N);
+            for (int j = j0; j < j1; ++j) {
+                double d2 = 0.0, ux = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const double xk = g.xs[(int64_t)k * g.N + j];
+                    const double a = xk - p[k];
+                    d2 = fma(a, a, d2);
+                    ux = fma(uu[k], xk, ux);
+                }
+                if (!(ux > c)) continue;
+                const int gi = g.sidx[j];
+                if (d2 < best || (d2 == best && gi < bi)) { best = d2; bi = gi; }
+            }
+        }
+        node = hv_next_node(node);
+    }
+    idx[t] = bi >= 0 ? (int64_t)bi + 1 : 0;
+    dist[t] = bi >= 0 ? sqrt(best) : INFINITY;
+}
+
+template <int D>
+__global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const double* __restrict__ rad,
+                          const int64_t* __restrict__ ptr, int64_t* __restrict__ out, int64_t* __restrict__ counts) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    double p[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) p[k] = q[(int64_t)t * D + k];
+    const double r2 = rad[t] * rad[t];
+    int64_t cnt = 0;
+    const int64_t o = out ? ptr[t] : 0;
+    int node = 1;
+    while (node) {
+        if (hv_box_d2<D>(g, node, p) <= r2) {
+            if (node < g.P2) { node = 2 * node; continue; }
+            const int leaf = node - g.P2;
+            const int j0 = leaf * g.L, j1 = min(j0 + g.L, g.N);
+            for (int j = j0; j < j1; ++j) {
+                double d2 = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const double a = g.xs[(int64_t)k * g.N + j] - p[k];
+                    d2 = fma(a, a, d2);
+                }
+                if (d2 <= r2) {
+                    if (out) out[o + cnt] = (int64_t)g.sidx[j] + 1;
+                    ++cnt;
+                }
+            }
+        }
+        node = hv_next_node(node);
+    }
+    if (counts) counts[t] = cnt;
+}
+
+template <int D>
+int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
+                          const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status) {
+    if (precision != HV_PREC_FP64) return hv_fail(c, HV_ERR_UNSUPPORTED, "precision mode not built yet");
+    if (nrays == 0) return HV_OK;
+    if (nrays > (1 << 29)) return hv_fail(c, HV_ERR_ARG, "too many rays");
+    const int N = (int)c->N;
+    const int P = c->P;
+    int rc = check_box(c, D);
+    if (rc) return rc;
+    // host validation: sorted keys, an edge with a real generator
+    std::vector<int> hkey((size_t)nrays * (D + 1));
+    std::vector<int> hdrop(nrays);
+    for (int64_t k = 0; k < nrays; ++k) {
+        for (int j = 0; j <= D; ++j) {
+            const int64_t e = sig[k * (D + 1) + j];
+            if (e < 1 || e > (int64_t)N + P) return hv_fail(c, HV_ERR_ARG, "key element out of range");
+            if (j && e <= sig[k * (D + 1) + j - 1]) return hv_fail(c, HV_ERR_ARG, "keys must be sorted and distinct");
+            hkey[k * (D + 1) + j] = (int)(e - 1);
+        }
+        if (drop[k] < 0 || drop[k] > D) return hv_fail(c, HV_ERR_ARG, "drop position out of range");
+        hdrop[k] = drop[k];
+        const int first = (drop[k] == 0) ? hkey[k * (D + 1) + 1] : hkey[k * (D + 1)];
+        if (first >= N) return hv_fail(c, HV_ERR_ARG, "the walked edge must contain a generator");
+    }
+    Tree T;
+    rc = build_tree<D>(c, 32, T);
+    if (rc) return rc;
+    DevGeo& g = T.g;
+    int* vkey = pget<int>(c, "b_vkey", (size_t)2 * nrays * (D + 1));
+    double* vx = pget<double>(c, "b_vx", (size_t)2 * nrays * D);
+    int* dd = pget<int>(c, "b_drop", nrays);
+    DevRays R;
+    R.v = pget<int>(c, "b_rv", nrays);
+    R.drop = pget<int>(c, "b_rd", nrays);
+    R.u = pget<double>(c, "b_ru", (size_t)nrays * D);
+    R.c = pget<double>(c, "b_rc", nrays);
+    R.du = pget<double>(c, "b_rdu", nrays);
+    R.tinit = nullptr;
+    R.code = pget<uint64_t>(c, "b_code", nrays);
+    uint64_t* code2 = pget<uint64_t>(c, "b_code2", nrays);
+    int* ord0 = pget<int>(c, "b_ord0", nrays);
+    int* ord = pget<int>(c, "b_ord", nrays);
+    int* status = pget<int>(c, "b_status", nrays);
+    double* tout = pget<double>(c, "b_t", nrays);
+    int* rwin = pget<int>(c, "b_win", nrays);
+    unsigned long long* ctr = pget<unsigned long long>(c, "b_ctr", 16);
+    if (!vkey || !vx || !dd || !R.v || !R.drop || !R.u || !R.c || !R.du || !R.code || !code2 || !ord0 || !ord ||
+        !status || !tout || !ctr || !rwin)
+        return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+    HVCK(hipMemcpyAsync(vkey, hkey.data(), sizeof(int) * hkey.size(), hipMemcpyHostToDevice, c->stream));
+    HVCK(hipMemcpyAsync(vx, r, sizeof(double) * nrays * D, hipMemcpyHostToDevice, c->stream));
+    HVCK(hipMemcpyAsync(dd, hdrop.data(), sizeof(int) * nrays, hipMemcpyHostToDevice, c->stream));
+    HVCK(hipMemsetAsync(ctr, 0, 16 * sizeof(unsigned long long), c->stream));
+    HVCK(hipMemsetAsync(status, 0, sizeof(int) * nrays, c->stream));
+    const double plane_tol = 1e-12;
+    k_setup_rays<D><<<nblk(nrays, 256), 256, 0, c->stream>>>((int)nrays, vkey, vx, dd, g, R, plane_tol);
+    HVCK(hipGetLastError());
+    size_t tb = 0;
+    HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, R.code, code2, ord0, ord, (int)nrays, 0, 64, c->stream));
+    void* tmp = c->pool->get("cubtmp", tb, false, c->stream);
+    if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
+    k_iota<<<nblk(nrays, 256), 256, 0, c->stream>>>(ord0, (int)nrays);
+    HVCK(hipGetLastError());
+    HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, (int)nrays, 0, 64, c->stream));
+    k_raycast64<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, status, rwin, tout,
+                                                            plane_tol, ctr);
+    HVCK(hipGetLastError());
+    k_finalize<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, nrays, status, rwin);
+    HVCK(hipGetLastError());
+    std::vector<int> hord(nrays), hst(nrays), hk((size_t)nrays * (D + 1));
+    std::vector<double> hx((size_t)nrays * D), ht(nrays);
+    HVCK(hipMemcpyAsync(hord.data(), ord, sizeof(int) * nrays, hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipMemcpyAsync(hst.data(), status, sizeof(int) * nrays, hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipMemcpyAsync(hk.data(), vkey + (size_t)nrays * (D + 1), sizeof(int) * hk.size(), hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipMemcpyAsync(hx.data(), vx + (size_t)nrays * D, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipMemcpyAsync(ht.data(), tout, sizeof(double) * nrays, hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    for (int64_t rid = 0; rid < nrays; ++rid) {
+        const int k = hord[rid];
+        const int s = hst[rid];
+        out_status[k] = s == HV_ST_FOUND ? 0 : (s == HV_ST_TIE ? 2 : (s == HV_ST_NONE ? 1 : -1));
+        for (int j = 0; j <= D; ++j) out_sig[(int64_t)k * (D + 1) + j] = s > 0 ? (int64_t)hk[rid * (D + 1) + j] + 1 : 0;
+        for (int q = 0; q < D; ++q) out_r[(int64_t)k * D + q] = s > 0 ? hx[rid * D + q] : NAN;
+        out_t[k] = s > 0 ? ht[rid] : INFINITY;
+    }
+    return HV_OK;
+}
+
+template <int D>
+int hv_nn_halfspace_impl(hv_ctx* c, int64_t nq, const double* q, const double* u, const double* cc,
+                         int64_t* idx, double* dist) {
+    if (nq == 0) return HV_OK;
+    Tree T;
+    int rc = build_tree<D>(c, 32, T);
+    if (rc) return rc;
+    double* dq = pget<double>(c, "q_q", (size_t)nq * D);
+    double* du_ = pget<double>(c, "q_u", (size_t)nq * D);
+    double* dc = pget<double>(c, "q_c", nq);
+    int64_t* di = pget<int64_t>(c, "q_i", nq);
+    double* dd = pget<double>(c, "q_d", nq);
+    if (!dq || !du_ || !dc || !di || !dd) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+    HVCK(hipMemcpyAsync(dq, q, sizeof(double) * nq * D, hipMemcpyHostToDevice, c->stream));
+    HVCK(hipMemcpyAsync(du_, u, sizeof(double) * nq * D, hipMemcpyHostToDevice, c->stream));
+    HVCK(hipMemcpyAsync(dc, cc, sizeof(double) * nq, hipMemcpyHostToDevice, c->stream));
+    k_nn_halfspace<D><<<nblk(nq, 128), 128, 0, c->stream>>>(T.g, (int)nq, dq, du_, dc, di, dd);
+    HVCK(hipGetLastError());
+    HVCK(hipMemcpyAsync(idx, di, sizeof(int64_t) * nq, hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipMemcpyAsync(dist, dd, sizeof(double) * nq, hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    return HV_OK;
+}
+
+template <int D>
+int hv_inrange_impl(hv_ctx* c, int64_t nq, const double* q, const double* radius, int64_t* ptr, int64_t* idx) {
+    ptr[0] = 0;
+    if (nq == 0) return HV_OK;
+    Tree T;
+    int rc = build_tree<D>(c, 32, T);
+    if (rc) return rc;
+    double* dq = pget<double>(c, "q_q", (size_t)nq * D);
+    double* dr = pget<double>(c, "q_r", nq);
+    int64_t* dcnt = pget<int64_t>(c, "q_cnt", nq);
+    int64_t* dptr = pget<int64_t>(c, "q_ptr", nq + 1);
+    if (!dq || !dr || !dcnt || !dptr) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+    HVCK(hipMemcpyAsync(dq, q, sizeof(double) * nq * D, hipMemcpyHostToDevice, c->stream));
+    HVCK(hipMemcpyAsync(dr, radius, sizeof(double) * nq, hipMemcpyHostToDevice, c->stream));
+    if (!idx) {
+        k_inrange<D><<<nblk(nq, 128), 128, 0, c->stream>>>(T.g, (int)nq, dq, dr, nullptr, nullptr, dcnt);
+        HVCK(hipGetLastError());
+        std::vector<int64_t> cnt(nq);
+        HVCK(hipMemcpyAsync(cnt.data(), dcnt, sizeof(int64_t) * nq, hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        for (int64_t k = 0; k < nq; ++k) ptr[k + 1] = ptr[k] + cnt[k];
+        return HV_OK;
+    }
+    const int64_t total = ptr[nq];
+    int64_t* dout = pget<int64_t>(c, "q_out", std::max<int64_t>(total, 1));
+    if (!dout) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+    HVCK(hipMemcpyAsync(dptr, ptr, sizeof(int64_t) * (nq + 1), hipMemcpyHostToDevice, c->stream));
+    k_inrange<D><<<nblk(nq, 128), 128, 0, c->stream>>>(T.g, (int)nq, dq, dr, dptr, dout, nullptr);
+    HVCK(hipGetLastError());
+    if (total) HVCK(hipMemcpyAsync(idx, dout, sizeof(int64_t) * total, hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    for (int64_t k = 0; k < nq; ++k) std::sort(idx + ptr[k], idx + ptr[k + 1]);
+    return HV_OK;
+}
+
+#define HV_INST(D)                                                                                 \
+    template int hv_build_mesh_impl<D>(hv_ctx*, const hv_opts*, hv_mesh**);                        \
+    template int hv_mesh_export_impl<D>(const hv_mesh*, int64_t*, double*);                        \
+    template int hv_raycast_batch_impl<D>(hv_ctx*, int, int64_t, const int64_t*, const int32_t*,   \
+                                          const double*, int64_t*, double*, double*, int32_t*);    \
+    template int hv_nn_halfspace_impl<D>(hv_ctx*, int64_t, const double*, const double*,           \
+                                         const double*, int64_t*, double*);                        \
+    template int hv_inrange_impl<D>(hv_ctx*, int64_t, const double*, const double*, int64_t*, int64_t*);
+HV_INST(2)
+HV_INST(3)
+HV_INST(4)
+HV_INST(5)
+HV_INST(6)

@@ -1,0 +1,208 @@
+"""ctypes binding of libhv.so (C ABI in include/hv.h).
+
+The shared library is the product: there is no CPU fallback.  Importing this module
+without a built libhv.so raises, and so does any call when no GPU is visible.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_PKG), "libhv.so")
+
+HV_OK = 0
+ERRORS = {
+    -1: "HV_ERR_ARG", -2: "HV_ERR_DOMAIN", -3: "HV_ERR_FEW_POINTS", -4: "HV_ERR_DEGENERATE",
+    -5: "HV_ERR_DEVICE", -6: "HV_ERR_CAPACITY", -7: "HV_ERR_UNSUPPORTED", -8: "HV_ERR_STATE",
+}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
+
+
+class HVError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class hv_opts(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("leaf_size", C.c_int32), ("rays_per_block", C.c_int32),
+                ("reserved0", C.c_int32), ("plane_tolerance", C.c_double), ("ray_tol", C.c_double),
+                ("vertex_capacity", C.c_int64), ("seed", C.c_uint64)]
+
+
+class hv_stats_t(C.Structure):
+    _fields_ = [("rays", C.c_int64), ("levels", C.c_int64), ("vertices", C.c_int64), ("duplicates", C.c_int64),
+                ("pairs_scanned", C.c_int64), ("leaves_visited", C.c_int64), ("near_ties", C.c_int64),
+                ("degenerate", C.c_int64), ("fallbacks", C.c_int64), ("fp64_rechecks", C.c_int64),
+                ("retries", C.c_int64), ("unbounded", C.c_int64), ("max_t_relerr", C.c_double),
+                ("ms_total", C.c_double), ("ms_raycast", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc for gfx950)")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    L.hv_create.argtypes = [P(vp), C.c_int]
+    L.hv_destroy.argtypes = [vp]
+    L.hv_last_error.argtypes = [vp]
+    L.hv_last_error.restype = C.c_char_p
+    L.hv_set_points.argtypes = [vp, C.c_int, C.c_int64, P(C.c_double)]
+    L.hv_set_points_device.argtypes = [vp, C.c_int, C.c_int64, vp]
+    L.hv_set_boundary.argtypes = [vp, C.c_int, P(C.c_double), P(C.c_double), P(C.c_int16)]
+    L.hv_build_mesh.argtypes = [vp, P(hv_opts), P(vp)]
+    L.hv_mesh_counts.argtypes = [vp, P(C.c_int64), P(C.c_int32), P(C.c_int64)]
+    L.hv_mesh_export.argtypes = [vp, P(C.c_int64), P(C.c_double)]
+    L.hv_mesh_stats.argtypes = [vp, P(hv_stats_t)]
+    L.hv_mesh_free.argtypes = [vp]
+    L.hv_raycast_batch.argtypes = [vp, C.c_int, C.c_int64, P(C.c_int64), P(C.c_int32), P(C.c_double),
+                                   P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_int32)]
+    L.hv_nn_halfspace_batch.argtypes = [vp, C.c_int64, P(C.c_double), P(C.c_double), P(C.c_double),
+                                        P(C.c_int64), P(C.c_double)]
+    L.hv_inrange_batch.argtypes = [vp, C.c_int64, P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_int64)]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+class Mesh:
+    """A device-resident vertex set (hv_mesh)."""
+
+    def __init__(self, ctx: "Context", handle):
+        self._ctx = ctx
+        self._h = handle
+        self.d = ctx.d
+
+    def __del__(self):
+        self.free()
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().hv_mesh_free(self._h)
+            self._h = None
+
+    def counts(self):
+        nv = C.c_int64(0); ms = C.c_int32(0); nu = C.c_int64(0)
+        lib().hv_mesh_counts(self._h, C.byref(nv), C.byref(ms), C.byref(nu))
+        return nv.value, ms.value, nu.value
+
+    def export(self):
+        """(keys[V, d+1] 1-based sorted int64, coords[V, d]) in lexicographic key order."""
+        nv, ms, _ = self.counts()
+        keys = np.zeros((nv, ms), dtype=np.int64)
+        coords = np.zeros((nv, self.d), dtype=np.float64)
+        rc = lib().hv_mesh_export(self._h, _i64(keys), _dp(coords))
+        if rc != HV_OK:
+            raise HVError(rc, "export failed")
+        return keys, coords
+
+    def stats(self) -> dict:
+        s = hv_stats_t()
+        lib().hv_mesh_stats(self._h, C.byref(s))
+        return s.as_dict()
+
+
+class Context:
+    """One hv_ctx: points, boundary and a HIP stream on one device."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.hv_create(C.byref(h), device)
+        if rc != HV_OK:
+            raise HVError(rc, "hv_create failed (no GPU visible?)")
+        self._h = h
+        self.d = 0
+        self.n = 0
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().hv_destroy(self._h)
+            self._h = None
+
+    def _check(self, rc):
+        if rc != HV_OK:
+            raise HVError(rc, lib().hv_last_error(self._h).decode())
+
+    def set_points(self, xs: np.ndarray):
+        xs = np.ascontiguousarray(xs, dtype=np.float64)
+        n, d = xs.shape
+        self._check(lib().hv_set_points(self._h, d, n, _dp(xs)))
+        self.d, self.n = d, n
+
+    def set_points_device(self, ptr: int, n: int, d: int):
+        self._check(lib().hv_set_points_device(self._h, d, n, C.c_void_p(ptr)))
+        self.d, self.n = d, n
+
+    def set_boundary(self, base: np.ndarray, normal: np.ndarray, bc=None):
+        base = np.ascontiguousarray(base, dtype=np.float64)
+        normal = np.ascontiguousarray(normal, dtype=np.float64)
+        npl = base.shape[0]
+        bcp = None
+        if bc is not None:
+            bcp = np.ascontiguousarray(bc, dtype=np.int16).ctypes.data_as(C.POINTER(C.c_int16))
+        self._check(lib().hv_set_boundary(self._h, npl, _dp(base), _dp(normal), bcp))
+
+    def build_mesh(self, precision: str = "fp64", leaf_size: int = 0, vertex_capacity: int = 0) -> Mesh:
+        o = hv_opts()
+        o.precision = PRECISIONS[precision]
+        o.leaf_size = leaf_size
+        o.vertex_capacity = vertex_capacity
+        m = C.c_void_p()
+        self._check(lib().hv_build_mesh(self._h, C.byref(o), C.byref(m)))
+        return Mesh(self, m)
+
+    def raycast_batch(self, sig, drop, r, precision: str = "fp64"):
+        sig = np.ascontiguousarray(sig, dtype=np.int64)
+        drop = np.ascontiguousarray(drop, dtype=np.int32)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        n = sig.shape[0]
+        d = self.d
+        out_sig = np.zeros((n, d + 1), dtype=np.int64)
+        out_r = np.zeros((n, d))
+        out_t = np.zeros(n)
+        out_st = np.zeros(n, dtype=np.int32)
+        self._check(lib().hv_raycast_batch(self._h, PRECISIONS[precision], n, _i64(sig),
+                                           drop.ctypes.data_as(C.POINTER(C.c_int32)), _dp(r), _i64(out_sig),
+                                           _dp(out_r), _dp(out_t), out_st.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out_sig, out_r, out_t, out_st
+
+    def nn_halfspace(self, q, u, c):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        n = q.shape[0]
+        idx = np.zeros(n, dtype=np.int64)
+        dist = np.zeros(n)
+        self._check(lib().hv_nn_halfspace_batch(self._h, n, _dp(q), _dp(u), _dp(c), _i64(idx), _dp(dist)))
+        return idx, dist
+
+    def inrange(self, q, radius):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        radius = np.ascontiguousarray(radius, dtype=np.float64)
+        n = q.shape[0]
+        ptr = np.zeros(n + 1, dtype=np.int64)
+        self._check(lib().hv_inrange_batch(self._h, n, _dp(q), _dp(radius), _i64(ptr), None))
+        idx = np.zeros(max(int(ptr[-1]), 1), dtype=np.int64)
+        self._check(lib().hv_inrange_batch(self._h, n, _dp(q), _dp(radius), _i64(ptr), _i64(idx)))
+        return ptr, idx[: ptr[-1]]

@@ -1,0 +1,124 @@
+/*
+ * hv.h -- C ABI of libhv.so, the MI355X (gfx950) raycast Voronoi path.
+ *
+ * The library replaces the data-parallel hot path of HighVoronoi.jl v1.4.2:
+ * the raycast vertex walk (src/sysvoronoi.jl, src/raycast.jl), the half-space
+ * nearest-neighbour / range queries behind it (src/searchtrees.jl, src/extended.jl,
+ * src/NearestNeighborModified/) and, later, the volume / interface integration
+ * (src/polyintegrator.jl, src/mcintegrator.jl).  Plain pointers and sizes only.
+ *
+ * Conventions (shared by every entry point):
+ *  - Points are d x n column-major FP64 (Julia's Vector{SVector{d,Float64}} layout).
+ *  - Keys are sorted, 1-based int64: generator i is i, boundary plane p (1-based) is
+ *    n + p, exactly the reference's vertex keys (src/extended.jl:27-34).
+ *  - Every call returns an int status: HV_OK (0) or a negative HV_ERR_* code; the
+ *    message of the last failure on a context is hv_last_error(ctx).
+ *  - A context owns one HIP stream and the device copies of its points; it is not
+ *    thread-safe (one per host thread, like one RaycastIncircleSkip per Julia thread,
+ *    src/raycast-types.jl:298-307).  Calls are synchronous on return.
+ *  - The caller allocates every host output; variable-size outputs use the two-step
+ *    counts -> export protocol.
+ */
+#ifndef HV_H
+#define HV_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HV_OK 0
+#define HV_ERR_ARG -1          /* bad argument / shape */
+#define HV_ERR_DOMAIN -2       /* a generator lies outside the domain (src/boundary.jl:437-441) */
+#define HV_ERR_FEW_POINTS -3   /* n <= d (src/sysvoronoi.jl:25-27) */
+#define HV_ERR_DEGENERATE -4   /* input not in general position (near-tie / co-spherical vertex) */
+#define HV_ERR_DEVICE -5       /* HIP runtime error */
+#define HV_ERR_CAPACITY -6     /* a device table exceeded its capacity */
+#define HV_ERR_UNSUPPORTED -7  /* configuration outside what the GPU path implements */
+#define HV_ERR_STATE -8        /* call order (e.g. no points set) */
+
+#define HV_PREC_FP64 0         /* plain FP64 candidate scan */
+#define HV_PREC_FP32 1         /* FP32 candidate filter + FP64 re-verification */
+#define HV_PREC_BF16 2         /* BF16 MFMA candidate filter + FP64 re-verification */
+
+typedef struct hv_ctx hv_ctx;
+typedef struct hv_mesh hv_mesh;
+
+typedef struct hv_opts {
+    int32_t precision;        /* HV_PREC_* */
+    int32_t leaf_size;        /* generators per tree leaf (0 = default 32) */
+    int32_t rays_per_block;   /* threads per raycast block (0 = default 256) */
+    int32_t reserved0;
+    double plane_tolerance;   /* src/raycast-types.jl:229 (0 = 1e-12) */
+    double ray_tol;           /* src/raycast-types.jl:230 (0 = 1e-12) */
+    int64_t vertex_capacity;  /* device vertex-table capacity hint (0 = auto) */
+    uint64_t seed;
+} hv_opts;
+
+typedef struct hv_stats_t {
+    int64_t rays;             /* walkrays cast (SRI_walkray analogue) */
+    int64_t levels;           /* BFS levels of the device walk */
+    int64_t vertices;         /* vertices in the mesh */
+    int64_t duplicates;       /* rays whose vertex was found by another ray of its level */
+    int64_t pairs_scanned;    /* (ray, generator) pairs evaluated by the candidate scan */
+    int64_t leaves_visited;   /* (packet, leaf) visits */
+    int64_t near_ties;        /* rays resolved by the reference's near-tie rule */
+    int64_t degenerate;       /* rays flagged non-general (key longer than d+1) */
+    int64_t fallbacks;        /* mixed precision: rays re-verified with >1 FP64 candidate */
+    int64_t fp64_rechecks;    /* mixed precision: candidates re-evaluated in FP64 */
+    int64_t retries;          /* rays re-cast with a larger certificate radius */
+    int64_t unbounded;        /* rays without an end point */
+    double max_t_relerr;      /* mixed precision: max |t_lowprec - t_fp64| / t_fp64 of winners */
+    double ms_total;          /* device time of the last hv_build_mesh */
+    double ms_raycast;        /* device time spent in the candidate-scan kernel */
+} hv_stats_t;
+
+/* Context and inputs ------------------------------------------------------ */
+int hv_create(hv_ctx** out, int device_ordinal);
+int hv_destroy(hv_ctx* ctx);
+const char* hv_last_error(const hv_ctx* ctx);
+
+/* Replaces building the searcher's point store: VoronoiNodes(xs) + Raycast(xs)
+ * (src/voronoinodes.jl:14-21, src/raycast.jl:26).  xs is host memory, d x n. */
+int hv_set_points(hv_ctx* ctx, int d, int64_t n, const double* xs);
+/* Same, with xs already in device memory (HBM) on the context's device. */
+int hv_set_points_device(hv_ctx* ctx, int d, int64_t n, const double* xs_dev);
+/* Boundary planes (src/boundary.jl:22-29): base and outer normal, d x nplanes each;
+ * bc as the reference's Plane.BC (0 Dirichlet, -1 Neumann, >0 periodic partner). */
+int hv_set_boundary(hv_ctx* ctx, int nplanes, const double* base, const double* normal, const int16_t* bc);
+
+/* Mesh -------------------------------------------------------------------- */
+/* Replaces voronoi(mesh; searcher) with threading dispatch (src/sysvoronoi.jl:21-82):
+ * the whole vertex set of the tessellation, computed on the device. */
+int hv_build_mesh(hv_ctx* ctx, const hv_opts* opts, hv_mesh** out);
+int hv_mesh_counts(const hv_mesh* mesh, int64_t* nvert, int32_t* max_sig, int64_t* nrays_unbounded);
+/* sig: max_sig x nvert int64 (1-based, sorted, 0 padded); coords: d x nvert FP64.
+ * Vertices are exported sorted lexicographically by key. */
+int hv_mesh_export(const hv_mesh* mesh, int64_t* sig, double* coords);
+int hv_mesh_stats(const hv_mesh* mesh, hv_stats_t* out);
+int hv_mesh_free(hv_mesh* mesh);
+
+/* Ray-level entry point: one walkray per input (src/raycast.jl:125-164 via
+ * raycast_des2, src/raycast.jl:794-970).  For ray k: sig[k*(d+1) ...] is the sorted
+ * 1-based key of a vertex, drop[k] the 0-based position of the generator to walk
+ * away from, r[k*d ...] the vertex.  Outputs: out_sig (d+1 per ray, sorted, 1-based),
+ * out_r (d per ray), out_t (ray parameter), out_status (0 found, 1 unbounded,
+ * 2 near-tie resolved, <0 degenerate). */
+int hv_raycast_batch(hv_ctx* ctx, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
+                     const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status);
+
+/* Half-space nearest neighbour (src/extended.jl:108-142 with the predicate of
+ * src/raycast.jl:385-392): for query k, the generator minimising |x - q_k| among
+ * those with <x, u_k> > c_k (c_k = -inf disables the filter).  idx is 1-based
+ * (0 when none), dist the Euclidean distance (inf when none). */
+int hv_nn_halfspace_batch(hv_ctx* ctx, int64_t nq, const double* q, const double* u, const double* c,
+                          int64_t* idx, double* dist);
+
+/* Range query (src/searchtrees.jl:245, inclusive radius): generators with
+ * |x - q_k| <= radius_k.  Two-step: pass idx == NULL to receive counts in ptr[1..nq]
+ * (ptr[0] = 0, prefix sums), then call again with idx of size ptr[nq]. */
+int hv_inrange_batch(hv_ctx* ctx, int64_t nq, const double* q, const double* radius, int64_t* ptr, int64_t* idx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
