@@ -32,6 +32,7 @@ struct DevGeo {
 };
 
 struct DevRays {
+    int64_t* slot;   // edge-hash slot of the walked edge
     int* v;          // source vertex id
     int* drop;       // position in the source key of the generator walked away from
     double* u;       // D per ray
@@ -40,6 +41,28 @@ struct DevRays {
     double* tinit;   // certificate radius parameter for the first search
     uint64_t* code;  // Morton code of the expected end point
 };
+
+// Wavefront-aggregated atomics: one atomic per wave instead of one per lane (counters
+// and slot allocation hit a single address; per-lane atomics there serialise).  Both
+// must be reached by every lane of the wave (no divergent early return before them).
+__device__ __forceinline__ int hv_wave_alloc(int* counter, bool want) {
+    const unsigned long long m = __ballot(want);
+    if (m == 0ull) return -1;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    const unsigned long long below = lane ? (m & ((1ull << lane) - 1ull)) : 0ull;
+    return base + __popcll(below);
+}
+
+__device__ __forceinline__ void hv_wave_count(unsigned long long* ctr, bool cond) {
+    const unsigned long long m = __ballot(cond);
+    if (m == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    if (lane == __ffsll((long long)m) - 1) atomicAdd(ctr, (unsigned long long)__popcll(m));
+}
 
 __device__ __forceinline__ uint64_t hv_mix64(uint64_t z) {
     z ^= z >> 33; z *= 0xff51afd7ed558ccdULL;

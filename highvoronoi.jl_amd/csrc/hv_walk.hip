@@ -174,27 +174,33 @@ __device__ __forceinline__ bool keys_equal(const int* __restrict__ a, const int*
 }
 
 template <int D>
-__global__ void k_insert(const int* __restrict__ status, int n, const int* __restrict__ vkey, int64_t base,
-                         int* __restrict__ htab, uint64_t hmask, uint8_t* __restrict__ win,
-                         unsigned long long* __restrict__ ctr) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    win[t] = 0;
-    if (status[t] <= 0) return;
-    const int id = (int)(base + t);
-    const int* key = vkey + (int64_t)id * (D + 1);
-    uint64_t h = hv_hash_ints<D + 1>(key) & hmask;
-    for (uint64_t probe = 0; probe <= hmask; ++probe) {
-        int cur = htab[h];
-        if (cur < 0) {
-            int old = atomicCAS(&htab[h], -1, id);
-            if (old == -1) { win[t] = 1; return; }
-            cur = old;
+__global__ __launch_bounds__(256) void k_insert(const int* __restrict__ status, int n, const int* __restrict__ vkey,
+                                                int64_t base, int* __restrict__ htab, uint64_t hmask,
+                                                uint8_t* __restrict__ win, unsigned long long* __restrict__ ctr) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool dup = false, full = false;
+    if (t < n) {
+        uint8_t won = 0;
+        if (status[t] > 0) {
+            const int id = (int)(base + t);
+            const int* key = vkey + (int64_t)id * (D + 1);
+            uint64_t h = hv_hash_ints<D + 1>(key) & hmask;
+            full = true;
+            for (uint64_t probe = 0; probe <= hmask; ++probe) {
+                int cur = htab[h];
+                if (cur < 0) {
+                    const int old = atomicCAS(&htab[h], -1, id);
+                    if (old == -1) { won = 1; full = false; break; }
+                    cur = old;
+                }
+                if (keys_equal<D>(key, vkey + (int64_t)cur * (D + 1))) { dup = true; full = false; break; }
+                h = (h + 1) & hmask;
+            }
         }
-        if (keys_equal<D>(key, vkey + (int64_t)cur * (D + 1))) { atomicAdd(&ctr[0], 1ull); return; }
-        h = (h + 1) & hmask;
+        win[t] = won;
     }
-    atomicAdd(&ctr[1], 1ull);  // table full
+    hv_wave_count(&ctr[0], dup);
+    hv_wave_count(&ctr[1], full);
 }
 
 template <int D>
@@ -214,42 +220,48 @@ __global__ void k_rehash_vertices(const int* __restrict__ ids, int64_t n, const 
 // edges: registration (pushedge! semantics) and ray emission
 
 template <int D>
-__global__ void k_register(const int* __restrict__ frontier, int nf, const int* __restrict__ vkey, int N,
-                           unsigned long long* __restrict__ etab, uint64_t emask, int64_t* __restrict__ reg,
-                           unsigned long long* __restrict__ ctr) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nf * (D + 1)) return;
-    const int fi = t / (D + 1), pos = t % (D + 1);
-    const int v = frontier[fi];
-    const int* key = vkey + (int64_t)v * (D + 1);
-    int e[D];
-    hv_edge_of<D>(key, pos, e);
-    if (e[0] >= N) { reg[t] = -1; return; }   // an edge of planes only is a corner, not a line
-    const unsigned long long mine = ((((unsigned long long)v) << 4) | (unsigned)pos) + 1ull;
-    uint64_t h = hv_hash_ints<D>(e) & emask;
-    for (uint64_t probe = 0; probe <= emask; ++probe) {
-        unsigned long long cur = etab[h];
-        if (cur == 0ull) {
-            unsigned long long old = atomicCAS(&etab[h], 0ull, mine);
-            if (old == 0ull) { reg[t] = (int64_t)h; atomicAdd(&ctr[0], 1ull); return; }
-            cur = old;
-        }
-        const unsigned long long raw = (cur & ~HV_EDGE_DONE) - 1ull;
-        const int v2 = (int)(raw >> 4), p2 = (int)(raw & 15ull);
-        int e2[D];
-        hv_edge_of<D>(vkey + (int64_t)v2 * (D + 1), p2, e2);
-        bool eq = true;
+__global__ __launch_bounds__(256) void k_register(const int* __restrict__ frontier, int nf, const int* __restrict__ vkey,
+                                                  int N, unsigned long long* __restrict__ etab, uint64_t emask,
+                                                  int64_t* __restrict__ reg, unsigned long long* __restrict__ ctr) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool fresh = false, full = false;
+    if (t < nf * (D + 1)) {
+        const int fi = t / (D + 1), pos = t % (D + 1);
+        const int v = frontier[fi];
+        const int* key = vkey + (int64_t)v * (D + 1);
+        int e[D];
+        hv_edge_of<D>(key, pos, e);
+        int64_t out = -1;   // an edge of planes only is a corner point, not a line
+        if (e[0] < N) {
+            const unsigned long long mine = ((((unsigned long long)v) << 4) | (unsigned)pos) + 1ull;
+            uint64_t h = hv_hash_ints<D>(e) & emask;
+            out = -2;
+            for (uint64_t probe = 0; probe <= emask; ++probe) {
+                unsigned long long cur = etab[h];
+                if (cur == 0ull) {
+                    const unsigned long long old = atomicCAS(&etab[h], 0ull, mine);
+                    if (old == 0ull) { out = (int64_t)h; fresh = true; break; }
+                    cur = old;
+                }
+                const unsigned long long raw = (cur & ~HV_EDGE_DONE) - 1ull;
+                int e2[D];
+                hv_edge_of<D>(vkey + (int64_t)(raw >> 4) * (D + 1), (int)(raw & 15ull), e2);
+                bool eq = true;
 #pragma unroll
-        for (int j = 0; j < D; ++j) eq = eq && (e[j] == e2[j]);
-        if (eq) {
-            atomicOr(&etab[h], HV_EDGE_DONE);
-            reg[t] = (int64_t)h;
-            return;
+                for (int j = 0; j < D; ++j) eq = eq && (e[j] == e2[j]);
+                if (eq) {
+                    atomicOr(&etab[h], HV_EDGE_DONE);
+                    out = (int64_t)h;
+                    break;
+                }
+                h = (h + 1) & emask;
+            }
+            full = (out == -2);
         }
-        h = (h + 1) & emask;
+        reg[t] = out;
     }
-    reg[t] = -2;
-    atomicAdd(&ctr[1], 1ull);
+    hv_wave_count(&ctr[0], fresh);
+    hv_wave_count(&ctr[1], full);
 }
 
 template <int D>
@@ -267,6 +279,47 @@ __global__ void k_rehash_edges(const unsigned long long* __restrict__ old, uint6
         if (atomicCAS(&etab[h], 0ull, cur) == 0ull) return;
         h = (h + 1) & emask;
     }
+}
+
+// After the edge hash grew, stored slot indices are stale: find each edge again.
+template <int D>
+__device__ __forceinline__ int64_t hv_find_edge(const int* __restrict__ vkey, const unsigned long long* __restrict__ etab,
+                                                uint64_t emask, const int* e) {
+    uint64_t h = hv_hash_ints<D>(e) & emask;
+    for (uint64_t probe = 0; probe <= emask; ++probe) {
+        const unsigned long long cur = etab[h];
+        if (cur == 0ull) return -2;
+        const unsigned long long raw = (cur & ~HV_EDGE_DONE) - 1ull;
+        int e2[D];
+        hv_edge_of<D>(vkey + (int64_t)(raw >> 4) * (D + 1), (int)(raw & 15ull), e2);
+        bool eq = true;
+#pragma unroll
+        for (int j = 0; j < D; ++j) eq = eq && (e[j] == e2[j]);
+        if (eq) return (int64_t)h;
+        h = (h + 1) & emask;
+    }
+    return -2;
+}
+
+template <int D>
+__global__ void k_relookup_vertices(const int* __restrict__ ids, int count, const int* __restrict__ vkey,
+                                    const unsigned long long* __restrict__ etab, uint64_t emask, int64_t* __restrict__ reg) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * (D + 1)) return;
+    if (reg[t] < 0) return;
+    int e[D];
+    hv_edge_of<D>(vkey + (int64_t)ids[t / (D + 1)] * (D + 1), t % (D + 1), e);
+    reg[t] = hv_find_edge<D>(vkey, etab, emask, e);
+}
+
+template <int D>
+__global__ void k_relookup_rays(DevRays R, int nr, const int* __restrict__ vkey,
+                                const unsigned long long* __restrict__ etab, uint64_t emask) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nr) return;
+    int e[D];
+    hv_edge_of<D>(vkey + (int64_t)R.v[t] * (D + 1), R.drop[t], e);
+    R.slot[t] = hv_find_edge<D>(vkey, etab, emask, e);
 }
 
 // Direction of the walk along the edge key\{key[pos]} (u_qr, src/tools.jl:773-790):
@@ -378,27 +431,35 @@ __device__ __forceinline__ void hv_ray_setup(const DevGeo& g, const int* key, in
 
 template <int D>
 __global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, int nf, const int* __restrict__ vkey,
-                       const double* __restrict__ vx, DevGeo g, const unsigned long long* __restrict__ etab,
-                       const int64_t* __restrict__ reg, DevRays R, int* __restrict__ nrays, double plane_tol) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nf * (D + 1)) return;
-    const int64_t h = reg[t];
-    if (h < 0) return;
-    const unsigned long long cur = etab[h];
-    if (cur & HV_EDGE_DONE) return;
-    const int fi = t / (D + 1), pos = t % (D + 1);
-    const int v = frontier[fi];
-    const unsigned long long mine = ((((unsigned long long)v) << 4) | (unsigned)pos) + 1ull;
-    if (cur != mine) return;
-    int key[D + 1];
-#pragma unroll
-    for (int j = 0; j <= D; ++j) key[j] = vkey[(int64_t)v * (D + 1) + j];
+                                              const double* __restrict__ vx, DevGeo g,
+                                              const unsigned long long* __restrict__ etab,
+                                              const int64_t* __restrict__ reg, DevRays R, int* __restrict__ nrays,
+                                              double plane_tol) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool want = false;
+    int v = 0, pos = 0;
+    int64_t h = -1;
+    if (t < nf * (D + 1)) {
+        h = reg[t];
+        if (h >= 0) {
+            const unsigned long long cur = etab[h];
+            const int fi = t / (D + 1);
+            pos = t % (D + 1);
+            v = frontier[fi];
+            const unsigned long long mine = ((((unsigned long long)v) << 4) | (unsigned)pos) + 1ull;
+            want = !(cur & HV_EDGE_DONE) && (cur == mine);
+        }
+    }
+    const int idx = hv_wave_alloc(nrays, want);
+    if (!want) return;
+    // key read straight from global memory: a private copy indexed by `pos` would live in scratch
+    const int* key = vkey + (int64_t)v * (D + 1);
     double r[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) r[k] = vx[(int64_t)v * D + k];
     double u[D], c, du, rho;
     hv_ray_setup<D>(g, key, pos, r, plane_tol, u, c, du, rho);
-    const int idx = atomicAdd(nrays, 1);
+    R.slot[idx] = h;
     R.v[idx] = v;
     R.drop[idx] = pos;
 #pragma unroll
@@ -420,21 +481,44 @@ __global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, 
 #define HV_ST_SINGULAR -2
 
 template <int D>
+__device__ __forceinline__ double hv_box_dist2(const DevGeo& g, int node, const double* p) {
+    double d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double lo = g.blo[(int64_t)k * g.NN + node], hi = g.bhi[(int64_t)k * g.NN + node];
+        const double a = fmax(fmax(lo - p[k], p[k] - hi), 0.0);
+        d2 = fma(a, a, d2);
+    }
+    return d2;
+}
+
+// One wavefront walks the generator tree for 64 rays at once (a packet).  Per ray:
+//   t(x) = <x - x0, x + x0 - 2r> / (2 <u, x - x0>)   for x strictly ahead: <x,u> > c
+// (src/raycast.jl:394-432, 802-805).  Every x with t(x) < tau lies in the certificate
+// ball B(tau) = {x : |x - (r + tau u)| < |x0 - (r + tau u)|} (the half-space part of
+// B(tau) is exactly {t < tau}, and B(tau) holds no point behind the edge for tau > 0),
+// so a tree node is skipped when its box misses every live ray's ball.
+//   phase 1: each lane scans the leaf holding its expected end point r + |r-x0| u,
+//   phase 2: the packet traverses the tree nearest-first (to the packet centre),
+//            each visited leaf is scanned by the lanes whose ball reaches it.
+// The two smallest t are kept; the reference's near-tie rule (src/raycast.jl:877-916)
+// is applied to them at the end.
+template <int D>
 __global__ __launch_bounds__(256) void k_raycast64(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
                                                    const int* __restrict__ vkey, const double* __restrict__ vx,
                                                    int* __restrict__ status, int* __restrict__ win,
                                                    double* __restrict__ tout, double plane_tol,
                                                    unsigned long long* __restrict__ ctr) {
     __shared__ int stk[4][64];
+    __shared__ double leafx[4][D + 1][64];   // per wavefront: the visited leaf, SoA, plus |x|^2
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int rid = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = rid < nrays;
     const int ray = act ? order[rid] : order[0];
     const int N = g.N;
     const int v = R.v[ray], pos = R.drop[ray];
-    int key[D + 1];
-#pragma unroll
-    for (int j = 0; j <= D; ++j) key[j] = vkey[(int64_t)v * (D + 1) + j];
+    // key read straight from global memory: a private copy indexed by `pos` would live in scratch
+    const int* key = vkey + (int64_t)v * (D + 1);
     int e[D];
     hv_edge_of<D>(key, pos, e);
     const int s = hv_key_at<D>(key, pos);
@@ -453,13 +537,42 @@ __global__ __launch_bounds__(256) void k_raycast64(DevGeo g, DevRays R, const in
         ux0 = fma(u[k], x0[k], ux0);
         rx0 = fma(r[k], x0[k], rx0);
         x0x0 = fma(x0[k], x0[k], x0x0);
-        double t = r[k] - x0[k];
+        const double t = r[k] - x0[k];
         rho2 = fma(t, t, rho2);
     }
     const double K = 2.0 * rx0 - x0x0;
+    constexpr double W = 1e-7;   // widest near-tie window of the reference (src/raycast.jl:902)
     // best two candidates (t, index); generators in sorted index space, planes as N+p
     double t1 = 1e300, t2 = 1e300;
     int i1 = -1, i2 = -1;
+    // (selects only: branchy updates get sunk into a scratch store)
+    auto offer = [&](double t, int id) {
+        const bool b1 = t < t1;
+        const bool b2 = !b1 && (t < t2) && (id != i1);
+        const double nt2 = b1 ? t1 : (b2 ? t : t2);
+        const int ni2 = b1 ? i1 : (b2 ? id : i2);
+        t1 = b1 ? t : t1;
+        i1 = b1 ? id : i1;
+        t2 = nt2;
+        i2 = ni2;
+    };
+    // generator j (sorted index) staged at LDS slot q: t = (|x|^2 - 2<r,x> + K) / (2<u, x - x0>)
+    auto scan_point = [&](int q, int j) {
+        double ux = 0.0, rxj = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double xk = leafx[wv][k][q];
+            ux = fma(u[k], xk, ux);
+            rxj = fma(r[k], xk, rxj);
+        }
+        if (!(ux > cth)) return;
+        const double den2 = 2.0 * (ux - ux0);
+        const double num = fma(-2.0, rxj, leafx[wv][D][q] + K);
+        if (num < den2 * (t1 + W)) {
+            const double t = num / den2;
+            if (t >= plane_tol) offer(t, j);
+        }
+    };
     // boundary planes (mirror pseudo-generators, src/raycast.jl:354-375, src/extended.jl:131-141)
     for (int p = 0; p < g.P; ++p) {
         const int id = N + p;
@@ -477,15 +590,12 @@ __global__ __launch_bounds__(256) void k_raycast64(DevGeo g, DevRays R, const in
         }
         if (!(ux0 + 2.0 * dist * nu > cth)) continue;
         const double tp = (g.ph[p] - nr) / nu;
-        if (!(tp >= plane_tol)) continue;
-        if (tp < t1) { t2 = t1; i2 = i1; t1 = tp; i1 = id; }
-        else if (tp < t2) { t2 = tp; i2 = id; }
+        if (tp >= plane_tol) offer(tp, id);
     }
-    constexpr double W = 1e-7;   // widest near-tie window of the reference (src/raycast.jl:902)
-    // packet centre: mean expected end point r + rho u over the active lanes
+    const double rho = sqrt(rho2);
+    // packet centre: mean expected end point over the active lanes
     double pc[D];
     {
-        const double rho = sqrt(rho2);
         const double wgt = act ? 1.0 : 0.0;
         double cnt = wgt;
 #pragma unroll
@@ -500,141 +610,118 @@ __global__ __launch_bounds__(256) void k_raycast64(DevGeo g, DevRays R, const in
 #pragma unroll
         for (int k = 0; k < D; ++k) pc[k] *= ic;
     }
-    // certificate ball B(tau) = { x : |x - (r + tau u)| < |x0 - (r + tau u)| }
     double bc[D], R2;
     auto set_ball = [&](double tau) {
         R2 = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             bc[k] = r[k] + tau * u[k];
-            double t = bc[k] - x0[k];
+            const double t = bc[k] - x0[k];
             R2 = fma(t, t, R2);
         }
         R2 = R2 * (1.0 + 1e-9) + 1e-300;
     };
     set_ball(t1 + W);
-    auto box_hit = [&](int node) -> bool {
-        double d2 = 0.0;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const double lo = g.blo[(int64_t)k * g.NN + node], hi = g.bhi[(int64_t)k * g.NN + node];
-            const double a = fmax(fmax(lo - bc[k], bc[k] - hi), 0.0);
-            d2 = fma(a, a, d2);
-        }
-        return act && (d2 <= R2);
-    };
     unsigned long long pairs = 0, leaves = 0;
     int sp = 1;
     stk[wv][0] = 1;
     while (sp > 0) {
         --sp;
         const int node = __builtin_amdgcn_readfirstlane(stk[wv][sp]);
-        const bool hit = box_hit(node);
+        const bool hit = act && (hv_box_dist2<D>(g, node, bc) <= R2);
         const unsigned long long m = __ballot(hit);
         if (m == 0ull) continue;
         if (node >= g.P2) {
-            const int leaf = node - g.P2;
-            const int j0 = leaf * g.L, j1 = min(j0 + g.L, N);
+            const int j0 = (node - g.P2) * g.L, j1 = min(j0 + g.L, N);
             leaves += 1;
             pairs += (unsigned long long)__popcll(m) * (unsigned long long)(j1 - j0);
+            // the wavefront stages the leaf in LDS (coalesced), then every hit lane scans it
+            if (lane < j1 - j0) {
+#pragma unroll
+                for (int k = 0; k < D; ++k) leafx[wv][k][lane] = g.xs[(int64_t)k * N + j0 + lane];
+                leafx[wv][D][lane] = g.xx[j0 + lane];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (hit) {
-                const double lim = t1 + W;
-                double tl = lim;
-                for (int j = j0; j < j1; ++j) {
-                    double ux = 0.0, rxj = 0.0;
-#pragma unroll
-                    for (int k = 0; k < D; ++k) {
-                        const double xk = g.xs[(int64_t)k * N + j];
-                        ux = fma(u[k], xk, ux);
-                        rxj = fma(r[k], xk, rxj);
-                    }
-                    if (!(ux > cth)) continue;
-                    const double den2 = 2.0 * (ux - ux0);
-                    const double num = fma(-2.0, rxj, g.xx[j] + K);
-                    if (num < den2 * (t1 + W)) {
-                        const double t = num / den2;
-                        if (t >= plane_tol) {
-                            if (t < t1) { t2 = t1; i2 = i1; t1 = t; i1 = j; }
-                            else if (t < t2) { t2 = t; i2 = j; }
-                        }
-                    }
-                }
-                if (t1 + W < tl) set_ball(t1 + W);
+                const double before = t1;
+                for (int q = 0; q < j1 - j0; ++q) scan_point(q, j0 + q);
+                if (t1 < before) set_ball(t1 + W);
             }
+            __builtin_amdgcn_wave_barrier();
         } else {
-            // children, nearer (to the packet centre) popped first
-            const int a = 2 * node, b = 2 * node + 1;
-            double da = 0.0, db = 0.0;
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                const double la = g.blo[(int64_t)k * g.NN + a], ha = g.bhi[(int64_t)k * g.NN + a];
-                const double lb = g.blo[(int64_t)k * g.NN + b], hb = g.bhi[(int64_t)k * g.NN + b];
-                const double xa = fmax(fmax(la - pc[k], pc[k] - ha), 0.0);
-                const double xb = fmax(fmax(lb - pc[k], pc[k] - hb), 0.0);
-                da = fma(xa, xa, da);
-                db = fma(xb, xb, db);
-            }
-            const int nearc = (da <= db) ? a : b, farc = (da <= db) ? b : a;
-            stk[wv][sp] = farc;
-            stk[wv][sp + 1] = nearc;
+            // children, the one nearer the packet centre popped first
+            const int a = 2 * node, b = a + 1;
+            const double da = hv_box_dist2<D>(g, a, pc), db = hv_box_dist2<D>(g, b, pc);
+            stk[wv][sp] = (da <= db) ? b : a;
+            stk[wv][sp + 1] = (da <= db) ? a : b;
             sp += 2;
         }
     }
-    // counters (one atomic per wavefront)
     if (lane == 0) {
         atomicAdd(&ctr[0], pairs);
         atomicAdd(&ctr[1], leaves);
     }
-    if (!act) return;
-    if (i1 < 0) { status[rid] = HV_ST_NONE; return; }
-    // original indices and the reference's form of t (src/raycast.jl:424-431)
-    auto orig = [&](int i) { return i < N ? g.sidx[i] : i; };
-    auto t_ref = [&](int w, double tscan, double& score, double& cosang) -> double {
-        if (w >= N) {
-            const int p = w - N;
-            double nu = 0.0, dist = g.ph[p];
+    bool tie = false;
+    if (act) {
+        if (i1 < 0) {
+            status[rid] = HV_ST_NONE;
+        } else {
+            // original indices and the reference's form of t (src/raycast.jl:424-431)
+            auto orig = [&](int i) { return i < N ? g.sidx[i] : i; };
+            auto t_ref = [&](int w, double tscan, double& score, double& cosang) -> double {
+                if (w >= N) {
+                    const int p = w - N;
+                    double nu = 0.0, dist = g.ph[p];
 #pragma unroll
-            for (int k = 0; k < D; ++k) { nu = fma(g.pn[p * D + k], u[k], nu); dist = fma(-g.pn[p * D + k], x0[k], dist); }
-            score = 2.0 * dist * nu;
-            cosang = nu;
-            return tscan;
-        }
-        double num = 0.0, den = 0.0, dd = 0.0;
+                    for (int k = 0; k < D; ++k) {
+                        nu = fma(g.pn[p * D + k], u[k], nu);
+                        dist = fma(-g.pn[p * D + k], x0[k], dist);
+                    }
+                    score = 2.0 * dist * nu;
+                    cosang = nu;
+                    return tscan;
+                }
+                double num = 0.0, den = 0.0, dd = 0.0;
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const double xk = g.xo[(int64_t)k * N + w];
-            const double dx = xk - x0[k];
-            num = fma(dx, x0[k] + xk - 2.0 * r[k], num);
-            den = fma(u[k], dx, den);
-            dd = fma(dx, dx, dd);
-        }
-        score = den;
-        cosang = den / sqrt(dd);
-        return num / (2.0 * den);
-    };
-    int w1 = orig(i1);
-    double s1, c1;
-    double tw = t_ref(w1, t1, s1, c1);
-    int st = HV_ST_FOUND;
-    if (i2 >= 0) {
-        const int w2 = orig(i2);
-        double s2, c2;
-        const double tw2 = t_ref(w2, t2, s2, c2);
-        double rn = 0.0;
+                for (int k = 0; k < D; ++k) {
+                    const double xk = g.xo[(int64_t)k * N + w];
+                    const double dx = xk - x0[k];
+                    num = fma(dx, x0[k] + xk - 2.0 * r[k], num);
+                    den = fma(u[k], dx, den);
+                    dd = fma(dx, dx, dd);
+                }
+                score = den;
+                cosang = den / sqrt(dd);
+                return num / (2.0 * den);
+            };
+            int w1 = orig(i1);
+            double s1, c1;
+            double tw = t_ref(w1, t1, s1, c1);
+            int st = HV_ST_FOUND;
+            if (i2 >= 0) {
+                const int w2 = orig(i2);
+                double s2, c2;
+                const double tw2 = t_ref(w2, t2, s2, c2);
+                double rn = 0.0;
 #pragma unroll
-        for (int k = 0; k < D; ++k) rn = fma(r[k], r[k], rn);
-        const double full_error = (tw * du + sqrt(rn) * 1e-15) / c1;
-        const double window = fmin(1e-7, (10.0 + D) * full_error);
-        if (tw2 < tw || tw2 <= tw + window) {
-            // near tie: the larger <u, x - x0> wins (src/raycast.jl:903-916)
-            st = HV_ST_TIE;
-            if (s2 > s1) { w1 = w2; tw = tw2; }
+                for (int k = 0; k < D; ++k) rn = fma(r[k], r[k], rn);
+                const double full_error = (tw * du + sqrt(rn) * 1e-15) / c1;
+                const double window = fmin(1e-7, (10.0 + D) * full_error);
+                if (tw2 < tw || tw2 <= tw + window) {
+                    // near tie: the larger <u, x - x0> wins (src/raycast.jl:903-916)
+                    st = HV_ST_TIE;
+                    tie = true;
+                    if (s2 > s1) { w1 = w2; tw = tw2; }
+                }
+            }
+            win[rid] = w1;
+            status[rid] = st;
+            tout[rid] = tw;
         }
     }
-    win[rid] = w1;
-    status[rid] = st;
-    tout[rid] = tw;
-    if (st == HV_ST_TIE) atomicAdd(&ctr[2], 1ull);
+    hv_wave_count(&ctr[2], tie);
 }
 
 // New key = edge + winner (sorted) and its circumcentre, written to the provisional slot.
@@ -647,9 +734,8 @@ __global__ __launch_bounds__(256) void k_finalize(DevGeo g, DevRays R, const int
     if (status[rid] <= 0) return;
     const int ray = order[rid];
     const int v = R.v[ray], pos = R.drop[ray];
-    int key[D + 1];
-#pragma unroll
-    for (int j = 0; j <= D; ++j) key[j] = vkey[(int64_t)v * (D + 1) + j];
+    // key read straight from global memory: a private copy indexed by `pos` would live in scratch
+    const int* key = vkey + (int64_t)v * (D + 1);
     int e[D];
     hv_edge_of<D>(key, pos, e);
     const int w1 = win[rid];
@@ -795,12 +881,30 @@ double vertices_per_point(int D) {
 
 }  // namespace
 
+// Chunk selection (64-ray blocks interleaved over K chunks); a ray of a later chunk is
+// dropped when its edge got its second end point from a vertex found by an earlier
+// chunk of the same level (the sequential walk's edge hash would not walk it either).
+__global__ void k_chunk(const int* __restrict__ ord, int nr, int c, int K, const int64_t* __restrict__ slot,
+                        const unsigned long long* __restrict__ etab, int* __restrict__ cand,
+                        uint8_t* __restrict__ keep) {
+    // chunk c holds the sorted 64-ray blocks c, c+K, c+2K, ...: packets stay spatially
+    // coherent while rays that end at the same vertex (neighbours in the sort) tend to
+    // fall into different chunks
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t pos = ((((int64_t)(i >> 6)) * K + c) << 6) + (i & 63);
+    if (pos >= nr) return;
+    const int ray = ord[pos];
+    cand[i] = ray;
+    keep[i] = (c == 0) ? 1 : ((etab[slot[ray]] & HV_EDGE_DONE) ? 0 : 1);
+}
+
 template <int D>
 int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     hv_opts opt{};
     if (o) opt = *o;
     const int L = opt.leaf_size > 0 ? opt.leaf_size : 32;
     const double plane_tol = opt.plane_tolerance > 0 ? opt.plane_tolerance : 1e-12;
+    const int KCH = opt.reserved0 > 0 ? opt.reserved0 : 4;   // chunks per level
     if (opt.precision != HV_PREC_FP64) return hv_fail(c, HV_ERR_UNSUPPORTED, "precision mode not built yet");
     int rc = check_box(c, D);
     if (rc) return rc;
@@ -811,13 +915,13 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     rc = build_tree<D>(c, L, T);
     if (rc) return rc;
     DevGeo& g = T.g;
-    // capacities
     const int nseed = 1 << D;
-    int64_t vcap = opt.vertex_capacity > 0 ? opt.vertex_capacity
-                                           : (int64_t)(2.5 * vertices_per_point(D) * N) + 4 * nseed + 4096;
+    // initial sizes from the expected vertex count (or the caller's hint); all tables grow on demand
+    const double est = opt.vertex_capacity > 0 ? (double)opt.vertex_capacity : vertices_per_point(D) * N;
+    int64_t vcap = (int64_t)(2.0 * est) + 4 * nseed + 1024;
     auto hpow = [](int64_t n) { uint64_t p = 1024; while ((int64_t)p < n) p <<= 1; return p; };
-    uint64_t hsize = hpow(2 * (int64_t)(1.2 * vertices_per_point(D) * N) + 4 * nseed);
-    uint64_t esize = hpow((int64_t)((D + 1) * 1.2 * vertices_per_point(D) * N) + 16 * nseed);
+    uint64_t hsize = hpow((int64_t)(2.4 * est) + 4 * nseed);
+    uint64_t esize = hpow((int64_t)((D + 1) * 1.2 * est) + 16 * nseed);
     int* vkey = pget<int>(c, "vkey", (size_t)vcap * (D + 1));
     double* vx = pget<double>(c, "vx", (size_t)vcap * D);
     int* ids = pget<int>(c, "ids", (size_t)vcap);
@@ -831,59 +935,114 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     HVCK(hipMemsetAsync(etab, 0, esize * sizeof(unsigned long long), c->stream));
     HVCK(hipMemsetAsync(ctr, 0, 16 * sizeof(unsigned long long), c->stream));
     hv_stats_t st{};
-    // seeds
-    int64_t vnext = 0;   // next provisional id
-    int64_t nlive = 0;   // live vertices (ids[0..nlive))
-    {
-        int* status = pget<int>(c, "status", nseed);
-        uint8_t* win = pget<uint8_t>(c, "win", nseed);
-        if (!status || !win) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
-        k_seed_corners<D><<<nseed, 256, 0, c->stream>>>(g, vkey, vx, vnext, status);
+    unsigned long long hctr[16];
+    int64_t vnext = 0;   // next provisional vertex id
+    int64_t nlive = 0;   // live vertices: ids[0 .. nlive)
+
+    // counters in ctr[]: 0 pairs, 1 leaves, 2 ties, 4 unique edges, 5 edge overflow, 8 dup, 9 vertex overflow
+    auto read_ctr = [&]() -> int {
+        HVCK(hipMemcpyAsync(hctr, ctr, sizeof hctr, hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        return HV_OK;
+    };
+    // edge hash below half load for `incoming` more registrations (sets `grown` on rehash)
+    bool grown = false;
+    auto ensure_edges = [&](int64_t incoming) -> int {
+        int r_ = read_ctr();
+        if (r_) return r_;
+        if ((int64_t)(hctr[4] + incoming) * 2 <= (int64_t)esize) return HV_OK;
+        grown = true;
+        uint64_t nsz = esize;
+        while ((int64_t)(hctr[4] + incoming) * 2 > (int64_t)nsz) nsz <<= 1;
+        DBuf old = c->pool->take("etab");
+        unsigned long long* ne = pget<unsigned long long>(c, "etab", nsz);
+        if (!ne) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (edge hash)");
+        HVCK(hipMemsetAsync(ne, 0, nsz * sizeof(unsigned long long), c->stream));
+        k_rehash_edges<D><<<nblk(esize, 256), 256, 0, c->stream>>>((unsigned long long*)old.p, esize - 1, vkey, ne, nsz - 1);
         HVCK(hipGetLastError());
-        k_insert<D><<<nblk(nseed, 256), 256, 0, c->stream>>>(status, nseed, vkey, vnext, htab, hsize - 1, win, ctr + 8);
+        HVCK(hipStreamSynchronize(c->stream));
+        HVCK(hipFree(old.p));
+        etab = ne;
+        esize = nsz;
+        return HV_OK;
+    };
+    // vertex table and vertex hash room for `incoming` more candidates
+    auto ensure_vertices = [&](int64_t incoming) -> int {
+        if (vnext + incoming > (int64_t)INT32_MAX) return hv_fail(c, HV_ERR_CAPACITY, "more than 2^31 vertex slots");
+        if (vnext + incoming > vcap) {
+            const int64_t ncap = std::max<int64_t>(vcap * 3 / 2, vnext + incoming + 1024);
+            int* nk2 = (int*)c->pool->get("vkey", (size_t)ncap * (D + 1) * sizeof(int), true, c->stream);
+            double* nx2 = (double*)c->pool->get("vx", (size_t)ncap * D * sizeof(double), true, c->stream);
+            int* ni2 = (int*)c->pool->get("ids", (size_t)ncap * sizeof(int), true, c->stream);
+            if (!nk2 || !nx2 || !ni2) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (vertex table)");
+            vkey = nk2; vx = nx2; ids = ni2; vcap = ncap;
+        }
+        if ((nlive + incoming) * 2 > (int64_t)hsize) {
+            uint64_t nsz = hsize;
+            while ((nlive + incoming) * 2 > (int64_t)nsz) nsz <<= 1;
+            int* nh = (int*)c->pool->get("htab", nsz * sizeof(int), false, c->stream);
+            if (!nh) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (vertex hash)");
+            HVCK(hipMemsetAsync(nh, 0xff, nsz * sizeof(int), c->stream));
+            if (nlive) {
+                k_rehash_vertices<D><<<nblk(nlive, 256), 256, 0, c->stream>>>(ids, nlive, vkey, nh, nsz - 1);
+                HVCK(hipGetLastError());
+            }
+            htab = nh;
+            hsize = nsz;
+        }
+        return HV_OK;
+    };
+    // candidates at provisional ids base .. base+n with status > 0 -> hash -> new ids appended at ids[nlive]
+    auto insert_and_select = [&](const int* status, int n, int64_t base, int& nsel) -> int {
+        uint8_t* win = pget<uint8_t>(c, "win", n);
+        if (!win) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (win)");
+        k_insert<D><<<nblk(n, 256), 256, 0, c->stream>>>(status, n, vkey, base, htab, hsize - 1, win, ctr + 8);
         HVCK(hipGetLastError());
         size_t tb = 0;
-        hipcub::CountingInputIterator<int> it((int)vnext);
-        HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, win, ids + nlive, dcount, nseed, c->stream));
+        hipcub::CountingInputIterator<int> it((int)base);
+        HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, win, ids + nlive, dcount, n, c->stream));
         void* tmp = c->pool->get("cubtmp2", tb, false, c->stream);
-        HVCK(hipcub::DeviceSelect::Flagged(tmp, tb, it, win, ids + nlive, dcount, nseed, c->stream));
-        int nsel = 0;
+        if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
+        HVCK(hipcub::DeviceSelect::Flagged(tmp, tb, it, win, ids + nlive, dcount, n, c->stream));
         HVCK(hipMemcpyAsync(&nsel, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HVCK(hipStreamSynchronize(c->stream));
+        return HV_OK;
+    };
+    auto register_new = [&](int64_t first, int count, int64_t* reg) -> int {
+        int r_ = ensure_edges((int64_t)count * (D + 1));
+        if (r_) return r_;
+        k_register<D><<<nblk((int64_t)count * (D + 1), 256), 256, 0, c->stream>>>(ids + first, count, vkey, N, etab,
+                                                                                  esize - 1, reg, ctr + 4);
+        HVCK(hipGetLastError());
+        return HV_OK;
+    };
+
+    // seeds: the 2^d box corners
+    int64_t* reg_cur = pget<int64_t>(c, "regA", (size_t)nseed * (D + 1));
+    if (!reg_cur) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+    {
+        int* status = pget<int>(c, "status", nseed);
+        if (!status) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+        k_seed_corners<D><<<nseed, 256, 0, c->stream>>>(g, vkey, vx, vnext, status);
+        HVCK(hipGetLastError());
+        int nsel = 0;
+        rc = insert_and_select(status, nseed, vnext, nsel);
+        if (rc) return rc;
+        rc = register_new(nlive, nsel, reg_cur);
+        if (rc) return rc;
         vnext += nseed;
         nlive += nsel;
     }
+    bool reg_is_a = true;
     int64_t fbeg = 0, fcount = nlive;   // frontier = ids[fbeg .. fbeg+fcount)
     double ms_ray = 0.0;
-    int64_t levels = 0;
-    unsigned long long hctr[16];
+    int64_t levels = 0, skipped = 0;
     while (fcount > 0) {
-        ++levels;
+        if (++levels > 1000000) return hv_fail(c, HV_ERR_CAPACITY, "walk did not terminate");
         const int64_t nslots = fcount * (D + 1);
-        // grow the edge hash when it could pass half load
-        HVCK(hipMemcpyAsync(hctr, ctr, sizeof hctr, hipMemcpyDeviceToHost, c->stream));
-        HVCK(hipStreamSynchronize(c->stream));
-        if ((int64_t)(hctr[4] + nslots) * 2 > (int64_t)esize) {
-            uint64_t nsz = esize;
-            while ((int64_t)(hctr[4] + nslots) * 2 > (int64_t)nsz) nsz <<= 1;
-            DBuf old = c->pool->take("etab");
-            unsigned long long* ne = pget<unsigned long long>(c, "etab", nsz);
-            if (!ne) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (edge hash)");
-            HVCK(hipMemsetAsync(ne, 0, nsz * sizeof(unsigned long long), c->stream));
-            k_rehash_edges<D><<<nblk(esize, 256), 256, 0, c->stream>>>((unsigned long long*)old.p, esize - 1, vkey, ne, nsz - 1);
-            HVCK(hipGetLastError());
-            HVCK(hipStreamSynchronize(c->stream));
-            HVCK(hipFree(old.p));
-            etab = ne;
-            esize = nsz;
-        }
-        int64_t* reg = pget<int64_t>(c, "reg", nslots);
-        if (!reg) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (reg)");
-        const int* frontier = ids + fbeg;
-        // ctr[4] counts unique edges, ctr[5] edge-table overflow
-        k_register<D><<<nblk(nslots, 256), 256, 0, c->stream>>>(frontier, (int)fcount, vkey, N, etab, esize - 1, reg, ctr + 4);
-        HVCK(hipGetLastError());
+        if (nslots > (int64_t)INT32_MAX) return hv_fail(c, HV_ERR_CAPACITY, "level too large");
         DevRays R;
+        R.slot = pget<int64_t>(c, "ray_slot", nslots);
         R.v = pget<int>(c, "ray_v", nslots);
         R.drop = pget<int>(c, "ray_drop", nslots);
         R.u = pget<double>(c, "ray_u", nslots * D);
@@ -894,38 +1053,26 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         uint64_t* code2 = pget<uint64_t>(c, "ray_code2", nslots);
         int* ord0 = pget<int>(c, "ray_ord0", nslots);
         int* ord = pget<int>(c, "ray_ord", nslots);
-        if (!R.v || !R.drop || !R.u || !R.c || !R.du || !R.code || !code2 || !ord0 || !ord)
+        int* cand = pget<int>(c, "ray_cand", nslots);
+        int* ordc = pget<int>(c, "ray_ordc", nslots);
+        uint8_t* keep = pget<uint8_t>(c, "ray_keep", nslots);
+        int* status = pget<int>(c, "status", nslots);
+        int* rwin = pget<int>(c, "ray_win", nslots);
+        double* rt = pget<double>(c, "ray_t", nslots);
+        int64_t* reg_next = pget<int64_t>(c, reg_is_a ? "regB" : "regA", nslots * (D + 1));
+        if (!R.slot || !R.v || !R.drop || !R.u || !R.c || !R.du || !R.code || !code2 || !ord0 || !ord || !cand ||
+            !ordc || !keep || !status || !rwin || !rt || !reg_next)
             return hv_fail(c, HV_ERR_DEVICE, "out of device memory (rays)");
         HVCK(hipMemsetAsync(dcount, 0, sizeof(int), c->stream));
-        k_emit<D><<<nblk(nslots, 256), 256, 0, c->stream>>>(frontier, (int)fcount, vkey, vx, g, etab, reg, R, dcount, plane_tol);
+        k_emit<D><<<nblk(nslots, 256), 256, 0, c->stream>>>(ids + fbeg, (int)fcount, vkey, vx, g, etab, reg_cur, R,
+                                                           dcount, plane_tol);
         HVCK(hipGetLastError());
         int nr = 0;
         HVCK(hipMemcpyAsync(&nr, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HVCK(hipMemcpyAsync(hctr, ctr, sizeof hctr, hipMemcpyDeviceToHost, c->stream));
-        HVCK(hipStreamSynchronize(c->stream));
+        rc = read_ctr();
+        if (rc) return rc;
         if (hctr[5]) return hv_fail(c, HV_ERR_CAPACITY, "edge hash overflow");
         if (nr == 0) break;
-        // vertex table / hash capacity for this level's candidates
-        if (vnext + nr > vcap) {
-            int64_t ncap = std::max<int64_t>(vcap * 3 / 2, vnext + nr + 1024);
-            int* nk2 = (int*)c->pool->get("vkey", (size_t)ncap * (D + 1) * sizeof(int), true, c->stream);
-            double* nx2 = (double*)c->pool->get("vx", (size_t)ncap * D * sizeof(double), true, c->stream);
-            int* ni2 = (int*)c->pool->get("ids", (size_t)ncap * sizeof(int), true, c->stream);
-            if (!nk2 || !nx2 || !ni2) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (vertex table)");
-            vkey = nk2; vx = nx2; ids = ni2; vcap = ncap;
-            frontier = ids + fbeg;
-        }
-        if ((nlive + nr) * 2 > (int64_t)hsize) {
-            uint64_t nsz = hsize;
-            while ((nlive + nr) * 2 > (int64_t)nsz) nsz <<= 1;
-            int* nh = (int*)c->pool->get("htab", nsz * sizeof(int), false, c->stream);
-            if (!nh) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (vertex hash)");
-            HVCK(hipMemsetAsync(nh, 0xff, nsz * sizeof(int), c->stream));
-            k_rehash_vertices<D><<<nblk(nlive, 256), 256, 0, c->stream>>>(ids, nlive, vkey, nh, nsz - 1);
-            HVCK(hipGetLastError());
-            htab = nh;
-            hsize = nsz;
-        }
         // sort rays by the Morton code of their expected end point
         {
             size_t tb = 0;
@@ -936,49 +1083,82 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             HVCK(hipGetLastError());
             HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, nr, 0, 64, c->stream));
         }
-        int* status = pget<int>(c, "status", nr);
-        uint8_t* win = pget<uint8_t>(c, "win", nr);
-        int* rwin = pget<int>(c, "ray_win", nr);
-        double* rt = pget<double>(c, "ray_t", nr);
-        if (!status || !win || !rwin || !rt) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (status)");
-        HVCK(hipEventRecord(r0, c->stream));
-        k_raycast64<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, ord, nr, vkey, vx, status, rwin, rt, plane_tol, ctr + 0);
-        HVCK(hipGetLastError());
-        HVCK(hipEventRecord(r1, c->stream));
-        k_finalize<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, ord, nr, vkey, vx, vnext, status, rwin);
-        HVCK(hipGetLastError());
-        k_insert<D><<<nblk(nr, 256), 256, 0, c->stream>>>(status, nr, vkey, vnext, htab, hsize - 1, win, ctr + 8);
-        HVCK(hipGetLastError());
-        size_t tb = 0;
-        hipcub::CountingInputIterator<int> it((int)vnext);
-        HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, win, ids + nlive, dcount, nr, c->stream));
-        void* tmp = c->pool->get("cubtmp2", tb, false, c->stream);
-        if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
-        HVCK(hipcub::DeviceSelect::Flagged(tmp, tb, it, win, ids + nlive, dcount, nr, c->stream));
-        int nsel = 0;
-        HVCK(hipMemcpyAsync(&nsel, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HVCK(hipStreamSynchronize(c->stream));
-        float ms = 0.f;
-        HVCK(hipEventElapsedTime(&ms, r0, r1));
-        ms_ray += ms;
-        st.rays += nr;
-        fbeg = nlive;
-        fcount = nsel;
-        nlive += nsel;
-        vnext += nr;
+        const int64_t level_begin = nlive;
+        // small levels are not split unless the caller asked for a chunk count explicitly
+        const int K = (opt.reserved0 > 0) ? std::min(KCH, nr) : (nr >= 65536 ? KCH : 1);
+        const int nfull = nr >> 6, rem = nr & 63;
+        for (int ch = 0; ch < K; ++ch) {
+            const int nb = nfull > ch ? (nfull - ch + K - 1) / K : 0;
+            const int nc = nb * 64 + ((nfull % K == ch) ? rem : 0);
+            if (nc <= 0) continue;
+            k_chunk<<<nblk(nc, 256), 256, 0, c->stream>>>(ord, nr, ch, K, R.slot, etab, cand, keep);
+            HVCK(hipGetLastError());
+            int n = nc;
+            const int* rays_c = cand;
+            if (ch > 0) {
+                size_t tb = 0;
+                HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb, cand, keep, ordc, dcount, nc, c->stream));
+                void* tmp = c->pool->get("cubtmp2", tb, false, c->stream);
+                if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
+                HVCK(hipcub::DeviceSelect::Flagged(tmp, tb, cand, keep, ordc, dcount, nc, c->stream));
+                HVCK(hipMemcpyAsync(&n, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+                HVCK(hipStreamSynchronize(c->stream));
+                rays_c = ordc;
+                skipped += nc - n;
+            }
+            if (n == 0) continue;
+            rc = ensure_vertices(n);
+            if (rc) return rc;
+            HVCK(hipEventRecord(r0, c->stream));
+            k_raycast64<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, rays_c, n, vkey, vx, status, rwin, rt, plane_tol, ctr + 0);
+            HVCK(hipGetLastError());
+            HVCK(hipEventRecord(r1, c->stream));
+            k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, rays_c, n, vkey, vx, vnext, status, rwin);
+            HVCK(hipGetLastError());
+            int nsel = 0;
+            rc = insert_and_select(status, n, vnext, nsel);
+            if (rc) return rc;
+            float ms = 0.f;
+            HVCK(hipEventElapsedTime(&ms, r0, r1));
+            ms_ray += ms;
+            if (nsel) {
+                grown = false;
+                rc = register_new(nlive, nsel, reg_next + (nlive - level_begin) * (D + 1));
+                if (rc) return rc;
+                if (grown) {
+                    // slots stored before the rehash: this level's rays and its earlier new vertices
+                    k_relookup_rays<D><<<nblk(nr, 256), 256, 0, c->stream>>>(R, nr, vkey, etab, esize - 1);
+                    HVCK(hipGetLastError());
+                    if (nlive > level_begin) {
+                        k_relookup_vertices<D><<<nblk((nlive - level_begin) * (D + 1), 256), 256, 0, c->stream>>>(
+                            ids + level_begin, (int)(nlive - level_begin), vkey, etab, esize - 1, reg_next);
+                        HVCK(hipGetLastError());
+                    }
+                }
+            }
+            st.rays += n;
+            nlive += nsel;
+            vnext += n;
+        }
+        reg_cur = reg_next;
+        reg_is_a = !reg_is_a;
+        fbeg = level_begin;
+        fcount = nlive - level_begin;
     }
     HVCK(hipEventRecord(e1, c->stream));
-    HVCK(hipMemcpyAsync(hctr, ctr, sizeof hctr, hipMemcpyDeviceToHost, c->stream));
-    HVCK(hipStreamSynchronize(c->stream));
+    rc = read_ctr();
+    if (rc) return rc;
     float ms_all = 0.f;
     HVCK(hipEventElapsedTime(&ms_all, e0, e1));
     if (hctr[9]) return hv_fail(c, HV_ERR_CAPACITY, "vertex hash overflow");
+    if (hctr[5]) return hv_fail(c, HV_ERR_CAPACITY, "edge hash overflow");
     st.levels = levels;
     st.vertices = nlive;
     st.pairs_scanned = (int64_t)hctr[0];
     st.leaves_visited = (int64_t)hctr[1];
     st.near_ties = (int64_t)hctr[2];
     st.duplicates = (int64_t)hctr[8];
+    st.rays_skipped = skipped;
     st.ms_total = ms_all;
     st.ms_raycast = ms_ray;
     hv_mesh* m = new hv_mesh();
@@ -1029,9 +1209,8 @@ __global__ __launch_bounds__(256) void k_setup_rays(int n, const int* __restrict
                              const int* __restrict__ drop, DevGeo g, DevRays R, double plane_tol) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    int key[D + 1];
-#pragma unroll
-    for (int j = 0; j <= D; ++j) key[j] = vkey[(int64_t)t * (D + 1) + j];
+    // key read straight from global memory: a private copy indexed by `pos` would live in scratch
+    const int* key = vkey + (int64_t)t * (D + 1);
     double r[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) r[k] = vx[(int64_t)t * D + k];

@@ -38,7 +38,7 @@ class hv_stats_t(C.Structure):
                 ("pairs_scanned", C.c_int64), ("leaves_visited", C.c_int64), ("near_ties", C.c_int64),
                 ("degenerate", C.c_int64), ("fallbacks", C.c_int64), ("fp64_rechecks", C.c_int64),
                 ("retries", C.c_int64), ("unbounded", C.c_int64), ("max_t_relerr", C.c_double),
-                ("ms_total", C.c_double), ("ms_raycast", C.c_double)]
+                ("ms_total", C.c_double), ("ms_raycast", C.c_double), ("rays_skipped", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -163,10 +163,12 @@ class Context:
             bcp = np.ascontiguousarray(bc, dtype=np.int16).ctypes.data_as(C.POINTER(C.c_int16))
         self._check(lib().hv_set_boundary(self._h, npl, _dp(base), _dp(normal), bcp))
 
-    def build_mesh(self, precision: str = "fp64", leaf_size: int = 0, vertex_capacity: int = 0) -> Mesh:
+    def build_mesh(self, precision: str = "fp64", leaf_size: int = 0, vertex_capacity: int = 0,
+                   chunks: int = 0) -> Mesh:
         o = hv_opts()
         o.precision = PRECISIONS[precision]
         o.leaf_size = leaf_size
+        o.reserved0 = chunks
         o.vertex_capacity = vertex_capacity
         m = C.c_void_p()
         self._check(lib().hv_build_mesh(self._h, C.byref(o), C.byref(m)))

@@ -47,7 +47,7 @@ typedef struct hv_opts {
     int32_t precision;        /* HV_PREC_* */
     int32_t leaf_size;        /* generators per tree leaf (0 = default 32) */
     int32_t rays_per_block;   /* threads per raycast block (0 = default 256) */
-    int32_t reserved0;
+    int32_t reserved0;        /* chunks per walk level (0 = default 4) */
     double plane_tolerance;   /* src/raycast-types.jl:229 (0 = 1e-12) */
     double ray_tol;           /* src/raycast-types.jl:230 (0 = 1e-12) */
     int64_t vertex_capacity;  /* device vertex-table capacity hint (0 = auto) */
@@ -70,6 +70,7 @@ typedef struct hv_stats_t {
     double max_t_relerr;      /* mixed precision: max |t_lowprec - t_fp64| / t_fp64 of winners */
     double ms_total;          /* device time of the last hv_build_mesh */
     double ms_raycast;        /* device time spent in the candidate-scan kernel */
+    int64_t rays_skipped;     /* rays dropped because an earlier chunk of their level closed their edge */
 } hv_stats_t;
 
 /* Context and inputs ------------------------------------------------------ */
