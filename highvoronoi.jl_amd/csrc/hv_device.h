@@ -28,8 +28,15 @@ struct DevGeo {
     const double* __restrict__ ph;   // P offsets <n_p, b_p>
     const double* __restrict__ blo;  // D x NN
     const double* __restrict__ bhi;
+    const float* __restrict__ cb;    // 8-ary tree: child boxes of internal node n at cb[(n*8+c)*2D ..], lo then hi
+    int h;                           // 8-ary tree height (leaves at level h, 8^h padded leaves)
+    const int* __restrict__ rank;    // original index -> sorted position
+    const int* __restrict__ knn;     // HV_KNN nearest generators of each sorted position (sorted indices)
     double dlo[HV_MAXD], dhi[HV_MAXD];  // domain box (Morton frame)
+    double xmax;                        // bound on |x| over the domain (rounding bounds)
 };
+
+#define HV_KNN 16
 
 struct DevRays {
     int64_t* slot;   // edge-hash slot of the walked edge
@@ -38,8 +45,12 @@ struct DevRays {
     double* u;       // D per ray
     double* c;       // half-space threshold (src/raycast.jl:802-805)
     double* du;      // orthogonality defect of u (src/edgeiteratebase.jl:64-74)
-    double* tinit;   // certificate radius parameter for the first search
-    uint64_t* code;  // Morton code of the expected end point
+    double* tinit;   // unused
+    uint64_t* code;  // Morton code of the initial certificate-ball centre (packet order)
+    double* t1;      // best two candidates after the start phase (planes + nearest neighbours)
+    double* t2;
+    int* i1;
+    int* i2;
 };
 
 // Wavefront-aggregated atomics: one atomic per wave instead of one per lane (counters

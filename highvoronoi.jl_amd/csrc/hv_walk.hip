@@ -62,11 +62,13 @@ __global__ void k_points_in(const double* __restrict__ xin, int N, DevGeo g, dou
 
 template <int D>
 __global__ void k_points_sorted(const double* __restrict__ xo, const int* __restrict__ idx, int N,
-                                double* __restrict__ xs, double* __restrict__ xx, int* __restrict__ sidx) {
+                                double* __restrict__ xs, double* __restrict__ xx, int* __restrict__ sidx,
+                                int* __restrict__ rank) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
     int gi = idx[j];
     sidx[j] = gi;
+    rank[gi] = j;
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -106,6 +108,111 @@ __global__ void k_node_boxes(int first, int count, int NN, double* __restrict__ 
         blo[(int64_t)k * NN + node] = fmin(blo[(int64_t)k * NN + 2 * node], blo[(int64_t)k * NN + 2 * node + 1]);
         bhi[(int64_t)k * NN + node] = fmax(bhi[(int64_t)k * NN + 2 * node], bhi[(int64_t)k * NN + 2 * node + 1]);
     }
+}
+
+// HV_KNN nearest generators of every generator (sorted index space), used to start each
+// ray's search with a small certificate ball.  Own leaf first, then a stackless walk.
+template <int D>
+__global__ __launch_bounds__(256) void k_knn(DevGeo g, int* __restrict__ knn) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= g.N) return;
+    const int N = g.N;
+    double p[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) p[k] = g.xs[(int64_t)k * N + j];
+    double bd[HV_KNN];
+    int bi[HV_KNN];
+#pragma unroll
+    for (int m = 0; m < HV_KNN; ++m) { bd[m] = 1e300; bi[m] = j; }
+    auto consider = [&](int q) {
+        if (q == j) return;
+        double d2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double a = g.xs[(int64_t)k * N + q] - p[k];
+            d2 = fma(a, a, d2);
+        }
+        if (!(d2 < bd[HV_KNN - 1])) return;
+        // sorted insert with static indices only
+        int pos = 0;
+#pragma unroll
+        for (int m = 0; m < HV_KNN; ++m) pos += (bd[m] <= d2) ? 1 : 0;
+#pragma unroll
+        for (int m = HV_KNN - 1; m > 0; --m) {
+            bd[m] = (m > pos) ? bd[m - 1] : ((m == pos) ? d2 : bd[m]);
+            bi[m] = (m > pos) ? bi[m - 1] : ((m == pos) ? q : bi[m]);
+        }
+        bd[0] = (pos == 0) ? d2 : bd[0];
+        bi[0] = (pos == 0) ? q : bi[0];
+    };
+    const int own = j / g.L;
+    {
+        const int j0 = own * g.L, j1 = min(j0 + g.L, N);
+        for (int q = j0; q < j1; ++q) consider(q);
+    }
+    int node = 1;
+    while (node) {
+        double d2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double lo = g.blo[(int64_t)k * g.NN + node], hi = g.bhi[(int64_t)k * g.NN + node];
+            const double a = fmax(fmax(lo - p[k], p[k] - hi), 0.0);
+            d2 = fma(a, a, d2);
+        }
+        if (d2 < bd[HV_KNN - 1]) {
+            if (node < g.P2) { node = 2 * node; continue; }
+            const int leaf = node - g.P2;
+            if (leaf != own) {
+                const int j0 = leaf * g.L, j1 = min(j0 + g.L, N);
+                for (int q = j0; q < j1; ++q) consider(q);
+            }
+        }
+        while (node & 1) node >>= 1;
+        node = node ? node + 1 : 0;
+    }
+#pragma unroll
+    for (int m = 0; m < HV_KNN; ++m) knn[(int64_t)j * HV_KNN + m] = bi[m];
+}
+
+// 8-ary tree over the same leaves (level h, 8^h slots, node (l,i) has children (l+1, 8i+c)).
+// Internal node (l,i) has global id (8^l - 1)/7 + i and stores the boxes of its 8 children
+// contiguously, in FP32 rounded outward, so one load per visited node tests 8 children.
+template <int D>
+__global__ void k_tree8_level(DevGeo g, int l, int nodes_at_l, float* __restrict__ cb) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nodes_at_l * 8) return;
+    const int i = t >> 3, c = t & 7;
+    int off = 0, pw = 1;
+    for (int q = 0; q < l; ++q) { off += pw; pw *= 8; }
+    const int gid = off + i;
+    float lo[D], hi[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) { lo[k] = 1e30f; hi[k] = -1e30f; }
+    const int child = 8 * i + c;
+    if (l + 1 == g.h) {
+        const int j0 = child * g.L, j1 = min(j0 + g.L, g.N);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            double mn = 1e300, mx = -1e300;
+            for (int j = j0; j < j1; ++j) {
+                const double v = g.xs[(int64_t)k * g.N + j];
+                mn = fmin(mn, v);
+                mx = fmax(mx, v);
+            }
+            if (j1 > j0) { lo[k] = __double2float_rd(mn); hi[k] = __double2float_ru(mx); }
+        }
+    } else {
+        const int cg = off + pw + child;   // global id of the child node at level l+1
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float* b = cb + ((int64_t)cg * 8 + q) * (2 * D);
+#pragma unroll
+            for (int k = 0; k < D; ++k) { lo[k] = fminf(lo[k], b[k]); hi[k] = fmaxf(hi[k], b[D + k]); }
+        }
+    }
+    float* o = cb + ((int64_t)gid * 8 + c) * (2 * D);
+#pragma unroll
+    for (int k = 0; k < D; ++k) { o[k] = lo[k]; o[D + k] = hi[k]; }
 }
 
 // ---------------------------------------------------------------------------------
@@ -466,10 +573,7 @@ __global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, 
     for (int k = 0; k < D; ++k) R.u[(int64_t)idx * D + k] = u[k];
     R.c[idx] = c;
     R.du[idx] = du;
-    double pe[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) pe[k] = r[k] + rho * u[k];
-    R.code[idx] = hv_morton<D>(g, pe);
+    (void)rho;
 }
 
 // ---------------------------------------------------------------------------------
@@ -492,61 +596,54 @@ __device__ __forceinline__ double hv_box_dist2(const DevGeo& g, int node, const 
     return d2;
 }
 
-// One wavefront walks the generator tree for 64 rays at once (a packet).  Per ray:
-//   t(x) = <x - x0, x + x0 - 2r> / (2 <u, x - x0>)   for x strictly ahead: <x,u> > c
-// (src/raycast.jl:394-432, 802-805).  Every x with t(x) < tau lies in the certificate
-// ball B(tau) = {x : |x - (r + tau u)| < |x0 - (r + tau u)|} (the half-space part of
-// B(tau) is exactly {t < tau}, and B(tau) holds no point behind the edge for tau > 0),
-// so a tree node is skipped when its box misses every live ray's ball.
-//   phase 1: each lane scans the leaf holding its expected end point r + |r-x0| u,
-//   phase 2: the packet traverses the tree nearest-first (to the packet centre),
-//            each visited leaf is scanned by the lanes whose ball reaches it.
-// The two smallest t are kept; the reference's near-tie rule (src/raycast.jl:877-916)
-// is applied to them at the end.
+__device__ __forceinline__ void hv_wave_sync() {
+    // lanes of one wavefront: LDS operations complete in order, this only stops the
+    // compiler from moving LDS accesses across the point
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr double HV_W = 1e-7;   // widest near-tie window of the reference (src/raycast.jl:902)
+
+// Per-ray geometry shared by the start phase and the packet scan.
 template <int D>
-__global__ __launch_bounds__(256) void k_raycast64(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
-                                                   const int* __restrict__ vkey, const double* __restrict__ vx,
-                                                   int* __restrict__ status, int* __restrict__ win,
-                                                   double* __restrict__ tout, double plane_tol,
-                                                   unsigned long long* __restrict__ ctr) {
-    __shared__ int stk[4][64];
-    __shared__ double leafx[4][D + 1][64];   // per wavefront: the visited leaf, SoA, plus |x|^2
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int rid = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = rid < nrays;
-    const int ray = act ? order[rid] : order[0];
-    const int N = g.N;
+struct RayGeo {
+    double u[D], r[D], x0[D];
+    int e[D];
+    int s;
+    double cth, du, ux0, K;
+};
+
+template <int D>
+__device__ __forceinline__ void hv_load_ray(const DevGeo& g, const DevRays& R, int ray, const int* __restrict__ vkey,
+                                            const double* __restrict__ vx, RayGeo<D>& G) {
     const int v = R.v[ray], pos = R.drop[ray];
     // key read straight from global memory: a private copy indexed by `pos` would live in scratch
     const int* key = vkey + (int64_t)v * (D + 1);
-    int e[D];
-    hv_edge_of<D>(key, pos, e);
-    const int s = hv_key_at<D>(key, pos);
-    double u[D], r[D], x0[D];
+    hv_edge_of<D>(key, pos, G.e);
+    G.s = hv_key_at<D>(key, pos);
+    double rx0 = 0.0, x0x0 = 0.0;
+    G.ux0 = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        u[k] = R.u[(int64_t)ray * D + k];
-        r[k] = vx[(int64_t)v * D + k];
-        x0[k] = g.xo[(int64_t)k * N + e[0]];
+        G.u[k] = R.u[(int64_t)ray * D + k];
+        G.r[k] = vx[(int64_t)v * D + k];
+        G.x0[k] = g.xo[(int64_t)k * g.N + G.e[0]];
+        G.ux0 = fma(G.u[k], G.x0[k], G.ux0);
+        rx0 = fma(G.r[k], G.x0[k], rx0);
+        x0x0 = fma(G.x0[k], G.x0[k], x0x0);
     }
-    const double cth = R.c[ray];
-    const double du = R.du[ray];
-    double ux0 = 0.0, rx0 = 0.0, x0x0 = 0.0, rho2 = 0.0;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        ux0 = fma(u[k], x0[k], ux0);
-        rx0 = fma(r[k], x0[k], rx0);
-        x0x0 = fma(x0[k], x0[k], x0x0);
-        const double t = r[k] - x0[k];
-        rho2 = fma(t, t, rho2);
-    }
-    const double K = 2.0 * rx0 - x0x0;
-    constexpr double W = 1e-7;   // widest near-tie window of the reference (src/raycast.jl:902)
-    // best two candidates (t, index); generators in sorted index space, planes as N+p
+    G.K = 2.0 * rx0 - x0x0;
+    G.cth = R.c[ray];
+    G.du = R.du[ray];
+}
+
+// Top-2 candidates; selects only (branchy updates get sunk into a scratch store).
+struct Top2 {
     double t1 = 1e300, t2 = 1e300;
     int i1 = -1, i2 = -1;
-    // (selects only: branchy updates get sunk into a scratch store)
-    auto offer = [&](double t, int id) {
+    __device__ __forceinline__ void offer(double t, int id) {
         const bool b1 = t < t1;
         const bool b2 = !b1 && (t < t2) && (id != i1);
         const double nt2 = b1 ? t1 : (b2 ? t : t2);
@@ -555,51 +652,140 @@ __global__ __launch_bounds__(256) void k_raycast64(DevGeo g, DevRays R, const in
         i1 = b1 ? id : i1;
         t2 = nt2;
         i2 = ni2;
-    };
-    // generator j (sorted index) staged at LDS slot q: t = (|x|^2 - 2<r,x> + K) / (2<u, x - x0>)
-    auto scan_point = [&](int q, int j) {
-        double ux = 0.0, rxj = 0.0;
+    }
+};
+
+// exact t of generator j (sorted index) with coordinates xk, |x|^2 = xxj (src/raycast.jl:424-431)
+template <int D>
+__device__ __forceinline__ void hv_consider(const RayGeo<D>& G, Top2& T, const double* xk, double xxj, int j,
+                                            double plane_tol) {
+    double ux = 0.0, rxj = 0.0;
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const double xk = leafx[wv][k][q];
-            ux = fma(u[k], xk, ux);
-            rxj = fma(r[k], xk, rxj);
-        }
-        if (!(ux > cth)) return;
-        const double den2 = 2.0 * (ux - ux0);
-        const double num = fma(-2.0, rxj, leafx[wv][D][q] + K);
-        if (num < den2 * (t1 + W)) {
-            const double t = num / den2;
-            if (t >= plane_tol) offer(t, j);
-        }
-    };
-    // boundary planes (mirror pseudo-generators, src/raycast.jl:354-375, src/extended.jl:131-141)
+    for (int k = 0; k < D; ++k) {
+        ux = fma(G.u[k], xk[k], ux);
+        rxj = fma(G.r[k], xk[k], rxj);
+    }
+    if (!(ux > G.cth)) return;
+    const double den2 = 2.0 * (ux - G.ux0);
+    const double num = fma(-2.0, rxj, xxj + G.K);
+    if (num < den2 * (T.t1 + HV_W)) {
+        const double t = num / den2;
+        if (t >= plane_tol) T.offer(t, j);
+    }
+}
+
+// Start phase, one thread per ray: boundary planes (mirror pseudo-generators,
+// src/raycast.jl:354-375, src/extended.jl:131-141) and the HV_KNN nearest generators of
+// x0 (the winner is a Delaunay neighbour of x0, usually among them).  The Morton code
+// of the resulting certificate-ball centre orders the rays into packets.
+template <int D>
+__global__ __launch_bounds__(256) void k_ray_init(DevGeo g, DevRays R, int nrays, const int* __restrict__ vkey,
+                                                  const double* __restrict__ vx, double plane_tol) {
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= nrays) return;
+    const int N = g.N;
+    RayGeo<D> G;
+    hv_load_ray<D>(g, R, ray, vkey, vx, G);
+    Top2 T;
     for (int p = 0; p < g.P; ++p) {
         const int id = N + p;
-        bool member = (id == s);
+        bool member = (id == G.s);
 #pragma unroll
-        for (int j = 0; j < D; ++j) member = member || (e[j] == id);
+        for (int j = 0; j < D; ++j) member = member || (G.e[j] == id);
         if (member) continue;
         double nu = 0.0, dist = g.ph[p], nr = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const double nk = g.pn[p * D + k];
-            nu = fma(nk, u[k], nu);
-            dist = fma(-nk, x0[k], dist);
-            nr = fma(nk, r[k], nr);
+            nu = fma(nk, G.u[k], nu);
+            dist = fma(-nk, G.x0[k], dist);
+            nr = fma(nk, G.r[k], nr);
         }
-        if (!(ux0 + 2.0 * dist * nu > cth)) continue;
+        if (!(G.ux0 + 2.0 * dist * nu > G.cth)) continue;
         const double tp = (g.ph[p] - nr) / nu;
-        if (tp >= plane_tol) offer(tp, id);
+        if (tp >= plane_tol) T.offer(tp, id);
     }
-    const double rho = sqrt(rho2);
-    // packet centre: mean expected end point over the active lanes
+    const int* nb = g.knn + (int64_t)g.rank[G.e[0]] * HV_KNN;
+#pragma unroll 4
+    for (int m = 0; m < HV_KNN; ++m) {
+        const int j = nb[m];
+        double xk[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) xk[k] = g.xs[(int64_t)k * N + j];
+        hv_consider<D>(G, T, xk, g.xx[j], j, plane_tol);
+    }
+    R.t1[ray] = T.t1;
+    R.t2[ray] = T.t2;
+    R.i1[ray] = T.i1;
+    R.i2[ray] = T.i2;
+    double bc[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) bc[k] = G.r[k] + (T.t1 + HV_W) * G.u[k];
+    R.code[ray] = hv_morton<D>(g, bc);
+}
+
+// One wavefront serves 64 rays at once (a packet).  Every generator x with t(x) < tau
+// lies strictly inside the certificate ball B(tau) = {x : |x - p|^2 < |x0 - p|^2},
+// p = r + tau u: for x ahead of the edge, |x-p|^2 - |x0-p|^2 = 2<u,x-x0>(t(x) - tau), and
+// B(tau) holds no generator behind the edge.  So
+//   * a tree node is skipped when its box misses every live ray's ball, and
+//   * the leaf scan is the membership test f(x) = |x|^2 - 2<x,p> + |p|^2 - R^2 < eps,
+//     exact t only for members.
+// The walk is depth-first, children nearer to the packet centre first; each visited leaf
+// is staged in LDS by the whole wavefront (coalesced) and scanned by the lanes whose
+// ball reaches it.  The two smallest t are kept; the reference's near-tie rule
+// (src/raycast.jl:877-916) is applied to them at the end.
+// One wavefront per workgroup: packets take very different times, and a workgroup of
+// several wavefronts holds its slots until its slowest packet is done.
+template <int D>
+__global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
+                                                  const int* __restrict__ vkey, const double* __restrict__ vx,
+                                                  int* __restrict__ status, int* __restrict__ win,
+                                                  double* __restrict__ tout, double plane_tol,
+                                                  unsigned long long* __restrict__ ctr) {
+    __shared__ int stk[1][128], stkl[1][128];
+    __shared__ float cbx[1][16 * D];
+    __shared__ unsigned long long hms[1][8];
+    __shared__ float dcs[1][8];
+    __shared__ double leafx[1][D + 1][64];
+    const int lane = threadIdx.x & 63, wv = 0;
+    const int rid = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = rid < nrays;
+    const int ray = act ? order[rid] : order[0];
+    const int N = g.N;
+    RayGeo<D> G;
+    hv_load_ray<D>(g, R, ray, vkey, vx, G);
+    Top2 T;
+    T.t1 = R.t1[ray];
+    T.t2 = R.t2[ray];
+    T.i1 = R.i1[ray];
+    T.i2 = R.i2[ray];
+    // certificate ball and the membership form f(x) = |x|^2 - 2<x, bc> + fC
+    double bc[D], R2, fC, feps;
+    auto set_ball = [&](double tau) {
+        R2 = 0.0;
+        double pp = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            bc[k] = G.r[k] + tau * G.u[k];
+            const double t = bc[k] - G.x0[k];
+            R2 = fma(t, t, R2);
+            pp = fma(bc[k], bc[k], pp);
+        }
+        fC = pp - R2;
+        // rounding of the expanded form, with room: |x| <= xmax for every generator
+        const double sc = sqrt(pp) + g.xmax;
+        feps = 64.0 * 2.220446049250313e-16 * sc * sc;
+        R2 = R2 * (1.0 + 1e-9) + 1e-300;
+    };
+    set_ball(T.t1 + HV_W);
+    // packet centre: mean ball centre over the active lanes
     double pc[D];
     {
         const double wgt = act ? 1.0 : 0.0;
         double cnt = wgt;
 #pragma unroll
-        for (int k = 0; k < D; ++k) pc[k] = wgt * (r[k] + rho * u[k]);
+        for (int k = 0; k < D; ++k) pc[k] = wgt * bc[k];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
@@ -608,64 +794,169 @@ __global__ __launch_bounds__(256) void k_raycast64(DevGeo g, DevRays R, const in
         }
         const double ic = cnt > 0 ? 1.0 / cnt : 0.0;
 #pragma unroll
-        for (int k = 0; k < D; ++k) pc[k] *= ic;
+        for (int k = 0; k < D; ++k) {
+            // wave-uniform: keep it in scalar registers
+            const double v = pc[k] * ic;
+            const long long bits = __double_as_longlong(v);
+            const int lo = __builtin_amdgcn_readfirstlane((int)(bits & 0xffffffffll));
+            const int hi = __builtin_amdgcn_readfirstlane((int)(bits >> 32));
+            pc[k] = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+        }
     }
-    double bc[D], R2;
-    auto set_ball = [&](double tau) {
-        R2 = 0.0;
+    unsigned long long pairs = 0, leaves = 0;
+    // box of child c of the staged node, distance^2 to a point
+    auto child_d2 = [&](int c, const double* p) {
+        double d2 = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            bc[k] = r[k] + tau * u[k];
-            const double t = bc[k] - x0[k];
-            R2 = fma(t, t, R2);
+            const double lo = (double)cbx[wv][c * 2 * D + k], hi = (double)cbx[wv][c * 2 * D + D + k];
+            const double a = fmax(fmax(lo - p[k], p[k] - hi), 0.0);
+            d2 = fma(a, a, d2);
         }
-        R2 = R2 * (1.0 + 1e-9) + 1e-300;
+        return d2;
     };
-    set_ball(t1 + W);
-    unsigned long long pairs = 0, leaves = 0;
+    // nearest (or farthest) child still in `rem`, by distance to the packet centre
+    auto pick = [&](unsigned rem, bool nearest) {
+        int best = -1;
+        float bd = 0.f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float v = dcs[wv][c];
+            const bool in = (rem >> c) & 1u;
+            const bool better = best < 0 || (nearest ? v < bd : v > bd);
+            if (in && better) { best = c; bd = v; }
+        }
+        return best;
+    };
+    auto load_leaf = [&](int leaf, double* pf) {
+        const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
+        if (lane < npts) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) pf[k] = g.xs[(int64_t)k * N + j0 + lane];
+            pf[D] = g.xx[j0 + lane];
+        }
+    };
     int sp = 1;
-    stk[wv][0] = 1;
+    stk[wv][0] = 0;
+    stkl[wv][0] = 0;
     while (sp > 0) {
         --sp;
-        const int node = __builtin_amdgcn_readfirstlane(stk[wv][sp]);
-        const bool hit = act && (hv_box_dist2<D>(g, node, bc) <= R2);
-        const unsigned long long m = __ballot(hit);
-        if (m == 0ull) continue;
-        if (node >= g.P2) {
-            const int j0 = (node - g.P2) * g.L, j1 = min(j0 + g.L, N);
-            leaves += 1;
-            pairs += (unsigned long long)__popcll(m) * (unsigned long long)(j1 - j0);
-            // the wavefront stages the leaf in LDS (coalesced), then every hit lane scans it
-            if (lane < j1 - j0) {
+        hv_wave_sync();
+        const int gid = __builtin_amdgcn_readfirstlane(stk[wv][sp]);
+        const int l = __builtin_amdgcn_readfirstlane(stkl[wv][sp]);
+        // stage the 8 child boxes of the node (one contiguous block)
+        {
+            const float* src = g.cb + (int64_t)gid * (16 * D);
+            for (int q = lane; q < 16 * D; q += 64) cbx[wv][q] = src[q];
+        }
+        hv_wave_sync();
+        unsigned rem = 0u;
 #pragma unroll
-                for (int k = 0; k < D; ++k) leafx[wv][k][lane] = g.xs[(int64_t)k * N + j0 + lane];
-                leafx[wv][D][lane] = g.xx[j0 + lane];
+        for (int c = 0; c < 8; ++c) {
+            const unsigned long long m = __ballot(act && (child_d2(c, bc) <= R2));
+            if (lane == 0) {
+                hms[wv][c] = m;
+                dcs[wv][c] = (float)child_d2(c, pc);
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (hit) {
-                const double before = t1;
-                for (int q = 0; q < j1 - j0; ++q) scan_point(q, j0 + q);
-                if (t1 < before) set_ball(t1 + W);
+            rem |= (m != 0ull) ? (1u << c) : 0u;
+        }
+        rem = __builtin_amdgcn_readfirstlane(rem);
+        hv_wave_sync();
+        if (rem == 0u) continue;
+        int off = 0, pw = 1;
+        for (int q = 0; q < l; ++q) { off += pw; pw *= 8; }
+        const int i = gid - off;
+        if (l + 1 == g.h) {
+            // children are leaves: nearest first, the next one loaded while this one is scanned
+            double pf[D + 1];
+            int cc = __builtin_amdgcn_readfirstlane(pick(rem, true));
+            rem &= ~(1u << cc);
+            load_leaf(8 * i + cc, pf);
+            while (cc >= 0) {
+                const int cn = rem ? __builtin_amdgcn_readfirstlane(pick(rem, true)) : -1;
+                if (cn >= 0) rem &= ~(1u << cn);
+                const int leaf = 8 * i + cc;
+                const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
+                // balls only shrink: re-test with the current one
+                const bool hit = act && ((hms[wv][cc] >> lane) & 1ull) && (child_d2(cc, bc) <= R2);
+                const unsigned long long m = __ballot(hit);
+                if (m && lane < npts) {
+#pragma unroll
+                    for (int k = 0; k <= D; ++k) leafx[wv][k][lane] = pf[k];
+                }
+                if (cn >= 0) load_leaf(8 * i + cn, pf);
+                if (m) {
+                    hv_wave_sync();
+                    leaves += 1;
+                    pairs += (unsigned long long)__popcll(m) * (unsigned long long)npts;
+                    if (hit) {
+                        const double before = T.t1;
+                        // membership f = |x|^2 + fC - 2<x, bc>, four independent chains
+                        int q = 0;
+                        for (; q + 4 <= npts; q += 4) {
+                            double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0;
+#pragma unroll
+                            for (int k = 0; k < D; ++k) {
+                                f0 = fma(leafx[wv][k][q], bc[k], f0);
+                                f1 = fma(leafx[wv][k][q + 1], bc[k], f1);
+                                f2 = fma(leafx[wv][k][q + 2], bc[k], f2);
+                                f3 = fma(leafx[wv][k][q + 3], bc[k], f3);
+                            }
+                            f0 = fma(-2.0, f0, leafx[wv][D][q] + fC);
+                            f1 = fma(-2.0, f1, leafx[wv][D][q + 1] + fC);
+                            f2 = fma(-2.0, f2, leafx[wv][D][q + 2] + fC);
+                            f3 = fma(-2.0, f3, leafx[wv][D][q + 3] + fC);
+                            if (fmin(fmin(f0, f1), fmin(f2, f3)) < feps) {
+                                const double fs[4] = {f0, f1, f2, f3};
+#pragma unroll
+                                for (int z = 0; z < 4; ++z) {
+                                    if (fs[z] < feps) {
+                                        double xk[D];
+#pragma unroll
+                                        for (int k = 0; k < D; ++k) xk[k] = leafx[wv][k][q + z];
+                                        hv_consider<D>(G, T, xk, leafx[wv][D][q + z], j0 + q + z, plane_tol);
+                                    }
+                                }
+                            }
+                        }
+                        for (; q < npts; ++q) {
+                            double f = 0.0;
+#pragma unroll
+                            for (int k = 0; k < D; ++k) f = fma(leafx[wv][k][q], bc[k], f);
+                            f = fma(-2.0, f, leafx[wv][D][q] + fC);
+                            if (f < feps) {
+                                double xk[D];
+#pragma unroll
+                                for (int k = 0; k < D; ++k) xk[k] = leafx[wv][k][q];
+                                hv_consider<D>(G, T, xk, leafx[wv][D][q], j0 + q, plane_tol);
+                            }
+                        }
+                        if (T.t1 < before) set_ball(T.t1 + HV_W);
+                    }
+                    hv_wave_sync();
+                }
+                cc = cn;
             }
-            __builtin_amdgcn_wave_barrier();
         } else {
-            // children, the one nearer the packet centre popped first
-            const int a = 2 * node, b = a + 1;
-            const double da = hv_box_dist2<D>(g, a, pc), db = hv_box_dist2<D>(g, b, pc);
-            stk[wv][sp] = (da <= db) ? b : a;
-            stk[wv][sp + 1] = (da <= db) ? a : b;
-            sp += 2;
+            // internal children: farthest pushed first so the nearest is walked next
+            const int cbase = off + pw + 8 * i;
+            while (rem) {
+                const int c = __builtin_amdgcn_readfirstlane(pick(rem, false));
+                rem &= ~(1u << c);
+                stk[wv][sp] = cbase + c;
+                stkl[wv][sp] = l + 1;
+                ++sp;
+            }
         }
     }
     if (lane == 0) {
         atomicAdd(&ctr[0], pairs);
         atomicAdd(&ctr[1], leaves);
+        atomicMax(&ctr[3], leaves);
     }
     bool tie = false;
     if (act) {
-        if (i1 < 0) {
+        if (T.i1 < 0) {
             status[rid] = HV_ST_NONE;
         } else {
             // original indices and the reference's form of t (src/raycast.jl:424-431)
@@ -676,8 +967,8 @@ __global__ __launch_bounds__(256) void k_raycast64(DevGeo g, DevRays R, const in
                     double nu = 0.0, dist = g.ph[p];
 #pragma unroll
                     for (int k = 0; k < D; ++k) {
-                        nu = fma(g.pn[p * D + k], u[k], nu);
-                        dist = fma(-g.pn[p * D + k], x0[k], dist);
+                        nu = fma(g.pn[p * D + k], G.u[k], nu);
+                        dist = fma(-g.pn[p * D + k], G.x0[k], dist);
                     }
                     score = 2.0 * dist * nu;
                     cosang = nu;
@@ -687,27 +978,27 @@ __global__ __launch_bounds__(256) void k_raycast64(DevGeo g, DevRays R, const in
 #pragma unroll
                 for (int k = 0; k < D; ++k) {
                     const double xk = g.xo[(int64_t)k * N + w];
-                    const double dx = xk - x0[k];
-                    num = fma(dx, x0[k] + xk - 2.0 * r[k], num);
-                    den = fma(u[k], dx, den);
+                    const double dx = xk - G.x0[k];
+                    num = fma(dx, G.x0[k] + xk - 2.0 * G.r[k], num);
+                    den = fma(G.u[k], dx, den);
                     dd = fma(dx, dx, dd);
                 }
                 score = den;
                 cosang = den / sqrt(dd);
                 return num / (2.0 * den);
             };
-            int w1 = orig(i1);
+            int w1 = orig(T.i1);
             double s1, c1;
-            double tw = t_ref(w1, t1, s1, c1);
+            double tw = t_ref(w1, T.t1, s1, c1);
             int st = HV_ST_FOUND;
-            if (i2 >= 0) {
-                const int w2 = orig(i2);
+            if (T.i2 >= 0) {
+                const int w2 = orig(T.i2);
                 double s2, c2;
-                const double tw2 = t_ref(w2, t2, s2, c2);
+                const double tw2 = t_ref(w2, T.t2, s2, c2);
                 double rn = 0.0;
 #pragma unroll
-                for (int k = 0; k < D; ++k) rn = fma(r[k], r[k], rn);
-                const double full_error = (tw * du + sqrt(rn) * 1e-15) / c1;
+                for (int k = 0; k < D; ++k) rn = fma(G.r[k], G.r[k], rn);
+                const double full_error = (tw * G.du + sqrt(rn) * 1e-15) / c1;
                 const double window = fmin(1e-7, (10.0 + D) * full_error);
                 if (tw2 < tw || tw2 <= tw + window) {
                     // near tie: the larger <u, x - x0> wins (src/raycast.jl:903-916)
@@ -794,6 +1085,14 @@ int build_tree(hv_ctx* c, int L, Tree& T) {
             if (nk > 0.5) g.dhi[k] = c->pbase[p * D + k];
             if (nk < -0.5) g.dlo[k] = c->pbase[p * D + k];
         }
+    {
+        double s = 0.0;
+        for (int k = 0; k < D; ++k) {
+            const double a = std::max(std::fabs(g.dlo[k]), std::fabs(g.dhi[k]));
+            s += a * a;
+        }
+        g.xmax = std::sqrt(s);
+    }
     double* pn = pget<double>(c, "pn", std::max(1, c->P * D));
     double* ph = pget<double>(c, "ph", std::max(1, c->P));
     if (!pn || !ph) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (planes)");
@@ -831,12 +1130,17 @@ int build_tree(hv_ctx* c, int L, Tree& T) {
     void* tmp = c->pool->get("cubtmp", tmpb, false, c->stream);
     if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
     HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tmpb, code, code2, idx, idx2, N, 0, 64, c->stream));
-    k_points_sorted<D><<<nblk(N, 256), 256, 0, c->stream>>>(xo, idx2, N, xs, xx, sidx);
+    int* rank = pget<int>(c, "rank", N);
+    int* knn = pget<int>(c, "knn", (size_t)N * HV_KNN);
+    if (!rank || !knn) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (knn)");
+    k_points_sorted<D><<<nblk(N, 256), 256, 0, c->stream>>>(xo, idx2, N, xs, xx, sidx, rank);
     HVCK(hipGetLastError());
     g.xo = xo;
     g.xs = xs;
     g.xx = xx;
     g.sidx = sidx;
+    g.rank = rank;
+    g.knn = knn;
     g.blo = blo;
     g.bhi = bhi;
     k_leaf_boxes<D><<<nblk(g.P2, 256), 256, 0, c->stream>>>(g, blo, bhi);
@@ -845,6 +1149,25 @@ int build_tree(hv_ctx* c, int L, Tree& T) {
         k_node_boxes<D><<<nblk(first, 256), 256, 0, c->stream>>>(first, first, g.NN, blo, bhi);
         HVCK(hipGetLastError());
     }
+    // 8-ary tree of child boxes (levels bottom-up)
+    g.h = 1;
+    {
+        int64_t cap = 8;
+        while (cap < g.nleaf) { cap *= 8; ++g.h; }
+        int64_t ninternal = 0, pw = 1;
+        for (int l = 0; l < g.h; ++l) { ninternal += pw; pw *= 8; }
+        float* cb = pget<float>(c, "cb8", (size_t)ninternal * 8 * 2 * D);
+        if (!cb) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tree)");
+        g.cb = cb;
+        for (int l = g.h - 1; l >= 0; --l) {
+            int nodes = 1;
+            for (int q = 0; q < l; ++q) nodes *= 8;
+            k_tree8_level<D><<<nblk((int64_t)nodes * 8, 256), 256, 0, c->stream>>>(g, l, nodes, cb);
+            HVCK(hipGetLastError());
+        }
+    }
+    k_knn<D><<<nblk(N, 256), 256, 0, c->stream>>>(g, knn);
+    HVCK(hipGetLastError());
     int hbad = 0;
     HVCK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HVCK(hipStreamSynchronize(c->stream));
@@ -1050,6 +1373,10 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         R.du = pget<double>(c, "ray_du", nslots);
         R.tinit = nullptr;
         R.code = pget<uint64_t>(c, "ray_code", nslots);
+        R.t1 = pget<double>(c, "ray_t1", nslots);
+        R.t2 = pget<double>(c, "ray_t2", nslots);
+        R.i1 = pget<int>(c, "ray_i1", nslots);
+        R.i2 = pget<int>(c, "ray_i2", nslots);
         uint64_t* code2 = pget<uint64_t>(c, "ray_code2", nslots);
         int* ord0 = pget<int>(c, "ray_ord0", nslots);
         int* ord = pget<int>(c, "ray_ord", nslots);
@@ -1061,6 +1388,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         double* rt = pget<double>(c, "ray_t", nslots);
         int64_t* reg_next = pget<int64_t>(c, reg_is_a ? "regB" : "regA", nslots * (D + 1));
         if (!R.slot || !R.v || !R.drop || !R.u || !R.c || !R.du || !R.code || !code2 || !ord0 || !ord || !cand ||
+            !R.t1 || !R.t2 || !R.i1 || !R.i2 ||
             !ordc || !keep || !status || !rwin || !rt || !reg_next)
             return hv_fail(c, HV_ERR_DEVICE, "out of device memory (rays)");
         HVCK(hipMemsetAsync(dcount, 0, sizeof(int), c->stream));
@@ -1073,7 +1401,9 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         if (rc) return rc;
         if (hctr[5]) return hv_fail(c, HV_ERR_CAPACITY, "edge hash overflow");
         if (nr == 0) break;
-        // sort rays by the Morton code of their expected end point
+        // start phase, then sort rays by the Morton code of their certificate-ball centre
+        k_ray_init<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, nr, vkey, vx, plane_tol);
+        HVCK(hipGetLastError());
         {
             size_t tb = 0;
             HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, R.code, code2, ord0, ord, nr, 0, 64, c->stream));
@@ -1110,7 +1440,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             rc = ensure_vertices(n);
             if (rc) return rc;
             HVCK(hipEventRecord(r0, c->stream));
-            k_raycast64<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, rays_c, n, vkey, vx, status, rwin, rt, plane_tol, ctr + 0);
+            k_raycast64<D><<<nblk(n, 64), 64, 0, c->stream>>>(g, R, rays_c, n, vkey, vx, status, rwin, rt, plane_tol, ctr + 0);
             HVCK(hipGetLastError());
             HVCK(hipEventRecord(r1, c->stream));
             k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, rays_c, n, vkey, vx, vnext, status, rwin);
@@ -1159,6 +1489,10 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     st.near_ties = (int64_t)hctr[2];
     st.duplicates = (int64_t)hctr[8];
     st.rays_skipped = skipped;
+    st.fallbacks = (int64_t)hctr[7];      // FP64 path: rays whose kNN start found no generator ahead
+    st.retries = (int64_t)hctr[6];        // packets that visited more than 512 leaves
+    st.unbounded = 0;
+    st.max_t_relerr = (double)hctr[3];    // most leaves visited by one packet
     st.ms_total = ms_all;
     st.ms_raycast = ms_ray;
     hv_mesh* m = new hv_mesh();
@@ -1358,6 +1692,10 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     R.du = pget<double>(c, "b_rdu", nrays);
     R.tinit = nullptr;
     R.code = pget<uint64_t>(c, "b_code", nrays);
+    R.t1 = pget<double>(c, "b_t1", nrays);
+    R.t2 = pget<double>(c, "b_t2", nrays);
+    R.i1 = pget<int>(c, "b_i1", nrays);
+    R.i2 = pget<int>(c, "b_i2", nrays);
     uint64_t* code2 = pget<uint64_t>(c, "b_code2", nrays);
     int* ord0 = pget<int>(c, "b_ord0", nrays);
     int* ord = pget<int>(c, "b_ord", nrays);
@@ -1366,7 +1704,7 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     int* rwin = pget<int>(c, "b_win", nrays);
     unsigned long long* ctr = pget<unsigned long long>(c, "b_ctr", 16);
     if (!vkey || !vx || !dd || !R.v || !R.drop || !R.u || !R.c || !R.du || !R.code || !code2 || !ord0 || !ord ||
-        !status || !tout || !ctr || !rwin)
+        !status || !tout || !ctr || !rwin || !R.t1 || !R.t2 || !R.i1 || !R.i2)
         return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
     HVCK(hipMemcpyAsync(vkey, hkey.data(), sizeof(int) * hkey.size(), hipMemcpyHostToDevice, c->stream));
     HVCK(hipMemcpyAsync(vx, r, sizeof(double) * nrays * D, hipMemcpyHostToDevice, c->stream));
@@ -1376,6 +1714,8 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     const double plane_tol = 1e-12;
     k_setup_rays<D><<<nblk(nrays, 256), 256, 0, c->stream>>>((int)nrays, vkey, vx, dd, g, R, plane_tol);
     HVCK(hipGetLastError());
+    k_ray_init<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, (int)nrays, vkey, vx, plane_tol);
+    HVCK(hipGetLastError());
     size_t tb = 0;
     HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, R.code, code2, ord0, ord, (int)nrays, 0, 64, c->stream));
     void* tmp = c->pool->get("cubtmp", tb, false, c->stream);
@@ -1383,7 +1723,7 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     k_iota<<<nblk(nrays, 256), 256, 0, c->stream>>>(ord0, (int)nrays);
     HVCK(hipGetLastError());
     HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, (int)nrays, 0, 64, c->stream));
-    k_raycast64<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, status, rwin, tout,
+    k_raycast64<D><<<nblk(nrays, 64), 64, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, status, rwin, tout,
                                                             plane_tol, ctr);
     HVCK(hipGetLastError());
     k_finalize<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, nrays, status, rwin);
