@@ -1,9 +1,10 @@
 // hv_device.h -- device-side data layout and helpers of the gfx950 raycast walk.
 //
 // HBM layout (all tables live for one hv_build_mesh call):
-//   generators  xo[k*N + g]   SoA FP64, original index order (gathers by key element)
-//               xs[k*N + j]   SoA FP64, Morton-sorted order (leaf scans), xx[j] = |x_j|^2
-//               sidx[j]       sorted position -> original index
+//   generators  xo[g*D + k]       FP64 records in original index order (gathers by key)
+//               xs[j*(D+1) + k]   FP64 records in Morton order, slot D = |x_j|^2 (leaf
+//                                 scans and neighbour gathers: one record per point)
+//               sidx[j]           sorted position -> original index
 //   tree        complete binary tree in heap order over leaves of L sorted points:
 //               node 1 is the root, node P2+l is leaf l; boxes blo/bhi[k*NN + node]
 //   vertices    key[v*(D+1) + j] sorted int32 (generator g or N+p for plane p),
@@ -20,9 +21,8 @@
 
 struct DevGeo {
     int N, P, L, nleaf, P2, NN;
-    const double* __restrict__ xo;
-    const double* __restrict__ xs;
-    const double* __restrict__ xx;
+    const double* __restrict__ xo;   // N x D
+    const double* __restrict__ xs;   // N x (D+1), slot D = |x|^2
     const int* __restrict__ sidx;
     const double* __restrict__ pn;   // P x D normals (row per plane)
     const double* __restrict__ ph;   // P offsets <n_p, b_p>
@@ -115,7 +115,7 @@ __device__ __forceinline__ void hv_vertex_row(const DevGeo& g, int e, const doub
         double s = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const double dk = g.xo[(int64_t)k * N + e] - x0[k];
+            const double dk = g.xo[(int64_t)(e) * D + k] - x0[k];
             row[k] = 2.0 * dk;
             s = fma(dk, dk, s);
         }
@@ -144,7 +144,7 @@ __device__ __forceinline__ bool hv_vertex_coords(const DevGeo& g, const int* key
     const int N = g.N;
     double x0[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) x0[k] = g.xo[(int64_t)k * N + key[0]];
+    for (int k = 0; k < D; ++k) x0[k] = g.xo[(int64_t)(key[0]) * D + k];
     double Q[D][D], Rl[D][D], inv[D], beta[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {

@@ -43,7 +43,7 @@ __global__ void k_points_in(const double* __restrict__ xin, int N, DevGeo g, dou
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         p[k] = xin[(int64_t)i * D + k];
-        xo[(int64_t)k * N + i] = p[k];
+        xo[(int64_t)i * D + k] = p[k];
     }
     // check_boundary (src/boundary.jl:427-441): <b_p - x, n_p> >= 0 for every plane
     bool ok = true;
@@ -62,8 +62,7 @@ __global__ void k_points_in(const double* __restrict__ xin, int N, DevGeo g, dou
 
 template <int D>
 __global__ void k_points_sorted(const double* __restrict__ xo, const int* __restrict__ idx, int N,
-                                double* __restrict__ xs, double* __restrict__ xx, int* __restrict__ sidx,
-                                int* __restrict__ rank) {
+                                double* __restrict__ xs, int* __restrict__ sidx, int* __restrict__ rank) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
     int gi = idx[j];
@@ -72,11 +71,11 @@ __global__ void k_points_sorted(const double* __restrict__ xo, const int* __rest
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        double v = xo[(int64_t)k * N + gi];
-        xs[(int64_t)k * N + j] = v;
+        double v = xo[(int64_t)gi * D + k];
+        xs[(int64_t)j * (D + 1) + k] = v;
         s = fma(v, v, s);
     }
-    xx[j] = s;
+    xs[(int64_t)j * (D + 1) + D] = s;
 }
 
 template <int D>
@@ -89,7 +88,7 @@ __global__ void k_leaf_boxes(DevGeo g, double* __restrict__ blo, double* __restr
     for (int k = 0; k < D; ++k) {
         double lo = 1e300, hi = -1e300;
         for (int j = j0; j < j1; ++j) {
-            double v = g.xs[(int64_t)k * g.N + j];
+            double v = g.xs[(int64_t)(j) * (D + 1) + k];
             lo = fmin(lo, v);
             hi = fmax(hi, v);
         }
@@ -119,7 +118,7 @@ __global__ __launch_bounds__(256) void k_knn(DevGeo g, int* __restrict__ knn) {
     const int N = g.N;
     double p[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) p[k] = g.xs[(int64_t)k * N + j];
+    for (int k = 0; k < D; ++k) p[k] = g.xs[(int64_t)(j) * (D + 1) + k];
     double bd[HV_KNN];
     int bi[HV_KNN];
 #pragma unroll
@@ -129,7 +128,7 @@ __global__ __launch_bounds__(256) void k_knn(DevGeo g, int* __restrict__ knn) {
         double d2 = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const double a = g.xs[(int64_t)k * N + q] - p[k];
+            const double a = g.xs[(int64_t)(q) * (D + 1) + k] - p[k];
             d2 = fma(a, a, d2);
         }
         if (!(d2 < bd[HV_KNN - 1])) return;
@@ -195,7 +194,7 @@ __global__ void k_tree8_level(DevGeo g, int l, int nodes_at_l, float* __restrict
         for (int k = 0; k < D; ++k) {
             double mn = 1e300, mx = -1e300;
             for (int j = j0; j < j1; ++j) {
-                const double v = g.xs[(int64_t)k * g.N + j];
+                const double v = g.xs[(int64_t)(j) * (D + 1) + k];
                 mn = fmin(mn, v);
                 mx = fmax(mx, v);
             }
@@ -238,7 +237,7 @@ __global__ __launch_bounds__(256) void k_seed_corners(DevGeo g, int* __restrict_
         double s = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            double t = g.xo[(int64_t)k * g.N + j] - cp[k];
+            double t = g.xo[(int64_t)(j) * D + k] - cp[k];
             s = fma(t, t, s);
         }
         if (s < best || (s == best && j < bi)) { best = s; bi = j; }
@@ -439,7 +438,7 @@ template <int D>
 __device__ __forceinline__ void hv_span_vec(const DevGeo& g, int el, const double* x0, double* out) {
     if (el < g.N) {
 #pragma unroll
-        for (int k = 0; k < D; ++k) out[k] = g.xo[(int64_t)k * g.N + el] - x0[k];
+        for (int k = 0; k < D; ++k) out[k] = g.xo[(int64_t)(el) * D + k] - x0[k];
     } else {
 #pragma unroll
         for (int k = 0; k < D; ++k) out[k] = g.pn[(el - g.N) * D + k];
@@ -455,7 +454,7 @@ __device__ __forceinline__ void hv_ray_setup(const DevGeo& g, const int* key, in
     const int s = hv_key_at<D>(key, pos);
     double x0[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) x0[k] = g.xo[(int64_t)k * N + e[0]];
+    for (int k = 0; k < D; ++k) x0[k] = g.xo[(int64_t)(e[0]) * D + k];
     double q[D - 1][D];
     // slots from the top so that plane normals (largest indices) are orthogonalised first
 #pragma unroll
@@ -519,7 +518,7 @@ __device__ __forceinline__ void hv_ray_setup(const DevGeo& g, const int* key, in
         if (el < N) {
             v = 0.0;
 #pragma unroll
-            for (int k = 0; k < D; ++k) v = fma(u[k], g.xo[(int64_t)k * N + el], v);
+            for (int k = 0; k < D; ++k) v = fma(u[k], g.xo[(int64_t)(el) * D + k], v);
         } else {
             const int p = el - N;
             double dist = g.ph[p], nu = 0.0;
@@ -604,7 +603,19 @@ __device__ __forceinline__ void hv_wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr double HV_W = 1e-7;   // widest near-tie window of the reference (src/raycast.jl:902)
+constexpr double HV_W = 1e-7;
+
+// read-only tables seen through the constant address space: uniform loads become SMEM
+typedef const __attribute__((address_space(4))) double hv_cdouble;
+typedef const __attribute__((address_space(4))) float hv_cfloat;
+
+// a value known to be equal on every lane, moved to scalar registers
+__device__ __forceinline__ double hv_uniform(double v) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(bits & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(bits >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}   // widest near-tie window of the reference (src/raycast.jl:902)
 
 // Per-ray geometry shared by the start phase and the packet scan.
 template <int D>
@@ -629,7 +640,7 @@ __device__ __forceinline__ void hv_load_ray(const DevGeo& g, const DevRays& R, i
     for (int k = 0; k < D; ++k) {
         G.u[k] = R.u[(int64_t)ray * D + k];
         G.r[k] = vx[(int64_t)v * D + k];
-        G.x0[k] = g.xo[(int64_t)k * g.N + G.e[0]];
+        G.x0[k] = g.xo[(int64_t)(G.e[0]) * D + k];
         G.ux0 = fma(G.u[k], G.x0[k], G.ux0);
         rx0 = fma(G.r[k], G.x0[k], rx0);
         x0x0 = fma(G.x0[k], G.x0[k], x0x0);
@@ -705,14 +716,20 @@ __global__ __launch_bounds__(256) void k_ray_init(DevGeo g, DevRays R, int nrays
         const double tp = (g.ph[p] - nr) / nu;
         if (tp >= plane_tol) T.offer(tp, id);
     }
-    const int* nb = g.knn + (int64_t)g.rank[G.e[0]] * HV_KNN;
+    // the winner is a Delaunay neighbour of every generator of the edge: scan their lists
+#pragma unroll 1
+    for (int a = 0; a < D; ++a) {
+        const int ea = hv_key_at<D - 1>(G.e, a);
+        if (ea >= N) break;   // planes sort last
+        const int* nb = g.knn + (int64_t)g.rank[ea] * HV_KNN;
 #pragma unroll 4
-    for (int m = 0; m < HV_KNN; ++m) {
-        const int j = nb[m];
-        double xk[D];
+        for (int m = 0; m < HV_KNN; ++m) {
+            const int j = nb[m];
+            double xk[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) xk[k] = g.xs[(int64_t)k * N + j];
-        hv_consider<D>(G, T, xk, g.xx[j], j, plane_tol);
+            for (int k = 0; k < D; ++k) xk[k] = g.xs[(int64_t)(j) * (D + 1) + k];
+            hv_consider<D>(G, T, xk, g.xs[(int64_t)(j) * (D + 1) + D], j, plane_tol);
+        }
     }
     R.t1[ray] = T.t1;
     R.t2[ray] = T.t2;
@@ -832,8 +849,8 @@ __global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int
         const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
         if (lane < npts) {
 #pragma unroll
-            for (int k = 0; k < D; ++k) pf[k] = g.xs[(int64_t)k * N + j0 + lane];
-            pf[D] = g.xx[j0 + lane];
+            for (int k = 0; k < D; ++k) pf[k] = g.xs[(int64_t)(j0 + lane) * (D + 1) + k];
+            pf[D] = g.xs[(int64_t)(j0 + lane) * (D + 1) + D];
         }
     };
     int sp = 1;
@@ -977,7 +994,7 @@ __global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int
                 double num = 0.0, den = 0.0, dd = 0.0;
 #pragma unroll
                 for (int k = 0; k < D; ++k) {
-                    const double xk = g.xo[(int64_t)k * N + w];
+                    const double xk = g.xo[(int64_t)(w) * D + k];
                     const double dx = xk - G.x0[k];
                     num = fma(dx, G.x0[k] + xk - 2.0 * G.r[k], num);
                     den = fma(G.u[k], dx, den);
@@ -1110,8 +1127,7 @@ int build_tree(hv_ctx* c, int L, Tree& T) {
     g.ph = ph;
     const double* xin = (const double*)c->pool->get("xin", 0);
     double* xo = pget<double>(c, "xo", (size_t)N * D);
-    double* xs = pget<double>(c, "xs", (size_t)N * D);
-    double* xx = pget<double>(c, "xx", N);
+    double* xs = pget<double>(c, "xs", (size_t)N * (D + 1));
     int* sidx = pget<int>(c, "sidx", N);
     uint64_t* code = pget<uint64_t>(c, "code", N);
     uint64_t* code2 = pget<uint64_t>(c, "code2", N);
@@ -1120,7 +1136,7 @@ int build_tree(hv_ctx* c, int L, Tree& T) {
     int* bad = pget<int>(c, "bad", 1);
     double* blo = pget<double>(c, "blo", (size_t)g.NN * D);
     double* bhi = pget<double>(c, "bhi", (size_t)g.NN * D);
-    if (!xin || !xo || !xs || !xx || !sidx || !code || !code2 || !idx || !idx2 || !bad || !blo || !bhi)
+    if (!xin || !xo || !xs || !sidx || !code || !code2 || !idx || !idx2 || !bad || !blo || !bhi)
         return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tree)");
     HVCK(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
     k_points_in<D><<<nblk(N, 256), 256, 0, c->stream>>>(xin, N, g, xo, code, idx, bad);
@@ -1133,11 +1149,10 @@ int build_tree(hv_ctx* c, int L, Tree& T) {
     int* rank = pget<int>(c, "rank", N);
     int* knn = pget<int>(c, "knn", (size_t)N * HV_KNN);
     if (!rank || !knn) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (knn)");
-    k_points_sorted<D><<<nblk(N, 256), 256, 0, c->stream>>>(xo, idx2, N, xs, xx, sidx, rank);
+    k_points_sorted<D><<<nblk(N, 256), 256, 0, c->stream>>>(xo, idx2, N, xs, sidx, rank);
     HVCK(hipGetLastError());
     g.xo = xo;
     g.xs = xs;
-    g.xx = xx;
     g.sidx = sidx;
     g.rank = rank;
     g.knn = knn;
@@ -1204,6 +1219,83 @@ double vertices_per_point(int D) {
 
 }  // namespace
 
+// Tentative keys: after the start phase each ray knows a candidate vertex
+// key = edge + nearest-neighbour winner.  Rays of one level with the same tentative key
+// are grouped; only the first (the primary) is scanned.  When the primary's scan confirms
+// the key, it is a Voronoi vertex containing every grouped ray's edge, so (general
+// position: an edge has two end points) it is the far end point of all of them and the
+// other rays need no scan.  Unconfirmed groups are scanned in a second pass.
+template <int D>
+__device__ __forceinline__ bool hv_tent_key(const DevGeo& g, const DevRays& R, const int* __restrict__ vkey, int ray,
+                                            int* tk) {
+    const int i1 = R.i1[ray];
+    if (i1 < 0) return false;
+    const int w = i1 < g.N ? g.sidx[i1] : i1;
+    int e[D];
+    hv_edge_of<D>(vkey + (int64_t)R.v[ray] * (D + 1), R.drop[ray], e);
+    int ins = 0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) ins += (e[j] < w) ? 1 : 0;
+#pragma unroll
+    for (int j = 0; j <= D; ++j) tk[j] = (j < ins) ? e[j] : (j == ins ? w : e[j > 0 ? j - 1 : 0]);
+    return true;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_tent_group(DevGeo g, DevRays R, int nr, const int* __restrict__ vkey,
+                                                    int* __restrict__ ttab, uint64_t tmask, int* __restrict__ prim_of,
+                                                    uint8_t* __restrict__ primary) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nr) return;
+    int tk[D + 1];
+    if (!hv_tent_key<D>(g, R, vkey, t, tk)) { prim_of[t] = t; primary[t] = 1; return; }
+    uint64_t h = hv_hash_ints<D + 1>(tk) & tmask;
+    for (uint64_t probe = 0; probe <= tmask; ++probe) {
+        int cur = ttab[h];
+        if (cur < 0) {
+            const int old = atomicCAS(&ttab[h], -1, t);
+            if (old == -1) { prim_of[t] = t; primary[t] = 1; return; }
+            cur = old;
+        }
+        int ok[D + 1];
+        hv_tent_key<D>(g, R, vkey, cur, ok);
+        if (keys_equal<D>(tk, ok)) { prim_of[t] = cur; primary[t] = 0; return; }
+        h = (h + 1) & tmask;
+    }
+    prim_of[t] = t;
+    primary[t] = 1;
+}
+
+__global__ void k_gather_flags(const int* __restrict__ ord, int nr, const uint8_t* __restrict__ by_ray,
+                               uint8_t* __restrict__ keep) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nr) keep[i] = by_ray[ord[i]];
+}
+
+__global__ void k_scatter_results(const int* __restrict__ list, int n, const int* __restrict__ status,
+                                  const int* __restrict__ win, int* __restrict__ st_by_ray, int* __restrict__ win_by_ray) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ray = list[i];
+    st_by_ray[ray] = status[i];
+    win_by_ray[ray] = win[i];
+}
+
+// secondary rays: resolved when the primary found exactly its tentative key
+template <int D>
+__global__ void k_secondary(DevGeo g, DevRays R, int nr, const int* __restrict__ prim_of,
+                            const uint8_t* __restrict__ primary, const int* __restrict__ st_by_ray,
+                            const int* __restrict__ win_by_ray, uint8_t* __restrict__ rescan) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nr) return;
+    if (primary[t]) { rescan[t] = 0; return; }
+    const int p = prim_of[t];
+    const int i1 = R.i1[p];
+    const int tent = i1 < g.N ? g.sidx[i1] : i1;
+    const bool confirmed = st_by_ray[p] == HV_ST_FOUND && win_by_ray[p] == tent;
+    rescan[t] = confirmed ? 0 : 1;
+}
+
 // Chunk selection (64-ray blocks interleaved over K chunks); a ray of a later chunk is
 // dropped when its edge got its second end point from a vertex found by an earlier
 // chunk of the same level (the sequential walk's edge hash would not walk it either).
@@ -1227,7 +1319,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     if (o) opt = *o;
     const int L = opt.leaf_size > 0 ? opt.leaf_size : 32;
     const double plane_tol = opt.plane_tolerance > 0 ? opt.plane_tolerance : 1e-12;
-    const int KCH = opt.reserved0 > 0 ? opt.reserved0 : 4;   // chunks per level
+    const int KCH = opt.reserved0 > 1 ? opt.reserved0 : 1;   // chunks per level (<0: tentative-key grouping)
     if (opt.precision != HV_PREC_FP64) return hv_fail(c, HV_ERR_UNSUPPORTED, "precision mode not built yet");
     int rc = check_box(c, D);
     if (rc) return rc;
@@ -1414,47 +1506,26 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, nr, 0, 64, c->stream));
         }
         const int64_t level_begin = nlive;
-        // small levels are not split unless the caller asked for a chunk count explicitly
-        const int K = (opt.reserved0 > 0) ? std::min(KCH, nr) : (nr >= 65536 ? KCH : 1);
-        const int nfull = nr >> 6, rem = nr & 63;
-        for (int ch = 0; ch < K; ++ch) {
-            const int nb = nfull > ch ? (nfull - ch + K - 1) / K : 0;
-            const int nc = nb * 64 + ((nfull % K == ch) ? rem : 0);
-            if (nc <= 0) continue;
-            k_chunk<<<nblk(nc, 256), 256, 0, c->stream>>>(ord, nr, ch, K, R.slot, etab, cand, keep);
-            HVCK(hipGetLastError());
-            int n = nc;
-            const int* rays_c = cand;
-            if (ch > 0) {
-                size_t tb = 0;
-                HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb, cand, keep, ordc, dcount, nc, c->stream));
-                void* tmp = c->pool->get("cubtmp2", tb, false, c->stream);
-                if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
-                HVCK(hipcub::DeviceSelect::Flagged(tmp, tb, cand, keep, ordc, dcount, nc, c->stream));
-                HVCK(hipMemcpyAsync(&n, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-                HVCK(hipStreamSynchronize(c->stream));
-                rays_c = ordc;
-                skipped += nc - n;
-            }
-            if (n == 0) continue;
-            rc = ensure_vertices(n);
-            if (rc) return rc;
+        // scan a list of rays, then new vertices -> hash -> registered edges
+        auto process = [&](const int* list, int n) -> int {
+            int r_ = ensure_vertices(n);
+            if (r_) return r_;
             HVCK(hipEventRecord(r0, c->stream));
-            k_raycast64<D><<<nblk(n, 64), 64, 0, c->stream>>>(g, R, rays_c, n, vkey, vx, status, rwin, rt, plane_tol, ctr + 0);
+            k_raycast64<D><<<nblk(n, 64), 64, 0, c->stream>>>(g, R, list, n, vkey, vx, status, rwin, rt, plane_tol, ctr + 0);
             HVCK(hipGetLastError());
             HVCK(hipEventRecord(r1, c->stream));
-            k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, rays_c, n, vkey, vx, vnext, status, rwin);
+            k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, list, n, vkey, vx, vnext, status, rwin);
             HVCK(hipGetLastError());
             int nsel = 0;
-            rc = insert_and_select(status, n, vnext, nsel);
-            if (rc) return rc;
+            r_ = insert_and_select(status, n, vnext, nsel);
+            if (r_) return r_;
             float ms = 0.f;
             HVCK(hipEventElapsedTime(&ms, r0, r1));
             ms_ray += ms;
             if (nsel) {
                 grown = false;
-                rc = register_new(nlive, nsel, reg_next + (nlive - level_begin) * (D + 1));
-                if (rc) return rc;
+                r_ = register_new(nlive, nsel, reg_next + (nlive - level_begin) * (D + 1));
+                if (r_) return r_;
                 if (grown) {
                     // slots stored before the rehash: this level's rays and its earlier new vertices
                     k_relookup_rays<D><<<nblk(nr, 256), 256, 0, c->stream>>>(R, nr, vkey, etab, esize - 1);
@@ -1469,6 +1540,78 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             st.rays += n;
             nlive += nsel;
             vnext += n;
+            return HV_OK;
+        };
+        auto select = [&](const int* in, const uint8_t* flags, int n, int* outl, int& nout) -> int {
+            size_t tb = 0;
+            HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb, in, flags, outl, dcount, n, c->stream));
+            void* tmp = c->pool->get("cubtmp2", tb, false, c->stream);
+            if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
+            HVCK(hipcub::DeviceSelect::Flagged(tmp, tb, in, flags, outl, dcount, n, c->stream));
+            HVCK(hipMemcpyAsync(&nout, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HVCK(hipStreamSynchronize(c->stream));
+            return HV_OK;
+        };
+        if (opt.reserved0 < 0) {
+            // tentative-key grouping: primaries first, unconfirmed secondaries after
+            uint64_t tsz = 1024;
+            while ((int64_t)tsz < 2 * (int64_t)nr) tsz <<= 1;
+            int* ttab = pget<int>(c, "ttab", tsz);
+            int* prim_of = pget<int>(c, "prim_of", nr);
+            uint8_t* pflag = pget<uint8_t>(c, "pflag", nr);
+            uint8_t* rescan = pget<uint8_t>(c, "rescan", nr);
+            int* st_by_ray = pget<int>(c, "st_by_ray", nr);
+            int* win_by_ray = pget<int>(c, "win_by_ray", nr);
+            if (!ttab || !prim_of || !pflag || !rescan || !st_by_ray || !win_by_ray)
+                return hv_fail(c, HV_ERR_DEVICE, "out of device memory (groups)");
+            HVCK(hipMemsetAsync(ttab, 0xff, tsz * sizeof(int), c->stream));
+            k_tent_group<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, nr, vkey, ttab, tsz - 1, prim_of, pflag);
+            HVCK(hipGetLastError());
+            k_gather_flags<<<nblk(nr, 256), 256, 0, c->stream>>>(ord, nr, pflag, keep);
+            HVCK(hipGetLastError());
+            int np = 0;
+            rc = select(ord, keep, nr, ordc, np);
+            if (rc) return rc;
+            if (np) {
+                rc = process(ordc, np);
+                if (rc) return rc;
+                k_scatter_results<<<nblk(np, 256), 256, 0, c->stream>>>(ordc, np, status, rwin, st_by_ray, win_by_ray);
+                HVCK(hipGetLastError());
+            }
+            k_secondary<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, nr, prim_of, pflag, st_by_ray, win_by_ray, rescan);
+            HVCK(hipGetLastError());
+            k_gather_flags<<<nblk(nr, 256), 256, 0, c->stream>>>(ord, nr, rescan, keep);
+            HVCK(hipGetLastError());
+            int ns = 0;
+            rc = select(ord, keep, nr, cand, ns);
+            if (rc) return rc;
+            if (ns) {
+                rc = process(cand, ns);
+                if (rc) return rc;
+            }
+            skipped += nr - np - ns;
+        } else {
+            // interleaved chunks: 64-ray blocks over K chunks, edge registration after each
+            const int K = std::min(KCH, nr);
+            const int nfull = nr >> 6, rem = nr & 63;
+            for (int ch = 0; ch < K; ++ch) {
+                const int nb = nfull > ch ? (nfull - ch + K - 1) / K : 0;
+                const int nc = nb * 64 + ((nfull % K == ch) ? rem : 0);
+                if (nc <= 0) continue;
+                k_chunk<<<nblk(nc, 256), 256, 0, c->stream>>>(ord, nr, ch, K, R.slot, etab, cand, keep);
+                HVCK(hipGetLastError());
+                int n = nc;
+                const int* rays_c = cand;
+                if (ch > 0) {
+                    rc = select(cand, keep, nc, ordc, n);
+                    if (rc) return rc;
+                    rays_c = ordc;
+                    skipped += nc - n;
+                }
+                if (n == 0) continue;
+                rc = process(rays_c, n);
+                if (rc) return rc;
+            }
         }
         reg_cur = reg_next;
         reg_is_a = !reg_is_a;
@@ -1601,7 +1744,7 @@ __global__ void k_nn_halfspace(DevGeo g, int nq, const double* __restrict__ q, c
                 double d2 = 0.0, ux = 0.0;
 #pragma unroll
                 for (int k = 0; k < D; ++k) {
-                    const double xk = g.xs[(int64_t)k * g.N + j];
+                    const double xk = g.xs[(int64_t)(j) * (D + 1) + k];
                     const double a = xk - p[k];
                     d2 = fma(a, a, d2);
                     ux = fma(uu[k], xk, ux);
@@ -1638,7 +1781,7 @@ __global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const 
                 double d2 = 0.0;
 #pragma unroll
                 for (int k = 0; k < D; ++k) {
-                    const double a = g.xs[(int64_t)k * g.N + j] - p[k];
+                    const double a = g.xs[(int64_t)(j) * (D + 1) + k] - p[k];
                     d2 = fma(a, a, d2);
                 }
                 if (d2 <= r2) {

@@ -47,7 +47,9 @@ typedef struct hv_opts {
     int32_t precision;        /* HV_PREC_* */
     int32_t leaf_size;        /* generators per tree leaf (0 = default 32) */
     int32_t rays_per_block;   /* threads per raycast block (0 = default 256) */
-    int32_t reserved0;        /* chunks per walk level (0 = default 4) */
+    int32_t reserved0;        /* walk-level schedule: 0 or 1 = every ray of a level in one scan
+                                 (default); >1 = that many chunks, edges registered after each;
+                                 <0 = rays grouped by tentative key, one scan per group first */
     double plane_tolerance;   /* src/raycast-types.jl:229 (0 = 1e-12) */
     double ray_tol;           /* src/raycast-types.jl:230 (0 = 1e-12) */
     int64_t vertex_capacity;  /* device vertex-table capacity hint (0 = auto) */
@@ -70,7 +72,8 @@ typedef struct hv_stats_t {
     double max_t_relerr;      /* mixed precision: max |t_lowprec - t_fp64| / t_fp64 of winners */
     double ms_total;          /* device time of the last hv_build_mesh */
     double ms_raycast;        /* device time spent in the candidate-scan kernel */
-    int64_t rays_skipped;     /* rays dropped because an earlier chunk of their level closed their edge */
+    int64_t rays_skipped;     /* rays resolved without a scan (their far end point was found by
+                                 another ray of the level: same tentative key, or closed edge) */
 } hv_stats_t;
 
 /* Context and inputs ------------------------------------------------------ */
