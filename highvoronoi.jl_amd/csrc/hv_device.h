@@ -34,9 +34,90 @@ struct DevGeo {
     const int* __restrict__ knn;     // HV_KNN nearest generators of each sorted position (sorted indices)
     double dlo[HV_MAXD], dhi[HV_MAXD];  // domain box (Morton frame)
     double xmax;                        // bound on |x| over the domain (rounding bounds)
+    // mixed-precision scan (leaves of 32): per sorted point a 32-slot BF16 K-vector relative
+    // to the origin of its group of 8 leaves (16 dwords); per group the origin (slots
+    // 0..D-1) and the radius of the group about it (slot 7)
+    const uint32_t* __restrict__ pkv;
+    const double* __restrict__ lorg;
 };
 
 #define HV_KNN 16
+
+typedef unsigned int hv_u32x4 __attribute__((ext_vector_type(4)));
+typedef float hv_f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 hv_bf16x8 __attribute__((ext_vector_type(8)));
+
+// BF16 round-to-nearest-even of a finite float, as the 16-bit pattern
+__device__ __forceinline__ uint32_t hv_bf16_rn(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ __forceinline__ double hv_bf16_val(uint32_t h) { return (double)__uint_as_float(h << 16); }
+// v ~ h + m with two BF16 terms: |v - h - m| <= (2^-16 + 2^-23) |v|
+__device__ __forceinline__ void hv_split2(double v, uint32_t& h, uint32_t& m) {
+    h = hv_bf16_rn((float)v);
+    m = hv_bf16_rn((float)(v - hv_bf16_val(h)));
+}
+// one 16x16x32 BF16 tile product, FP32 accumulate from zero: lane l receives rows
+// 4*(l/16)+e (e = 0..3) of column l%16
+__device__ __forceinline__ hv_f32x4 hv_mfma_bf16(hv_u32x4 a, hv_u32x4 b) {
+    const hv_f32x4 z = {0.f, 0.f, 0.f, 0.f};
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(hv_bf16x8, a), __builtin_bit_cast(hv_bf16x8, b),
+                                                   z, 0, 0, 0);
+}
+// K-slot layout of the mixed-precision membership form (x' = x - o, w = -2 (c - o)):
+//   point: [0,D) xh | [D,2D) xh | [2D,3D) xm | 3D: sh | 3D+1: sm | 3D+2: 1 | 3D+3: 1
+//   ray:   [0,D) wh | [D,2D) wm | [2D,3D) wh | 3D: 1  | 3D+1: 1  | 3D+2: ch | 3D+3: cm
+// so the dot product is |x'|^2 - 2<x', c - o> + (|c - o|^2 - R^2 - margin) up to the split terms.
+template <int D>
+__device__ __forceinline__ hv_u32x4 hv_pack_slots(const uint32_t* sl, int q) {
+    hv_u32x4 r;
+    r[0] = sl[8 * q + 0] | (sl[8 * q + 1] << 16);
+    r[1] = sl[8 * q + 2] | (sl[8 * q + 3] << 16);
+    r[2] = sl[8 * q + 4] | (sl[8 * q + 5] << 16);
+    r[3] = sl[8 * q + 6] | (sl[8 * q + 7] << 16);
+    return r;
+}
+
+// Ray K-vector for one group of leaves: ball centre c, radius^2 R2, FP64 filter slack feps,
+// group origin o and radius rad (every point of the group within rad of o).  The margin bounds the BF16 split
+// error (2^-16 per split, three omitted cross terms per coordinate) and the FP32 accumulation
+// of at most 22 exact products (21 roundings of 2^-23), so that acc < 0 for every generator
+// with f(x) = |x - c|^2 - R2 <= feps.
+template <int D>
+__device__ __forceinline__ void hv_ray_kvec(const double* c, double R2, double feps, const double* o, double rad,
+                                            hv_u32x4* row) {
+    double beta[D], bb = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        beta[k] = c[k] - o[k];
+        bb = fma(beta[k], beta[k], bb);
+    }
+    const double fcp = bb - R2;
+    const double margin = 1.2e-4 * rad * sqrt(bb) + 2.5e-5 * (rad * rad + fabs(fcp)) + feps +
+                          1e-13 * (bb + R2 + rad * rad);
+    uint32_t sl[24];
+#pragma unroll
+    for (int s = 0; s < 24; ++s) sl[s] = 0u;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        uint32_t h, m;
+        hv_split2(-2.0 * beta[k], h, m);
+        sl[k] = h;
+        sl[D + k] = m;
+        sl[2 * D + k] = h;
+    }
+    sl[3 * D] = 0x3f80u;
+    sl[3 * D + 1] = 0x3f80u;
+    uint32_t ch, cm;
+    hv_split2(fcp - margin, ch, cm);
+    sl[3 * D + 2] = ch;
+    sl[3 * D + 3] = cm;
+    row[0] = hv_pack_slots<D>(sl, 0);
+    row[1] = hv_pack_slots<D>(sl, 1);
+    row[2] = hv_pack_slots<D>(sl, 2);
+}
 
 struct DevRays {
     int64_t* slot;   // edge-hash slot of the walked edge

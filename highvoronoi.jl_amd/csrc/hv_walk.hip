@@ -214,6 +214,78 @@ __global__ void k_tree8_level(DevGeo g, int l, int nodes_at_l, float* __restrict
     for (int k = 0; k < D; ++k) { o[k] = lo[k]; o[D + k] = hi[k]; }
 }
 
+// Mixed-precision scan tables (leaves of 32 points): the origin of each group of 8 leaves
+// (the children of one last-level tree node; centre of the group's bounding box) with the
+// radius of the group about it, then one BF16 K-vector per sorted point relative to its
+// group origin.  Records past N are padding whose form is
+// +1e30 for every ray.
+template <int D>
+__global__ void k_leaf_origin(DevGeo g, double* __restrict__ lorg) {
+    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;   // group of 8 leaves (256 points)
+    if (leaf >= (g.nleaf + 7) / 8) return;
+    const int j0 = leaf * 256, j1 = min(j0 + 256, g.N);
+    double o[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        double mn = 1e300, mx = -1e300;
+        for (int j = j0; j < j1; ++j) {
+            const double v = g.xs[(int64_t)j * (D + 1) + k];
+            mn = fmin(mn, v);
+            mx = fmax(mx, v);
+        }
+        o[k] = 0.5 * (mn + mx);
+    }
+    double r2 = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double t = g.xs[(int64_t)j * (D + 1) + k] - o[k];
+            s = fma(t, t, s);
+        }
+        r2 = fmax(r2, s);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) lorg[(int64_t)leaf * 8 + k] = k < D ? o[k] : 0.0;
+    lorg[(int64_t)leaf * 8 + 7] = sqrt(r2) * (1.0 + 1e-12) + 1e-300;
+}
+
+template <int D>
+__global__ void k_point_kvec(DevGeo g, const double* __restrict__ lorg, hv_u32x4* __restrict__ pkv) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= g.nleaf * 32) return;
+    uint32_t sl[32];
+#pragma unroll
+    for (int s = 0; s < 32; ++s) sl[s] = 0u;
+    if (j < g.N) {
+        const int leaf = j >> 8;
+        double xp[D], s2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            xp[k] = g.xs[(int64_t)j * (D + 1) + k] - lorg[(int64_t)leaf * 8 + k];
+            s2 = fma(xp[k], xp[k], s2);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            uint32_t h, m;
+            hv_split2(xp[k], h, m);
+            sl[k] = h;
+            sl[D + k] = h;
+            sl[2 * D + k] = m;
+        }
+        uint32_t sh, sm;
+        hv_split2(s2, sh, sm);
+        sl[3 * D] = sh;
+        sl[3 * D + 1] = sm;
+        sl[3 * D + 2] = 0x3f80u;
+        sl[3 * D + 3] = 0x3f80u;
+    } else {
+        sl[3 * D] = hv_bf16_rn(1e30f);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pkv[(int64_t)j * 4 + q] = hv_pack_slots<D>(sl, q);
+}
+
 // ---------------------------------------------------------------------------------
 // seeds: one block per box corner, brute-force nearest generator (ties -> smaller index)
 
@@ -605,6 +677,16 @@ __device__ __forceinline__ void hv_wave_sync() {
 
 constexpr double HV_W = 1e-7;
 
+// Workgroups are handed to the 8 XCDs round-robin (block b -> XCD b % 8), each XCD with its
+// own L2.  Renumber so that XCD x works on one contiguous range of blocks: neighbouring
+// packets (Morton order) then share an L2 instead of spreading every region over all eight.
+__device__ __forceinline__ int hv_xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int x = b & 7, per = nb >> 3, extra = nb & 7;
+    return x * per + min(x, extra) + (b >> 3);
+}
+
+
 // read-only tables seen through the constant address space: uniform loads become SMEM
 typedef const __attribute__((address_space(4))) double hv_cdouble;
 typedef const __attribute__((address_space(4))) float hv_cfloat;
@@ -754,19 +836,26 @@ __global__ __launch_bounds__(256) void k_ray_init(DevGeo g, DevRays R, int nrays
 // (src/raycast.jl:877-916) is applied to them at the end.
 // One wavefront per workgroup: packets take very different times, and a workgroup of
 // several wavefronts holds its slots until its slowest packet is done.
-template <int D>
+template <int D, int MODE>
 __global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
                                                   const int* __restrict__ vkey, const double* __restrict__ vx,
                                                   int* __restrict__ status, int* __restrict__ win,
                                                   double* __restrict__ tout, double plane_tol,
                                                   unsigned long long* __restrict__ ctr) {
-    __shared__ int stk[1][128], stkl[1][128];
-    __shared__ float cbx[1][16 * D];
-    __shared__ unsigned long long hms[1][8];
-    __shared__ float dcs[1][8];
-    __shared__ double leafx[1][D + 1][64];
-    const int lane = threadIdx.x & 63, wv = 0;
-    const int rid = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int stk[128], stkl[128];
+    __shared__ float cbx[16 * D];
+    __shared__ unsigned long long hms[8];
+    __shared__ float dcs[8];
+    // MODE 0: the staged leaf (FP64).  MODE 1: ray K-vectors of the current node (MFMA B
+    // operand source, 80-byte rows) and, per ray, which points of the current leaf it
+    // already holds as candidates.
+    __shared__ double leafx[D + 1][MODE == 0 ? 64 : 1];
+    __shared__ hv_u32x4 rk4[MODE == 1 ? 64 : 1][5];
+    __shared__ uint32_t kmk[MODE == 1 ? 64 : 1];
+    __shared__ int kid[MODE == 1 ? 64 : 1][D];
+    __shared__ double nog[MODE == 1 ? 8 : 1];
+    const int lane = threadIdx.x & 63;
+    const int rid = hv_xcd_block() * 64 + lane;
     const bool act = rid < nrays;
     const int ray = act ? order[rid] : order[0];
     const int N = g.N;
@@ -777,32 +866,43 @@ __global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int
     T.t2 = R.t2[ray];
     T.i1 = R.i1[ray];
     T.i2 = R.i2[ray];
-    // certificate ball and the membership form f(x) = |x|^2 - 2<x, bc> + fC
-    double bc[D], R2, fC, feps;
+    // certificate ball B(tau) (centre r + tau u) and the membership form
+    // f(x) = |x|^2 - 2<x, bc> + fC; bcf / R2f are an FP32 centre and a radius^2 widened so
+    // that the FP32 box test never drops a box the FP64 ball reaches.  MODE 1 keeps no FP64
+    // centre: it is formed again where needed (registers bound the occupancy).
+    const int e0 = G.e[0];
+    double bc[MODE == 0 ? D : 1], tball, R2, fC, feps;
+    float bcf[D], R2f;
+    const double slackf = 1e-6 * (g.xmax + 1.0);
     auto set_ball = [&](double tau) {
+        tball = tau;
         R2 = 0.0;
         double pp = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            bc[k] = G.r[k] + tau * G.u[k];
-            const double t = bc[k] - G.x0[k];
+            const double b = fma(tau, G.u[k], G.r[k]);
+            if constexpr (MODE == 0) bc[k] = b;
+            const double t = b - g.xo[(int64_t)e0 * D + k];
             R2 = fma(t, t, R2);
-            pp = fma(bc[k], bc[k], pp);
+            pp = fma(b, b, pp);
+            bcf[k] = (float)b;
         }
         fC = pp - R2;
         // rounding of the expanded form, with room: |x| <= xmax for every generator
         const double sc = sqrt(pp) + g.xmax;
         feps = 64.0 * 2.220446049250313e-16 * sc * sc;
         R2 = R2 * (1.0 + 1e-9) + 1e-300;
+        const double rr = sqrt(R2) + slackf;
+        R2f = (float)(rr * rr * (1.0 + 1e-5));
     };
     set_ball(T.t1 + HV_W);
-    // packet centre: mean ball centre over the active lanes
-    double pc[D];
+    // packet centre: mean ball centre over the active lanes (orders the children)
+    float pcf[D];
     {
         const double wgt = act ? 1.0 : 0.0;
-        double cnt = wgt;
+        double cnt = wgt, pc[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) pc[k] = wgt * bc[k];
+        for (int k = 0; k < D; ++k) pc[k] = wgt * (double)bcf[k];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
@@ -811,24 +911,34 @@ __global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int
         }
         const double ic = cnt > 0 ? 1.0 / cnt : 0.0;
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            // wave-uniform: keep it in scalar registers
-            const double v = pc[k] * ic;
-            const long long bits = __double_as_longlong(v);
-            const int lo = __builtin_amdgcn_readfirstlane((int)(bits & 0xffffffffll));
-            const int hi = __builtin_amdgcn_readfirstlane((int)(bits >> 32));
-            pc[k] = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-        }
+        for (int k = 0; k < D; ++k)
+            pcf[k] = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint((float)(pc[k] * ic))));
     }
-    unsigned long long pairs = 0, leaves = 0;
-    // box of child c of the staged node, distance^2 to a point
-    auto child_d2 = [&](int c, const double* p) {
-        double d2 = 0.0;
+    unsigned long long pairs = 0, leaves = 0, rechecks = 0;
+    bool flagged = false;
+    // candidates the ray already holds: its edge generators (sorted ids) and the best two
+    if constexpr (MODE == 1) {
+#pragma unroll
+        for (int z = 0; z < D; ++z) kid[lane][z] = G.e[z] < N ? g.rank[G.e[z]] : -1;
+        rk4[lane][3] = hv_u32x4{0u, 0u, 0u, 0u};   // K-slots 24..31
+    }
+    auto leaf_known = [&](int leaf) -> uint32_t {
+        uint32_t km = 0u;
+#pragma unroll
+        for (int z = 0; z < D + 2; ++z) {
+            const int id = z < D ? kid[lane][z] : (z == D ? T.i1 : T.i2);
+            km |= (id >= 0 && (id >> 5) == leaf) ? (1u << (id & 31)) : 0u;
+        }
+        return km;
+    };
+    // box of child c of the staged node (FP32), distance^2 to a point
+    auto child_d2f = [&](int c, const float* p) {
+        float d2 = 0.f;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const double lo = (double)cbx[wv][c * 2 * D + k], hi = (double)cbx[wv][c * 2 * D + D + k];
-            const double a = fmax(fmax(lo - p[k], p[k] - hi), 0.0);
-            d2 = fma(a, a, d2);
+            const float lo = cbx[c * 2 * D + k], hi = cbx[c * 2 * D + D + k];
+            const float a = fmaxf(fmaxf(lo - p[k], p[k] - hi), 0.f);
+            d2 = fmaf(a, a, d2);
         }
         return d2;
     };
@@ -838,7 +948,7 @@ __global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int
         float bd = 0.f;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const float v = dcs[wv][c];
+            const float v = dcs[c];
             const bool in = (rem >> c) & 1u;
             const bool better = best < 0 || (nearest ? v < bd : v > bd);
             if (in && better) { best = c; bd = v; }
@@ -849,34 +959,54 @@ __global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int
         const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
         if (lane < npts) {
 #pragma unroll
-            for (int k = 0; k < D; ++k) pf[k] = g.xs[(int64_t)(j0 + lane) * (D + 1) + k];
-            pf[D] = g.xs[(int64_t)(j0 + lane) * (D + 1) + D];
+            for (int k = 0; k <= D; ++k) pf[k] = g.xs[(int64_t)(j0 + lane) * (D + 1) + k];
         }
     };
+    // The child boxes of the node walked next (and, MODE 1, the origin of its leaf group)
+    // are loaded into registers while the current node is worked on.
+    float cbr0 = 0.f, cbr1 = 0.f;
+    double lorr = 0.0;
+    int pref = -1;
+    auto prefetch = [&](int node, int lvl) {
+        const float* src = g.cb + (int64_t)node * (16 * D);
+        if (lane < 16 * D) cbr0 = src[lane];
+        if (lane < 16 * D - 64) cbr1 = src[lane + 64];
+        if constexpr (MODE == 1) {
+            if (lvl + 1 == g.h && lane < 8) lorr = g.lorg[(int64_t)(node - ((1 << (3 * lvl)) - 1) / 7) * 8 + lane];   // index in its level
+        }
+        pref = node;
+    };
     int sp = 1;
-    stk[wv][0] = 0;
-    stkl[wv][0] = 0;
+    stk[0] = 0;
+    stkl[0] = 0;
+    prefetch(0, 0);
     while (sp > 0) {
         --sp;
         hv_wave_sync();
-        const int gid = __builtin_amdgcn_readfirstlane(stk[wv][sp]);
-        const int l = __builtin_amdgcn_readfirstlane(stkl[wv][sp]);
+        const int gid = __builtin_amdgcn_readfirstlane(stk[sp]);
+        const int l = __builtin_amdgcn_readfirstlane(stkl[sp]);
+        if (gid != pref) prefetch(gid, l);
         // stage the 8 child boxes of the node (one contiguous block)
-        {
-            const float* src = g.cb + (int64_t)gid * (16 * D);
-            for (int q = lane; q < 16 * D; q += 64) cbx[wv][q] = src[q];
+        if (lane < 16 * D) cbx[lane] = cbr0;
+        if (lane < 16 * D - 64) cbx[lane + 64] = cbr1;
+        if constexpr (MODE == 1) {
+            if (l + 1 == g.h && lane < 8) nog[lane] = lorr;
+        }
+        // a node whose children are leaves pushes nothing: the stack top is walked next
+        if (l + 1 == g.h && sp > 0) {
+            const int nx = __builtin_amdgcn_readfirstlane(stk[sp - 1]);
+            const int nl = __builtin_amdgcn_readfirstlane(stkl[sp - 1]);
+            prefetch(nx, nl);
         }
         hv_wave_sync();
         unsigned rem = 0u;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const unsigned long long m = __ballot(act && (child_d2(c, bc) <= R2));
-            if (lane == 0) {
-                hms[wv][c] = m;
-                dcs[wv][c] = (float)child_d2(c, pc);
-            }
+            const unsigned long long m = __ballot(act && (child_d2f(c, bcf) <= R2f));
+            if (lane == 0) hms[c] = m;
             rem |= (m != 0ull) ? (1u << c) : 0u;
         }
+        if (lane < 8) dcs[lane] = child_d2f(lane, pcf);
         rem = __builtin_amdgcn_readfirstlane(rem);
         hv_wave_sync();
         if (rem == 0u) continue;
@@ -884,92 +1014,195 @@ __global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int
         for (int q = 0; q < l; ++q) { off += pw; pw *= 8; }
         const int i = gid - off;
         if (l + 1 == g.h) {
-            // children are leaves: nearest first, the next one loaded while this one is scanned
-            double pf[D + 1];
-            int cc = __builtin_amdgcn_readfirstlane(pick(rem, true));
-            rem &= ~(1u << cc);
-            load_leaf(8 * i + cc, pf);
-            while (cc >= 0) {
-                const int cn = rem ? __builtin_amdgcn_readfirstlane(pick(rem, true)) : -1;
-                if (cn >= 0) rem &= ~(1u << cn);
-                const int leaf = 8 * i + cc;
-                const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
-                // balls only shrink: re-test with the current one
-                const bool hit = act && ((hms[wv][cc] >> lane) & 1ull) && (child_d2(cc, bc) <= R2);
-                const unsigned long long m = __ballot(hit);
-                if (m && lane < npts) {
+            if constexpr (MODE == 0) {
+                // children are leaves: nearest first, the next one loaded while this one is scanned
+                double pf[D + 1];
+                int cc = __builtin_amdgcn_readfirstlane(pick(rem, true));
+                rem &= ~(1u << cc);
+                load_leaf(8 * i + cc, pf);
+                while (cc >= 0) {
+                    const int cn = rem ? __builtin_amdgcn_readfirstlane(pick(rem, true)) : -1;
+                    if (cn >= 0) rem &= ~(1u << cn);
+                    const int leaf = 8 * i + cc;
+                    const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
+                    // balls only shrink: re-test with the current one
+                    const bool hit = act && ((hms[cc] >> lane) & 1ull) && (child_d2f(cc, bcf) <= R2f);
+                    const unsigned long long m = __ballot(hit);
+                    if (m && lane < npts) {
 #pragma unroll
-                    for (int k = 0; k <= D; ++k) leafx[wv][k][lane] = pf[k];
-                }
-                if (cn >= 0) load_leaf(8 * i + cn, pf);
-                if (m) {
-                    hv_wave_sync();
-                    leaves += 1;
-                    pairs += (unsigned long long)__popcll(m) * (unsigned long long)npts;
-                    if (hit) {
-                        const double before = T.t1;
-                        // membership f = |x|^2 + fC - 2<x, bc>, four independent chains
-                        int q = 0;
-                        for (; q + 4 <= npts; q += 4) {
-                            double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0;
+                        for (int k = 0; k <= D; ++k) leafx[k][lane] = pf[k];
+                    }
+                    if (cn >= 0) load_leaf(8 * i + cn, pf);
+                    if (m) {
+                        hv_wave_sync();
+                        leaves += 1;
+                        pairs += (unsigned long long)__popcll(m) * (unsigned long long)npts;
+                        if (hit) {
+                            const double before = T.t1;
+                            // membership f = |x|^2 + fC - 2<x, bc>, four independent chains
+                            int q = 0;
+                            for (; q + 4 <= npts; q += 4) {
+                                double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0;
 #pragma unroll
-                            for (int k = 0; k < D; ++k) {
-                                f0 = fma(leafx[wv][k][q], bc[k], f0);
-                                f1 = fma(leafx[wv][k][q + 1], bc[k], f1);
-                                f2 = fma(leafx[wv][k][q + 2], bc[k], f2);
-                                f3 = fma(leafx[wv][k][q + 3], bc[k], f3);
+                                for (int k = 0; k < D; ++k) {
+                                    f0 = fma(leafx[k][q], bc[k], f0);
+                                    f1 = fma(leafx[k][q + 1], bc[k], f1);
+                                    f2 = fma(leafx[k][q + 2], bc[k], f2);
+                                    f3 = fma(leafx[k][q + 3], bc[k], f3);
+                                }
+                                f0 = fma(-2.0, f0, leafx[D][q] + fC);
+                                f1 = fma(-2.0, f1, leafx[D][q + 1] + fC);
+                                f2 = fma(-2.0, f2, leafx[D][q + 2] + fC);
+                                f3 = fma(-2.0, f3, leafx[D][q + 3] + fC);
+                                if (fmin(fmin(f0, f1), fmin(f2, f3)) < feps) {
+                                    const double fs[4] = {f0, f1, f2, f3};
+#pragma unroll
+                                    for (int z = 0; z < 4; ++z) {
+                                        if (fs[z] < feps) {
+                                            double xk[D];
+#pragma unroll
+                                            for (int k = 0; k < D; ++k) xk[k] = leafx[k][q + z];
+                                            hv_consider<D>(G, T, xk, leafx[D][q + z], j0 + q + z, plane_tol);
+                                        }
+                                    }
+                                }
                             }
-                            f0 = fma(-2.0, f0, leafx[wv][D][q] + fC);
-                            f1 = fma(-2.0, f1, leafx[wv][D][q + 1] + fC);
-                            f2 = fma(-2.0, f2, leafx[wv][D][q + 2] + fC);
-                            f3 = fma(-2.0, f3, leafx[wv][D][q + 3] + fC);
-                            if (fmin(fmin(f0, f1), fmin(f2, f3)) < feps) {
-                                const double fs[4] = {f0, f1, f2, f3};
+                            for (; q < npts; ++q) {
+                                double f = 0.0;
 #pragma unroll
-                                for (int z = 0; z < 4; ++z) {
-                                    if (fs[z] < feps) {
-                                        double xk[D];
+                                for (int k = 0; k < D; ++k) f = fma(leafx[k][q], bc[k], f);
+                                f = fma(-2.0, f, leafx[D][q] + fC);
+                                if (f < feps) {
+                                    double xk[D];
 #pragma unroll
-                                        for (int k = 0; k < D; ++k) xk[k] = leafx[wv][k][q + z];
-                                        hv_consider<D>(G, T, xk, leafx[wv][D][q + z], j0 + q + z, plane_tol);
+                                    for (int k = 0; k < D; ++k) xk[k] = leafx[k][q];
+                                    hv_consider<D>(G, T, xk, leafx[D][q], j0 + q, plane_tol);
+                                }
+                            }
+                            if (T.t1 < before) set_ball(T.t1 + HV_W);
+                        }
+                        hv_wave_sync();
+                    }
+                    cc = cn;
+                }
+            } else {
+                // Children are leaves of 32 points (two 16-point tiles).  Each ray's K-vector
+                // is formed once per node, relative to the node origin that the point vectors
+                // were built with; the B operands of the four 16-ray tiles stay in registers.
+                // A ball that shrinks inside the node keeps the older (larger) vector, which
+                // only lets more points through the filter.
+                hv_u32x4 pa0, pa1;
+                auto load_leaf1 = [&](int leaf, hv_u32x4& a0, hv_u32x4& a1) {
+                    const hv_u32x4* base = (const hv_u32x4*)(g.pkv) + ((int64_t)leaf * 32 + (lane & 15)) * 4 + (lane >> 4);
+                    a0 = base[0];
+                    a1 = base[16 * 4];
+                };
+                int cc = __builtin_amdgcn_readfirstlane(pick(rem, true));
+                rem &= ~(1u << cc);
+                load_leaf1(8 * i + cc, pa0, pa1);
+                {
+                    double no[D], cb_[D];
+#pragma unroll
+                    for (int k = 0; k < D; ++k) {
+                        no[k] = nog[k];
+                        cb_[k] = fma(tball, G.u[k], G.r[k]);
+                    }
+                    hv_ray_kvec<D>(cb_, R2, feps, no, nog[7], rk4[lane]);
+                }
+                hv_wave_sync();
+                while (cc >= 0) {
+                    const int cn = rem ? __builtin_amdgcn_readfirstlane(pick(rem, true)) : -1;
+                    if (cn >= 0) rem &= ~(1u << cn);
+                    const int leaf = 8 * i + cc;
+                    const int j0 = leaf * 32, npts = max(0, min(32, N - j0));
+                    const bool hit = act && ((hms[cc] >> lane) & 1ull) && (child_d2f(cc, bcf) <= R2f);
+                    const unsigned long long m = __ballot(hit);
+                    const hv_u32x4 ca0 = pa0, ca1 = pa1;
+                    if (cn >= 0) load_leaf1(8 * i + cn, pa0, pa1);
+                    if (m) {
+                        leaves += 1;
+                        kmk[lane] = leaf_known(leaf);
+                        hv_wave_sync();
+#pragma unroll
+                        for (int g4 = 0; g4 < 4; ++g4) {
+                            if (((m >> (16 * g4)) & 0xffffull) == 0ull) continue;
+#pragma unroll
+                            for (int t = 0; t < 2; ++t) {
+                                if (16 * t >= npts) continue;
+                                pairs += (unsigned long long)(16 * min(16, npts - 16 * t));
+                                const hv_f32x4 acc = hv_mfma_bf16(t == 0 ? ca0 : ca1, rk4[16 * g4 + (lane & 15)][lane >> 4]);
+                                // lane holds points 16t + 4*(lane/16) + e of ray 16*g4 + lane%16
+                                const uint32_t ob = __float_as_uint(acc[0]) | __float_as_uint(acc[1]) |
+                                                    __float_as_uint(acc[2]) | __float_as_uint(acc[3]);
+                                if (__ballot((int)ob < 0) == 0ull) continue;
+                                // survivors that the ray does not already hold go to FP64
+                                const int rho = 16 * g4 + (lane & 15);
+                                const int sh = 16 * t + 4 * (lane >> 4);
+                                unsigned real = 0u;
+                                if ((int)ob < 0) {
+                                    const uint32_t known = kmk[rho] >> sh;
+#pragma unroll
+                                    for (int e = 0; e < 4; ++e)
+                                        if ((int)__float_as_uint(acc[e]) < 0 && !((known >> e) & 1u) && sh + e < npts)
+                                            real |= 1u << e;
+                                }
+                                unsigned long long rm = __ballot(real != 0u);
+                                while (rm) {
+                                    const int l2 = __ffsll((long long)rm) - 1;
+                                    rm &= rm - 1ull;
+                                    const unsigned bits = __builtin_amdgcn_readlane(real, l2);
+                                    const int rho2 = 16 * g4 + (l2 & 15);
+                                    const int pb2 = j0 + 16 * t + 4 * (l2 >> 4);
+                                    if (lane == rho2) {
+                                        const double before = T.t1;
+#pragma unroll
+                                        for (int e = 0; e < 4; ++e) {
+                                            if ((bits >> e) & 1u) {
+                                                double xk[D];
+#pragma unroll
+                                                for (int k = 0; k < D; ++k) xk[k] = g.xs[(int64_t)(pb2 + e) * (D + 1) + k];
+                                                hv_consider<D>(G, T, xk, g.xs[(int64_t)(pb2 + e) * (D + 1) + D], pb2 + e,
+                                                               plane_tol);
+                                                ++rechecks;
+                                            }
+                                        }
+                                        if (T.t1 < before) set_ball(T.t1 + HV_W);
+                                        kmk[lane] = leaf_known(leaf);
+                                        flagged = true;
                                     }
                                 }
                             }
                         }
-                        for (; q < npts; ++q) {
-                            double f = 0.0;
-#pragma unroll
-                            for (int k = 0; k < D; ++k) f = fma(leafx[wv][k][q], bc[k], f);
-                            f = fma(-2.0, f, leafx[wv][D][q] + fC);
-                            if (f < feps) {
-                                double xk[D];
-#pragma unroll
-                                for (int k = 0; k < D; ++k) xk[k] = leafx[wv][k][q];
-                                hv_consider<D>(G, T, xk, leafx[wv][D][q], j0 + q, plane_tol);
-                            }
-                        }
-                        if (T.t1 < before) set_ball(T.t1 + HV_W);
+                        hv_wave_sync();
                     }
-                    hv_wave_sync();
+                    cc = cn;
                 }
-                cc = cn;
             }
         } else {
             // internal children: farthest pushed first so the nearest is walked next
             const int cbase = off + pw + 8 * i;
+            int last = -1;
             while (rem) {
                 const int c = __builtin_amdgcn_readfirstlane(pick(rem, false));
                 rem &= ~(1u << c);
-                stk[wv][sp] = cbase + c;
-                stkl[wv][sp] = l + 1;
+                stk[sp] = cbase + c;
+                stkl[sp] = l + 1;
                 ++sp;
+                last = cbase + c;
             }
+            prefetch(last, l + 1);
         }
     }
     if (lane == 0) {
         atomicAdd(&ctr[0], pairs);
         atomicAdd(&ctr[1], leaves);
-        atomicMax(&ctr[3], leaves);
+    }
+    if constexpr (MODE == 1) {
+        // FP64 re-verifications and the rays that needed any
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) rechecks += __shfl_xor(rechecks, off);
+        if (lane == 0 && rechecks) atomicAdd(&ctr[6], rechecks);
+        hv_wave_count(&ctr[7], flagged && act);
     }
     bool tie = false;
     if (act) {
@@ -977,6 +1210,9 @@ __global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int
             status[rid] = HV_ST_NONE;
         } else {
             // original indices and the reference's form of t (src/raycast.jl:424-431)
+            double x0t[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) x0t[k] = g.xo[(int64_t)e0 * D + k];
             auto orig = [&](int i) { return i < N ? g.sidx[i] : i; };
             auto t_ref = [&](int w, double tscan, double& score, double& cosang) -> double {
                 if (w >= N) {
@@ -985,7 +1221,7 @@ __global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int
 #pragma unroll
                     for (int k = 0; k < D; ++k) {
                         nu = fma(g.pn[p * D + k], G.u[k], nu);
-                        dist = fma(-g.pn[p * D + k], G.x0[k], dist);
+                        dist = fma(-g.pn[p * D + k], x0t[k], dist);
                     }
                     score = 2.0 * dist * nu;
                     cosang = nu;
@@ -995,8 +1231,8 @@ __global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int
 #pragma unroll
                 for (int k = 0; k < D; ++k) {
                     const double xk = g.xo[(int64_t)(w) * D + k];
-                    const double dx = xk - G.x0[k];
-                    num = fma(dx, G.x0[k] + xk - 2.0 * G.r[k], num);
+                    const double dx = xk - x0t[k];
+                    num = fma(dx, x0t[k] + xk - 2.0 * G.r[k], num);
                     den = fma(G.u[k], dx, den);
                     dd = fma(dx, dx, dd);
                 }
@@ -1015,7 +1251,7 @@ __global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int
                 double rn = 0.0;
 #pragma unroll
                 for (int k = 0; k < D; ++k) rn = fma(G.r[k], G.r[k], rn);
-                const double full_error = (tw * G.du + sqrt(rn) * 1e-15) / c1;
+                const double full_error = (tw * R.du[ray] + sqrt(rn) * 1e-15) / c1;
                 const double window = fmin(1e-7, (10.0 + D) * full_error);
                 if (tw2 < tw || tw2 <= tw + window) {
                     // near tie: the larger <u, x - x0> wins (src/raycast.jl:903-916)
@@ -1083,7 +1319,7 @@ T* pget(hv_ctx* c, const char* name, size_t count, bool keep = false) {
 
 // Points -> SoA + Morton order + leaf boxes + complete binary tree of boxes.
 template <int D>
-int build_tree(hv_ctx* c, int L, Tree& T) {
+int build_tree(hv_ctx* c, int L, Tree& T, bool kvec = false) {
     const int N = (int)c->N;
     DevGeo& g = T.g;
     memset(&g, 0, sizeof g);
@@ -1183,6 +1419,18 @@ int build_tree(hv_ctx* c, int L, Tree& T) {
     }
     k_knn<D><<<nblk(N, 256), 256, 0, c->stream>>>(g, knn);
     HVCK(hipGetLastError());
+    if (kvec) {
+        if (L != 32) return hv_fail(c, HV_ERR_UNSUPPORTED, "the mixed-precision scan uses leaves of 32 points");
+        double* lorg = pget<double>(c, "lorg", (size_t)((g.nleaf + 7) / 8) * 8);
+        hv_u32x4* pkv = pget<hv_u32x4>(c, "pkv", (size_t)g.nleaf * 32 * 4);
+        if (!lorg || !pkv) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (mixed-precision tables)");
+        k_leaf_origin<D><<<nblk((g.nleaf + 7) / 8, 256), 256, 0, c->stream>>>(g, lorg);
+        HVCK(hipGetLastError());
+        k_point_kvec<D><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, c->stream>>>(g, lorg, pkv);
+        HVCK(hipGetLastError());
+        g.lorg = lorg;
+        g.pkv = (const uint32_t*)pkv;
+    }
     int hbad = 0;
     HVCK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HVCK(hipStreamSynchronize(c->stream));
@@ -1317,17 +1565,21 @@ template <int D>
 int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     hv_opts opt{};
     if (o) opt = *o;
+    if (opt.precision != HV_PREC_FP64 && opt.precision != HV_PREC_BF16)
+        return hv_fail(c, HV_ERR_UNSUPPORTED, "precision: HV_PREC_FP64 or HV_PREC_BF16");
+    const bool mixed = opt.precision == HV_PREC_BF16;
+    if (mixed && opt.leaf_size > 0 && opt.leaf_size != 32)
+        return hv_fail(c, HV_ERR_UNSUPPORTED, "the mixed-precision scan uses leaves of 32 points");
     const int L = opt.leaf_size > 0 ? opt.leaf_size : 32;
     const double plane_tol = opt.plane_tolerance > 0 ? opt.plane_tolerance : 1e-12;
     const int KCH = opt.reserved0 > 1 ? opt.reserved0 : 1;   // chunks per level (<0: tentative-key grouping)
-    if (opt.precision != HV_PREC_FP64) return hv_fail(c, HV_ERR_UNSUPPORTED, "precision mode not built yet");
     int rc = check_box(c, D);
     if (rc) return rc;
     const int N = (int)c->N;
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1], r0 = c->ev[2], r1 = c->ev[3];
     HVCK(hipEventRecord(e0, c->stream));
     Tree T;
-    rc = build_tree<D>(c, L, T);
+    rc = build_tree<D>(c, L, T, mixed);
     if (rc) return rc;
     DevGeo& g = T.g;
     const int nseed = 1 << D;
@@ -1511,7 +1763,12 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             int r_ = ensure_vertices(n);
             if (r_) return r_;
             HVCK(hipEventRecord(r0, c->stream));
-            k_raycast64<D><<<nblk(n, 64), 64, 0, c->stream>>>(g, R, list, n, vkey, vx, status, rwin, rt, plane_tol, ctr + 0);
+            if (mixed)
+                k_raycast64<D, 1><<<nblk(n, 64), 64, 0, c->stream>>>(g, R, list, n, vkey, vx, status, rwin, rt, plane_tol,
+                                                                     ctr + 0);
+            else
+                k_raycast64<D, 0><<<nblk(n, 64), 64, 0, c->stream>>>(g, R, list, n, vkey, vx, status, rwin, rt, plane_tol,
+                                                                     ctr + 0);
             HVCK(hipGetLastError());
             HVCK(hipEventRecord(r1, c->stream));
             k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, list, n, vkey, vx, vnext, status, rwin);
@@ -1632,10 +1889,11 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     st.near_ties = (int64_t)hctr[2];
     st.duplicates = (int64_t)hctr[8];
     st.rays_skipped = skipped;
-    st.fallbacks = (int64_t)hctr[7];      // FP64 path: rays whose kNN start found no generator ahead
-    st.retries = (int64_t)hctr[6];        // packets that visited more than 512 leaves
+    st.fallbacks = (int64_t)hctr[7];
+    st.fp64_rechecks = (int64_t)hctr[6];
+    st.retries = 0;
     st.unbounded = 0;
-    st.max_t_relerr = (double)hctr[3];    // most leaves visited by one packet
+    st.max_t_relerr = 0.0;
     st.ms_total = ms_all;
     st.ms_raycast = ms_ray;
     hv_mesh* m = new hv_mesh();
@@ -1798,7 +2056,9 @@ __global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const 
 template <int D>
 int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
                           const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status) {
-    if (precision != HV_PREC_FP64) return hv_fail(c, HV_ERR_UNSUPPORTED, "precision mode not built yet");
+    if (precision != HV_PREC_FP64 && precision != HV_PREC_BF16)
+        return hv_fail(c, HV_ERR_UNSUPPORTED, "precision: HV_PREC_FP64 or HV_PREC_BF16");
+    const bool mixed = precision == HV_PREC_BF16;
     if (nrays == 0) return HV_OK;
     if (nrays > (1 << 29)) return hv_fail(c, HV_ERR_ARG, "too many rays");
     const int N = (int)c->N;
@@ -1821,7 +2081,7 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
         if (first >= N) return hv_fail(c, HV_ERR_ARG, "the walked edge must contain a generator");
     }
     Tree T;
-    rc = build_tree<D>(c, 32, T);
+    rc = build_tree<D>(c, 32, T, mixed);
     if (rc) return rc;
     DevGeo& g = T.g;
     int* vkey = pget<int>(c, "b_vkey", (size_t)2 * nrays * (D + 1));
@@ -1866,8 +2126,12 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     k_iota<<<nblk(nrays, 256), 256, 0, c->stream>>>(ord0, (int)nrays);
     HVCK(hipGetLastError());
     HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, (int)nrays, 0, 64, c->stream));
-    k_raycast64<D><<<nblk(nrays, 64), 64, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, status, rwin, tout,
-                                                            plane_tol, ctr);
+    if (mixed)
+        k_raycast64<D, 1><<<nblk(nrays, 64), 64, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, status, rwin, tout,
+                                                                 plane_tol, ctr);
+    else
+        k_raycast64<D, 0><<<nblk(nrays, 64), 64, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, status, rwin, tout,
+                                                                 plane_tol, ctr);
     HVCK(hipGetLastError());
     k_finalize<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, nrays, status, rwin);
     HVCK(hipGetLastError());

@@ -37,8 +37,10 @@ extern "C" {
 #define HV_ERR_STATE -8        /* call order (e.g. no points set) */
 
 #define HV_PREC_FP64 0         /* plain FP64 candidate scan */
-#define HV_PREC_FP32 1         /* FP32 candidate filter + FP64 re-verification */
-#define HV_PREC_BF16 2         /* BF16 MFMA candidate filter + FP64 re-verification */
+#define HV_PREC_FP32 1         /* reserved: returns HV_ERR_UNSUPPORTED */
+#define HV_PREC_BF16 2         /* BF16 MFMA candidate filter (hi/lo split, error-bounded margin)
+                                  + FP64 re-verification of every survivor: the vertex set and
+                                  coordinates equal those of HV_PREC_FP64; leaves of 32 points */
 
 typedef struct hv_ctx hv_ctx;
 typedef struct hv_mesh hv_mesh;
@@ -61,15 +63,18 @@ typedef struct hv_stats_t {
     int64_t levels;           /* BFS levels of the device walk */
     int64_t vertices;         /* vertices in the mesh */
     int64_t duplicates;       /* rays whose vertex was found by another ray of its level */
-    int64_t pairs_scanned;    /* (ray, generator) pairs evaluated by the candidate scan */
+    int64_t pairs_scanned;    /* (ray, generator) pairs evaluated by the candidate scan (BF16:
+                                 the 16x16 MFMA tiles formed) */
     int64_t leaves_visited;   /* (packet, leaf) visits */
     int64_t near_ties;        /* rays resolved by the reference's near-tie rule */
     int64_t degenerate;       /* rays flagged non-general (key longer than d+1) */
-    int64_t fallbacks;        /* mixed precision: rays re-verified with >1 FP64 candidate */
-    int64_t fp64_rechecks;    /* mixed precision: candidates re-evaluated in FP64 */
+    int64_t fallbacks;        /* mixed precision: rays whose filter passed a candidate the ray
+                                 did not already hold (each such ray is re-verified in FP64) */
+    int64_t fp64_rechecks;    /* mixed precision: (ray, generator) pairs re-evaluated in FP64 */
     int64_t retries;          /* rays re-cast with a larger certificate radius */
     int64_t unbounded;        /* rays without an end point */
-    double max_t_relerr;      /* mixed precision: max |t_lowprec - t_fp64| / t_fp64 of winners */
+    double max_t_relerr;      /* 0: the filter computes no t of its own (winners are chosen by
+                                 the FP64 rule only) */
     double ms_total;          /* device time of the last hv_build_mesh */
     double ms_raycast;        /* device time spent in the candidate-scan kernel */
     int64_t rays_skipped;     /* rays resolved without a scan (their far end point was found by

@@ -1,0 +1,103 @@
+"""GPU parity of the mixed-precision scan (BF16 MFMA filter + FP64 re-verification).
+
+Bar: the BF16 build returns exactly the vertex set of the FP64 build and of the oracle
+(bit-exact keys) with the same coordinates, because every survivor of the filter is
+decided by the FP64 rule; at the 5D N=100k mixture the check is the whole key table.
+"""
+import numpy as np
+import pytest
+
+from helpers import golden_cases, key_digest, key_set, load_golden, rows_view
+from hvgpu import synth
+from hvgpu._lib import HVError
+from oracle_py import Oracle
+
+pytestmark = pytest.mark.gpu
+
+COORD_TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from hvgpu._lib import Context
+    return Context(0)
+
+
+def build(ctx, xs, precision):
+    b, nrm = synth.cube_planes(xs.shape[1])
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    m = ctx.build_mesh(precision=precision)
+    keys, coords = m.export()
+    st = m.stats()
+    m.free()
+    return keys, coords, st
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_bf16_matches_exact_fixture(ctx, name):
+    g, xs, keys, exact = load_golden(name)
+    k, c, st = build(ctx, xs, "bf16")
+    assert key_set(k) == key_set(keys)
+
+
+@pytest.mark.parametrize("d,n,seed", [(2, 1000, 0), (3, 3000, 5), (4, 2000, 1), (5, 1000, 9), (6, 300, 8)])
+def test_bf16_matches_oracle_and_fp64(ctx, d, n, seed):
+    xs = synth.uniform(d, n, seed)
+    k, c, st = build(ctx, xs, "bf16")
+    k64, c64, _ = build(ctx, xs, "fp64")
+    b, nrm = synth.cube_planes(d)
+    ok, oc, _ = Oracle(xs, b, nrm).voronoi()
+    assert np.array_equal(rows_view(k), rows_view(ok))
+    assert np.array_equal(k, k64)
+    # same FP64 circumcentre path: identical coordinates, and within the oracle's tolerance
+    assert np.array_equal(c, c64)
+    order = np.lexsort(ok.T[::-1])
+    assert float(np.max(np.abs(c - oc[order]))) < COORD_TOL
+    # the filter passes some candidates to FP64, never more than it scanned
+    assert 0 < st["fp64_rechecks"] < st["pairs_scanned"]
+    assert st["fallbacks"] <= st["rays"]
+
+
+def test_bf16_identical_vertex_set_at_5d_100k_mixture(ctx):
+    # the benchmark workload: 5D N=100k Gaussian mixture, cuboid(5, periodic=[])
+    xs = synth.gaussian_mixture_5d()
+    k, c, st = build(ctx, xs, "bf16")
+    k64, c64, st64 = build(ctx, xs, "fp64")
+    assert st["vertices"] == st64["vertices"] > 10_000_000
+    assert key_digest(k) == key_digest(k64)
+    assert np.array_equal(k, k64)
+    assert np.array_equal(c, c64)
+
+
+def test_bf16_rejects_other_leaf_sizes(ctx):
+    xs = synth.uniform(3, 200, 1)
+    b, nrm = synth.cube_planes(3)
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    with pytest.raises(HVError) as e:
+        ctx.build_mesh(precision="bf16", leaf_size=16)
+    assert e.value.code == -7
+
+
+def test_bf16_raycast_batch_matches_fp64(ctx):
+    d, n = 5, 800
+    xs = synth.uniform(d, n, 31)
+    b, nrm = synth.cube_planes(d)
+    k64, c64, _ = build(ctx, xs, "fp64")
+    rng = np.random.default_rng(3)
+    rows = rng.permutation(len(k64))[:500]
+    sig, drop, r = [], [], []
+    for i in rows:
+        for p in range(d + 1):
+            e = np.delete(k64[i], p)
+            if e[0] > n:
+                continue
+            sig.append(k64[i]); drop.append(p); r.append(c64[i])
+    sig, drop, r = np.array(sig), np.array(drop), np.array(r)
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    a = ctx.raycast_batch(sig, drop, r, precision="fp64")
+    m = ctx.raycast_batch(sig, drop, r, precision="bf16")
+    for x, y in zip(a, m):
+        assert np.array_equal(x, y)
