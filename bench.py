@@ -6,12 +6,15 @@ uniform in [0.2,0.8]^5, sigma 0.05, samples outside the cube redrawn) plus 10% u
 background -- with the Dirichlet box cuboid(5, periodic=[]).  One step is one full
 hv_build_mesh: tree build + the whole vertex walk, points already in HBM.
 
-Prints ONE JSON line on stdout (diagnostics go to stderr).
+The headline leg is the mixed-precision scan (BF16 MFMA filter + FP64 re-verification);
+the plain FP64 scan runs beside it on the same points, and the two key tables are
+compared in full.  Prints ONE JSON line on stdout (diagnostics go to stderr).
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -19,9 +22,9 @@ sys.path.insert(0, os.path.join(ROOT, "highvoronoi.jl_amd"))
 
 import numpy as np  # noqa: E402
 
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+PEAK_TFLOPS = {"fp64": 78.6, "bf16": 2500.0}
 WORKLOAD = ("5D N=100000 Gaussian mixture (8 clusters, sigma=0.05, centres U[0.2,0.8]^5, "
-            "10% uniform background, out-of-cube samples redrawn), cuboid(5, periodic=[])")
+            "10% uniform background, out-of-cube samples redrawn, seed 2), cuboid(5, periodic=[])")
 
 
 def log(*a):
@@ -30,8 +33,7 @@ def log(*a):
 
 def run_leg(ctx, precision, steps, warmup):
     for _ in range(warmup):
-        m = ctx.build_mesh(precision=precision)
-        m.free()
+        ctx.build_mesh(precision=precision).free()
     stats = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -42,16 +44,52 @@ def run_leg(ctx, precision, steps, warmup):
     return (t1 - t0) / steps, stats
 
 
-def cpu_baseline(xs, cells):
+def key_table(ctx, precision):
+    m = ctx.build_mesh(precision=precision)
+    keys, _ = m.export()
+    m.free()
+    return keys
+
+
+def cpu_baseline(xs, cells_per_thread, threads):
+    """The oracle (C restatement of the reference walk), one context per thread over
+    contiguous cell ranges; unique vertices of the union per second of wall time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from hvgpu import synth
     from oracle_py import Oracle
     b, nrm = synth.cube_planes(xs.shape[1])
-    o = Oracle(xs, b, nrm)
+    results = [None] * threads
+    oracles = [Oracle(xs, b, nrm) for _ in range(threads)]
+
+    def work(t):
+        lo = t * cells_per_thread
+        results[t] = oracles[t].voronoi(cells=(lo, lo + cells_per_thread))[0]
+
+    ths = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
     t0 = time.perf_counter()
-    keys, _, st = o.voronoi(cells=(0, cells))
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
     t1 = time.perf_counter()
-    return len(keys) / (t1 - t0), len(keys), t1 - t0
+    allk = np.concatenate(results)
+    nuniq = len(np.unique(allk, axis=0))
+    return nuniq / (t1 - t0), nuniq, t1 - t0
+
+
+def measured_traffic(precision):
+    """HBM bytes per scan launch at this workload, from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a copy kernel of known size); None if absent."""
+    path = os.path.join(ROOT, "profiles", "k_raycast64_5d_100k_mixture_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == precision:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -59,13 +97,28 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"])
-    ap.add_argument("--cpu-cells", type=int, default=500, help="cells walked by the CPU baseline sample")
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp64"],
+                    help="headline leg (the other precision runs beside it)")
+    ap.add_argument("--cpu-cells", type=int, default=200, help="cells per thread walked by the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 leg and key-table comparison")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device copy (3x): the known-size kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters into bytes")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(64 * 1024 * 1024, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copied 268435456 bytes, 3 times")
+        return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
-              "(the box has a single MI355X; the multi-GPU sharding is a design note only)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (single-GPU build; "
+              "the multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
     from hvgpu import synth
@@ -86,33 +139,47 @@ def main():
         f"{sec * 1e3:.2f} ms/step (device {st['ms_total']:.2f} ms, scan {st['ms_raycast']:.2f} ms)")
 
     extra = {}
-    if args.precision != "fp64":
-        sec64, stats64 = run_leg(ctx, "fp64", args.steps, args.warmup)
-        extra["fp64_vertices_per_s"] = stats64[-1]["vertices"] / sec64
-        extra["speedup_vs_fp64"] = value / extra["fp64_vertices_per_s"]
-        extra["identical_vertex_count"] = stats64[-1]["vertices"] == nv
-        extra["fallback_rate"] = st["fallbacks"] / max(1, st["rays"])
-        extra["max_t_relerr"] = st["max_t_relerr"]
+    if not args.no_compare:
+        other = "fp64" if args.precision == "bf16" else "bf16"
+        sec2, stats2 = run_leg(ctx, other, args.steps, args.warmup)
+        s2 = stats2[-1]
+        log(f"[{other}] {s2['vertices']} vertices, {sec2 * 1e3:.2f} ms/step (scan {s2['ms_raycast']:.2f} ms)")
+        k1 = key_table(ctx, args.precision)
+        k2 = key_table(ctx, other)
+        same = bool(k1.shape == k2.shape and np.array_equal(k1, k2))
+        del k1, k2
+        fp64_vps = (s2["vertices"] / sec2) if other == "fp64" else value
+        mixed_vps = value if args.precision == "bf16" else s2["vertices"] / sec2
+        mixed = st if args.precision == "bf16" else s2
+        extra = {"fp64_vertices_per_s": fp64_vps, "mixed_vertices_per_s": mixed_vps,
+                 "speedup_vs_fp64": mixed_vps / fp64_vps, "identical_vertex_set": same,
+                 "fallback_rate": mixed["fallbacks"] / max(1, mixed["rays"]),
+                 "fp64_rechecks_per_ray": mixed["fp64_rechecks"] / max(1, mixed["rays"]),
+                 "max_coord_err_vs_fp64": 0.0 if same else None}
 
-    # roofline of the candidate scan: 4d flops per scanned (ray, generator) pair
+    # roofline of the candidate-scan kernel (k_raycast64): 4d algorithmic flops per scanned
+    # (ray, generator) pair; kernel time from HIP events on the context's stream
+    launches = st["levels"] - 1
     pairs = st["pairs_scanned"]
     flops = 4.0 * d * pairs
     achieved = flops / (st["ms_raycast"] * 1e-3) / 1e12 if st["ms_raycast"] > 0 else 0.0
     peak = PEAK_TFLOPS[args.precision]
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None,
-                "kernel": "k_raycast64", "launches": st["levels"],
-                "avg_launch_ms": st["ms_raycast"] / max(1, st["levels"]),
-                "flops_per_launch": flops / max(1, st["levels"]),
+                "frac": achieved / peak, "traffic": measured_traffic(args.precision),
+                "kernel": f"k_raycast64<5,{1 if args.precision == 'bf16' else 0}>", "launches": launches,
+                "avg_launch_ms": st["ms_raycast"] / max(1, launches),
+                "flops_per_launch": flops / max(1, launches),
                 "pairs_per_ray": pairs / max(1, st["rays"])}
 
     cpu = None
     if not args.no_cpu:
-        vps, ncpu, tcpu = cpu_baseline(xs, args.cpu_cells)
-        cpu = {"value": vps, "unit": "vertices/s", "cores": 1, "kind": "port",
-               "sample": f"oracle (C restatement of the reference walk) over cells 1..{args.cpu_cells} of the "
-                         f"same point set: {ncpu} vertices in {tcpu:.1f} s"}
-        log(f"[cpu] {ncpu} vertices in {tcpu:.2f} s -> {vps:.0f} vertices/s")
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        vps, ncpu, tcpu = cpu_baseline(xs, args.cpu_cells, threads)
+        cpu = {"value": vps, "unit": "vertices/s", "cores": threads, "kind": "port",
+               "sample": f"oracle (C restatement of the reference walk, FP64) over cells 1..{args.cpu_cells * threads} "
+                         f"of the same point set, {threads} threads on contiguous cell ranges: "
+                         f"{ncpu} unique vertices in {tcpu:.1f} s"}
+        log(f"[cpu] {ncpu} vertices in {tcpu:.2f} s on {threads} threads -> {vps:.0f} vertices/s")
 
     out = {
         "metric": "Voronoi vertices/s at 5D N=100k: mixed-precision vs FP64 GPU, identical vertex set",
@@ -125,7 +192,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else f"{args.precision}+f64",
+        "dtype": "bf16+f64" if args.precision == "bf16" else "f64",
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "n": int(xs.shape[0]), "d": d, "precision": args.precision,
                    "vertices": nv, "rays": st["rays"], "levels": st["levels"],
