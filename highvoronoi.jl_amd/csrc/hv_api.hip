@@ -223,4 +223,34 @@ int hv_inrange_batch(hv_ctx* c, int64_t nq, const double* q, const double* radiu
     HV_DISPATCH(c->D, hv_inrange_impl, c, nq, q, radius, ptr, idx)
 }
 
+int hv_integrate_polytope(hv_ctx* c, const hv_mesh* mesh, hv_integral** out) {
+    if (!c || !mesh || !out) return hv_fail(c, HV_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    (void)hipSetDevice(c->device);
+    HV_DISPATCH(c->D, hv_integrate_polytope_impl, c, mesh, out)
+}
+
+int hv_integrate_mc(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t seed, hv_integral** out) {
+    if (!c || !mesh || !out) return hv_fail(c, HV_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    (void)hipSetDevice(c->device);
+    HV_DISPATCH(c->D, hv_integrate_mc_impl, c, mesh, ndir, seed, out)
+}
+
+int hv_integral_counts(const hv_integral* I, int64_t* ncells, int64_t* nnbr) {
+    return hv_integral_counts_impl(I, ncells, nnbr);
+}
+
+int hv_integral_export(const hv_integral* I, double* vol, int64_t* nbr_ptr, int64_t* nbr, double* area, double* ms) {
+    return hv_integral_export_impl(I, vol, nbr_ptr, nbr, area, ms);
+}
+
+int hv_integral_free(hv_integral* I) {
+    if (!I) return HV_ERR_ARG;
+    hv_integral_free_impl(I);
+    return HV_OK;
+}
+
 }  // extern "C"

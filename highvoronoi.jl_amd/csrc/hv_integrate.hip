@@ -1,0 +1,825 @@
+// hv_integrate.hip -- gfx950 kernels and host driver of cell volumes and interface areas.
+//
+// Exact polytope integrator (the reference's src/polyintegrator.jl:80-468 computes, cell by
+// cell, each interface F_ik by a recursive cone decomposition and vol_i = sum_k A_ik h_ik/d,
+// :296-312).  Here the same decomposition runs over all faces of the tessellation at once,
+// one face dimension per pass:
+//   * in general position every subset S of a vertex key that holds a generator is a
+//     face; faces with |S| = m have dimension k = d + 1 - m and are identified by their
+//     generator set, packed into 128 bits (21 bits per id);
+//   * pass m: emit the (S, vertex) incidences of every vertex, radix-sort by S, run heads
+//     give the faces and their vertex lists;
+//   * edges (m = d) measure |v_a - v_b|; a face of dimension k > 1 is the union of cones
+//     from one of its vertices p over its (k-1)-faces T = S + {j} that miss p:
+//     |F_S| = (1/k) sum_T h(p, aff F_T) |F_T|, h measured inside aff F_S;
+//   * facets (m = 2) are the interfaces A_ik; volumes take the generator as apex.
+// One thread per face reads its children in a fixed order: no atomics, the result does not
+// depend on scheduling.
+//
+// Monte-Carlo integrator (src/mcintegrator.jl:98-204): one workgroup per cell casts ndir
+// rays from the generator (mc_raycast, :207-226) against the cell's neighbours (boundary
+// planes through their mirror image of the generator), accumulates t^d and the facet
+// hits, then the neighbour estimates are symmetrised (:176-200).  Each cell draws its own
+// directions (counter-based splitmix64 + Box-Muller); the reference reuses one direction
+// for every sample (the draw loop at :85-93 never re-enters), which is not carried over.
+#include "hv_device.h"
+#include "hv_internal.h"
+#include <hipcub/hipcub.hpp>
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#define HVCK(call)                                                                 \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return hv_fail(c, HV_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+template <typename T>
+static T* iget(hv_ctx* c, const char* name, size_t count) {
+    return (T*)c->pool->get(name, std::max<size_t>(1, count) * sizeof(T), false, c->stream);
+}
+
+constexpr uint64_t HV_F21 = 0x1FFFFFull;   // id field; all ones = empty slot
+
+__device__ __forceinline__ void hv_fkey_put(uint64_t& lo, uint64_t& hi, int s, uint64_t id) {
+    if (s < 3) lo = (lo & ~(HV_F21 << (21 * s))) | (id << (21 * s));
+    else hi = (hi & ~(HV_F21 << (21 * (s - 3)))) | (id << (21 * (s - 3)));
+}
+__device__ __forceinline__ int hv_fkey_get(uint64_t lo, uint64_t hi, int s) {
+    return (int)(s < 3 ? (lo >> (21 * s)) & HV_F21 : (hi >> (21 * (s - 3))) & HV_F21);
+}
+// (hi, lo) lexicographic: the order the two-pass radix sort leaves
+__device__ __forceinline__ bool hv_fkey_less(uint64_t ah, uint64_t al, uint64_t bh, uint64_t bl) {
+    return ah < bh || (ah == bh && al < bl);
+}
+
+// dense copy of the live vertex records
+template <int D>
+__global__ void k_int_gather(const int* __restrict__ ids, int nv, const int* __restrict__ key,
+                             const double* __restrict__ x, int* __restrict__ vk, double* __restrict__ vc) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const int s = ids[v];
+#pragma unroll
+    for (int j = 0; j <= D; ++j) vk[(int64_t)v * (D + 1) + j] = key[(int64_t)s * (D + 1) + j];
+#pragma unroll
+    for (int k = 0; k < D; ++k) vc[(int64_t)v * D + k] = x[(int64_t)s * D + k];
+}
+
+// packed key words of each vertex (21 bits per id): word 0 holds ids 0..2, word 1 ids 3..5,
+// word 2 id 6; sorting by (w0, w1, w2) gives a numbering fixed by the vertex set alone
+template <int D>
+__global__ void k_vkey_words(const int* __restrict__ vk, int nv, uint64_t* __restrict__ w0, uint64_t* __restrict__ w1,
+                             uint64_t* __restrict__ w2, int* __restrict__ idx) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    uint64_t a = 0, b = 0, c = 0;
+#pragma unroll
+    for (int j = 0; j <= D; ++j) {
+        const uint64_t id = (uint64_t)vk[(int64_t)v * (D + 1) + j];
+        if (j < 3) a |= id << (21 * (2 - j));
+        else if (j < 6) b |= id << (21 * (5 - j));
+        else c = id;
+    }
+    w0[v] = a;
+    w1[v] = b;
+    w2[v] = c;
+    idx[v] = v;
+}
+
+template <int D>
+__global__ void k_int_permute(const int* __restrict__ perm, int nv, const int* __restrict__ vk, const double* __restrict__ vc,
+                              int* __restrict__ vk2, double* __restrict__ vc2) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const int s = perm[v];
+#pragma unroll
+    for (int j = 0; j <= D; ++j) vk2[(int64_t)v * (D + 1) + j] = vk[(int64_t)s * (D + 1) + j];
+#pragma unroll
+    for (int k = 0; k < D; ++k) vc2[(int64_t)v * D + k] = vc[(int64_t)s * D + k];
+}
+
+// (face, vertex) incidences of pass m: the m-subsets of each key holding a generator
+template <int D>
+__global__ void k_face_emit(const int* __restrict__ vk, int nv, int m, int N, int C, uint64_t* __restrict__ lo,
+                            uint64_t* __restrict__ hi, int* __restrict__ idx, unsigned long long* __restrict__ nsent) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const int* k = vk + (int64_t)v * (D + 1);
+    int c = 0, sent = 0;
+    for (int mask = 0; mask < (1 << (D + 1)); ++mask) {
+        if (__popc(mask) != m) continue;
+        uint64_t l = ~0ull, h = ~0ull;
+        int s = 0;
+        bool real = false;
+#pragma unroll
+        for (int j = 0; j <= D; ++j) {
+            if ((mask >> j) & 1) {
+                const int id = k[j];
+                real = real || id < N;
+                hv_fkey_put(l, h, s, (uint64_t)id);
+                ++s;
+            }
+        }
+        if (!real) { l = ~0ull; h = ~0ull; ++sent; }
+        const int64_t o = (int64_t)v * C + c;
+        lo[o] = l;
+        hi[o] = h;
+        idx[o] = (int)o;
+        ++c;
+    }
+    if (sent) atomicAdd(nsent, (unsigned long long)sent);
+}
+
+__global__ void k_gather_u64(const uint64_t* __restrict__ src, const int* __restrict__ perm, int64_t n,
+                             uint64_t* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
+// sorted incidences -> run heads; incidence o belongs to vertex o / C
+__global__ void k_face_heads(const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi,
+                             const int* __restrict__ perm, int64_t n, int C, int* __restrict__ vid,
+                             uint8_t* __restrict__ head) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    vid[i] = perm[i] / C;
+    head[i] = (i == 0 || lo[i] != lo[i - 1] || hi[i] != hi[i - 1]) ? 1 : 0;
+}
+
+__global__ void k_iota64(int* __restrict__ a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (int)i;
+}
+
+// face table of a pass: key, first incidence, apex vertex
+__global__ void k_face_table(const int* __restrict__ fstart, int nface, const uint64_t* __restrict__ lo,
+                             const uint64_t* __restrict__ hi, uint64_t* __restrict__ flo, uint64_t* __restrict__ fhi) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    flo[f] = lo[fstart[f]];
+    fhi[f] = hi[fstart[f]];
+}
+
+template <int D>
+__global__ void k_edge_len(const int* __restrict__ fstart, int nface, const int* __restrict__ vid,
+                           const double* __restrict__ vc, double* __restrict__ meas, int* __restrict__ bad) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    const int b = fstart[f], e = fstart[f + 1];
+    if (e - b != 2) { atomicOr(bad, 1); meas[f] = 0.0; return; }
+    const int va = vid[b], vb = vid[b + 1];
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double t = vc[(int64_t)va * D + k] - vc[(int64_t)vb * D + k];
+        s = fma(t, t, s);
+    }
+    meas[f] = sqrt(s);
+}
+
+struct IntGeo {
+    int N, P;
+    const double* xo;   // N x D, original order
+    const double* pn;   // P x D outer normals
+    const double* ph;   // P offsets <n_p, b_p>
+};
+
+// constraint normal of member b of a face whose first generator is a
+template <int D>
+__device__ __forceinline__ void hv_face_grad(const IntGeo& g, int a, int b, double* w) {
+    if (b >= g.N) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) w[k] = g.pn[(b - g.N) * D + k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < D; ++k) w[k] = g.xo[(int64_t)b * D + k] - g.xo[(int64_t)a * D + k];
+    }
+}
+
+// Measure of each face of pass m from the faces of pass m + 1 (one thread per face).
+template <int D>
+__global__ void k_face_measure(IntGeo g, int m, int nface, const int* __restrict__ fstart, const int* __restrict__ vid,
+                               const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi, int nface1,
+                               const int* __restrict__ fstart1, const int* __restrict__ vid1,
+                               const uint64_t* __restrict__ flo1, const uint64_t* __restrict__ fhi1,
+                               const double* __restrict__ meas1, const int* __restrict__ vk,
+                               const double* __restrict__ vc, double* __restrict__ meas, int* __restrict__ bad) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    const uint64_t lo = flo[f], hi = fhi[f];
+    int S[D + 1];
+#pragma unroll
+    for (int s = 0; s <= D; ++s) S[s] = s < m ? hv_fkey_get(lo, hi, s) : -1;
+    const int a = S[0];   // ids sorted: generators before planes, the face holds one
+    // orthonormal basis of the face's constraint normals (two Gram-Schmidt passes)
+    double Q[D][D];
+    int q = 0;
+#pragma unroll
+    for (int s = 1; s < D + 1; ++s) {
+        if (s >= m) break;
+        double w[D];
+        hv_face_grad<D>(g, a, S[s], w);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int r = 0; r < q; ++r) {
+                double cdot = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) cdot = fma(w[k], Q[r][k], cdot);
+#pragma unroll
+                for (int k = 0; k < D; ++k) w[k] = fma(-cdot, Q[r][k], w[k]);
+            }
+        double nw = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) nw = fma(w[k], w[k], nw);
+        nw = sqrt(nw);
+        if (!(nw > 0.0)) { atomicOr(bad, 2); meas[f] = 0.0; return; }
+#pragma unroll
+        for (int k = 0; k < D; ++k) Q[q][k] = w[k] / nw;
+        ++q;
+    }
+    const int b0 = fstart[f], e0 = fstart[f + 1];
+    const int apex = vid[b0];
+    const int* kp = vk + (int64_t)apex * (D + 1);
+    double p[D], da = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        p[k] = vc[(int64_t)apex * D + k];
+        const double t = p[k] - g.xo[(int64_t)a * D + k];
+        da = fma(t, t, da);
+    }
+    double sum = 0.0;
+    for (int i = b0; i < e0; ++i) {
+        const int v = vid[i];
+        const int* kv = vk + (int64_t)v * (D + 1);
+        for (int e = 0; e <= D; ++e) {
+            const int j = kv[e];
+            bool skip = false;
+#pragma unroll
+            for (int s = 0; s <= D; ++s) skip = skip || (S[s] == j) || (kp[s] == j);
+            if (skip) continue;   // j in S, or the sub-face holds the apex (zero height)
+            // T = S + {j}, sorted
+            uint64_t tl = ~0ull, th = ~0ull;
+            int s2 = 0;
+            bool put = false;
+            for (int s = 0; s < m; ++s) {
+                if (!put && j < S[s]) { hv_fkey_put(tl, th, s2++, (uint64_t)j); put = true; }
+                hv_fkey_put(tl, th, s2++, (uint64_t)S[s]);
+            }
+            if (!put) hv_fkey_put(tl, th, s2++, (uint64_t)j);
+            // lower bound in the pass-(m+1) face table
+            int lo_i = 0, hi_i = nface1;
+            while (lo_i < hi_i) {
+                const int mid = (lo_i + hi_i) >> 1;
+                if (hv_fkey_less(fhi1[mid], flo1[mid], th, tl)) lo_i = mid + 1;
+                else hi_i = mid;
+            }
+            if (lo_i >= nface1 || flo1[lo_i] != tl || fhi1[lo_i] != th) { atomicOr(bad, 4); continue; }
+            // each sub-face once: from the first of its vertices in this face's list
+            if (vid1[fstart1[lo_i]] != v) continue;
+            double w[D];
+            hv_face_grad<D>(g, a, j, w);
+            for (int pass = 0; pass < 2; ++pass)
+                for (int r = 0; r < q; ++r) {
+                    double cdot = 0.0;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) cdot = fma(w[k], Q[r][k], cdot);
+#pragma unroll
+                    for (int k = 0; k < D; ++k) w[k] = fma(-cdot, Q[r][k], w[k]);
+                }
+            double nr = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) nr = fma(w[k], w[k], nr);
+            nr = sqrt(nr);
+            double h;
+            if (j >= g.N) {
+                double np_ = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) np_ = fma(g.pn[(j - g.N) * D + k], p[k], np_);
+                h = fabs(g.ph[j - g.N] - np_) / nr;
+            } else {
+                double dj = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const double t = p[k] - g.xo[(int64_t)j * D + k];
+                    dj = fma(t, t, dj);
+                }
+                h = fabs(dj - da) / (2.0 * nr);
+            }
+            sum = fma(h, meas1[lo_i], sum);
+        }
+    }
+    meas[f] = sum / (double)(D + 1 - m);
+}
+
+// facets {a, b} -> (cell, neighbour, area) rows for both sides
+__global__ void k_facet_pairs(int nface, const uint64_t* __restrict__ flo, const double* __restrict__ meas, int N,
+                              uint64_t* __restrict__ pkey, double* __restrict__ parea, unsigned long long* __restrict__ np) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    const int a = hv_fkey_get(flo[f], 0, 0), b = hv_fkey_get(flo[f], 0, 1);
+    const int n = b < N ? 2 : 1;
+    const unsigned long long o = atomicAdd(np, (unsigned long long)n);
+    pkey[o] = ((uint64_t)a << 32) | (uint64_t)b;
+    parea[o] = meas[f];
+    if (n == 2) {
+        pkey[o + 1] = ((uint64_t)b << 32) | (uint64_t)a;
+        parea[o + 1] = meas[f];
+    }
+}
+
+template <int D>
+__global__ void k_cell_volumes(IntGeo g, const uint64_t* __restrict__ pkey, const double* __restrict__ parea,
+                               int64_t npair, double* __restrict__ vol, int64_t* __restrict__ ptr) {
+    // one thread per cell: its rows are contiguous after the sort; binary search the range
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > g.N) return;
+    int64_t lo = 0, hi = npair;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int)(pkey[mid] >> 32) < i) lo = mid + 1;
+        else hi = mid;
+    }
+    ptr[i] = lo;
+    if (i == g.N) return;
+    double V = 0.0;
+    for (int64_t t = lo; t < npair && (int)(pkey[t] >> 32) == i; ++t) {
+        const int k = (int)(pkey[t] & 0xffffffffull);
+        double h;
+        if (k >= g.N) {
+            double nx = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) nx = fma(g.pn[(k - g.N) * D + c], g.xo[(int64_t)i * D + c], nx);
+            h = g.ph[k - g.N] - nx;
+        } else {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const double t2 = g.xo[(int64_t)i * D + c] - g.xo[(int64_t)k * D + c];
+                s = fma(t2, t2, s);
+            }
+            h = 0.5 * sqrt(s);
+        }
+        V = fma(parea[t], h / (double)D, V);
+    }
+    vol[i] = V;
+}
+
+// ---- Monte-Carlo ------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t hv_sm64(uint64_t seed, uint64_t s) {
+    uint64_t z = seed + (s + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double hv_unif(uint64_t seed, uint64_t s) {
+    return (double)(hv_sm64(seed, s) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+constexpr int HV_MC_MAXK = 512;
+
+template <int D>
+__global__ __launch_bounds__(256) void k_mc_cell(IntGeo g, const int64_t* __restrict__ ptr,
+                                                 const uint64_t* __restrict__ pkey, int64_t ndir, uint64_t seed,
+                                                 double cvol, double carea, double* __restrict__ vol,
+                                                 double* __restrict__ area) {
+    __shared__ double ext[HV_MC_MAXK][D], nrm[HV_MC_MAXK][D];
+    __shared__ double acc[HV_MC_MAXK];
+    __shared__ double vred[256];
+    const int i = blockIdx.x;
+    const int64_t b = ptr[i];
+    const int K = (int)(ptr[i + 1] - b);
+    double x[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) x[c] = g.xo[(int64_t)i * D + c];
+    for (int t = threadIdx.x; t < K; t += blockDim.x) {
+        const int k = (int)(pkey[b + t] & 0xffffffffull);
+        double e[D];
+        if (k >= g.N) {
+            // mirror image of the generator in the boundary plane (src/boundary.jl:206-211)
+            double nx = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) nx = fma(g.pn[(k - g.N) * D + c], x[c], nx);
+            const double h = g.ph[k - g.N] - nx;
+#pragma unroll
+            for (int c = 0; c < D; ++c) e[c] = x[c] + 2.0 * h * g.pn[(k - g.N) * D + c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < D; ++c) e[c] = g.xo[(int64_t)k * D + c];
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) s = fma(e[c] - x[c], e[c] - x[c], s);
+        s = sqrt(s);
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            ext[t][c] = e[c];
+            nrm[t][c] = (e[c] - x[c]) / s;
+        }
+        acc[t] = 0.0;
+    }
+    __syncthreads();
+    constexpr int DP = (D + 1) & ~1;
+    double V = 0.0;
+    for (int64_t q = threadIdx.x; q < ndir; q += blockDim.x) {
+        const uint64_t base = ((uint64_t)i * (uint64_t)ndir + (uint64_t)q) * (uint64_t)DP;
+        double u[D], uu = 0.0;
+#pragma unroll
+        for (int c = 0; c < D; c += 2) {
+            const uint64_t p = (base + c) >> 1;
+            const double u1 = fmax(hv_unif(seed, 2 * p), 1e-300), u2 = hv_unif(seed, 2 * p + 1);
+            const double rad = sqrt(-2.0 * log(u1));
+            double sn, cs;
+            sincos(2.0 * M_PI * u2, &sn, &cs);
+            u[c] = rad * cs;
+            if (c + 1 < D) u[c + 1] = rad * sn;
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) uu = fma(u[c], u[c], uu);
+        uu = sqrt(uu);
+        double c0 = 0.0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            u[c] /= uu;
+            c0 = fma(x[c], u[c], c0);
+        }
+        double ts = INFINITY;
+        int win = -1;
+        for (int t = 0; t < K; ++t) {
+            double eu = 0.0, num = 0.0, den = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const double e = ext[t][c];
+                eu = fma(e, u[c], eu);
+                num = fma(x[c] - e, x[c] - e, num);
+                den = fma(u[c], e - x[c], den);
+            }
+            if (eu <= c0) continue;
+            const double tt = num / (2.0 * den);
+            if (tt > 0.0 && tt < ts) { ts = tt; win = t; }
+        }
+        double tp = 1.0;
+#pragma unroll
+        for (int c = 0; c < D - 1; ++c) tp *= ts;
+        V += tp * ts;
+        if (win >= 0 && ts < INFINITY) {
+            double cn = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) cn = fma(nrm[win][c], u[c], cn);
+            atomicAdd(&acc[win], tp / fabs(cn));
+        }
+    }
+    vred[threadIdx.x] = V;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) vred[threadIdx.x] += vred[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) vol[i] = vred[0] * cvol / (double)ndir;
+    for (int t = threadIdx.x; t < K; t += blockDim.x) area[b + t] = acc[t] * carea / (double)ndir;
+}
+
+// symmetrise each generator-generator interface with the neighbour's estimate (pairs k < i)
+template <int D>
+__global__ void k_mc_sym(IntGeo g, const int64_t* __restrict__ ptr, const uint64_t* __restrict__ pkey, int64_t npair,
+                         double* __restrict__ vol, double* __restrict__ area) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npair) return;
+    const int i = (int)(pkey[t] >> 32), k = (int)(pkey[t] & 0xffffffffull);
+    if (k >= g.N || k >= i) return;
+    int64_t lo = ptr[k], hi = ptr[k + 1];
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int)(pkey[mid] & 0xffffffffull) < i) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo >= ptr[k + 1] || (int)(pkey[lo] & 0xffffffffull) != i) return;
+    const double old = area[t], neigh = area[lo];
+    const double nw = fabs(neigh) < old * 1.0e-10 ? old : 0.5 * (old + neigh);
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        const double d2 = g.xo[(int64_t)i * D + c] - g.xo[(int64_t)k * D + c];
+        s = fma(d2, d2, s);
+    }
+    const double factor = 0.5 * sqrt(s) / (double)D;
+    atomicAdd(&vol[k], (nw - neigh) * factor);
+    atomicAdd(&vol[i], (nw - old) * factor);
+    area[t] = nw;
+    area[lo] = nw;
+}
+
+// ---- host driver -------------------------------------------------------------------------
+
+struct hv_integral {
+    int64_t N = 0;
+    std::vector<double> vol, area;
+    std::vector<int64_t> ptr, nbr;
+    double ms = 0.0;
+};
+
+static int64_t binom(int n, int k) {
+    int64_t r = 1;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+
+// faces of the tessellation, all passes; leaves the sorted (cell, neighbour) rows with areas
+template <int D>
+static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey, double*& parea, int64_t& npair) {
+    const int N = (int)c->N;
+    const int nv = (int)mesh->nv;
+    if ((int64_t)N + c->P >= (int64_t)HV_F21) return hv_fail(c, HV_ERR_UNSUPPORTED, "integration packs ids in 21 bits");
+    int* vk = iget<int>(c, "i_vk", (size_t)nv * (D + 1));
+    double* vc = iget<double>(c, "i_vc", (size_t)nv * D);
+    int* bad = iget<int>(c, "i_bad", 1);
+    unsigned long long* cnt = iget<unsigned long long>(c, "i_cnt", 2);
+    if (!vk || !vc || !bad || !cnt) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integration)");
+    HVCK(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+    {
+        // the live vertices in a numbering fixed by the vertex set (not by the walk's
+        // insertion order), so that apexes and summation orders are reproducible
+        int* vk0 = iget<int>(c, "i_vk0", (size_t)nv * (D + 1));
+        double* vc0 = iget<double>(c, "i_vc0", (size_t)nv * D);
+        uint64_t* w0 = iget<uint64_t>(c, "i_w0", nv);
+        uint64_t* w1 = iget<uint64_t>(c, "i_w1", nv);
+        uint64_t* w2 = iget<uint64_t>(c, "i_w2", nv);
+        uint64_t* ws = iget<uint64_t>(c, "i_ws", nv);
+        uint64_t* wso = iget<uint64_t>(c, "i_wso", nv);
+        int* pa = iget<int>(c, "i_vpa", nv);
+        int* pb = iget<int>(c, "i_vpb", nv);
+        if (!vk0 || !vc0 || !w0 || !w1 || !w2 || !ws || !wso || !pa || !pb)
+            return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integration)");
+        k_int_gather<D><<<nblk(nv, 256), 256, 0, c->stream>>>((const int*)mesh->ids.p, nv, (const int*)mesh->key.p,
+                                                              (const double*)mesh->x.p, vk0, vc0);
+        HVCK(hipGetLastError());
+        k_vkey_words<D><<<nblk(nv, 256), 256, 0, c->stream>>>(vk0, nv, w0, w1, w2, pa);
+        HVCK(hipGetLastError());
+        size_t tb = 0;
+        HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, ws, wso, pa, pb, nv, 0, 64, c->stream));
+        void* tmp = c->pool->get("i_vtmp", tb, false, c->stream);
+        if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
+        // least significant word first; each later pass is stable
+        const uint64_t* words[3] = {w2, w1, w0};
+        int* src = pa;
+        int* dst = pb;
+        for (int wi = (D >= 6 ? 0 : 1); wi < 3; ++wi) {
+            k_gather_u64<<<nblk(nv, 256), 256, 0, c->stream>>>(words[wi], src, nv, ws);
+            HVCK(hipGetLastError());
+            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, ws, wso, src, dst, nv, 0, 64, c->stream));
+            std::swap(src, dst);
+        }
+        k_int_permute<D><<<nblk(nv, 256), 256, 0, c->stream>>>(src, nv, vk0, vc0, vk, vc);
+        HVCK(hipGetLastError());
+    }
+    // per pass: sorted incidences (vid), face starts, keys, measures; "1" = pass m + 1
+    int nface1 = 0;
+    for (int m = D; m >= 2; --m) {
+        const int C = (int)binom(D + 1, m);
+        const int64_t tot = (int64_t)nv * C;
+        if (tot >= (1ll << 31)) return hv_fail(c, HV_ERR_CAPACITY, "too many face incidences for one pass");
+        const char* sfx = (m & 1) ? "a" : "b";
+        auto nm = [&](const char* s) { return std::string(s) + sfx; };
+        uint64_t* lo = iget<uint64_t>(c, "i_lo", tot);
+        uint64_t* hi = iget<uint64_t>(c, "i_hi", tot);
+        uint64_t* lo2 = iget<uint64_t>(c, "i_lo2", tot);
+        uint64_t* hi2 = iget<uint64_t>(c, "i_hi2", tot);
+        int* pa = iget<int>(c, "i_pa", tot);
+        int* pb = iget<int>(c, "i_pb", tot);
+        uint8_t* head = iget<uint8_t>(c, "i_head", tot);
+        if (!lo || !hi || !lo2 || !hi2 || !pa || !pb || !head) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
+        HVCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+        k_face_emit<D><<<nblk(nv, 256), 256, 0, c->stream>>>(vk, nv, m, N, C, lo, hi, pa, cnt);
+        HVCK(hipGetLastError());
+        // two stable radix passes: by lo, then by hi -> (hi, lo) order
+        size_t tb = 0;
+        HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, lo, lo2, pa, pb, (int)tot, 0, 64, c->stream));
+        void* tmp = c->pool->get("i_tmp", tb, false, c->stream);
+        if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
+        HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, lo, lo2, pa, pb, (int)tot, 0, 64, c->stream));
+        int* perm = pb;
+        if (m > 3) {
+            k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(hi, pb, tot, hi2);
+            HVCK(hipGetLastError());
+            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, hi2, hi, pb, pa, (int)tot, 0, 64, c->stream));
+            perm = pa;
+            k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(lo, perm, tot, lo2);
+            HVCK(hipGetLastError());
+        } else {
+            k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(hi, perm, tot, hi2);
+            HVCK(hipGetLastError());
+            HVCK(hipMemcpyAsync(hi, hi2, sizeof(uint64_t) * tot, hipMemcpyDeviceToDevice, c->stream));
+        }
+        // lo2 / hi: sorted keys; perm: sorted incidence numbers
+        unsigned long long hsent = 0;
+        HVCK(hipMemcpyAsync(&hsent, cnt, sizeof(hsent), hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        const int64_t nvalid = tot - (int64_t)hsent;
+        int* vid = iget<int>(c, nm("i_vid").c_str(), nvalid);
+        int* fstart = iget<int>(c, nm("i_fst").c_str(), nvalid + 1);
+        int* iota = iget<int>(c, "i_iota", nvalid);
+        if (!vid || !fstart || !iota) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
+        k_face_heads<<<nblk(nvalid, 256), 256, 0, c->stream>>>(lo2, hi, perm, nvalid, C, vid, head);
+        HVCK(hipGetLastError());
+        k_iota64<<<nblk(nvalid, 256), 256, 0, c->stream>>>(iota, nvalid);
+        HVCK(hipGetLastError());
+        size_t tb2 = 0;
+        HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb2, iota, head, fstart, cnt + 1, (int)nvalid, c->stream));
+        void* tmp2 = c->pool->get("i_tmp2", tb2, false, c->stream);
+        if (!tmp2) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
+        HVCK(hipcub::DeviceSelect::Flagged(tmp2, tb2, iota, head, fstart, cnt + 1, (int)nvalid, c->stream));
+        unsigned long long hnf = 0;
+        HVCK(hipMemcpyAsync(&hnf, cnt + 1, sizeof(hnf), hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        const int nface = (int)hnf;
+        const int endv = (int)nvalid;
+        HVCK(hipMemcpyAsync(fstart + nface, &endv, sizeof(int), hipMemcpyHostToDevice, c->stream));
+        uint64_t* flo = iget<uint64_t>(c, nm("i_flo").c_str(), nface);
+        uint64_t* fhi = iget<uint64_t>(c, nm("i_fhi").c_str(), nface);
+        double* meas = iget<double>(c, nm("i_meas").c_str(), nface);
+        if (!flo || !fhi || !meas) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
+        k_face_table<<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, lo2, hi, flo, fhi);
+        HVCK(hipGetLastError());
+        if (m == D) {
+            k_edge_len<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vc, meas, bad);
+        } else {
+            const char* s1 = (m & 1) ? "b" : "a";
+            auto n1 = [&](const char* s) { return std::string(s) + s1; };
+            const int* fstart1 = (const int*)c->pool->get(n1("i_fst"), 0);
+            const int* vid1 = (const int*)c->pool->get(n1("i_vid"), 0);
+            const uint64_t* flo1 = (const uint64_t*)c->pool->get(n1("i_flo"), 0);
+            const uint64_t* fhi1 = (const uint64_t*)c->pool->get(n1("i_fhi"), 0);
+            const double* meas1 = (const double*)c->pool->get(n1("i_meas"), 0);
+            k_face_measure<D><<<nblk(nface, 128), 128, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi, nface1,
+                                                                       fstart1, vid1, flo1, fhi1, meas1, vk, vc, meas,
+                                                                       bad);
+        }
+        HVCK(hipGetLastError());
+        nface1 = nface;
+        if (m == 2) {
+            pkey = iget<uint64_t>(c, "i_pkey", (size_t)2 * nface);
+            uint64_t* pkey2 = iget<uint64_t>(c, "i_pkey2", (size_t)2 * nface);
+            parea = iget<double>(c, "i_parea", (size_t)2 * nface);
+            double* parea2 = iget<double>(c, "i_parea2", (size_t)2 * nface);
+            if (!pkey || !pkey2 || !parea || !parea2) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (facets)");
+            HVCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+            k_facet_pairs<<<nblk(nface, 256), 256, 0, c->stream>>>(nface, flo, meas, N, pkey2, parea2, cnt);
+            HVCK(hipGetLastError());
+            unsigned long long hp = 0;
+            HVCK(hipMemcpyAsync(&hp, cnt, sizeof(hp), hipMemcpyDeviceToHost, c->stream));
+            HVCK(hipStreamSynchronize(c->stream));
+            npair = (int64_t)hp;
+            size_t tb3 = 0;
+            HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb3, pkey2, pkey, parea2, parea, (int)npair, 0, 64, c->stream));
+            void* tmp3 = c->pool->get("i_tmp3", tb3, false, c->stream);
+            if (!tmp3) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
+            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp3, tb3, pkey2, pkey, parea2, parea, (int)npair, 0, 64, c->stream));
+        }
+    }
+    int hbad = 0;
+    HVCK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    if (hbad) return hv_fail(c, HV_ERR_DEGENERATE, "a face is not in general position (integration)");
+    return HV_OK;
+}
+
+template <int D>
+static int make_geo(hv_ctx* c, const hv_mesh* mesh, IntGeo& g) {
+    if (!mesh || mesh->D != D || mesh->N != c->N) return hv_fail(c, HV_ERR_STATE, "the mesh was not built from this context's points");
+    g.N = (int)c->N;
+    g.P = c->P;
+    g.xo = (const double*)c->pool->get("xin", 0);
+    double* pn = iget<double>(c, "i_pn", (size_t)std::max(1, c->P) * D);
+    double* ph = iget<double>(c, "i_ph", (size_t)std::max(1, c->P));
+    if (!g.xo || !pn || !ph) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integration)");
+    std::vector<double> hph(std::max(1, c->P), 0.0);
+    for (int p = 0; p < c->P; ++p)
+        for (int k = 0; k < D; ++k) hph[p] += c->pnormal[p * D + k] * c->pbase[p * D + k];
+    if (c->P) {
+        HVCK(hipMemcpyAsync(pn, c->pnormal.data(), sizeof(double) * c->P * D, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipMemcpyAsync(ph, hph.data(), sizeof(double) * c->P, hipMemcpyHostToDevice, c->stream));
+    }
+    g.pn = pn;
+    g.ph = ph;
+    return HV_OK;
+}
+
+template <int D>
+static int finish(hv_ctx* c, const uint64_t* pkey, const double* area_d, const double* vol_d, const int64_t* ptr_d,
+                  int64_t npair, float ms, hv_integral** out) {
+    hv_integral* I = new hv_integral();
+    I->N = c->N;
+    I->vol.resize(c->N);
+    I->ptr.resize(c->N + 1);
+    I->area.resize(npair);
+    I->nbr.resize(npair);
+    std::vector<uint64_t> hk(npair);
+    hipError_t e = hipSuccess;
+    if (npair) {
+        e = hipMemcpyAsync(hk.data(), pkey, sizeof(uint64_t) * npair, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(I->area.data(), area_d, sizeof(double) * npair, hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(I->vol.data(), vol_d, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(I->ptr.data(), ptr_d, sizeof(int64_t) * (c->N + 1), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        delete I;
+        return hv_fail(c, HV_ERR_DEVICE, std::string("integration copy-out: ") + hipGetErrorString(e));
+    }
+    for (int64_t t = 0; t < npair; ++t) I->nbr[t] = (int64_t)(hk[t] & 0xffffffffull) + 1;
+    I->ms = ms;
+    *out = I;
+    return HV_OK;
+}
+
+template <int D>
+int hv_integrate_polytope_impl(hv_ctx* c, const hv_mesh* mesh, hv_integral** out) {
+    IntGeo g;
+    int rc = make_geo<D>(c, mesh, g);
+    if (rc) return rc;
+    HVCK(hipEventRecord(c->ev[0], c->stream));
+    uint64_t* pkey = nullptr;
+    double* parea = nullptr;
+    int64_t npair = 0;
+    rc = facet_rows<D>(c, mesh, g, pkey, parea, npair);
+    if (rc) return rc;
+    double* vol = iget<double>(c, "i_vol", c->N);
+    int64_t* ptr = iget<int64_t>(c, "i_ptr", c->N + 1);
+    if (!vol || !ptr) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (volumes)");
+    k_cell_volumes<D><<<nblk(c->N + 1, 256), 256, 0, c->stream>>>(g, pkey, parea, npair, vol, ptr);
+    HVCK(hipGetLastError());
+    HVCK(hipEventRecord(c->ev[1], c->stream));
+    HVCK(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HVCK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    return finish<D>(c, pkey, parea, vol, ptr, npair, ms, out);
+}
+
+template <int D>
+int hv_integrate_mc_impl(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t seed, hv_integral** out) {
+    if (ndir < 1 || ndir > (1ll << 40)) return hv_fail(c, HV_ERR_ARG, "ndir must be positive");
+    IntGeo g;
+    int rc = make_geo<D>(c, mesh, g);
+    if (rc) return rc;
+    HVCK(hipEventRecord(c->ev[0], c->stream));
+    uint64_t* pkey = nullptr;
+    double* parea = nullptr;
+    int64_t npair = 0;
+    rc = facet_rows<D>(c, mesh, g, pkey, parea, npair);
+    if (rc) return rc;
+    double* vol = iget<double>(c, "i_vol", c->N);
+    int64_t* ptr = iget<int64_t>(c, "i_ptr", c->N + 1);
+    double* area = iget<double>(c, "i_marea", npair);
+    if (!vol || !ptr || !area) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (volumes)");
+    // neighbour ranges (the exact volumes computed on the way are overwritten)
+    k_cell_volumes<D><<<nblk(c->N + 1, 256), 256, 0, c->stream>>>(g, pkey, parea, npair, vol, ptr);
+    HVCK(hipGetLastError());
+    std::vector<int64_t> hptr(c->N + 1);
+    HVCK(hipMemcpyAsync(hptr.data(), ptr, sizeof(int64_t) * (c->N + 1), hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < c->N; ++i)
+        if (hptr[i + 1] - hptr[i] > HV_MC_MAXK) return hv_fail(c, HV_ERR_CAPACITY, "a cell has more neighbours than the Monte-Carlo kernel holds");
+    const double cvol = std::pow(M_PI, 0.5 * D) / std::tgamma(1.0 + 0.5 * D);
+    k_mc_cell<D><<<(unsigned)c->N, 256, 0, c->stream>>>(g, ptr, pkey, ndir, seed, cvol, D * cvol, vol, area);
+    HVCK(hipGetLastError());
+    k_mc_sym<D><<<nblk(npair, 256), 256, 0, c->stream>>>(g, ptr, pkey, npair, vol, area);
+    HVCK(hipGetLastError());
+    HVCK(hipEventRecord(c->ev[1], c->stream));
+    HVCK(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HVCK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    return finish<D>(c, pkey, area, vol, ptr, npair, ms, out);
+}
+
+int hv_integral_counts_impl(const hv_integral* I, int64_t* ncells, int64_t* nnbr) {
+    if (!I) return HV_ERR_ARG;
+    if (ncells) *ncells = I->N;
+    if (nnbr) *nnbr = (int64_t)I->nbr.size();
+    return HV_OK;
+}
+
+int hv_integral_export_impl(const hv_integral* I, double* vol, int64_t* nbr_ptr, int64_t* nbr, double* area,
+                            double* ms) {
+    if (!I) return HV_ERR_ARG;
+    if (vol) std::copy(I->vol.begin(), I->vol.end(), vol);
+    if (nbr_ptr) std::copy(I->ptr.begin(), I->ptr.end(), nbr_ptr);
+    if (nbr) std::copy(I->nbr.begin(), I->nbr.end(), nbr);
+    if (area) std::copy(I->area.begin(), I->area.end(), area);
+    if (ms) *ms = I->ms;
+    return HV_OK;
+}
+
+void hv_integral_free_impl(hv_integral* I) { delete I; }
+
+#define HV_IINST(D)                                                                                 \
+    template int hv_integrate_polytope_impl<D>(hv_ctx*, const hv_mesh*, hv_integral**);          \
+    template int hv_integrate_mc_impl<D>(hv_ctx*, const hv_mesh*, int64_t, uint64_t, hv_integral**);
+HV_IINST(2)
+HV_IINST(3)
+HV_IINST(4)
+HV_IINST(5)
+HV_IINST(6)

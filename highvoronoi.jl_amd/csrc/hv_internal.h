@@ -63,3 +63,14 @@ template <int D>
 int hv_inrange_impl(hv_ctx* c, int64_t nq, const double* q, const double* radius, int64_t* ptr, int64_t* idx);
 template <int D>
 int hv_mesh_export_impl(const hv_mesh* m, int64_t* sig, double* coords);
+
+// cell volumes and interface areas (hv_integrate.hip)
+struct hv_integral;
+template <int D>
+int hv_integrate_polytope_impl(hv_ctx* c, const hv_mesh* mesh, hv_integral** out);
+template <int D>
+int hv_integrate_mc_impl(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t seed, hv_integral** out);
+int hv_integral_counts_impl(const hv_integral* I, int64_t* ncells, int64_t* nnbr);
+int hv_integral_export_impl(const hv_integral* I, double* vol, int64_t* nbr_ptr, int64_t* nbr, double* area,
+                            double* ms);
+void hv_integral_free_impl(hv_integral* I);

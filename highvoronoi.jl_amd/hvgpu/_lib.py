@@ -73,6 +73,11 @@ def lib():
     L.hv_nn_halfspace_batch.argtypes = [vp, C.c_int64, P(C.c_double), P(C.c_double), P(C.c_double),
                                         P(C.c_int64), P(C.c_double)]
     L.hv_inrange_batch.argtypes = [vp, C.c_int64, P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_int64)]
+    L.hv_integrate_polytope.argtypes = [vp, vp, P(vp)]
+    L.hv_integrate_mc.argtypes = [vp, vp, C.c_int64, C.c_uint64, P(vp)]
+    L.hv_integral_counts.argtypes = [vp, P(C.c_int64), P(C.c_int64)]
+    L.hv_integral_export.argtypes = [vp, P(C.c_double), P(C.c_int64), P(C.c_int64), P(C.c_double), P(C.c_double)]
+    L.hv_integral_free.argtypes = [vp]
     _lib = L
     return L
 
@@ -198,6 +203,34 @@ class Context:
         dist = np.zeros(n)
         self._check(lib().hv_nn_halfspace_batch(self._h, n, _dp(q), _dp(u), _dp(c), _i64(idx), _dp(dist)))
         return idx, dist
+
+    def integrate(self, mesh: Mesh, method: str = "polytope", ndir: int = 1000, seed: int = 0) -> dict:
+        """Cell volumes and interface areas of `mesh` (built from this context's points):
+        method "polytope" (exact) or "montecarlo" (ndir directions per cell).  Returns
+        {"volumes", "nbr_ptr", "neighbors" (1-based, planes n+p), "areas", "ms"}."""
+        L = lib()
+        h = C.c_void_p()
+        if method == "polytope":
+            self._check(L.hv_integrate_polytope(self._h, mesh._h, C.byref(h)))
+        elif method == "montecarlo":
+            self._check(L.hv_integrate_mc(self._h, mesh._h, int(ndir), int(seed), C.byref(h)))
+        else:
+            raise ValueError(f"unknown integration method {method!r}")
+        try:
+            nc = C.c_int64(0); nn = C.c_int64(0)
+            L.hv_integral_counts(h, C.byref(nc), C.byref(nn))
+            vol = np.zeros(nc.value)
+            ptr = np.zeros(nc.value + 1, dtype=np.int64)
+            nbr = np.zeros(max(1, nn.value), dtype=np.int64)
+            area = np.zeros(max(1, nn.value))
+            ms = C.c_double(0.0)
+            rc = L.hv_integral_export(h, _dp(vol), _i64(ptr), _i64(nbr), _dp(area), C.byref(ms))
+            if rc != HV_OK:
+                raise HVError(rc, "integral export failed")
+        finally:
+            L.hv_integral_free(h)
+        return {"volumes": vol, "nbr_ptr": ptr, "neighbors": nbr[: nn.value], "areas": area[: nn.value],
+                "ms": ms.value}
 
     def inrange(self, q, radius):
         q = np.ascontiguousarray(q, dtype=np.float64)

@@ -44,6 +44,7 @@ extern "C" {
 
 typedef struct hv_ctx hv_ctx;
 typedef struct hv_mesh hv_mesh;
+typedef struct hv_integral hv_integral;
 
 typedef struct hv_opts {
     int32_t precision;        /* HV_PREC_* */
@@ -126,6 +127,23 @@ int hv_nn_halfspace_batch(hv_ctx* ctx, int64_t nq, const double* q, const double
  * |x - q_k| <= radius_k.  Two-step: pass idx == NULL to receive counts in ptr[1..nq]
  * (ptr[0] = 0, prefix sums), then call again with idx of size ptr[nq]. */
 int hv_inrange_batch(hv_ctx* ctx, int64_t nq, const double* q, const double* radius, int64_t* ptr, int64_t* idx);
+
+/* Integration ------------------------------------------------------------ */
+/* Exact cell volumes and interface areas of a mesh built from ctx's current points and
+ * boundary: the integrator=VI_POLYGON path, Polygon_Integrator + _integrate
+ * (src/polyintegrator.jl:33, :80-468; driver src/integrate.jl:252-382; neighbours
+ * src/neighbors.jl:214-258).  General position only (HV_ERR_DEGENERATE otherwise). */
+int hv_integrate_polytope(hv_ctx* ctx, const hv_mesh* mesh, hv_integral** out);
+/* Directional Monte-Carlo volumes and areas, ndir rays per cell: the integrator=VI_MONTECARLO
+ * path, Montecarlo_Integrator (src/mcintegrator.jl:29-226, src/cleanup_cell.jl:6-48), with
+ * independent directions per cell drawn from a counter-based stream keyed by seed. */
+int hv_integrate_mc(hv_ctx* ctx, const hv_mesh* mesh, int64_t ndir, uint64_t seed, hv_integral** out);
+/* Voronoi_Integral's parity surface (src/integral.jl:11-21): volumes[ncells]; for cell i
+ * its neighbours nbr[nbr_ptr[i] .. nbr_ptr[i+1]) sorted, 1-based (planes n+p), and the
+ * interface areas aligned with them.  ms (optional) receives the device time. */
+int hv_integral_counts(const hv_integral* I, int64_t* ncells, int64_t* nnbr);
+int hv_integral_export(const hv_integral* I, double* vol, int64_t* nbr_ptr, int64_t* nbr, double* area, double* ms);
+int hv_integral_free(hv_integral* I);
 
 #ifdef __cplusplus
 }

@@ -17,8 +17,8 @@ LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 
 def build(force: bool = False) -> str:
-    src = os.path.join(_HERE, "hv_oracle.c")
-    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+    srcs = [os.path.join(_HERE, f) for f in ("hv_oracle.c", "hv_integrate.c")]
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < max(map(os.path.getmtime, srcs)):
         subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
     return LIB_PATH
 
@@ -45,6 +45,12 @@ def lib():
         L.orc_walkray.argtypes = [C.c_void_p, P(C.c_int64), C.c_int, P(C.c_double), C.c_int64,
                                   P(C.c_int64), P(C.c_int), P(C.c_double), P(C.c_double)]
         L.orc_last_error.restype = C.c_char_p
+        L.orc_polytope.argtypes = [C.c_int, C.c_int64, P(C.c_double), C.c_int, P(C.c_double), P(C.c_double),
+                                   C.c_int64, P(C.c_int64), P(C.c_double), P(C.c_int64), P(C.c_int64),
+                                   P(C.c_int64), P(C.c_int64), P(C.c_double), P(C.c_double)]
+        L.orc_mc_dirs.argtypes = [C.c_int, C.c_int64, P(C.c_double), C.c_int, P(C.c_double), P(C.c_double),
+                                  P(C.c_int64), P(C.c_int64), C.c_int64, C.c_uint64, P(C.c_double), P(C.c_double)]
+        L.orc_mc_direction.argtypes = [C.c_int, C.c_uint64, C.c_int64, C.c_int64, C.c_int64, P(C.c_double)]
         _lib = L
     return _lib
 
@@ -128,3 +134,71 @@ def sort_rows(keys: np.ndarray) -> np.ndarray:
         return keys
     order = np.lexsort(keys.T[::-1])
     return keys[order]
+
+
+# ---- integration (src/polyintegrator.jl, src/mcintegrator.jl) ---------------------------
+
+def adjacency(keys: np.ndarray, n: int):
+    """Cell -> vertex rows and cell -> neighbour ids (sorted, 1-based, planes n+p) from a
+    key table (src/neighbors.jl:219-251: every id co-occurring in a vertex key)."""
+    keys = np.asarray(keys, dtype=np.int64)
+    nv, w = keys.shape
+    rows = np.repeat(np.arange(nv, dtype=np.int64), w)
+    ids = keys.reshape(-1)
+    real = (ids >= 1) & (ids <= n)
+    cell = ids[real] - 1
+    order = np.lexsort((rows[real], cell))
+    cell_vert = rows[real][order]
+    cell_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(cell_ptr, cell[order] + 1, 1)
+    cell_ptr = np.cumsum(cell_ptr)
+    a = np.repeat(keys, w, axis=1).reshape(-1)
+    b = np.tile(keys, (1, w)).reshape(-1)
+    keep = (a != b) & (a >= 1) & (a <= n) & (b >= 1)
+    pairs = np.unique(np.stack([a[keep] - 1, b[keep]], axis=1), axis=0)
+    nbr_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(nbr_ptr, pairs[:, 0] + 1, 1)
+    nbr_ptr = np.cumsum(nbr_ptr)
+    return cell_ptr, cell_vert, nbr_ptr, np.ascontiguousarray(pairs[:, 1])
+
+
+def polytope(xs, base, normal, keys, coords):
+    """Exact polytope volumes and interface areas: (vol[n], nbr_ptr, nbr, area)."""
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    n, d = xs.shape
+    base = np.ascontiguousarray(base, dtype=np.float64)
+    normal = np.ascontiguousarray(normal, dtype=np.float64)
+    keys = np.ascontiguousarray(keys, dtype=np.int64)
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    cell_ptr, cell_vert, nbr_ptr, nbr = adjacency(keys, n)
+    vol = np.zeros(n)
+    area = np.zeros(len(nbr))
+    rc = lib().orc_polytope(d, n, _dp(xs), base.shape[0], _dp(base), _dp(normal), len(keys), _ip(keys), _dp(coords),
+                            _ip(cell_ptr), _ip(np.ascontiguousarray(cell_vert)), _ip(nbr_ptr), _ip(nbr),
+                            _dp(vol), _dp(area))
+    if rc != 0:
+        raise RuntimeError(f"oracle polytope integration failed ({rc})")
+    return vol, nbr_ptr, nbr, area
+
+
+def mc_dirs(xs, base, normal, nbr_ptr, nbr, ndir: int, seed: int):
+    """Directional Monte-Carlo volumes and areas (directions drawn per cell): (vol, area)."""
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    n, d = xs.shape
+    base = np.ascontiguousarray(base, dtype=np.float64)
+    normal = np.ascontiguousarray(normal, dtype=np.float64)
+    nbr_ptr = np.ascontiguousarray(nbr_ptr, dtype=np.int64)
+    nbr = np.ascontiguousarray(nbr, dtype=np.int64)
+    vol = np.zeros(n)
+    area = np.zeros(len(nbr))
+    rc = lib().orc_mc_dirs(d, n, _dp(xs), base.shape[0], _dp(base), _dp(normal), _ip(nbr_ptr), _ip(nbr),
+                           ndir, seed, _dp(vol), _dp(area))
+    if rc != 0:
+        raise RuntimeError(f"oracle Monte-Carlo integration failed ({rc})")
+    return vol, area
+
+
+def mc_direction(d: int, seed: int, cell: int, ndir: int, q: int):
+    u = np.zeros(d)
+    lib().orc_mc_direction(d, seed, cell, ndir, q, _dp(u))
+    return u
