@@ -2,7 +2,18 @@
 
 The compute lives in libhv.so (HIP kernels for gfx950 behind the C ABI declared in
 include/hv.h); this package binds it with ctypes and mirrors the reference's
-public API for the path: ``VoronoiGeometry``, ``VoronoiData``, ``cuboid``,
-``RaycastParameter`` and the integrator singletons.
+public API for the path: ``VoronoiGeometry``, ``VoronoiData``, ``cuboid`` and the
+integrator singletons (hvgpu.geometry), imported lazily so that the CPU-only parts
+(synth) load without the HIP library.
 """
 from . import synth  # noqa: F401
+
+__all__ = ["synth", "VoronoiGeometry", "VoronoiData", "cuboid", "Boundary", "Plane",
+           "VI_GEOMETRY", "VI_POLYGON", "VI_MONTECARLO"]
+
+
+def __getattr__(name):
+    if name in __all__ and name != "synth":
+        from . import geometry
+        return getattr(geometry, name)
+    raise AttributeError(name)

@@ -1,0 +1,165 @@
+"""User-facing mirror of the reference's API for this path, on top of libhv.so.
+
+    VG = VoronoiGeometry(xs, cuboid(5, periodic=[]), integrator=VI_POLYGON,
+                         search_settings={"precision": "bf16"})
+    data = VoronoiData(VG)
+    data.volume[i], data.neighbors[i], data.area[i], data.vertices[i]
+
+follows VoronoiGeometry(xs, boundary; integrator=..., search_settings=(...))
+(src/geometry.jl:136-260) and VoronoiData(VG) (src/voronoidata.jl:545-621) with the same
+field meanings: neighbours sorted, 1-based, boundary plane p as n + p; area[i][k] belongs to
+neighbors[i][k]; vertices[i] maps each vertex key of cell i (sorted 1-based tuple) to its
+coordinates.  The mesh and the integrals are computed by the device path only; there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import Context
+
+# integrator singletons (src/HighVoronoi.jl:102-117)
+VI_GEOMETRY = "geometry"
+VI_POLYGON = "polygon"
+VI_MONTECARLO = "montecarlo"
+VI_FAST_POLYGON = "fast_polygon"
+
+DIRICHLET = 0
+NEUMANN = -1
+
+
+@dataclass
+class Plane:
+    """A boundary plane (src/boundary.jl:22-29): base point, outer normal, and BC
+    (0 Dirichlet, -1 Neumann, > 0 the 1-based index of the periodic partner plane)."""
+    base: np.ndarray
+    normal: np.ndarray
+    BC: int = DIRICHLET
+
+
+@dataclass
+class Boundary:
+    planes: list = field(default_factory=list)
+
+    @property
+    def dimension(self):
+        return len(self.planes[0].base) if self.planes else 0
+
+    def arrays(self):
+        base = np.array([p.base for p in self.planes], dtype=np.float64)
+        normal = np.array([p.normal for p in self.planes], dtype=np.float64)
+        bc = np.array([p.BC for p in self.planes], dtype=np.int16)
+        return base, normal, bc
+
+
+def cuboid(dim: int, dimensions=None, periodic=None, neumann=(), offset=None) -> Boundary:
+    """cuboid(dim; dimensions=ones, periodic=1:dim, neumann=[], offset=zeros)
+    (src/boundary.jl:510-534).  As in the reference the default is periodic in every
+    direction; pass periodic=[] for the Dirichlet box."""
+    dims = np.ones(dim) if dimensions is None else np.asarray(dimensions, dtype=np.float64)
+    off = np.zeros(dim) if offset is None else np.asarray(offset, dtype=np.float64)
+    per = set(range(1, dim + 1)) if periodic is None else set(periodic)
+    planes = [None] * (2 * dim)
+    for i in range(1, dim + 1):
+        unit = np.zeros(dim)
+        unit[i - 1] = 1.0
+        if i in per:
+            planes[2 * i - 2] = Plane(off + dims[i - 1] * unit, unit.copy(), 2 * i)
+            planes[2 * i - 1] = Plane(off.copy(), -unit, 2 * i - 1)
+            continue
+        planes[2 * i - 2] = Plane(off + dims[i - 1] * unit, unit.copy(), NEUMANN if i in neumann else DIRICHLET)
+        planes[2 * i - 1] = Plane(off.copy(), -unit, NEUMANN if -i in neumann else DIRICHLET)
+    return Boundary(planes)
+
+
+def _cell_vertex_csr(keys: np.ndarray, n: int):
+    nv, w = keys.shape
+    rows = np.repeat(np.arange(nv, dtype=np.int64), w)
+    ids = keys.reshape(-1)
+    real = (ids >= 1) & (ids <= n)
+    cell = ids[real] - 1
+    order = np.argsort(cell, kind="stable")
+    ptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(ptr, cell + 1, 1)
+    return np.cumsum(ptr), rows[real][order]
+
+
+class VoronoiGeometry:
+    """Mesh (and, with an integrator, volumes and interface areas) of the generators xs
+    in `boundary`, computed on one MI355X.
+
+    search_settings: {"precision": "bf16" | "fp64", "leaf_size": int, "device": int}.
+    mc_accurate: (directions per cell, unused, unused) as the reference's tuple
+    (src/geometry.jl:139); seed drives the Monte-Carlo directions.
+    """
+
+    def __init__(self, xs, boundary: Boundary | None = None, integrator=VI_GEOMETRY, search_settings=None,
+                 mc_accurate=(1000, 100, 20), seed: int = 0):
+        self.nodes = np.ascontiguousarray(np.asarray(xs, dtype=np.float64))
+        n, d = self.nodes.shape
+        self.boundary = boundary if boundary is not None else Boundary([])
+        s = dict(search_settings or {})
+        self.precision = s.get("precision", "bf16")
+        self.integrator = integrator
+        if integrator == VI_FAST_POLYGON:
+            raise NotImplementedError("VI_FAST_POLYGON is not part of the device path; use VI_POLYGON")
+        self._ctx = Context(int(s.get("device", 0)))
+        self._ctx.set_points(self.nodes)
+        base, normal, bc = self.boundary.arrays()
+        self._ctx.set_boundary(base.reshape(-1, d), normal.reshape(-1, d), bc if len(bc) else None)
+        self._mesh = self._ctx.build_mesh(precision=self.precision, leaf_size=int(s.get("leaf_size", 0)))
+        self.stats = self._mesh.stats()
+        self.integral = None
+        if integrator == VI_POLYGON:
+            self.integral = self._ctx.integrate(self._mesh, "polytope")
+        elif integrator == VI_MONTECARLO:
+            self.integral = self._ctx.integrate(self._mesh, "montecarlo", ndir=int(mc_accurate[0]), seed=seed)
+        self._keys = None
+        self._coords = None
+
+    def vertex_table(self):
+        """(keys[V, d+1] sorted 1-based, coords[V, d]) in lexicographic key order."""
+        if self._keys is None:
+            self._keys, self._coords = self._mesh.export()
+        return self._keys, self._coords
+
+    def __del__(self):
+        m = getattr(self, "_mesh", None)
+        if m is not None:
+            m.free()
+
+
+class _CellVertices:
+    def __init__(self, keys, coords, ptr, rows):
+        self._k, self._c, self._p, self._r = keys, coords, ptr, rows
+
+    def __len__(self):
+        return len(self._p) - 1
+
+    def __getitem__(self, i):
+        rows = self._r[self._p[i]:self._p[i + 1]]
+        return {tuple(int(v) for v in self._k[r]): self._c[r] for r in rows}
+
+
+class VoronoiData:
+    """Read-only views of a VoronoiGeometry (src/voronoidata.jl:545-621): nodes, vertices,
+    neighbors, volume, area.  volume / neighbors / area need an integrator."""
+
+    def __init__(self, VG: VoronoiGeometry):
+        self.geometry = VG
+        self.nodes = VG.nodes
+        self.boundary = VG.boundary
+        n = len(VG.nodes)
+        keys, coords = VG.vertex_table()
+        ptr, rows = _cell_vertex_csr(keys, n)
+        self.vertices = _CellVertices(keys, coords, ptr, rows)
+        I = VG.integral
+        if I is None:
+            self.volume = self.neighbors = self.area = None
+            return
+        self.volume = I["volumes"]
+        p = I["nbr_ptr"]
+        self.neighbors = [I["neighbors"][p[i]:p[i + 1]] for i in range(n)]
+        self.area = [I["areas"][p[i]:p[i + 1]] for i in range(n)]

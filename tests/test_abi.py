@@ -1,6 +1,8 @@
 """CPU tests of the C-ABI library: it is built, it loads, and it exports every entry
 point include/hv.h declares (no compute call: there is no GPU in this container)."""
 import ctypes
+
+import numpy as np
 import os
 import re
 
@@ -40,3 +42,18 @@ def test_library_is_built_for_gfx950():
     with open(LIB, "rb") as f:
         blob = f.read()
     assert b"gfx950" in blob
+
+
+def test_cuboid_mirrors_reference_plane_order():
+    from hvgpu.geometry import cuboid
+    from hvgpu import synth
+    b = cuboid(4, periodic=[])
+    base, normal, bc = b.arrays()
+    want_b, want_n = synth.cube_planes(4)
+    assert np.array_equal(base, want_b) and np.array_equal(normal, want_n)
+    assert np.all(bc == 0)
+    # the reference's default is periodic in every direction, partners 2i <-> 2i-1
+    bp = cuboid(3).arrays()[2]
+    assert list(bp) == [2, 1, 4, 3, 6, 5]
+    bn = cuboid(2, periodic=[], neumann=[1, -2]).arrays()[2]
+    assert list(bn) == [-1, 0, 0, -1]

@@ -129,3 +129,25 @@ def test_montecarlo_matches_oracle(ctx, d, n, seed):
     err = (I["volumes"] - E["volumes"]) / E["volumes"]
     assert abs(np.median(err)) < 0.01
     assert abs(I["volumes"].sum() - 1.0) < 0.02
+
+
+def test_voronoi_geometry_and_data_mirror(ctx):
+    from hvgpu import VI_POLYGON, VoronoiData, VoronoiGeometry, cuboid
+    xs = synth.uniform(3, 300, 41)
+    VG = VoronoiGeometry(xs, cuboid(3, periodic=[]), integrator=VI_POLYGON, search_settings={"precision": "bf16"})
+    data = VoronoiData(VG)
+    keys, coords = VG.vertex_table()
+    b, nrm = synth.cube_planes(3)
+    ok, oc, _ = Oracle(xs, b, nrm).voronoi()
+    assert np.array_equal(keys, ok[np.lexsort(ok.T[::-1])])
+    vol, ptr, nbr, area = polytope(xs, b, nrm, keys, coords)
+    assert rel(data.volume, vol) < TOL
+    for i in (0, 17, 299):
+        assert np.array_equal(data.neighbors[i], nbr[ptr[i]:ptr[i + 1]])
+        assert rel_area(data.area[i], area[ptr[i]:ptr[i + 1]]) < TOL
+        for key, r in data.vertices[i].items():
+            assert i + 1 in key and len(key) == 4
+    # the reference's default cuboid is periodic: refused, not silently changed
+    from hvgpu._lib import HVError
+    with pytest.raises(HVError):
+        VoronoiGeometry(xs, cuboid(3))
