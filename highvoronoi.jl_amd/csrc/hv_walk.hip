@@ -31,6 +31,12 @@
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// ray outcome codes of the scan
+#define HV_ST_NONE 0
+#define HV_ST_FOUND 1
+#define HV_ST_TIE 2
+#define HV_ST_SINGULAR -2
+
 // ---------------------------------------------------------------------------------
 // points and tree
 
@@ -356,9 +362,11 @@ __global__ __launch_bounds__(256) void k_insert(const int* __restrict__ status, 
                                                 int64_t base, int* __restrict__ htab, uint64_t hmask,
                                                 uint8_t* __restrict__ win, unsigned long long* __restrict__ ctr) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool dup = false, full = false;
+    bool dup = false, full = false, none = false, singular = false;
     if (t < n) {
         uint8_t won = 0;
+        none = status[t] == HV_ST_NONE;
+        singular = status[t] == HV_ST_SINGULAR;
         if (status[t] > 0) {
             const int id = (int)(base + t);
             const int* key = vkey + (int64_t)id * (D + 1);
@@ -379,6 +387,8 @@ __global__ __launch_bounds__(256) void k_insert(const int* __restrict__ status, 
     }
     hv_wave_count(&ctr[0], dup);
     hv_wave_count(&ctr[1], full);
+    hv_wave_count(&ctr[2], none);
+    hv_wave_count(&ctr[3], singular);
 }
 
 template <int D>
@@ -650,10 +660,6 @@ __global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, 
 // ---------------------------------------------------------------------------------
 // The candidate scan: one wavefront walks the generator tree for 64 rays at once.
 
-#define HV_ST_NONE 0
-#define HV_ST_FOUND 1
-#define HV_ST_TIE 2
-#define HV_ST_SINGULAR -2
 
 template <int D>
 __device__ __forceinline__ double hv_box_dist2(const DevGeo& g, int node, const double* p) {
@@ -1882,6 +1888,8 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     HVCK(hipEventElapsedTime(&ms_all, e0, e1));
     if (hctr[9]) return hv_fail(c, HV_ERR_CAPACITY, "vertex hash overflow");
     if (hctr[5]) return hv_fail(c, HV_ERR_CAPACITY, "edge hash overflow");
+    if (hctr[11])
+        return hv_fail(c, HV_ERR_DEGENERATE, "a new vertex has a singular circumcentre system (input not in general position)");
     st.levels = levels;
     st.vertices = nlive;
     st.pairs_scanned = (int64_t)hctr[0];
@@ -1892,7 +1900,8 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     st.fallbacks = (int64_t)hctr[7];
     st.fp64_rechecks = (int64_t)hctr[6];
     st.retries = 0;
-    st.unbounded = 0;
+    st.unbounded = (int64_t)hctr[10];
+    st.degenerate = (int64_t)hctr[11];
     st.max_t_relerr = 0.0;
     st.ms_total = ms_all;
     st.ms_raycast = ms_ray;

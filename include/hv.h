@@ -48,15 +48,17 @@ typedef struct hv_integral hv_integral;
 
 typedef struct hv_opts {
     int32_t precision;        /* HV_PREC_* */
-    int32_t leaf_size;        /* generators per tree leaf (0 = default 32) */
-    int32_t rays_per_block;   /* threads per raycast block (0 = default 256) */
+    int32_t leaf_size;        /* generators per tree leaf (0 = default 32; HV_PREC_BF16 needs 32) */
+    int32_t rays_per_block;   /* ignored: the scan works on packets of 64 rays, one wavefront each */
     int32_t reserved0;        /* walk-level schedule: 0 or 1 = every ray of a level in one scan
                                  (default); >1 = that many chunks, edges registered after each;
                                  <0 = rays grouped by tentative key, one scan per group first */
     double plane_tolerance;   /* src/raycast-types.jl:229 (0 = 1e-12) */
-    double ray_tol;           /* src/raycast-types.jl:230 (0 = 1e-12) */
+    double ray_tol;           /* src/raycast-types.jl:230; ignored (it acts on non-general vertices,
+                                 which the device walk reports as HV_ERR_DEGENERATE) */
     int64_t vertex_capacity;  /* device vertex-table capacity hint (0 = auto) */
-    uint64_t seed;
+    uint64_t seed;            /* ignored: the walk starts from the 2^d box corners, not from the
+                                 reference's random descent (src/raycast.jl:45-109, 211-239) */
 } hv_opts;
 
 typedef struct hv_stats_t {

@@ -59,6 +59,8 @@ def test_mesh_matches_oracle(ctx, d, n, seed):
     assert np.array_equal(rows_view(k), rows_view(ok))
     assert max_rel_coord_err(k, c, ok, oc) < COORD_TOL
     assert st["vertices"] == len(k)
+    # every ray ended at a vertex: none unbounded in the box, no singular circumcentre
+    assert st["unbounded"] == 0 and st["degenerate"] == 0
 
 
 def test_config_4d_20k_matches_oracle(ctx):
