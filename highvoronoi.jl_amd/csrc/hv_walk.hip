@@ -115,70 +115,6 @@ __global__ void k_node_boxes(int first, int count, int NN, double* __restrict__ 
     }
 }
 
-// HV_KNN nearest generators of every generator (sorted index space), used to start each
-// ray's search with a small certificate ball.  Own leaf first, then a stackless walk.
-template <int D>
-__global__ __launch_bounds__(256) void k_knn(DevGeo g, int* __restrict__ knn) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= g.N) return;
-    const int N = g.N;
-    double p[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) p[k] = g.xs[(int64_t)(j) * (D + 1) + k];
-    double bd[HV_KNN];
-    int bi[HV_KNN];
-#pragma unroll
-    for (int m = 0; m < HV_KNN; ++m) { bd[m] = 1e300; bi[m] = j; }
-    auto consider = [&](int q) {
-        if (q == j) return;
-        double d2 = 0.0;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const double a = g.xs[(int64_t)(q) * (D + 1) + k] - p[k];
-            d2 = fma(a, a, d2);
-        }
-        if (!(d2 < bd[HV_KNN - 1])) return;
-        // sorted insert with static indices only
-        int pos = 0;
-#pragma unroll
-        for (int m = 0; m < HV_KNN; ++m) pos += (bd[m] <= d2) ? 1 : 0;
-#pragma unroll
-        for (int m = HV_KNN - 1; m > 0; --m) {
-            bd[m] = (m > pos) ? bd[m - 1] : ((m == pos) ? d2 : bd[m]);
-            bi[m] = (m > pos) ? bi[m - 1] : ((m == pos) ? q : bi[m]);
-        }
-        bd[0] = (pos == 0) ? d2 : bd[0];
-        bi[0] = (pos == 0) ? q : bi[0];
-    };
-    const int own = j / g.L;
-    {
-        const int j0 = own * g.L, j1 = min(j0 + g.L, N);
-        for (int q = j0; q < j1; ++q) consider(q);
-    }
-    int node = 1;
-    while (node) {
-        double d2 = 0.0;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const double lo = g.blo[(int64_t)k * g.NN + node], hi = g.bhi[(int64_t)k * g.NN + node];
-            const double a = fmax(fmax(lo - p[k], p[k] - hi), 0.0);
-            d2 = fma(a, a, d2);
-        }
-        if (d2 < bd[HV_KNN - 1]) {
-            if (node < g.P2) { node = 2 * node; continue; }
-            const int leaf = node - g.P2;
-            if (leaf != own) {
-                const int j0 = leaf * g.L, j1 = min(j0 + g.L, N);
-                for (int q = j0; q < j1; ++q) consider(q);
-            }
-        }
-        while (node & 1) node >>= 1;
-        node = node ? node + 1 : 0;
-    }
-#pragma unroll
-    for (int m = 0; m < HV_KNN; ++m) knn[(int64_t)j * HV_KNN + m] = bi[m];
-}
-
 // 8-ary tree over the same leaves (level h, 8^h slots, node (l,i) has children (l+1, 8i+c)).
 // Internal node (l,i) has global id (8^l - 1)/7 + i and stores the boxes of its 8 children
 // contiguously, in FP32 rounded outward, so one load per visited node tests 8 children.
@@ -842,8 +778,180 @@ __global__ __launch_bounds__(256) void k_ray_init(DevGeo g, DevRays R, int nrays
 // (src/raycast.jl:877-916) is applied to them at the end.
 // One wavefront per workgroup: packets take very different times, and a workgroup of
 // several wavefronts holds its slots until its slowest packet is done.
+// The HV_KNN nearest generators of every generator, 64 Morton-consecutive queries per
+// wavefront: the packet's own 64 points first, then the 8-ary tree nearest-first, entering a
+// child when its box is within the current 16th distance of some query (FP32 test, widened);
+// each visited leaf is staged in LDS and scanned by every lane into a sorted top-16.
+template <int D>
+__global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
+    __shared__ int stk[128], stkl[128];
+    __shared__ float cbx[16 * D];
+    __shared__ unsigned long long hms[8];
+    __shared__ float dcs[8];
+    __shared__ double px[D][64];
+    const int lane = threadIdx.x;
+    const int N = g.N;
+    const int base = blockIdx.x * 64;
+    const int j = base + lane;
+    const bool act = j < N;
+    const int jj = act ? j : N - 1;
+    double p[D];
+    float pf[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        p[k] = g.xs[(int64_t)jj * (D + 1) + k];
+        pf[k] = (float)p[k];
+    }
+    double bd[HV_KNN];
+    int bi[HV_KNN];
+#pragma unroll
+    for (int m = 0; m < HV_KNN; ++m) { bd[m] = 1e300; bi[m] = jj; }
+    auto consider = [&](int q, double d2) {
+        if (!(d2 < bd[HV_KNN - 1]) || q == jj) return;
+        int pos = 0;
+#pragma unroll
+        for (int m = 0; m < HV_KNN; ++m) pos += (bd[m] <= d2) ? 1 : 0;
+#pragma unroll
+        for (int m = HV_KNN - 1; m > 0; --m) {
+            bd[m] = (m > pos) ? bd[m - 1] : ((m == pos) ? d2 : bd[m]);
+            bi[m] = (m > pos) ? bi[m - 1] : ((m == pos) ? q : bi[m]);
+        }
+        bd[0] = (pos == 0) ? d2 : bd[0];
+        bi[0] = (pos == 0) ? q : bi[0];
+    };
+    // scan the npts points staged in px (global ids q0 + s)
+    auto scan_staged = [&](int q0, int npts, bool skip_own) {
+        for (int s = 0; s < npts; ++s) {
+            const int q = q0 + s;
+            if (skip_own && q >= base && q < base + 64) continue;
+            double d2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const double a = px[k][s] - p[k];
+                d2 = fma(a, a, d2);
+            }
+            consider(q, d2);
+        }
+    };
+    {
+        const int np = min(64, N - base);
+#pragma unroll
+        for (int k = 0; k < D; ++k) px[k][lane] = p[k];
+        hv_wave_sync();
+        scan_staged(base, np, false);
+        hv_wave_sync();
+    }
+    // packet centre (uniform) orders the children
+    float pcf[D];
+    {
+        double c[D], cnt = act ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) c[k] = act ? p[k] : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) c[k] += __shfl_xor(c[k], off);
+            cnt += __shfl_xor(cnt, off);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            pcf[k] = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint((float)(c[k] / cnt))));
+    }
+    auto child_d2f = [&](int c, const float* q) {
+        float d2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float lo = cbx[c * 2 * D + k], hi = cbx[c * 2 * D + D + k];
+            const float a = fmaxf(fmaxf(lo - q[k], q[k] - hi), 0.f);
+            d2 = fmaf(a, a, d2);
+        }
+        return d2;
+    };
+    const double slack = 1e-6 * (g.xmax + 1.0);
+    auto reach = [&]() {   // widened FP32 radius^2 of the current 16th neighbour
+        const double r = sqrt(bd[HV_KNN - 1]) + slack;
+        return (float)fmin(r * r * (1.0 + 1e-5), 1e30);
+    };
+    auto pick = [&](unsigned rem, bool nearest) {
+        int best = -1;
+        float bdist = 0.f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float v = dcs[c];
+            const bool in = (rem >> c) & 1u;
+            const bool better = best < 0 || (nearest ? v < bdist : v > bdist);
+            if (in && better) { best = c; bdist = v; }
+        }
+        return best;
+    };
+    int sp = 1;
+    stk[0] = 0;
+    stkl[0] = 0;
+    while (sp > 0) {
+        --sp;
+        hv_wave_sync();
+        const int gid = __builtin_amdgcn_readfirstlane(stk[sp]);
+        const int l = __builtin_amdgcn_readfirstlane(stkl[sp]);
+        {
+            const float* src = g.cb + (int64_t)gid * (16 * D);
+            for (int q = lane; q < 16 * D; q += 64) cbx[q] = src[q];
+        }
+        hv_wave_sync();
+        const float R2f = reach();
+        unsigned rem = 0u;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const unsigned long long m = __ballot(act && (child_d2f(c, pf) <= R2f));
+            if (lane == 0) hms[c] = m;
+            rem |= (m != 0ull) ? (1u << c) : 0u;
+        }
+        if (lane < 8) dcs[lane] = child_d2f(lane, pcf);
+        rem = __builtin_amdgcn_readfirstlane(rem);
+        hv_wave_sync();
+        if (rem == 0u) continue;
+        const int off = ((1 << (3 * l)) - 1) / 7;
+        const int i = gid - off;
+        if (l + 1 == g.h) {
+            while (rem) {
+                const int cc = __builtin_amdgcn_readfirstlane(pick(rem, true));
+                rem &= ~(1u << cc);
+                const int leaf = 8 * i + cc;
+                const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
+                if (j0 >= base && j0 + npts <= base + 64) continue;   // the packet's own points
+                const bool hit = act && ((hms[cc] >> lane) & 1ull) && (child_d2f(cc, pf) <= reach());
+                if (__ballot(hit) == 0ull) continue;
+                for (int s0 = 0; s0 < npts; s0 += 64) {
+                    const int cnt = min(64, npts - s0);
+                    if (lane < cnt) {
+#pragma unroll
+                        for (int k = 0; k < D; ++k) px[k][lane] = g.xs[(int64_t)(j0 + s0 + lane) * (D + 1) + k];
+                    }
+                    hv_wave_sync();
+                    if (hit) scan_staged(j0 + s0, cnt, true);
+                    hv_wave_sync();
+                }
+            }
+        } else {
+            const int cbase = off + (1 << (3 * l)) + 8 * i;
+            while (rem) {
+                const int c = __builtin_amdgcn_readfirstlane(pick(rem, false));
+                rem &= ~(1u << c);
+                stk[sp] = cbase + c;
+                stkl[sp] = l + 1;
+                ++sp;
+            }
+        }
+    }
+    if (act) {
+#pragma unroll
+        for (int m = 0; m < HV_KNN; ++m) knn[(int64_t)j * HV_KNN + m] = bi[m];
+    }
+}
+
+// MODE 1 is held to 128 VGPRs (4 wavefronts per SIMD): the walk is latency-bound, and the
+// few values that spill are the rarely used FP64 ray data.
 template <int D, int MODE>
-__global__ __launch_bounds__(64) void k_raycast64(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
+__global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
                                                   const int* __restrict__ vkey, const double* __restrict__ vx,
                                                   int* __restrict__ status, int* __restrict__ win,
                                                   double* __restrict__ tout, double plane_tol,
@@ -1423,7 +1531,7 @@ int build_tree(hv_ctx* c, int L, Tree& T, bool kvec = false) {
             HVCK(hipGetLastError());
         }
     }
-    k_knn<D><<<nblk(N, 256), 256, 0, c->stream>>>(g, knn);
+    k_knn64<D><<<nblk(N, 64), 64, 0, c->stream>>>(g, knn);
     HVCK(hipGetLastError());
     if (kvec) {
         if (L != 32) return hv_fail(c, HV_ERR_UNSUPPORTED, "the mixed-precision scan uses leaves of 32 points");
