@@ -18,6 +18,11 @@
 
 #define HV_MAXD 6
 #define HV_EDGE_DONE (1ull << 62)
+// edge-hash slot: bits 0..34 (vertex << 4 | position) + 1, bits 35..61 a fingerprint of the
+// edge (the top 27 bits of its 64-bit hash), bit 62 = both end points known
+#define HV_EDGE_VMASK ((1ull << 35) - 1ull)
+#define HV_EDGE_FMASK (((1ull << 27) - 1ull) << 35)
+#define HV_EDGE_FP(hash64) (((hash64) >> 37) << 35)
 
 struct DevGeo {
     int N, P, L, nleaf, P2, NN;

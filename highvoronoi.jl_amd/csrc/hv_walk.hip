@@ -357,8 +357,10 @@ __global__ __launch_bounds__(256) void k_register(const int* __restrict__ fronti
         hv_edge_of<D>(key, pos, e);
         int64_t out = -1;   // an edge of planes only is a corner point, not a line
         if (e[0] < N) {
-            const unsigned long long mine = ((((unsigned long long)v) << 4) | (unsigned)pos) + 1ull;
-            uint64_t h = hv_hash_ints<D>(e) & emask;
+            const uint64_t hfull = hv_hash_ints<D>(e);
+            const unsigned long long fp = HV_EDGE_FP(hfull);
+            const unsigned long long mine = fp | (((((unsigned long long)v) << 4) | (unsigned)pos) + 1ull);
+            uint64_t h = hfull & emask;
             out = -2;
             for (uint64_t probe = 0; probe <= emask; ++probe) {
                 unsigned long long cur = etab[h];
@@ -367,7 +369,9 @@ __global__ __launch_bounds__(256) void k_register(const int* __restrict__ fronti
                     if (old == 0ull) { out = (int64_t)h; fresh = true; break; }
                     cur = old;
                 }
-                const unsigned long long raw = (cur & ~HV_EDGE_DONE) - 1ull;
+                // another edge's slot unless the fingerprints agree (then compare exactly)
+                if ((cur & HV_EDGE_FMASK) != fp) { h = (h + 1) & emask; continue; }
+                const unsigned long long raw = (cur & HV_EDGE_VMASK) - 1ull;
                 int e2[D];
                 hv_edge_of<D>(vkey + (int64_t)(raw >> 4) * (D + 1), (int)(raw & 15ull), e2);
                 bool eq = true;
@@ -395,7 +399,7 @@ __global__ void k_rehash_edges(const unsigned long long* __restrict__ old, uint6
     if (t > omask) return;
     unsigned long long cur = old[t];
     if (cur == 0ull) return;
-    const unsigned long long raw = (cur & ~HV_EDGE_DONE) - 1ull;
+    const unsigned long long raw = (cur & HV_EDGE_VMASK) - 1ull;
     int e[D];
     hv_edge_of<D>(vkey + (int64_t)(raw >> 4) * (D + 1), (int)(raw & 15ull), e);
     uint64_t h = hv_hash_ints<D>(e) & emask;
@@ -409,11 +413,14 @@ __global__ void k_rehash_edges(const unsigned long long* __restrict__ old, uint6
 template <int D>
 __device__ __forceinline__ int64_t hv_find_edge(const int* __restrict__ vkey, const unsigned long long* __restrict__ etab,
                                                 uint64_t emask, const int* e) {
-    uint64_t h = hv_hash_ints<D>(e) & emask;
+    const uint64_t full = hv_hash_ints<D>(e);
+    const unsigned long long fp = HV_EDGE_FP(full);
+    uint64_t h = full & emask;
     for (uint64_t probe = 0; probe <= emask; ++probe) {
         const unsigned long long cur = etab[h];
         if (cur == 0ull) return -2;
-        const unsigned long long raw = (cur & ~HV_EDGE_DONE) - 1ull;
+        if ((cur & HV_EDGE_FMASK) != fp) { h = (h + 1) & emask; continue; }
+        const unsigned long long raw = (cur & HV_EDGE_VMASK) - 1ull;
         int e2[D];
         hv_edge_of<D>(vkey + (int64_t)(raw >> 4) * (D + 1), (int)(raw & 15ull), e2);
         bool eq = true;
@@ -571,7 +578,7 @@ __global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, 
             pos = t % (D + 1);
             v = frontier[fi];
             const unsigned long long mine = ((((unsigned long long)v) << 4) | (unsigned)pos) + 1ull;
-            want = !(cur & HV_EDGE_DONE) && (cur == mine);
+            want = !(cur & HV_EDGE_DONE) && ((cur & HV_EDGE_VMASK) == mine);
         }
     }
     const int idx = hv_wave_alloc(nrays, want);
@@ -741,9 +748,9 @@ __global__ __launch_bounds__(256) void k_ray_init(DevGeo g, DevRays R, int nrays
         if (tp >= plane_tol) T.offer(tp, id);
     }
     // the winner is a Delaunay neighbour of every generator of the edge: scan their lists
-#pragma unroll 1
+#pragma unroll
     for (int a = 0; a < D; ++a) {
-        const int ea = hv_key_at<D - 1>(G.e, a);
+        const int ea = G.e[a];
         if (ea >= N) break;   // planes sort last
         const int* nb = g.knn + (int64_t)g.rank[ea] * HV_KNN;
 #pragma unroll 4
@@ -1604,13 +1611,14 @@ __device__ __forceinline__ bool hv_tent_key(const DevGeo& g, const DevRays& R, c
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_tent_group(DevGeo g, DevRays R, int nr, const int* __restrict__ vkey,
-                                                    int* __restrict__ ttab, uint64_t tmask, int* __restrict__ prim_of,
-                                                    uint8_t* __restrict__ primary) {
+__global__ __launch_bounds__(256) void k_tent_group(DevGeo g, DevRays R, int nr, int nfresh,
+                                                    const int* __restrict__ vkey, int* __restrict__ ttab, uint64_t tmask,
+                                                    int* __restrict__ prim_of, uint8_t* __restrict__ primary) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nr) return;
     int tk[D + 1];
-    if (!hv_tent_key<D>(g, R, vkey, t, tk)) { prim_of[t] = t; primary[t] = 1; return; }
+    // carried rays (t >= nfresh) already waited once: scanned, and they claim no key
+    if (t >= nfresh || !hv_tent_key<D>(g, R, vkey, t, tk)) { prim_of[t] = t; primary[t] = 1; return; }
     uint64_t h = hv_hash_ints<D + 1>(tk) & tmask;
     for (uint64_t probe = 0; probe <= tmask; ++probe) {
         int cur = ttab[h];
@@ -1620,7 +1628,7 @@ __global__ __launch_bounds__(256) void k_tent_group(DevGeo g, DevRays R, int nr,
             cur = old;
         }
         int ok[D + 1];
-        hv_tent_key<D>(g, R, vkey, cur, ok);
+        hv_tent_key<D>(g, R, vkey, cur, ok);   // a claimer is a fresh ray with a key
         if (keys_equal<D>(tk, ok)) { prim_of[t] = cur; primary[t] = 0; return; }
         h = (h + 1) & tmask;
     }
@@ -1634,28 +1642,42 @@ __global__ void k_gather_flags(const int* __restrict__ ord, int nr, const uint8_
     if (i < nr) keep[i] = by_ray[ord[i]];
 }
 
-__global__ void k_scatter_results(const int* __restrict__ list, int n, const int* __restrict__ status,
-                                  const int* __restrict__ win, int* __restrict__ st_by_ray, int* __restrict__ win_by_ray) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int ray = list[i];
-    st_by_ray[ray] = status[i];
-    win_by_ray[ray] = win[i];
+// Rays that share a tentative end point with an earlier ray of the level wait: after the
+// level's vertices are registered, a waiting ray whose edge got its second end point is done
+// (the sequential walk would not walk that edge either); the others are carried into the
+// next level's rays, where they are scanned.
+template <int D>
+__global__ void k_carry_out(DevRays R, int nr, const uint8_t* __restrict__ primary,
+                            const unsigned long long* __restrict__ etab, int64_t* __restrict__ cs, int* __restrict__ cv,
+                            int* __restrict__ cd, double* __restrict__ cu, double* __restrict__ cc,
+                            double* __restrict__ cdu, int* __restrict__ count) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool carry = t < nr && !primary[t] && !(etab[R.slot[t]] & HV_EDGE_DONE);
+    const int o = hv_wave_alloc(count, carry);
+    if (!carry) return;
+    cs[o] = R.slot[t];
+    cv[o] = R.v[t];
+    cd[o] = R.drop[t];
+#pragma unroll
+    for (int k = 0; k < D; ++k) cu[(int64_t)o * D + k] = R.u[(int64_t)t * D + k];
+    cc[o] = R.c[t];
+    cdu[o] = R.du[t];
 }
 
-// secondary rays: resolved when the primary found exactly its tentative key
 template <int D>
-__global__ void k_secondary(DevGeo g, DevRays R, int nr, const int* __restrict__ prim_of,
-                            const uint8_t* __restrict__ primary, const int* __restrict__ st_by_ray,
-                            const int* __restrict__ win_by_ray, uint8_t* __restrict__ rescan) {
+__global__ void k_carry_in(const int64_t* __restrict__ cs, const int* __restrict__ cv, const int* __restrict__ cd,
+                           const double* __restrict__ cu, const double* __restrict__ cc, const double* __restrict__ cdu,
+                           int n, DevRays R, int base) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nr) return;
-    if (primary[t]) { rescan[t] = 0; return; }
-    const int p = prim_of[t];
-    const int i1 = R.i1[p];
-    const int tent = i1 < g.N ? g.sidx[i1] : i1;
-    const bool confirmed = st_by_ray[p] == HV_ST_FOUND && win_by_ray[p] == tent;
-    rescan[t] = confirmed ? 0 : 1;
+    if (t >= n) return;
+    const int o = base + t;
+    R.slot[o] = cs[t];
+    R.v[o] = cv[t];
+    R.drop[o] = cd[t];
+#pragma unroll
+    for (int k = 0; k < D; ++k) R.u[(int64_t)o * D + k] = cu[(int64_t)t * D + k];
+    R.c[o] = cc[t];
+    R.du[o] = cdu[t];
 }
 
 // Chunk selection (64-ray blocks interleaved over K chunks); a ray of a later chunk is
@@ -1686,7 +1708,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         return hv_fail(c, HV_ERR_UNSUPPORTED, "the mixed-precision scan uses leaves of 32 points");
     const int L = opt.leaf_size > 0 ? opt.leaf_size : 32;
     const double plane_tol = opt.plane_tolerance > 0 ? opt.plane_tolerance : 1e-12;
-    const int KCH = opt.reserved0 > 1 ? opt.reserved0 : 1;   // chunks per level (<0: tentative-key grouping)
+    const int KCH = opt.reserved0 > 1 ? opt.reserved0 : 1;   // chunks per level (0: tentative-key grouping)
     int rc = check_box(c, D);
     if (rc) return rc;
     const int N = (int)c->N;
@@ -1818,9 +1840,10 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     int64_t fbeg = 0, fcount = nlive;   // frontier = ids[fbeg .. fbeg+fcount)
     double ms_ray = 0.0;
     int64_t levels = 0, skipped = 0;
-    while (fcount > 0) {
+    int ncar = 0;   // rays carried from the previous level (waiting rays whose edge stayed open)
+    while (fcount > 0 || ncar > 0) {
         if (++levels > 1000000) return hv_fail(c, HV_ERR_CAPACITY, "walk did not terminate");
-        const int64_t nslots = fcount * (D + 1);
+        const int64_t nslots = fcount * (D + 1) + ncar;
         if (nslots > (int64_t)INT32_MAX) return hv_fail(c, HV_ERR_CAPACITY, "level too large");
         DevRays R;
         R.slot = pget<int64_t>(c, "ray_slot", nslots);
@@ -1858,6 +1881,16 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         rc = read_ctr();
         if (rc) return rc;
         if (hctr[5]) return hv_fail(c, HV_ERR_CAPACITY, "edge hash overflow");
+        const int nfresh = nr;
+        if (ncar > 0) {
+            k_carry_in<D><<<nblk(ncar, 256), 256, 0, c->stream>>>(
+                (const int64_t*)c->pool->get("car_slot", 0), (const int*)c->pool->get("car_v", 0),
+                (const int*)c->pool->get("car_drop", 0), (const double*)c->pool->get("car_u", 0),
+                (const double*)c->pool->get("car_c", 0), (const double*)c->pool->get("car_du", 0), ncar, R, nr);
+            HVCK(hipGetLastError());
+            nr += ncar;
+            ncar = 0;
+        }
         if (nr == 0) break;
         // start phase, then sort rays by the Morton code of their certificate-ball centre
         k_ray_init<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, nr, vkey, vx, plane_tol);
@@ -1923,20 +1956,24 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             HVCK(hipStreamSynchronize(c->stream));
             return HV_OK;
         };
-        if (opt.reserved0 < 0) {
-            // tentative-key grouping: primaries first, unconfirmed secondaries after
+        if (opt.reserved0 <= 0) {
+            // one scan per tentative end point (the ray start's best candidate); the other
+            // rays of that end point wait and are carried to the next level if still open
             uint64_t tsz = 1024;
             while ((int64_t)tsz < 2 * (int64_t)nr) tsz <<= 1;
             int* ttab = pget<int>(c, "ttab", tsz);
             int* prim_of = pget<int>(c, "prim_of", nr);
             uint8_t* pflag = pget<uint8_t>(c, "pflag", nr);
-            uint8_t* rescan = pget<uint8_t>(c, "rescan", nr);
-            int* st_by_ray = pget<int>(c, "st_by_ray", nr);
-            int* win_by_ray = pget<int>(c, "win_by_ray", nr);
-            if (!ttab || !prim_of || !pflag || !rescan || !st_by_ray || !win_by_ray)
+            int64_t* cs = pget<int64_t>(c, "car_slot", nr);
+            int* cv = pget<int>(c, "car_v", nr);
+            int* cd = pget<int>(c, "car_drop", nr);
+            double* cu = pget<double>(c, "car_u", (size_t)nr * D);
+            double* cc = pget<double>(c, "car_c", nr);
+            double* cdu = pget<double>(c, "car_du", nr);
+            if (!ttab || !prim_of || !pflag || !cs || !cv || !cd || !cu || !cc || !cdu)
                 return hv_fail(c, HV_ERR_DEVICE, "out of device memory (groups)");
             HVCK(hipMemsetAsync(ttab, 0xff, tsz * sizeof(int), c->stream));
-            k_tent_group<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, nr, vkey, ttab, tsz - 1, prim_of, pflag);
+            k_tent_group<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, nr, nfresh, vkey, ttab, tsz - 1, prim_of, pflag);
             HVCK(hipGetLastError());
             k_gather_flags<<<nblk(nr, 256), 256, 0, c->stream>>>(ord, nr, pflag, keep);
             HVCK(hipGetLastError());
@@ -1946,21 +1983,13 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             if (np) {
                 rc = process(ordc, np);
                 if (rc) return rc;
-                k_scatter_results<<<nblk(np, 256), 256, 0, c->stream>>>(ordc, np, status, rwin, st_by_ray, win_by_ray);
-                HVCK(hipGetLastError());
             }
-            k_secondary<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, nr, prim_of, pflag, st_by_ray, win_by_ray, rescan);
+            HVCK(hipMemsetAsync(dcount, 0, sizeof(int), c->stream));
+            k_carry_out<D><<<nblk(nr, 256), 256, 0, c->stream>>>(R, nr, pflag, etab, cs, cv, cd, cu, cc, cdu, dcount);
             HVCK(hipGetLastError());
-            k_gather_flags<<<nblk(nr, 256), 256, 0, c->stream>>>(ord, nr, rescan, keep);
-            HVCK(hipGetLastError());
-            int ns = 0;
-            rc = select(ord, keep, nr, cand, ns);
-            if (rc) return rc;
-            if (ns) {
-                rc = process(cand, ns);
-                if (rc) return rc;
-            }
-            skipped += nr - np - ns;
+            HVCK(hipMemcpyAsync(&ncar, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HVCK(hipStreamSynchronize(c->stream));
+            skipped += (int64_t)(nr - np) - ncar;
         } else {
             // interleaved chunks: 64-ray blocks over K chunks, edge registration after each
             const int K = std::min(KCH, nr);
