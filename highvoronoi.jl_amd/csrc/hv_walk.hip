@@ -965,14 +965,11 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                                                   unsigned long long* __restrict__ ctr) {
     __shared__ int stk[128], stkl[128];
     __shared__ float cbx[16 * D];
-    __shared__ unsigned long long hms[8];
-    __shared__ float dcs[8];
     // MODE 0: the staged leaf (FP64).  MODE 1: ray K-vectors of the current node (MFMA B
     // operand source, 80-byte rows) and, per ray, which points of the current leaf it
     // already holds as candidates.
     __shared__ double leafx[D + 1][MODE == 0 ? 64 : 1];
     __shared__ hv_u32x4 rk4[MODE == 1 ? 64 : 1][5];
-    __shared__ uint32_t kmk[MODE == 1 ? 64 : 1];
     __shared__ int kid[MODE == 1 ? 64 : 1][D];
     __shared__ double nog[MODE == 1 ? 8 : 1];
     const int lane = threadIdx.x & 63;
@@ -1063,16 +1060,17 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
         }
         return d2;
     };
-    // nearest (or farthest) child still in `rem`, by distance to the packet centre
+    // nearest (or farthest) child still in `rem`, by distance to the packet centre (uniform
+    // float bits: non-negative floats order as unsigned integers)
+    uint32_t key[8];
     auto pick = [&](unsigned rem, bool nearest) {
         int best = -1;
-        float bd = 0.f;
+        uint32_t bd = 0u;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const float v = dcs[c];
             const bool in = (rem >> c) & 1u;
-            const bool better = best < 0 || (nearest ? v < bd : v > bd);
-            if (in && better) { best = c; bd = v; }
+            const bool better = best < 0 || (nearest ? key[c] < bd : key[c] > bd);
+            if (in && better) { best = c; bd = key[c]; }
         }
         return best;
     };
@@ -1120,16 +1118,21 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
             prefetch(nx, nl);
         }
         hv_wave_sync();
-        unsigned rem = 0u;
+        // children each lane's ball reaches (kept per lane), children some ball reaches
+        unsigned rem = 0u, mine = 0u;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const unsigned long long m = __ballot(act && (child_d2f(c, bcf) <= R2f));
-            if (lane == 0) hms[c] = m;
-            rem |= (m != 0ull) ? (1u << c) : 0u;
+            const bool h = act && (child_d2f(c, bcf) <= R2f);
+            mine |= h ? (1u << c) : 0u;
+            rem |= (__ballot(h) != 0ull) ? (1u << c) : 0u;
         }
-        if (lane < 8) dcs[lane] = child_d2f(lane, pcf);
+        {
+            const uint32_t kb = __float_as_uint(child_d2f(lane & 7, pcf));
+#pragma unroll
+            for (int c = 0; c < 8; ++c) key[c] = __builtin_amdgcn_readlane(kb, c);
+        }
+        const double tnode = tball;   // a ball that shrinks later in the node is re-tested
         rem = __builtin_amdgcn_readfirstlane(rem);
-        hv_wave_sync();
         if (rem == 0u) continue;
         int off = 0, pw = 1;
         for (int q = 0; q < l; ++q) { off += pw; pw *= 8; }
@@ -1147,7 +1150,7 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                     const int leaf = 8 * i + cc;
                     const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
                     // balls only shrink: re-test with the current one
-                    const bool hit = act && ((hms[cc] >> lane) & 1ull) && (child_d2f(cc, bcf) <= R2f);
+                    const bool hit = act && ((mine >> cc) & 1u) && (tball == tnode || child_d2f(cc, bcf) <= R2f);
                     const unsigned long long m = __ballot(hit);
                     if (m && lane < npts) {
 #pragma unroll
@@ -1236,14 +1239,12 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                     if (cn >= 0) rem &= ~(1u << cn);
                     const int leaf = 8 * i + cc;
                     const int j0 = leaf * 32, npts = max(0, min(32, N - j0));
-                    const bool hit = act && ((hms[cc] >> lane) & 1ull) && (child_d2f(cc, bcf) <= R2f);
+                    const bool hit = act && ((mine >> cc) & 1u) && (tball == tnode || child_d2f(cc, bcf) <= R2f);
                     const unsigned long long m = __ballot(hit);
                     const hv_u32x4 ca0 = pa0, ca1 = pa1;
                     if (cn >= 0) load_leaf1(8 * i + cn, pa0, pa1);
                     if (m) {
                         leaves += 1;
-                        kmk[lane] = leaf_known(leaf);
-                        hv_wave_sync();
 #pragma unroll
                         for (int g4 = 0; g4 < 4; ++g4) {
                             if (((m >> (16 * g4)) & 0xffffull) == 0ull) continue;
@@ -1260,8 +1261,11 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                                 const int rho = 16 * g4 + (lane & 15);
                                 const int sh = 16 * t + 4 * (lane >> 4);
                                 unsigned real = 0u;
+                                // which points of this leaf each ray already holds, from its lane
+                                const uint32_t kmine = leaf_known(leaf);
+                                const uint32_t kcol = (uint32_t)__shfl((int)kmine, rho);
                                 if ((int)ob < 0) {
-                                    const uint32_t known = kmk[rho] >> sh;
+                                    const uint32_t known = kcol >> sh;
 #pragma unroll
                                     for (int e = 0; e < 4; ++e)
                                         if ((int)__float_as_uint(acc[e]) < 0 && !((known >> e) & 1u) && sh + e < npts)
@@ -1288,13 +1292,11 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                                             }
                                         }
                                         if (T.t1 < before) set_ball(T.t1 + HV_W);
-                                        kmk[lane] = leaf_known(leaf);
                                         flagged = true;
                                     }
                                 }
                             }
                         }
-                        hv_wave_sync();
                     }
                     cc = cn;
                 }

@@ -50,9 +50,10 @@ typedef struct hv_opts {
     int32_t precision;        /* HV_PREC_* */
     int32_t leaf_size;        /* generators per tree leaf (0 = default 32; HV_PREC_BF16 needs 32) */
     int32_t rays_per_block;   /* ignored: the scan works on packets of 64 rays, one wavefront each */
-    int32_t reserved0;        /* walk-level schedule: 0 or 1 = every ray of a level in one scan
-                                 (default); >1 = that many chunks, edges registered after each;
-                                 <0 = rays grouped by tentative key, one scan per group first */
+    int32_t reserved0;        /* walk-level schedule: <= 0 (default) = one scan per tentative end
+                                 point, the other rays of that end point wait and are carried to
+                                 the next level if their edge stays open; 1 = every ray of a level
+                                 in one scan; > 1 = that many chunks, edges registered after each */
     double plane_tolerance;   /* src/raycast-types.jl:229 (0 = 1e-12) */
     double ray_tol;           /* src/raycast-types.jl:230; ignored (it acts on non-general vertices,
                                  which the device walk reports as HV_ERR_DEGENERATE) */
