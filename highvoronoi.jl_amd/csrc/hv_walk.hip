@@ -583,6 +583,19 @@ __global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, 
     }
     const int idx = hv_wave_alloc(nrays, want);
     if (!want) return;
+    R.slot[idx] = h;
+    R.v[idx] = v;
+    R.drop[idx] = pos;
+}
+
+// Direction, half-space threshold and orthogonality defect of rays [0, n) (dense: one
+// thread per emitted ray).
+template <int D>
+__global__ __launch_bounds__(256) void k_ray_dir(DevGeo g, DevRays R, int n, const int* __restrict__ vkey,
+                                                 const double* __restrict__ vx, double plane_tol) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int v = R.v[t], pos = R.drop[t];
     // key read straight from global memory: a private copy indexed by `pos` would live in scratch
     const int* key = vkey + (int64_t)v * (D + 1);
     double r[D];
@@ -590,13 +603,10 @@ __global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, 
     for (int k = 0; k < D; ++k) r[k] = vx[(int64_t)v * D + k];
     double u[D], c, du, rho;
     hv_ray_setup<D>(g, key, pos, r, plane_tol, u, c, du, rho);
-    R.slot[idx] = h;
-    R.v[idx] = v;
-    R.drop[idx] = pos;
 #pragma unroll
-    for (int k = 0; k < D; ++k) R.u[(int64_t)idx * D + k] = u[k];
-    R.c[idx] = c;
-    R.du[idx] = du;
+    for (int k = 0; k < D; ++k) R.u[(int64_t)t * D + k] = u[k];
+    R.c[t] = c;
+    R.du[t] = du;
     (void)rho;
 }
 
@@ -1884,6 +1894,10 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         if (rc) return rc;
         if (hctr[5]) return hv_fail(c, HV_ERR_CAPACITY, "edge hash overflow");
         const int nfresh = nr;
+        if (nfresh > 0) {
+            k_ray_dir<D><<<nblk(nfresh, 256), 256, 0, c->stream>>>(g, R, nfresh, vkey, vx, plane_tol);
+            HVCK(hipGetLastError());
+        }
         if (ncar > 0) {
             k_carry_in<D><<<nblk(ncar, 256), 256, 0, c->stream>>>(
                 (const int64_t*)c->pool->get("car_slot", 0), (const int*)c->pool->get("car_v", 0),
