@@ -343,12 +343,61 @@ __global__ void k_rehash_vertices(const int* __restrict__ ids, int64_t n, const 
 // ---------------------------------------------------------------------------------
 // edges: registration (pushedge! semantics) and ray emission
 
+// Edge bookkeeping, per level and in two small tables instead of one table of every edge
+// (which would be a random access into ~1 GB per lookup):
+//   ray-edge table:  the edges of this level's rays (value: ray index + 1);
+//   new-edge table:  the edges of this level's new vertices (value: (vertex << 4 | pos) + 1).
+// An edge of a new vertex is either the edge of a ray of this level (its other end point is
+// that ray's source: closed, and the ray's entry is marked), or shared with another new
+// vertex of the level (closed), or open: then its owner emits a ray in the next level.  An
+// open edge of an older vertex always has a ray (scanned, waiting or carried) in the level
+// that finds its other end point, so nothing older needs to be kept.  Slots: bits 0..34 the
+// value, bits 35..61 a fingerprint of the edge, bit 62 closed.
+template <int D>
+__global__ __launch_bounds__(256) void k_rt_build(DevRays R, int nr, const int* __restrict__ vkey,
+                                                  unsigned long long* __restrict__ rt, uint64_t rmask,
+                                                  unsigned long long* __restrict__ ctr) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool full = false;
+    if (t < nr) {
+        int e[D];
+        hv_edge_of<D>(vkey + (int64_t)R.v[t] * (D + 1), R.drop[t], e);
+        const uint64_t hfull = hv_hash_ints<D>(e);
+        const unsigned long long fp = HV_EDGE_FP(hfull);
+        const unsigned long long mine = fp | ((unsigned long long)t + 1ull);
+        uint64_t h = hfull & rmask;
+        int64_t out = -1;
+        for (uint64_t probe = 0; probe <= rmask; ++probe) {
+            unsigned long long cur = rt[h];
+            if (cur == 0ull) {
+                const unsigned long long old = atomicCAS(&rt[h], 0ull, mine);
+                if (old == 0ull) { out = (int64_t)h; break; }
+                cur = old;
+            }
+            if ((cur & HV_EDGE_FMASK) == fp) {
+                const int r2 = (int)((cur & HV_EDGE_VMASK) - 1ull);
+                int e2[D];
+                hv_edge_of<D>(vkey + (int64_t)R.v[r2] * (D + 1), R.drop[r2], e2);
+                bool eq = true;
+#pragma unroll
+                for (int j = 0; j < D; ++j) eq = eq && (e[j] == e2[j]);
+                if (eq) { out = (int64_t)h; break; }
+            }
+            h = (h + 1) & rmask;
+        }
+        full = out < 0;
+        R.slot[t] = out < 0 ? 0 : out;
+    }
+    hv_wave_count(&ctr[1], full);
+}
+
 template <int D>
 __global__ __launch_bounds__(256) void k_register(const int* __restrict__ frontier, int nf, const int* __restrict__ vkey,
-                                                  int N, unsigned long long* __restrict__ etab, uint64_t emask,
+                                                  int N, DevRays R, unsigned long long* __restrict__ rt, uint64_t rmask,
+                                                  unsigned long long* __restrict__ nt, uint64_t nmask,
                                                   int64_t* __restrict__ reg, unsigned long long* __restrict__ ctr) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool fresh = false, full = false;
+    bool full = false;
     if (t < nf * (D + 1)) {
         const int fi = t / (D + 1), pos = t % (D + 1);
         const int v = frontier[fi];
@@ -359,98 +408,61 @@ __global__ __launch_bounds__(256) void k_register(const int* __restrict__ fronti
         if (e[0] < N) {
             const uint64_t hfull = hv_hash_ints<D>(e);
             const unsigned long long fp = HV_EDGE_FP(hfull);
-            const unsigned long long mine = fp | (((((unsigned long long)v) << 4) | (unsigned)pos) + 1ull);
-            uint64_t h = hfull & emask;
-            out = -2;
-            for (uint64_t probe = 0; probe <= emask; ++probe) {
-                unsigned long long cur = etab[h];
-                if (cur == 0ull) {
-                    const unsigned long long old = atomicCAS(&etab[h], 0ull, mine);
-                    if (old == 0ull) { out = (int64_t)h; fresh = true; break; }
-                    cur = old;
-                }
-                // another edge's slot unless the fingerprints agree (then compare exactly)
-                if ((cur & HV_EDGE_FMASK) != fp) { h = (h + 1) & emask; continue; }
-                const unsigned long long raw = (cur & HV_EDGE_VMASK) - 1ull;
-                int e2[D];
-                hv_edge_of<D>(vkey + (int64_t)(raw >> 4) * (D + 1), (int)(raw & 15ull), e2);
-                bool eq = true;
+            // the edge of a ray of this level: its other end point is the ray's source
+            bool ray_edge = false;
+            {
+                uint64_t h = hfull & rmask;
+                for (uint64_t probe = 0; probe <= rmask; ++probe) {
+                    const unsigned long long cur = rt[h];
+                    if (cur == 0ull) break;
+                    if ((cur & HV_EDGE_FMASK) == fp) {
+                        const int r2 = (int)((cur & HV_EDGE_VMASK) - 1ull);
+                        int e2[D];
+                        hv_edge_of<D>(vkey + (int64_t)R.v[r2] * (D + 1), R.drop[r2], e2);
+                        bool eq = true;
 #pragma unroll
-                for (int j = 0; j < D; ++j) eq = eq && (e[j] == e2[j]);
-                if (eq) {
-                    atomicOr(&etab[h], HV_EDGE_DONE);
-                    out = (int64_t)h;
-                    break;
+                        for (int j = 0; j < D; ++j) eq = eq && (e[j] == e2[j]);
+                        if (eq) {
+                            atomicOr(&rt[h], HV_EDGE_DONE);
+                            ray_edge = true;
+                            break;
+                        }
+                    }
+                    h = (h + 1) & rmask;
                 }
-                h = (h + 1) & emask;
             }
-            full = (out == -2);
+            if (!ray_edge) {
+                const unsigned long long mine = fp | (((((unsigned long long)v) << 4) | (unsigned)pos) + 1ull);
+                uint64_t h = hfull & nmask;
+                out = -2;
+                for (uint64_t probe = 0; probe <= nmask; ++probe) {
+                    unsigned long long cur = nt[h];
+                    if (cur == 0ull) {
+                        const unsigned long long old = atomicCAS(&nt[h], 0ull, mine);
+                        if (old == 0ull) { out = (int64_t)h; break; }
+                        cur = old;
+                    }
+                    // another edge's slot unless the fingerprints agree (then compare exactly)
+                    if ((cur & HV_EDGE_FMASK) != fp) { h = (h + 1) & nmask; continue; }
+                    const unsigned long long raw = (cur & HV_EDGE_VMASK) - 1ull;
+                    int e2[D];
+                    hv_edge_of<D>(vkey + (int64_t)(raw >> 4) * (D + 1), (int)(raw & 15ull), e2);
+                    bool eq = true;
+#pragma unroll
+                    for (int j = 0; j < D; ++j) eq = eq && (e[j] == e2[j]);
+                    if (eq) {
+                        atomicOr(&nt[h], HV_EDGE_DONE);
+                        out = (int64_t)h;
+                        break;
+                    }
+                    h = (h + 1) & nmask;
+                }
+                full = (out == -2);
+            }
         }
         reg[t] = out;
     }
-    hv_wave_count(&ctr[0], fresh);
     hv_wave_count(&ctr[1], full);
-}
-
-template <int D>
-__global__ void k_rehash_edges(const unsigned long long* __restrict__ old, uint64_t omask, const int* __restrict__ vkey,
-                               unsigned long long* __restrict__ etab, uint64_t emask) {
-    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t > omask) return;
-    unsigned long long cur = old[t];
-    if (cur == 0ull) return;
-    const unsigned long long raw = (cur & HV_EDGE_VMASK) - 1ull;
-    int e[D];
-    hv_edge_of<D>(vkey + (int64_t)(raw >> 4) * (D + 1), (int)(raw & 15ull), e);
-    uint64_t h = hv_hash_ints<D>(e) & emask;
-    for (;;) {
-        if (atomicCAS(&etab[h], 0ull, cur) == 0ull) return;
-        h = (h + 1) & emask;
-    }
-}
-
-// After the edge hash grew, stored slot indices are stale: find each edge again.
-template <int D>
-__device__ __forceinline__ int64_t hv_find_edge(const int* __restrict__ vkey, const unsigned long long* __restrict__ etab,
-                                                uint64_t emask, const int* e) {
-    const uint64_t full = hv_hash_ints<D>(e);
-    const unsigned long long fp = HV_EDGE_FP(full);
-    uint64_t h = full & emask;
-    for (uint64_t probe = 0; probe <= emask; ++probe) {
-        const unsigned long long cur = etab[h];
-        if (cur == 0ull) return -2;
-        if ((cur & HV_EDGE_FMASK) != fp) { h = (h + 1) & emask; continue; }
-        const unsigned long long raw = (cur & HV_EDGE_VMASK) - 1ull;
-        int e2[D];
-        hv_edge_of<D>(vkey + (int64_t)(raw >> 4) * (D + 1), (int)(raw & 15ull), e2);
-        bool eq = true;
-#pragma unroll
-        for (int j = 0; j < D; ++j) eq = eq && (e[j] == e2[j]);
-        if (eq) return (int64_t)h;
-        h = (h + 1) & emask;
-    }
-    return -2;
-}
-
-template <int D>
-__global__ void k_relookup_vertices(const int* __restrict__ ids, int count, const int* __restrict__ vkey,
-                                    const unsigned long long* __restrict__ etab, uint64_t emask, int64_t* __restrict__ reg) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count * (D + 1)) return;
-    if (reg[t] < 0) return;
-    int e[D];
-    hv_edge_of<D>(vkey + (int64_t)ids[t / (D + 1)] * (D + 1), t % (D + 1), e);
-    reg[t] = hv_find_edge<D>(vkey, etab, emask, e);
-}
-
-template <int D>
-__global__ void k_relookup_rays(DevRays R, int nr, const int* __restrict__ vkey,
-                                const unsigned long long* __restrict__ etab, uint64_t emask) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nr) return;
-    int e[D];
-    hv_edge_of<D>(vkey + (int64_t)R.v[t] * (D + 1), R.drop[t], e);
-    R.slot[t] = hv_find_edge<D>(vkey, etab, emask, e);
 }
 
 // Direction of the walk along the edge key\{key[pos]} (u_qr, src/tools.jl:773-790):
@@ -563,7 +575,7 @@ __device__ __forceinline__ void hv_ray_setup(const DevGeo& g, const int* key, in
 template <int D>
 __global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, int nf, const int* __restrict__ vkey,
                                               const double* __restrict__ vx, DevGeo g,
-                                              const unsigned long long* __restrict__ etab,
+                                              const unsigned long long* __restrict__ nt,
                                               const int64_t* __restrict__ reg, DevRays R, int* __restrict__ nrays,
                                               double plane_tol) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -573,7 +585,7 @@ __global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, 
     if (t < nf * (D + 1)) {
         h = reg[t];
         if (h >= 0) {
-            const unsigned long long cur = etab[h];
+            const unsigned long long cur = nt[h];
             const int fi = t / (D + 1);
             pos = t % (D + 1);
             v = frontier[fi];
@@ -1660,11 +1672,11 @@ __global__ void k_gather_flags(const int* __restrict__ ord, int nr, const uint8_
 // next level's rays, where they are scanned.
 template <int D>
 __global__ void k_carry_out(DevRays R, int nr, const uint8_t* __restrict__ primary,
-                            const unsigned long long* __restrict__ etab, int64_t* __restrict__ cs, int* __restrict__ cv,
+                            const unsigned long long* __restrict__ rt, int64_t* __restrict__ cs, int* __restrict__ cv,
                             int* __restrict__ cd, double* __restrict__ cu, double* __restrict__ cc,
                             double* __restrict__ cdu, int* __restrict__ count) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool carry = t < nr && !primary[t] && !(etab[R.slot[t]] & HV_EDGE_DONE);
+    const bool carry = t < nr && !primary[t] && !(rt[R.slot[t]] & HV_EDGE_DONE);
     const int o = hv_wave_alloc(count, carry);
     if (!carry) return;
     cs[o] = R.slot[t];
@@ -1696,7 +1708,7 @@ __global__ void k_carry_in(const int64_t* __restrict__ cs, const int* __restrict
 // dropped when its edge got its second end point from a vertex found by an earlier
 // chunk of the same level (the sequential walk's edge hash would not walk it either).
 __global__ void k_chunk(const int* __restrict__ ord, int nr, int c, int K, const int64_t* __restrict__ slot,
-                        const unsigned long long* __restrict__ etab, int* __restrict__ cand,
+                        const unsigned long long* __restrict__ rt, int* __restrict__ cand,
                         uint8_t* __restrict__ keep) {
     // chunk c holds the sorted 64-ray blocks c, c+K, c+2K, ...: packets stay spatially
     // coherent while rays that end at the same vertex (neighbours in the sort) tend to
@@ -1706,7 +1718,7 @@ __global__ void k_chunk(const int* __restrict__ ord, int nr, int c, int K, const
     if (pos >= nr) return;
     const int ray = ord[pos];
     cand[i] = ray;
-    keep[i] = (c == 0) ? 1 : ((etab[slot[ray]] & HV_EDGE_DONE) ? 0 : 1);
+    keep[i] = (c == 0) ? 1 : ((rt[slot[ray]] & HV_EDGE_DONE) ? 0 : 1);
 }
 
 template <int D>
@@ -1736,18 +1748,15 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     int64_t vcap = (int64_t)(2.0 * est) + 4 * nseed + 1024;
     auto hpow = [](int64_t n) { uint64_t p = 1024; while ((int64_t)p < n) p <<= 1; return p; };
     uint64_t hsize = hpow((int64_t)(2.4 * est) + 4 * nseed);
-    uint64_t esize = hpow((int64_t)((D + 1) * 1.2 * est) + 16 * nseed);
     int* vkey = pget<int>(c, "vkey", (size_t)vcap * (D + 1));
     double* vx = pget<double>(c, "vx", (size_t)vcap * D);
     int* ids = pget<int>(c, "ids", (size_t)vcap);
     int* htab = pget<int>(c, "htab", hsize);
-    unsigned long long* etab = pget<unsigned long long>(c, "etab", esize);
     unsigned long long* ctr = pget<unsigned long long>(c, "ctr", 16);
     int* dcount = pget<int>(c, "dcount", 4);
-    if (!vkey || !vx || !ids || !htab || !etab || !ctr || !dcount)
+    if (!vkey || !vx || !ids || !htab || !ctr || !dcount)
         return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tables)");
     HVCK(hipMemsetAsync(htab, 0xff, hsize * sizeof(int), c->stream));
-    HVCK(hipMemsetAsync(etab, 0, esize * sizeof(unsigned long long), c->stream));
     HVCK(hipMemsetAsync(ctr, 0, 16 * sizeof(unsigned long long), c->stream));
     hv_stats_t st{};
     unsigned long long hctr[16];
@@ -1760,25 +1769,23 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         HVCK(hipStreamSynchronize(c->stream));
         return HV_OK;
     };
-    // edge hash below half load for `incoming` more registrations (sets `grown` on rehash)
-    bool grown = false;
-    auto ensure_edges = [&](int64_t incoming) -> int {
-        int r_ = read_ctr();
-        if (r_) return r_;
-        if ((int64_t)(hctr[4] + incoming) * 2 <= (int64_t)esize) return HV_OK;
-        grown = true;
-        uint64_t nsz = esize;
-        while ((int64_t)(hctr[4] + incoming) * 2 > (int64_t)nsz) nsz <<= 1;
-        DBuf old = c->pool->take("etab");
-        unsigned long long* ne = pget<unsigned long long>(c, "etab", nsz);
-        if (!ne) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (edge hash)");
-        HVCK(hipMemsetAsync(ne, 0, nsz * sizeof(unsigned long long), c->stream));
-        k_rehash_edges<D><<<nblk(esize, 256), 256, 0, c->stream>>>((unsigned long long*)old.p, esize - 1, vkey, ne, nsz - 1);
-        HVCK(hipGetLastError());
-        HVCK(hipStreamSynchronize(c->stream));
-        HVCK(hipFree(old.p));
-        etab = ne;
-        esize = nsz;
+    // per-level edge tables (see k_register): the ray-edge table of the current level and the
+    // new-edge table, cleared at the first registration of a level (after that level's rays
+    // were emitted from it)
+    unsigned long long* rtab = nullptr;
+    uint64_t rsize = 0;
+    unsigned long long* ntab = nullptr;
+    uint64_t nsize = 0;
+    bool nt_fresh = true;   // the next registration starts a new level's table
+    auto build_rt = [&](const DevRays& R, int nr) -> int {
+        rsize = hpow(2 * (int64_t)nr + 64);
+        rtab = pget<unsigned long long>(c, "rtab", rsize);
+        if (!rtab) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (ray-edge table)");
+        HVCK(hipMemsetAsync(rtab, 0, rsize * sizeof(unsigned long long), c->stream));
+        if (nr > 0) {
+            k_rt_build<D><<<nblk(nr, 256), 256, 0, c->stream>>>(R, nr, vkey, rtab, rsize - 1, ctr + 4);
+            HVCK(hipGetLastError());
+        }
         return HV_OK;
     };
     // vertex table and vertex hash room for `incoming` more candidates
@@ -1823,11 +1830,19 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         HVCK(hipStreamSynchronize(c->stream));
         return HV_OK;
     };
-    auto register_new = [&](int64_t first, int count, int64_t* reg) -> int {
-        int r_ = ensure_edges((int64_t)count * (D + 1));
-        if (r_) return r_;
-        k_register<D><<<nblk((int64_t)count * (D + 1), 256), 256, 0, c->stream>>>(ids + first, count, vkey, N, etab,
-                                                                                  esize - 1, reg, ctr + 4);
+    // edges of new vertices ids[first .. first+count); `bound` caps the new vertices the
+    // level may still register (sizes the new-edge table at the level's first registration)
+    auto register_new = [&](const DevRays& R, int64_t first, int count, int64_t bound, int64_t* reg) -> int {
+        if (nt_fresh) {
+            nsize = hpow(2 * (D + 1) * std::max<int64_t>(bound, count) + 64);
+            ntab = pget<unsigned long long>(c, "ntab", nsize);
+            if (!ntab) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (new-edge table)");
+            HVCK(hipMemsetAsync(ntab, 0, nsize * sizeof(unsigned long long), c->stream));
+            nt_fresh = false;
+        }
+        if (count == 0) return HV_OK;
+        k_register<D><<<nblk((int64_t)count * (D + 1), 256), 256, 0, c->stream>>>(
+            ids + first, count, vkey, N, R, rtab, rsize - 1, ntab, nsize - 1, reg, ctr + 4);
         HVCK(hipGetLastError());
         return HV_OK;
     };
@@ -1843,7 +1858,10 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         int nsel = 0;
         rc = insert_and_select(status, nseed, vnext, nsel);
         if (rc) return rc;
-        rc = register_new(nlive, nsel, reg_cur);
+        DevRays none{};
+        rc = build_rt(none, 0);
+        if (rc) return rc;
+        rc = register_new(none, nlive, nsel, nsel, reg_cur);
         if (rc) return rc;
         vnext += nseed;
         nlive += nsel;
@@ -1885,7 +1903,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             !ordc || !keep || !status || !rwin || !rt || !reg_next)
             return hv_fail(c, HV_ERR_DEVICE, "out of device memory (rays)");
         HVCK(hipMemsetAsync(dcount, 0, sizeof(int), c->stream));
-        k_emit<D><<<nblk(nslots, 256), 256, 0, c->stream>>>(ids + fbeg, (int)fcount, vkey, vx, g, etab, reg_cur, R,
+        k_emit<D><<<nblk(nslots, 256), 256, 0, c->stream>>>(ids + fbeg, (int)fcount, vkey, vx, g, ntab, reg_cur, R,
                                                            dcount, plane_tol);
         HVCK(hipGetLastError());
         int nr = 0;
@@ -1908,6 +1926,9 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             ncar = 0;
         }
         if (nr == 0) break;
+        rc = build_rt(R, nr);
+        if (rc) return rc;
+        nt_fresh = true;
         // start phase, then sort rays by the Morton code of their certificate-ball centre
         k_ray_init<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, nr, vkey, vx, plane_tol);
         HVCK(hipGetLastError());
@@ -1942,20 +1963,10 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             float ms = 0.f;
             HVCK(hipEventElapsedTime(&ms, r0, r1));
             ms_ray += ms;
-            if (nsel) {
-                grown = false;
-                r_ = register_new(nlive, nsel, reg_next + (nlive - level_begin) * (D + 1));
+            if (nsel || nt_fresh) {
+                r_ = register_new(R, nlive, nsel, KCH > 1 ? (int64_t)nr : nsel,
+                                  reg_next + (nlive - level_begin) * (D + 1));
                 if (r_) return r_;
-                if (grown) {
-                    // slots stored before the rehash: this level's rays and its earlier new vertices
-                    k_relookup_rays<D><<<nblk(nr, 256), 256, 0, c->stream>>>(R, nr, vkey, etab, esize - 1);
-                    HVCK(hipGetLastError());
-                    if (nlive > level_begin) {
-                        k_relookup_vertices<D><<<nblk((nlive - level_begin) * (D + 1), 256), 256, 0, c->stream>>>(
-                            ids + level_begin, (int)(nlive - level_begin), vkey, etab, esize - 1, reg_next);
-                        HVCK(hipGetLastError());
-                    }
-                }
             }
             st.rays += n;
             nlive += nsel;
@@ -2001,7 +2012,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
                 if (rc) return rc;
             }
             HVCK(hipMemsetAsync(dcount, 0, sizeof(int), c->stream));
-            k_carry_out<D><<<nblk(nr, 256), 256, 0, c->stream>>>(R, nr, pflag, etab, cs, cv, cd, cu, cc, cdu, dcount);
+            k_carry_out<D><<<nblk(nr, 256), 256, 0, c->stream>>>(R, nr, pflag, rtab, cs, cv, cd, cu, cc, cdu, dcount);
             HVCK(hipGetLastError());
             HVCK(hipMemcpyAsync(&ncar, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
             HVCK(hipStreamSynchronize(c->stream));
@@ -2014,7 +2025,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
                 const int nb = nfull > ch ? (nfull - ch + K - 1) / K : 0;
                 const int nc = nb * 64 + ((nfull % K == ch) ? rem : 0);
                 if (nc <= 0) continue;
-                k_chunk<<<nblk(nc, 256), 256, 0, c->stream>>>(ord, nr, ch, K, R.slot, etab, cand, keep);
+                k_chunk<<<nblk(nc, 256), 256, 0, c->stream>>>(ord, nr, ch, K, R.slot, rtab, cand, keep);
                 HVCK(hipGetLastError());
                 int n = nc;
                 const int* rays_c = cand;
