@@ -572,32 +572,65 @@ __device__ __forceinline__ void hv_ray_setup(const DevGeo& g, const int* key, in
     rho = sqrt(rr);
 }
 
+// One thread per 4 edge slots (t = block*1024 + j*256 + thread, coalesced per j), and one
+// counter atomic per workgroup: a wavefront-level allocation made this pass bound by the
+// serialised same-address atomics (one per 64 slots) instead of by its loads.
+constexpr int HV_EMIT_PER = 4;
+
 template <int D>
 __global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, int nf, const int* __restrict__ vkey,
                                               const double* __restrict__ vx, DevGeo g,
                                               const unsigned long long* __restrict__ nt,
                                               const int64_t* __restrict__ reg, DevRays R, int* __restrict__ nrays,
                                               double plane_tol) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool want = false;
-    int v = 0, pos = 0;
-    int64_t h = -1;
-    if (t < nf * (D + 1)) {
-        h = reg[t];
-        if (h >= 0) {
-            const unsigned long long cur = nt[h];
-            const int fi = t / (D + 1);
-            pos = t % (D + 1);
-            v = frontier[fi];
-            const unsigned long long mine = ((((unsigned long long)v) << 4) | (unsigned)pos) + 1ull;
-            want = !(cur & HV_EDGE_DONE) && ((cur & HV_EDGE_VMASK) == mine);
+    __shared__ int wsum[4];
+    __shared__ int wbase[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t nslot = (int64_t)nf * (D + 1);
+    const int64_t t0 = (int64_t)blockIdx.x * (256 * HV_EMIT_PER) + threadIdx.x;
+    int64_t h[HV_EMIT_PER];
+    unsigned long long m[HV_EMIT_PER];
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < HV_EMIT_PER; ++j) {
+        const int64_t t = t0 + j * 256;
+        bool want = false;
+        h[j] = -1;
+        if (t < nslot) {
+            h[j] = reg[t];
+            if (h[j] >= 0) {
+                const unsigned long long cur = nt[h[j]];
+                const int fi = (int)(t / (D + 1)), pos = (int)(t % (D + 1));
+                const unsigned long long mine =
+                    ((((unsigned long long)frontier[fi]) << 4) | (unsigned)pos) + 1ull;
+                want = !(cur & HV_EDGE_DONE) && ((cur & HV_EDGE_VMASK) == mine);
+            }
         }
+        m[j] = __ballot(want);
+        cnt += __popcll(m[j]);
     }
-    const int idx = hv_wave_alloc(nrays, want);
-    if (!want) return;
-    R.slot[idx] = h;
-    R.v[idx] = v;
-    R.drop[idx] = pos;
+    if (lane == 0) wsum[wv] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        int b = tot ? atomicAdd(nrays, tot) : 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) { wbase[w] = b; b += wsum[w]; }
+    }
+    __syncthreads();
+    int base = wbase[wv];
+    const unsigned long long below = lane ? ((1ull << lane) - 1ull) : 0ull;
+#pragma unroll
+    for (int j = 0; j < HV_EMIT_PER; ++j) {
+        if ((m[j] >> lane) & 1ull) {
+            const int idx = base + __popcll(m[j] & below);
+            const int64_t t = t0 + j * 256;
+            R.slot[idx] = h[j];
+            R.v[idx] = frontier[(int)(t / (D + 1))];
+            R.drop[idx] = (int)(t % (D + 1));
+        }
+        base += __popcll(m[j]);
+    }
 }
 
 // Direction, half-space threshold and orthogonality defect of rays [0, n) (dense: one
@@ -1903,9 +1936,11 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             !ordc || !keep || !status || !rwin || !rt || !reg_next)
             return hv_fail(c, HV_ERR_DEVICE, "out of device memory (rays)");
         HVCK(hipMemsetAsync(dcount, 0, sizeof(int), c->stream));
-        k_emit<D><<<nblk(nslots, 256), 256, 0, c->stream>>>(ids + fbeg, (int)fcount, vkey, vx, g, ntab, reg_cur, R,
-                                                           dcount, plane_tol);
-        HVCK(hipGetLastError());
+        if (fcount > 0) {
+            k_emit<D><<<nblk(fcount * (D + 1), 256 * HV_EMIT_PER), 256, 0, c->stream>>>(
+                ids + fbeg, (int)fcount, vkey, vx, g, ntab, reg_cur, R, dcount, plane_tol);
+            HVCK(hipGetLastError());
+        }
         int nr = 0;
         HVCK(hipMemcpyAsync(&nr, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         rc = read_ctr();
