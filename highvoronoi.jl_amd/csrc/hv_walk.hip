@@ -293,6 +293,25 @@ __device__ __forceinline__ bool keys_equal(const int* __restrict__ a, const int*
     return eq;
 }
 
+// Adds the workgroup's counts of four conditions to ctr[0..3]: partial sums in LDS, then at
+// most one global atomic per counter and workgroup (same-address atomics serialise).  Every
+// thread of the workgroup calls it.
+__device__ __forceinline__ void hv_block_count4(unsigned long long* ctr, bool c0, bool c1, bool c2, bool c3) {
+    __shared__ unsigned int part[4];
+    if (threadIdx.x < 4) part[threadIdx.x] = 0u;
+    __syncthreads();
+    const unsigned n0 = __popcll(__ballot(c0)), n1 = __popcll(__ballot(c1));
+    const unsigned n2 = __popcll(__ballot(c2)), n3 = __popcll(__ballot(c3));
+    if ((threadIdx.x & 63) == 0) {
+        if (n0) atomicAdd(&part[0], n0);
+        if (n1) atomicAdd(&part[1], n1);
+        if (n2) atomicAdd(&part[2], n2);
+        if (n3) atomicAdd(&part[3], n3);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && part[threadIdx.x]) atomicAdd(&ctr[threadIdx.x], (unsigned long long)part[threadIdx.x]);
+}
+
 template <int D>
 __global__ __launch_bounds__(256) void k_insert(const int* __restrict__ status, int n, const int* __restrict__ vkey,
                                                 int64_t base, int* __restrict__ htab, uint64_t hmask,
@@ -321,10 +340,7 @@ __global__ __launch_bounds__(256) void k_insert(const int* __restrict__ status, 
         }
         win[t] = won;
     }
-    hv_wave_count(&ctr[0], dup);
-    hv_wave_count(&ctr[1], full);
-    hv_wave_count(&ctr[2], none);
-    hv_wave_count(&ctr[3], singular);
+    hv_block_count4(ctr, dup, full, none, singular);
 }
 
 template <int D>
@@ -808,9 +824,18 @@ __global__ __launch_bounds__(256) void k_ray_init(DevGeo g, DevRays R, int nrays
         const int ea = G.e[a];
         if (ea >= N) break;   // planes sort last
         const int* nb = g.knn + (int64_t)g.rank[ea] * HV_KNN;
-#pragma unroll 4
+        int jv[HV_KNN];   // the list's ids in 16-byte loads, ahead of the dependent row gathers
+#pragma unroll
+        for (int q = 0; q < HV_KNN / 4; ++q) {
+            const int4 w = ((const int4*)nb)[q];
+            jv[4 * q] = w.x;
+            jv[4 * q + 1] = w.y;
+            jv[4 * q + 2] = w.z;
+            jv[4 * q + 3] = w.w;
+        }
+#pragma unroll
         for (int m = 0; m < HV_KNN; ++m) {
-            const int j = nb[m];
+            const int j = jv[m];
             double xk[D];
 #pragma unroll
             for (int k = 0; k < D; ++k) xk[k] = g.xs[(int64_t)(j) * (D + 1) + k];
