@@ -1037,9 +1037,11 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
 
 // MODE 1 is held to 128 VGPRs (4 wavefronts per SIMD): the walk is latency-bound, and the
 // few values that spill are the rarely used FP64 ray data.
+// A packet holds `ppw` <= 64 rays (lanes ppw..63 idle): small levels use fewer rays per
+// packet, so that there are enough packets to fill the chip and each one walks fewer leaves.
 template <int D, int MODE>
 __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
-                                                  const int* __restrict__ vkey, const double* __restrict__ vx,
+                                                  int ppw, const int* __restrict__ vkey, const double* __restrict__ vx,
                                                   int* __restrict__ status, int* __restrict__ win,
                                                   double* __restrict__ tout, double plane_tol,
                                                   unsigned long long* __restrict__ ctr) {
@@ -1053,8 +1055,8 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
     __shared__ int kid[MODE == 1 ? 64 : 1][D];
     __shared__ double nog[MODE == 1 ? 8 : 1];
     const int lane = threadIdx.x & 63;
-    const int rid = hv_xcd_block() * 64 + lane;
-    const bool act = rid < nrays;
+    const int rid = hv_xcd_block() * ppw + lane;
+    const bool act = lane < ppw && rid < nrays;
     const int ray = act ? order[rid] : order[0];
     const int N = g.N;
     RayGeo<D> G;
@@ -1472,6 +1474,26 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
 }
 
 // New key = edge + winner (sorted) and its circumcentre, written to the provisional slot.
+// Packets of 64 rays, halved (down to 8) while the level would give fewer than
+// HV_MIN_PACKETS packets: a packet's walk is serial, and a few long walks leave the chip idle.
+constexpr int64_t HV_MIN_PACKETS = 2048;
+static inline int hv_packet_rays(int64_t n) {
+    int p = 64;
+    while (p > 8 && n < (int64_t)p * HV_MIN_PACKETS) p >>= 1;
+    return p;
+}
+
+template <int D>
+static void launch_scan(bool mixed, const DevGeo& g, const DevRays& R, const int* list, int n, const int* vkey,
+                        const double* vx, int* status, int* win, double* tout, double plane_tol,
+                        unsigned long long* ctr, hipStream_t st) {
+    const int ppw = hv_packet_rays(n);
+    if (mixed)
+        k_raycast64<D, 1><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
+    else
+        k_raycast64<D, 0><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
+}
+
 template <int D>
 __global__ __launch_bounds__(256) void k_finalize(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
                            int* __restrict__ vkey, double* __restrict__ vx, int64_t base,
@@ -2007,12 +2029,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             int r_ = ensure_vertices(n);
             if (r_) return r_;
             HVCK(hipEventRecord(r0, c->stream));
-            if (mixed)
-                k_raycast64<D, 1><<<nblk(n, 64), 64, 0, c->stream>>>(g, R, list, n, vkey, vx, status, rwin, rt, plane_tol,
-                                                                     ctr + 0);
-            else
-                k_raycast64<D, 0><<<nblk(n, 64), 64, 0, c->stream>>>(g, R, list, n, vkey, vx, status, rwin, rt, plane_tol,
-                                                                     ctr + 0);
+            launch_scan<D>(mixed, g, R, list, n, vkey, vx, status, rwin, rt, plane_tol, ctr + 0, c->stream);
             HVCK(hipGetLastError());
             HVCK(hipEventRecord(r1, c->stream));
             k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, list, n, vkey, vx, vnext, status, rwin);
@@ -2359,12 +2376,7 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     k_iota<<<nblk(nrays, 256), 256, 0, c->stream>>>(ord0, (int)nrays);
     HVCK(hipGetLastError());
     HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, (int)nrays, 0, 64, c->stream));
-    if (mixed)
-        k_raycast64<D, 1><<<nblk(nrays, 64), 64, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, status, rwin, tout,
-                                                                 plane_tol, ctr);
-    else
-        k_raycast64<D, 0><<<nblk(nrays, 64), 64, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, status, rwin, tout,
-                                                                 plane_tol, ctr);
+    launch_scan<D>(mixed, g, R, ord, (int)nrays, vkey, vx, status, rwin, tout, plane_tol, ctr, c->stream);
     HVCK(hipGetLastError());
     k_finalize<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, nrays, status, rwin);
     HVCK(hipGetLastError());
