@@ -118,6 +118,8 @@ __global__ void k_node_boxes(int first, int count, int NN, double* __restrict__ 
 // 8-ary tree over the same leaves (level h, 8^h slots, node (l,i) has children (l+1, 8i+c)).
 // Internal node (l,i) has global id (8^l - 1)/7 + i and stores the boxes of its 8 children
 // contiguously, in FP32 rounded outward, so one load per visited node tests 8 children.
+// An empty child is the point box lo = hi = 1e30: its distance^2 overflows to +inf for every
+// query, with the clamp form p - med3(p, lo, hi) as with max(lo - p, p - hi, 0).
 template <int D>
 __global__ void k_tree8_level(DevGeo g, int l, int nodes_at_l, float* __restrict__ cb) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -147,13 +149,18 @@ __global__ void k_tree8_level(DevGeo g, int l, int nodes_at_l, float* __restrict
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const float* b = cb + ((int64_t)cg * 8 + q) * (2 * D);
+            if (!(b[0] < 1e30f)) continue;   // empty grandchild
 #pragma unroll
             for (int k = 0; k < D; ++k) { lo[k] = fminf(lo[k], b[k]); hi[k] = fmaxf(hi[k], b[D + k]); }
         }
     }
+    const bool empty = !(lo[0] <= hi[0]);
     float* o = cb + ((int64_t)gid * 8 + c) * (2 * D);
 #pragma unroll
-    for (int k = 0; k < D; ++k) { o[k] = lo[k]; o[D + k] = hi[k]; }
+    for (int k = 0; k < D; ++k) {
+        o[k] = empty ? 1e30f : lo[k];
+        o[D + k] = empty ? 1e30f : hi[k];
+    }
 }
 
 // Mixed-precision scan tables (leaves of 32 points): the origin of each group of 8 leaves
@@ -948,8 +955,7 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
         float d2 = 0.f;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const float lo = cbx[c * 2 * D + k], hi = cbx[c * 2 * D + D + k];
-            const float a = fmaxf(fmaxf(lo - q[k], q[k] - hi), 0.f);
+            const float a = q[k] - __builtin_amdgcn_fmed3f(q[k], cbx[c * 2 * D + k], cbx[c * 2 * D + D + k]);
             d2 = fmaf(a, a, d2);
         }
         return d2;
@@ -1114,7 +1120,26 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
         for (int k = 0; k < D; ++k)
             pcf[k] = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint((float)(pc[k] * ic))));
     }
-    unsigned long long pairs = 0, leaves = 0, rechecks = 0;
+    // packet sphere about pcf holding every lane's starting ball (FP32, padded for rounding).
+    // Valid candidates lie ahead of their edge, where later (smaller) balls are nested in
+    // the first one, so a child box beyond this sphere holds no candidate of any lane.
+    float rp2f;
+    {
+        float dd = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float a = bcf[k] - pcf[k];
+            dd = fmaf(a, a, dd);
+        }
+        float rl = act ? sqrtf(dd) + sqrtf(R2f) : 0.f;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) rl = fmaxf(rl, __shfl_xor(rl, off));
+        rl = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(rl))) * (1.0f + 1e-5f) + 1e-30f;
+        rp2f = rl * rl * (1.0f + 1e-5f);
+    }
+    // per-packet counts in 32 bits: at most 2^25 leaves (N < 2^30) and 8 tiles per leaf
+    unsigned long long pairs = 0, rechecks = 0;
+    uint32_t leaves = 0, tiles = 0;
     bool flagged = false;
     // candidates the ray already holds: its edge generators (sorted ids) and the best two
     if constexpr (MODE == 1) {
@@ -1136,8 +1161,7 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
         float d2 = 0.f;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const float lo = cbx[c * 2 * D + k], hi = cbx[c * 2 * D + D + k];
-            const float a = fmaxf(fmaxf(lo - p[k], p[k] - hi), 0.f);
+            const float a = p[k] - __builtin_amdgcn_fmed3f(p[k], cbx[c * 2 * D + k], cbx[c * 2 * D + D + k]);
             d2 = fmaf(a, a, d2);
         }
         return d2;
@@ -1200,18 +1224,24 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
             prefetch(nx, nl);
         }
         hv_wave_sync();
-        // children each lane's ball reaches (kept per lane), children some ball reaches
-        unsigned rem = 0u, mine = 0u;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const bool h = act && (child_d2f(c, bcf) <= R2f);
-            mine |= h ? (1u << c) : 0u;
-            rem |= (__ballot(h) != 0ull) ? (1u << c) : 0u;
-        }
+        // child distances to the packet centre (order, and the packet-sphere prefilter), then
+        // the children each lane's ball reaches (kept per lane) among those the sphere reaches
+        unsigned near = 0u;
         {
             const uint32_t kb = __float_as_uint(child_d2f(lane & 7, pcf));
 #pragma unroll
-            for (int c = 0; c < 8; ++c) key[c] = __builtin_amdgcn_readlane(kb, c);
+            for (int c = 0; c < 8; ++c) {
+                key[c] = __builtin_amdgcn_readlane(kb, c);
+                near |= (key[c] <= __float_as_uint(rp2f)) ? (1u << c) : 0u;   // non-negative floats
+            }
+        }
+        unsigned rem = 0u, mine = 0u;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (!((near >> c) & 1u)) continue;
+            const bool h = (child_d2f(c, bcf) <= R2f) & act;
+            mine |= h ? (1u << c) : 0u;
+            rem |= (__ballot(h) != 0ull) ? (1u << c) : 0u;
         }
         const double tnode = tball;   // a ball that shrinks later in the node is re-tested
         rem = __builtin_amdgcn_readfirstlane(rem);
@@ -1333,7 +1363,7 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
 #pragma unroll
                             for (int t = 0; t < 2; ++t) {
                                 if (16 * t >= npts) continue;
-                                pairs += (unsigned long long)(16 * min(16, npts - 16 * t));
+                                ++tiles;
                                 const hv_f32x4 acc = hv_mfma_bf16(t == 0 ? ca0 : ca1, rk4[16 * g4 + (lane & 15)][lane >> 4]);
                                 // lane holds points 16t + 4*(lane/16) + e of ray 16*g4 + lane%16
                                 const uint32_t ob = __float_as_uint(acc[0]) | __float_as_uint(acc[1]) |
@@ -1399,8 +1429,9 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
         }
     }
     if (lane == 0) {
-        atomicAdd(&ctr[0], pairs);
-        atomicAdd(&ctr[1], leaves);
+        // BF16: every MFMA tile forms 16 x 16 (ray, point) pairs
+        atomicAdd(&ctr[0], MODE == 1 ? 256ull * tiles : pairs);
+        atomicAdd(&ctr[1], (unsigned long long)leaves);
     }
     if constexpr (MODE == 1) {
         // FP64 re-verifications and the rays that needed any
