@@ -142,6 +142,15 @@ struct DevRays {
 // Wavefront-aggregated atomics: one atomic per wave instead of one per lane (counters
 // and slot allocation hit a single address; per-lane atomics there serialise).  Both
 // must be reached by every lane of the wave (no divergent early return before them).
+// Workgroups are handed to the 8 XCDs round-robin (block b -> XCD b % 8), each XCD with its
+// own L2.  Renumber so that XCD x works on one contiguous range of blocks: neighbouring
+// packets (Morton order) then share an L2 instead of spreading every region over all eight.
+__device__ __forceinline__ int hv_xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int x = b & 7, per = nb >> 3, extra = nb & 7;
+    return x * per + min(x, extra) + (b >> 3);
+}
+
 __device__ __forceinline__ int hv_wave_alloc(int* counter, bool want) {
     const unsigned long long m = __ballot(want);
     if (m == 0ull) return -1;

@@ -661,7 +661,7 @@ __global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, 
 template <int D>
 __global__ __launch_bounds__(256) void k_ray_dir(DevGeo g, DevRays R, int n, const int* __restrict__ vkey,
                                                  const double* __restrict__ vx, double plane_tol) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = hv_xcd_block() * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int v = R.v[t], pos = R.drop[t];
     // key read straight from global memory: a private copy indexed by `pos` would live in scratch
@@ -703,15 +703,6 @@ __device__ __forceinline__ void hv_wave_sync() {
 }
 
 constexpr double HV_W = 1e-7;
-
-// Workgroups are handed to the 8 XCDs round-robin (block b -> XCD b % 8), each XCD with its
-// own L2.  Renumber so that XCD x works on one contiguous range of blocks: neighbouring
-// packets (Morton order) then share an L2 instead of spreading every region over all eight.
-__device__ __forceinline__ int hv_xcd_block() {
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int x = b & 7, per = nb >> 3, extra = nb & 7;
-    return x * per + min(x, extra) + (b >> 3);
-}
 
 
 // read-only tables seen through the constant address space: uniform loads become SMEM
@@ -801,7 +792,7 @@ __device__ __forceinline__ void hv_consider(const RayGeo<D>& G, Top2& T, const d
 template <int D>
 __global__ __launch_bounds__(256) void k_ray_init(DevGeo g, DevRays R, int nrays, const int* __restrict__ vkey,
                                                   const double* __restrict__ vx, double plane_tol) {
-    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ray = hv_xcd_block() * blockDim.x + threadIdx.x;
     if (ray >= nrays) return;
     const int N = g.N;
     RayGeo<D> G;
@@ -885,7 +876,7 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
     __shared__ double px[D][64];
     const int lane = threadIdx.x;
     const int N = g.N;
-    const int base = blockIdx.x * 64;
+    const int base = hv_xcd_block() * 64;
     const int j = base + lane;
     const bool act = j < N;
     const int jj = act ? j : N - 1;
@@ -1529,7 +1520,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_finalize(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
                            int* __restrict__ vkey, double* __restrict__ vx, int64_t base,
                            int* __restrict__ status, const int* __restrict__ win) {
-    const int rid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int rid = hv_xcd_block() * blockDim.x + threadIdx.x;
     if (rid >= nrays) return;
     if (status[rid] <= 0) return;
     const int ray = order[rid];
