@@ -44,11 +44,20 @@ def run_leg(ctx, precision, steps, warmup):
     return (t1 - t0) / steps, stats
 
 
-def key_table(ctx, precision):
+def mesh_table(ctx, precision):
     m = ctx.build_mesh(precision=precision)
-    keys, _ = m.export()
+    keys, coords = m.export()
     m.free()
-    return keys
+    return keys, coords
+
+
+def compare(ctx, p1, p2):
+    """(identical key tables, max |coordinate difference| or None when the keys differ)"""
+    k1, c1 = mesh_table(ctx, p1)
+    k2, c2 = mesh_table(ctx, p2)
+    same = bool(k1.shape == k2.shape and np.array_equal(k1, k2))
+    err = float(np.max(np.abs(c1 - c2))) if same and len(c1) else None
+    return same, err
 
 
 def cpu_baseline(xs, cells_per_thread, threads):
@@ -98,7 +107,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp64"],
-                    help="headline leg (the other precision runs beside it)")
+                    help="headline leg (the other precision, and the FP32 filter, run beside it)")
     ap.add_argument("--cpu-cells", type=int, default=200, help="cells per thread walked by the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -144,18 +153,22 @@ def main():
         sec2, stats2 = run_leg(ctx, other, args.steps, args.warmup)
         s2 = stats2[-1]
         log(f"[{other}] {s2['vertices']} vertices, {sec2 * 1e3:.2f} ms/step (scan {s2['ms_raycast']:.2f} ms)")
-        k1 = key_table(ctx, args.precision)
-        k2 = key_table(ctx, other)
-        same = bool(k1.shape == k2.shape and np.array_equal(k1, k2))
-        del k1, k2
+        same, err = compare(ctx, args.precision, other)
         fp64_vps = (s2["vertices"] / sec2) if other == "fp64" else value
         mixed_vps = value if args.precision == "bf16" else s2["vertices"] / sec2
         mixed = st if args.precision == "bf16" else s2
+        sec3, stats3 = run_leg(ctx, "fp32", args.steps, args.warmup)
+        s3 = stats3[-1]
+        log(f"[fp32] {s3['vertices']} vertices, {sec3 * 1e3:.2f} ms/step (scan {s3['ms_raycast']:.2f} ms)")
+        same32, err32 = compare(ctx, "fp32", "fp64")
         extra = {"fp64_vertices_per_s": fp64_vps, "mixed_vertices_per_s": mixed_vps,
                  "speedup_vs_fp64": mixed_vps / fp64_vps, "identical_vertex_set": same,
+                 "fp32_vertices_per_s": s3["vertices"] / sec3, "fp32_identical_vertex_set": same32,
+                 "fp32_max_coord_err_vs_fp64": err32,
+                 "fp32_fallback_rate": s3["fallbacks"] / max(1, s3["rays"]),
                  "fallback_rate": mixed["fallbacks"] / max(1, mixed["rays"]),
                  "fp64_rechecks_per_ray": mixed["fp64_rechecks"] / max(1, mixed["rays"]),
-                 "max_coord_err_vs_fp64": 0.0 if same else None}
+                 "max_coord_err_vs_fp64": err}
 
     # roofline of the candidate-scan kernel (k_raycast64): 4d algorithmic flops per scanned
     # (ray, generator) pair; kernel time from HIP events on the context's stream

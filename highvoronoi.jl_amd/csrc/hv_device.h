@@ -44,6 +44,9 @@ struct DevGeo {
     // 0..D-1) and the radius of the group about it (slot 7)
     const uint32_t* __restrict__ pkv;
     const double* __restrict__ lorg;
+    // FP32 scan (leaves of 32): per sorted point 8 floats, its coordinates relative to the same
+    // group origin, then |x - o|^2 (slot D; +1e30 for padding records past N), zeros after
+    const float* __restrict__ pxf;
 };
 
 #define HV_KNN 16

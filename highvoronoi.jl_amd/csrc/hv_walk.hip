@@ -235,6 +235,30 @@ __global__ void k_point_kvec(DevGeo g, const double* __restrict__ lorg, hv_u32x4
     for (int q = 0; q < 4; ++q) pkv[(int64_t)j * 4 + q] = hv_pack_slots<D>(sl, q);
 }
 
+template <int D>
+__global__ void k_point_f32(DevGeo g, const double* __restrict__ lorg, float* __restrict__ pxf) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= g.nleaf * 32) return;
+    float row[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) row[k] = 0.f;
+    if (j < g.N) {
+        const int grp = j >> 8;
+        double s2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double xp = g.xs[(int64_t)j * (D + 1) + k] - lorg[(int64_t)grp * 8 + k];
+            row[k] = (float)xp;
+            s2 = fma(xp, xp, s2);
+        }
+        row[D] = (float)s2;
+    } else {
+        row[D] = 1e30f;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pxf[(int64_t)j * 8 + k] = row[k];
+}
+
 // ---------------------------------------------------------------------------------
 // seeds: one block per box corner, brute-force nearest generator (ties -> smaller index)
 
@@ -1032,8 +1056,9 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
     }
 }
 
-// MODE 1 is held to 128 VGPRs (4 wavefronts per SIMD): the walk is latency-bound, and the
-// few values that spill are the rarely used FP64 ray data.
+// MODE 0: FP64 leaf scan.  MODE 1: BF16 MFMA filter, MODE 2: FP32 filter, each with FP64
+// re-verification of the survivors (leaves of 32 points).  MODE 1 is held to 128 VGPRs
+// (4 wavefronts per SIMD): the few values that spill are the rarely used FP64 ray data.
 // A packet holds `ppw` <= 64 rays (lanes ppw..63 idle): small levels use fewer rays per
 // packet, so that there are enough packets to fill the chip and each one walks fewer leaves.
 template <int D, int MODE>
@@ -1045,12 +1070,14 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
     __shared__ int stk[128], stkl[128];
     __shared__ float cbx[16 * D];
     // MODE 0: the staged leaf (FP64).  MODE 1: ray K-vectors of the current node (MFMA B
-    // operand source, 80-byte rows) and, per ray, which points of the current leaf it
-    // already holds as candidates.
+    // operand source, 80-byte rows).  MODE 2: the staged leaf (FP32 rows relative to the
+    // group origin).  MODE 1 and 2: the sorted ids of each ray's edge generators (the
+    // candidates it already holds) and the group origin.
     __shared__ double leafx[D + 1][MODE == 0 ? 64 : 1];
     __shared__ hv_u32x4 rk4[MODE == 1 ? 64 : 1][5];
-    __shared__ int kid[MODE == 1 ? 64 : 1][D];
-    __shared__ double nog[MODE == 1 ? 8 : 1];
+    __shared__ float4 leaf32[MODE == 2 ? 32 : 1][2];
+    __shared__ int kid[MODE >= 1 ? 64 : 1][D];
+    __shared__ double nog[MODE >= 1 ? 8 : 1];
     const int lane = threadIdx.x & 63;
     const int rid = hv_xcd_block() * ppw + lane;
     const bool act = lane < ppw && rid < nrays;
@@ -1133,11 +1160,11 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
     uint32_t leaves = 0, tiles = 0;
     bool flagged = false;
     // candidates the ray already holds: its edge generators (sorted ids) and the best two
-    if constexpr (MODE == 1) {
+    if constexpr (MODE >= 1) {
 #pragma unroll
         for (int z = 0; z < D; ++z) kid[lane][z] = G.e[z] < N ? g.rank[G.e[z]] : -1;
-        rk4[lane][3] = hv_u32x4{0u, 0u, 0u, 0u};   // K-slots 24..31
     }
+    if constexpr (MODE == 1) rk4[lane][3] = hv_u32x4{0u, 0u, 0u, 0u};   // K-slots 24..31
     auto leaf_known = [&](int leaf) -> uint32_t {
         uint32_t km = 0u;
 #pragma unroll
@@ -1187,7 +1214,7 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
         const float* src = g.cb + (int64_t)node * (16 * D);
         if (lane < 16 * D) cbr0 = src[lane];
         if (lane < 16 * D - 64) cbr1 = src[lane + 64];
-        if constexpr (MODE == 1) {
+        if constexpr (MODE >= 1) {
             if (lvl + 1 == g.h && lane < 8) lorr = g.lorg[(int64_t)(node - ((1 << (3 * lvl)) - 1) / 7) * 8 + lane];   // index in its level
         }
         pref = node;
@@ -1205,7 +1232,7 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
         // stage the 8 child boxes of the node (one contiguous block)
         if (lane < 16 * D) cbx[lane] = cbr0;
         if (lane < 16 * D - 64) cbx[lane + 64] = cbr1;
-        if constexpr (MODE == 1) {
+        if constexpr (MODE >= 1) {
             if (l + 1 == g.h && lane < 8) nog[lane] = lorr;
         }
         // a node whose children are leaves pushes nothing: the stack top is walked next
@@ -1307,6 +1334,90 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                                 }
                             }
                             if (T.t1 < before) set_ball(T.t1 + HV_W);
+                        }
+                        hv_wave_sync();
+                    }
+                    cc = cn;
+                }
+            } else if constexpr (MODE == 2) {
+                // Children are leaves of 32 points, staged in LDS as FP32 rows relative to the
+                // group origin o.  Per ray, once per node: b = bc - o (FP64, then FP32) and the
+                // test |x-o|^2 - 2<x-o, b> + (|b|^2 - R2) < thr in FP32, where thr bounds the
+                // FP32 rounding of every term (relative coordinates: all terms are of the size
+                // of the group and the ball) plus the FP64 filter's own slack 2 feps, so every
+                // point the FP64 scan would consider survives.  A ball that shrinks inside the
+                // node keeps the older (larger) form.
+                float m2b[D], cf, thrf;
+                {
+                    double bb = 0.0;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) {
+                        const double b = fma(tball, G.u[k], G.r[k]) - nog[k];
+                        bb = fma(b, b, bb);
+                        m2b[k] = (float)(-2.0 * b);
+                    }
+                    const double rad = nog[7], C = bb - R2;
+                    cf = (float)C;
+                    const double bound = 2.0 * (D + 4) * 5.9604644775390625e-08 *
+                                             (rad * rad + fabs(C) + 2.0 * rad * sqrt(bb)) +
+                                         2.0 * feps + 1e-13 * (bb + R2 + rad * rad);
+                    thrf = __double2float_ru(bound);
+                }
+                float4 pa = make_float4(0.f, 0.f, 0.f, 0.f), pb = pa;
+                auto load_leaf2 = [&](int leaf, float4& a, float4& b) {
+                    if (lane < 32) {
+                        const float4* src = (const float4*)(g.pxf) + ((int64_t)leaf * 32 + lane) * 2;
+                        a = src[0];
+                        b = src[1];
+                    }
+                };
+                int cc = __builtin_amdgcn_readfirstlane(pick(rem, true));
+                rem &= ~(1u << cc);
+                load_leaf2(8 * i + cc, pa, pb);
+                while (cc >= 0) {
+                    const int cn = rem ? __builtin_amdgcn_readfirstlane(pick(rem, true)) : -1;
+                    if (cn >= 0) rem &= ~(1u << cn);
+                    const int leaf = 8 * i + cc;
+                    const int j0 = leaf * 32, npts = max(0, min(32, N - j0));
+                    const bool hit = act && ((mine >> cc) & 1u) && (tball == tnode || child_d2f(cc, bcf) <= R2f);
+                    const unsigned long long m = __ballot(hit);
+                    if (m && lane < 32) {
+                        leaf32[lane][0] = pa;
+                        leaf32[lane][1] = pb;
+                    }
+                    if (cn >= 0) load_leaf2(8 * i + cn, pa, pb);
+                    if (m) {
+                        hv_wave_sync();
+                        leaves += 1;
+                        pairs += (unsigned long long)__popcll(m) * (unsigned long long)npts;
+                        if (hit) {
+                            uint32_t surv = 0u;
+#pragma unroll 4
+                            for (int q = 0; q < 32; ++q) {
+                                const float4 a = leaf32[q][0], b = leaf32[q][1];
+                                const float xr[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+                                float f = xr[D] + cf;
+#pragma unroll
+                                for (int k = 0; k < D; ++k) f = fmaf(xr[k], m2b[k], f);
+                                surv |= (f < thrf) ? (1u << q) : 0u;
+                            }
+                            surv &= (npts >= 32 ? 0xffffffffu : ((1u << npts) - 1u)) & ~leaf_known(leaf);
+                            if (surv) {
+                                // survivors that the ray does not already hold, in FP64
+                                const double before = T.t1;
+                                flagged = true;
+                                while (surv) {
+                                    const int q = __builtin_ctz(surv);
+                                    surv &= surv - 1u;
+                                    const int j = j0 + q;
+                                    double xk[D];
+#pragma unroll
+                                    for (int k = 0; k < D; ++k) xk[k] = g.xs[(int64_t)j * (D + 1) + k];
+                                    hv_consider<D>(G, T, xk, g.xs[(int64_t)j * (D + 1) + D], j, plane_tol);
+                                    ++rechecks;
+                                }
+                                if (T.t1 < before) set_ball(T.t1 + HV_W);
+                            }
                         }
                         hv_wave_sync();
                     }
@@ -1424,7 +1535,7 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
         atomicAdd(&ctr[0], MODE == 1 ? 256ull * tiles : pairs);
         atomicAdd(&ctr[1], (unsigned long long)leaves);
     }
-    if constexpr (MODE == 1) {
+    if constexpr (MODE >= 1) {
         // FP64 re-verifications and the rays that needed any
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) rechecks += __shfl_xor(rechecks, off);
@@ -1506,12 +1617,14 @@ static inline int hv_packet_rays(int64_t n) {
 }
 
 template <int D>
-static void launch_scan(bool mixed, const DevGeo& g, const DevRays& R, const int* list, int n, const int* vkey,
+static void launch_scan(int precision, const DevGeo& g, const DevRays& R, const int* list, int n, const int* vkey,
                         const double* vx, int* status, int* win, double* tout, double plane_tol,
                         unsigned long long* ctr, hipStream_t st) {
     const int ppw = hv_packet_rays(n);
-    if (mixed)
+    if (precision == HV_PREC_BF16)
         k_raycast64<D, 1><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
+    else if (precision == HV_PREC_FP32)
+        k_raycast64<D, 2><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
     else
         k_raycast64<D, 0><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
 }
@@ -1564,9 +1677,11 @@ T* pget(hv_ctx* c, const char* name, size_t count, bool keep = false) {
     return (T*)c->pool->get(name, count * sizeof(T), keep, c->stream);
 }
 
-// Points -> SoA + Morton order + leaf boxes + complete binary tree of boxes.
+// Points -> SoA + Morton order + leaf boxes + complete binary tree of boxes.  tables: the
+// per-point scan tables to build besides the tree (HV_PREC_BF16: BF16 K-vectors,
+// HV_PREC_FP32: FP32 relative rows; both with the leaf-group origins).
 template <int D>
-int build_tree(hv_ctx* c, int L, Tree& T, bool kvec = false) {
+int build_tree(hv_ctx* c, int L, Tree& T, int tables = HV_PREC_FP64) {
     const int N = (int)c->N;
     DevGeo& g = T.g;
     memset(&g, 0, sizeof g);
@@ -1666,17 +1781,26 @@ int build_tree(hv_ctx* c, int L, Tree& T, bool kvec = false) {
     }
     k_knn64<D><<<nblk(N, 64), 64, 0, c->stream>>>(g, knn);
     HVCK(hipGetLastError());
-    if (kvec) {
-        if (L != 32) return hv_fail(c, HV_ERR_UNSUPPORTED, "the mixed-precision scan uses leaves of 32 points");
+    if (tables != HV_PREC_FP64) {
+        if (L != 32) return hv_fail(c, HV_ERR_UNSUPPORTED, "the mixed-precision scans use leaves of 32 points");
         double* lorg = pget<double>(c, "lorg", (size_t)((g.nleaf + 7) / 8) * 8);
-        hv_u32x4* pkv = pget<hv_u32x4>(c, "pkv", (size_t)g.nleaf * 32 * 4);
-        if (!lorg || !pkv) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (mixed-precision tables)");
+        if (!lorg) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (mixed-precision tables)");
         k_leaf_origin<D><<<nblk((g.nleaf + 7) / 8, 256), 256, 0, c->stream>>>(g, lorg);
         HVCK(hipGetLastError());
-        k_point_kvec<D><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, c->stream>>>(g, lorg, pkv);
-        HVCK(hipGetLastError());
         g.lorg = lorg;
-        g.pkv = (const uint32_t*)pkv;
+        if (tables == HV_PREC_BF16) {
+            hv_u32x4* pkv = pget<hv_u32x4>(c, "pkv", (size_t)g.nleaf * 32 * 4);
+            if (!pkv) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (mixed-precision tables)");
+            k_point_kvec<D><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, c->stream>>>(g, lorg, pkv);
+            HVCK(hipGetLastError());
+            g.pkv = (const uint32_t*)pkv;
+        } else {
+            float* pxf = pget<float>(c, "pxf", (size_t)g.nleaf * 32 * 8);
+            if (!pxf) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (mixed-precision tables)");
+            k_point_f32<D><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, c->stream>>>(g, lorg, pxf);
+            HVCK(hipGetLastError());
+            g.pxf = pxf;
+        }
     }
     int hbad = 0;
     HVCK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1827,11 +1951,11 @@ template <int D>
 int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     hv_opts opt{};
     if (o) opt = *o;
-    if (opt.precision != HV_PREC_FP64 && opt.precision != HV_PREC_BF16)
-        return hv_fail(c, HV_ERR_UNSUPPORTED, "precision: HV_PREC_FP64 or HV_PREC_BF16");
-    const bool mixed = opt.precision == HV_PREC_BF16;
-    if (mixed && opt.leaf_size > 0 && opt.leaf_size != 32)
-        return hv_fail(c, HV_ERR_UNSUPPORTED, "the mixed-precision scan uses leaves of 32 points");
+    if (opt.precision != HV_PREC_FP64 && opt.precision != HV_PREC_BF16 && opt.precision != HV_PREC_FP32)
+        return hv_fail(c, HV_ERR_UNSUPPORTED, "precision: HV_PREC_FP64, HV_PREC_FP32 or HV_PREC_BF16");
+    const int prec = opt.precision;
+    if (prec != HV_PREC_FP64 && opt.leaf_size > 0 && opt.leaf_size != 32)
+        return hv_fail(c, HV_ERR_UNSUPPORTED, "the mixed-precision scans use leaves of 32 points");
     const int L = opt.leaf_size > 0 ? opt.leaf_size : 32;
     const double plane_tol = opt.plane_tolerance > 0 ? opt.plane_tolerance : 1e-12;
     const int KCH = opt.reserved0 > 1 ? opt.reserved0 : 1;   // chunks per level (0: tentative-key grouping)
@@ -1841,7 +1965,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1], r0 = c->ev[2], r1 = c->ev[3];
     HVCK(hipEventRecord(e0, c->stream));
     Tree T;
-    rc = build_tree<D>(c, L, T, mixed);
+    rc = build_tree<D>(c, L, T, prec);
     if (rc) return rc;
     DevGeo& g = T.g;
     const int nseed = 1 << D;
@@ -2051,7 +2175,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             int r_ = ensure_vertices(n);
             if (r_) return r_;
             HVCK(hipEventRecord(r0, c->stream));
-            launch_scan<D>(mixed, g, R, list, n, vkey, vx, status, rwin, rt, plane_tol, ctr + 0, c->stream);
+            launch_scan<D>(prec, g, R, list, n, vkey, vx, status, rwin, rt, plane_tol, ctr + 0, c->stream);
             HVCK(hipGetLastError());
             HVCK(hipEventRecord(r1, c->stream));
             k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, list, n, vkey, vx, vnext, status, rwin);
@@ -2328,9 +2452,8 @@ __global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const 
 template <int D>
 int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
                           const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status) {
-    if (precision != HV_PREC_FP64 && precision != HV_PREC_BF16)
-        return hv_fail(c, HV_ERR_UNSUPPORTED, "precision: HV_PREC_FP64 or HV_PREC_BF16");
-    const bool mixed = precision == HV_PREC_BF16;
+    if (precision != HV_PREC_FP64 && precision != HV_PREC_BF16 && precision != HV_PREC_FP32)
+        return hv_fail(c, HV_ERR_UNSUPPORTED, "precision: HV_PREC_FP64, HV_PREC_FP32 or HV_PREC_BF16");
     if (nrays == 0) return HV_OK;
     if (nrays > (1 << 29)) return hv_fail(c, HV_ERR_ARG, "too many rays");
     const int N = (int)c->N;
@@ -2353,7 +2476,7 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
         if (first >= N) return hv_fail(c, HV_ERR_ARG, "the walked edge must contain a generator");
     }
     Tree T;
-    rc = build_tree<D>(c, 32, T, mixed);
+    rc = build_tree<D>(c, 32, T, precision);
     if (rc) return rc;
     DevGeo& g = T.g;
     int* vkey = pget<int>(c, "b_vkey", (size_t)2 * nrays * (D + 1));
@@ -2398,7 +2521,7 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     k_iota<<<nblk(nrays, 256), 256, 0, c->stream>>>(ord0, (int)nrays);
     HVCK(hipGetLastError());
     HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, (int)nrays, 0, 64, c->stream));
-    launch_scan<D>(mixed, g, R, ord, (int)nrays, vkey, vx, status, rwin, tout, plane_tol, ctr, c->stream);
+    launch_scan<D>(precision, g, R, ord, (int)nrays, vkey, vx, status, rwin, tout, plane_tol, ctr, c->stream);
     HVCK(hipGetLastError());
     k_finalize<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, nrays, status, rwin);
     HVCK(hipGetLastError());

@@ -37,7 +37,10 @@ extern "C" {
 #define HV_ERR_STATE -8        /* call order (e.g. no points set) */
 
 #define HV_PREC_FP64 0         /* plain FP64 candidate scan */
-#define HV_PREC_FP32 1         /* reserved: returns HV_ERR_UNSUPPORTED */
+#define HV_PREC_FP32 1         /* FP32 VALU candidate filter (coordinates relative to a leaf-group
+                                  origin, error-bounded threshold) + FP64 re-verification of every
+                                  survivor: same vertex set and coordinates as HV_PREC_FP64;
+                                  leaves of 32 points */
 #define HV_PREC_BF16 2         /* BF16 MFMA candidate filter (hi/lo split, error-bounded margin)
                                   + FP64 re-verification of every survivor: the vertex set and
                                   coordinates equal those of HV_PREC_FP64; leaves of 32 points */
@@ -48,7 +51,8 @@ typedef struct hv_integral hv_integral;
 
 typedef struct hv_opts {
     int32_t precision;        /* HV_PREC_* */
-    int32_t leaf_size;        /* generators per tree leaf (0 = default 32; HV_PREC_BF16 needs 32) */
+    int32_t leaf_size;        /* generators per tree leaf (0 = default 32; HV_PREC_BF16 and
+                                 HV_PREC_FP32 need 32) */
     int32_t rays_per_block;   /* ignored: the scan works on packets of 64 rays, one wavefront each */
     int32_t reserved0;        /* walk-level schedule: <= 0 (default) = one scan per tentative end
                                  point, the other rays of that end point wait and are carried to

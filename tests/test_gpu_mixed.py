@@ -1,7 +1,8 @@
-"""GPU parity of the mixed-precision scan (BF16 MFMA filter + FP64 re-verification).
+"""GPU parity of the mixed-precision scans (BF16 MFMA filter, FP32 VALU filter; each with
+FP64 re-verification of its survivors).
 
-Bar: the BF16 build returns exactly the vertex set of the FP64 build and of the oracle
-(bit-exact keys) with the same coordinates, because every survivor of the filter is
+Bar: a mixed-precision build returns exactly the vertex set of the FP64 build and of the
+oracle (bit-exact keys) with the same coordinates, because every survivor of the filter is
 decided by the FP64 rule; at the 5D N=100k mixture the check is the whole key table.
 """
 import numpy as np
@@ -34,17 +35,22 @@ def build(ctx, xs, precision):
     return keys, coords, st
 
 
+MIXED = ["bf16", "fp32"]
+
+
+@pytest.mark.parametrize("prec", MIXED)
 @pytest.mark.parametrize("name", golden_cases())
-def test_bf16_matches_exact_fixture(ctx, name):
+def test_mixed_matches_exact_fixture(ctx, name, prec):
     g, xs, keys, exact = load_golden(name)
-    k, c, st = build(ctx, xs, "bf16")
+    k, c, st = build(ctx, xs, prec)
     assert key_set(k) == key_set(keys)
 
 
+@pytest.mark.parametrize("prec", MIXED)
 @pytest.mark.parametrize("d,n,seed", [(2, 1000, 0), (3, 3000, 5), (4, 2000, 1), (5, 1000, 9), (6, 300, 8)])
-def test_bf16_matches_oracle_and_fp64(ctx, d, n, seed):
+def test_mixed_matches_oracle_and_fp64(ctx, d, n, seed, prec):
     xs = synth.uniform(d, n, seed)
-    k, c, st = build(ctx, xs, "bf16")
+    k, c, st = build(ctx, xs, prec)
     k64, c64, _ = build(ctx, xs, "fp64")
     b, nrm = synth.cube_planes(d)
     ok, oc, _ = Oracle(xs, b, nrm).voronoi()
@@ -59,10 +65,11 @@ def test_bf16_matches_oracle_and_fp64(ctx, d, n, seed):
     assert st["fallbacks"] <= st["rays"]
 
 
-def test_bf16_identical_vertex_set_at_5d_100k_mixture(ctx):
+@pytest.mark.parametrize("prec", MIXED)
+def test_mixed_identical_vertex_set_at_5d_100k_mixture(ctx, prec):
     # the benchmark workload: 5D N=100k Gaussian mixture, cuboid(5, periodic=[])
     xs = synth.gaussian_mixture_5d()
-    k, c, st = build(ctx, xs, "bf16")
+    k, c, st = build(ctx, xs, prec)
     k64, c64, st64 = build(ctx, xs, "fp64")
     assert st["vertices"] == st64["vertices"] > 10_000_000
     assert key_digest(k) == key_digest(k64)
@@ -70,17 +77,19 @@ def test_bf16_identical_vertex_set_at_5d_100k_mixture(ctx):
     assert np.array_equal(c, c64)
 
 
-def test_bf16_rejects_other_leaf_sizes(ctx):
+@pytest.mark.parametrize("prec", MIXED)
+def test_mixed_rejects_other_leaf_sizes(ctx, prec):
     xs = synth.uniform(3, 200, 1)
     b, nrm = synth.cube_planes(3)
     ctx.set_points(xs)
     ctx.set_boundary(b, nrm)
     with pytest.raises(HVError) as e:
-        ctx.build_mesh(precision="bf16", leaf_size=16)
+        ctx.build_mesh(precision=prec, leaf_size=16)
     assert e.value.code == -7
 
 
-def test_bf16_raycast_batch_matches_fp64(ctx):
+@pytest.mark.parametrize("prec", MIXED)
+def test_mixed_raycast_batch_matches_fp64(ctx, prec):
     d, n = 5, 800
     xs = synth.uniform(d, n, 31)
     b, nrm = synth.cube_planes(d)
@@ -98,6 +107,6 @@ def test_bf16_raycast_batch_matches_fp64(ctx):
     ctx.set_points(xs)
     ctx.set_boundary(b, nrm)
     a = ctx.raycast_batch(sig, drop, r, precision="fp64")
-    m = ctx.raycast_batch(sig, drop, r, precision="bf16")
+    m = ctx.raycast_batch(sig, drop, r, precision=prec)
     for x, y in zip(a, m):
         assert np.array_equal(x, y)
