@@ -101,6 +101,85 @@ def measured_traffic(precision):
     return None
 
 
+def bench_batched(args):
+    """512 independent meshes: sequential hv_build_mesh calls on one context against one
+    hv_build_mesh_batch call (args.workers host threads, one stream each); every mesh of the
+    batch is compared with its sequential build (keys)."""
+    from hvgpu import synth
+    from hvgpu._lib import Context, build_mesh_batch
+
+    meshes = synth.batched_meshes(count=512, seed=3)
+    pts = [x for _, x in meshes]
+    planes = [synth.cube_planes(x.shape[1]) for x in pts]
+    prec = args.precision
+    ctx = Context(0)
+
+    def sequential():
+        out = []
+        for x, (b, nrm) in zip(pts, planes):
+            ctx.set_points(x)
+            ctx.set_boundary(b, nrm)
+            out.append(ctx.build_mesh(precision=prec))
+        return out
+
+    def batched():
+        out, status = build_mesh_batch(pts, planes, precision=prec, nworkers=args.workers)
+        if any(status):
+            raise RuntimeError(f"batched build failed: {sorted(set(status.tolist()))}")
+        return out
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            for m in fn():
+                m.free()
+        best = None
+        res = None
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            out = fn()
+            dt = time.perf_counter() - t0
+            if res is not None:
+                for m in res:
+                    m.free()
+            res = out
+            best = dt if best is None else min(best, dt)
+        return best, res
+
+    tseq, mseq = timed(sequential)
+    tbat, mbat = timed(batched)
+    nv = sum(m.stats()["vertices"] for m in mbat)
+    same = all(m.stats()["vertices"] == q.stats()["vertices"] for m, q in zip(mbat, mseq))
+    same = same and all(np.array_equal(m.export()[0], q.export()[0]) for m, q in zip(mbat[:64], mseq[:64]))
+    for m in mbat + mseq:
+        m.free()
+    dims = {d: sum(1 for dd, _ in meshes if dd == d) for d in (3, 4, 5, 6)}
+    log(f"[batched {prec}] {nv} vertices in 512 meshes: sequential {tseq * 1e3:.1f} ms, "
+        f"batched ({args.workers} workers) {tbat * 1e3:.1f} ms")
+    out = {
+        "metric": "Voronoi vertices/s, 512 independent meshes in one batched call",
+        "value": nv / tbat,
+        "unit": "vertices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tbat * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16+f64" if prec == "bf16" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "512 meshes, d uniform in {3,4,5,6}, N = floor(exp(U[ln 200, ln 5000])), "
+                               "points U[0,1]^d (seed 3*2^32+k), cuboid(d, periodic=[])",
+                   "meshes_per_dimension": dims, "vertices": nv, "precision": prec, "workers": args.workers},
+        "sequential_vertices_per_s": nv / tseq,
+        "sequential_ms": tseq * 1e3,
+        "speedup_vs_sequential": tseq / tbat,
+        "identical_to_sequential": bool(same),
+        "timing": "best of steps, host wall clock around the call",
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +191,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 leg and key-table comparison")
+    ap.add_argument("--batched", action="store_true",
+                    help="the 512-mesh workload instead (d in 3..6, N log-uniform in 200..5000): one "
+                         "hv_build_mesh_batch call against 512 sequential hv_build_mesh calls")
+    ap.add_argument("--workers", type=int, default=8, help="host workers (streams) of the batched call")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy (3x): the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters into bytes")
@@ -132,6 +215,9 @@ def main():
 
     from hvgpu import synth
     from hvgpu._lib import Context
+
+    if args.batched:
+        return bench_batched(args)
 
     xs = synth.gaussian_mixture_5d()
     d = xs.shape[1]

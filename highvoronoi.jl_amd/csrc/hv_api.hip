@@ -1,5 +1,8 @@
 // hv_api.hip -- the extern "C" surface of libhv.so (declared in include/hv.h).
 #include "hv_internal.h"
+#include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cstring>
 #include <string>
 
@@ -164,6 +167,41 @@ int hv_build_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
     (void)hipSetDevice(c->device);
     HV_DISPATCH(c->D, hv_build_mesh_impl, c, o, out)
+}
+
+int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, const int64_t* n,
+                        const double* const* xs, const int32_t* nplanes, const double* const* base,
+                        const double* const* normal, const hv_opts* opts, int32_t nworkers,
+                        hv_mesh** out, int32_t* status) {
+    if (nmesh < 0 || (nmesh > 0 && (!d || !n || !xs || !nplanes || !base || !normal || !out || !status)))
+        return HV_ERR_ARG;
+    for (int k = 0; k < nmesh; ++k) {
+        out[k] = nullptr;
+        status[k] = HV_ERR_DEVICE;   // overwritten by the worker that builds it
+    }
+    if (nmesh == 0) return HV_OK;
+    const int nw = std::max(1, std::min<int>(nworkers > 0 ? nworkers : 8, nmesh));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        hv_ctx* c = nullptr;
+        if (hv_create(&c, device_ordinal) != HV_OK) return;   // the other workers take its share
+        for (;;) {
+            const int k = next.fetch_add(1);
+            if (k >= nmesh) break;
+            int rc = hv_set_points(c, d[k], n[k], xs[k]);
+            if (rc == HV_OK) rc = hv_set_boundary(c, nplanes[k], base[k], normal[k], nullptr);
+            if (rc == HV_OK) rc = hv_build_mesh(c, opts, &out[k]);
+            status[k] = rc;
+        }
+        hv_destroy(c);
+    };
+    std::vector<std::thread> th;
+    th.reserve(nw);
+    for (int w = 0; w < nw; ++w) th.emplace_back(work);
+    for (auto& t : th) t.join();
+    for (int k = 0; k < nmesh; ++k)
+        if (status[k] != HV_OK) return status[k];
+    return HV_OK;
 }
 
 int hv_mesh_counts(const hv_mesh* m, int64_t* nvert, int32_t* max_sig, int64_t* nrays_unbounded) {

@@ -64,6 +64,9 @@ def lib():
     L.hv_set_points_device.argtypes = [vp, C.c_int, C.c_int64, vp]
     L.hv_set_boundary.argtypes = [vp, C.c_int, P(C.c_double), P(C.c_double), P(C.c_int16)]
     L.hv_build_mesh.argtypes = [vp, P(hv_opts), P(vp)]
+    L.hv_build_mesh_batch.argtypes = [C.c_int, C.c_int32, P(C.c_int32), P(C.c_int64), P(P(C.c_double)),
+                                      P(C.c_int32), P(P(C.c_double)), P(P(C.c_double)), P(hv_opts), C.c_int32,
+                                      P(vp), P(C.c_int32)]
     L.hv_mesh_counts.argtypes = [vp, P(C.c_int64), P(C.c_int32), P(C.c_int64)]
     L.hv_mesh_export.argtypes = [vp, P(C.c_int64), P(C.c_double)]
     L.hv_mesh_stats.argtypes = [vp, P(hv_stats_t)]
@@ -93,10 +96,10 @@ def _i64(a):
 class Mesh:
     """A device-resident vertex set (hv_mesh)."""
 
-    def __init__(self, ctx: "Context", handle):
+    def __init__(self, ctx, handle, d: int = 0):
         self._ctx = ctx
         self._h = handle
-        self.d = ctx.d
+        self.d = d if d else ctx.d
 
     def __del__(self):
         self.free()
@@ -241,3 +244,38 @@ class Context:
         idx = np.zeros(max(int(ptr[-1]), 1), dtype=np.int64)
         self._check(lib().hv_inrange_batch(self._h, n, _dp(q), _dp(radius), _i64(ptr), _i64(idx)))
         return ptr, idx[: ptr[-1]]
+
+
+def _opts(precision: str = "fp64", leaf_size: int = 0, vertex_capacity: int = 0, chunks: int = 0):
+    o = hv_opts()
+    o.precision = PRECISIONS[precision]
+    o.leaf_size = leaf_size
+    o.reserved0 = chunks
+    o.vertex_capacity = vertex_capacity
+    return o
+
+
+def build_mesh_batch(point_sets, planes, precision: str = "fp64", nworkers: int = 0, device: int = 0):
+    """Independent meshes in one hv_build_mesh_batch call.  point_sets[k]: (n_k, d_k) array;
+    planes[k]: (base, normal), each (nplanes_k, d_k).  Returns (meshes, status): a Mesh or
+    None per input and the per-mesh HV_* codes."""
+    L = lib()
+    m = len(point_sets)
+    xs = [np.ascontiguousarray(x, dtype=np.float64) for x in point_sets]
+    bases = [np.ascontiguousarray(b, dtype=np.float64) for b, _ in planes]
+    normals = [np.ascontiguousarray(nr, dtype=np.float64) for _, nr in planes]
+    d = np.array([x.shape[1] for x in xs], dtype=np.int32)
+    n = np.array([x.shape[0] for x in xs], dtype=np.int64)
+    npl = np.array([b.shape[0] for b in bases], dtype=np.int32)
+    PD = C.POINTER(C.c_double)
+    xp = (PD * max(1, m))(*[_dp(x) for x in xs])
+    bp = (PD * max(1, m))(*[_dp(b) for b in bases])
+    np_ = (PD * max(1, m))(*[_dp(nr) for nr in normals])
+    out = (C.c_void_p * max(1, m))()
+    status = np.zeros(max(1, m), dtype=np.int32)
+    o = _opts(precision)
+    L.hv_build_mesh_batch(device, m, d.ctypes.data_as(C.POINTER(C.c_int32)), _i64(n), xp,
+                          npl.ctypes.data_as(C.POINTER(C.c_int32)), bp, np_, C.byref(o), nworkers, out,
+                          status.ctypes.data_as(C.POINTER(C.c_int32)))
+    meshes = [Mesh(None, C.c_void_p(out[k]), int(d[k])) if out[k] else None for k in range(m)]
+    return meshes, status[:m].copy()

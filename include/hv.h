@@ -114,6 +114,19 @@ int hv_mesh_export(const hv_mesh* mesh, int64_t* sig, double* coords);
 int hv_mesh_stats(const hv_mesh* mesh, hv_stats_t* out);
 int hv_mesh_free(hv_mesh* mesh);
 
+/* Many independent meshes in one call: mesh k has d[k] in 2..6, n[k] generators xs[k] (d[k] x n[k],
+ * host memory) and the nplanes[k] boundary planes base[k], normal[k] (d[k] x nplanes[k] each,
+ * Dirichlet), all built with *opts.  Replaces a host loop of VoronoiGeometry(xs_k, boundary_k)
+ * calls (src/geometry.jl:156 -> voronoi, src/sysvoronoi.jl:21-82), one per mesh.  nworkers host
+ * threads (0 = 8), each with its own context and HIP stream on device_ordinal, take meshes in
+ * index order, so the level loops of different meshes overlap on the device.  out[k] receives
+ * the mesh (free each with hv_mesh_free) or NULL, status[k] its HV_* code; the return value is
+ * HV_OK or the first failing mesh's code. */
+int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, const int64_t* n,
+                        const double* const* xs, const int32_t* nplanes, const double* const* base,
+                        const double* const* normal, const hv_opts* opts, int32_t nworkers,
+                        hv_mesh** out, int32_t* status);
+
 /* Ray-level entry point: one walkray per input (src/raycast.jl:125-164 via
  * raycast_des2, src/raycast.jl:794-970).  For ray k: sig[k*(d+1) ...] is the sorted
  * 1-based key of a vertex, drop[k] the 0-based position of the generator to walk

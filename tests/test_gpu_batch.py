@@ -1,0 +1,76 @@
+"""GPU parity of hv_build_mesh_batch (many independent meshes, one call, several streams).
+
+Bar: every mesh of a batch equals the mesh that a single context builds from the same points
+(keys and coordinates bit-exact), small meshes equal the oracle's vertex set, and a mesh that
+fails reports its own status without stopping the others.
+"""
+import numpy as np
+import pytest
+
+from helpers import rows_view
+from hvgpu import synth
+from hvgpu._lib import Context, build_mesh_batch
+from oracle_py import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def single(ctx, xs, precision):
+    b, nrm = synth.cube_planes(xs.shape[1])
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    m = ctx.build_mesh(precision=precision)
+    out = m.export()
+    m.free()
+    return out
+
+
+@pytest.mark.parametrize("precision,nworkers", [("fp64", 4), ("bf16", 8), ("bf16", 1)])
+def test_batch_equals_single_builds(precision, nworkers):
+    meshes = synth.batched_meshes(count=24, seed=3, nmax=1500)
+    meshes.append((2, synth.uniform(2, 500, 11)))
+    pts = [x for _, x in meshes]
+    planes = [synth.cube_planes(x.shape[1]) for x in pts]
+    out, status = build_mesh_batch(pts, planes, precision=precision, nworkers=nworkers)
+    assert list(status) == [0] * len(pts)
+    ctx = Context(0)
+    for m, x in zip(out, pts):
+        k, c = m.export()
+        k1, c1 = single(ctx, x, precision)
+        assert np.array_equal(k, k1)
+        assert np.array_equal(c, c1)
+        m.free()
+
+
+def test_batch_small_meshes_match_oracle():
+    pts = [synth.uniform(2, 300, 21), synth.uniform(3, 200, 22), synth.uniform(4, 120, 23)]
+    planes = [synth.cube_planes(x.shape[1]) for x in pts]
+    out, status = build_mesh_batch(pts, planes, precision="bf16", nworkers=3)
+    assert list(status) == [0, 0, 0]
+    for m, x in zip(out, pts):
+        b, nrm = synth.cube_planes(x.shape[1])
+        ok, oc, _ = Oracle(x, b, nrm).voronoi()
+        k, c = m.export()
+        assert np.array_equal(rows_view(k), rows_view(ok[np.lexsort(ok.T[::-1])]))
+        m.free()
+
+
+def test_batch_reports_failures_per_mesh():
+    good = synth.uniform(3, 200, 5)
+    few = synth.uniform(3, 3, 6)                       # n <= d: HV_ERR_FEW_POINTS
+    outside = synth.uniform(3, 200, 7) + 2.0           # outside the unit cube: HV_ERR_DOMAIN
+    pts = [good, few, outside, good]
+    planes = [synth.cube_planes(3)] * 4
+    out, status = build_mesh_batch(pts, planes, precision="fp64", nworkers=2)
+    assert list(status) == [0, -3, -2, 0]
+    assert out[1] is None and out[2] is None
+    k0, c0 = out[0].export()
+    k3, c3 = out[3].export()
+    assert np.array_equal(k0, k3) and np.array_equal(c0, c3)
+    out[0].free()
+    out[3].free()
+
+
+def test_empty_batch():
+    out, status = build_mesh_batch([], [], precision="fp64")
+    assert out == [] and len(status) == 0
