@@ -1459,6 +1459,8 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                     if (cn >= 0) load_leaf1(8 * i + cn, pa0, pa1);
                     if (m) {
                         leaves += 1;
+                        // survivors of this leaf for the lane's own ray (bit q = point j0 + q)
+                        uint32_t sv = 0u;
 #pragma unroll
                         for (int g4 = 0; g4 < 4; ++g4) {
                             if (((m >> (16 * g4)) & 0xffffull) == 0ull) continue;
@@ -1471,44 +1473,34 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                                 const uint32_t ob = __float_as_uint(acc[0]) | __float_as_uint(acc[1]) |
                                                     __float_as_uint(acc[2]) | __float_as_uint(acc[3]);
                                 if (__ballot((int)ob < 0) == 0ull) continue;
-                                // survivors that the ray does not already hold go to FP64
-                                const int rho = 16 * g4 + (lane & 15);
-                                const int sh = 16 * t + 4 * (lane >> 4);
-                                unsigned real = 0u;
-                                // which points of this leaf each ray already holds, from its lane
-                                const uint32_t kmine = leaf_known(leaf);
-                                const uint32_t kcol = (uint32_t)__shfl((int)kmine, rho);
-                                if ((int)ob < 0) {
-                                    const uint32_t known = kcol >> sh;
+                                uint32_t nib = 0u;
 #pragma unroll
-                                    for (int e = 0; e < 4; ++e)
-                                        if ((int)__float_as_uint(acc[e]) < 0 && !((known >> e) & 1u) && sh + e < npts)
-                                            real |= 1u << e;
+                                for (int e = 0; e < 4; ++e) nib |= ((int)__float_as_uint(acc[e]) < 0) ? (1u << e) : 0u;
+                                // the ray's lane 16*g4 + c gathers the nibbles of lanes c, c+16, c+32, c+48
+                                uint32_t col = 0u;
+#pragma unroll
+                                for (int q = 0; q < 4; ++q)
+                                    col |= (uint32_t)__shfl((int)nib, (lane & 15) + 16 * q) << (4 * q);
+                                if ((lane >> 4) == g4) sv |= col << (16 * t);
+                            }
+                        }
+                        // survivors the ray does not already hold go to FP64, every ray at once
+                        if (sv) sv &= ~leaf_known(leaf) & (npts >= 32 ? 0xffffffffu : ((1u << npts) - 1u));
+                        if (__ballot(sv != 0u)) {
+                            if (sv) {
+                                const double before = T.t1;
+                                flagged = true;
+                                while (sv) {
+                                    const int q = __builtin_ctz(sv);
+                                    sv &= sv - 1u;
+                                    const int j = j0 + q;
+                                    double xk[D];
+#pragma unroll
+                                    for (int k = 0; k < D; ++k) xk[k] = g.xs[(int64_t)j * (D + 1) + k];
+                                    hv_consider<D>(G, T, xk, g.xs[(int64_t)j * (D + 1) + D], j, plane_tol);
+                                    ++rechecks;
                                 }
-                                unsigned long long rm = __ballot(real != 0u);
-                                while (rm) {
-                                    const int l2 = __ffsll((long long)rm) - 1;
-                                    rm &= rm - 1ull;
-                                    const unsigned bits = __builtin_amdgcn_readlane(real, l2);
-                                    const int rho2 = 16 * g4 + (l2 & 15);
-                                    const int pb2 = j0 + 16 * t + 4 * (l2 >> 4);
-                                    if (lane == rho2) {
-                                        const double before = T.t1;
-#pragma unroll
-                                        for (int e = 0; e < 4; ++e) {
-                                            if ((bits >> e) & 1u) {
-                                                double xk[D];
-#pragma unroll
-                                                for (int k = 0; k < D; ++k) xk[k] = g.xs[(int64_t)(pb2 + e) * (D + 1) + k];
-                                                hv_consider<D>(G, T, xk, g.xs[(int64_t)(pb2 + e) * (D + 1) + D], pb2 + e,
-                                                               plane_tol);
-                                                ++rechecks;
-                                            }
-                                        }
-                                        if (T.t1 < before) set_ball(T.t1 + HV_W);
-                                        flagged = true;
-                                    }
-                                }
+                                if (T.t1 < before) set_ball(T.t1 + HV_W);
                             }
                         }
                     }
