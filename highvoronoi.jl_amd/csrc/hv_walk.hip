@@ -1184,19 +1184,17 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
         }
         return d2;
     };
-    // nearest (or farthest) child still in `rem`, by distance to the packet centre (uniform
-    // float bits: non-negative floats order as unsigned integers)
+    // children of the staged node ordered by distance to the packet centre (ties: lower
+    // index first): rank r holds child (ord >> 4r) & 7.  Distances are non-negative floats
+    // (or +inf), which order as unsigned integers.
     uint32_t key[8];
-    auto pick = [&](unsigned rem, bool nearest) {
-        int best = -1;
-        uint32_t bd = 0u;
+    uint32_t ord = 0u;
+    // child mask -> rank mask
+    auto to_rank = [&](unsigned cm) {
+        unsigned rr = 0u;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const bool in = (rem >> c) & 1u;
-            const bool better = best < 0 || (nearest ? key[c] < bd : key[c] > bd);
-            if (in && better) { best = c; bd = key[c]; }
-        }
-        return best;
+        for (int q = 0; q < 8; ++q) rr |= ((cm >> ((ord >> (4 * q)) & 7u)) & 1u) << q;
+        return rr;
     };
     auto load_leaf = [&](int leaf, double* pf) {
         const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
@@ -1247,11 +1245,18 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
         unsigned near = 0u;
         {
             const uint32_t kb = __float_as_uint(child_d2f(lane & 7, pcf));
+            const int me = lane & 7;
+            int rk = 0;
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 key[c] = __builtin_amdgcn_readlane(kb, c);
                 near |= (key[c] <= __float_as_uint(rp2f)) ? (1u << c) : 0u;   // non-negative floats
+                rk += (key[c] < kb || (key[c] == kb && c < me)) ? 1 : 0;
             }
+            uint32_t o = 0u;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) o |= (uint32_t)c << (4 * __builtin_amdgcn_readlane(rk, c));
+            ord = o;
         }
         unsigned rem = 0u, mine = 0u;
 #pragma unroll
@@ -1271,12 +1276,13 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
             if constexpr (MODE == 0) {
                 // children are leaves: nearest first, the next one loaded while this one is scanned
                 double pf[D + 1];
-                int cc = __builtin_amdgcn_readfirstlane(pick(rem, true));
-                rem &= ~(1u << cc);
+                unsigned rr = to_rank(rem);
+                int cc = (ord >> (4 * __builtin_ctz(rr))) & 7u;
+                rr &= rr - 1u;
                 load_leaf(8 * i + cc, pf);
                 while (cc >= 0) {
-                    const int cn = rem ? __builtin_amdgcn_readfirstlane(pick(rem, true)) : -1;
-                    if (cn >= 0) rem &= ~(1u << cn);
+                    const int cn = rr ? (int)((ord >> (4 * __builtin_ctz(rr))) & 7u) : -1;
+                    rr &= rr - 1u;
                     const int leaf = 8 * i + cc;
                     const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
                     // balls only shrink: re-test with the current one
@@ -1371,12 +1377,13 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                         b = src[1];
                     }
                 };
-                int cc = __builtin_amdgcn_readfirstlane(pick(rem, true));
-                rem &= ~(1u << cc);
+                unsigned rr = to_rank(rem);
+                int cc = (ord >> (4 * __builtin_ctz(rr))) & 7u;
+                rr &= rr - 1u;
                 load_leaf2(8 * i + cc, pa, pb);
                 while (cc >= 0) {
-                    const int cn = rem ? __builtin_amdgcn_readfirstlane(pick(rem, true)) : -1;
-                    if (cn >= 0) rem &= ~(1u << cn);
+                    const int cn = rr ? (int)((ord >> (4 * __builtin_ctz(rr))) & 7u) : -1;
+                    rr &= rr - 1u;
                     const int leaf = 8 * i + cc;
                     const int j0 = leaf * 32, npts = max(0, min(32, N - j0));
                     const bool hit = act && ((mine >> cc) & 1u) && (tball == tnode || child_d2f(cc, bcf) <= R2f);
@@ -1435,8 +1442,9 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                     a0 = base[0];
                     a1 = base[16 * 4];
                 };
-                int cc = __builtin_amdgcn_readfirstlane(pick(rem, true));
-                rem &= ~(1u << cc);
+                unsigned rr = to_rank(rem);
+                int cc = (ord >> (4 * __builtin_ctz(rr))) & 7u;
+                rr &= rr - 1u;
                 load_leaf1(8 * i + cc, pa0, pa1);
                 {
                     double no[D], cb_[D];
@@ -1449,8 +1457,8 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                 }
                 hv_wave_sync();
                 while (cc >= 0) {
-                    const int cn = rem ? __builtin_amdgcn_readfirstlane(pick(rem, true)) : -1;
-                    if (cn >= 0) rem &= ~(1u << cn);
+                    const int cn = rr ? (int)((ord >> (4 * __builtin_ctz(rr))) & 7u) : -1;
+                    rr &= rr - 1u;
                     const int leaf = 8 * i + cc;
                     const int j0 = leaf * 32, npts = max(0, min(32, N - j0));
                     const bool hit = act && ((mine >> cc) & 1u) && (tball == tnode || child_d2f(cc, bcf) <= R2f);
@@ -1511,9 +1519,11 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
             // internal children: farthest pushed first so the nearest is walked next
             const int cbase = off + pw + 8 * i;
             int last = -1;
-            while (rem) {
-                const int c = __builtin_amdgcn_readfirstlane(pick(rem, false));
-                rem &= ~(1u << c);
+            unsigned rr = to_rank(rem);
+            while (rr) {
+                const int q = 31 - __builtin_clz(rr);
+                rr &= ~(1u << q);
+                const int c = (ord >> (4 * q)) & 7u;
                 stk[sp] = cbase + c;
                 stkl[sp] = l + 1;
                 ++sp;
