@@ -439,13 +439,16 @@ __global__ __launch_bounds__(256) void k_rt_build(DevRays R, int nr, const int* 
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_register(const int* __restrict__ frontier, int nf, const int* __restrict__ vkey,
+__global__ __launch_bounds__(256) void k_register(const int* __restrict__ frontier, int nf,
+                                                  const int* __restrict__ nf_dev, const int* __restrict__ vkey,
                                                   int N, DevRays R, unsigned long long* __restrict__ rt, uint64_t rmask,
                                                   unsigned long long* __restrict__ nt, uint64_t nmask,
                                                   int64_t* __restrict__ reg, unsigned long long* __restrict__ ctr) {
+    // nf_dev (when given): the vertex count in device memory, nf then only bounds the grid
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nv = nf_dev ? min(nf, *nf_dev) : nf;
     bool full = false;
-    if (t < nf * (D + 1)) {
+    if (t < nv * (D + 1)) {
         const int fi = t / (D + 1), pos = t % (D + 1);
         const int v = frontier[fi];
         const int* key = vkey + (int64_t)v * (D + 1);
@@ -1612,6 +1615,9 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
 // Packets of 64 rays, halved (down to 8) while the level would give fewer than
 // HV_MIN_PACKETS packets: a packet's walk is serial, and a few long walks leave the chip idle.
 constexpr int64_t HV_MIN_PACKETS = 2048;
+// Rays are ordered by the top 32 of the 60 Morton bits of their ball centre (2^(32/d) cells
+// per axis: far finer than a level's rays are spread), half the radix passes of all 60.
+constexpr int HV_RAY_SORT_LO = 28;
 static inline int hv_packet_rays(int64_t n) {
     int p = 64;
     while (p > 8 && n < (int64_t)p * HV_MIN_PACKETS) p >>= 1;
@@ -2043,24 +2049,29 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         return HV_OK;
     };
     // candidates at provisional ids base .. base+n with status > 0 -> hash -> new ids appended at ids[nlive]
-    auto insert_and_select = [&](const int* status, int n, int64_t base, int& nsel) -> int {
+    // cnt_dev: leave the count in device memory (no host wait) instead of returning it in nsel
+    auto insert_and_select = [&](const int* status, int n, int64_t base, int& nsel, int* cnt_dev = nullptr) -> int {
         uint8_t* win = pget<uint8_t>(c, "win", n);
         if (!win) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (win)");
         k_insert<D><<<nblk(n, 256), 256, 0, c->stream>>>(status, n, vkey, base, htab, hsize - 1, win, ctr + 8);
         HVCK(hipGetLastError());
         size_t tb = 0;
         hipcub::CountingInputIterator<int> it((int)base);
-        HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, win, ids + nlive, dcount, n, c->stream));
+        int* cnt = cnt_dev ? cnt_dev : dcount;
+        HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, win, ids + nlive, cnt, n, c->stream));
         void* tmp = c->pool->get("cubtmp2", tb, false, c->stream);
         if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
-        HVCK(hipcub::DeviceSelect::Flagged(tmp, tb, it, win, ids + nlive, dcount, n, c->stream));
+        HVCK(hipcub::DeviceSelect::Flagged(tmp, tb, it, win, ids + nlive, cnt, n, c->stream));
+        if (cnt_dev) return HV_OK;
         HVCK(hipMemcpyAsync(&nsel, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HVCK(hipStreamSynchronize(c->stream));
         return HV_OK;
     };
     // edges of new vertices ids[first .. first+count); `bound` caps the new vertices the
     // level may still register (sizes the new-edge table at the level's first registration)
-    auto register_new = [&](const DevRays& R, int64_t first, int count, int64_t bound, int64_t* reg) -> int {
+    // count_dev: the vertex count is in device memory and `count` only bounds it
+    auto register_new = [&](const DevRays& R, int64_t first, int count, int64_t bound, int64_t* reg,
+                            const int* count_dev = nullptr) -> int {
         if (nt_fresh) {
             nsize = hpow(2 * (D + 1) * std::max<int64_t>(bound, count) + 64);
             ntab = pget<unsigned long long>(c, "ntab", nsize);
@@ -2070,7 +2081,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         }
         if (count == 0) return HV_OK;
         k_register<D><<<nblk((int64_t)count * (D + 1), 256), 256, 0, c->stream>>>(
-            ids + first, count, vkey, N, R, rtab, rsize - 1, ntab, nsize - 1, reg, ctr + 4);
+            ids + first, count, count_dev, vkey, N, R, rtab, rsize - 1, ntab, nsize - 1, reg, ctr + 4);
         HVCK(hipGetLastError());
         return HV_OK;
     };
@@ -2164,16 +2175,18 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         HVCK(hipGetLastError());
         {
             size_t tb = 0;
-            HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, R.code, code2, ord0, ord, nr, 0, 64, c->stream));
+            HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, R.code, code2, ord0, ord, nr, HV_RAY_SORT_LO, 60, c->stream));
             void* tmp = c->pool->get("cubtmp", tb, false, c->stream);
             if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
             k_iota<<<nblk(nr, 256), 256, 0, c->stream>>>(ord0, nr);
             HVCK(hipGetLastError());
-            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, nr, 0, 64, c->stream));
+            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, nr, HV_RAY_SORT_LO, 60, c->stream));
         }
         const int64_t level_begin = nlive;
         // scan a list of rays, then new vertices -> hash -> registered edges
-        auto process = [&](const int* list, int n) -> int {
+        // defer: the new-vertex count stays in dcount[1] (the caller reads it with its next
+        // wait and then advances nlive and the scan time)
+        auto process = [&](const int* list, int n, bool defer) -> int {
             int r_ = ensure_vertices(n);
             if (r_) return r_;
             HVCK(hipEventRecord(r0, c->stream));
@@ -2183,6 +2196,15 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, list, n, vkey, vx, vnext, status, rwin);
             HVCK(hipGetLastError());
             int nsel = 0;
+            if (defer) {
+                r_ = insert_and_select(status, n, vnext, nsel, dcount + 1);
+                if (r_) return r_;
+                r_ = register_new(R, nlive, n, n, reg_next + (nlive - level_begin) * (D + 1), dcount + 1);
+                if (r_) return r_;
+                st.rays += n;
+                vnext += n;
+                return HV_OK;
+            }
             r_ = insert_and_select(status, n, vnext, nsel);
             if (r_) return r_;
             float ms = 0.f;
@@ -2233,14 +2255,23 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             rc = select(ord, keep, nr, ordc, np);
             if (rc) return rc;
             if (np) {
-                rc = process(ordc, np);
+                rc = process(ordc, np, true);
                 if (rc) return rc;
             }
             HVCK(hipMemsetAsync(dcount, 0, sizeof(int), c->stream));
             k_carry_out<D><<<nblk(nr, 256), 256, 0, c->stream>>>(R, nr, pflag, rtab, cs, cv, cd, cu, cc, cdu, dcount);
             HVCK(hipGetLastError());
-            HVCK(hipMemcpyAsync(&ncar, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            // one wait for the carried rays (dcount[0]) and the level's new vertices (dcount[1])
+            int hc[2] = {0, 0};
+            HVCK(hipMemcpyAsync(hc, dcount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
             HVCK(hipStreamSynchronize(c->stream));
+            ncar = hc[0];
+            if (np) {
+                nlive += hc[1];
+                float ms = 0.f;
+                HVCK(hipEventElapsedTime(&ms, r0, r1));
+                ms_ray += ms;
+            }
             skipped += (int64_t)(nr - np) - ncar;
         } else {
             // interleaved chunks: 64-ray blocks over K chunks, edge registration after each
@@ -2261,7 +2292,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
                     skipped += nc - n;
                 }
                 if (n == 0) continue;
-                rc = process(rays_c, n);
+                rc = process(rays_c, n, false);
                 if (rc) return rc;
             }
         }
@@ -2517,12 +2548,12 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     k_ray_init<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, (int)nrays, vkey, vx, plane_tol);
     HVCK(hipGetLastError());
     size_t tb = 0;
-    HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, R.code, code2, ord0, ord, (int)nrays, 0, 64, c->stream));
+    HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, R.code, code2, ord0, ord, (int)nrays, HV_RAY_SORT_LO, 60, c->stream));
     void* tmp = c->pool->get("cubtmp", tb, false, c->stream);
     if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
     k_iota<<<nblk(nrays, 256), 256, 0, c->stream>>>(ord0, (int)nrays);
     HVCK(hipGetLastError());
-    HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, (int)nrays, 0, 64, c->stream));
+    HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, (int)nrays, HV_RAY_SORT_LO, 60, c->stream));
     launch_scan<D>(precision, g, R, ord, (int)nrays, vkey, vx, status, rwin, tout, plane_tol, ctr, c->stream);
     HVCK(hipGetLastError());
     k_finalize<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, nrays, status, rwin);
