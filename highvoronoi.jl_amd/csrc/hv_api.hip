@@ -277,6 +277,15 @@ int hv_integrate_mc(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t seed,
     HV_DISPATCH(c->D, hv_integrate_mc_impl, c, mesh, ndir, seed, out)
 }
 
+int hv_integrate_mc_points(hv_ctx* c, const hv_mesh* mesh, int64_t nsamples, uint64_t seed, double slab,
+                           hv_integral** out) {
+    if (!c || !mesh || !out) return hv_fail(c, HV_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    (void)hipSetDevice(c->device);
+    HV_DISPATCH(c->D, hv_integrate_mc_points_impl, c, mesh, nsamples, seed, slab, out)
+}
+
 int hv_integral_counts(const hv_integral* I, int64_t* ncells, int64_t* nnbr) {
     return hv_integral_counts_impl(I, ncells, nnbr);
 }

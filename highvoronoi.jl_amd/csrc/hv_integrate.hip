@@ -795,6 +795,73 @@ int hv_integrate_mc_impl(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t 
     return finish<D>(c, pkey, area, vol, ptr, npair, ms, out);
 }
 
+// Point-sample counts -> volumes (sample fraction of the box) and interface areas.  A slab of
+// width w beside an interface holds w * A - O(w^2) of volume (samples near the interface's
+// own boundary see a third generator), so from the counts c1 (width eps) and c2 (width
+// 2 eps): A = 2 A(eps) - A(2 eps) = V (2 c1 - c2 / 2) / (n eps), first order removed; a
+// generator-generator interface averages the counts of its two sides.
+__global__ void k_mcp_finish(int N, const int64_t* __restrict__ ptr, const uint64_t* __restrict__ pkey, int64_t npair,
+                             const unsigned long long* __restrict__ ccnt, const unsigned long long* __restrict__ pcnt,
+                             double n, double eps, double vbox, double* __restrict__ vol, double* __restrict__ area) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) vol[t] = vbox * (double)ccnt[t] / n;
+    if (t >= npair) return;
+    const int i = (int)(pkey[t] >> 32), k = (int)(pkey[t] & 0xffffffffull);
+    double c1 = (double)pcnt[2 * t], c2 = (double)pcnt[2 * t + 1];
+    if (k < N) {
+        int64_t lo = ptr[k], hi = ptr[k + 1];
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if ((int)(pkey[mid] & 0xffffffffull) < i) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < ptr[k + 1] && (int)(pkey[lo] & 0xffffffffull) == i) {
+            c1 = 0.5 * (c1 + (double)pcnt[2 * lo]);
+            c2 = 0.5 * (c2 + (double)pcnt[2 * lo + 1]);
+        }
+    }
+    area[t] = fmax(0.0, vbox * (2.0 * c1 - 0.5 * c2) / (n * eps));
+}
+
+template <int D>
+int hv_integrate_mc_points_impl(hv_ctx* c, const hv_mesh* mesh, int64_t nsamples, uint64_t seed, double slab,
+                                hv_integral** out) {
+    if (nsamples < 1 || nsamples > (1ll << 40)) return hv_fail(c, HV_ERR_ARG, "nsamples must be positive");
+    if (!(slab > 0.0)) slab = 0.03;
+    IntGeo g;
+    int rc = make_geo<D>(c, mesh, g);
+    if (rc) return rc;
+    HVCK(hipEventRecord(c->ev[0], c->stream));
+    uint64_t* pkey = nullptr;
+    double* parea = nullptr;
+    int64_t npair = 0;
+    rc = facet_rows<D>(c, mesh, g, pkey, parea, npair);
+    if (rc) return rc;
+    double* vol = iget<double>(c, "i_vol", c->N);
+    int64_t* ptr = iget<int64_t>(c, "i_ptr", c->N + 1);
+    double* area = iget<double>(c, "i_marea", std::max<int64_t>(npair, 1));
+    unsigned long long* ccnt = iget<unsigned long long>(c, "i_ccnt", c->N);
+    unsigned long long* pcnt = iget<unsigned long long>(c, "i_pcnt", 2 * std::max<int64_t>(npair, 1));
+    if (!vol || !ptr || !area || !ccnt || !pcnt) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (volumes)");
+    // neighbour ranges (the exact volumes computed on the way are overwritten)
+    k_cell_volumes<D><<<nblk(c->N + 1, 256), 256, 0, c->stream>>>(g, pkey, parea, npair, vol, ptr);
+    HVCK(hipGetLastError());
+    HVCK(hipMemsetAsync(ccnt, 0, sizeof(unsigned long long) * c->N, c->stream));
+    HVCK(hipMemsetAsync(pcnt, 0, sizeof(unsigned long long) * 2 * std::max<int64_t>(npair, 1), c->stream));
+    int64_t n = 0;
+    double eps = 0.0, vbox = 0.0;
+    rc = hv_mc_points_impl<D>(c, nsamples, seed, slab, ptr, pkey, ccnt, pcnt, &n, &eps, &vbox);
+    if (rc) return rc;
+    k_mcp_finish<<<nblk(std::max<int64_t>(c->N, npair), 256), 256, 0, c->stream>>>(
+        (int)c->N, ptr, pkey, npair, ccnt, pcnt, (double)n, eps, vbox, vol, area);
+    HVCK(hipGetLastError());
+    HVCK(hipEventRecord(c->ev[1], c->stream));
+    HVCK(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HVCK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    return finish<D>(c, pkey, area, vol, ptr, npair, ms, out);
+}
+
 int hv_integral_counts_impl(const hv_integral* I, int64_t* ncells, int64_t* nnbr) {
     if (!I) return HV_ERR_ARG;
     if (ncells) *ncells = I->N;
@@ -817,7 +884,8 @@ void hv_integral_free_impl(hv_integral* I) { delete I; }
 
 #define HV_IINST(D)                                                                                 \
     template int hv_integrate_polytope_impl<D>(hv_ctx*, const hv_mesh*, hv_integral**);          \
-    template int hv_integrate_mc_impl<D>(hv_ctx*, const hv_mesh*, int64_t, uint64_t, hv_integral**);
+    template int hv_integrate_mc_impl<D>(hv_ctx*, const hv_mesh*, int64_t, uint64_t, hv_integral**); \
+    template int hv_integrate_mc_points_impl<D>(hv_ctx*, const hv_mesh*, int64_t, uint64_t, double, hv_integral**);
 HV_IINST(2)
 HV_IINST(3)
 HV_IINST(4)

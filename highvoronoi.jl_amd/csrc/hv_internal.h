@@ -54,6 +54,10 @@ int hv_fail(hv_ctx* c, int code, const std::string& msg);
 template <int D>
 int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out);
 template <int D>
+int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, const int64_t* ptr,
+                      const uint64_t* pkey, unsigned long long* ccnt, unsigned long long* pcnt, int64_t* nused,
+                      double* eps_out, double* vbox_out);
+template <int D>
 int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
                           const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status);
 template <int D>
@@ -70,6 +74,9 @@ template <int D>
 int hv_integrate_polytope_impl(hv_ctx* c, const hv_mesh* mesh, hv_integral** out);
 template <int D>
 int hv_integrate_mc_impl(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t seed, hv_integral** out);
+template <int D>
+int hv_integrate_mc_points_impl(hv_ctx* c, const hv_mesh* mesh, int64_t nsamples, uint64_t seed, double slab,
+                                hv_integral** out);
 int hv_integral_counts_impl(const hv_integral* I, int64_t* ncells, int64_t* nnbr);
 int hv_integral_export_impl(const hv_integral* I, double* vol, int64_t* nbr_ptr, int64_t* nbr, double* area,
                             double* ms);

@@ -1059,6 +1059,204 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
     }
 }
 
+// Point-sample Monte Carlo: wavefront w holds samples 64w .. 64w+63, all in stratum
+// floor(64w / S) of a G^d grid over the domain box (S samples per stratum, a multiple of 64),
+// each uniform in its stratum from a counter-based stream.  A packet walk as in k_knn64 finds
+// each sample's nearest and second-nearest generators (FP64 distances to the staged leaves).
+// Accumulated: the samples of each nearest cell (one atomic per distinct cell of the
+// wavefront), and per (cell, neighbour) row the samples that lie within eps (counter 2 row)
+// and within 2 eps (counter 2 row + 1) of the bisector of their two nearest generators, or of
+// a boundary plane, on the cell's side.
+__device__ __forceinline__ double hv_sample_unif(uint64_t seed, uint64_t idx) {
+    return (double)(hv_mix64(seed ^ ((idx + 1ull) * 0x9E3779B97F4A7C15ull)) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ void hv_row_add(const int64_t* __restrict__ ptr, const uint64_t* __restrict__ pkey,
+                                           unsigned long long* __restrict__ pcnt, int cell, int nb, bool inner) {
+    int64_t lo = ptr[cell], hi = ptr[cell + 1];
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int)(pkey[mid] & 0xffffffffull) < nb) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < ptr[cell + 1] && (int)(pkey[lo] & 0xffffffffull) == nb) {
+        atomicAdd(&pcnt[2 * lo + 1], 1ull);
+        if (inner) atomicAdd(&pcnt[2 * lo], 1ull);
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, uint64_t seed, double eps,
+                                                  const int64_t* __restrict__ ptr, const uint64_t* __restrict__ pkey,
+                                                  unsigned long long* __restrict__ ccnt,
+                                                  unsigned long long* __restrict__ pcnt) {
+    __shared__ int stk[128], stkl[128];
+    __shared__ float cbx[16 * D];
+    __shared__ unsigned long long hms[8];
+    __shared__ float dcs[8];
+    __shared__ double px[D][64];
+    const int lane = threadIdx.x;
+    const int N = g.N;
+    const int64_t smp = (int64_t)hv_xcd_block() * 64 + lane;
+    double p[D];
+    float pf[D];
+    {
+        int64_t b = smp / S;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const int ck = (int)(b % G);
+            b /= G;
+            const double wk = (g.dhi[k] - g.dlo[k]) / (double)G;
+            p[k] = g.dlo[k] + ((double)ck + hv_sample_unif(seed, (uint64_t)smp * D + k)) * wk;
+            pf[k] = (float)p[k];
+        }
+    }
+    double bd0 = 1e300, bd1 = 1e300;
+    int bi0 = -1, bi1 = -1;
+    auto consider = [&](int q, double d2) {
+        const bool f = d2 < bd0, s2 = !f && d2 < bd1;
+        bd1 = f ? bd0 : (s2 ? d2 : bd1);
+        bi1 = f ? bi0 : (s2 ? q : bi1);
+        bd0 = f ? d2 : bd0;
+        bi0 = f ? q : bi0;
+    };
+    // packet centre (uniform) orders the children
+    float pcf[D];
+    {
+        double cc[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) cc[k] = p[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) cc[k] += __shfl_xor(cc[k], off);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            pcf[k] = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint((float)(cc[k] * (1.0 / 64.0)))));
+    }
+    auto child_d2f = [&](int c, const float* q) {
+        float d2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float a = q[k] - __builtin_amdgcn_fmed3f(q[k], cbx[c * 2 * D + k], cbx[c * 2 * D + D + k]);
+            d2 = fmaf(a, a, d2);
+        }
+        return d2;
+    };
+    const double slack = 1e-6 * (g.xmax + 1.0);
+    auto reach = [&]() {   // widened FP32 radius^2 of the current second neighbour
+        const double r = sqrt(bd1) + slack;
+        return (float)fmin(r * r * (1.0 + 1e-5), 1e30);
+    };
+    auto pick = [&](unsigned rem, bool nearest) {
+        int best = -1;
+        float bdist = 0.f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float v = dcs[c];
+            const bool in = (rem >> c) & 1u;
+            const bool better = best < 0 || (nearest ? v < bdist : v > bdist);
+            if (in && better) { best = c; bdist = v; }
+        }
+        return best;
+    };
+    int sp = 1;
+    stk[0] = 0;
+    stkl[0] = 0;
+    while (sp > 0) {
+        --sp;
+        hv_wave_sync();
+        const int gid = __builtin_amdgcn_readfirstlane(stk[sp]);
+        const int l = __builtin_amdgcn_readfirstlane(stkl[sp]);
+        {
+            const float* src = g.cb + (int64_t)gid * (16 * D);
+            for (int q = lane; q < 16 * D; q += 64) cbx[q] = src[q];
+        }
+        hv_wave_sync();
+        const float R2f = reach();
+        unsigned rem = 0u;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const unsigned long long m = __ballot(child_d2f(c, pf) <= R2f);
+            if (lane == 0) hms[c] = m;
+            rem |= (m != 0ull) ? (1u << c) : 0u;
+        }
+        if (lane < 8) dcs[lane] = child_d2f(lane, pcf);
+        rem = __builtin_amdgcn_readfirstlane(rem);
+        hv_wave_sync();
+        if (rem == 0u) continue;
+        const int off = ((1 << (3 * l)) - 1) / 7;
+        const int i = gid - off;
+        if (l + 1 == g.h) {
+            while (rem) {
+                const int cc = __builtin_amdgcn_readfirstlane(pick(rem, true));
+                rem &= ~(1u << cc);
+                const int leaf = 8 * i + cc;
+                const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
+                const bool hit = ((hms[cc] >> lane) & 1ull) && (child_d2f(cc, pf) <= reach());
+                if (__ballot(hit) == 0ull) continue;
+                for (int s0 = 0; s0 < npts; s0 += 64) {
+                    const int cnt = min(64, npts - s0);
+                    if (lane < cnt) {
+#pragma unroll
+                        for (int k = 0; k < D; ++k) px[k][lane] = g.xs[(int64_t)(j0 + s0 + lane) * (D + 1) + k];
+                    }
+                    hv_wave_sync();
+                    if (hit) {
+                        for (int q = 0; q < cnt; ++q) {
+                            double d2 = 0.0;
+#pragma unroll
+                            for (int k = 0; k < D; ++k) {
+                                const double a = px[k][q] - p[k];
+                                d2 = fma(a, a, d2);
+                            }
+                            consider(j0 + s0 + q, d2);
+                        }
+                    }
+                    hv_wave_sync();
+                }
+            }
+        } else {
+            const int cbase = off + (1 << (3 * l)) + 8 * i;
+            while (rem) {
+                const int c = __builtin_amdgcn_readfirstlane(pick(rem, false));
+                rem &= ~(1u << c);
+                stk[sp] = cbase + c;
+                stkl[sp] = l + 1;
+                ++sp;
+            }
+        }
+    }
+    const int cell = g.sidx[bi0];
+    // samples per nearest cell: one atomic per distinct cell of the wavefront
+    unsigned long long todo = __ballot(true);
+    while (todo) {
+        const int l0 = __ffsll((long long)todo) - 1;
+        const int cl = __builtin_amdgcn_readlane(cell, l0);
+        const unsigned long long same = __ballot(cell == cl) & todo;
+        if (lane == l0) atomicAdd(&ccnt[cl], (unsigned long long)__popcll(same));
+        todo &= ~same;
+    }
+    // slabs: the bisector of the two nearest generators, the boundary planes
+    if (bi1 >= 0) {
+        double dd = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double a = g.xs[(int64_t)bi0 * (D + 1) + k] - g.xs[(int64_t)bi1 * (D + 1) + k];
+            dd = fma(a, a, dd);
+        }
+        const double gap = bd1 - bd0, w = 2.0 * eps * sqrt(dd);   // distance to the bisector = gap / (2|xi - xj|)
+        if (gap < 2.0 * w) hv_row_add(ptr, pkey, pcnt, cell, g.sidx[bi1], gap < w);
+    }
+    for (int q = 0; q < g.P; ++q) {
+        double dist = g.ph[q];
+#pragma unroll
+        for (int k = 0; k < D; ++k) dist = fma(-g.pn[q * D + k], p[k], dist);
+        if (dist < 2.0 * eps) hv_row_add(ptr, pkey, pcnt, cell, N + q, dist < eps);
+    }
+}
+
 // MODE 0: FP64 leaf scan.  MODE 1: BF16 MFMA filter, MODE 2: FP32 filter, each with FP64
 // re-verification of the survivors (leaves of 32 points).  MODE 1 is held to 128 VGPRs
 // (4 wavefronts per SIMD): the few values that spill are the rarely used FP64 ray data.
@@ -2636,7 +2834,42 @@ int hv_inrange_impl(hv_ctx* c, int64_t nq, const double* q, const double* radius
     return HV_OK;
 }
 
+// Point-sample Monte Carlo counts (k_mc_points) for the rows (ptr, pkey) of the context's
+// mesh.  nsamples is rounded up to a whole number of strata of S samples (a multiple of 64);
+// the number used, the slab width and the box volume are returned.
+template <int D>
+int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, const int64_t* ptr,
+                      const uint64_t* pkey, unsigned long long* ccnt, unsigned long long* pcnt, int64_t* nused,
+                      double* eps_out, double* vbox_out) {
+    int rc = check_box(c, D);
+    if (rc) return rc;
+    Tree T;
+    rc = build_tree<D>(c, 32, T);
+    if (rc) return rc;
+    const DevGeo& g = T.g;
+    double vbox = 1.0;
+    for (int k = 0; k < D; ++k) vbox *= g.dhi[k] - g.dlo[k];
+    int G = std::max(1, (int)std::floor(std::pow((double)nsamples / 256.0, 1.0 / D)));
+    G = std::min(G, (int)std::floor(std::pow(2.0, 40.0 / D)));   // G^d strata stay below 2^40
+    int64_t B = 1;
+    for (int k = 0; k < D; ++k) B *= G;
+    int64_t S = (nsamples + B - 1) / B;
+    S = (S + 63) / 64 * 64;
+    const int64_t n = B * S;
+    if (n / 64 > (int64_t)INT32_MAX) return hv_fail(c, HV_ERR_CAPACITY, "too many samples");
+    const double eps = slab * std::pow(vbox / (double)c->N, 1.0 / D);
+    k_mc_points<D><<<(unsigned)(n / 64), 64, 0, c->stream>>>(g, G, S, seed, eps, ptr, pkey, ccnt, pcnt);
+    HVCK(hipGetLastError());
+    *nused = n;
+    *eps_out = eps;
+    *vbox_out = vbox;
+    return HV_OK;
+}
+
 #define HV_INST(D)                                                                                 \
+    template int hv_mc_points_impl<D>(hv_ctx*, int64_t, uint64_t, double, const int64_t*,         \
+                                      const uint64_t*, unsigned long long*, unsigned long long*,   \
+                                      int64_t*, double*, double*);                                 \
     template int hv_build_mesh_impl<D>(hv_ctx*, const hv_opts*, hv_mesh**);                        \
     template int hv_mesh_export_impl<D>(const hv_mesh*, int64_t*, double*);                        \
     template int hv_raycast_batch_impl<D>(hv_ctx*, int, int64_t, const int64_t*, const int32_t*,   \

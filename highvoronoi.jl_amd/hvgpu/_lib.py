@@ -78,6 +78,7 @@ def lib():
     L.hv_inrange_batch.argtypes = [vp, C.c_int64, P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_int64)]
     L.hv_integrate_polytope.argtypes = [vp, vp, P(vp)]
     L.hv_integrate_mc.argtypes = [vp, vp, C.c_int64, C.c_uint64, P(vp)]
+    L.hv_integrate_mc_points.argtypes = [vp, vp, C.c_int64, C.c_uint64, C.c_double, P(vp)]
     L.hv_integral_counts.argtypes = [vp, P(C.c_int64), P(C.c_int64)]
     L.hv_integral_export.argtypes = [vp, P(C.c_double), P(C.c_int64), P(C.c_int64), P(C.c_double), P(C.c_double)]
     L.hv_integral_free.argtypes = [vp]
@@ -207,9 +208,11 @@ class Context:
         self._check(lib().hv_nn_halfspace_batch(self._h, n, _dp(q), _dp(u), _dp(c), _i64(idx), _dp(dist)))
         return idx, dist
 
-    def integrate(self, mesh: Mesh, method: str = "polytope", ndir: int = 1000, seed: int = 0) -> dict:
+    def integrate(self, mesh: Mesh, method: str = "polytope", ndir: int = 1000, seed: int = 0,
+                  nsamples: int = 10_000_000, slab: float = 0.0) -> dict:
         """Cell volumes and interface areas of `mesh` (built from this context's points):
-        method "polytope" (exact) or "montecarlo" (ndir directions per cell).  Returns
+        method "polytope" (exact), "montecarlo" (ndir directions per cell) or "points"
+        (nsamples stratified point samples, slab width factor `slab`).  Returns
         {"volumes", "nbr_ptr", "neighbors" (1-based, planes n+p), "areas", "ms"}."""
         L = lib()
         h = C.c_void_p()
@@ -217,6 +220,9 @@ class Context:
             self._check(L.hv_integrate_polytope(self._h, mesh._h, C.byref(h)))
         elif method == "montecarlo":
             self._check(L.hv_integrate_mc(self._h, mesh._h, int(ndir), int(seed), C.byref(h)))
+        elif method == "points":
+            self._check(L.hv_integrate_mc_points(self._h, mesh._h, int(nsamples), int(seed), float(slab),
+                                                 C.byref(h)))
         else:
             raise ValueError(f"unknown integration method {method!r}")
         try:

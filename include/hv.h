@@ -158,6 +158,17 @@ int hv_integrate_polytope(hv_ctx* ctx, const hv_mesh* mesh, hv_integral** out);
  * path, Montecarlo_Integrator (src/mcintegrator.jl:29-226, src/cleanup_cell.jl:6-48), with
  * independent directions per cell drawn from a counter-based stream keyed by seed. */
 int hv_integrate_mc(hv_ctx* ctx, const hv_mesh* mesh, int64_t ndir, uint64_t seed, hv_integral** out);
+/* Point-sample Monte-Carlo volumes and areas: the Monte-Carlo integration of the cells
+ * (Montecarlo_Integrator, src/mcintegrator.jl:29-226, with cleanup_cell, src/cleanup_cell.jl:6-48)
+ * computed from nsamples stratified uniform samples of the domain box instead of directions per
+ * cell: each sample's nearest and second-nearest generators (tree search, FP64 distances) give
+ * volume_i = V_box * (samples in cell i) / n and, for each interface, the samples c1 within
+ * eps = slab * (V_box / n_generators)^(1/d) of it and c2 within 2 eps (bisector of the two
+ * nearest generators, or a boundary plane): area = V_box * (2 c1 - c2 / 2) / (n * eps) (the
+ * first-order slab bias extrapolated away), generator interfaces averaged over both sides.
+ * slab <= 0 selects 0.03.  nsamples is rounded up to whole strata (multiples of 64). */
+int hv_integrate_mc_points(hv_ctx* ctx, const hv_mesh* mesh, int64_t nsamples, uint64_t seed, double slab,
+                           hv_integral** out);
 /* Voronoi_Integral's parity surface (src/integral.jl:11-21): volumes[ncells]; for cell i
  * its neighbours nbr[nbr_ptr[i] .. nbr_ptr[i+1]) sorted, 1-based (planes n+p), and the
  * interface areas aligned with them.  ms (optional) receives the device time. */
