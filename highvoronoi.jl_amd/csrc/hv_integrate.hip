@@ -687,6 +687,142 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey
     return HV_OK;
 }
 
+// ---- neighbour rows without face measures (the Monte-Carlo integrators) -------------------
+// Cells i and k are neighbours when i and k occur together in a vertex key (src/neighbors.jl:
+// 219-251; general position).  Every unordered pair (a < b, a a generator) of every live
+// vertex key goes into a device hash set; the set is compacted, mirrored to (a, b) and (b, a)
+// rows and sorted, which gives the rows facet_rows leaves (keys (cell << 32) | neighbour).
+constexpr uint64_t HV_PAIR_EMPTY = ~0ull;
+
+template <int D>
+__global__ void k_pair_insert(const int* __restrict__ ids, int nv, const int* __restrict__ key, int N,
+                              unsigned long long* __restrict__ tab, uint64_t mask, int* __restrict__ full) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;
+    const int* k = key + (int64_t)ids[t] * (D + 1);
+    int kk[D + 1];
+#pragma unroll
+    for (int j = 0; j <= D; ++j) kk[j] = k[j];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        if (kk[a] >= N) break;   // planes sort last
+#pragma unroll
+        for (int b = a + 1; b <= D; ++b) {
+            const unsigned long long pk = ((unsigned long long)kk[a] << 32) | (unsigned)kk[b];
+            uint64_t h = hv_mix64(pk) & mask;
+            bool done = false;
+            for (uint64_t probe = 0; probe <= mask && !done; ++probe) {
+                const unsigned long long cur = tab[h];
+                if (cur == pk) { done = true; break; }
+                if (cur == HV_PAIR_EMPTY) {
+                    const unsigned long long old = atomicCAS(&tab[h], HV_PAIR_EMPTY, pk);
+                    if (old == HV_PAIR_EMPTY || old == pk) { done = true; break; }
+                }
+                h = (h + 1) & mask;
+            }
+            if (!done) atomicExch(full, 1);
+        }
+    }
+}
+
+__global__ void k_pair_mirror(const unsigned long long* __restrict__ u, int64_t nu, int N,
+                              uint64_t* __restrict__ rows, int* __restrict__ count) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ok = t < nu;
+    const unsigned long long pk = ok ? u[t] : 0ull;
+    const int a = (int)(pk >> 32), b = (int)(pk & 0xffffffffull);
+    const bool both = ok && b < N;
+    int o = hv_wave_alloc(count, ok);
+    if (ok) rows[o] = pk;
+    o = hv_wave_alloc(count, both);
+    if (both) rows[o] = ((uint64_t)b << 32) | (unsigned)a;
+}
+
+__global__ void k_row_ptr(const uint64_t* __restrict__ rows, int64_t nrow, int N, int64_t* __restrict__ ptr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > N) return;
+    const uint64_t want = (uint64_t)i << 32;
+    int64_t lo = 0, hi = nrow;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (rows[mid] < want) lo = mid + 1;
+        else hi = mid;
+    }
+    ptr[i] = lo;
+}
+
+struct PairNotEmpty {
+    __host__ __device__ bool operator()(const unsigned long long& v) const { return v != HV_PAIR_EMPTY; }
+};
+
+// neighbours per cell, a generous bound for the hash-set size (d = 2..6, unit-cube densities)
+static int64_t neighbours_bound(int D) {
+    static const int nb[7] = {0, 0, 8, 24, 64, 160, 400};
+    return nb[std::min(std::max(D, 2), 6)];
+}
+
+template <int D>
+static int neighbor_rows(hv_ctx* c, const hv_mesh* mesh, uint64_t*& pkey, int64_t& npair, int64_t*& ptr) {
+    const int N = (int)c->N;
+    const int nv = (int)mesh->nv;
+    int* full = iget<int>(c, "n_full", 1);
+    int* cnt = iget<int>(c, "n_cnt", 2);
+    ptr = iget<int64_t>(c, "i_ptr", (size_t)N + 1);
+    if (!full || !cnt || !ptr) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (neighbours)");
+    uint64_t tsize = 1024;
+    while ((int64_t)tsize < 2 * (int64_t)N * neighbours_bound(D) / 2 + 1024) tsize <<= 1;
+    unsigned long long* tab = nullptr;
+    for (int attempt = 0;; ++attempt) {
+        tab = iget<unsigned long long>(c, "n_tab", tsize);
+        if (!tab) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (neighbour set)");
+        HVCK(hipMemsetAsync(tab, 0xff, tsize * sizeof(unsigned long long), c->stream));
+        HVCK(hipMemsetAsync(full, 0, sizeof(int), c->stream));
+        if (nv > 0) {
+            k_pair_insert<D><<<nblk(nv, 256), 256, 0, c->stream>>>((const int*)mesh->ids.p, nv, (const int*)mesh->key.p,
+                                                                   N, tab, tsize - 1, full);
+            HVCK(hipGetLastError());
+        }
+        int hfull = 0;
+        HVCK(hipMemcpyAsync(&hfull, full, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        if (!hfull) break;
+        if (attempt >= 4) return hv_fail(c, HV_ERR_CAPACITY, "neighbour set overflow");
+        tsize <<= 2;
+    }
+    // compact the set, mirror, sort
+    unsigned long long* u = iget<unsigned long long>(c, "n_u", tsize);
+    if (!u) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (neighbours)");
+    size_t tb = 0;
+    HVCK(hipcub::DeviceSelect::If(nullptr, tb, tab, u, cnt, (int)tsize, PairNotEmpty(), c->stream));
+    void* tmp = c->pool->get("n_tmp", tb, false, c->stream);
+    if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (neighbours)");
+    HVCK(hipcub::DeviceSelect::If(tmp, tb, tab, u, cnt, (int)tsize, PairNotEmpty(), c->stream));
+    int nu = 0;
+    HVCK(hipMemcpyAsync(&nu, cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    uint64_t* rows = iget<uint64_t>(c, "n_rows", 2 * (size_t)nu);
+    uint64_t* rows2 = iget<uint64_t>(c, "n_rows2", 2 * (size_t)nu);
+    if (!rows || !rows2) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (neighbours)");
+    HVCK(hipMemsetAsync(cnt, 0, sizeof(int), c->stream));
+    if (nu > 0) {
+        k_pair_mirror<<<nblk(nu, 256), 256, 0, c->stream>>>(u, nu, N, rows, cnt);
+        HVCK(hipGetLastError());
+    }
+    int nr = 0;
+    HVCK(hipMemcpyAsync(&nr, cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    tb = 0;
+    HVCK(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, rows, rows2, nr, 0, 64, c->stream));
+    tmp = c->pool->get("n_tmp", tb, false, c->stream);
+    if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (neighbours)");
+    HVCK(hipcub::DeviceRadixSort::SortKeys(tmp, tb, rows, rows2, nr, 0, 64, c->stream));
+    k_row_ptr<<<nblk((int64_t)N + 1, 256), 256, 0, c->stream>>>(rows2, nr, N, ptr);
+    HVCK(hipGetLastError());
+    pkey = rows2;
+    npair = nr;
+    return HV_OK;
+}
+
 template <int D>
 static int make_geo(hv_ctx* c, const hv_mesh* mesh, IntGeo& g) {
     if (!mesh || mesh->D != D || mesh->N != c->N) return hv_fail(c, HV_ERR_STATE, "the mesh was not built from this context's points");
@@ -767,17 +903,13 @@ int hv_integrate_mc_impl(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t 
     if (rc) return rc;
     HVCK(hipEventRecord(c->ev[0], c->stream));
     uint64_t* pkey = nullptr;
-    double* parea = nullptr;
     int64_t npair = 0;
-    rc = facet_rows<D>(c, mesh, g, pkey, parea, npair);
+    int64_t* ptr = nullptr;
+    rc = neighbor_rows<D>(c, mesh, pkey, npair, ptr);
     if (rc) return rc;
     double* vol = iget<double>(c, "i_vol", c->N);
-    int64_t* ptr = iget<int64_t>(c, "i_ptr", c->N + 1);
     double* area = iget<double>(c, "i_marea", npair);
-    if (!vol || !ptr || !area) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (volumes)");
-    // neighbour ranges (the exact volumes computed on the way are overwritten)
-    k_cell_volumes<D><<<nblk(c->N + 1, 256), 256, 0, c->stream>>>(g, pkey, parea, npair, vol, ptr);
-    HVCK(hipGetLastError());
+    if (!vol || !area) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (volumes)");
     std::vector<int64_t> hptr(c->N + 1);
     HVCK(hipMemcpyAsync(hptr.data(), ptr, sizeof(int64_t) * (c->N + 1), hipMemcpyDeviceToHost, c->stream));
     HVCK(hipStreamSynchronize(c->stream));
@@ -833,19 +965,15 @@ int hv_integrate_mc_points_impl(hv_ctx* c, const hv_mesh* mesh, int64_t nsamples
     if (rc) return rc;
     HVCK(hipEventRecord(c->ev[0], c->stream));
     uint64_t* pkey = nullptr;
-    double* parea = nullptr;
     int64_t npair = 0;
-    rc = facet_rows<D>(c, mesh, g, pkey, parea, npair);
+    int64_t* ptr = nullptr;
+    rc = neighbor_rows<D>(c, mesh, pkey, npair, ptr);
     if (rc) return rc;
     double* vol = iget<double>(c, "i_vol", c->N);
-    int64_t* ptr = iget<int64_t>(c, "i_ptr", c->N + 1);
     double* area = iget<double>(c, "i_marea", std::max<int64_t>(npair, 1));
     unsigned long long* ccnt = iget<unsigned long long>(c, "i_ccnt", c->N);
     unsigned long long* pcnt = iget<unsigned long long>(c, "i_pcnt", 2 * std::max<int64_t>(npair, 1));
-    if (!vol || !ptr || !area || !ccnt || !pcnt) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (volumes)");
-    // neighbour ranges (the exact volumes computed on the way are overwritten)
-    k_cell_volumes<D><<<nblk(c->N + 1, 256), 256, 0, c->stream>>>(g, pkey, parea, npair, vol, ptr);
-    HVCK(hipGetLastError());
+    if (!vol || !area || !ccnt || !pcnt) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (volumes)");
     HVCK(hipMemsetAsync(ccnt, 0, sizeof(unsigned long long) * c->N, c->stream));
     HVCK(hipMemsetAsync(pcnt, 0, sizeof(unsigned long long) * 2 * std::max<int64_t>(npair, 1), c->stream));
     int64_t n = 0;
