@@ -235,6 +235,69 @@ def bench_mc_points(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_integrators(args):
+    """Wall time (best of steps, after warm-up) of one mesh build and of each integrator on it,
+    and the Monte-Carlo errors (relative L1 of volumes and of areas) against the exact one."""
+    from hvgpu import synth
+    from hvgpu._lib import Context
+
+    ctx = Context(0)
+
+    def timed(fn):
+        for _ in range(max(1, args.warmup)):
+            r = fn()
+            if hasattr(r, "free"):
+                r.free()
+        best, res = None, None
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            r = fn()
+            dt = time.perf_counter() - t0
+            if res is not None and hasattr(res, "free"):
+                res.free()
+            res = r
+            best = dt if best is None else min(best, dt)
+        return res, best * 1e3
+
+    rows = {}
+    for name, d, n, seed in [("2D N=1000", 2, 1000, 0), ("4D N=20000", 4, 20000, 1), ("6D N=50000", 6, 50000, 4)]:
+        xs = synth.uniform(d, n, seed)
+        base, normal = synth.cube_planes(d)
+        ctx.set_points(xs)
+        ctx.set_boundary(base, normal)
+        m, tm = timed(lambda: ctx.build_mesh(precision=args.precision))
+        ex, tp = timed(lambda: ctx.integrate(m, "polytope"))
+        ndir = 1000 if d < 6 else 200
+        md, td = timed(lambda: ctx.integrate(m, "montecarlo", ndir=ndir, seed=1))
+        mp, tq = timed(lambda: ctx.integrate(m, "points", nsamples=100_000_000, seed=1))
+        nv = m.stats()["vertices"]
+        m.free()
+        A = ex["areas"]
+
+        def err(r):
+            return [float(np.sum(np.abs(r["volumes"] - ex["volumes"])) / np.sum(ex["volumes"])),
+                    float(np.sum(np.abs(r["areas"] - A)) / np.sum(A))]
+
+        rows[name] = {"vertices": nv, "mesh_ms": tm, "polytope_ms": tp, "directional_mc_ms": td,
+                      "directional_mc_ndir": ndir, "directional_mc_l1_vol_area": err(md),
+                      "points_mc_ms": tq, "points_mc_samples": 100_000_000, "points_mc_l1_vol_area": err(mp)}
+        log(f"[integrators] {name}: {rows[name]}")
+    out = {
+        "metric": "integrator wall time per call (mesh, exact polytope, directional MC, point-sample MC)",
+        "value": rows["6D N=50000"]["points_mc_ms"],
+        "unit": "ms (6D N=50k point-sample MC, 1e8 samples)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "higher_is_better": False,
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "uniform points (seeds 0, 1, 4), cuboid(d, periodic=[])", "cases": rows},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +316,9 @@ def main():
     ap.add_argument("--mc-points", action="store_true",
                     help="the point-sample Monte-Carlo integrator instead: 6D N=50k uniform, cuboid(6, periodic=[]), "
                          "1e8 samples, against the exact integrator on the same mesh")
+    ap.add_argument("--integrators", action="store_true",
+                    help="mesh + the three integrators (exact polytope, directional MC, point-sample MC) at "
+                         "2D N=1000, 4D N=20k and 6D N=50k, with their errors against the exact integrator")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy (3x): the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters into bytes")
@@ -278,6 +344,8 @@ def main():
         return bench_batched(args)
     if args.mc_points:
         return bench_mc_points(args)
+    if args.integrators:
+        return bench_integrators(args)
 
     xs = synth.gaussian_mixture_5d()
     d = xs.shape[1]
