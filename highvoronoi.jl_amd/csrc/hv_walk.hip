@@ -899,7 +899,6 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
     __shared__ int stk[128], stkl[128];
     __shared__ float cbx[16 * D];
     __shared__ unsigned long long hms[8];
-    __shared__ float dcs[8];
     __shared__ double px[D][64];
     const int lane = threadIdx.x;
     const int N = g.N;
@@ -983,17 +982,14 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
         const double r = sqrt(bd[HV_KNN - 1]) + slack;
         return (float)fmin(r * r * (1.0 + 1e-5), 1e30);
     };
-    auto pick = [&](unsigned rem, bool nearest) {
-        int best = -1;
-        float bdist = 0.f;
+    // children ordered by distance to the packet centre (ties: lower index first): rank r
+    // holds child (ord >> 4r) & 7
+    uint32_t ord = 0u;
+    auto to_rank = [&](unsigned cm) {
+        unsigned rr = 0u;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const float v = dcs[c];
-            const bool in = (rem >> c) & 1u;
-            const bool better = best < 0 || (nearest ? v < bdist : v > bdist);
-            if (in && better) { best = c; bdist = v; }
-        }
-        return best;
+        for (int q = 0; q < 8; ++q) rr |= ((cm >> ((ord >> (4 * q)) & 7u)) & 1u) << q;
+        return rr;
     };
     int sp = 1;
     stk[0] = 0;
@@ -1016,16 +1012,30 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
             if (lane == 0) hms[c] = m;
             rem |= (m != 0ull) ? (1u << c) : 0u;
         }
-        if (lane < 8) dcs[lane] = child_d2f(lane, pcf);
+        {
+            const uint32_t kb = __float_as_uint(child_d2f(lane & 7, pcf));   // non-negative: orders as unsigned
+            const int me = lane & 7;
+            int rk = 0;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const uint32_t kc = __builtin_amdgcn_readlane(kb, c);
+                rk += (kc < kb || (kc == kb && c < me)) ? 1 : 0;
+            }
+            uint32_t o = 0u;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) o |= (uint32_t)c << (4 * __builtin_amdgcn_readlane(rk, c));
+            ord = o;
+        }
         rem = __builtin_amdgcn_readfirstlane(rem);
         hv_wave_sync();
         if (rem == 0u) continue;
         const int off = ((1 << (3 * l)) - 1) / 7;
         const int i = gid - off;
         if (l + 1 == g.h) {
-            while (rem) {
-                const int cc = __builtin_amdgcn_readfirstlane(pick(rem, true));
-                rem &= ~(1u << cc);
+            unsigned rr = to_rank(rem);
+            while (rr) {
+                const int cc = (ord >> (4 * __builtin_ctz(rr))) & 7u;
+                rr &= rr - 1u;
                 const int leaf = 8 * i + cc;
                 const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
                 if (j0 >= base && j0 + npts <= base + 64) continue;   // the packet's own points
@@ -1044,9 +1054,11 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
             }
         } else {
             const int cbase = off + (1 << (3 * l)) + 8 * i;
-            while (rem) {
-                const int c = __builtin_amdgcn_readfirstlane(pick(rem, false));
-                rem &= ~(1u << c);
+            unsigned rr = to_rank(rem);
+            while (rr) {
+                const int q = 31 - __builtin_clz(rr);
+                rr &= ~(1u << q);
+                const int c = (ord >> (4 * q)) & 7u;
                 stk[sp] = cbase + c;
                 stkl[sp] = l + 1;
                 ++sp;
@@ -1093,7 +1105,6 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
     __shared__ int stk[128], stkl[128];
     __shared__ float cbx[16 * D];
     __shared__ unsigned long long hms[8];
-    __shared__ float dcs[8];
     __shared__ double px[D][64];
     const int lane = threadIdx.x;
     const int N = g.N;
@@ -1149,17 +1160,12 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
         const double r = sqrt(bd1) + slack;
         return (float)fmin(r * r * (1.0 + 1e-5), 1e30);
     };
-    auto pick = [&](unsigned rem, bool nearest) {
-        int best = -1;
-        float bdist = 0.f;
+    uint32_t ord = 0u;   // children by distance to the packet centre: rank r holds (ord >> 4r) & 7
+    auto to_rank = [&](unsigned cm) {
+        unsigned rr = 0u;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const float v = dcs[c];
-            const bool in = (rem >> c) & 1u;
-            const bool better = best < 0 || (nearest ? v < bdist : v > bdist);
-            if (in && better) { best = c; bdist = v; }
-        }
-        return best;
+        for (int q = 0; q < 8; ++q) rr |= ((cm >> ((ord >> (4 * q)) & 7u)) & 1u) << q;
+        return rr;
     };
     int sp = 1;
     stk[0] = 0;
@@ -1182,16 +1188,30 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
             if (lane == 0) hms[c] = m;
             rem |= (m != 0ull) ? (1u << c) : 0u;
         }
-        if (lane < 8) dcs[lane] = child_d2f(lane, pcf);
+        {
+            const uint32_t kb = __float_as_uint(child_d2f(lane & 7, pcf));
+            const int me = lane & 7;
+            int rk = 0;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const uint32_t kc = __builtin_amdgcn_readlane(kb, c);
+                rk += (kc < kb || (kc == kb && c < me)) ? 1 : 0;
+            }
+            uint32_t o = 0u;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) o |= (uint32_t)c << (4 * __builtin_amdgcn_readlane(rk, c));
+            ord = o;
+        }
         rem = __builtin_amdgcn_readfirstlane(rem);
         hv_wave_sync();
         if (rem == 0u) continue;
         const int off = ((1 << (3 * l)) - 1) / 7;
         const int i = gid - off;
         if (l + 1 == g.h) {
-            while (rem) {
-                const int cc = __builtin_amdgcn_readfirstlane(pick(rem, true));
-                rem &= ~(1u << cc);
+            unsigned rr = to_rank(rem);
+            while (rr) {
+                const int cc = (ord >> (4 * __builtin_ctz(rr))) & 7u;
+                rr &= rr - 1u;
                 const int leaf = 8 * i + cc;
                 const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
                 const bool hit = ((hms[cc] >> lane) & 1ull) && (child_d2f(cc, pf) <= reach());
@@ -1219,9 +1239,11 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
             }
         } else {
             const int cbase = off + (1 << (3 * l)) + 8 * i;
-            while (rem) {
-                const int c = __builtin_amdgcn_readfirstlane(pick(rem, false));
-                rem &= ~(1u << c);
+            unsigned rr = to_rank(rem);
+            while (rr) {
+                const int q = 31 - __builtin_clz(rr);
+                rr &= ~(1u << q);
+                const int c = (ord >> (4 * q)) & 7u;
                 stk[sp] = cbase + c;
                 stkl[sp] = l + 1;
                 ++sp;
