@@ -148,11 +148,15 @@ struct DevRays {
 // Workgroups are handed to the 8 XCDs round-robin (block b -> XCD b % 8), each XCD with its
 // own L2.  Renumber so that XCD x works on one contiguous range of blocks: neighbouring
 // packets (Morton order) then share an L2 instead of spreading every region over all eight.
-__device__ __forceinline__ int hv_xcd_block() {
-    const int nb = gridDim.x, b = blockIdx.x;
+__device__ __forceinline__ int hv_xcd_block_of(int nb) {
+    const int b = blockIdx.x;
     const int x = b & 7, per = nb >> 3, extra = nb & 7;
     return x * per + min(x, extra) + (b >> 3);
 }
+__device__ __forceinline__ int hv_xcd_block() { return hv_xcd_block_of(gridDim.x); }
+// The same for the first nb blocks of a larger grid (the count known only on the device):
+// blocks 0 .. nb-1 still go round-robin to all eight XCDs; -1 for the blocks past nb.
+__device__ __forceinline__ int hv_xcd_block_first(int nb) { return (int)blockIdx.x < nb ? hv_xcd_block_of(nb) : -1; }
 
 __device__ __forceinline__ int hv_wave_alloc(int* counter, bool want) {
     const unsigned long long m = __ballot(want);

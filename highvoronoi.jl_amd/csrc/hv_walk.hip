@@ -344,12 +344,16 @@ __device__ __forceinline__ void hv_block_count4(unsigned long long* ctr, bool c0
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_insert(const int* __restrict__ status, int n, const int* __restrict__ vkey,
-                                                int64_t base, int* __restrict__ htab, uint64_t hmask,
-                                                uint8_t* __restrict__ win, unsigned long long* __restrict__ ctr) {
+__global__ __launch_bounds__(256) void k_insert(const int* __restrict__ status, int n, const int* __restrict__ n_dev,
+                                                const int* __restrict__ vkey, int64_t base, int* __restrict__ htab,
+                                                uint64_t hmask, uint8_t* __restrict__ win,
+                                                unsigned long long* __restrict__ ctr) {
+    // n_dev (when given): the candidate count in device memory; flags up to n are written
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nv = n_dev ? min(n, *n_dev) : n;
     bool dup = false, full = false, none = false, singular = false;
-    if (t < n) {
+    if (t >= nv && t < n) win[t] = 0;
+    if (t < nv) {
         uint8_t won = 0;
         none = status[t] == HV_ST_NONE;
         singular = status[t] == HV_ST_SINGULAR;
@@ -1286,7 +1290,8 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
 // packet, so that there are enough packets to fill the chip and each one walks fewer leaves.
 template <int D, int MODE>
 __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
-                                                  int ppw, const int* __restrict__ vkey, const double* __restrict__ vx,
+                                                  const int* __restrict__ nrays_dev, int ppw,
+                                                  const int* __restrict__ vkey, const double* __restrict__ vx,
                                                   int* __restrict__ status, int* __restrict__ win,
                                                   double* __restrict__ tout, double plane_tol,
                                                   unsigned long long* __restrict__ ctr) {
@@ -1302,7 +1307,11 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
     __shared__ int kid[MODE >= 1 ? 64 : 1][D];
     __shared__ double nog[MODE >= 1 ? 8 : 1];
     const int lane = threadIdx.x & 63;
-    const int rid = hv_xcd_block() * ppw + lane;
+    // nrays_dev (when given): the ray count in device memory, nrays then only bounds the grid
+    if (nrays_dev) nrays = min(nrays, *nrays_dev);
+    const int pk = hv_xcd_block_first((nrays + ppw - 1) / ppw);
+    if (pk < 0) return;   // the whole packet is past the last ray
+    const int rid = pk * ppw + lane;
     const bool act = lane < ppw && rid < nrays;
     const int ray = act ? order[rid] : order[0];
     const int N = g.N;
@@ -1845,23 +1854,26 @@ static inline int hv_packet_rays(int64_t n) {
 }
 
 template <int D>
-static void launch_scan(int precision, const DevGeo& g, const DevRays& R, const int* list, int n, const int* vkey,
-                        const double* vx, int* status, int* win, double* tout, double plane_tol,
+static void launch_scan(int precision, const DevGeo& g, const DevRays& R, const int* list, int n, const int* n_dev,
+                        const int* vkey, const double* vx, int* status, int* win, double* tout, double plane_tol,
                         unsigned long long* ctr, hipStream_t st) {
     const int ppw = hv_packet_rays(n);
     if (precision == HV_PREC_BF16)
-        k_raycast64<D, 1><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
+        k_raycast64<D, 1><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, n_dev, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
     else if (precision == HV_PREC_FP32)
-        k_raycast64<D, 2><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
+        k_raycast64<D, 2><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, n_dev, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
     else
-        k_raycast64<D, 0><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
+        k_raycast64<D, 0><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, n_dev, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
 }
 
 template <int D>
 __global__ __launch_bounds__(256) void k_finalize(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
-                           int* __restrict__ vkey, double* __restrict__ vx, int64_t base,
-                           int* __restrict__ status, const int* __restrict__ win) {
-    const int rid = hv_xcd_block() * blockDim.x + threadIdx.x;
+                           const int* __restrict__ nrays_dev, int* __restrict__ vkey, double* __restrict__ vx,
+                           int64_t base, int* __restrict__ status, const int* __restrict__ win) {
+    if (nrays_dev) nrays = min(nrays, *nrays_dev);
+    const int blk = hv_xcd_block_first((nrays + (int)blockDim.x - 1) / (int)blockDim.x);
+    if (blk < 0) return;
+    const int rid = blk * blockDim.x + threadIdx.x;
     if (rid >= nrays) return;
     if (status[rid] <= 0) return;
     const int ray = order[rid];
@@ -2270,10 +2282,12 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     };
     // candidates at provisional ids base .. base+n with status > 0 -> hash -> new ids appended at ids[nlive]
     // cnt_dev: leave the count in device memory (no host wait) instead of returning it in nsel
-    auto insert_and_select = [&](const int* status, int n, int64_t base, int& nsel, int* cnt_dev = nullptr) -> int {
+    // n_dev: the candidate count in device memory (n then bounds it)
+    auto insert_and_select = [&](const int* status, int n, int64_t base, int& nsel, int* cnt_dev = nullptr,
+                                 const int* n_dev = nullptr) -> int {
         uint8_t* win = pget<uint8_t>(c, "win", n);
         if (!win) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (win)");
-        k_insert<D><<<nblk(n, 256), 256, 0, c->stream>>>(status, n, vkey, base, htab, hsize - 1, win, ctr + 8);
+        k_insert<D><<<nblk(n, 256), 256, 0, c->stream>>>(status, n, n_dev, vkey, base, htab, hsize - 1, win, ctr + 8);
         HVCK(hipGetLastError());
         size_t tb = 0;
         hipcub::CountingInputIterator<int> it((int)base);
@@ -2406,23 +2420,23 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         // scan a list of rays, then new vertices -> hash -> registered edges
         // defer: the new-vertex count stays in dcount[1] (the caller reads it with its next
         // wait and then advances nlive and the scan time)
-        auto process = [&](const int* list, int n, bool defer) -> int {
+        // n_dev (defer only): the ray count is in device memory and n bounds it
+        auto process = [&](const int* list, int n, bool defer, const int* n_dev = nullptr) -> int {
             int r_ = ensure_vertices(n);
             if (r_) return r_;
             HVCK(hipEventRecord(r0, c->stream));
-            launch_scan<D>(prec, g, R, list, n, vkey, vx, status, rwin, rt, plane_tol, ctr + 0, c->stream);
+            launch_scan<D>(prec, g, R, list, n, n_dev, vkey, vx, status, rwin, rt, plane_tol, ctr + 0, c->stream);
             HVCK(hipGetLastError());
             HVCK(hipEventRecord(r1, c->stream));
-            k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, list, n, vkey, vx, vnext, status, rwin);
+            k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, list, n, n_dev, vkey, vx, vnext, status, rwin);
             HVCK(hipGetLastError());
             int nsel = 0;
             if (defer) {
-                r_ = insert_and_select(status, n, vnext, nsel, dcount + 1);
+                r_ = insert_and_select(status, n, vnext, nsel, dcount + 1, n_dev);
                 if (r_) return r_;
                 r_ = register_new(R, nlive, n, n, reg_next + (nlive - level_begin) * (D + 1), dcount + 1);
                 if (r_) return r_;
-                st.rays += n;
-                vnext += n;
+                vnext += n;   // provisional slots of all n; st.rays and nlive follow the caller's wait
                 return HV_OK;
             }
             r_ = insert_and_select(status, n, vnext, nsel);
@@ -2471,23 +2485,29 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
             HVCK(hipGetLastError());
             k_gather_flags<<<nblk(nr, 256), 256, 0, c->stream>>>(ord, nr, pflag, keep);
             HVCK(hipGetLastError());
-            int np = 0;
-            rc = select(ord, keep, nr, ordc, np);
-            if (rc) return rc;
-            if (np) {
-                rc = process(ordc, np, true);
-                if (rc) return rc;
+            // the primary rays (count in dcount[2]) are scanned without a host wait: the kernels
+            // read the count and nr bounds their grids
+            {
+                size_t tb = 0;
+                HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb, ord, keep, ordc, dcount + 2, nr, c->stream));
+                void* tmp = c->pool->get("cubtmp2", tb, false, c->stream);
+                if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
+                HVCK(hipcub::DeviceSelect::Flagged(tmp, tb, ord, keep, ordc, dcount + 2, nr, c->stream));
             }
+            rc = process(ordc, nr, true, dcount + 2);
+            if (rc) return rc;
             HVCK(hipMemsetAsync(dcount, 0, sizeof(int), c->stream));
             k_carry_out<D><<<nblk(nr, 256), 256, 0, c->stream>>>(R, nr, pflag, rtab, cs, cv, cd, cu, cc, cdu, dcount);
             HVCK(hipGetLastError());
-            // one wait for the carried rays (dcount[0]) and the level's new vertices (dcount[1])
-            int hc[2] = {0, 0};
-            HVCK(hipMemcpyAsync(hc, dcount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            // one wait: carried rays (dcount[0]), new vertices (dcount[1]), scanned rays (dcount[2])
+            int hc[3] = {0, 0, 0};
+            HVCK(hipMemcpyAsync(hc, dcount, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
             HVCK(hipStreamSynchronize(c->stream));
             ncar = hc[0];
-            if (np) {
-                nlive += hc[1];
+            const int np = hc[2];
+            nlive += hc[1];
+            st.rays += np;
+            {
                 float ms = 0.f;
                 HVCK(hipEventElapsedTime(&ms, r0, r1));
                 ms_ray += ms;
@@ -2774,9 +2794,9 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     k_iota<<<nblk(nrays, 256), 256, 0, c->stream>>>(ord0, (int)nrays);
     HVCK(hipGetLastError());
     HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, (int)nrays, HV_RAY_SORT_LO, 60, c->stream));
-    launch_scan<D>(precision, g, R, ord, (int)nrays, vkey, vx, status, rwin, tout, plane_tol, ctr, c->stream);
+    launch_scan<D>(precision, g, R, ord, (int)nrays, nullptr, vkey, vx, status, rwin, tout, plane_tol, ctr, c->stream);
     HVCK(hipGetLastError());
-    k_finalize<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, nrays, status, rwin);
+    k_finalize<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, ord, (int)nrays, nullptr, vkey, vx, nrays, status, rwin);
     HVCK(hipGetLastError());
     std::vector<int> hord(nrays), hst(nrays), hk((size_t)nrays * (D + 1));
     std::vector<double> hx((size_t)nrays * D), ht(nrays);
