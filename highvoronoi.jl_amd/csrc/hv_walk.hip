@@ -36,6 +36,7 @@ static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / 
 #define HV_ST_FOUND 1
 #define HV_ST_TIE 2
 #define HV_ST_SINGULAR -2
+#define HV_ST_DROPPED -1   // a seed candidate that was not kept (counted nowhere)
 
 // ---------------------------------------------------------------------------------
 // points and tree
@@ -1844,6 +1845,8 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
 // Packets of 64 rays, halved (down to 8) while the level would give fewer than
 // HV_MIN_PACKETS packets: a packet's walk is serial, and a few long walks leave the chip idle.
 constexpr int64_t HV_MIN_PACKETS = 2048;
+// one interior seed vertex per this many generators (Morton order)
+constexpr int HV_SEED_STRIDE = 256;
 // Rays are ordered by the top 32 of the 60 Morton bits of their ball centre (2^(32/d) cells
 // per axis: far finer than a level's rays are spread), half the radix passes of all 60.
 constexpr int HV_RAY_SORT_LO = 28;
@@ -1905,6 +1908,10 @@ __global__ void k_iota(int* __restrict__ a, int n) {
 
 // ---------------------------------------------------------------------------------
 // host driver
+
+template <int D>
+__global__ void k_seed_descent(DevGeo g, int nseed, int stride, int* __restrict__ vkey, double* __restrict__ vx,
+                               int64_t base, int* __restrict__ status);
 
 namespace {
 
@@ -2320,23 +2327,33 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         return HV_OK;
     };
 
-    // seeds: the 2^d box corners
-    int64_t* reg_cur = pget<int64_t>(c, "regA", (size_t)nseed * (D + 1));
+    // seeds: the 2^d box corners and interior vertices found by descent, one per
+    // HV_SEED_STRIDE generators in Morton order (more fronts, fewer levels)
+    const int ndesc = N >= 2 * HV_SEED_STRIDE ? N / HV_SEED_STRIDE : 0;
+    const int nstart = nseed + ndesc;
+    int64_t* reg_cur = pget<int64_t>(c, "regA", (size_t)nstart * (D + 1));
     if (!reg_cur) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
     {
-        int* status = pget<int>(c, "status", nseed);
+        rc = ensure_vertices(nstart);
+        if (rc) return rc;
+        int* status = pget<int>(c, "status", nstart);
         if (!status) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
         k_seed_corners<D><<<nseed, 256, 0, c->stream>>>(g, vkey, vx, vnext, status);
         HVCK(hipGetLastError());
+        if (ndesc > 0) {
+            k_seed_descent<D><<<nblk(ndesc, 128), 128, 0, c->stream>>>(g, ndesc, HV_SEED_STRIDE, vkey, vx,
+                                                                       vnext + nseed, status + nseed);
+            HVCK(hipGetLastError());
+        }
         int nsel = 0;
-        rc = insert_and_select(status, nseed, vnext, nsel);
+        rc = insert_and_select(status, nstart, vnext, nsel);
         if (rc) return rc;
         DevRays none{};
         rc = build_rt(none, 0);
         if (rc) return rc;
         rc = register_new(none, nlive, nsel, nsel, reg_cur);
         if (rc) return rc;
-        vnext += nseed;
+        vnext += nstart;
         nlive += nsel;
     }
     bool reg_is_a = true;
@@ -2720,6 +2737,172 @@ __global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const 
         node = hv_next_node(node);
     }
     if (counts) counts[t] = cnt;
+}
+
+// Interior seed vertices (the reference's descent, src/raycast.jl:45-109, without its random
+// restarts): from generator i = the one at sorted position (t + 1/2) * stride, d steps, each
+// moving along a direction inside the current face (orthogonal to x_s - x_i for the members s
+// found so far) to the first generator that becomes equidistant, its candidates the 16 nearest
+// neighbours of the members.  A seed is kept only if it is certain: away from every boundary
+// plane, and exactly its d + 1 generators within (1 + 1e-9) times its radius (tree count), so
+// the walk, reaching the same vertex, gives it the same key; its coordinates are computed
+// from the key as every other vertex's.  status[t] = HV_ST_FOUND kept, HV_ST_DROPPED not.
+template <int D>
+__global__ void k_seed_descent(DevGeo g, int nseed, int stride, int* __restrict__ vkey, double* __restrict__ vx,
+                               int64_t base, int* __restrict__ status) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nseed) return;
+    const int N = g.N;
+    int S[D + 1];
+    const int js = min(N - 1, t * stride + stride / 2);
+    S[0] = g.sidx[js];
+    double xi[D], p[D], q[D][D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) { xi[k] = g.xs[(int64_t)js * (D + 1) + k]; p[k] = xi[k]; }
+    bool ok = true;
+#pragma unroll
+    for (int step = 0; step < D; ++step) {
+        // direction: a fixed pseudo-random vector, orthogonalised against the face's normals
+        double u[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            u[k] = (double)(hv_mix64(((uint64_t)t << 8) + (uint64_t)(step * D + k) + 0x5eedull) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+#pragma unroll
+        for (int rep = 0; rep < 2; ++rep) {
+#pragma unroll
+            for (int a = 0; a < step; ++a) {
+                double dp = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) dp = fma(u[k], q[a][k], dp);
+#pragma unroll
+                for (int k = 0; k < D; ++k) u[k] = fma(-dp, q[a][k], u[k]);
+            }
+        }
+        double nu = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) nu = fma(u[k], u[k], nu);
+        if (!(nu > 1e-12)) { ok = false; break; }
+        nu = 1.0 / sqrt(nu);
+#pragma unroll
+        for (int k = 0; k < D; ++k) u[k] *= nu;
+        double pi2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) pi2 = fma(p[k] - xi[k], p[k] - xi[k], pi2);
+        double best = INFINITY;
+        int bo = -1, bj = -1;
+        for (int a = 0; a <= step; ++a) {
+            const int* nb = g.knn + (int64_t)g.rank[S[a]] * HV_KNN;
+            for (int m = 0; m < HV_KNN; ++m) {
+                const int j = nb[m];
+                const int o = g.sidx[j];
+                bool member = false;
+                for (int b = 0; b <= step; ++b) member = member || (S[b] == o);
+                if (member) continue;
+                double den = 0.0, px2 = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const double x = g.xs[(int64_t)j * (D + 1) + k];
+                    den = fma(u[k], x - xi[k], den);
+                    px2 = fma(p[k] - x, p[k] - x, px2);
+                }
+                if (!(den > 0.0)) continue;
+                const double tt = (px2 - pi2) / (2.0 * den);
+                if (tt < best || (tt == best && o < bo)) { best = tt; bo = o; bj = j; }
+            }
+        }
+        if (bo < 0 || !(best > 0.0) || !(best < INFINITY)) { ok = false; break; }
+        // a boundary plane reached first: not an interior seed
+        for (int pl = 0; pl < g.P; ++pl) {
+            double nu2 = 0.0, np = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) { nu2 = fma(g.pn[pl * D + k], u[k], nu2); np = fma(g.pn[pl * D + k], p[k], np); }
+            if (nu2 > 0.0 && (g.ph[pl] - np) / nu2 <= best) ok = false;
+        }
+        if (!ok) break;
+#pragma unroll
+        for (int k = 0; k < D; ++k) p[k] = fma(best, u[k], p[k]);
+        S[step + 1] = bo;
+        if (step + 1 < D) {
+            // normal of the new bisector, orthonormalised into the face basis
+            double w[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) w[k] = g.xs[(int64_t)bj * (D + 1) + k] - xi[k];
+#pragma unroll
+            for (int rep = 0; rep < 2; ++rep) {
+#pragma unroll
+                for (int a = 0; a < step; ++a) {
+                    double dp = 0.0;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) dp = fma(w[k], q[a][k], dp);
+#pragma unroll
+                    for (int k = 0; k < D; ++k) w[k] = fma(-dp, q[a][k], w[k]);
+                }
+            }
+            double nw = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) nw = fma(w[k], w[k], nw);
+            if (!(nw > 1e-24)) { ok = false; break; }
+            nw = 1.0 / sqrt(nw);
+#pragma unroll
+            for (int k = 0; k < D; ++k) q[step][k] = w[k] * nw;
+        }
+    }
+    int key[D + 1];
+    double xn[D];
+    if (ok) {
+        // sorted key (original ids), then the vertex exactly as the walk computes it
+#pragma unroll
+        for (int a = 0; a <= D; ++a) key[a] = S[a];
+        for (int a = 1; a <= D; ++a)
+            for (int b = a; b > 0 && key[b - 1] > key[b]; --b) { const int x = key[b]; key[b] = key[b - 1]; key[b - 1] = x; }
+        ok = hv_vertex_coords<D>(g, key, xn);
+    }
+    if (ok) {
+        // well inside the box, and exactly the key's generators on or within the sphere
+        double r2 = 0.0, ext = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double a = xn[k] - g.xo[(int64_t)key[0] * D + k];
+            r2 = fma(a, a, r2);
+            ext = fmax(ext, g.dhi[k] - g.dlo[k]);
+        }
+        for (int pl = 0; pl < g.P && ok; ++pl) {
+            double np = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) np = fma(g.pn[pl * D + k], xn[k], np);
+            if (!(g.ph[pl] - np > 1e-9 * ext + sqrt(r2) * 1e-6)) ok = false;
+        }
+        const double rr = r2 * (1.0 + 2e-9);
+        int cnt = 0;
+        int node = 1;
+        while (node && ok) {
+            if (hv_box_d2<D>(g, node, xn) <= rr) {
+                if (node < g.P2) { node = 2 * node; continue; }
+                const int leaf = node - g.P2;
+                const int j0 = leaf * g.L, j1 = min(j0 + g.L, N);
+                for (int j = j0; j < j1; ++j) {
+                    double d2 = 0.0;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) {
+                        const double a = g.xs[(int64_t)j * (D + 1) + k] - xn[k];
+                        d2 = fma(a, a, d2);
+                    }
+                    cnt += (d2 <= rr) ? 1 : 0;
+                }
+                if (cnt > D + 1) ok = false;
+            }
+            node = hv_next_node(node);
+        }
+        ok = ok && cnt == D + 1;
+    }
+    if (ok) {
+        const int64_t id = base + t;
+#pragma unroll
+        for (int a = 0; a <= D; ++a) vkey[id * (D + 1) + a] = key[a];
+#pragma unroll
+        for (int k = 0; k < D; ++k) vx[id * D + k] = xn[k];
+    }
+    status[t] = ok ? HV_ST_FOUND : HV_ST_DROPPED;
 }
 
 template <int D>
