@@ -1846,7 +1846,7 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
 // HV_MIN_PACKETS packets: a packet's walk is serial, and a few long walks leave the chip idle.
 constexpr int64_t HV_MIN_PACKETS = 2048;
 // one interior seed vertex per this many generators (Morton order)
-constexpr int HV_SEED_STRIDE = 256;
+constexpr int HV_SEED_STRIDE = 8;
 // Rays are ordered by the top 32 of the 60 Morton bits of their ball centre (2^(32/d) cells
 // per axis: far finer than a level's rays are spread), half the radix passes of all 60.
 constexpr int HV_RAY_SORT_LO = 28;
