@@ -187,3 +187,13 @@ def test_repeated_builds_are_identical(ctx):
     assert key_digest(k1) == key_digest(k2)
     assert np.array_equal(k1, k2)
     assert np.array_equal(c1, c2)
+
+
+def test_interior_seeds_keep_the_walk_shallow(ctx):
+    # descent seeds every few generators: the breadth-first walk needs a handful of levels
+    # instead of a number growing like N^(1/d) from the box corners alone
+    xs = synth.uniform(2, 200_000, 3)
+    k, c, st = build(ctx, xs)
+    assert st["vertices"] == len(k) == 2 * 200_000 + 2
+    assert st["levels"] <= 20
+    assert st["unbounded"] == 0 and st["degenerate"] == 0
