@@ -47,6 +47,9 @@ struct DevGeo {
     // FP32 scan (leaves of 32): per sorted point 8 floats, its coordinates relative to the same
     // group origin, then |x - o|^2 (slot D; +1e30 for padding records past N), zeros after
     const float* __restrict__ pxf;
+    // per sorted point its coordinates in FP32, 8-float rows (zero padded): the ray start's
+    // pre-ranking of its candidates
+    const float* __restrict__ xf;
 };
 
 #define HV_KNN 16

@@ -237,6 +237,14 @@ __global__ void k_point_kvec(DevGeo g, const double* __restrict__ lorg, hv_u32x4
 }
 
 template <int D>
+__global__ void k_point_rows32(DevGeo g, float* __restrict__ xf) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= g.N) return;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) xf[(int64_t)j * 8 + k] = k < D ? (float)g.xs[(int64_t)j * (D + 1) + k] : 0.f;
+}
+
+template <int D>
 __global__ void k_point_f32(DevGeo g, const double* __restrict__ lorg, float* __restrict__ pxf) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= g.nleaf * 32) return;
@@ -848,7 +856,25 @@ __global__ __launch_bounds__(256) void k_ray_init(DevGeo g, DevRays R, int nrays
         const double tp = (g.ph[p] - nr) / nu;
         if (tp >= plane_tol) T.offer(tp, id);
     }
-    // the winner is a Delaunay neighbour of every generator of the edge: scan their lists
+    // the winner is a Delaunay neighbour of every generator of the edge: scan their lists.
+    // FP32 pre-ranking on the difference form t = (|v|^2 + 2<v, x0 - r>) / (2<u, v>), v = x - x0
+    // (small magnitudes), then the best three in FP64 (hv_consider): the start only needs
+    // exact t of real candidates, which candidates win it changes nothing downstream.
+    float uf[D], wf[D], x0f[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        uf[k] = (float)G.u[k];
+        wf[k] = (float)(G.x0[k] - G.r[k]);
+        x0f[k] = (float)G.x0[k];
+    }
+    const float cf = (float)(G.cth - G.ux0);
+    // the edge's own generators lie on the bisector (their FP32 <u, v> is rounding noise)
+    int er[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) er[a] = G.e[a] < N ? g.rank[G.e[a]] : -1;
+    float b0 = INFINITY, b1 = INFINITY, b2 = INFINITY;
+    int j0b = -1, j1b = -1, j2b = -1;
+    const float4* xf4 = (const float4*)g.xf;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         const int ea = G.e[a];
@@ -866,6 +892,42 @@ __global__ __launch_bounds__(256) void k_ray_init(DevGeo g, DevRays R, int nrays
 #pragma unroll
         for (int m = 0; m < HV_KNN; ++m) {
             const int j = jv[m];
+            bool edge = false;
+#pragma unroll
+            for (int a2 = 0; a2 < D; ++a2) edge = edge || (j == er[a2]);
+            if (edge) continue;
+            const float4 A = xf4[(int64_t)j * 2];
+            float x[8] = {A.x, A.y, A.z, A.w, 0.f, 0.f, 0.f, 0.f};
+            if constexpr (D > 4) {
+                const float4 B = xf4[(int64_t)j * 2 + 1];
+                x[4] = B.x; x[5] = B.y; x[6] = B.z; x[7] = B.w;
+            }
+            float vu = 0.f, vv = 0.f, vw = 0.f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float v = x[k] - x0f[k];
+                vu = fmaf(uf[k], v, vu);
+                vv = fmaf(v, v, vv);
+                vw = fmaf(v, wf[k], vw);
+            }
+            if (!(vu > cf)) continue;
+            const float tt = __fdividef(fmaf(2.f, vw, vv), 2.f * vu);
+            if (j == j0b || j == j1b || j == j2b) continue;
+            const bool c0 = tt < b0, c1 = !c0 && tt < b1, c2 = !c0 && !c1 && tt < b2;
+            b2 = (c0 || c1) ? b1 : (c2 ? tt : b2);
+            j2b = (c0 || c1) ? j1b : (c2 ? j : j2b);
+            b1 = c0 ? b0 : (c1 ? tt : b1);
+            j1b = c0 ? j0b : (c1 ? j : j1b);
+            b0 = c0 ? tt : b0;
+            j0b = c0 ? j : j0b;
+        }
+    }
+    {
+        const int jb[3] = {j0b, j1b, j2b};
+#pragma unroll
+        for (int z = 0; z < 3; ++z) {
+            const int j = jb[z];
+            if (j < 0) continue;
             double xk[D];
 #pragma unroll
             for (int k = 0; k < D; ++k) xk[k] = g.xs[(int64_t)(j) * (D + 1) + k];
@@ -1910,8 +1972,8 @@ __global__ void k_iota(int* __restrict__ a, int n) {
 // host driver
 
 template <int D>
-__global__ void k_seed_descent(DevGeo g, int nseed, int stride, int* __restrict__ vkey, double* __restrict__ vx,
-                               int64_t base, int* __restrict__ status);
+__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int stride, int* __restrict__ vkey,
+                                                     double* __restrict__ vx, int64_t base, int* __restrict__ status);
 
 namespace {
 
@@ -1993,7 +2055,8 @@ int build_tree(hv_ctx* c, int L, Tree& T, int tables = HV_PREC_FP64) {
     HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tmpb, code, code2, idx, idx2, N, 0, 64, c->stream));
     int* rank = pget<int>(c, "rank", N);
     int* knn = pget<int>(c, "knn", (size_t)N * HV_KNN);
-    if (!rank || !knn) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (knn)");
+    float* xf = pget<float>(c, "xf", (size_t)N * 8);
+    if (!rank || !knn || !xf) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (knn)");
     k_points_sorted<D><<<nblk(N, 256), 256, 0, c->stream>>>(xo, idx2, N, xs, sidx, rank);
     HVCK(hipGetLastError());
     g.xo = xo;
@@ -2001,6 +2064,9 @@ int build_tree(hv_ctx* c, int L, Tree& T, int tables = HV_PREC_FP64) {
     g.sidx = sidx;
     g.rank = rank;
     g.knn = knn;
+    k_point_rows32<D><<<nblk(N, 256), 256, 0, c->stream>>>(g, xf);
+    HVCK(hipGetLastError());
+    g.xf = xf;
     g.blo = blo;
     g.bhi = bhi;
     k_leaf_boxes<D><<<nblk(g.P2, 256), 256, 0, c->stream>>>(g, blo, bhi);
@@ -2341,7 +2407,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         k_seed_corners<D><<<nseed, 256, 0, c->stream>>>(g, vkey, vx, vnext, status);
         HVCK(hipGetLastError());
         if (ndesc > 0) {
-            k_seed_descent<D><<<nblk(ndesc, 128), 128, 0, c->stream>>>(g, ndesc, HV_SEED_STRIDE, vkey, vx,
+            k_seed_descent<D><<<nblk(ndesc, 64), 64, 0, c->stream>>>(g, ndesc, HV_SEED_STRIDE, vkey, vx,
                                                                        vnext + nseed, status + nseed);
             HVCK(hipGetLastError());
         }
@@ -2748,8 +2814,8 @@ __global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const 
 // the walk, reaching the same vertex, gives it the same key; its coordinates are computed
 // from the key as every other vertex's.  status[t] = HV_ST_FOUND kept, HV_ST_DROPPED not.
 template <int D>
-__global__ void k_seed_descent(DevGeo g, int nseed, int stride, int* __restrict__ vkey, double* __restrict__ vx,
-                               int64_t base, int* __restrict__ status) {
+__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int stride, int* __restrict__ vkey,
+                                                     double* __restrict__ vx, int64_t base, int* __restrict__ status) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nseed) return;
     const int N = g.N;
