@@ -298,6 +298,46 @@ def bench_integrators(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_scale(args):
+    """Large uniform inputs in the unit cube: best-of-steps build time for BF16 and FP64 after a
+    warm-up build, levels, and the key tables of the two precisions compared."""
+    from hvgpu import synth
+    from hvgpu._lib import Context
+
+    ctx = Context(0)
+    cases = {}
+    for d, n in [(2, 10_000_000), (3, 5_000_000), (5, 1_000_000), (6, 200_000)]:
+        xs = synth.uniform(d, n, 77)
+        base, normal = synth.cube_planes(d)
+        ctx.set_points(xs)
+        ctx.set_boundary(base, normal)
+        row, keys = {}, {}
+        for prec in ("bf16", "fp64"):
+            sec, stats = run_leg(ctx, prec, max(1, args.steps), max(1, args.warmup))
+            st = stats[-1]
+            row[prec] = {"ms": sec * 1e3, "vertices_per_s": st["vertices"] / sec, "levels": st["levels"]}
+            row["vertices"] = st["vertices"]
+            keys[prec] = mesh_table(ctx, prec)[0]
+        row["identical_vertex_set"] = bool(np.array_equal(keys["bf16"], keys["fp64"]))
+        del keys
+        cases[f"{d}D N={n}"] = row
+        log(f"[scale] {d}D N={n}: {row}")
+    out = {
+        "metric": "Voronoi vertices/s of large uniform inputs (BF16 build)",
+        "value": cases["5D N=1000000"]["bf16"]["vertices_per_s"],
+        "unit": "vertices/s (5D N=1M)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "bf16+f64",
+        "data": "synthetic",
+        "config": {"workload": "uniform points in [0,1]^d (seed 77), cuboid(d, periodic=[])", "cases": cases},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,6 +359,9 @@ def main():
     ap.add_argument("--integrators", action="store_true",
                     help="mesh + the three integrators (exact polytope, directional MC, point-sample MC) at "
                          "2D N=1000, 4D N=20k and 6D N=50k, with their errors against the exact integrator")
+    ap.add_argument("--scale", action="store_true",
+                    help="large uniform inputs (2D N=10M, 3D N=5M, 5D N=1M, 6D N=200k): vertices/s of the BF16 "
+                         "and FP64 builds and whether their key tables agree")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy (3x): the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters into bytes")
@@ -346,6 +389,8 @@ def main():
         return bench_mc_points(args)
     if args.integrators:
         return bench_integrators(args)
+    if args.scale:
+        return bench_scale(args)
 
     xs = synth.gaussian_mixture_5d()
     d = xs.shape[1]
