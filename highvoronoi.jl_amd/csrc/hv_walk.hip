@@ -966,25 +966,22 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
     __shared__ int stk[128], stkl[128];
     __shared__ float cbx[16 * D];
     __shared__ unsigned long long hms[8];
-    __shared__ double px[D][64];
+    __shared__ float px[D][64];
     const int lane = threadIdx.x;
     const int N = g.N;
     const int base = hv_xcd_block() * 64;
     const int j = base + lane;
     const bool act = j < N;
     const int jj = act ? j : N - 1;
-    double p[D];
+    // FP32 distances: the lists only supply candidates that are always evaluated in FP64
     float pf[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-        p[k] = g.xs[(int64_t)jj * (D + 1) + k];
-        pf[k] = (float)p[k];
-    }
-    double bd[HV_KNN];
+    for (int k = 0; k < D; ++k) pf[k] = (float)g.xs[(int64_t)jj * (D + 1) + k];
+    float bd[HV_KNN];
     int bi[HV_KNN];
 #pragma unroll
-    for (int m = 0; m < HV_KNN; ++m) { bd[m] = 1e300; bi[m] = jj; }
-    auto consider = [&](int q, double d2) {
+    for (int m = 0; m < HV_KNN; ++m) { bd[m] = 3.0e38f; bi[m] = jj; }
+    auto consider = [&](int q, float d2) {
         if (!(d2 < bd[HV_KNN - 1]) || q == jj) return;
         int pos = 0;
 #pragma unroll
@@ -1002,11 +999,11 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
         for (int s = 0; s < npts; ++s) {
             const int q = q0 + s;
             if (skip_own && q >= base && q < base + 64) continue;
-            double d2 = 0.0;
+            float d2 = 0.f;
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-                const double a = px[k][s] - p[k];
-                d2 = fma(a, a, d2);
+                const float a = px[k][s] - pf[k];
+                d2 = fmaf(a, a, d2);
             }
             consider(q, d2);
         }
@@ -1014,7 +1011,7 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
     {
         const int np = min(64, N - base);
 #pragma unroll
-        for (int k = 0; k < D; ++k) px[k][lane] = p[k];
+        for (int k = 0; k < D; ++k) px[k][lane] = pf[k];
         hv_wave_sync();
         scan_staged(base, np, false);
         hv_wave_sync();
@@ -1024,7 +1021,7 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
     {
         double c[D], cnt = act ? 1.0 : 0.0;
 #pragma unroll
-        for (int k = 0; k < D; ++k) c[k] = act ? p[k] : 0.0;
+        for (int k = 0; k < D; ++k) c[k] = act ? (double)pf[k] : 0.0;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
@@ -1044,10 +1041,10 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
         }
         return d2;
     };
-    const double slack = 1e-6 * (g.xmax + 1.0);
-    auto reach = [&]() {   // widened FP32 radius^2 of the current 16th neighbour
-        const double r = sqrt(bd[HV_KNN - 1]) + slack;
-        return (float)fmin(r * r * (1.0 + 1e-5), 1e30);
+    const float slack = 1e-6f * (float)(g.xmax + 1.0);
+    auto reach = [&]() {   // widened radius^2 of the current 16th neighbour
+        const float r = sqrtf(bd[HV_KNN - 1]) + slack;
+        return fminf(r * r * (1.0f + 1e-5f), 1e30f);
     };
     // children ordered by distance to the packet centre (ties: lower index first): rank r
     // holds child (ord >> 4r) & 7
@@ -1112,7 +1109,7 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
                     const int cnt = min(64, npts - s0);
                     if (lane < cnt) {
 #pragma unroll
-                        for (int k = 0; k < D; ++k) px[k][lane] = g.xs[(int64_t)(j0 + s0 + lane) * (D + 1) + k];
+                        for (int k = 0; k < D; ++k) px[k][lane] = (float)g.xs[(int64_t)(j0 + s0 + lane) * (D + 1) + k];
                     }
                     hv_wave_sync();
                     if (hit) scan_staged(j0 + s0, cnt, true);
