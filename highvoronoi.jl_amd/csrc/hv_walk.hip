@@ -1764,11 +1764,17 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
 #pragma unroll
                         for (int g4 = 0; g4 < 4; ++g4) {
                             if (((m >> (16 * g4)) & 0xffffull) == 0ull) continue;
+                            // both tiles issued before either result is read, so their latencies
+                            // overlap (the second tile of a leaf with at most 16 points reads the
+                            // padding records, whose form is positive for every ray)
+                            const hv_u32x4 bop = rk4[16 * g4 + (lane & 15)][lane >> 4];
+                            const hv_f32x4 acc0 = hv_mfma_bf16(ca0, bop);
+                            const hv_f32x4 acc1 = hv_mfma_bf16(ca1, bop);
 #pragma unroll
                             for (int t = 0; t < 2; ++t) {
                                 if (16 * t >= npts) continue;
                                 ++tiles;
-                                const hv_f32x4 acc = hv_mfma_bf16(t == 0 ? ca0 : ca1, rk4[16 * g4 + (lane & 15)][lane >> 4]);
+                                const hv_f32x4 acc = t == 0 ? acc0 : acc1;
                                 // lane holds points 16t + 4*(lane/16) + e of ray 16*g4 + lane%16
                                 const uint32_t ob = __float_as_uint(acc[0]) | __float_as_uint(acc[1]) |
                                                     __float_as_uint(acc[2]) | __float_as_uint(acc[3]);
