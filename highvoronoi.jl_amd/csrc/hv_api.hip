@@ -196,8 +196,13 @@ int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, con
         hv_destroy(c);
     };
     std::vector<std::thread> th;
-    th.reserve(nw);
-    for (int w = 0; w < nw; ++w) th.emplace_back(work);
+    try {
+        th.reserve(nw);
+        for (int w = 0; w < nw; ++w) th.emplace_back(work);
+    } catch (...) {
+        // fewer workers than asked (at least the ones started run every mesh); none: fail
+        if (th.empty()) return HV_ERR_DEVICE;
+    }
     for (auto& t : th) t.join();
     for (int k = 0; k < nmesh; ++k)
         if (status[k] != HV_OK) return status[k];

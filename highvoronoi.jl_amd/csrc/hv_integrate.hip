@@ -771,6 +771,7 @@ static int neighbor_rows(hv_ctx* c, const hv_mesh* mesh, uint64_t*& pkey, int64_
     if (!full || !cnt || !ptr) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (neighbours)");
     uint64_t tsize = 1024;
     while ((int64_t)tsize < 2 * (int64_t)N * neighbours_bound(D) / 2 + 1024) tsize <<= 1;
+    if (tsize > (1ull << 31)) return hv_fail(c, HV_ERR_CAPACITY, "too many cells for the neighbour set");
     unsigned long long* tab = nullptr;
     for (int attempt = 0;; ++attempt) {
         tab = iget<unsigned long long>(c, "n_tab", tsize);
@@ -786,7 +787,7 @@ static int neighbor_rows(hv_ctx* c, const hv_mesh* mesh, uint64_t*& pkey, int64_
         HVCK(hipMemcpyAsync(&hfull, full, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HVCK(hipStreamSynchronize(c->stream));
         if (!hfull) break;
-        if (attempt >= 4) return hv_fail(c, HV_ERR_CAPACITY, "neighbour set overflow");
+        if (attempt >= 4 || tsize >= (1ull << 29)) return hv_fail(c, HV_ERR_CAPACITY, "neighbour set overflow");
         tsize <<= 2;
     }
     // compact the set, mirror, sort
