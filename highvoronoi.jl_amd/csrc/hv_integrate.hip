@@ -771,7 +771,7 @@ static int neighbor_rows(hv_ctx* c, const hv_mesh* mesh, uint64_t*& pkey, int64_
     if (!full || !cnt || !ptr) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (neighbours)");
     uint64_t tsize = 1024;
     while ((int64_t)tsize < 2 * (int64_t)N * neighbours_bound(D) / 2 + 1024) tsize <<= 1;
-    if (tsize > (1ull << 31)) return hv_fail(c, HV_ERR_CAPACITY, "too many cells for the neighbour set");
+    if (tsize > (1ull << 30)) return hv_fail(c, HV_ERR_CAPACITY, "too many cells for the neighbour set");
     unsigned long long* tab = nullptr;
     for (int attempt = 0;; ++attempt) {
         tab = iget<unsigned long long>(c, "n_tab", tsize);
