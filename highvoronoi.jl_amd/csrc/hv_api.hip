@@ -136,6 +136,29 @@ static int set_points_common(hv_ctx* c, int d, int64_t n, const double* xs, hipM
     }
     c->D = d;
     c->N = n;
+    c->nreal = n;
+    c->extended = false;
+    c->img_orig.clear();
+    c->img_shift.clear();
+    ++c->gen;
+    return HV_OK;
+}
+
+// back from the extended set of a periodic build to the originals
+static int restore_originals(hv_ctx* c) {
+    if (!c->extended) return HV_OK;
+    const size_t bytes = sizeof(double) * c->D * c->nreal;
+    const void* src = c->pool->get("xorig", 0);
+    void* dst = c->pool->get("xin", bytes, false, c->stream);
+    if (!src || !dst) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return hv_fail(c, HV_ERR_DEVICE, "copy of the points failed");
+    c->N = c->nreal;
+    c->extended = false;
+    c->img_orig.clear();
+    c->img_shift.clear();
+    ++c->gen;
     return HV_OK;
 }
 
@@ -152,6 +175,9 @@ int hv_set_boundary(hv_ctx* c, int nplanes, const double* base, const double* no
     if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "hv_set_points must come first");
     if (nplanes < 0 || (nplanes > 0 && (!base || !normal))) return hv_fail(c, HV_ERR_ARG, "bad planes");
     const int D = c->D;
+    (void)hipSetDevice(c->device);
+    const int rc = restore_originals(c);
+    if (rc) return rc;
     c->P = nplanes;
     c->pbase.assign(base, base + (size_t)nplanes * D);
     c->pnormal.assign(normal, normal + (size_t)nplanes * D);
@@ -221,6 +247,18 @@ int hv_mesh_export(const hv_mesh* m, int64_t* sig, double* coords) {
     if (!m || !sig || !coords) return HV_ERR_ARG;
     (void)hipSetDevice(m->device);
     HV_DISPATCH(m->D, hv_mesh_export_impl, m, sig, coords)
+}
+
+int hv_mesh_images(const hv_mesh* m, int64_t* n_ext, int64_t* orig, int8_t* shift) {
+    if (!m || !n_ext) return HV_ERR_ARG;
+    *n_ext = m->N;
+    const bool per = !m->img_orig.empty();
+    for (int64_t i = 0; i < m->N; ++i) {
+        if (orig) orig[i] = (per ? (int64_t)m->img_orig[i] : i) + 1;
+        if (shift)
+            for (int k = 0; k < m->D; ++k) shift[i * m->D + k] = per ? m->img_shift[i * m->D + k] : 0;
+    }
+    return HV_OK;
 }
 
 int hv_mesh_stats(const hv_mesh* m, hv_stats_t* out) {

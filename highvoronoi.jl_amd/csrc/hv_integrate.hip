@@ -184,6 +184,7 @@ __global__ void k_edge_len(const int* __restrict__ fstart, int nface, const int*
 
 struct IntGeo {
     int N, P;
+    int nreal;          // cells integrated: 0 .. nreal-1 (N - nreal periodic images follow)
     const double* xo;   // N x D, original order
     const double* pn;   // P x D outer normals
     const double* ph;   // P offsets <n_p, b_p>
@@ -490,7 +491,7 @@ __global__ void k_mc_sym(IntGeo g, const int64_t* __restrict__ ptr, const uint64
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npair) return;
     const int i = (int)(pkey[t] >> 32), k = (int)(pkey[t] & 0xffffffffull);
-    if (k >= g.N || k >= i) return;
+    if (k >= g.N || k >= i || i >= g.nreal) return;
     int64_t lo = ptr[k], hi = ptr[k + 1];
     while (lo < hi) {
         const int64_t mid = (lo + hi) >> 1;
@@ -593,7 +594,8 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey
         uint8_t* head = iget<uint8_t>(c, "i_head", tot);
         if (!lo || !hi || !lo2 || !hi2 || !pa || !pb || !head) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
         HVCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
-        k_face_emit<D><<<nblk(nv, 256), 256, 0, c->stream>>>(vk, nv, m, N, C, lo, hi, pa, cnt);
+        // faces with an original generator (with periodic images, the others are incomplete)
+        k_face_emit<D><<<nblk(nv, 256), 256, 0, c->stream>>>(vk, nv, m, g.nreal, C, lo, hi, pa, cnt);
         HVCK(hipGetLastError());
         // two stable radix passes: by lo, then by hi -> (hi, lo) order
         size_t tb = 0;
@@ -769,8 +771,10 @@ static int neighbor_rows(hv_ctx* c, const hv_mesh* mesh, uint64_t*& pkey, int64_
     int* cnt = iget<int>(c, "n_cnt", 2);
     ptr = iget<int64_t>(c, "i_ptr", (size_t)N + 1);
     if (!full || !cnt || !ptr) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (neighbours)");
+    // rows of the walked cells (images add the pairs of their vertices' other generators)
+    const int64_t nrows = (int64_t)mesh->nreal * neighbours_bound(D) * (mesh->nreal < N ? 2 : 1);
     uint64_t tsize = 1024;
-    while ((int64_t)tsize < 2 * (int64_t)N * neighbours_bound(D) / 2 + 1024) tsize <<= 1;
+    while ((int64_t)tsize < nrows + 1024) tsize <<= 1;
     if (tsize > (1ull << 30)) return hv_fail(c, HV_ERR_CAPACITY, "too many cells for the neighbour set");
     unsigned long long* tab = nullptr;
     for (int attempt = 0;; ++attempt) {
@@ -827,7 +831,10 @@ static int neighbor_rows(hv_ctx* c, const hv_mesh* mesh, uint64_t*& pkey, int64_
 template <int D>
 static int make_geo(hv_ctx* c, const hv_mesh* mesh, IntGeo& g) {
     if (!mesh || mesh->D != D || mesh->N != c->N) return hv_fail(c, HV_ERR_STATE, "the mesh was not built from this context's points");
+    if (mesh->nreal != mesh->N && mesh->gen != c->gen)
+        return hv_fail(c, HV_ERR_STATE, "the periodic mesh was not built from this context's current point set");
     g.N = (int)c->N;
+    g.nreal = (int)mesh->nreal;
     g.P = c->P;
     g.xo = (const double*)c->pool->get("xin", 0);
     double* pn = iget<double>(c, "i_pn", (size_t)std::max(1, c->P) * D);
@@ -846,12 +853,20 @@ static int make_geo(hv_ctx* c, const hv_mesh* mesh, IntGeo& g) {
 }
 
 template <int D>
-static int finish(hv_ctx* c, const uint64_t* pkey, const double* area_d, const double* vol_d, const int64_t* ptr_d,
-                  int64_t npair, float ms, hv_integral** out) {
+static int finish(hv_ctx* c, int64_t ncell, const uint64_t* pkey, const double* area_d, const double* vol_d,
+                  const int64_t* ptr_d, int64_t npair, float ms, hv_integral** out) {
+    // cells 0 .. ncell-1 (the originals) and their rows, which come first in the sorted order
     hv_integral* I = new hv_integral();
-    I->N = c->N;
-    I->vol.resize(c->N);
-    I->ptr.resize(c->N + 1);
+    I->N = ncell;
+    I->vol.resize(ncell);
+    I->ptr.resize(ncell + 1);
+    if (ncell < c->N) {
+        if (hipMemcpyAsync(&npair, ptr_d + ncell, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            delete I;
+            return hv_fail(c, HV_ERR_DEVICE, "integration copy-out failed");
+        }
+    }
     I->area.resize(npair);
     I->nbr.resize(npair);
     std::vector<uint64_t> hk(npair);
@@ -860,8 +875,8 @@ static int finish(hv_ctx* c, const uint64_t* pkey, const double* area_d, const d
         e = hipMemcpyAsync(hk.data(), pkey, sizeof(uint64_t) * npair, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(I->area.data(), area_d, sizeof(double) * npair, hipMemcpyDeviceToHost, c->stream);
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(I->vol.data(), vol_d, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(I->ptr.data(), ptr_d, sizeof(int64_t) * (c->N + 1), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(I->vol.data(), vol_d, sizeof(double) * ncell, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(I->ptr.data(), ptr_d, sizeof(int64_t) * (ncell + 1), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         delete I;
@@ -893,7 +908,7 @@ int hv_integrate_polytope_impl(hv_ctx* c, const hv_mesh* mesh, hv_integral** out
     HVCK(hipEventSynchronize(c->ev[1]));
     float ms = 0.f;
     HVCK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    return finish<D>(c, pkey, parea, vol, ptr, npair, ms, out);
+    return finish<D>(c, g.nreal, pkey, parea, vol, ptr, npair, ms, out);
 }
 
 template <int D>
@@ -914,10 +929,11 @@ int hv_integrate_mc_impl(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t 
     std::vector<int64_t> hptr(c->N + 1);
     HVCK(hipMemcpyAsync(hptr.data(), ptr, sizeof(int64_t) * (c->N + 1), hipMemcpyDeviceToHost, c->stream));
     HVCK(hipStreamSynchronize(c->stream));
-    for (int64_t i = 0; i < c->N; ++i)
+    for (int64_t i = 0; i < g.nreal; ++i)
         if (hptr[i + 1] - hptr[i] > HV_MC_MAXK) return hv_fail(c, HV_ERR_CAPACITY, "a cell has more neighbours than the Monte-Carlo kernel holds");
     const double cvol = std::pow(M_PI, 0.5 * D) / std::tgamma(1.0 + 0.5 * D);
-    k_mc_cell<D><<<(unsigned)c->N, 256, 0, c->stream>>>(g, ptr, pkey, ndir, seed, cvol, D * cvol, vol, area);
+    HVCK(hipMemsetAsync(area, 0, sizeof(double) * std::max<int64_t>(npair, 1), c->stream));
+    k_mc_cell<D><<<(unsigned)g.nreal, 256, 0, c->stream>>>(g, ptr, pkey, ndir, seed, cvol, D * cvol, vol, area);
     HVCK(hipGetLastError());
     k_mc_sym<D><<<nblk(npair, 256), 256, 0, c->stream>>>(g, ptr, pkey, npair, vol, area);
     HVCK(hipGetLastError());
@@ -925,7 +941,7 @@ int hv_integrate_mc_impl(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t 
     HVCK(hipEventSynchronize(c->ev[1]));
     float ms = 0.f;
     HVCK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    return finish<D>(c, pkey, area, vol, ptr, npair, ms, out);
+    return finish<D>(c, g.nreal, pkey, area, vol, ptr, npair, ms, out);
 }
 
 // Point-sample counts -> volumes (sample fraction of the box) and interface areas.  A slab of
@@ -982,13 +998,13 @@ int hv_integrate_mc_points_impl(hv_ctx* c, const hv_mesh* mesh, int64_t nsamples
     rc = hv_mc_points_impl<D>(c, nsamples, seed, slab, ptr, pkey, ccnt, pcnt, &n, &eps, &vbox);
     if (rc) return rc;
     k_mcp_finish<<<nblk(std::max<int64_t>(c->N, npair), 256), 256, 0, c->stream>>>(
-        (int)c->N, ptr, pkey, npair, ccnt, pcnt, (double)n, eps, vbox, vol, area);
+        g.nreal, ptr, pkey, npair, ccnt, pcnt, (double)n, eps, vbox, vol, area);
     HVCK(hipGetLastError());
     HVCK(hipEventRecord(c->ev[1], c->stream));
     HVCK(hipEventSynchronize(c->ev[1]));
     float ms = 0.f;
     HVCK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    return finish<D>(c, pkey, area, vol, ptr, npair, ms, out);
+    return finish<D>(c, g.nreal, pkey, area, vol, ptr, npair, ms, out);
 }
 
 int hv_integral_counts_impl(const hv_integral* I, int64_t* ncells, int64_t* nnbr) {

@@ -37,6 +37,15 @@ struct hv_ctx {
     std::vector<int16_t> pbc;
     std::string err;
     hipEvent_t ev[4];
+    // periodic boundaries: a build on planes with periodic partners replaces the working point
+    // set ("xin", N points) by the nreal originals followed by their images ("ghosts"); the
+    // originals stay in "xorig".  gen changes whenever the working set does.
+    int64_t nreal = 0;
+    bool extended = false;
+    uint64_t gen = 0;
+    double delta = 0.0;                  // margin of the extended set beyond the box
+    std::vector<int32_t> img_orig;       // per working-set id: the original (0-based)
+    std::vector<int8_t> img_shift;       // per working-set id: d shifts in box lengths
 };
 
 struct hv_mesh {
@@ -47,7 +56,18 @@ struct hv_mesh {
     int64_t nv = 0;
     DBuf key, x, ids;   // vertex table (provisional ids) and the list of live ids
     hv_stats_t stats{};
+    // periodic meshes: generators are the nreal originals and N - nreal ghosts (image table
+    // below); every vertex has at least one original generator.  nreal == N otherwise.
+    int64_t nreal = 0;
+    uint64_t gen = 0;
+    double delta = 0.0;
+    std::vector<int32_t> img_orig;
+    std::vector<int8_t> img_shift;
 };
+
+// planes of the context as an axis-aligned box: 0 and the mask of periodic directions
+// (bit k: planes 2k and 2k+1 are each other's periodic partners), or an HV_ERR_* code
+int hv_box_periodic_mask(hv_ctx* c, int D, int* mask);
 
 int hv_fail(hv_ctx* c, int code, const std::string& msg);
 

@@ -1147,6 +1147,10 @@ __device__ __forceinline__ double hv_sample_unif(uint64_t seed, uint64_t idx) {
     return (double)(hv_mix64(seed ^ ((idx + 1ull) * 0x9E3779B97F4A7C15ull)) >> 11) * (1.0 / 9007199254740992.0);
 }
 
+struct HvBox {
+    double lo[6], hi[6];
+};
+
 __device__ __forceinline__ void hv_row_add(const int64_t* __restrict__ ptr, const uint64_t* __restrict__ pkey,
                                            unsigned long long* __restrict__ pcnt, int cell, int nb, bool inner) {
     int64_t lo = ptr[cell], hi = ptr[cell + 1];
@@ -1165,7 +1169,12 @@ template <int D>
 __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, uint64_t seed, double eps,
                                                   const int64_t* __restrict__ ptr, const uint64_t* __restrict__ pkey,
                                                   unsigned long long* __restrict__ ccnt,
-                                                  unsigned long long* __restrict__ pcnt) {
+                                                  unsigned long long* __restrict__ pcnt, HvBox box, int pmask,
+                                                  int nreal, const uint32_t* __restrict__ pshift) {
+    // samples are drawn in `box`.  Periodic directions (pmask): the working set holds images
+    // of the nreal originals (pshift: per id, 4 bits per direction, shift + 8); a sample whose
+    // nearest generator is an image is moved by the opposite shift into the original's cell
+    // and searched again, so that it is counted in the originals' frame.
     __shared__ int stk[128], stkl[128];
     __shared__ float cbx[16 * D];
     __shared__ unsigned long long hms[8];
@@ -1181,13 +1190,27 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
         for (int k = 0; k < D; ++k) {
             const int ck = (int)(b % G);
             b /= G;
-            const double wk = (g.dhi[k] - g.dlo[k]) / (double)G;
-            p[k] = g.dlo[k] + ((double)ck + hv_sample_unif(seed, (uint64_t)smp * D + k)) * wk;
+            const double wk = (box.hi[k] - box.lo[k]) / (double)G;
+            p[k] = box.lo[k] + ((double)ck + hv_sample_unif(seed, (uint64_t)smp * D + k)) * wk;
             pf[k] = (float)p[k];
         }
     }
     double bd0 = 1e300, bd1 = 1e300;
     int bi0 = -1, bi1 = -1;
+    // pass 0: every lane.  Then, with periodic images, one pass per distinct shift among the
+    // lanes whose nearest generator is an image: moved by the same shift, a group stays as
+    // compact as the stratum it came from (other lanes idle, their results kept)
+    bool act = true;
+    uint32_t mycode = 0u;
+    unsigned long long todo = 0ull;
+    for (int pass = 0;; ++pass) {
+    if (act) {
+        bd0 = 1e300;
+        bd1 = 1e300;
+        bi0 = -1;
+        bi1 = -1;
+    }
+    const double nact = (double)__popcll(__ballot(act));
     auto consider = [&](int q, double d2) {
         const bool f = d2 < bd0, s2 = !f && d2 < bd1;
         bd1 = f ? bd0 : (s2 ? d2 : bd1);
@@ -1200,7 +1223,7 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
     {
         double cc[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) cc[k] = p[k];
+        for (int k = 0; k < D; ++k) cc[k] = act ? p[k] : 0.0;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
@@ -1208,7 +1231,7 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
         }
 #pragma unroll
         for (int k = 0; k < D; ++k)
-            pcf[k] = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint((float)(cc[k] * (1.0 / 64.0)))));
+            pcf[k] = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint((float)(cc[k] / nact))));
     }
     auto child_d2f = [&](int c, const float* q) {
         float d2 = 0.f;
@@ -1248,7 +1271,7 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
         unsigned rem = 0u;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const unsigned long long m = __ballot(child_d2f(c, pf) <= R2f);
+            const unsigned long long m = __ballot(act && child_d2f(c, pf) <= R2f);
             if (lane == 0) hms[c] = m;
             rem |= (m != 0ull) ? (1u << c) : 0u;
         }
@@ -1314,9 +1337,31 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
             }
         }
     }
+    if (pmask == 0) break;
+    if (pass == 0) {
+        const int near = g.sidx[bi0];
+        const bool moved = near >= nreal;
+        mycode = moved ? pshift[near] : 0u;
+        todo = __ballot(moved);
+    }
+    if (todo == 0ull) break;
+    const int l0 = __ffsll((long long)todo) - 1;
+    const uint32_t code = __builtin_amdgcn_readlane(mycode, l0);
+    const unsigned long long grp = __ballot(mycode == code) & todo;
+    todo &= ~grp;
+    act = (grp >> lane) & 1ull;
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const int sh = (int)((code >> (4 * k)) & 15u) - 8;
+            p[k] -= (double)sh * (box.hi[k] - box.lo[k]);
+            pf[k] = (float)p[k];
+        }
+    }
+    }
     const int cell = g.sidx[bi0];
     // samples per nearest cell: one atomic per distinct cell of the wavefront
-    unsigned long long todo = __ballot(true);
+    todo = __ballot(true);
     while (todo) {
         const int l0 = __ffsll((long long)todo) - 1;
         const int cl = __builtin_amdgcn_readlane(cell, l0);
@@ -1336,6 +1381,7 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
         if (gap < 2.0 * w) hv_row_add(ptr, pkey, pcnt, cell, g.sidx[bi1], gap < w);
     }
     for (int q = 0; q < g.P; ++q) {
+        if ((pmask >> (q >> 1)) & 1) continue;   // periodic planes bound no cell
         double dist = g.ph[q];
 #pragma unroll
         for (int k = 0; k < D; ++k) dist = fma(-g.pn[q * D + k], p[k], dist);
@@ -1975,7 +2021,7 @@ __global__ void k_iota(int* __restrict__ a, int n) {
 // host driver
 
 template <int D>
-__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int stride, int* __restrict__ vkey,
+__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int stride, int nwalk, int* __restrict__ vkey,
                                                      double* __restrict__ vx, int64_t base, int* __restrict__ status);
 
 namespace {
@@ -2263,8 +2309,11 @@ __global__ void k_chunk(const int* __restrict__ ord, int nr, int c, int K, const
     keep[i] = (c == 0) ? 1 : ((rt[slot[ray]] & HV_EDGE_DONE) ? 0 : 1);
 }
 
+// The walk over the context's working set.  Only edges with a generator below nwalk are
+// followed and only those generators seed it, so the mesh holds the vertices of the cells
+// 0 .. nwalk-1 (all of them when nwalk == N).
 template <int D>
-int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
+int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk) {
     hv_opts opt{};
     if (o) opt = *o;
     if (opt.precision != HV_PREC_FP64 && opt.precision != HV_PREC_BF16 && opt.precision != HV_PREC_FP32)
@@ -2286,7 +2335,7 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     DevGeo& g = T.g;
     const int nseed = 1 << D;
     // initial sizes from the expected vertex count (or the caller's hint); all tables grow on demand
-    const double est = opt.vertex_capacity > 0 ? (double)opt.vertex_capacity : vertices_per_point(D) * N;
+    const double est = opt.vertex_capacity > 0 ? (double)opt.vertex_capacity : vertices_per_point(D) * nwalk;
     int64_t vcap = (int64_t)(2.0 * est) + 4 * nseed + 1024;
     auto hpow = [](int64_t n) { uint64_t p = 1024; while ((int64_t)p < n) p <<= 1; return p; };
     uint64_t hsize = hpow((int64_t)(2.4 * est) + 4 * nseed);
@@ -2391,14 +2440,18 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         }
         if (count == 0) return HV_OK;
         k_register<D><<<nblk((int64_t)count * (D + 1), 256), 256, 0, c->stream>>>(
-            ids + first, count, count_dev, vkey, N, R, rtab, rsize - 1, ntab, nsize - 1, reg, ctr + 4);
+            ids + first, count, count_dev, vkey, nwalk, R, rtab, rsize - 1, ntab, nsize - 1, reg, ctr + 4);
         HVCK(hipGetLastError());
         return HV_OK;
     };
 
     // seeds: the 2^d box corners and interior vertices found by descent, one per
     // HV_SEED_STRIDE generators in Morton order (more fronts, fewer levels)
-    const int ndesc = N >= 2 * HV_SEED_STRIDE ? N / HV_SEED_STRIDE : 0;
+    // (every generator's cell when only some cells are walked: their union need not be
+    // connected through walked edges)
+    const bool part = nwalk < N;
+    const int stride = part ? 1 : HV_SEED_STRIDE;
+    const int ndesc = part ? N : (N >= 2 * HV_SEED_STRIDE ? N / HV_SEED_STRIDE : 0);
     const int nstart = nseed + ndesc;
     int64_t* reg_cur = pget<int64_t>(c, "regA", (size_t)nstart * (D + 1));
     if (!reg_cur) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
@@ -2407,10 +2460,14 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
         if (rc) return rc;
         int* status = pget<int>(c, "status", nstart);
         if (!status) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
-        k_seed_corners<D><<<nseed, 256, 0, c->stream>>>(g, vkey, vx, vnext, status);
-        HVCK(hipGetLastError());
+        if (part) {   // a corner belongs to a cell that is not walked: status HV_ST_DROPPED (-1)
+            HVCK(hipMemsetAsync(status, 0xff, sizeof(int) * nseed, c->stream));
+        } else {
+            k_seed_corners<D><<<nseed, 256, 0, c->stream>>>(g, vkey, vx, vnext, status);
+            HVCK(hipGetLastError());
+        }
         if (ndesc > 0) {
-            k_seed_descent<D><<<nblk(ndesc, 64), 64, 0, c->stream>>>(g, ndesc, HV_SEED_STRIDE, vkey, vx,
+            k_seed_descent<D><<<nblk(ndesc, 64), 64, 0, c->stream>>>(g, ndesc, stride, nwalk, vkey, vx,
                                                                        vnext + nseed, status + nseed);
             HVCK(hipGetLastError());
         }
@@ -2661,8 +2718,210 @@ int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     m->x = c->pool->take("vx");
     m->ids = c->pool->take("ids");
     m->stats = st;
+    m->nreal = N;
+    m->gen = c->gen;
     *out = m;
     return HV_OK;
+}
+
+// ---- periodic boundaries ------------------------------------------------------------------
+// A periodic box is meshed as the originals plus their images within a margin delta of the
+// box ("ghosts"), in the box widened by delta in the periodic directions (Dirichlet), walking
+// only the originals' cells.  The result is the periodic tessellation's cells of the
+// originals when no vertex of them has a circumsphere reaching beyond the widened box (then
+// every image that could lie inside it is present) and no circumradius exceeds delta (then the
+// nearest generator of every point of the box is present, which the point-sample integrator
+// relies on).  k_periodic_reach returns the largest of those two amounts over the mesh; the
+// build is repeated with a wider margin while it exceeds delta.
+
+template <int D>
+__global__ __launch_bounds__(256) void k_periodic_reach(const int* __restrict__ ids, int nv, const int* __restrict__ vkey,
+                                                        const double* __restrict__ vx, const double* __restrict__ xin,
+                                                        HvBox box, int mask, unsigned long long* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double need = 0.0;
+    if (t < nv) {
+        const int v = ids[t];
+        const int g0 = vkey[(int64_t)v * (D + 1)];   // the smallest id: a generator, never a plane
+        double r2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double a = vx[(int64_t)v * D + k] - xin[(int64_t)g0 * D + k];
+            r2 = fma(a, a, r2);
+        }
+        const double r = sqrt(r2);
+        need = r;
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            if ((mask >> k) & 1) {
+                const double x = vx[(int64_t)v * D + k];
+                need = fmax(need, fmax(x + r - box.hi[k], box.lo[k] - (x - r)));
+            }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) need = fmax(need, __shfl_xor(need, off));
+    if ((threadIdx.x & 63) == 0 && need > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(need));
+}
+
+template <int D>
+static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
+    const int64_t n = c->nreal;
+    const size_t obytes = sizeof(double) * D * n;
+    if (!c->extended) {   // keep the originals: the working set is about to be replaced
+        void* dst = c->pool->get("xorig", obytes, false, c->stream);
+        const void* src = c->pool->get("xin", 0);
+        if (!dst || !src) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+        HVCK(hipMemcpyAsync(dst, src, obytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    std::vector<double> hx((size_t)n * D);
+    HVCK(hipMemcpyAsync(hx.data(), c->pool->get("xorig", 0), obytes, hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    HvBox box{};
+    double vol = 1.0;
+    for (int k = 0; k < D; ++k) {   // planes 2k: x_k = hi, 2k+1: x_k = lo (checked by hv_box_periodic_mask)
+        box.hi[k] = c->pbase[(2 * k) * D + k];
+        box.lo[k] = c->pbase[(2 * k + 1) * D + k];
+        vol *= box.hi[k] - box.lo[k];
+    }
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < D; ++k) {
+            const double x = hx[(size_t)i * D + k];
+            if (!(x >= box.lo[k] && x <= box.hi[k]))
+                return hv_fail(c, HV_ERR_DOMAIN, "a node does not lie in the domain given by the boundary");
+        }
+    // first margin: about three times the mean circumradius of a Poisson-Delaunay simplex at this
+    // density (n omega_d r^d / vol is Gamma(d, 1) distributed, mean d)
+    const double omega = std::pow(M_PI, 0.5 * D) / std::tgamma(1.0 + 0.5 * D);
+    double delta = 2.9 * std::pow((double)D * vol / (omega * (double)n), 1.0 / D);
+    const std::vector<double> pbase = c->pbase;
+    const std::vector<int16_t> pbc = c->pbc;
+    for (int attempt = 0;; ++attempt) {
+        // the extended set: originals (shift 0), then per original its images in [lo - delta, hi + delta]
+        std::vector<double> ext(hx);
+        std::vector<int32_t> iorig((size_t)n);
+        std::vector<int8_t> ishift((size_t)n * D, 0);
+        for (int64_t i = 0; i < n; ++i) iorig[(size_t)i] = (int32_t)i;
+        int lo_s[6], cnt_s[6];
+        for (int64_t i = 0; i < n; ++i) {
+            const double* x = &hx[(size_t)i * D];
+            int64_t combos = 1;
+            for (int k = 0; k < D; ++k) {
+                lo_s[k] = 0;
+                cnt_s[k] = 1;
+                if (!((mask >> k) & 1)) continue;
+                const double L = box.hi[k] - box.lo[k];
+                const int m = (int)std::ceil(delta / L) + 1;
+                int a = 1, b = -1;   // the admissible shifts form one interval
+                for (int s = -m; s <= m; ++s) {
+                    const double y = x[k] + s * L;
+                    if (y >= box.lo[k] - delta && y <= box.hi[k] + delta) { a = std::min(a, s); b = std::max(b, s); }
+                }
+                lo_s[k] = a;
+                cnt_s[k] = b - a + 1;
+                combos *= cnt_s[k];
+            }
+            for (int64_t q = 0; q < combos; ++q) {
+                int sh[6];
+                int64_t r = q;
+                bool zero = true;
+                for (int k = 0; k < D; ++k) {
+                    sh[k] = lo_s[k] + (int)(r % cnt_s[k]);
+                    r /= cnt_s[k];
+                    zero = zero && sh[k] == 0;
+                }
+                if (zero) continue;
+                if ((int64_t)iorig.size() >= (1ll << 30) - 64)
+                    return hv_fail(c, HV_ERR_CAPACITY, "too many periodic images for 32-bit keys");
+                for (int k = 0; k < D; ++k) {
+                    ext.push_back(x[k] + sh[k] * (box.hi[k] - box.lo[k]));
+                    ishift.push_back((int8_t)sh[k]);
+                }
+                iorig.push_back((int32_t)i);
+            }
+        }
+        const int64_t next = (int64_t)iorig.size();
+        double* xin = (double*)c->pool->get("xin", sizeof(double) * D * next, false, c->stream);
+        if (!xin) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (periodic images)");
+        HVCK(hipMemcpyAsync(xin, ext.data(), sizeof(double) * D * next, hipMemcpyHostToDevice, c->stream));
+        c->N = next;
+        c->extended = true;
+        ++c->gen;
+        c->delta = delta;
+        c->img_orig = iorig;
+        c->img_shift = ishift;
+        // the widened box, Dirichlet, for the walk; the caller's planes come back afterwards
+        for (int k = 0; k < D; ++k)
+            if ((mask >> k) & 1) {
+                c->pbase[(2 * k) * D + k] = box.hi[k] + delta;
+                c->pbase[(2 * k + 1) * D + k] = box.lo[k] - delta;
+                c->pbc[2 * k] = c->pbc[2 * k + 1] = 0;
+            }
+        hv_mesh* m = nullptr;
+        int rc = hv_walk_mesh<D>(c, o, &m, (int)n);
+        c->pbase = pbase;
+        c->pbc = pbc;
+        if (rc) return rc;
+        if (m->nv == 0) {
+            hv_mesh_free(m);
+            return hv_fail(c, HV_ERR_DEGENERATE, "no seed vertex of the periodic cells was found");
+        }
+        unsigned long long* rb = pget<unsigned long long>(c, "preach", 1);
+        if (!rb) {
+            hv_mesh_free(m);
+            return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+        }
+        HVCK(hipMemsetAsync(rb, 0, sizeof(unsigned long long), c->stream));
+        if (m->nv > 0) {
+            k_periodic_reach<D><<<nblk(m->nv, 256), 256, 0, c->stream>>>((const int*)m->ids.p, (int)m->nv,
+                                                                         (const int*)m->key.p, (const double*)m->x.p,
+                                                                         xin, box, mask, rb);
+            HVCK(hipGetLastError());
+        }
+        unsigned long long hb = 0;
+        HVCK(hipMemcpyAsync(&hb, rb, sizeof hb, hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        double need = 0.0;
+        std::memcpy(&need, &hb, sizeof need);
+        if (need <= delta * (1.0 - 1e-9)) {
+            m->nreal = n;
+            m->gen = c->gen;
+            m->delta = delta;
+            m->img_orig = std::move(iorig);
+            m->img_shift = std::move(ishift);
+            *out = m;
+            return HV_OK;
+        }
+        hv_mesh_free(m);
+        if (attempt >= 8) return hv_fail(c, HV_ERR_CAPACITY, "the periodic margin did not settle");
+        delta = std::max(1.3 * delta, 1.05 * need);
+    }
+}
+
+int hv_box_periodic_mask(hv_ctx* c, int D, int* mask) {
+    *mask = 0;
+    for (int k = 0; k < D && k < c->P / 2; ++k) {
+        const int a = c->pbc[2 * k], b = c->pbc[2 * k + 1];
+        if (a <= 0 && b <= 0) continue;
+        // cuboid's pairing (1-based): plane 2k-1 <-> 2k
+        if (a != 2 * k + 2 || b != 2 * k + 1)
+            return hv_fail(c, HV_ERR_UNSUPPORTED, "periodic planes must be the pairs of cuboid(d; periodic=...)");
+        *mask |= 1 << k;
+    }
+    if (!*mask) return check_box(c, D);
+    std::vector<int16_t> keep = c->pbc;
+    std::fill(c->pbc.begin(), c->pbc.end(), (int16_t)0);
+    const int rc = check_box(c, D);
+    c->pbc = keep;
+    return rc;
+}
+
+template <int D>
+int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
+    int mask = 0;
+    const int rc = hv_box_periodic_mask(c, D, &mask);
+    if (rc) return rc;
+    if (!mask) return hv_walk_mesh<D>(c, o, out, (int)c->N);
+    return periodic_mesh<D>(c, o, out, mask);
 }
 
 template <int D>
@@ -2817,7 +3076,7 @@ __global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const 
 // the walk, reaching the same vertex, gives it the same key; its coordinates are computed
 // from the key as every other vertex's.  status[t] = HV_ST_FOUND kept, HV_ST_DROPPED not.
 template <int D>
-__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int stride, int* __restrict__ vkey,
+__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int stride, int nwalk, int* __restrict__ vkey,
                                                      double* __restrict__ vx, int64_t base, int* __restrict__ status) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nseed) return;
@@ -2825,6 +3084,10 @@ __global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int st
     int S[D + 1];
     const int js = min(N - 1, t * stride + stride / 2);
     S[0] = g.sidx[js];
+    if (S[0] >= nwalk) {   // only cells of generators below nwalk are walked
+        status[t] = HV_ST_DROPPED;
+        return;
+    }
     double xi[D], p[D], q[D][D];
 #pragma unroll
     for (int k = 0; k < D; ++k) { xi[k] = g.xs[(int64_t)js * (D + 1) + k]; p[k] = xi[k]; }
@@ -3135,14 +3398,50 @@ template <int D>
 int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, const int64_t* ptr,
                       const uint64_t* pkey, unsigned long long* ccnt, unsigned long long* pcnt, int64_t* nused,
                       double* eps_out, double* vbox_out) {
-    int rc = check_box(c, D);
+    int pmask = 0;
+    int rc = hv_box_periodic_mask(c, D, &pmask);
     if (rc) return rc;
+    if (pmask && !c->extended) return hv_fail(c, HV_ERR_STATE, "a periodic box needs its mesh built first");
+    HvBox box{};
+    double vbox = 1.0;
+    for (int k = 0; k < D; ++k) {
+        box.hi[k] = c->pbase[(2 * k) * D + k];
+        box.lo[k] = c->pbase[(2 * k + 1) * D + k];
+        vbox *= box.hi[k] - box.lo[k];
+    }
+    // the tree over the working set; with periodic images, in the box widened as for the walk
+    const std::vector<double> pbase = c->pbase;
+    const std::vector<int16_t> pbc = c->pbc;
+    for (int k = 0; k < D; ++k)
+        if ((pmask >> k) & 1) {
+            c->pbase[(2 * k) * D + k] = box.hi[k] + c->delta;
+            c->pbase[(2 * k + 1) * D + k] = box.lo[k] - c->delta;
+            c->pbc[2 * k] = c->pbc[2 * k + 1] = 0;
+        }
     Tree T;
     rc = build_tree<D>(c, 32, T);
+    c->pbase = pbase;
+    c->pbc = pbc;
     if (rc) return rc;
     const DevGeo& g = T.g;
-    double vbox = 1.0;
-    for (int k = 0; k < D; ++k) vbox *= g.dhi[k] - g.dlo[k];
+    const uint32_t* pshift = nullptr;
+    if (pmask) {
+        std::vector<uint32_t> code((size_t)c->N);
+        for (int64_t i = 0; i < c->N; ++i) {
+            uint32_t v = 0u;
+            for (int k = 0; k < D; ++k) {
+                const int sh = c->img_shift[(size_t)i * D + k];
+                if (sh < -8 || sh > 7) return hv_fail(c, HV_ERR_CAPACITY, "periodic image shift beyond 7 box lengths");
+                v |= (uint32_t)(sh + 8) << (4 * k);
+            }
+            code[(size_t)i] = v;
+        }
+        uint32_t* d = pget<uint32_t>(c, "pshift", (size_t)c->N);
+        if (!d) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+        HVCK(hipMemcpyAsync(d, code.data(), sizeof(uint32_t) * c->N, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        pshift = d;
+    }
     int G = std::max(1, (int)std::floor(std::pow((double)nsamples / 256.0, 1.0 / D)));
     G = std::min(G, (int)std::floor(std::pow(2.0, 40.0 / D)));   // G^d strata stay below 2^40
     int64_t B = 1;
@@ -3151,8 +3450,9 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     S = (S + 63) / 64 * 64;
     const int64_t n = B * S;
     if (n / 64 > (int64_t)INT32_MAX) return hv_fail(c, HV_ERR_CAPACITY, "too many samples");
-    const double eps = slab * std::pow(vbox / (double)c->N, 1.0 / D);
-    k_mc_points<D><<<(unsigned)(n / 64), 64, 0, c->stream>>>(g, G, S, seed, eps, ptr, pkey, ccnt, pcnt);
+    const double eps = slab * std::pow(vbox / (double)c->nreal, 1.0 / D);
+    k_mc_points<D><<<(unsigned)(n / 64), 64, 0, c->stream>>>(g, G, S, seed, eps, ptr, pkey, ccnt, pcnt, box, pmask,
+                                                             (int)c->nreal, pshift);
     HVCK(hipGetLastError());
     *nused = n;
     *eps_out = eps;

@@ -71,6 +71,7 @@ def lib():
     L.hv_mesh_export.argtypes = [vp, P(C.c_int64), P(C.c_double)]
     L.hv_mesh_stats.argtypes = [vp, P(hv_stats_t)]
     L.hv_mesh_free.argtypes = [vp]
+    L.hv_mesh_images.argtypes = [vp, P(C.c_int64), P(C.c_int64), P(C.c_int8)]
     L.hv_raycast_batch.argtypes = [vp, C.c_int, C.c_int64, P(C.c_int64), P(C.c_int32), P(C.c_double),
                                    P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_int32)]
     L.hv_nn_halfspace_batch.argtypes = [vp, C.c_int64, P(C.c_double), P(C.c_double), P(C.c_double),
@@ -129,6 +130,18 @@ class Mesh:
         s = hv_stats_t()
         lib().hv_mesh_stats(self._h, C.byref(s))
         return s.as_dict()
+
+    def images(self):
+        """(orig[n_ext] 1-based, shift[n_ext, d] int8): generator id i+1 of this mesh is the
+        original orig[i] moved by shift[i] box lengths (the identity for a non-periodic mesh)."""
+        n = C.c_int64(0)
+        rc = lib().hv_mesh_images(self._h, C.byref(n), None, None)
+        if rc != HV_OK:
+            raise HVError(rc, "images failed")
+        orig = np.zeros(n.value, dtype=np.int64)
+        shift = np.zeros((n.value, self.d), dtype=np.int8)
+        lib().hv_mesh_images(self._h, C.byref(n), _i64(orig), shift.ctypes.data_as(C.POINTER(C.c_int8)))
+        return orig, shift
 
 
 class Context:

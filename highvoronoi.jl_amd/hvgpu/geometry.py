@@ -145,7 +145,12 @@ class _CellVertices:
 
 class VoronoiData:
     """Read-only views of a VoronoiGeometry (src/voronoidata.jl:545-621): nodes, vertices,
-    neighbors, volume, area.  volume / neighbors / area need an integrator."""
+    neighbors, volume, area.  volume / neighbors / area need an integrator.
+
+    Periodic boxes: ids are public as in the reference (VoronoiDataShift,
+    src/voronoidata.jl:540-542): a periodic image is named by its original's index and
+    plane p is n + p; vertices[i] holds cell i's vertices in its own frame.  `images` keeps the
+    device labels: (orig, shift) per device generator id (identity without periodic planes)."""
 
     def __init__(self, VG: VoronoiGeometry):
         self.geometry = VG
@@ -153,13 +158,26 @@ class VoronoiData:
         self.boundary = VG.boundary
         n = len(VG.nodes)
         keys, coords = VG.vertex_table()
+        orig, shift = VG._mesh.images()
+        self.images = (orig, shift)
+        n_ext = len(orig)
+
+        def public(ids):
+            ids = np.asarray(ids, dtype=np.int64)
+            out = ids - n_ext + n
+            inner = ids <= n_ext
+            out[inner] = orig[ids[inner] - 1]
+            return out
+
         ptr, rows = _cell_vertex_csr(keys, n)
-        self.vertices = _CellVertices(keys, coords, ptr, rows)
+        pkeys = np.sort(public(keys), axis=1) if n_ext > n else keys
+        self.vertices = _CellVertices(pkeys, coords, ptr, rows)
         I = VG.integral
         if I is None:
             self.volume = self.neighbors = self.area = None
             return
         self.volume = I["volumes"]
         p = I["nbr_ptr"]
-        self.neighbors = [I["neighbors"][p[i]:p[i + 1]] for i in range(n)]
+        nb = public(I["neighbors"]) if n_ext > n else I["neighbors"]
+        self.neighbors = [nb[p[i]:p[i + 1]] for i in range(n)]
         self.area = [I["areas"][p[i]:p[i + 1]] for i in range(n)]

@@ -114,6 +114,23 @@ int hv_mesh_export(const hv_mesh* mesh, int64_t* sig, double* coords);
 int hv_mesh_stats(const hv_mesh* mesh, hv_stats_t* out);
 int hv_mesh_free(hv_mesh* mesh);
 
+/* Periodic boundaries (cuboid(d; periodic=...), src/boundary.jl:510-534, where the reference
+ * adds periodic copies of the nodes near the boundary, src/domain.jl:139-390).  A box whose
+ * planes 2k-1 and 2k (1-based) name each other as periodic partners is periodic in direction
+ * k.  hv_build_mesh then meshes the n originals together with their images ("ghosts") within
+ * a margin of the box, and the mesh holds exactly the vertices of the originals' cells: every
+ * vertex has at least one original generator, each in that cell's own frame, so a vertex of
+ * the periodic tessellation appears once per original cell it belongs to, translated.
+ * Generator ids in such a mesh: 1..n the originals, n+1..n_ext the ghosts, n_ext + p boundary
+ * plane p (only non-periodic planes occur).  This call returns n_ext (n for a non-periodic
+ * mesh) and, when orig / shift are not NULL, for every id 1..n_ext the original's id (n_ext
+ * int64) and its shift in box lengths per direction (d x n_ext int8): generator id sits at
+ * x[orig] + shift .* box size.  The integrators accept such a mesh (cells 1..n; a neighbour
+ * id above n is an image); hv_raycast_batch and the queries reject periodic planes.
+ * Replaces the reference's periodic node numbering (src/domain.jl:338-390) with an explicit
+ * (index, shift) label per generator. */
+int hv_mesh_images(const hv_mesh* mesh, int64_t* n_ext, int64_t* orig, int8_t* shift);
+
 /* Many independent meshes in one call: mesh k has d[k] in 2..6, n[k] generators xs[k] (d[k] x n[k],
  * host memory) and the nplanes[k] boundary planes base[k], normal[k] (d[k] x nplanes[k] each,
  * Dirichlet), all built with *opts.  Replaces a host loop of VoronoiGeometry(xs_k, boundary_k)

@@ -68,3 +68,70 @@ def edge_closure_defects(keys, n):
 
 def coords_by_key(keys, coords):
     return {tuple(k): c for k, c in zip(np.asarray(keys).tolist(), np.asarray(coords))}
+
+
+# ---- periodic boxes ---------------------------------------------------------------------------
+# Labels make a periodic mesh comparable across constructions: generator (original o, shift s)
+# is (0, o, *s), boundary plane p (1-based) is (1, p, 0, ..., 0); a vertex is the sorted tuple
+# of its generators' labels.
+
+def periodic_tiling(xs, per, copies=1, lo=0.0, hi=1.0):
+    """The points and their images shifted by -copies..copies box lengths in the directions
+    `per` (0-based), originals first, in the box widened by `copies` lengths there (Dirichlet
+    planes in the cuboid order).  Returns (tiled, orig, shift, base, normal)."""
+    from itertools import product
+    n, d = xs.shape
+    L = hi - lo
+    shifts = [np.zeros(d, dtype=np.int64)]
+    for s in product(range(-copies, copies + 1), repeat=len(per)):
+        if any(s):
+            v = np.zeros(d, dtype=np.int64)
+            v[list(per)] = s
+            shifts.append(v)
+    tiled = np.vstack([xs + L * s for s in shifts])
+    orig = np.tile(np.arange(n), len(shifts))
+    shift = np.repeat(np.array(shifts), n, axis=0)
+    base, normal = [], []
+    for k in range(d):
+        w = copies * L if k in per else 0.0
+        e = np.zeros(d)
+        e[k] = 1.0
+        base.append(np.where(e > 0, hi + w, 0.0)); normal.append(e.copy())
+        b = np.zeros(d); b[k] = lo - w
+        base.append(b); normal.append(-e)
+    return tiled, orig, shift, np.array(base), np.array(normal)
+
+
+def vertex_labels(keys, n_ext, orig0, shift):
+    """Label tuples of the rows of a 1-based key table whose ids 1..n_ext are generators
+    (orig0: 0-based originals, shift: int rows) and whose larger ids are planes."""
+    d = shift.shape[1]
+    out = []
+    for row in np.asarray(keys).tolist():
+        lab = []
+        for i in row:
+            if i <= n_ext:
+                lab.append((0, int(orig0[i - 1])) + tuple(int(v) for v in shift[i - 1]))
+            else:
+                lab.append((1, int(i - n_ext)) + (0,) * d)
+        out.append(tuple(sorted(lab)))
+    return out
+
+
+def periodic_oracle(xs, per, copies=1):
+    """Vertices of the originals' cells in the periodic unit box (directions `per`), from the
+    oracle's walk over the cells 1..n of the tiled set: {label tuple: coordinates}."""
+    from oracle_py import Oracle
+    n = xs.shape[0]
+    tiled, orig, shift, base, normal = periodic_tiling(xs, per, copies)
+    keys, coords, _ = Oracle(tiled, base, normal).voronoi(cells=(0, n))
+    labs = vertex_labels(keys, len(tiled), orig, shift)
+    return dict(zip(labs, coords))
+
+
+def periodic_mesh_labels(mesh):
+    """{label tuple: coordinates} of a device mesh built on a periodic box."""
+    keys, coords = mesh.export()
+    orig, shift = mesh.images()
+    labs = vertex_labels(keys, len(orig), orig - 1, shift)
+    return dict(zip(labs, coords))

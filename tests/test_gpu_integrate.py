@@ -147,7 +147,14 @@ def test_voronoi_geometry_and_data_mirror(ctx):
         assert rel_area(data.area[i], area[ptr[i]:ptr[i + 1]]) < TOL
         for key, r in data.vertices[i].items():
             assert i + 1 in key and len(key) == 4
-    # the reference's default cuboid is periodic: refused, not silently changed
-    from hvgpu._lib import HVError
-    with pytest.raises(HVError):
-        VoronoiGeometry(xs, cuboid(3))
+    # the reference's default cuboid is periodic in every direction: public ids name
+    # originals only (images reduced to their original, no boundary planes), the cells tile
+    # the torus, and cell i's vertices all hold i + 1
+    VG = VoronoiGeometry(xs, cuboid(3), integrator=VI_POLYGON, search_settings={"precision": "bf16"})
+    data = VoronoiData(VG)
+    assert abs(data.volume.sum() - 1.0) < 1e-12
+    for i in (0, 17, 299):
+        assert np.all((data.neighbors[i] >= 1) & (data.neighbors[i] <= 300)) and i + 1 not in data.neighbors[i]
+        assert len(data.vertices[i]) >= 4
+        for key in data.vertices[i]:
+            assert i + 1 in key and max(key) <= 300
