@@ -180,7 +180,7 @@ __device__ __forceinline__ void hv_wave_count(unsigned long long* ctr, bool cond
     if (lane == __ffsll((long long)m) - 1) atomicAdd(ctr, (unsigned long long)__popcll(m));
 }
 
-__device__ __forceinline__ uint64_t hv_mix64(uint64_t z) {
+__host__ __device__ __forceinline__ uint64_t hv_mix64(uint64_t z) {
     z ^= z >> 33; z *= 0xff51afd7ed558ccdULL;
     z ^= z >> 33; z *= 0xc4ceb9fe1a85ec53ULL;
     z ^= z >> 33;

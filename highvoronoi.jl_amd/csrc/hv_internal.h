@@ -44,6 +44,7 @@ struct hv_ctx {
     bool extended = false;
     uint64_t gen = 0;
     double delta = 0.0;                  // margin of the extended set beyond the box
+    double rmax = 0.0;                   // largest circumradius of the periodic mesh
     std::vector<int32_t> img_orig;       // per working-set id: the original (0-based)
     std::vector<int8_t> img_shift;       // per working-set id: d shifts in box lengths
 };

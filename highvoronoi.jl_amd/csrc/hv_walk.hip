@@ -1170,11 +1170,16 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
                                                   const int64_t* __restrict__ ptr, const uint64_t* __restrict__ pkey,
                                                   unsigned long long* __restrict__ ccnt,
                                                   unsigned long long* __restrict__ pcnt, HvBox box, int pmask,
-                                                  int nreal, const uint32_t* __restrict__ pshift) {
+                                                  int nreal, const int* __restrict__ porig,
+                                                  const uint32_t* __restrict__ pshift,
+                                                  const unsigned long long* __restrict__ pkeys,
+                                                  const int* __restrict__ pvals, uint64_t pmaskh) {
     // samples are drawn in `box`.  Periodic directions (pmask): the working set holds images
-    // of the nreal originals (pshift: per id, 4 bits per direction, shift + 8); a sample whose
-    // nearest generator is an image is moved by the opposite shift into the original's cell
-    // and searched again, so that it is counted in the originals' frame.
+    // of the nreal originals (porig / pshift per id: the original and the shift, 4 bits per
+    // direction, shift + 8).  A sample whose nearest generator a is an image lies, moved by
+    // -shift(a), in the cell of a's original; it is counted there, and its second generator b
+    // is renamed to the image of b's original at shift(b) - shift(a), looked up in the table
+    // (pkeys, pvals): key (original << 32) | shift code.
     __shared__ int stk[128], stkl[128];
     __shared__ float cbx[16 * D];
     __shared__ unsigned long long hms[8];
@@ -1197,20 +1202,6 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
     }
     double bd0 = 1e300, bd1 = 1e300;
     int bi0 = -1, bi1 = -1;
-    // pass 0: every lane.  Then, with periodic images, one pass per distinct shift among the
-    // lanes whose nearest generator is an image: moved by the same shift, a group stays as
-    // compact as the stratum it came from (other lanes idle, their results kept)
-    bool act = true;
-    uint32_t mycode = 0u;
-    unsigned long long todo = 0ull;
-    for (int pass = 0;; ++pass) {
-    if (act) {
-        bd0 = 1e300;
-        bd1 = 1e300;
-        bi0 = -1;
-        bi1 = -1;
-    }
-    const double nact = (double)__popcll(__ballot(act));
     auto consider = [&](int q, double d2) {
         const bool f = d2 < bd0, s2 = !f && d2 < bd1;
         bd1 = f ? bd0 : (s2 ? d2 : bd1);
@@ -1223,7 +1214,7 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
     {
         double cc[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) cc[k] = act ? p[k] : 0.0;
+        for (int k = 0; k < D; ++k) cc[k] = p[k];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
@@ -1231,7 +1222,7 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
         }
 #pragma unroll
         for (int k = 0; k < D; ++k)
-            pcf[k] = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint((float)(cc[k] / nact))));
+            pcf[k] = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint((float)(cc[k] * (1.0 / 64.0)))));
     }
     auto child_d2f = [&](int c, const float* q) {
         float d2 = 0.f;
@@ -1271,7 +1262,7 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
         unsigned rem = 0u;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const unsigned long long m = __ballot(act && child_d2f(c, pf) <= R2f);
+            const unsigned long long m = __ballot(child_d2f(c, pf) <= R2f);
             if (lane == 0) hms[c] = m;
             rem |= (m != 0ull) ? (1u << c) : 0u;
         }
@@ -1337,31 +1328,38 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
             }
         }
     }
-    if (pmask == 0) break;
-    if (pass == 0) {
-        const int near = g.sidx[bi0];
-        const bool moved = near >= nreal;
-        mycode = moved ? pshift[near] : 0u;
-        todo = __ballot(moved);
-    }
-    if (todo == 0ull) break;
-    const int l0 = __ffsll((long long)todo) - 1;
-    const uint32_t code = __builtin_amdgcn_readlane(mycode, l0);
-    const unsigned long long grp = __ballot(mycode == code) & todo;
-    todo &= ~grp;
-    act = (grp >> lane) & 1ull;
-    if (act) {
+    int cell = g.sidx[bi0];
+    int nb1 = bi1 >= 0 ? g.sidx[bi1] : -1;
+    if (pmask && cell >= nreal) {
+        const uint32_t ca = pshift[cell];
+        cell = porig[cell];
+        if (nb1 >= 0) {
+            const uint32_t cb = pshift[nb1];
+            uint32_t code = 0u;
+            bool ok = true;
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const int sh = (int)((code >> (4 * k)) & 15u) - 8;
-            p[k] -= (double)sh * (box.hi[k] - box.lo[k]);
-            pf[k] = (float)p[k];
+            for (int k = 0; k < D; ++k) {
+                const int sh = (int)((cb >> (4 * k)) & 15u) - (int)((ca >> (4 * k)) & 15u);
+                ok = ok && sh >= -8 && sh <= 7;
+                code |= (uint32_t)((sh + 8) & 15) << (4 * k);
+            }
+            const int ob = porig[nb1];
+            int found = -1;
+            if (ok) {
+                const unsigned long long key = ((unsigned long long)(unsigned)ob << 32) | code;
+                uint64_t h = hv_mix64(key) & pmaskh;
+                for (uint64_t probe = 0; probe <= pmaskh; ++probe) {
+                    const unsigned long long cur = pkeys[h];
+                    if (cur == key) { found = pvals[h]; break; }
+                    if (cur == ~0ull) break;
+                    h = (h + 1) & pmaskh;
+                }
+            }
+            nb1 = found;   // -1: that image is not in the working set (then no interface count)
         }
     }
-    }
-    const int cell = g.sidx[bi0];
     // samples per nearest cell: one atomic per distinct cell of the wavefront
-    todo = __ballot(true);
+    unsigned long long todo = __ballot(true);
     while (todo) {
         const int l0 = __ffsll((long long)todo) - 1;
         const int cl = __builtin_amdgcn_readlane(cell, l0);
@@ -1378,7 +1376,7 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
             dd = fma(a, a, dd);
         }
         const double gap = bd1 - bd0, w = 2.0 * eps * sqrt(dd);   // distance to the bisector = gap / (2|xi - xj|)
-        if (gap < 2.0 * w) hv_row_add(ptr, pkey, pcnt, cell, g.sidx[bi1], gap < w);
+        if (gap < 2.0 * w && nb1 >= 0) hv_row_add(ptr, pkey, pcnt, cell, nb1, gap < w);
     }
     for (int q = 0; q < g.P; ++q) {
         if ((pmask >> (q >> 1)) & 1) continue;   // periodic planes bound no cell
@@ -2758,9 +2756,25 @@ __global__ __launch_bounds__(256) void k_periodic_reach(const int* __restrict__ 
                 need = fmax(need, fmax(x + r - box.hi[k], box.lo[k] - (x - r)));
             }
     }
+    double rr = need;   // the circumradius alone (need starts from it)
+    if (t < nv) {
+        const int v = ids[t];
+        const int g0 = vkey[(int64_t)v * (D + 1)];
+        double r2 = 0.0;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) need = fmax(need, __shfl_xor(need, off));
+        for (int k = 0; k < D; ++k) {
+            const double a = vx[(int64_t)v * D + k] - xin[(int64_t)g0 * D + k];
+            r2 = fma(a, a, r2);
+        }
+        rr = sqrt(r2);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        need = fmax(need, __shfl_xor(need, off));
+        rr = fmax(rr, __shfl_xor(rr, off));
+    }
     if ((threadIdx.x & 63) == 0 && need > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(need));
+    if ((threadIdx.x & 63) == 0 && rr > 0.0) atomicMax(out + 1, (unsigned long long)__double_as_longlong(rr));
 }
 
 template <int D>
@@ -2865,23 +2879,24 @@ static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
             hv_mesh_free(m);
             return hv_fail(c, HV_ERR_DEGENERATE, "no seed vertex of the periodic cells was found");
         }
-        unsigned long long* rb = pget<unsigned long long>(c, "preach", 1);
+        unsigned long long* rb = pget<unsigned long long>(c, "preach", 2);
         if (!rb) {
             hv_mesh_free(m);
             return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
         }
-        HVCK(hipMemsetAsync(rb, 0, sizeof(unsigned long long), c->stream));
+        HVCK(hipMemsetAsync(rb, 0, 2 * sizeof(unsigned long long), c->stream));
         if (m->nv > 0) {
             k_periodic_reach<D><<<nblk(m->nv, 256), 256, 0, c->stream>>>((const int*)m->ids.p, (int)m->nv,
                                                                          (const int*)m->key.p, (const double*)m->x.p,
                                                                          xin, box, mask, rb);
             HVCK(hipGetLastError());
         }
-        unsigned long long hb = 0;
-        HVCK(hipMemcpyAsync(&hb, rb, sizeof hb, hipMemcpyDeviceToHost, c->stream));
+        unsigned long long hb[2] = {0, 0};
+        HVCK(hipMemcpyAsync(hb, rb, sizeof hb, hipMemcpyDeviceToHost, c->stream));
         HVCK(hipStreamSynchronize(c->stream));
         double need = 0.0;
-        std::memcpy(&need, &hb, sizeof need);
+        std::memcpy(&need, &hb[0], sizeof need);
+        std::memcpy(&c->rmax, &hb[1], sizeof need);
         if (need <= delta * (1.0 - 1e-9)) {
             m->nreal = n;
             m->gen = c->gen;
@@ -3425,9 +3440,19 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     if (rc) return rc;
     const DevGeo& g = T.g;
     const uint32_t* pshift = nullptr;
+    const int* porig = nullptr;
+    const unsigned long long* pkeys = nullptr;
+    const int* pvals = nullptr;
+    uint64_t hmask = 0;
+    const double eps = slab * std::pow(vbox / (double)c->nreal, 1.0 / D);
     if (pmask) {
-        std::vector<uint32_t> code((size_t)c->N);
-        for (int64_t i = 0; i < c->N; ++i) {
+        // a sample's second generator lies within rmax + 4 eps of it when it counts for an
+        // interface; the working set must hold it
+        if (c->rmax + 4.0 * eps > c->delta)
+            return hv_fail(c, HV_ERR_UNSUPPORTED, "slab too wide for the margin of the periodic images");
+        const int64_t ne = c->N;
+        std::vector<uint32_t> code((size_t)ne);
+        for (int64_t i = 0; i < ne; ++i) {
             uint32_t v = 0u;
             for (int k = 0; k < D; ++k) {
                 const int sh = c->img_shift[(size_t)i * D + k];
@@ -3436,11 +3461,33 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
             }
             code[(size_t)i] = v;
         }
-        uint32_t* d = pget<uint32_t>(c, "pshift", (size_t)c->N);
-        if (!d) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
-        HVCK(hipMemcpyAsync(d, code.data(), sizeof(uint32_t) * c->N, hipMemcpyHostToDevice, c->stream));
+        // (original, shift code) -> id, open addressing at load <= 1/2
+        uint64_t hs = 1024;
+        while ((int64_t)hs < 2 * ne) hs <<= 1;
+        std::vector<unsigned long long> hk(hs, ~0ull);
+        std::vector<int> hv(hs, -1);
+        for (int64_t i = 0; i < ne; ++i) {
+            const unsigned long long key = ((unsigned long long)(unsigned)c->img_orig[(size_t)i] << 32) | code[(size_t)i];
+            uint64_t h = hv_mix64(key) & (hs - 1);
+            while (hk[h] != ~0ull) h = (h + 1) & (hs - 1);
+            hk[h] = key;
+            hv[h] = (int)i;
+        }
+        uint32_t* dcode = pget<uint32_t>(c, "pshift", (size_t)ne);
+        int* dorig = pget<int>(c, "porig", (size_t)ne);
+        unsigned long long* dkeys = pget<unsigned long long>(c, "pkeys", hs);
+        int* dvals = pget<int>(c, "pvals", hs);
+        if (!dcode || !dorig || !dkeys || !dvals) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+        HVCK(hipMemcpyAsync(dcode, code.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipMemcpyAsync(dorig, c->img_orig.data(), sizeof(int) * ne, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipMemcpyAsync(dkeys, hk.data(), sizeof(unsigned long long) * hs, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipMemcpyAsync(dvals, hv.data(), sizeof(int) * hs, hipMemcpyHostToDevice, c->stream));
         HVCK(hipStreamSynchronize(c->stream));
-        pshift = d;
+        pshift = dcode;
+        porig = dorig;
+        pkeys = dkeys;
+        pvals = dvals;
+        hmask = hs - 1;
     }
     int G = std::max(1, (int)std::floor(std::pow((double)nsamples / 256.0, 1.0 / D)));
     G = std::min(G, (int)std::floor(std::pow(2.0, 40.0 / D)));   // G^d strata stay below 2^40
@@ -3450,9 +3497,8 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     S = (S + 63) / 64 * 64;
     const int64_t n = B * S;
     if (n / 64 > (int64_t)INT32_MAX) return hv_fail(c, HV_ERR_CAPACITY, "too many samples");
-    const double eps = slab * std::pow(vbox / (double)c->nreal, 1.0 / D);
     k_mc_points<D><<<(unsigned)(n / 64), 64, 0, c->stream>>>(g, G, S, seed, eps, ptr, pkey, ccnt, pcnt, box, pmask,
-                                                             (int)c->nreal, pshift);
+                                                             (int)c->nreal, porig, pshift, pkeys, pvals, hmask);
     HVCK(hipGetLastError());
     *nused = n;
     *eps_out = eps;
