@@ -312,6 +312,20 @@ int hv_integrate_polytope(hv_ctx* c, const hv_mesh* mesh, hv_integral** out) {
     HV_DISPATCH(c->D, hv_integrate_polytope_impl, c, mesh, out)
 }
 
+int hv_integrate_polytope_f(hv_ctx* c, const hv_mesh* mesh, int32_t nf, hv_integrand_fn f, void* user,
+                            hv_integral** out) {
+    if (!c || !mesh || !out) return hv_fail(c, HV_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    if (nf < 1 || !f) return hv_fail(c, HV_ERR_ARG, "integrand: nf >= 1 and a function");
+    (void)hipSetDevice(c->device);
+    HV_DISPATCH(c->D, hv_integrate_polytope_f_impl, c, mesh, nf, f, user, out)
+}
+
+int hv_integral_export_f(const hv_integral* I, int32_t* nf, double* bulk, double* iface) {
+    return hv_integral_export_f_impl(I, nf, bulk, iface);
+}
+
 int hv_integrate_mc(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t seed, hv_integral** out) {
     if (!c || !mesh || !out) return hv_fail(c, HV_ERR_ARG, "null argument");
     *out = nullptr;

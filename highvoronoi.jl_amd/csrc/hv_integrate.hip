@@ -316,9 +316,213 @@ __global__ void k_face_measure(IntGeo g, int m, int nface, const int* __restrict
     meas[f] = sum / (double)(D + 1 - m);
 }
 
+// ---- integrands (the reference's rule, src/polyintegrator.jl:174-353) -----------------------
+// Every face G is cut into the simplices of the chains of face centres (the mean of a face's
+// vertices) from G down to an edge; f enters only at the two ends of each edge:
+//   J(edge) = |e| (f(v1) + f(v2)) / 2,   J(G) = sum over sub-faces H of h_H(c_G) J(H) / dim G,
+// h_H(c_G) the distance within G from G's centre to H.  A facet F (dim d-1) then gets
+//   I(F) = (d-1)/d J(F) + f(c_F) |F| / d,
+// and a cell, with apex at its generator x,
+//   bulk = f(x) vol / (d+1) + sum over facets F of dist(x, F) I(F) / (d+1).
+// Exact for affine f.  nf <= HV_MAXF components per point.
+constexpr int HV_MAXF = 8;
+
+template <int D>
+__global__ void k_face_center(const int* __restrict__ fstart, int nface, const int* __restrict__ vid,
+                              const double* __restrict__ vc, double* __restrict__ cen) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    const int b = fstart[f], e = fstart[f + 1];
+    double acc[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) acc[k] = 0.0;
+    for (int i = b; i < e; ++i) {
+        const int v = vid[i];
+#pragma unroll
+        for (int k = 0; k < D; ++k) acc[k] += vc[(int64_t)v * D + k];
+    }
+    const double inv = 1.0 / (double)(e - b);
+#pragma unroll
+    for (int k = 0; k < D; ++k) cen[(int64_t)f * D + k] = acc[k] * inv;
+}
+
+__global__ void k_edge_int(const int* __restrict__ fstart, int nface, const int* __restrict__ vid,
+                           const double* __restrict__ meas, const double* __restrict__ fv, int nf,
+                           double* __restrict__ J) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    if (fstart[f + 1] - fstart[f] != 2) {   // flagged by k_edge_len; the integration fails
+        for (int c = 0; c < nf; ++c) J[(int64_t)f * nf + c] = 0.0;
+        return;
+    }
+    const int va = vid[fstart[f]], vb = vid[fstart[f] + 1];
+    for (int c = 0; c < nf; ++c)
+        J[(int64_t)f * nf + c] = meas[f] * 0.5 * (fv[(int64_t)va * nf + c] + fv[(int64_t)vb * nf + c]);
+}
+
+// J of each face of pass m from the faces of pass m + 1, apex at the face's centre
+template <int D>
+__global__ void k_face_int(IntGeo g, int m, int nface, const int* __restrict__ fstart, const int* __restrict__ vid,
+                           const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi,
+                           const double* __restrict__ cen, int nface1, const int* __restrict__ fstart1,
+                           const int* __restrict__ vid1, const uint64_t* __restrict__ flo1,
+                           const uint64_t* __restrict__ fhi1, const double* __restrict__ J1, int nf,
+                           const int* __restrict__ vk, double* __restrict__ J, int* __restrict__ bad) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    const uint64_t lo = flo[f], hi = fhi[f];
+    int S[D + 1];
+#pragma unroll
+    for (int s = 0; s <= D; ++s) S[s] = s < m ? hv_fkey_get(lo, hi, s) : -1;
+    const int a = S[0];
+    double Q[D][D];
+    int q = 0;
+#pragma unroll
+    for (int s = 1; s < D + 1; ++s) {
+        if (s >= m) break;
+        double w[D];
+        hv_face_grad<D>(g, a, S[s], w);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int r = 0; r < q; ++r) {
+                double cdot = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) cdot = fma(w[k], Q[r][k], cdot);
+#pragma unroll
+                for (int k = 0; k < D; ++k) w[k] = fma(-cdot, Q[r][k], w[k]);
+            }
+        double nw = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) nw = fma(w[k], w[k], nw);
+        nw = sqrt(nw);
+        if (!(nw > 0.0)) { atomicOr(bad, 2); return; }
+#pragma unroll
+        for (int k = 0; k < D; ++k) Q[q][k] = w[k] / nw;
+        ++q;
+    }
+    double p[D], da = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        p[k] = cen[(int64_t)f * D + k];
+        const double t = p[k] - g.xo[(int64_t)a * D + k];
+        da = fma(t, t, da);
+    }
+    double sum[HV_MAXF];
+    for (int c = 0; c < nf; ++c) sum[c] = 0.0;
+    const int b0 = fstart[f], e0 = fstart[f + 1];
+    for (int i = b0; i < e0; ++i) {
+        const int v = vid[i];
+        const int* kv = vk + (int64_t)v * (D + 1);
+        for (int e = 0; e <= D; ++e) {
+            const int j = kv[e];
+            bool skip = false;
+#pragma unroll
+            for (int s = 0; s <= D; ++s) skip = skip || (S[s] == j);
+            if (skip) continue;
+            uint64_t tl = ~0ull, th = ~0ull;
+            int s2 = 0;
+            bool put = false;
+            for (int s = 0; s < m; ++s) {
+                if (!put && j < S[s]) { hv_fkey_put(tl, th, s2++, (uint64_t)j); put = true; }
+                hv_fkey_put(tl, th, s2++, (uint64_t)S[s]);
+            }
+            if (!put) hv_fkey_put(tl, th, s2++, (uint64_t)j);
+            int lo_i = 0, hi_i = nface1;
+            while (lo_i < hi_i) {
+                const int mid = (lo_i + hi_i) >> 1;
+                if (hv_fkey_less(fhi1[mid], flo1[mid], th, tl)) lo_i = mid + 1;
+                else hi_i = mid;
+            }
+            if (lo_i >= nface1 || flo1[lo_i] != tl || fhi1[lo_i] != th) { atomicOr(bad, 4); continue; }
+            if (vid1[fstart1[lo_i]] != v) continue;   // each sub-face once
+            double w[D];
+            hv_face_grad<D>(g, a, j, w);
+            for (int pass = 0; pass < 2; ++pass)
+                for (int r = 0; r < q; ++r) {
+                    double cdot = 0.0;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) cdot = fma(w[k], Q[r][k], cdot);
+#pragma unroll
+                    for (int k = 0; k < D; ++k) w[k] = fma(-cdot, Q[r][k], w[k]);
+                }
+            double nr = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) nr = fma(w[k], w[k], nr);
+            nr = sqrt(nr);
+            double h;
+            if (j >= g.N) {
+                double np_ = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) np_ = fma(g.pn[(j - g.N) * D + k], p[k], np_);
+                h = fabs(g.ph[j - g.N] - np_) / nr;
+            } else {
+                double dj = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const double t = p[k] - g.xo[(int64_t)j * D + k];
+                    dj = fma(t, t, dj);
+                }
+                h = fabs(dj - da) / (2.0 * nr);
+            }
+            for (int c = 0; c < nf; ++c) sum[c] = fma(h, J1[(int64_t)lo_i * nf + c], sum[c]);
+        }
+    }
+    for (int c = 0; c < nf; ++c) J[(int64_t)f * nf + c] = sum[c] / (double)(D + 1 - m);
+}
+
+// facets: I(F) = (d-1)/d J(F) + f(c_F) |F| / d
+template <int D>
+__global__ void k_facet_int(int nface, const double* __restrict__ J, const double* __restrict__ fc,
+                            const double* __restrict__ meas, int nf, double* __restrict__ I) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    for (int c = 0; c < nf; ++c)
+        I[(int64_t)f * nf + c] = (double)(D - 1) / (double)D * J[(int64_t)f * nf + c] +
+                                 fc[(int64_t)f * nf + c] * meas[f] / (double)D;
+}
+
+// row t's interface integral from its facet (rows sorted with their facet numbers)
+__global__ void k_row_int(int64_t npair, const int* __restrict__ pface, const double* __restrict__ I, int nf,
+                          double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npair) return;
+    for (int c = 0; c < nf; ++c) out[t * nf + c] = I[(int64_t)pface[t] * nf + c];
+}
+
+// cells 0 .. nreal-1: f(x) vol / (d+1) + sum over rows of dist * I(row) / (d+1)
+template <int D>
+__global__ void k_cell_bulk(IntGeo g, const uint64_t* __restrict__ pkey, const int64_t* __restrict__ ptr,
+                            const double* __restrict__ vol, const double* __restrict__ rint,
+                            const double* __restrict__ fx, int nf, double* __restrict__ bulk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.nreal) return;
+    double y[HV_MAXF];
+    for (int c = 0; c < nf; ++c) y[c] = fx[(int64_t)i * nf + c] * vol[i];
+    for (int64_t t = ptr[i]; t < ptr[i + 1]; ++t) {
+        const int k = (int)(pkey[t] & 0xffffffffull);
+        double h;
+        if (k >= g.N) {
+            double nx = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) nx = fma(g.pn[(k - g.N) * D + c], g.xo[(int64_t)i * D + c], nx);
+            h = g.ph[k - g.N] - nx;
+        } else {
+            double s2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const double t2 = g.xo[(int64_t)i * D + c] - g.xo[(int64_t)k * D + c];
+                s2 = fma(t2, t2, s2);
+            }
+            h = 0.5 * sqrt(s2);
+        }
+        for (int c = 0; c < nf; ++c) y[c] = fma(h, rint[t * nf + c], y[c]);
+    }
+    for (int c = 0; c < nf; ++c) bulk[(int64_t)i * nf + c] = y[c] / (double)(D + 1);
+}
+
 // facets {a, b} -> (cell, neighbour, area) rows for both sides
 __global__ void k_facet_pairs(int nface, const uint64_t* __restrict__ flo, const double* __restrict__ meas, int N,
-                              uint64_t* __restrict__ pkey, double* __restrict__ parea, unsigned long long* __restrict__ np) {
+                              uint64_t* __restrict__ pkey, double* __restrict__ parea, int* __restrict__ pface,
+                              unsigned long long* __restrict__ np) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nface) return;
     const int a = hv_fkey_get(flo[f], 0, 0), b = hv_fkey_get(flo[f], 0, 1);
@@ -326,9 +530,11 @@ __global__ void k_facet_pairs(int nface, const uint64_t* __restrict__ flo, const
     const unsigned long long o = atomicAdd(np, (unsigned long long)n);
     pkey[o] = ((uint64_t)a << 32) | (uint64_t)b;
     parea[o] = meas[f];
+    pface[o] = f;
     if (n == 2) {
         pkey[o + 1] = ((uint64_t)b << 32) | (uint64_t)a;
         parea[o + 1] = meas[f];
+        pface[o + 1] = f;
     }
 }
 
@@ -521,6 +727,8 @@ struct hv_integral {
     std::vector<double> vol, area;
     std::vector<int64_t> ptr, nbr;
     double ms = 0.0;
+    int nf = 0;                        // integrand components (0: none)
+    std::vector<double> bulk, iface;   // nf x N, nf x rows
 };
 
 static int64_t binom(int n, int k) {
@@ -529,9 +737,30 @@ static int64_t binom(int n, int k) {
     return r;
 }
 
-// faces of the tessellation, all passes; leaves the sorted (cell, neighbour) rows with areas
+struct Integrand {
+    int nf;
+    hv_integrand_fn fn;
+    void* user;
+};
+
+// f at npts device points (D x npts) -> device values (nf x npts), through the caller's function
+static int eval_integrand(hv_ctx* c, const Integrand* fi, int D, const double* dpts, int64_t npts, double* dvals) {
+    if (npts == 0) return HV_OK;
+    std::vector<double> hp((size_t)npts * D), hv((size_t)npts * fi->nf, 0.0);
+    HVCK(hipMemcpyAsync(hp.data(), dpts, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    fi->fn(fi->user, npts, hp.data(), hv.data());
+    HVCK(hipMemcpyAsync(dvals, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    return HV_OK;
+}
+
+// faces of the tessellation, all passes; leaves the sorted (cell, neighbour) rows with areas.
+// fi (optional): also the integrals of the rows' interfaces (*rint, npair x nf) by the
+// reference's rule (see k_face_int)
 template <int D>
-static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey, double*& parea, int64_t& npair) {
+static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey, double*& parea, int64_t& npair,
+                      const Integrand* fi = nullptr, double** rint = nullptr) {
     const int N = (int)c->N;
     const int nv = (int)mesh->nv;
     if ((int64_t)N + c->P >= (int64_t)HV_F21) return hv_fail(c, HV_ERR_UNSUPPORTED, "integration packs ids in 21 bits");
@@ -576,6 +805,14 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey
         }
         k_int_permute<D><<<nblk(nv, 256), 256, 0, c->stream>>>(src, nv, vk0, vc0, vk, vc);
         HVCK(hipGetLastError());
+    }
+    const int nf = fi ? fi->nf : 0;
+    double* fv = nullptr;   // f at the vertices (edge ends)
+    if (fi) {
+        fv = iget<double>(c, "i_fv", (size_t)nv * nf);
+        if (!fv) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
+        int rc = eval_integrand(c, fi, D, vc, nv, fv);
+        if (rc) return rc;
     }
     // per pass: sorted incidences (vid), face starts, keys, measures; "1" = pass m + 1
     int nface1 = 0;
@@ -646,8 +883,14 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey
         if (!flo || !fhi || !meas) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
         k_face_table<<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, lo2, hi, flo, fhi);
         HVCK(hipGetLastError());
+        double* J = fi ? iget<double>(c, nm("i_J").c_str(), (size_t)nface * nf) : nullptr;
+        if (fi && !J) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
         if (m == D) {
             k_edge_len<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vc, meas, bad);
+            if (fi) {
+                HVCK(hipGetLastError());
+                k_edge_int<<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, meas, fv, nf, J);
+            }
         } else {
             const char* s1 = (m & 1) ? "b" : "a";
             auto n1 = [&](const char* s) { return std::string(s) + s1; };
@@ -659,17 +902,44 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey
             k_face_measure<D><<<nblk(nface, 128), 128, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi, nface1,
                                                                        fstart1, vid1, flo1, fhi1, meas1, vk, vc, meas,
                                                                        bad);
+            if (fi) {
+                HVCK(hipGetLastError());
+                double* cen = iget<double>(c, "i_cen", (size_t)nface * D);
+                const double* J1 = (const double*)c->pool->get(n1("i_J"), 0);
+                if (!cen || !J1) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
+                k_face_center<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vc, cen);
+                HVCK(hipGetLastError());
+                k_face_int<D><<<nblk(nface, 128), 128, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi, cen, nface1,
+                                                                       fstart1, vid1, flo1, fhi1, J1, nf, vk, J, bad);
+            }
         }
         HVCK(hipGetLastError());
+        double* Iface = nullptr;   // facet integrals (m == 2)
+        if (fi && m == 2) {
+            double* cen = iget<double>(c, "i_cen", (size_t)nface * D);
+            double* fc = iget<double>(c, "i_fc", (size_t)nface * nf);
+            Iface = iget<double>(c, "i_If", (size_t)nface * nf);
+            if (!cen || !fc || !Iface) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
+            if (m == D) {   // d = 2: the facets are the edges, no centres yet
+                k_face_center<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vc, cen);
+                HVCK(hipGetLastError());
+            }
+            int rc = eval_integrand(c, fi, D, cen, nface, fc);
+            if (rc) return rc;
+            k_facet_int<D><<<nblk(nface, 256), 256, 0, c->stream>>>(nface, J, fc, meas, nf, Iface);
+            HVCK(hipGetLastError());
+        }
         nface1 = nface;
         if (m == 2) {
             pkey = iget<uint64_t>(c, "i_pkey", (size_t)2 * nface);
             uint64_t* pkey2 = iget<uint64_t>(c, "i_pkey2", (size_t)2 * nface);
             parea = iget<double>(c, "i_parea", (size_t)2 * nface);
             double* parea2 = iget<double>(c, "i_parea2", (size_t)2 * nface);
-            if (!pkey || !pkey2 || !parea || !parea2) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (facets)");
+            int* pface2 = iget<int>(c, "i_pface2", (size_t)2 * nface);
+            if (!pkey || !pkey2 || !parea || !parea2 || !pface2)
+                return hv_fail(c, HV_ERR_DEVICE, "out of device memory (facets)");
             HVCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
-            k_facet_pairs<<<nblk(nface, 256), 256, 0, c->stream>>>(nface, flo, meas, N, pkey2, parea2, cnt);
+            k_facet_pairs<<<nblk(nface, 256), 256, 0, c->stream>>>(nface, flo, meas, N, pkey2, parea2, pface2, cnt);
             HVCK(hipGetLastError());
             unsigned long long hp = 0;
             HVCK(hipMemcpyAsync(&hp, cnt, sizeof(hp), hipMemcpyDeviceToHost, c->stream));
@@ -680,6 +950,22 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey
             void* tmp3 = c->pool->get("i_tmp3", tb3, false, c->stream);
             if (!tmp3) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
             HVCK(hipcub::DeviceRadixSort::SortPairs(tmp3, tb3, pkey2, pkey, parea2, parea, (int)npair, 0, 64, c->stream));
+            if (fi) {
+                // the same stable sort carrying the facet numbers, then the rows' integrals
+                uint64_t* pkey3 = iget<uint64_t>(c, "i_pkey3", (size_t)2 * nface);
+                int* pface = iget<int>(c, "i_pface", (size_t)2 * nface);
+                *rint = iget<double>(c, "i_rint", (size_t)std::max<int64_t>(npair, 1) * nf);
+                if (!pkey3 || !pface || !*rint) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
+                size_t tb4 = 0;
+                HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb4, pkey2, pkey3, pface2, pface, (int)npair, 0, 64, c->stream));
+                void* tmp4 = c->pool->get("i_tmp4", tb4, false, c->stream);
+                if (!tmp4) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
+                HVCK(hipcub::DeviceRadixSort::SortPairs(tmp4, tb4, pkey2, pkey3, pface2, pface, (int)npair, 0, 64, c->stream));
+                if (npair > 0) {
+                    k_row_int<<<nblk(npair, 256), 256, 0, c->stream>>>(npair, pface, Iface, nf, *rint);
+                    HVCK(hipGetLastError());
+                }
+            }
         }
     }
     int hbad = 0;
@@ -854,7 +1140,8 @@ static int make_geo(hv_ctx* c, const hv_mesh* mesh, IntGeo& g) {
 
 template <int D>
 static int finish(hv_ctx* c, int64_t ncell, const uint64_t* pkey, const double* area_d, const double* vol_d,
-                  const int64_t* ptr_d, int64_t npair, float ms, hv_integral** out) {
+                  const int64_t* ptr_d, int64_t npair, float ms, hv_integral** out, int nf = 0,
+                  const double* bulk_d = nullptr, const double* rint_d = nullptr) {
     // cells 0 .. ncell-1 (the originals) and their rows, which come first in the sorted order
     hv_integral* I = new hv_integral();
     I->N = ncell;
@@ -875,6 +1162,15 @@ static int finish(hv_ctx* c, int64_t ncell, const uint64_t* pkey, const double* 
         e = hipMemcpyAsync(hk.data(), pkey, sizeof(uint64_t) * npair, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(I->area.data(), area_d, sizeof(double) * npair, hipMemcpyDeviceToHost, c->stream);
     }
+    if (nf > 0) {
+        I->nf = nf;
+        I->bulk.resize((size_t)ncell * nf);
+        I->iface.resize((size_t)npair * nf);
+        if (e == hipSuccess && ncell)
+            e = hipMemcpyAsync(I->bulk.data(), bulk_d, sizeof(double) * I->bulk.size(), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && npair)
+            e = hipMemcpyAsync(I->iface.data(), rint_d, sizeof(double) * I->iface.size(), hipMemcpyDeviceToHost, c->stream);
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(I->vol.data(), vol_d, sizeof(double) * ncell, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(I->ptr.data(), ptr_d, sizeof(int64_t) * (ncell + 1), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -890,6 +1186,13 @@ static int finish(hv_ctx* c, int64_t ncell, const uint64_t* pkey, const double* 
 
 template <int D>
 int hv_integrate_polytope_impl(hv_ctx* c, const hv_mesh* mesh, hv_integral** out) {
+    return hv_integrate_polytope_f_impl<D>(c, mesh, 0, nullptr, nullptr, out);
+}
+
+template <int D>
+int hv_integrate_polytope_f_impl(hv_ctx* c, const hv_mesh* mesh, int nf, hv_integrand_fn fn, void* user,
+                                 hv_integral** out) {
+    if (nf < 0 || nf > HV_MAXF || (nf > 0 && !fn)) return hv_fail(c, HV_ERR_ARG, "integrand: 0 < nf <= 8 and a function");
     IntGeo g;
     int rc = make_geo<D>(c, mesh, g);
     if (rc) return rc;
@@ -897,18 +1200,31 @@ int hv_integrate_polytope_impl(hv_ctx* c, const hv_mesh* mesh, hv_integral** out
     uint64_t* pkey = nullptr;
     double* parea = nullptr;
     int64_t npair = 0;
-    rc = facet_rows<D>(c, mesh, g, pkey, parea, npair);
+    Integrand spec{nf, fn, user};
+    double* rint = nullptr;
+    rc = facet_rows<D>(c, mesh, g, pkey, parea, npair, nf > 0 ? &spec : nullptr, &rint);
     if (rc) return rc;
     double* vol = iget<double>(c, "i_vol", c->N);
     int64_t* ptr = iget<int64_t>(c, "i_ptr", c->N + 1);
     if (!vol || !ptr) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (volumes)");
     k_cell_volumes<D><<<nblk(c->N + 1, 256), 256, 0, c->stream>>>(g, pkey, parea, npair, vol, ptr);
     HVCK(hipGetLastError());
+    double* bulk = nullptr;
+    if (nf > 0) {
+        // f at the generators (the cells' apexes), then the bulk integrals
+        double* fx = iget<double>(c, "i_fx", (size_t)g.nreal * nf);
+        bulk = iget<double>(c, "i_bulk", (size_t)g.nreal * nf);
+        if (!fx || !bulk) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
+        rc = eval_integrand(c, &spec, D, g.xo, g.nreal, fx);
+        if (rc) return rc;
+        k_cell_bulk<D><<<nblk(g.nreal, 256), 256, 0, c->stream>>>(g, pkey, ptr, vol, rint, fx, nf, bulk);
+        HVCK(hipGetLastError());
+    }
     HVCK(hipEventRecord(c->ev[1], c->stream));
     HVCK(hipEventSynchronize(c->ev[1]));
     float ms = 0.f;
     HVCK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    return finish<D>(c, g.nreal, pkey, parea, vol, ptr, npair, ms, out);
+    return finish<D>(c, g.nreal, pkey, parea, vol, ptr, npair, ms, out, nf, bulk, rint);
 }
 
 template <int D>
@@ -1014,6 +1330,14 @@ int hv_integral_counts_impl(const hv_integral* I, int64_t* ncells, int64_t* nnbr
     return HV_OK;
 }
 
+int hv_integral_export_f_impl(const hv_integral* I, int32_t* nf, double* bulk, double* iface) {
+    if (!I) return HV_ERR_ARG;
+    if (nf) *nf = I->nf;
+    if (bulk) std::copy(I->bulk.begin(), I->bulk.end(), bulk);
+    if (iface) std::copy(I->iface.begin(), I->iface.end(), iface);
+    return HV_OK;
+}
+
 int hv_integral_export_impl(const hv_integral* I, double* vol, int64_t* nbr_ptr, int64_t* nbr, double* area,
                             double* ms) {
     if (!I) return HV_ERR_ARG;
@@ -1029,6 +1353,7 @@ void hv_integral_free_impl(hv_integral* I) { delete I; }
 
 #define HV_IINST(D)                                                                                 \
     template int hv_integrate_polytope_impl<D>(hv_ctx*, const hv_mesh*, hv_integral**);          \
+    template int hv_integrate_polytope_f_impl<D>(hv_ctx*, const hv_mesh*, int, hv_integrand_fn, void*, hv_integral**); \
     template int hv_integrate_mc_impl<D>(hv_ctx*, const hv_mesh*, int64_t, uint64_t, hv_integral**); \
     template int hv_integrate_mc_points_impl<D>(hv_ctx*, const hv_mesh*, int64_t, uint64_t, double, hv_integral**);
 HV_IINST(2)

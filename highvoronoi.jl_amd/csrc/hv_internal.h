@@ -94,6 +94,10 @@ struct hv_integral;
 template <int D>
 int hv_integrate_polytope_impl(hv_ctx* c, const hv_mesh* mesh, hv_integral** out);
 template <int D>
+int hv_integrate_polytope_f_impl(hv_ctx* c, const hv_mesh* mesh, int nf, hv_integrand_fn fn, void* user,
+                                 hv_integral** out);
+int hv_integral_export_f_impl(const hv_integral* I, int32_t* nf, double* bulk, double* iface);
+template <int D>
 int hv_integrate_mc_impl(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t seed, hv_integral** out);
 template <int D>
 int hv_integrate_mc_points_impl(hv_ctx* c, const hv_mesh* mesh, int64_t nsamples, uint64_t seed, double slab,

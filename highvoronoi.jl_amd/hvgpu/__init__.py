@@ -9,7 +9,7 @@ integrator singletons (hvgpu.geometry), imported lazily so that the CPU-only par
 from . import synth  # noqa: F401
 
 __all__ = ["synth", "VoronoiGeometry", "VoronoiData", "cuboid", "Boundary", "Plane",
-           "VI_GEOMETRY", "VI_POLYGON", "VI_MONTECARLO"]
+           "VI_GEOMETRY", "VI_POLYGON", "VI_MONTECARLO", "VI_FAST_POLYGON"]
 
 
 def __getattr__(name):

@@ -47,6 +47,10 @@ class hv_stats_t(C.Structure):
 _lib = None
 
 
+# void f(void* user, int64_t npts, const double* pts /* d x npts */, double* values /* nf x npts */)
+INTEGRAND = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
 def lib():
     global _lib
     if _lib is not None:
@@ -83,6 +87,8 @@ def lib():
     L.hv_integral_counts.argtypes = [vp, P(C.c_int64), P(C.c_int64)]
     L.hv_integral_export.argtypes = [vp, P(C.c_double), P(C.c_int64), P(C.c_int64), P(C.c_double), P(C.c_double)]
     L.hv_integral_free.argtypes = [vp]
+    L.hv_integrate_polytope_f.argtypes = [vp, vp, C.c_int32, INTEGRAND, vp, P(vp)]
+    L.hv_integral_export_f.argtypes = [vp, P(C.c_int32), P(C.c_double), P(C.c_double)]
     _lib = L
     return L
 
@@ -222,14 +228,36 @@ class Context:
         return idx, dist
 
     def integrate(self, mesh: Mesh, method: str = "polytope", ndir: int = 1000, seed: int = 0,
-                  nsamples: int = 10_000_000, slab: float = 0.0) -> dict:
+                  nsamples: int = 10_000_000, slab: float = 0.0,
+                  integrand=None, nf: int = 1) -> dict:
         """Cell volumes and interface areas of `mesh` (built from this context's points):
         method "polytope" (exact), "montecarlo" (ndir directions per cell) or "points"
         (nsamples stratified point samples, slab width factor `slab`).  Returns
-        {"volumes", "nbr_ptr", "neighbors" (1-based, planes n+p), "areas", "ms"}."""
+        {"volumes", "nbr_ptr", "neighbors" (1-based, planes n+p), "areas", "ms"}.
+        integrand (polytope only): f(points[npts, d]) -> values[npts, nf], called on the host;
+        adds "bulk" [ncells, nf] and "interface" [nrows, nf] (the reference's quadrature)."""
         L = lib()
         h = C.c_void_p()
-        if method == "polytope":
+        if method == "polytope" and integrand is not None:
+            d = self.d
+            err = []
+
+            def tramp(_user, npts, pts, vals):
+                try:
+                    x = np.ctypeslib.as_array(pts, shape=(npts, d))
+                    out = np.ctypeslib.as_array(vals, shape=(npts, nf))
+                    out[:] = np.asarray(integrand(x.copy()), dtype=np.float64).reshape(npts, nf)
+                except Exception as e:   # never raise through C; report after the call
+                    err.append(e)
+
+            cb = INTEGRAND(tramp)
+            rc = L.hv_integrate_polytope_f(self._h, mesh._h, int(nf), cb, None, C.byref(h))
+            if err:
+                if rc == HV_OK:
+                    L.hv_integral_free(h)
+                raise err[0]
+            self._check(rc)
+        elif method == "polytope":
             self._check(L.hv_integrate_polytope(self._h, mesh._h, C.byref(h)))
         elif method == "montecarlo":
             self._check(L.hv_integrate_mc(self._h, mesh._h, int(ndir), int(seed), C.byref(h)))
@@ -249,10 +277,18 @@ class Context:
             rc = L.hv_integral_export(h, _dp(vol), _i64(ptr), _i64(nbr), _dp(area), C.byref(ms))
             if rc != HV_OK:
                 raise HVError(rc, "integral export failed")
+            k = C.c_int32(0)
+            L.hv_integral_export_f(h, C.byref(k), None, None)
+            extra = {}
+            if k.value:
+                bulk = np.zeros((nc.value, k.value))
+                iface = np.zeros((max(1, nn.value), k.value))
+                L.hv_integral_export_f(h, C.byref(k), _dp(bulk), _dp(iface))
+                extra = {"bulk": bulk, "interface": iface[: nn.value]}
         finally:
             L.hv_integral_free(h)
         return {"volumes": vol, "nbr_ptr": ptr, "neighbors": nbr[: nn.value], "areas": area[: nn.value],
-                "ms": ms.value}
+                "ms": ms.value, **extra}
 
     def inrange(self, q, radius):
         q = np.ascontiguousarray(q, dtype=np.float64)

@@ -93,18 +93,23 @@ class VoronoiGeometry:
     search_settings: {"precision": "bf16" | "fp64", "leaf_size": int, "device": int}.
     mc_accurate: (directions per cell, unused, unused) as the reference's tuple
     (src/geometry.jl:139); seed drives the Monte-Carlo directions.
+    integrand (VI_POLYGON / VI_FAST_POLYGON): f(points[npts, d]) -> values[npts, k], the
+    reference's integrand=f (src/geometry.jl:136-160): bulk and interface integrals by the
+    polytope rule.  VI_FAST_POLYGON computes the same quantities as VI_POLYGON (the reference's
+    two exact integrators share the centre rule, src/fastpolyintegrator.jl:272-476) and runs the
+    same device integrator.
     """
 
     def __init__(self, xs, boundary: Boundary | None = None, integrator=VI_GEOMETRY, search_settings=None,
-                 mc_accurate=(1000, 100, 20), seed: int = 0):
+                 mc_accurate=(1000, 100, 20), seed: int = 0, integrand=None):
         self.nodes = np.ascontiguousarray(np.asarray(xs, dtype=np.float64))
         n, d = self.nodes.shape
         self.boundary = boundary if boundary is not None else Boundary([])
         s = dict(search_settings or {})
         self.precision = s.get("precision", "bf16")
         self.integrator = integrator
-        if integrator == VI_FAST_POLYGON:
-            raise NotImplementedError("VI_FAST_POLYGON is not part of the device path; use VI_POLYGON")
+        if integrand is not None and integrator not in (VI_POLYGON, VI_FAST_POLYGON):
+            raise NotImplementedError("integrands are integrated by VI_POLYGON / VI_FAST_POLYGON on the device path")
         self._ctx = Context(int(s.get("device", 0)))
         self._ctx.set_points(self.nodes)
         base, normal, bc = self.boundary.arrays()
@@ -112,8 +117,12 @@ class VoronoiGeometry:
         self._mesh = self._ctx.build_mesh(precision=self.precision, leaf_size=int(s.get("leaf_size", 0)))
         self.stats = self._mesh.stats()
         self.integral = None
-        if integrator == VI_POLYGON:
-            self.integral = self._ctx.integrate(self._mesh, "polytope")
+        if integrator in (VI_POLYGON, VI_FAST_POLYGON):
+            if integrand is None:
+                self.integral = self._ctx.integrate(self._mesh, "polytope")
+            else:
+                k = np.asarray(integrand(self.nodes[:1]), dtype=np.float64).reshape(1, -1).shape[1]
+                self.integral = self._ctx.integrate(self._mesh, "polytope", integrand=integrand, nf=k)
         elif integrator == VI_MONTECARLO:
             self.integral = self._ctx.integrate(self._mesh, "montecarlo", ndir=int(mc_accurate[0]), seed=seed)
         self._keys = None
@@ -173,6 +182,7 @@ class VoronoiData:
         pkeys = np.sort(public(keys), axis=1) if n_ext > n else keys
         self.vertices = _CellVertices(pkeys, coords, ptr, rows)
         I = VG.integral
+        self.bulk_integral = self.interface_integral = None
         if I is None:
             self.volume = self.neighbors = self.area = None
             return
@@ -181,3 +191,6 @@ class VoronoiData:
         nb = public(I["neighbors"]) if n_ext > n else I["neighbors"]
         self.neighbors = [nb[p[i]:p[i + 1]] for i in range(n)]
         self.area = [I["areas"][p[i]:p[i + 1]] for i in range(n)]
+        if "bulk" in I:   # integrand=f: per cell, and per interface aligned with neighbors[i]
+            self.bulk_integral = I["bulk"]
+            self.interface_integral = [I["interface"][p[i]:p[i + 1]] for i in range(n)]

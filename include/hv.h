@@ -171,6 +171,16 @@ int hv_inrange_batch(hv_ctx* ctx, int64_t nq, const double* q, const double* rad
  * (src/polyintegrator.jl:33, :80-468; driver src/integrate.jl:252-382; neighbours
  * src/neighbors.jl:214-258).  General position only (HV_ERR_DEGENERATE otherwise). */
 int hv_integrate_polytope(hv_ctx* ctx, const hv_mesh* mesh, hv_integral** out);
+/* The same with an integrand f: R^d -> R^nf (1 <= nf <= 8), the VoronoiGeometry(...;
+ * integrand=f) path of the polytope integrators (src/polyintegrator.jl:174-353, the rule of
+ * src/fastpolyintegrator.jl:272-476): bulk integrals of f over the cells and interface
+ * integrals over each (cell, neighbour) row, by the reference's quadrature (trapezoid on
+ * edges; chains of face centres; f at each facet centre and at the cell's generator; exact
+ * for affine f).  f is called on the host, a few times per integration, with npts points
+ * (d x npts column-major) and fills values (nf x npts); user is passed through. */
+typedef void (*hv_integrand_fn)(void* user, int64_t npts, const double* pts, double* values);
+int hv_integrate_polytope_f(hv_ctx* ctx, const hv_mesh* mesh, int32_t nf, hv_integrand_fn f, void* user,
+                            hv_integral** out);
 /* Directional Monte-Carlo volumes and areas, ndir rays per cell: the integrator=VI_MONTECARLO
  * path, Montecarlo_Integrator (src/mcintegrator.jl:29-226, src/cleanup_cell.jl:6-48), with
  * independent directions per cell drawn from a counter-based stream keyed by seed. */
@@ -191,6 +201,10 @@ int hv_integrate_mc_points(hv_ctx* ctx, const hv_mesh* mesh, int64_t nsamples, u
  * interface areas aligned with them.  ms (optional) receives the device time. */
 int hv_integral_counts(const hv_integral* I, int64_t* ncells, int64_t* nnbr);
 int hv_integral_export(const hv_integral* I, double* vol, int64_t* nbr_ptr, int64_t* nbr, double* area, double* ms);
+/* Integrand results of hv_integrate_polytope_f (Voronoi_Integral's bulk_integral and
+ * interface_integral, src/integral.jl:11-21): nf, bulk (nf x ncells) and iface (nf x nnbr,
+ * aligned with nbr); nf = 0 for an integral computed without an integrand. */
+int hv_integral_export_f(const hv_integral* I, int32_t* nf, double* bulk, double* iface);
 int hv_integral_free(hv_integral* I);
 
 #ifdef __cplusplus

@@ -202,3 +202,110 @@ def mc_direction(d: int, seed: int, cell: int, ndir: int, q: int):
     u = np.zeros(d)
     lib().orc_mc_direction(d, seed, cell, ndir, q, _dp(u))
     return u
+
+
+def polytope_integrand(xs, base, normal, keys, coords, f, nf: int, ncell: int | None = None):
+    """Integrals of f (points[npts, d] -> values[npts, nf]) over the cells and the interfaces
+    by the reference's polytope rule, pure Python over the face lattice of the vertex keys
+    (small cases only).  Returns (bulk[n, nf], {(i, j): interface[nf]}) with 0-based cell i and
+    1-based neighbour id j (planes n + p).
+
+    The rule (src/polyintegrator.jl:174-353; the same in src/fastpolyintegrator.jl:272-476):
+    a face G (the vertices holding the generator set S) is cut into the simplices of the chains
+    of face centres (mean of a face's vertices, src/polyintegrator.jl:470-507) down to an edge,
+    f entering only at the edge ends (src/polyintegrator.jl:262-269, the edge value
+    (f(r1) + f(r2)) / 2 times the chain's measure, :176-189):
+        J(edge) = |e| (f(v1) + f(v2)) / 2,   J(G) = sum_H dist(c_G, aff H) J(H) / dim G;
+    a facet F of cell i: I(F) = (d-1)/d J(F) + f(c_F) |F| / d (:317-322); and the cell, with
+    apex at its generator: bulk = f(x_i) vol_i / (d+1) + sum_F dist(x_i, F) I(F) / (d+1)
+    (:323-328), where |F| and vol_i follow the same recursion with f = 1.
+    ncell: integrate only the cells 0..ncell-1 (the originals of a periodic mesh).
+    """
+    xs = np.asarray(xs, dtype=np.float64)
+    n, d = xs.shape
+    keys = np.asarray(keys, dtype=np.int64)
+    coords = np.asarray(coords, dtype=np.float64)
+    base = np.asarray(base, dtype=np.float64)
+    normal = np.asarray(normal, dtype=np.float64)
+    fv = np.asarray(f(coords), dtype=np.float64).reshape(len(coords), nf)
+    ksets = [frozenset(int(v) for v in row) for row in keys]
+    by_gen: dict[int, list[int]] = {}
+    for r, ks in enumerate(ksets):
+        for g in ks:
+            by_gen.setdefault(g, []).append(r)
+    memo: dict[frozenset, tuple] = {}
+
+    def verts(S):
+        rows = None
+        for g in S:
+            rg = set(by_gen.get(g, ()))
+            rows = rg if rows is None else rows & rg
+        return sorted(rows)
+
+    def dist_to_aff(c, pts, k):
+        # distance from c to the affine hull of pts, of dimension k
+        v0 = pts[0]
+        if k == 0:
+            return float(np.linalg.norm(c - v0))
+        A = (pts[1:] - v0).T
+        u, sv, _ = np.linalg.svd(A, full_matrices=False)
+        B = u[:, :k]
+        r = (c - v0) - B @ (B.T @ (c - v0))
+        return float(np.linalg.norm(r))
+
+    def face(S):
+        """(measure, J[nf], centre, vertex rows) of the face with generator set S."""
+        if S in memo:
+            return memo[S]
+        rows = verts(S)
+        dimG = d + 1 - len(S)
+        pts = coords[rows]
+        if dimG == 1:
+            assert len(rows) == 2
+            L = float(np.linalg.norm(pts[0] - pts[1]))
+            out = (L, L * 0.5 * (fv[rows[0]] + fv[rows[1]]), pts.mean(axis=0), rows)
+        else:
+            c = pts.mean(axis=0)
+            meas = 0.0
+            J = np.zeros(nf)
+            subs = set()
+            for r in rows:
+                for g in ksets[r]:
+                    if g not in S:
+                        subs.add(g)
+            for g in sorted(subs):
+                H = S | {g}
+                mH, JH, _, rH = face(H)
+                h = dist_to_aff(c, coords[rH], dimG - 1)
+                meas += h * mH / dimG
+                J = J + h * JH / dimG
+            out = (meas, J, c, rows)
+        memo[S] = out
+        return out
+
+    nc = n if ncell is None else ncell
+    bulk = np.zeros((nc, nf))
+    iface = {}
+    fx = np.asarray(f(xs[:nc]), dtype=np.float64).reshape(nc, nf)
+    for i in range(nc):
+        gi = i + 1
+        nbrs = set()
+        for r in by_gen.get(gi, ()):
+            nbrs |= ksets[r]
+        nbrs.discard(gi)
+        vol = 0.0
+        y = np.zeros(nf)
+        for j in sorted(nbrs):
+            mF, JF, cF, _ = face(frozenset((gi, j)))
+            fc = np.asarray(f(cF[None, :]), dtype=np.float64).reshape(nf)
+            I = (d - 1) / d * JF + fc * mF / d
+            if j > n:
+                p = j - n - 1
+                dist = float(np.dot(normal[p], base[p] - xs[i]))
+            else:
+                dist = 0.5 * float(np.linalg.norm(xs[j - 1] - xs[i]))
+            iface[(i, j)] = I
+            vol += dist * mF / d
+            y = y + dist * I / (d + 1)
+        bulk[i] = fx[i] * vol / (d + 1) + y
+    return bulk, iface
