@@ -281,6 +281,8 @@ def bench_integrators(args):
         ctx.set_boundary(base, normal)
         m, tm = timed(lambda: ctx.build_mesh(precision=args.precision))
         ex, tp = timed(lambda: ctx.integrate(m, "polytope"))
+        f2 = lambda p: np.stack([p[:, 0], p[:, 1] ** 2], axis=1)
+        ei, ti = timed(lambda: ctx.integrate(m, "polytope", integrand=f2, nf=2))
         ndir = 1000 if d < 6 else 200
         md, td = timed(lambda: ctx.integrate(m, "montecarlo", ndir=ndir, seed=1))
         mp, tq = timed(lambda: ctx.integrate(m, "points", nsamples=100_000_000, seed=1))
@@ -292,12 +294,14 @@ def bench_integrators(args):
             return [float(np.sum(np.abs(r["volumes"] - ex["volumes"])) / np.sum(ex["volumes"])),
                     float(np.sum(np.abs(r["areas"] - A)) / np.sum(A))]
 
-        rows[name] = {"vertices": nv, "mesh_ms": tm, "polytope_ms": tp, "directional_mc_ms": td,
+        rows[name] = {"vertices": nv, "mesh_ms": tm, "polytope_ms": tp, "polytope_integrand_ms": ti,
+                      "integral_x0": float(ei["bulk"][:, 0].sum()), "integral_x1_sq": float(ei["bulk"][:, 1].sum()),
+                      "directional_mc_ms": td,
                       "directional_mc_ndir": ndir, "directional_mc_l1_vol_area": err(md),
                       "points_mc_ms": tq, "points_mc_samples": 100_000_000, "points_mc_l1_vol_area": err(mp)}
         log(f"[integrators] {name}: {rows[name]}")
     out = {
-        "metric": "integrator wall time per call (mesh, exact polytope, directional MC, point-sample MC)",
+        "metric": "integrator wall time per call (mesh, exact polytope, polytope with an integrand, directional MC, point-sample MC)",
         "value": rows["6D N=50000"]["points_mc_ms"],
         "unit": "ms (6D N=50k point-sample MC, 1e8 samples)",
         "n_gpus": 1,
