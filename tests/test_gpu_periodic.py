@@ -51,6 +51,19 @@ def test_periodic_mesh_matches_tiled_oracle(ctx, d, n, seed, per, prec):
     compare(periodic_oracle(xs, per), mine)
 
 
+@pytest.mark.parametrize("d,n,seed", [(2, 6, 81), (3, 9, 82), (2, 3, 83)])
+def test_periodic_few_points(ctx, d, n, seed):
+    # cells about as large as the box: images from up to two lengths away
+    xs = synth.uniform(d, n, seed)
+    m = build(ctx, xs, tuple(range(d)))
+    mine = periodic_mesh_labels(m)
+    assert np.abs(m.images()[1]).max() >= 1
+    I = ctx.integrate(m, "polytope")
+    m.free()
+    compare(periodic_oracle(xs, tuple(range(d)), copies=2), mine)
+    assert abs(I["volumes"].sum() - 1.0) < 1e-12
+
+
 def test_periodic_images_table(ctx):
     xs = synth.uniform(3, 400, 11)
     m = build(ctx, xs, (0, 1, 2))
@@ -130,6 +143,24 @@ def test_periodic_point_sample_mc(ctx, d, n, seed):
     assert np.array_equal(mc["nbr_ptr"], ex["nbr_ptr"]) and np.array_equal(mc["neighbors"], ex["neighbors"])
     tot_ex, tot_mc = ex["areas"].sum(), mc["areas"].sum()
     assert abs(tot_mc - tot_ex) < 0.02 * tot_ex
+    m.free()
+
+
+def test_partly_periodic_point_sample_mc(ctx):
+    # periodic in x only: the x planes bound nothing, the y and z planes are boundary rows
+    xs = synth.uniform(3, 1500, 53)
+    m = build(ctx, xs, (0,))
+    ex = ctx.integrate(m, "polytope")
+    mc = ctx.integrate(m, "points", nsamples=20_000_000, seed=4)
+    assert abs(mc["volumes"].sum() - 1.0) < 1e-12
+    assert np.array_equal(mc["neighbors"], ex["neighbors"])
+    n_ext = len(m.images()[0])
+    planes = mc["neighbors"] > n_ext
+    assert set(np.unique(mc["neighbors"][planes] - n_ext)) == {3, 4, 5, 6}
+    # the boundary faces (y, z sides: total area 4) and all interfaces, within sampling noise
+    assert abs(mc["areas"][planes].sum() - 4.0) < 0.02 * 4.0
+    assert abs(ex["areas"][planes].sum() - 4.0) < 1e-10
+    assert np.abs(mc["volumes"] - ex["volumes"]).sum() < 0.02
     m.free()
 
 
