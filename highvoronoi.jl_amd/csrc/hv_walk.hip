@@ -2815,9 +2815,20 @@ static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
     const std::vector<int16_t> pbc = c->pbc;
     for (int attempt = 0;; ++attempt) {
         // the extended set: originals (shift 0), then per original its images in [lo - delta, hi + delta]
-        std::vector<double> ext(hx);
-        std::vector<int32_t> iorig((size_t)n);
-        std::vector<int8_t> ishift((size_t)n * D, 0);
+        // expected size: the box widened by delta in the periodic directions, at the same density
+        double grow = 1.0;
+        for (int k = 0; k < D; ++k)
+            if ((mask >> k) & 1) grow *= 1.0 + 2.0 * delta / (box.hi[k] - box.lo[k]);
+        const size_t expect = (size_t)std::min(1.1 * grow * (double)n + 1024.0, (double)(1ll << 30));
+        std::vector<double> ext;
+        std::vector<int32_t> iorig;
+        std::vector<int8_t> ishift;
+        ext.reserve(expect * D);
+        iorig.reserve(expect);
+        ishift.reserve(expect * D);
+        ext.assign(hx.begin(), hx.end());
+        iorig.resize((size_t)n);
+        ishift.assign((size_t)n * D, 0);
         for (int64_t i = 0; i < n; ++i) iorig[(size_t)i] = (int32_t)i;
         int lo_s[6], cnt_s[6];
         for (int64_t i = 0; i < n; ++i) {
