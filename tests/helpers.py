@@ -81,6 +81,8 @@ def periodic_tiling(xs, per, copies=1, lo=0.0, hi=1.0):
     planes in the cuboid order).  Returns (tiled, orig, shift, base, normal)."""
     from itertools import product
     n, d = xs.shape
+    lo = np.broadcast_to(np.asarray(lo, dtype=np.float64), (d,))
+    hi = np.broadcast_to(np.asarray(hi, dtype=np.float64), (d,))
     L = hi - lo
     shifts = [np.zeros(d, dtype=np.int64)]
     for s in product(range(-copies, copies + 1), repeat=len(per)):
@@ -93,11 +95,12 @@ def periodic_tiling(xs, per, copies=1, lo=0.0, hi=1.0):
     shift = np.repeat(np.array(shifts), n, axis=0)
     base, normal = [], []
     for k in range(d):
-        w = copies * L if k in per else 0.0
+        w = copies * L[k] if k in per else 0.0
         e = np.zeros(d)
         e[k] = 1.0
-        base.append(np.where(e > 0, hi + w, 0.0)); normal.append(e.copy())
-        b = np.zeros(d); b[k] = lo - w
+        b = np.zeros(d); b[k] = hi[k] + w
+        base.append(b); normal.append(e.copy())
+        b = np.zeros(d); b[k] = lo[k] - w
         base.append(b); normal.append(-e)
     return tiled, orig, shift, np.array(base), np.array(normal)
 
@@ -118,12 +121,12 @@ def vertex_labels(keys, n_ext, orig0, shift):
     return out
 
 
-def periodic_oracle(xs, per, copies=1):
-    """Vertices of the originals' cells in the periodic unit box (directions `per`), from the
-    oracle's walk over the cells 1..n of the tiled set: {label tuple: coordinates}."""
+def periodic_oracle(xs, per, copies=1, lo=0.0, hi=1.0):
+    """Vertices of the originals' cells in the periodic box [lo, hi] (directions `per`), from
+    the oracle's walk over the cells 1..n of the tiled set: {label tuple: coordinates}."""
     from oracle_py import Oracle
     n = xs.shape[0]
-    tiled, orig, shift, base, normal = periodic_tiling(xs, per, copies)
+    tiled, orig, shift, base, normal = periodic_tiling(xs, per, copies, lo, hi)
     keys, coords, _ = Oracle(tiled, base, normal).voronoi(cells=(0, n))
     labs = vertex_labels(keys, len(tiled), orig, shift)
     return dict(zip(labs, coords))

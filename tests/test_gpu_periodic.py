@@ -64,6 +64,21 @@ def test_periodic_few_points(ctx, d, n, seed):
     assert abs(I["volumes"].sum() - 1.0) < 1e-12
 
 
+def test_periodic_box_with_offset_and_lengths(ctx):
+    # cuboid(3; dimensions=[2, 1, 0.5], offset=[-1, 0.5, 0], periodic=[1, 3]): volume 1
+    dims, off = np.array([2.0, 1.0, 0.5]), np.array([-1.0, 0.5, 0.0])
+    xs = off + synth.uniform(3, 250, 91) * dims
+    base, normal, bc = cuboid(3, dimensions=dims, offset=off, periodic=[1, 3]).arrays()
+    ctx.set_points(xs)
+    ctx.set_boundary(base, normal, bc)
+    m = ctx.build_mesh(precision="bf16")
+    mine = periodic_mesh_labels(m)
+    I = ctx.integrate(m, "polytope")
+    m.free()
+    compare(periodic_oracle(xs, (0, 2), lo=off, hi=off + dims), mine)
+    assert abs(I["volumes"].sum() - 1.0) < 1e-12
+
+
 def test_periodic_images_table(ctx):
     xs = synth.uniform(3, 400, 11)
     m = build(ctx, xs, (0, 1, 2))
