@@ -1183,7 +1183,7 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
     __shared__ int stk[128], stkl[128];
     __shared__ float cbx[16 * D];
     __shared__ unsigned long long hms[8];
-    __shared__ double px[D][64];
+    __shared__ float px[D][64];
     const int lane = threadIdx.x;
     const int N = g.N;
     const int64_t smp = (int64_t)hv_xcd_block() * 64 + lane;
@@ -1296,20 +1296,20 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
                 if (__ballot(hit) == 0ull) continue;
                 for (int s0 = 0; s0 < npts; s0 += 64) {
                     const int cnt = min(64, npts - s0);
-                    if (lane < cnt) {
+                    if (lane < cnt) {   // FP32 rows; the two nearest are measured again in FP64
 #pragma unroll
-                        for (int k = 0; k < D; ++k) px[k][lane] = g.xs[(int64_t)(j0 + s0 + lane) * (D + 1) + k];
+                        for (int k = 0; k < D; ++k) px[k][lane] = g.xf[(int64_t)(j0 + s0 + lane) * 8 + k];
                     }
                     hv_wave_sync();
                     if (hit) {
                         for (int q = 0; q < cnt; ++q) {
-                            double d2 = 0.0;
+                            float d2 = 0.f;
 #pragma unroll
                             for (int k = 0; k < D; ++k) {
-                                const double a = px[k][q] - p[k];
-                                d2 = fma(a, a, d2);
+                                const float a = px[k][q] - pf[k];
+                                d2 = fmaf(a, a, d2);
                             }
-                            consider(j0 + s0 + q, d2);
+                            consider(j0 + s0 + q, (double)d2);
                         }
                     }
                     hv_wave_sync();
@@ -1326,6 +1326,24 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
                 stkl[sp] = l + 1;
                 ++sp;
             }
+        }
+    }
+    // the two nearest in FP64 (the FP32 scan may order a near tie either way)
+    auto d2_64 = [&](int j) {
+        double s2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double a = g.xs[(int64_t)j * (D + 1) + k] - p[k];
+            s2 = fma(a, a, s2);
+        }
+        return s2;
+    };
+    bd0 = d2_64(bi0);   // bi0 >= 0: the walk starts with an unbounded radius and N > d
+    if (bi1 >= 0) {
+        bd1 = d2_64(bi1);
+        if (bd1 < bd0) {
+            const double td = bd0; bd0 = bd1; bd1 = td;
+            const int ti = bi0; bi0 = bi1; bi1 = ti;
         }
     }
     int cell = g.sidx[bi0];
