@@ -202,16 +202,19 @@ __device__ __forceinline__ void hv_face_grad(const IntGeo& g, int a, int b, doub
     }
 }
 
-// Measure of each face of pass m from the faces of pass m + 1 (one thread per face).
-template <int D>
-__global__ void k_face_measure(IntGeo g, int m, int nface, const int* __restrict__ fstart, const int* __restrict__ vid,
-                               const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi, int nface1,
-                               const int* __restrict__ fstart1, const int* __restrict__ vid1,
-                               const uint64_t* __restrict__ flo1, const uint64_t* __restrict__ fhi1,
-                               const double* __restrict__ meas1, const int* __restrict__ vk,
-                               const double* __restrict__ vc, double* __restrict__ meas, int* __restrict__ bad) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nface) return;
+// Measure of each face of pass m from the faces of pass m + 1.  WAVE = false: one thread per
+// face; WAVE = true: one wavefront per face, its lanes splitting the face's vertex list (faces
+// of many vertices: the facets and ridges of cells in 4D and up), the partial sums reduced.
+// (A dispatch holds at most 2^32 work-items, so the WAVE form walks the faces grid-stride.)
+template <int D, bool WAVE>
+__device__ __forceinline__ void hv_face_measure_one(int f, int lane, int step, const IntGeo& g, int m,
+                                                    const int* __restrict__ fstart, const int* __restrict__ vid,
+                                                    const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi,
+                                                    int nface1, const int* __restrict__ fstart1,
+                                                    const int* __restrict__ vid1, const uint64_t* __restrict__ flo1,
+                                                    const uint64_t* __restrict__ fhi1, const double* __restrict__ meas1,
+                                                    const int* __restrict__ vk, const double* __restrict__ vc,
+                                                    double* __restrict__ meas, int* __restrict__ bad) {
     const uint64_t lo = flo[f], hi = fhi[f];
     int S[D + 1];
 #pragma unroll
@@ -237,7 +240,10 @@ __global__ void k_face_measure(IntGeo g, int m, int nface, const int* __restrict
 #pragma unroll
         for (int k = 0; k < D; ++k) nw = fma(w[k], w[k], nw);
         nw = sqrt(nw);
-        if (!(nw > 0.0)) { atomicOr(bad, 2); meas[f] = 0.0; return; }
+        if (!(nw > 0.0)) {   // the same on every lane of a wavefront
+            if (lane == 0) { atomicOr(bad, 2); meas[f] = 0.0; }
+            return;
+        }
 #pragma unroll
         for (int k = 0; k < D; ++k) Q[q][k] = w[k] / nw;
         ++q;
@@ -253,7 +259,7 @@ __global__ void k_face_measure(IntGeo g, int m, int nface, const int* __restrict
         da = fma(t, t, da);
     }
     double sum = 0.0;
-    for (int i = b0; i < e0; ++i) {
+    for (int i = b0 + lane; i < e0; i += step) {
         const int v = vid[i];
         const int* kv = vk + (int64_t)v * (D + 1);
         for (int e = 0; e <= D; ++e) {
@@ -313,7 +319,31 @@ __global__ void k_face_measure(IntGeo g, int m, int nface, const int* __restrict
             sum = fma(h, meas1[lo_i], sum);
         }
     }
+    if constexpr (WAVE) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+        if (lane != 0) return;
+    }
     meas[f] = sum / (double)(D + 1 - m);
+}
+
+template <int D, bool WAVE>
+__global__ void k_face_measure(IntGeo g, int m, int nface, const int* __restrict__ fstart, const int* __restrict__ vid,
+                               const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi, int nface1,
+                               const int* __restrict__ fstart1, const int* __restrict__ vid1,
+                               const uint64_t* __restrict__ flo1, const uint64_t* __restrict__ fhi1,
+                               const double* __restrict__ meas1, const int* __restrict__ vk,
+                               const double* __restrict__ vc, double* __restrict__ meas, int* __restrict__ bad) {
+    if constexpr (WAVE) {
+        for (int f = blockIdx.x; f < nface; f += gridDim.x)
+            hv_face_measure_one<D, true>(f, (int)(threadIdx.x & 63), 64, g, m, fstart, vid, flo, fhi, nface1, fstart1,
+                                         vid1, flo1, fhi1, meas1, vk, vc, meas, bad);
+    } else {
+        const int f = blockIdx.x * blockDim.x + threadIdx.x;
+        if (f < nface)
+            hv_face_measure_one<D, false>(f, 0, 1, g, m, fstart, vid, flo, fhi, nface1, fstart1, vid1, flo1, fhi1,
+                                          meas1, vk, vc, meas, bad);
+    }
 }
 
 // ---- integrands (the reference's rule, src/polyintegrator.jl:174-353) -----------------------
@@ -360,16 +390,17 @@ __global__ void k_edge_int(const int* __restrict__ fstart, int nface, const int*
         J[(int64_t)f * nf + c] = meas[f] * 0.5 * (fv[(int64_t)va * nf + c] + fv[(int64_t)vb * nf + c]);
 }
 
-// J of each face of pass m from the faces of pass m + 1, apex at the face's centre
-template <int D>
-__global__ void k_face_int(IntGeo g, int m, int nface, const int* __restrict__ fstart, const int* __restrict__ vid,
-                           const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi,
-                           const double* __restrict__ cen, int nface1, const int* __restrict__ fstart1,
-                           const int* __restrict__ vid1, const uint64_t* __restrict__ flo1,
-                           const uint64_t* __restrict__ fhi1, const double* __restrict__ J1, int nf,
-                           const int* __restrict__ vk, double* __restrict__ J, int* __restrict__ bad) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nface) return;
+// J of each face of pass m from the faces of pass m + 1, apex at the face's centre (WAVE: one
+// wavefront per face, as k_face_measure)
+template <int D, bool WAVE>
+__device__ __forceinline__ void hv_face_int_one(int f, int lane, int step, const IntGeo& g, int m,
+                                                const int* __restrict__ fstart, const int* __restrict__ vid,
+                                                const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi,
+                                                const double* __restrict__ cen, int nface1,
+                                                const int* __restrict__ fstart1, const int* __restrict__ vid1,
+                                                const uint64_t* __restrict__ flo1, const uint64_t* __restrict__ fhi1,
+                                                const double* __restrict__ J1, int nf, const int* __restrict__ vk,
+                                                double* __restrict__ J, int* __restrict__ bad) {
     const uint64_t lo = flo[f], hi = fhi[f];
     int S[D + 1];
 #pragma unroll
@@ -394,7 +425,10 @@ __global__ void k_face_int(IntGeo g, int m, int nface, const int* __restrict__ f
 #pragma unroll
         for (int k = 0; k < D; ++k) nw = fma(w[k], w[k], nw);
         nw = sqrt(nw);
-        if (!(nw > 0.0)) { atomicOr(bad, 2); return; }
+        if (!(nw > 0.0)) {
+            if (lane == 0) atomicOr(bad, 2);
+            return;
+        }
 #pragma unroll
         for (int k = 0; k < D; ++k) Q[q][k] = w[k] / nw;
         ++q;
@@ -409,7 +443,7 @@ __global__ void k_face_int(IntGeo g, int m, int nface, const int* __restrict__ f
     double sum[HV_MAXF];
     for (int c = 0; c < nf; ++c) sum[c] = 0.0;
     const int b0 = fstart[f], e0 = fstart[f + 1];
-    for (int i = b0; i < e0; ++i) {
+    for (int i = b0 + lane; i < e0; i += step) {
         const int v = vid[i];
         const int* kv = vk + (int64_t)v * (D + 1);
         for (int e = 0; e <= D; ++e) {
@@ -466,8 +500,38 @@ __global__ void k_face_int(IntGeo g, int m, int nface, const int* __restrict__ f
             for (int c = 0; c < nf; ++c) sum[c] = fma(h, J1[(int64_t)lo_i * nf + c], sum[c]);
         }
     }
+    if constexpr (WAVE) {
+        for (int c = 0; c < nf; ++c) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) sum[c] += __shfl_xor(sum[c], off);
+        }
+        if (lane != 0) return;
+    }
     for (int c = 0; c < nf; ++c) J[(int64_t)f * nf + c] = sum[c] / (double)(D + 1 - m);
 }
+
+template <int D, bool WAVE>
+__global__ void k_face_int(IntGeo g, int m, int nface, const int* __restrict__ fstart, const int* __restrict__ vid,
+                           const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi,
+                           const double* __restrict__ cen, int nface1, const int* __restrict__ fstart1,
+                           const int* __restrict__ vid1, const uint64_t* __restrict__ flo1,
+                           const uint64_t* __restrict__ fhi1, const double* __restrict__ J1, int nf,
+                           const int* __restrict__ vk, double* __restrict__ J, int* __restrict__ bad) {
+    if constexpr (WAVE) {
+        for (int f = blockIdx.x; f < nface; f += gridDim.x)
+            hv_face_int_one<D, true>(f, (int)(threadIdx.x & 63), 64, g, m, fstart, vid, flo, fhi, cen, nface1, fstart1,
+                                     vid1, flo1, fhi1, J1, nf, vk, J, bad);
+    } else {
+        const int f = blockIdx.x * blockDim.x + threadIdx.x;
+        if (f < nface)
+            hv_face_int_one<D, false>(f, 0, 1, g, m, fstart, vid, flo, fhi, cen, nface1, fstart1, vid1, flo1, fhi1,
+                                      J1, nf, vk, J, bad);
+    }
+}
+
+// faces of many vertices get a wavefront each: the facets and ridges (2 and 3 generators)
+// from 5D on (6D N=50k: 5.4 -> 3.3 s; a thread per face is faster in 4D, and for 6D 3-faces)
+__host__ inline bool hv_face_wave(int D, int m) { return D >= 5 && m <= 3; }
 
 // facets: I(F) = (d-1)/d J(F) + f(c_F) |F| / d
 template <int D>
@@ -899,9 +963,14 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey
             const uint64_t* flo1 = (const uint64_t*)c->pool->get(n1("i_flo"), 0);
             const uint64_t* fhi1 = (const uint64_t*)c->pool->get(n1("i_fhi"), 0);
             const double* meas1 = (const double*)c->pool->get(n1("i_meas"), 0);
-            k_face_measure<D><<<nblk(nface, 128), 128, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi, nface1,
-                                                                       fstart1, vid1, flo1, fhi1, meas1, vk, vc, meas,
-                                                                       bad);
+            if (hv_face_wave(D, m))
+                k_face_measure<D, true><<<(unsigned)std::min(std::max(nface, 1), 1 << 20), 64, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi,
+                                                                               nface1, fstart1, vid1, flo1, fhi1,
+                                                                               meas1, vk, vc, meas, bad);
+            else
+                k_face_measure<D, false><<<nblk(nface, 128), 128, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi,
+                                                                                  nface1, fstart1, vid1, flo1, fhi1,
+                                                                                  meas1, vk, vc, meas, bad);
             if (fi) {
                 HVCK(hipGetLastError());
                 double* cen = iget<double>(c, "i_cen", (size_t)nface * D);
@@ -909,8 +978,12 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey
                 if (!cen || !J1) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
                 k_face_center<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vc, cen);
                 HVCK(hipGetLastError());
-                k_face_int<D><<<nblk(nface, 128), 128, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi, cen, nface1,
-                                                                       fstart1, vid1, flo1, fhi1, J1, nf, vk, J, bad);
+                if (hv_face_wave(D, m))
+                    k_face_int<D, true><<<(unsigned)std::min(std::max(nface, 1), 1 << 20), 64, 0, c->stream>>>(
+                        g, m, nface, fstart, vid, flo, fhi, cen, nface1, fstart1, vid1, flo1, fhi1, J1, nf, vk, J, bad);
+                else
+                    k_face_int<D, false><<<nblk(nface, 128), 128, 0, c->stream>>>(
+                        g, m, nface, fstart, vid, flo, fhi, cen, nface1, fstart1, vid1, flo1, fhi1, J1, nf, vk, J, bad);
             }
         }
         HVCK(hipGetLastError());

@@ -3543,7 +3543,8 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     int64_t S = (nsamples + B - 1) / B;
     S = (S + 63) / 64 * 64;
     const int64_t n = B * S;
-    if (n / 64 > (int64_t)INT32_MAX) return hv_fail(c, HV_ERR_CAPACITY, "too many samples");
+    // one work-item per sample, and a dispatch holds at most 2^32 work-items
+    if (n > (1ll << 32) - 64) return hv_fail(c, HV_ERR_CAPACITY, "too many samples for one call (at most 2^32)");
     k_mc_points<D><<<(unsigned)(n / 64), 64, 0, c->stream>>>(g, G, S, seed, eps, ptr, pkey, ccnt, pcnt, box, pmask,
                                                              (int)c->nreal, porig, pshift, pkeys, pvals, hmask);
     HVCK(hipGetLastError());

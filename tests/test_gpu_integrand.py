@@ -97,3 +97,19 @@ def test_voronoi_geometry_integrand(ctx):
     assert abs(a.bulk_integral[:, 0].sum() - 0.5) < 1e-12   # affine: exact in 3D
     for i in (0, 99, 199):
         assert a.interface_integral[i].shape == (len(a.neighbors[i]), 2)
+
+
+def test_6d_50k_face_passes_cover_every_face(ctx):
+    # 6D N=50k: millions of facets and ridges, one wavefront each, walked grid-stride (a
+    # dispatch holds at most 2^32 work-items); every face must be measured: the volumes sum to
+    # the box, f = 1 gives the volumes, x0 integrates to ~1/2
+    xs = synth.uniform(6, 50_000, 4)
+    base, normal = synth.cube_planes(6)
+    ctx.set_points(xs)
+    ctx.set_boundary(base, normal)
+    m = ctx.build_mesh(precision="bf16")
+    I = ctx.integrate(m, "polytope", integrand=lambda p: np.stack([np.ones(len(p)), p[:, 0]], axis=1), nf=2)
+    m.free()
+    assert abs(I["volumes"].sum() - 1.0) < 1e-10
+    assert np.allclose(I["bulk"][:, 0], I["volumes"], rtol=1e-10, atol=0)
+    assert abs(I["bulk"][:, 1].sum() - 0.5) < 1e-3
