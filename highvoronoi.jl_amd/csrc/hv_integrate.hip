@@ -1318,6 +1318,9 @@ int hv_integrate_mc_impl(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t 
     std::vector<int64_t> hptr(c->N + 1);
     HVCK(hipMemcpyAsync(hptr.data(), ptr, sizeof(int64_t) * (c->N + 1), hipMemcpyDeviceToHost, c->stream));
     HVCK(hipStreamSynchronize(c->stream));
+    // one workgroup of 256 per cell, and a dispatch holds at most 2^32 work-items
+    if ((int64_t)g.nreal > (1ll << 24) - 1)
+        return hv_fail(c, HV_ERR_CAPACITY, "directional Monte Carlo: at most 2^24 - 1 cells per call");
     for (int64_t i = 0; i < g.nreal; ++i)
         if (hptr[i + 1] - hptr[i] > HV_MC_MAXK) return hv_fail(c, HV_ERR_CAPACITY, "a cell has more neighbours than the Monte-Carlo kernel holds");
     const double cvol = std::pow(M_PI, 0.5 * D) / std::tgamma(1.0 + 0.5 * D);
