@@ -183,7 +183,8 @@ int hv_integrate_polytope_f(hv_ctx* ctx, const hv_mesh* mesh, int32_t nf, hv_int
                             hv_integral** out);
 /* Directional Monte-Carlo volumes and areas, ndir rays per cell: the integrator=VI_MONTECARLO
  * path, Montecarlo_Integrator (src/mcintegrator.jl:29-226, src/cleanup_cell.jl:6-48), with
- * independent directions per cell drawn from a counter-based stream keyed by seed. */
+ * independent directions per cell drawn from a counter-based stream keyed by seed.  At most
+ * 2^24 - 1 cells per call (HV_ERR_CAPACITY beyond). */
 int hv_integrate_mc(hv_ctx* ctx, const hv_mesh* mesh, int64_t ndir, uint64_t seed, hv_integral** out);
 /* Point-sample Monte-Carlo volumes and areas: the Monte-Carlo integration of the cells
  * (Montecarlo_Integrator, src/mcintegrator.jl:29-226, with cleanup_cell, src/cleanup_cell.jl:6-48)
@@ -193,7 +194,9 @@ int hv_integrate_mc(hv_ctx* ctx, const hv_mesh* mesh, int64_t ndir, uint64_t see
  * eps = slab * (V_box / n_generators)^(1/d) of it and c2 within 2 eps (bisector of the two
  * nearest generators, or a boundary plane): area = V_box * (2 c1 - c2 / 2) / (n * eps) (the
  * first-order slab bias extrapolated away), generator interfaces averaged over both sides.
- * slab <= 0 selects 0.03.  nsamples is rounded up to whole strata (multiples of 64). */
+ * slab <= 0 selects 0.03.  nsamples is rounded up to whole strata (multiples of 64), at most
+ * 2^32 per call (HV_ERR_CAPACITY beyond).  Periodic boxes: samples are drawn in the box and
+ * counted in the original cell they fall in modulo the period. */
 int hv_integrate_mc_points(hv_ctx* ctx, const hv_mesh* mesh, int64_t nsamples, uint64_t seed, double slab,
                            hv_integral** out);
 /* Voronoi_Integral's parity surface (src/integral.jl:11-21): volumes[ncells]; for cell i
