@@ -3,6 +3,10 @@
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this,
 and only as the checker / CPU baseline.  Keys follow the reference's 1-based
 convention: generators 1..n, boundary plane p is n+p (src/extended.jl:27-34).
+
+polytope_integrand (the reference's integrand rule) is pinned by a hand-computed 2D cell,
+by f = 1 and by affine exactness in 2D and 3D; for non-affine f in d >= 4 its values are
+parity unpinned beyond that dimension-independent recursion.
 """
 from __future__ import annotations
 
