@@ -197,8 +197,8 @@ int hv_build_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
 
 int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, const int64_t* n,
                         const double* const* xs, const int32_t* nplanes, const double* const* base,
-                        const double* const* normal, const hv_opts* opts, int32_t nworkers,
-                        hv_mesh** out, int32_t* status) {
+                        const double* const* normal, const int16_t* const* bc, const hv_opts* opts,
+                        int32_t nworkers, hv_mesh** out, int32_t* status) {
     if (nmesh < 0 || (nmesh > 0 && (!d || !n || !xs || !nplanes || !base || !normal || !out || !status)))
         return HV_ERR_ARG;
     for (int k = 0; k < nmesh; ++k) {
@@ -215,7 +215,7 @@ int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, con
             const int k = next.fetch_add(1);
             if (k >= nmesh) break;
             int rc = hv_set_points(c, d[k], n[k], xs[k]);
-            if (rc == HV_OK) rc = hv_set_boundary(c, nplanes[k], base[k], normal[k], nullptr);
+            if (rc == HV_OK) rc = hv_set_boundary(c, nplanes[k], base[k], normal[k], bc ? bc[k] : nullptr);
             if (rc == HV_OK) rc = hv_build_mesh(c, opts, &out[k]);
             status[k] = rc;
         }

@@ -69,8 +69,8 @@ def lib():
     L.hv_set_boundary.argtypes = [vp, C.c_int, P(C.c_double), P(C.c_double), P(C.c_int16)]
     L.hv_build_mesh.argtypes = [vp, P(hv_opts), P(vp)]
     L.hv_build_mesh_batch.argtypes = [C.c_int, C.c_int32, P(C.c_int32), P(C.c_int64), P(P(C.c_double)),
-                                      P(C.c_int32), P(P(C.c_double)), P(P(C.c_double)), P(hv_opts), C.c_int32,
-                                      P(vp), P(C.c_int32)]
+                                      P(C.c_int32), P(P(C.c_double)), P(P(C.c_double)), P(P(C.c_int16)),
+                                      P(hv_opts), C.c_int32, P(vp), P(C.c_int32)]
     L.hv_mesh_counts.argtypes = [vp, P(C.c_int64), P(C.c_int32), P(C.c_int64)]
     L.hv_mesh_export.argtypes = [vp, P(C.c_int64), P(C.c_double)]
     L.hv_mesh_stats.argtypes = [vp, P(hv_stats_t)]
@@ -312,13 +312,16 @@ def _opts(precision: str = "fp64", leaf_size: int = 0, vertex_capacity: int = 0,
 
 def build_mesh_batch(point_sets, planes, precision: str = "fp64", nworkers: int = 0, device: int = 0):
     """Independent meshes in one hv_build_mesh_batch call.  point_sets[k]: (n_k, d_k) array;
-    planes[k]: (base, normal), each (nplanes_k, d_k).  Returns (meshes, status): a Mesh or
-    None per input and the per-mesh HV_* codes."""
+    planes[k]: (base, normal), each (nplanes_k, d_k), or (base, normal, bc) with the planes'
+    conditions (cuboid(...).arrays(): periodic sides are built with images).  Returns (meshes,
+    status): a Mesh or None per input and the per-mesh HV_* codes."""
     L = lib()
     m = len(point_sets)
     xs = [np.ascontiguousarray(x, dtype=np.float64) for x in point_sets]
-    bases = [np.ascontiguousarray(b, dtype=np.float64) for b, _ in planes]
-    normals = [np.ascontiguousarray(nr, dtype=np.float64) for _, nr in planes]
+    bases = [np.ascontiguousarray(pl[0], dtype=np.float64) for pl in planes]
+    normals = [np.ascontiguousarray(pl[1], dtype=np.float64) for pl in planes]
+    bcs = [np.ascontiguousarray(pl[2], dtype=np.int16) if len(pl) > 2 and pl[2] is not None else None
+           for pl in planes]
     d = np.array([x.shape[1] for x in xs], dtype=np.int32)
     n = np.array([x.shape[0] for x in xs], dtype=np.int64)
     npl = np.array([b.shape[0] for b in bases], dtype=np.int32)
@@ -326,11 +329,15 @@ def build_mesh_batch(point_sets, planes, precision: str = "fp64", nworkers: int 
     xp = (PD * max(1, m))(*[_dp(x) for x in xs])
     bp = (PD * max(1, m))(*[_dp(b) for b in bases])
     np_ = (PD * max(1, m))(*[_dp(nr) for nr in normals])
+    PS = C.POINTER(C.c_int16)
+    bcp = None
+    if any(b is not None for b in bcs):
+        bcp = (PS * max(1, m))(*[b.ctypes.data_as(PS) if b is not None else PS() for b in bcs])
     out = (C.c_void_p * max(1, m))()
     status = np.zeros(max(1, m), dtype=np.int32)
     o = _opts(precision)
     L.hv_build_mesh_batch(device, m, d.ctypes.data_as(C.POINTER(C.c_int32)), _i64(n), xp,
-                          npl.ctypes.data_as(C.POINTER(C.c_int32)), bp, np_, C.byref(o), nworkers, out,
+                          npl.ctypes.data_as(C.POINTER(C.c_int32)), bp, np_, bcp, C.byref(o), nworkers, out,
                           status.ctypes.data_as(C.POINTER(C.c_int32)))
     meshes = [Mesh(None, C.c_void_p(out[k]), int(d[k])) if out[k] else None for k in range(m)]
     return meshes, status[:m].copy()

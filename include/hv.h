@@ -132,8 +132,9 @@ int hv_mesh_free(hv_mesh* mesh);
 int hv_mesh_images(const hv_mesh* mesh, int64_t* n_ext, int64_t* orig, int8_t* shift);
 
 /* Many independent meshes in one call: mesh k has d[k] in 2..6, n[k] generators xs[k] (d[k] x n[k],
- * host memory) and the nplanes[k] boundary planes base[k], normal[k] (d[k] x nplanes[k] each,
- * Dirichlet), all built with *opts.  Replaces a host loop of VoronoiGeometry(xs_k, boundary_k)
+ * host memory) and the nplanes[k] boundary planes base[k], normal[k] (d[k] x nplanes[k] each)
+ * with the conditions bc[k] (nplanes[k] values as in hv_set_boundary; bc == NULL or
+ * bc[k] == NULL: Dirichlet), all built with *opts.  Replaces a host loop of VoronoiGeometry(xs_k, boundary_k)
  * calls (src/geometry.jl:156 -> voronoi, src/sysvoronoi.jl:21-82), one per mesh.  nworkers host
  * threads (0 = 8), each with its own context and HIP stream on device_ordinal, take meshes in
  * index order, so the level loops of different meshes overlap on the device.  out[k] receives
@@ -141,8 +142,8 @@ int hv_mesh_images(const hv_mesh* mesh, int64_t* n_ext, int64_t* orig, int8_t* s
  * HV_OK or the first failing mesh's code. */
 int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, const int64_t* n,
                         const double* const* xs, const int32_t* nplanes, const double* const* base,
-                        const double* const* normal, const hv_opts* opts, int32_t nworkers,
-                        hv_mesh** out, int32_t* status);
+                        const double* const* normal, const int16_t* const* bc, const hv_opts* opts,
+                        int32_t nworkers, hv_mesh** out, int32_t* status);
 
 /* Ray-level entry point: one walkray per input (src/raycast.jl:125-164 via
  * raycast_des2, src/raycast.jl:794-970).  For ray k: sig[k*(d+1) ...] is the sorted

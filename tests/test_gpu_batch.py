@@ -74,3 +74,28 @@ def test_batch_reports_failures_per_mesh():
 def test_empty_batch():
     out, status = build_mesh_batch([], [], precision="fp64")
     assert out == [] and len(status) == 0
+
+
+def test_batch_with_periodic_boxes_equals_single_builds():
+    # per-mesh conditions: periodic, partly periodic and Dirichlet boxes in one batch
+    from hvgpu.geometry import cuboid
+    cases = [(synth.uniform(3, 400, 31), cuboid(3)), (synth.uniform(2, 300, 32), cuboid(2, periodic=[2])),
+             (synth.uniform(4, 200, 33), cuboid(4, periodic=[])), (synth.uniform(5, 150, 34), cuboid(5, periodic=[1, 3]))]
+    pts = [x for x, _ in cases]
+    planes = [b.arrays() for _, b in cases]
+    out, status = build_mesh_batch(pts, planes, precision="bf16", nworkers=3)
+    assert list(status) == [0] * len(cases)
+    ctx = Context(0)
+    for m, (x, b) in zip(out, cases):
+        base, normal, bc = b.arrays()
+        ctx.set_points(x)
+        ctx.set_boundary(base, normal, bc)
+        m1 = ctx.build_mesh(precision="bf16")
+        k, c = m.export()
+        k1, c1 = m1.export()
+        assert np.array_equal(k, k1) and np.array_equal(c, c1)
+        o, sh = m.images()
+        o1, sh1 = m1.images()
+        assert np.array_equal(o, o1) and np.array_equal(sh, sh1)
+        m.free()
+        m1.free()
