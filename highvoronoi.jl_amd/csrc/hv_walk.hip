@@ -2813,6 +2813,24 @@ __global__ __launch_bounds__(256) void k_periodic_reach(const int* __restrict__ 
     if ((threadIdx.x & 63) == 0 && rr > 0.0) atomicMax(out + 1, (unsigned long long)__double_as_longlong(rr));
 }
 
+// every original must own a vertex: mark the generators below nreal found in the keys
+template <int D>
+__global__ __launch_bounds__(256) void k_mark_cells(const int* __restrict__ ids, int nv, const int* __restrict__ vkey,
+                                                    int nreal, int* __restrict__ mark) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;
+    const int* key = vkey + (int64_t)ids[t] * (D + 1);
+#pragma unroll
+    for (int a = 0; a <= D; ++a)
+        if (key[a] < nreal) mark[key[a]] = 1;
+}
+
+__global__ __launch_bounds__(256) void k_count_unmarked(const int* __restrict__ mark, int n,
+                                                        unsigned long long* __restrict__ cnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    hv_wave_count(cnt, t < n && mark[t] == 0);
+}
+
 template <int D>
 static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
     const int64_t n = c->nreal;
@@ -2945,6 +2963,28 @@ static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
         std::memcpy(&need, &hb[0], sizeof need);
         std::memcpy(&c->rmax, &hb[1], sizeof need);
         if (need <= delta * (1.0 - 1e-9)) {
+            // a cell whose own seed was dropped is reached through a walked edge shared with
+            // another original's cell; one with no such neighbour would be missing: refuse
+            int* mark = pget<int>(c, "pmark", (size_t)n);
+            unsigned long long* miss = pget<unsigned long long>(c, "pmiss", 1);
+            if (!mark || !miss) {
+                hv_mesh_free(m);
+                return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+            }
+            HVCK(hipMemsetAsync(mark, 0, sizeof(int) * n, c->stream));
+            HVCK(hipMemsetAsync(miss, 0, sizeof(unsigned long long), c->stream));
+            k_mark_cells<D><<<nblk(m->nv, 256), 256, 0, c->stream>>>((const int*)m->ids.p, (int)m->nv,
+                                                                     (const int*)m->key.p, (int)n, mark);
+            HVCK(hipGetLastError());
+            k_count_unmarked<<<nblk(n, 256), 256, 0, c->stream>>>(mark, (int)n, miss);
+            HVCK(hipGetLastError());
+            unsigned long long hmiss = 0;
+            HVCK(hipMemcpyAsync(&hmiss, miss, sizeof hmiss, hipMemcpyDeviceToHost, c->stream));
+            HVCK(hipStreamSynchronize(c->stream));
+            if (hmiss) {
+                hv_mesh_free(m);
+                return hv_fail(c, HV_ERR_DEGENERATE, "a periodic cell was not reached by the walk");
+            }
             m->nreal = n;
             m->gen = c->gen;
             m->delta = delta;
