@@ -14,6 +14,7 @@ DevPool::~DevPool() {
 }
 
 void* DevPool::get(const std::string& name, size_t bytes, bool keep_contents, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(mu);
     DBuf& b = bufs[name];
     if (bytes <= b.n && b.p) return b.p;
     if (bytes == 0) return b.p;
@@ -40,6 +41,7 @@ void* DevPool::get(const std::string& name, size_t bytes, bool keep_contents, hi
 }
 
 DBuf DevPool::take(const std::string& name) {
+    std::lock_guard<std::mutex> lk(mu);
     DBuf b;
     auto it = bufs.find(name);
     if (it != bufs.end()) {
@@ -51,6 +53,7 @@ DBuf DevPool::take(const std::string& name) {
 
 void DevPool::give(const std::string& name, DBuf b) {
     if (!b.p) return;
+    std::lock_guard<std::mutex> lk(mu);
     DBuf& cur = bufs[name];
     if (!cur.p) {
         cur = b;
@@ -60,6 +63,78 @@ void DevPool::give(const std::string& name, DBuf b) {
     } else {
         (void)hipFree(b.p);
     }
+}
+
+DevArr::~DevArr() {
+    if (!p) return;
+    int cur = 0;
+    const bool had = hipGetDevice(&cur) == hipSuccess;
+    (void)hipSetDevice(device);
+    (void)hipFree(p);
+    if (had) (void)hipSetDevice(cur);
+}
+
+double* hv_points_for_write(hv_ctx* c, size_t bytes) {
+    if (c->xw && c->xw.use_count() == 1 && c->xw->n >= bytes) return (double*)c->xw->p;
+    auto a = std::make_shared<DevArr>();
+    a->device = c->device;
+    if (hipMalloc(&a->p, bytes ? bytes : 8) != hipSuccess) {
+        (void)hipGetLastError();
+        a->p = nullptr;
+        return nullptr;
+    }
+    a->n = bytes;
+    c->xw = std::move(a);
+    return (double*)c->xw->p;
+}
+
+void hv_mesh_snapshot(const hv_ctx* c, hv_mesh* m) {
+    m->xw = c->xw;
+    m->P = c->P;
+    m->pbase = c->pbase;
+    m->pnormal = c->pnormal;
+    m->pbc = c->pbc;
+    m->nreal = c->nreal;
+    m->extended = c->extended;
+    m->delta = c->delta;
+    m->rmax = c->rmax;
+    m->img_orig = c->img_orig;
+    m->img_shift = c->img_shift;
+}
+
+hv_bind_mesh::hv_bind_mesh(hv_ctx* ctx, const hv_mesh* m)
+    : c(ctx), D(ctx->D), N(ctx->N), nreal(ctx->nreal), xw(ctx->xw), P(ctx->P), pbase(ctx->pbase),
+      pnormal(ctx->pnormal), pbc(ctx->pbc), extended(ctx->extended), delta(ctx->delta), rmax(ctx->rmax),
+      img_orig(ctx->img_orig), img_shift(ctx->img_shift) {
+    c->D = m->D;
+    c->N = m->N;
+    c->nreal = m->nreal;
+    c->xw = m->xw;
+    c->P = m->P;
+    c->pbase = m->pbase;
+    c->pnormal = m->pnormal;
+    c->pbc = m->pbc;
+    c->extended = m->extended;
+    c->delta = m->delta;
+    c->rmax = m->rmax;
+    c->img_orig = m->img_orig;
+    c->img_shift = m->img_shift;
+}
+
+hv_bind_mesh::~hv_bind_mesh() {
+    c->D = D;
+    c->N = N;
+    c->nreal = nreal;
+    c->xw = std::move(xw);
+    c->P = P;
+    c->pbase = std::move(pbase);
+    c->pnormal = std::move(pnormal);
+    c->pbc = std::move(pbc);
+    c->extended = extended;
+    c->delta = delta;
+    c->rmax = rmax;
+    c->img_orig = std::move(img_orig);
+    c->img_shift = std::move(img_shift);
 }
 
 int hv_fail(hv_ctx* c, int code, const std::string& msg) {
@@ -110,6 +185,8 @@ int hv_destroy(hv_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < 4; ++i) (void)hipEventDestroy(c->ev[i]);
     (void)hipStreamDestroy(c->stream);
+    c->xw.reset();
+    c->xorig.reset();
     c->pool.reset();
     delete c;
     return HV_OK;
@@ -123,7 +200,7 @@ static int set_points_common(hv_ctx* c, int d, int64_t n, const double* xs, hipM
     if (n <= d) return hv_fail(c, HV_ERR_FEW_POINTS, "There are not enough points to create a Voronoi tessellation");
     if (n >= (1ll << 30)) return hv_fail(c, HV_ERR_UNSUPPORTED, "too many generators for 32-bit keys");
     (void)hipSetDevice(c->device);
-    void* p = c->pool->get("xin", sizeof(double) * d * n, false, c->stream);
+    void* p = hv_points_for_write(c, sizeof(double) * d * n);
     if (!p) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
     if (hipMemcpyAsync(p, xs, sizeof(double) * d * n, kind, c->stream) != hipSuccess)
         return hv_fail(c, HV_ERR_DEVICE, "copy of the points failed");
@@ -138,27 +215,23 @@ static int set_points_common(hv_ctx* c, int d, int64_t n, const double* xs, hipM
     c->N = n;
     c->nreal = n;
     c->extended = false;
-    c->img_orig.clear();
-    c->img_shift.clear();
-    ++c->gen;
+    c->xorig.reset();
+    c->img_orig.reset();
+    c->img_shift.reset();
+    c->delta = c->rmax = 0.0;
     return HV_OK;
 }
 
 // back from the extended set of a periodic build to the originals
 static int restore_originals(hv_ctx* c) {
     if (!c->extended) return HV_OK;
-    const size_t bytes = sizeof(double) * c->D * c->nreal;
-    const void* src = c->pool->get("xorig", 0);
-    void* dst = c->pool->get("xin", bytes, false, c->stream);
-    if (!src || !dst) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
-    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess)
-        return hv_fail(c, HV_ERR_DEVICE, "copy of the points failed");
+    if (!c->xorig) return hv_fail(c, HV_ERR_STATE, "the originals of the periodic set are gone");
+    c->xw = std::move(c->xorig);
     c->N = c->nreal;
     c->extended = false;
-    c->img_orig.clear();
-    c->img_shift.clear();
-    ++c->gen;
+    c->img_orig.reset();
+    c->img_shift.reset();
+    c->delta = c->rmax = 0.0;
     return HV_OK;
 }
 
@@ -192,7 +265,24 @@ int hv_build_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     *out = nullptr;
     if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
     (void)hipSetDevice(c->device);
-    HV_DISPATCH(c->D, hv_build_mesh_impl, c, o, out)
+    HV_DISPATCH(c->D, hv_build_mesh_impl, c, o, out, nullptr)
+}
+
+int hv_build_mesh_cells(hv_ctx* c, const hv_opts* o, int64_t ncells, const int64_t* cells, hv_mesh** out) {
+    if (!c || !out || ncells < 0 || (ncells > 0 && !cells)) return hv_fail(c, HV_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    const int64_t n = c->nreal;   // the caller's generators (originals of a periodic box)
+    std::vector<int> list;
+    list.reserve((size_t)ncells);
+    for (int64_t k = 0; k < ncells; ++k) {
+        if (cells[k] < 1 || cells[k] > n) return hv_fail(c, HV_ERR_ARG, "cell index out of range (1..n)");
+        list.push_back((int)(cells[k] - 1));
+    }
+    std::sort(list.begin(), list.end());
+    list.erase(std::unique(list.begin(), list.end()), list.end());
+    (void)hipSetDevice(c->device);
+    HV_DISPATCH(c->D, hv_build_mesh_impl, c, o, out, &list)
 }
 
 int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, const int64_t* n,
@@ -217,6 +307,8 @@ int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, con
             int rc = hv_set_points(c, d[k], n[k], xs[k]);
             if (rc == HV_OK) rc = hv_set_boundary(c, nplanes[k], base[k], normal[k], bc ? bc[k] : nullptr);
             if (rc == HV_OK) rc = hv_build_mesh(c, opts, &out[k]);
+            // the mesh owns its vertex tables outright: the worker's pool goes with the worker
+            if (rc == HV_OK) out[k]->pool.reset();
             status[k] = rc;
         }
         hv_destroy(c);
@@ -252,11 +344,11 @@ int hv_mesh_export(const hv_mesh* m, int64_t* sig, double* coords) {
 int hv_mesh_images(const hv_mesh* m, int64_t* n_ext, int64_t* orig, int8_t* shift) {
     if (!m || !n_ext) return HV_ERR_ARG;
     *n_ext = m->N;
-    const bool per = !m->img_orig.empty();
+    const bool per = m->img_orig && m->img_shift;
     for (int64_t i = 0; i < m->N; ++i) {
-        if (orig) orig[i] = (per ? (int64_t)m->img_orig[i] : i) + 1;
+        if (orig) orig[i] = (per ? (int64_t)(*m->img_orig)[i] : i) + 1;
         if (shift)
-            for (int k = 0; k < m->D; ++k) shift[i * m->D + k] = per ? m->img_shift[i * m->D + k] : 0;
+            for (int k = 0; k < m->D; ++k) shift[i * m->D + k] = per ? (*m->img_shift)[i * m->D + k] : 0;
     }
     return HV_OK;
 }
@@ -270,9 +362,14 @@ int hv_mesh_stats(const hv_mesh* m, hv_stats_t* out) {
 int hv_mesh_free(hv_mesh* m) {
     if (!m) return HV_ERR_ARG;
     (void)hipSetDevice(m->device);
-    m->pool->give("vkey", m->key);
-    m->pool->give("vx", m->x);
-    m->pool->give("ids", m->ids);
+    if (m->pool) {
+        m->pool->give("vkey", m->key);
+        m->pool->give("vx", m->x);
+        m->pool->give("ids", m->ids);
+    } else {
+        for (DBuf* b : {&m->key, &m->x, &m->ids})
+            if (b->p) (void)hipFree(b->p);
+    }
     delete m;
     return HV_OK;
 }
@@ -307,19 +404,21 @@ int hv_inrange_batch(hv_ctx* c, int64_t nq, const double* q, const double* radiu
 int hv_integrate_polytope(hv_ctx* c, const hv_mesh* mesh, hv_integral** out) {
     if (!c || !mesh || !out) return hv_fail(c, HV_ERR_ARG, "null argument");
     *out = nullptr;
-    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    if (mesh->device != c->device) return hv_fail(c, HV_ERR_STATE, "the mesh lives on another device");
     (void)hipSetDevice(c->device);
-    HV_DISPATCH(c->D, hv_integrate_polytope_impl, c, mesh, out)
+    hv_bind_mesh bind(c, mesh);
+    HV_DISPATCH(mesh->D, hv_integrate_polytope_impl, c, mesh, out)
 }
 
 int hv_integrate_polytope_f(hv_ctx* c, const hv_mesh* mesh, int32_t nf, hv_integrand_fn f, void* user,
                             hv_integral** out) {
     if (!c || !mesh || !out) return hv_fail(c, HV_ERR_ARG, "null argument");
     *out = nullptr;
-    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
     if (nf < 1 || !f) return hv_fail(c, HV_ERR_ARG, "integrand: nf >= 1 and a function");
+    if (mesh->device != c->device) return hv_fail(c, HV_ERR_STATE, "the mesh lives on another device");
     (void)hipSetDevice(c->device);
-    HV_DISPATCH(c->D, hv_integrate_polytope_f_impl, c, mesh, nf, f, user, out)
+    hv_bind_mesh bind(c, mesh);
+    HV_DISPATCH(mesh->D, hv_integrate_polytope_f_impl, c, mesh, nf, f, user, out)
 }
 
 int hv_integral_export_f(const hv_integral* I, int32_t* nf, double* bulk, double* iface) {
@@ -329,18 +428,22 @@ int hv_integral_export_f(const hv_integral* I, int32_t* nf, double* bulk, double
 int hv_integrate_mc(hv_ctx* c, const hv_mesh* mesh, int64_t ndir, uint64_t seed, hv_integral** out) {
     if (!c || !mesh || !out) return hv_fail(c, HV_ERR_ARG, "null argument");
     *out = nullptr;
-    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    if (mesh->device != c->device) return hv_fail(c, HV_ERR_STATE, "the mesh lives on another device");
+    if (mesh->cells) return hv_fail(c, HV_ERR_UNSUPPORTED, "the Monte-Carlo integrators take a mesh of all cells");
     (void)hipSetDevice(c->device);
-    HV_DISPATCH(c->D, hv_integrate_mc_impl, c, mesh, ndir, seed, out)
+    hv_bind_mesh bind(c, mesh);
+    HV_DISPATCH(mesh->D, hv_integrate_mc_impl, c, mesh, ndir, seed, out)
 }
 
 int hv_integrate_mc_points(hv_ctx* c, const hv_mesh* mesh, int64_t nsamples, uint64_t seed, double slab,
                            hv_integral** out) {
     if (!c || !mesh || !out) return hv_fail(c, HV_ERR_ARG, "null argument");
     *out = nullptr;
-    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    if (mesh->device != c->device) return hv_fail(c, HV_ERR_STATE, "the mesh lives on another device");
+    if (mesh->cells) return hv_fail(c, HV_ERR_UNSUPPORTED, "the Monte-Carlo integrators take a mesh of all cells");
     (void)hipSetDevice(c->device);
-    HV_DISPATCH(c->D, hv_integrate_mc_points_impl, c, mesh, nsamples, seed, slab, out)
+    hv_bind_mesh bind(c, mesh);
+    HV_DISPATCH(mesh->D, hv_integrate_mc_points_impl, c, mesh, nsamples, seed, slab, out)
 }
 
 int hv_integral_counts(const hv_integral* I, int64_t* ncells, int64_t* nnbr) {

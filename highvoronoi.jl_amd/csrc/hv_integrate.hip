@@ -44,6 +44,8 @@ static T* iget(hv_ctx* c, const char* name, size_t count) {
 }
 
 constexpr uint64_t HV_F21 = 0x1FFFFFull;   // id field; all ones = empty slot
+// face incidences one pass of a chunk may hold (a chunk of cells at a time beyond)
+constexpr int64_t HV_INC_BUDGET = 1600000000ll;
 
 __device__ __forceinline__ void hv_fkey_put(uint64_t& lo, uint64_t& hi, int s, uint64_t id) {
     if (s < 3) lo = (lo & ~(HV_F21 << (21 * s))) | (id << (21 * s));
@@ -103,10 +105,12 @@ __global__ void k_int_permute(const int* __restrict__ perm, int nv, const int* _
     for (int k = 0; k < D; ++k) vc2[(int64_t)v * D + k] = vc[(int64_t)s * D + k];
 }
 
-// (face, vertex) incidences of pass m: the m-subsets of each key holding a generator
+// (face, vertex) incidences of pass m: the m-subsets of each key holding a cell of the chunk
+// (cmask over the ids; ids below N are generators); the others are sent past the end
 template <int D>
-__global__ void k_face_emit(const int* __restrict__ vk, int nv, int m, int N, int C, uint64_t* __restrict__ lo,
-                            uint64_t* __restrict__ hi, int* __restrict__ idx, unsigned long long* __restrict__ nsent) {
+__global__ void k_face_emit(const int* __restrict__ vk, int nv, int m, int N, const uint8_t* __restrict__ cmask, int C,
+                            uint64_t* __restrict__ lo, uint64_t* __restrict__ hi, int* __restrict__ idx,
+                            unsigned long long* __restrict__ nsent) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     const int* k = vk + (int64_t)v * (D + 1);
@@ -120,7 +124,7 @@ __global__ void k_face_emit(const int* __restrict__ vk, int nv, int m, int N, in
         for (int j = 0; j <= D; ++j) {
             if ((mask >> j) & 1) {
                 const int id = k[j];
-                real = real || id < N;
+                real = real || (id < N && cmask[id] != 0);
                 hv_fkey_put(l, h, s, (uint64_t)id);
                 ++s;
             }
@@ -584,22 +588,89 @@ __global__ void k_cell_bulk(IntGeo g, const uint64_t* __restrict__ pkey, const i
 }
 
 // facets {a, b} -> (cell, neighbour, area) rows for both sides
+// rows (cell, neighbour) of the facets: one per side that is a cell of the chunk
 __global__ void k_facet_pairs(int nface, const uint64_t* __restrict__ flo, const double* __restrict__ meas, int N,
-                              uint64_t* __restrict__ pkey, double* __restrict__ parea, int* __restrict__ pface,
-                              unsigned long long* __restrict__ np) {
+                              const uint8_t* __restrict__ cmask, uint64_t* __restrict__ pkey, double* __restrict__ parea,
+                              int* __restrict__ pface, unsigned long long* __restrict__ np) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nface) return;
     const int a = hv_fkey_get(flo[f], 0, 0), b = hv_fkey_get(flo[f], 0, 1);
-    const int n = b < N ? 2 : 1;
-    const unsigned long long o = atomicAdd(np, (unsigned long long)n);
-    pkey[o] = ((uint64_t)a << 32) | (uint64_t)b;
-    parea[o] = meas[f];
-    pface[o] = f;
-    if (n == 2) {
-        pkey[o + 1] = ((uint64_t)b << 32) | (uint64_t)a;
-        parea[o + 1] = meas[f];
-        pface[o + 1] = f;
+    const bool ra = a < N && cmask[a] != 0, rb = b < N && cmask[b] != 0;
+    const int n = (ra ? 1 : 0) + (rb ? 1 : 0);
+    if (!n) return;
+    unsigned long long o = atomicAdd(np, (unsigned long long)n);
+    if (ra) {
+        pkey[o] = ((uint64_t)a << 32) | (uint64_t)b;
+        parea[o] = meas[f];
+        pface[o] = f;
+        ++o;
     }
+    if (rb) {
+        pkey[o] = ((uint64_t)b << 32) | (uint64_t)a;
+        parea[o] = meas[f];
+        pface[o] = f;
+    }
+}
+
+__global__ void k_set_mask(const int* __restrict__ ids, int n, uint8_t* __restrict__ mask) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) mask[ids[t]] = 1;
+}
+
+// vertices holding a cell of the chunk
+template <int D>
+__global__ void k_vert_in_chunk(const int* __restrict__ vk, int nv, int N, const uint8_t* __restrict__ cmask,
+                                uint8_t* __restrict__ flag, int* __restrict__ iota) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    bool in = false;
+#pragma unroll
+    for (int j = 0; j <= D; ++j) {
+        const int id = vk[(int64_t)v * (D + 1) + j];
+        in = in || (id < N && cmask[id] != 0);
+    }
+    flag[v] = in ? 1 : 0;
+    iota[v] = v;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_count_in_chunk(const int* __restrict__ vk, int nv, int N,
+                                                        const uint8_t* __restrict__ cmask, unsigned long long* __restrict__ cnt) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = false;
+    if (v < nv)
+#pragma unroll
+        for (int j = 0; j <= D; ++j) {
+            const int id = vk[(int64_t)v * (D + 1) + j];
+            in = in || (id < N && cmask[id] != 0);
+        }
+    hv_wave_count(cnt, in);
+}
+
+template <int D>
+__global__ void k_int_subset(const int* __restrict__ sel, int n, const int* __restrict__ vk, const double* __restrict__ vc,
+                             const double* __restrict__ fv, int nf, int* __restrict__ vk2, double* __restrict__ vc2,
+                             double* __restrict__ fv2) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int v = sel[t];
+#pragma unroll
+    for (int j = 0; j <= D; ++j) vk2[(int64_t)t * (D + 1) + j] = vk[(int64_t)v * (D + 1) + j];
+#pragma unroll
+    for (int k = 0; k < D; ++k) vc2[(int64_t)t * D + k] = vc[(int64_t)v * D + k];
+    if (fv)
+        for (int f = 0; f < nf; ++f) fv2[(int64_t)t * nf + f] = fv[(int64_t)v * nf + f];
+}
+
+// row payloads in the sorted order of the rows
+__global__ void k_gather_rows(const int* __restrict__ perm, int64_t n, const double* __restrict__ ra,
+                              double* __restrict__ parea, const double* __restrict__ ri, double* __restrict__ rint, int nf) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int s = perm[t];
+    parea[t] = ra[s];
+    if (ri)
+        for (int f = 0; f < nf; ++f) rint[t * nf + f] = ri[(int64_t)s * nf + f];
 }
 
 template <int D>
@@ -784,6 +855,59 @@ __global__ void k_mc_sym(IntGeo g, const int64_t* __restrict__ ptr, const uint64
     area[lo] = nw;
 }
 
+// ---- dense ids -----------------------------------------------------------------------------
+// Face keys pack 21 bits per id.  The face passes run on a dense numbering of the ids that
+// occur in the mesh's keys (order-preserving, so sorted keys stay sorted): with periodic images
+// the working set can exceed 2^21 generators while far fewer of them touch a walked cell.
+template <int D>
+__global__ void k_mark_used(const int* __restrict__ vk, int nv, int* __restrict__ used) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+#pragma unroll
+    for (int j = 0; j <= D; ++j) used[vk[(int64_t)v * (D + 1) + j]] = 1;
+}
+
+// dense[i] (exclusive prefix sum of used) -> gid[dense] = i for the used ids
+__global__ void k_dense_inverse(const int* __restrict__ used, const int* __restrict__ dense, int n, int* __restrict__ gid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && used[i]) gid[dense[i]] = i;
+}
+
+template <int D>
+__global__ void k_remap_keys(int* __restrict__ vk, int nv, const int* __restrict__ dense) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+#pragma unroll
+    for (int j = 0; j <= D; ++j) vk[(int64_t)v * (D + 1) + j] = dense[vk[(int64_t)v * (D + 1) + j]];
+}
+
+// generator rows and planes of the dense numbering: dense ids below ngen are generators
+template <int D>
+__global__ void k_dense_geo(const int* __restrict__ gid, int nused, int ngen, int N, const double* __restrict__ xo,
+                            const double* __restrict__ pn, const double* __restrict__ ph, double* __restrict__ xd,
+                            double* __restrict__ pnd, double* __restrict__ phd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nused) return;
+    const int o = gid[i];
+    if (i < ngen) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) xd[(int64_t)i * D + k] = xo[(int64_t)o * D + k];
+    } else {
+        const int p = o - N, q = i - ngen;
+#pragma unroll
+        for (int k = 0; k < D; ++k) pnd[q * D + k] = pn[p * D + k];
+        phd[q] = ph[p];
+    }
+}
+
+// rows (cell << 32 | neighbour) back to the caller's ids; the map preserves the order
+__global__ void k_rows_to_ids(uint64_t* __restrict__ rows, int64_t n, const int* __restrict__ gid) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint64_t r = rows[t];
+    rows[t] = ((uint64_t)gid[(int)(r >> 32)] << 32) | (uint64_t)(uint32_t)gid[(int)(r & 0xffffffffull)];
+}
+
 // ---- host driver -------------------------------------------------------------------------
 
 struct hv_integral {
@@ -823,17 +947,21 @@ static int eval_integrand(hv_ctx* c, const Integrand* fi, int D, const double* d
 // fi (optional): also the integrals of the rows' interfaces (*rint, npair x nf) by the
 // reference's rule (see k_face_int)
 template <int D>
-static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey, double*& parea, int64_t& npair,
+static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t*& pkey, double*& parea, int64_t& npair,
                       const Integrand* fi = nullptr, double** rint = nullptr) {
-    const int N = (int)c->N;
     const int nv = (int)mesh->nv;
-    if ((int64_t)N + c->P >= (int64_t)HV_F21) return hv_fail(c, HV_ERR_UNSUPPORTED, "integration packs ids in 21 bits");
     int* vk = iget<int>(c, "i_vk", (size_t)nv * (D + 1));
     double* vc = iget<double>(c, "i_vc", (size_t)nv * D);
     int* bad = iget<int>(c, "i_bad", 1);
     unsigned long long* cnt = iget<unsigned long long>(c, "i_cnt", 2);
     if (!vk || !vc || !bad || !cnt) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integration)");
     HVCK(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+    IntGeo g = g0;          // the face passes' geometry, in dense ids
+    int* gid = nullptr;     // dense id -> the caller's id
+    int* used = nullptr;    // caller's id -> occurs in a key
+    int* dense = nullptr;   // caller's id -> dense id (exclusive prefix sum of used)
+    const int nid = g0.N + g0.P;
+    int ngen = 0, nused = 0;
     {
         // the live vertices in a numbering fixed by the vertex set (not by the walk's
         // insertion order), so that apexes and summation orders are reproducible
@@ -851,6 +979,49 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey
         k_int_gather<D><<<nblk(nv, 256), 256, 0, c->stream>>>((const int*)mesh->ids.p, nv, (const int*)mesh->key.p,
                                                               (const double*)mesh->x.p, vk0, vc0);
         HVCK(hipGetLastError());
+        // dense ids of the generators and planes that occur in the keys
+        used = iget<int>(c, "i_used", (size_t)nid + 1);
+        dense = iget<int>(c, "i_dense", (size_t)nid + 1);
+        if (!used || !dense) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integration)");
+        HVCK(hipMemsetAsync(used, 0, sizeof(int) * ((size_t)nid + 1), c->stream));
+        if (nv) {
+            k_mark_used<D><<<nblk(nv, 256), 256, 0, c->stream>>>(vk0, nv, used);
+            HVCK(hipGetLastError());
+        }
+        size_t tbs = 0;
+        HVCK(hipcub::DeviceScan::ExclusiveSum(nullptr, tbs, used, dense, nid + 1, c->stream));
+        void* tmps = c->pool->get("i_stmp", tbs, false, c->stream);
+        if (!tmps) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (scan)");
+        HVCK(hipcub::DeviceScan::ExclusiveSum(tmps, tbs, used, dense, nid + 1, c->stream));
+        int hcnt[2] = {0, 0};   // used ids below N (generators), used ids in all
+        HVCK(hipMemcpyAsync(&hcnt[0], dense + g0.N, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipMemcpyAsync(&hcnt[1], dense + nid, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        ngen = hcnt[0];
+        nused = hcnt[1];
+        if (nused >= (int)HV_F21)
+            return hv_fail(c, HV_ERR_UNSUPPORTED, "more than 2^21 - 2 generators and planes occur in the mesh (21-bit face keys)");
+        gid = iget<int>(c, "i_gid", (size_t)std::max(1, nused));
+        double* xd = iget<double>(c, "i_xd", (size_t)std::max(1, ngen) * D);
+        double* pnd = iget<double>(c, "i_pnd", (size_t)std::max(1, nused - ngen) * D);
+        double* phd = iget<double>(c, "i_phd", (size_t)std::max(1, nused - ngen));
+        if (!gid || !xd || !pnd || !phd) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integration)");
+        k_dense_inverse<<<nblk(nid, 256), 256, 0, c->stream>>>(used, dense, nid, gid);
+        HVCK(hipGetLastError());
+        if (nv) {
+            k_remap_keys<D><<<nblk(nv, 256), 256, 0, c->stream>>>(vk0, nv, dense);
+            HVCK(hipGetLastError());
+        }
+        if (nused) {
+            k_dense_geo<D><<<nblk(nused, 256), 256, 0, c->stream>>>(gid, nused, ngen, g0.N, g0.xo, g0.pn, g0.ph, xd, pnd, phd);
+            HVCK(hipGetLastError());
+        }
+        g.N = ngen;
+        g.P = nused - ngen;
+        g.nreal = g0.nreal;
+        g.xo = xd;
+        g.pn = pnd;
+        g.ph = phd;
         k_vkey_words<D><<<nblk(nv, 256), 256, 0, c->stream>>>(vk0, nv, w0, w1, w2, pa);
         HVCK(hipGetLastError());
         size_t tb = 0;
@@ -878,168 +1049,345 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, IntGeo& g, uint64_t*& pkey
         int rc = eval_integrand(c, fi, D, vc, nv, fv);
         if (rc) return rc;
     }
-    // per pass: sorted incidences (vid), face starts, keys, measures; "1" = pass m + 1
-    int nface1 = 0;
-    for (int m = D; m >= 2; --m) {
-        const int C = (int)binom(D + 1, m);
-        const int64_t tot = (int64_t)nv * C;
-        if (tot >= (1ll << 31)) return hv_fail(c, HV_ERR_CAPACITY, "too many face incidences for one pass");
-        const char* sfx = (m & 1) ? "a" : "b";
-        auto nm = [&](const char* s) { return std::string(s) + sfx; };
-        uint64_t* lo = iget<uint64_t>(c, "i_lo", tot);
-        uint64_t* hi = iget<uint64_t>(c, "i_hi", tot);
-        uint64_t* lo2 = iget<uint64_t>(c, "i_lo2", tot);
-        uint64_t* hi2 = iget<uint64_t>(c, "i_hi2", tot);
-        int* pa = iget<int>(c, "i_pa", tot);
-        int* pb = iget<int>(c, "i_pb", tot);
-        uint8_t* head = iget<uint8_t>(c, "i_head", tot);
-        if (!lo || !hi || !lo2 || !hi2 || !pa || !pb || !head) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
-        HVCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
-        // faces with an original generator (with periodic images, the others are incomplete)
-        k_face_emit<D><<<nblk(nv, 256), 256, 0, c->stream>>>(vk, nv, m, g.nreal, C, lo, hi, pa, cnt);
-        HVCK(hipGetLastError());
-        // two stable radix passes: by lo, then by hi -> (hi, lo) order
-        size_t tb = 0;
-        HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, lo, lo2, pa, pb, (int)tot, 0, 64, c->stream));
-        void* tmp = c->pool->get("i_tmp", tb, false, c->stream);
-        if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
-        HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, lo, lo2, pa, pb, (int)tot, 0, 64, c->stream));
-        int* perm = pb;
-        if (m > 3) {
-            k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(hi, pb, tot, hi2);
-            HVCK(hipGetLastError());
-            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, hi2, hi, pb, pa, (int)tot, 0, 64, c->stream));
-            perm = pa;
-            k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(lo, perm, tot, lo2);
-            HVCK(hipGetLastError());
-        } else {
-            k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(hi, perm, tot, hi2);
-            HVCK(hipGetLastError());
-            HVCK(hipMemcpyAsync(hi, hi2, sizeof(uint64_t) * tot, hipMemcpyDeviceToDevice, c->stream));
-        }
-        // lo2 / hi: sorted keys; perm: sorted incidence numbers
-        unsigned long long hsent = 0;
-        HVCK(hipMemcpyAsync(&hsent, cnt, sizeof(hsent), hipMemcpyDeviceToHost, c->stream));
+    // Cells to integrate (dense ids), taken a chunk at a time.  A chunk's face passes hold the
+    // vertices of its cells only: a face with one of the chunk's cells has all its vertices
+    // there, and so do its sub-faces.  A facet of cells in two chunks is measured in both from
+    // the same vertex list in the same order (the canonical vertex order, restricted), so the
+    // two values are identical.  One chunk unless a pass would exceed HV_INC_BUDGET incidences.
+    std::vector<int> hcell;
+    {
+        std::vector<int> hd((size_t)g0.nreal + 1);
+        HVCK(hipMemcpyAsync(hd.data(), dense, sizeof(int) * ((size_t)g0.nreal + 1), hipMemcpyDeviceToHost, c->stream));
         HVCK(hipStreamSynchronize(c->stream));
-        const int64_t nvalid = tot - (int64_t)hsent;
-        int* vid = iget<int>(c, nm("i_vid").c_str(), nvalid);
-        int* fstart = iget<int>(c, nm("i_fst").c_str(), nvalid + 1);
-        int* iota = iget<int>(c, "i_iota", nvalid);
-        if (!vid || !fstart || !iota) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
-        k_face_heads<<<nblk(nvalid, 256), 256, 0, c->stream>>>(lo2, hi, perm, nvalid, C, vid, head);
-        HVCK(hipGetLastError());
-        k_iota64<<<nblk(nvalid, 256), 256, 0, c->stream>>>(iota, nvalid);
-        HVCK(hipGetLastError());
-        size_t tb2 = 0;
-        HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb2, iota, head, fstart, cnt + 1, (int)nvalid, c->stream));
-        void* tmp2 = c->pool->get("i_tmp2", tb2, false, c->stream);
-        if (!tmp2) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
-        HVCK(hipcub::DeviceSelect::Flagged(tmp2, tb2, iota, head, fstart, cnt + 1, (int)nvalid, c->stream));
-        unsigned long long hnf = 0;
-        HVCK(hipMemcpyAsync(&hnf, cnt + 1, sizeof(hnf), hipMemcpyDeviceToHost, c->stream));
-        HVCK(hipStreamSynchronize(c->stream));
-        const int nface = (int)hnf;
-        const int endv = (int)nvalid;
-        HVCK(hipMemcpyAsync(fstart + nface, &endv, sizeof(int), hipMemcpyHostToDevice, c->stream));
-        uint64_t* flo = iget<uint64_t>(c, nm("i_flo").c_str(), nface);
-        uint64_t* fhi = iget<uint64_t>(c, nm("i_fhi").c_str(), nface);
-        double* meas = iget<double>(c, nm("i_meas").c_str(), nface);
-        if (!flo || !fhi || !meas) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
-        k_face_table<<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, lo2, hi, flo, fhi);
-        HVCK(hipGetLastError());
-        double* J = fi ? iget<double>(c, nm("i_J").c_str(), (size_t)nface * nf) : nullptr;
-        if (fi && !J) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
-        if (m == D) {
-            k_edge_len<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vc, meas, bad);
-            if (fi) {
-                HVCK(hipGetLastError());
-                k_edge_int<<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, meas, fv, nf, J);
-            }
+        const auto add = [&](int i) -> bool {
+            if (hd[(size_t)i + 1] == hd[(size_t)i]) return false;   // no vertex holds cell i
+            hcell.push_back(hd[(size_t)i]);
+            return true;
+        };
+        if (mesh->cells) {
+            for (int i : *mesh->cells)
+                if (!add(i)) return hv_fail(c, HV_ERR_STATE, "a listed cell of the mesh has no vertex");
         } else {
-            const char* s1 = (m & 1) ? "b" : "a";
-            auto n1 = [&](const char* s) { return std::string(s) + s1; };
-            const int* fstart1 = (const int*)c->pool->get(n1("i_fst"), 0);
-            const int* vid1 = (const int*)c->pool->get(n1("i_vid"), 0);
-            const uint64_t* flo1 = (const uint64_t*)c->pool->get(n1("i_flo"), 0);
-            const uint64_t* fhi1 = (const uint64_t*)c->pool->get(n1("i_fhi"), 0);
-            const double* meas1 = (const double*)c->pool->get(n1("i_meas"), 0);
-            if (hv_face_wave(D, m))
-                k_face_measure<D, true><<<(unsigned)std::min(std::max(nface, 1), 1 << 20), 64, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi,
-                                                                               nface1, fstart1, vid1, flo1, fhi1,
-                                                                               meas1, vk, vc, meas, bad);
-            else
-                k_face_measure<D, false><<<nblk(nface, 128), 128, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi,
-                                                                                  nface1, fstart1, vid1, flo1, fhi1,
-                                                                                  meas1, vk, vc, meas, bad);
-            if (fi) {
-                HVCK(hipGetLastError());
-                double* cen = iget<double>(c, "i_cen", (size_t)nface * D);
-                const double* J1 = (const double*)c->pool->get(n1("i_J"), 0);
-                if (!cen || !J1) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
-                k_face_center<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vc, cen);
-                HVCK(hipGetLastError());
-                if (hv_face_wave(D, m))
-                    k_face_int<D, true><<<(unsigned)std::min(std::max(nface, 1), 1 << 20), 64, 0, c->stream>>>(
-                        g, m, nface, fstart, vid, flo, fhi, cen, nface1, fstart1, vid1, flo1, fhi1, J1, nf, vk, J, bad);
-                else
-                    k_face_int<D, false><<<nblk(nface, 128), 128, 0, c->stream>>>(
-                        g, m, nface, fstart, vid, flo, fhi, cen, nface1, fstart1, vid1, flo1, fhi1, J1, nf, vk, J, bad);
-            }
+            for (int i = 0; i < g0.nreal; ++i)
+                if (!add(i)) return hv_fail(c, HV_ERR_STATE, "a cell of the mesh has no vertex");
         }
-        HVCK(hipGetLastError());
-        double* Iface = nullptr;   // facet integrals (m == 2)
-        if (fi && m == 2) {
-            double* cen = iget<double>(c, "i_cen", (size_t)nface * D);
-            double* fc = iget<double>(c, "i_fc", (size_t)nface * nf);
-            Iface = iget<double>(c, "i_If", (size_t)nface * nf);
-            if (!cen || !fc || !Iface) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
-            if (m == D) {   // d = 2: the facets are the edges, no centres yet
-                k_face_center<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vc, cen);
-                HVCK(hipGetLastError());
+    }
+    const int ncell = (int)hcell.size();
+    int64_t cmax = 1;
+    for (int m = 2; m <= D; ++m) cmax = std::max(cmax, binom(D + 1, m));
+    int nchunk = 1;
+    if ((int64_t)nv * cmax > HV_INC_BUDGET)
+        nchunk = (int)std::min<int64_t>(std::max(ncell, 1), (int64_t)std::ceil(1.3 * (double)nv * (double)cmax / (double)HV_INC_BUDGET));
+    if (nchunk > 1) {
+        // chunks of nearby cells (Morton order of the generators), so that few vertices are in two
+        std::vector<double> hx((size_t)std::max(1, ngen) * D);
+        HVCK(hipMemcpyAsync(hx.data(), g.xo, sizeof(double) * (size_t)ngen * D, hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        double lo[D], hi[D];
+        for (int k = 0; k < D; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+        for (int i : hcell)
+            for (int k = 0; k < D; ++k) { lo[k] = std::min(lo[k], hx[(size_t)i * D + k]); hi[k] = std::max(hi[k], hx[(size_t)i * D + k]); }
+        constexpr int B = 60 / D;
+        std::vector<std::pair<uint64_t, int>> code(ncell);
+        for (int q = 0; q < ncell; ++q) {
+            const int i = hcell[q];
+            uint64_t z = 0;
+            uint32_t u[D];
+            for (int k = 0; k < D; ++k) {
+                const double w = hi[k] > lo[k] ? (hx[(size_t)i * D + k] - lo[k]) / (hi[k] - lo[k]) : 0.0;
+                u[k] = (uint32_t)std::min(std::max(w, 0.0) * (double)((1u << B) - 1), (double)((1u << B) - 1));
             }
-            int rc = eval_integrand(c, fi, D, cen, nface, fc);
+            for (int b = B - 1; b >= 0; --b)
+                for (int k = 0; k < D; ++k) z = (z << 1) | ((u[k] >> b) & 1u);
+            code[q] = {z, i};
+        }
+        std::sort(code.begin(), code.end());
+        for (int q = 0; q < ncell; ++q) hcell[q] = code[q].second;
+    }
+    uint8_t* cmask = iget<uint8_t>(c, "i_cmask", (size_t)std::max(1, nused));
+    int* dchunk = iget<int>(c, "i_chunk", (size_t)std::max(1, ncell));
+    if (!cmask || !dchunk) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integration)");
+    if (ncell) HVCK(hipMemcpyAsync(dchunk, hcell.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
+    const bool whole = nchunk == 1 && !mesh->cells;   // every vertex takes part: no selection
+    auto set_chunk = [&](int q, int nch) -> int {
+        const int cb = (int)((int64_t)ncell * q / nch), ce = (int)((int64_t)ncell * (q + 1) / nch);
+        HVCK(hipMemsetAsync(cmask, 0, (size_t)std::max(1, nused), c->stream));
+        if (ce > cb) {
+            k_set_mask<<<nblk(ce - cb, 256), 256, 0, c->stream>>>(dchunk + cb, ce - cb, cmask);
+            HVCK(hipGetLastError());
+        }
+        return HV_OK;
+    };
+    if (!whole && nchunk > 1) {
+        // a vertex of cells in several chunks is in each of them: more chunks until every
+        // chunk's passes fit (at most 2^31 - 1 incidences)
+        unsigned long long* vcnt = iget<unsigned long long>(c, "i_vcnt", 1);
+        if (!vcnt) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integration)");
+        for (;;) {
+            int64_t worst = 0;
+            for (int q = 0; q < nchunk; ++q) {
+                int rc = set_chunk(q, nchunk);
+                if (rc) return rc;
+                HVCK(hipMemsetAsync(vcnt, 0, sizeof(unsigned long long), c->stream));
+                k_count_in_chunk<D><<<nblk(nv, 256), 256, 0, c->stream>>>(vk, nv, ngen, cmask, vcnt);
+                HVCK(hipGetLastError());
+                unsigned long long h = 0;
+                HVCK(hipMemcpyAsync(&h, vcnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+                HVCK(hipStreamSynchronize(c->stream));
+                worst = std::max<int64_t>(worst, (int64_t)h);
+            }
+            if (worst * cmax < (1ll << 31)) break;
+            if (nchunk >= ncell) return hv_fail(c, HV_ERR_CAPACITY, "too many face incidences for one cell");
+            nchunk = std::min(ncell, 2 * nchunk);
+        }
+    }
+    int64_t ntot = 0;                                  // rows of the chunks so far
+    for (int q = 0; q < nchunk; ++q) {
+        {
+            int rc = set_chunk(q, nchunk);
             if (rc) return rc;
-            k_facet_int<D><<<nblk(nface, 256), 256, 0, c->stream>>>(nface, J, fc, meas, nf, Iface);
-            HVCK(hipGetLastError());
         }
-        nface1 = nface;
-        if (m == 2) {
-            pkey = iget<uint64_t>(c, "i_pkey", (size_t)2 * nface);
-            uint64_t* pkey2 = iget<uint64_t>(c, "i_pkey2", (size_t)2 * nface);
-            parea = iget<double>(c, "i_parea", (size_t)2 * nface);
-            double* parea2 = iget<double>(c, "i_parea2", (size_t)2 * nface);
-            int* pface2 = iget<int>(c, "i_pface2", (size_t)2 * nface);
-            if (!pkey || !pkey2 || !parea || !parea2 || !pface2)
-                return hv_fail(c, HV_ERR_DEVICE, "out of device memory (facets)");
-            HVCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
-            k_facet_pairs<<<nblk(nface, 256), 256, 0, c->stream>>>(nface, flo, meas, N, pkey2, parea2, pface2, cnt);
-            HVCK(hipGetLastError());
-            unsigned long long hp = 0;
-            HVCK(hipMemcpyAsync(&hp, cnt, sizeof(hp), hipMemcpyDeviceToHost, c->stream));
+        const int* vkc = vk;
+        const double* vcc = vc;
+        const double* fvc = fv;
+        int nvc = nv;
+        if (!whole) {
+            // the vertices holding a cell of the chunk, in the canonical order
+            uint8_t* vflag = iget<uint8_t>(c, "i_vflag", (size_t)std::max(1, nv));
+            int* viota = iget<int>(c, "i_viota", (size_t)std::max(1, nv));
+            int* vsel = iget<int>(c, "i_vsel", (size_t)std::max(1, nv));
+            if (!vflag || !viota || !vsel) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integration)");
+            if (nv) {
+                k_vert_in_chunk<D><<<nblk(nv, 256), 256, 0, c->stream>>>(vk, nv, ngen, cmask, vflag, viota);
+                HVCK(hipGetLastError());
+            }
+            size_t tbf = 0;
+            HVCK(hipcub::DeviceSelect::Flagged(nullptr, tbf, viota, vflag, vsel, cnt + 1, nv, c->stream));
+            void* tmpf = c->pool->get("i_tmpf", tbf, false, c->stream);
+            if (!tmpf) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
+            HVCK(hipcub::DeviceSelect::Flagged(tmpf, tbf, viota, vflag, vsel, cnt + 1, nv, c->stream));
+            unsigned long long hsel = 0;
+            HVCK(hipMemcpyAsync(&hsel, cnt + 1, sizeof(hsel), hipMemcpyDeviceToHost, c->stream));
             HVCK(hipStreamSynchronize(c->stream));
-            npair = (int64_t)hp;
-            size_t tb3 = 0;
-            HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb3, pkey2, pkey, parea2, parea, (int)npair, 0, 64, c->stream));
-            void* tmp3 = c->pool->get("i_tmp3", tb3, false, c->stream);
-            if (!tmp3) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
-            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp3, tb3, pkey2, pkey, parea2, parea, (int)npair, 0, 64, c->stream));
-            if (fi) {
-                // the same stable sort carrying the facet numbers, then the rows' integrals
-                uint64_t* pkey3 = iget<uint64_t>(c, "i_pkey3", (size_t)2 * nface);
-                int* pface = iget<int>(c, "i_pface", (size_t)2 * nface);
-                *rint = iget<double>(c, "i_rint", (size_t)std::max<int64_t>(npair, 1) * nf);
-                if (!pkey3 || !pface || !*rint) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
-                size_t tb4 = 0;
-                HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb4, pkey2, pkey3, pface2, pface, (int)npair, 0, 64, c->stream));
-                void* tmp4 = c->pool->get("i_tmp4", tb4, false, c->stream);
-                if (!tmp4) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
-                HVCK(hipcub::DeviceRadixSort::SortPairs(tmp4, tb4, pkey2, pkey3, pface2, pface, (int)npair, 0, 64, c->stream));
-                if (npair > 0) {
-                    k_row_int<<<nblk(npair, 256), 256, 0, c->stream>>>(npair, pface, Iface, nf, *rint);
+            nvc = (int)hsel;
+            int* vk2 = iget<int>(c, "i_vkc", (size_t)std::max(1, nvc) * (D + 1));
+            double* vc2 = iget<double>(c, "i_vcc", (size_t)std::max(1, nvc) * D);
+            double* fv2 = fi ? iget<double>(c, "i_fvc", (size_t)std::max(1, nvc) * nf) : nullptr;
+            if (!vk2 || !vc2 || (fi && !fv2)) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integration)");
+            if (nvc) {
+                k_int_subset<D><<<nblk(nvc, 256), 256, 0, c->stream>>>(vsel, nvc, vk, vc, fv, nf, vk2, vc2, fv2);
+                HVCK(hipGetLastError());
+            }
+            vkc = vk2;
+            vcc = vc2;
+            fvc = fv2;
+        }
+        // per pass: sorted incidences (vid), face starts, keys, measures; "1" = pass m + 1
+        int nface1 = 0;
+        for (int m = D; m >= 2; --m) {
+            const int C = (int)binom(D + 1, m);
+            const int64_t tot = (int64_t)nvc * C;
+            if (tot >= (1ll << 31)) return hv_fail(c, HV_ERR_CAPACITY, "too many face incidences for one pass");
+            const char* sfx = (m & 1) ? "a" : "b";
+            auto nm = [&](const char* s) { return std::string(s) + sfx; };
+            uint64_t* lo = iget<uint64_t>(c, "i_lo", tot);
+            uint64_t* hi = iget<uint64_t>(c, "i_hi", tot);
+            uint64_t* lo2 = iget<uint64_t>(c, "i_lo2", tot);
+            uint64_t* hi2 = iget<uint64_t>(c, "i_hi2", tot);
+            int* pa = iget<int>(c, "i_pa", tot);
+            int* pb = iget<int>(c, "i_pb", tot);
+            uint8_t* head = iget<uint8_t>(c, "i_head", tot);
+            if (!lo || !hi || !lo2 || !hi2 || !pa || !pb || !head) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
+            HVCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+            // faces with a cell of the chunk (the others may be incomplete here)
+            k_face_emit<D><<<nblk(nvc, 256), 256, 0, c->stream>>>(vkc, nvc, m, g.N, cmask, C, lo, hi, pa, cnt);
+            HVCK(hipGetLastError());
+            // two stable radix passes: by lo, then by hi -> (hi, lo) order
+            size_t tb = 0;
+            HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, lo, lo2, pa, pb, (int)tot, 0, 64, c->stream));
+            void* tmp = c->pool->get("i_tmp", tb, false, c->stream);
+            if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
+            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, lo, lo2, pa, pb, (int)tot, 0, 64, c->stream));
+            int* perm = pb;
+            if (m > 3) {
+                k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(hi, pb, tot, hi2);
+                HVCK(hipGetLastError());
+                HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, hi2, hi, pb, pa, (int)tot, 0, 64, c->stream));
+                perm = pa;
+                k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(lo, perm, tot, lo2);
+                HVCK(hipGetLastError());
+            } else {
+                k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(hi, perm, tot, hi2);
+                HVCK(hipGetLastError());
+                HVCK(hipMemcpyAsync(hi, hi2, sizeof(uint64_t) * tot, hipMemcpyDeviceToDevice, c->stream));
+            }
+            // lo2 / hi: sorted keys; perm: sorted incidence numbers
+            unsigned long long hsent = 0;
+            HVCK(hipMemcpyAsync(&hsent, cnt, sizeof(hsent), hipMemcpyDeviceToHost, c->stream));
+            HVCK(hipStreamSynchronize(c->stream));
+            const int64_t nvalid = tot - (int64_t)hsent;
+            int* vid = iget<int>(c, nm("i_vid").c_str(), nvalid);
+            int* fstart = iget<int>(c, nm("i_fst").c_str(), nvalid + 1);
+            int* iota = iget<int>(c, "i_iota", nvalid);
+            if (!vid || !fstart || !iota) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
+            k_face_heads<<<nblk(nvalid, 256), 256, 0, c->stream>>>(lo2, hi, perm, nvalid, C, vid, head);
+            HVCK(hipGetLastError());
+            k_iota64<<<nblk(nvalid, 256), 256, 0, c->stream>>>(iota, nvalid);
+            HVCK(hipGetLastError());
+            size_t tb2 = 0;
+            HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb2, iota, head, fstart, cnt + 1, (int)nvalid, c->stream));
+            void* tmp2 = c->pool->get("i_tmp2", tb2, false, c->stream);
+            if (!tmp2) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
+            HVCK(hipcub::DeviceSelect::Flagged(tmp2, tb2, iota, head, fstart, cnt + 1, (int)nvalid, c->stream));
+            unsigned long long hnf = 0;
+            HVCK(hipMemcpyAsync(&hnf, cnt + 1, sizeof(hnf), hipMemcpyDeviceToHost, c->stream));
+            HVCK(hipStreamSynchronize(c->stream));
+            const int nface = (int)hnf;
+            const int endv = (int)nvalid;
+            HVCK(hipMemcpyAsync(fstart + nface, &endv, sizeof(int), hipMemcpyHostToDevice, c->stream));
+            uint64_t* flo = iget<uint64_t>(c, nm("i_flo").c_str(), nface);
+            uint64_t* fhi = iget<uint64_t>(c, nm("i_fhi").c_str(), nface);
+            double* meas = iget<double>(c, nm("i_meas").c_str(), nface);
+            if (!flo || !fhi || !meas) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
+            k_face_table<<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, lo2, hi, flo, fhi);
+            HVCK(hipGetLastError());
+            double* J = fi ? iget<double>(c, nm("i_J").c_str(), (size_t)nface * nf) : nullptr;
+            if (fi && !J) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
+            if (m == D) {
+                k_edge_len<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vcc, meas, bad);
+                if (fi) {
                     HVCK(hipGetLastError());
+                    k_edge_int<<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, meas, fvc, nf, J);
+                }
+            } else {
+                const char* s1 = (m & 1) ? "b" : "a";
+                auto n1 = [&](const char* s) { return std::string(s) + s1; };
+                const int* fstart1 = (const int*)c->pool->get(n1("i_fst"), 0);
+                const int* vid1 = (const int*)c->pool->get(n1("i_vid"), 0);
+                const uint64_t* flo1 = (const uint64_t*)c->pool->get(n1("i_flo"), 0);
+                const uint64_t* fhi1 = (const uint64_t*)c->pool->get(n1("i_fhi"), 0);
+                const double* meas1 = (const double*)c->pool->get(n1("i_meas"), 0);
+                if (hv_face_wave(D, m))
+                    k_face_measure<D, true><<<(unsigned)std::min(std::max(nface, 1), 1 << 20), 64, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi,
+                                                                                   nface1, fstart1, vid1, flo1, fhi1,
+                                                                                   meas1, vkc, vcc, meas, bad);
+                else
+                    k_face_measure<D, false><<<nblk(nface, 128), 128, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi,
+                                                                                      nface1, fstart1, vid1, flo1, fhi1,
+                                                                                      meas1, vkc, vcc, meas, bad);
+                if (fi) {
+                    HVCK(hipGetLastError());
+                    double* cen = iget<double>(c, "i_cen", (size_t)nface * D);
+                    const double* J1 = (const double*)c->pool->get(n1("i_J"), 0);
+                    if (!cen || !J1) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
+                    k_face_center<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vcc, cen);
+                    HVCK(hipGetLastError());
+                    if (hv_face_wave(D, m))
+                        k_face_int<D, true><<<(unsigned)std::min(std::max(nface, 1), 1 << 20), 64, 0, c->stream>>>(
+                            g, m, nface, fstart, vid, flo, fhi, cen, nface1, fstart1, vid1, flo1, fhi1, J1, nf, vkc, J, bad);
+                    else
+                        k_face_int<D, false><<<nblk(nface, 128), 128, 0, c->stream>>>(
+                            g, m, nface, fstart, vid, flo, fhi, cen, nface1, fstart1, vid1, flo1, fhi1, J1, nf, vkc, J, bad);
+                }
+            }
+            HVCK(hipGetLastError());
+            double* Iface = nullptr;   // facet integrals (m == 2)
+            if (fi && m == 2) {
+                double* cen = iget<double>(c, "i_cen", (size_t)nface * D);
+                double* fc = iget<double>(c, "i_fc", (size_t)nface * nf);
+                Iface = iget<double>(c, "i_If", (size_t)nface * nf);
+                if (!cen || !fc || !Iface) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
+                if (m == D) {   // d = 2: the facets are the edges, no centres yet
+                    k_face_center<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vcc, cen);
+                    HVCK(hipGetLastError());
+                }
+                int rc = eval_integrand(c, fi, D, cen, nface, fc);
+                if (rc) return rc;
+                k_facet_int<D><<<nblk(nface, 256), 256, 0, c->stream>>>(nface, J, fc, meas, nf, Iface);
+                HVCK(hipGetLastError());
+            }
+            nface1 = nface;
+            if (m == 2) {
+                pkey = iget<uint64_t>(c, "i_pkey", (size_t)2 * nface);
+                uint64_t* pkey2 = iget<uint64_t>(c, "i_pkey2", (size_t)2 * nface);
+                parea = iget<double>(c, "i_parea", (size_t)2 * nface);
+                double* parea2 = iget<double>(c, "i_parea2", (size_t)2 * nface);
+                int* pface2 = iget<int>(c, "i_pface2", (size_t)2 * nface);
+                if (!pkey || !pkey2 || !parea || !parea2 || !pface2)
+                    return hv_fail(c, HV_ERR_DEVICE, "out of device memory (facets)");
+                HVCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+                k_facet_pairs<<<nblk(nface, 256), 256, 0, c->stream>>>(nface, flo, meas, g.N, cmask, pkey2, parea2, pface2, cnt);
+                HVCK(hipGetLastError());
+                unsigned long long hp = 0;
+                HVCK(hipMemcpyAsync(&hp, cnt, sizeof(hp), hipMemcpyDeviceToHost, c->stream));
+                HVCK(hipStreamSynchronize(c->stream));
+                npair = (int64_t)hp;
+                size_t tb3 = 0;
+                HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb3, pkey2, pkey, parea2, parea, (int)npair, 0, 64, c->stream));
+                void* tmp3 = c->pool->get("i_tmp3", tb3, false, c->stream);
+                if (!tmp3) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
+                HVCK(hipcub::DeviceRadixSort::SortPairs(tmp3, tb3, pkey2, pkey, parea2, parea, (int)npair, 0, 64, c->stream));
+                if (fi) {
+                    // the same stable sort carrying the facet numbers, then the rows' integrals
+                    uint64_t* pkey3 = iget<uint64_t>(c, "i_pkey3", (size_t)2 * nface);
+                    int* pface = iget<int>(c, "i_pface", (size_t)2 * nface);
+                    *rint = iget<double>(c, "i_rint", (size_t)std::max<int64_t>(npair, 1) * nf);
+                    if (!pkey3 || !pface || !*rint) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
+                    size_t tb4 = 0;
+                    HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb4, pkey2, pkey3, pface2, pface, (int)npair, 0, 64, c->stream));
+                    void* tmp4 = c->pool->get("i_tmp4", tb4, false, c->stream);
+                    if (!tmp4) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
+                    HVCK(hipcub::DeviceRadixSort::SortPairs(tmp4, tb4, pkey2, pkey3, pface2, pface, (int)npair, 0, 64, c->stream));
+                    if (npair > 0) {
+                        k_row_int<<<nblk(npair, 256), 256, 0, c->stream>>>(npair, pface, Iface, nf, *rint);
+                        HVCK(hipGetLastError());
+                    }
                 }
             }
         }
+        if (!whole || nchunk > 1) {
+            // this chunk's rows after the rows of the chunks before
+            uint64_t* rk = (uint64_t*)c->pool->get("i_rk", sizeof(uint64_t) * (size_t)std::max<int64_t>(ntot + npair, 1), true, c->stream);
+            double* ra = (double*)c->pool->get("i_ra", sizeof(double) * (size_t)std::max<int64_t>(ntot + npair, 1), true, c->stream);
+            double* ri = fi ? (double*)c->pool->get("i_ri", sizeof(double) * (size_t)std::max<int64_t>(ntot + npair, 1) * nf, true, c->stream)
+                            : nullptr;
+            if (!rk || !ra || (fi && !ri)) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (rows)");
+            if (npair) {
+                HVCK(hipMemcpyAsync(rk + ntot, pkey, sizeof(uint64_t) * npair, hipMemcpyDeviceToDevice, c->stream));
+                HVCK(hipMemcpyAsync(ra + ntot, parea, sizeof(double) * npair, hipMemcpyDeviceToDevice, c->stream));
+                if (fi) HVCK(hipMemcpyAsync(ri + ntot * nf, *rint, sizeof(double) * npair * nf, hipMemcpyDeviceToDevice, c->stream));
+            }
+            ntot += npair;
+        }
+    }
+    if (!whole || nchunk > 1) {
+        // rows of all chunks, sorted by (cell, neighbour)
+        uint64_t* rk = (uint64_t*)c->pool->get("i_rk", 0);
+        double* ra = (double*)c->pool->get("i_ra", 0);
+        double* ri = fi ? (double*)c->pool->get("i_ri", 0) : nullptr;
+        npair = ntot;
+        pkey = iget<uint64_t>(c, "i_pkey", (size_t)npair);
+        parea = iget<double>(c, "i_parea", (size_t)npair);
+        int* ia = iget<int>(c, "i_rpa", (size_t)npair);
+        int* ib = iget<int>(c, "i_rpb", (size_t)npair);
+        if (fi) *rint = iget<double>(c, "i_rint", (size_t)std::max<int64_t>(npair, 1) * nf);
+        if (!rk || !ra || !pkey || !parea || !ia || !ib || (fi && (!ri || !*rint)))
+            return hv_fail(c, HV_ERR_DEVICE, "out of device memory (rows)");
+        if (npair > 0) {
+            k_iota64<<<nblk(npair, 256), 256, 0, c->stream>>>(ia, npair);
+            HVCK(hipGetLastError());
+            size_t tbr = 0;
+            HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbr, rk, pkey, ia, ib, (int)npair, 0, 64, c->stream));
+            void* tmpr = c->pool->get("i_tmpr", tbr, false, c->stream);
+            if (!tmpr) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
+            HVCK(hipcub::DeviceRadixSort::SortPairs(tmpr, tbr, rk, pkey, ia, ib, (int)npair, 0, 64, c->stream));
+            k_gather_rows<<<nblk(npair, 256), 256, 0, c->stream>>>(ib, npair, ra, parea, ri, fi ? *rint : nullptr, nf);
+            HVCK(hipGetLastError());
+        }
+    }
+    if (npair > 0) {
+        k_rows_to_ids<<<nblk(npair, 256), 256, 0, c->stream>>>(pkey, npair, gid);
+        HVCK(hipGetLastError());
     }
     int hbad = 0;
     HVCK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1189,13 +1537,13 @@ static int neighbor_rows(hv_ctx* c, const hv_mesh* mesh, uint64_t*& pkey, int64_
 
 template <int D>
 static int make_geo(hv_ctx* c, const hv_mesh* mesh, IntGeo& g) {
-    if (!mesh || mesh->D != D || mesh->N != c->N) return hv_fail(c, HV_ERR_STATE, "the mesh was not built from this context's points");
-    if (mesh->nreal != mesh->N && mesh->gen != c->gen)
-        return hv_fail(c, HV_ERR_STATE, "the periodic mesh was not built from this context's current point set");
+    // the context is bound to the mesh's own working set and boundary (hv_bind_mesh)
+    if (!mesh || mesh->D != D || mesh->N != c->N || !c->xw || c->xw != mesh->xw)
+        return hv_fail(c, HV_ERR_STATE, "the context is not bound to the mesh's generators");
     g.N = (int)c->N;
     g.nreal = (int)mesh->nreal;
     g.P = c->P;
-    g.xo = (const double*)c->pool->get("xin", 0);
+    g.xo = (const double*)c->xw->p;
     double* pn = iget<double>(c, "i_pn", (size_t)std::max(1, c->P) * D);
     double* ph = iget<double>(c, "i_ph", (size_t)std::max(1, c->P));
     if (!g.xo || !pn || !ph) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integration)");

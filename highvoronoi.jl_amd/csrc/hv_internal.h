@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <vector>
@@ -15,8 +16,11 @@ struct DBuf {
     size_t n = 0;
 };
 
+// One lock per pool: a mesh returns its tables to the pool of the context that built it
+// (hv_mesh_free), possibly from another host thread than the one using that context.
 struct DevPool {
     int device = 0;
+    std::mutex mu;
     std::map<std::string, DBuf> bufs;
     ~DevPool();
     // returns nullptr (and keeps the old buffer) when hipMalloc fails
@@ -25,12 +29,25 @@ struct DevPool {
     void give(const std::string& name, DBuf b);
 };
 
+// A device allocation shared by a context and the meshes built from it (the generator
+// set a mesh refers to stays alive as long as the mesh does).
+struct DevArr {
+    int device = 0;
+    void* p = nullptr;
+    size_t n = 0;
+    ~DevArr();
+};
+using ImgOrig = std::shared_ptr<const std::vector<int32_t>>;
+using ImgShift = std::shared_ptr<const std::vector<int8_t>>;
+
 struct hv_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int D = 0;
     int64_t N = 0;
     std::shared_ptr<DevPool> pool;
+    // the working generator set (N x D FP64): written in place only while no mesh holds it
+    std::shared_ptr<DevArr> xw;
     // boundary (host copies)
     int P = 0;
     std::vector<double> pbase, pnormal;
@@ -38,32 +55,66 @@ struct hv_ctx {
     std::string err;
     hipEvent_t ev[4];
     // periodic boundaries: a build on planes with periodic partners replaces the working point
-    // set ("xin", N points) by the nreal originals followed by their images ("ghosts"); the
-    // originals stay in "xorig".  gen changes whenever the working set does.
+    // set (N points) by the nreal originals followed by their images ("ghosts"); the originals
+    // stay in xorig.
     int64_t nreal = 0;
     bool extended = false;
-    uint64_t gen = 0;
+    std::shared_ptr<DevArr> xorig;
     double delta = 0.0;                  // margin of the extended set beyond the box
     double rmax = 0.0;                   // largest circumradius of the periodic mesh
-    std::vector<int32_t> img_orig;       // per working-set id: the original (0-based)
-    std::vector<int8_t> img_shift;       // per working-set id: d shifts in box lengths
+    ImgOrig img_orig;                    // per working-set id: the original (0-based)
+    ImgShift img_shift;                  // per working-set id: d shifts in box lengths
 };
 
 struct hv_mesh {
-    std::shared_ptr<DevPool> pool;
+    std::shared_ptr<DevPool> pool;   // where the vertex tables go back on free (null: freed)
     int device = 0;
     int D = 0;
     int64_t N = 0;
     int64_t nv = 0;
     DBuf key, x, ids;   // vertex table (provisional ids) and the list of live ids
     hv_stats_t stats{};
+    // what the mesh was built on: the working set and the caller's boundary, so that any
+    // context on the same device can integrate it (hv_bind_mesh)
+    std::shared_ptr<DevArr> xw;
+    int P = 0;
+    std::vector<double> pbase, pnormal;
+    std::vector<int16_t> pbc;
     // periodic meshes: generators are the nreal originals and N - nreal ghosts (image table
     // below); every vertex has at least one original generator.  nreal == N otherwise.
     int64_t nreal = 0;
-    uint64_t gen = 0;
+    bool extended = false;
     double delta = 0.0;
-    std::vector<int32_t> img_orig;
-    std::vector<int8_t> img_shift;
+    double rmax = 0.0;
+    ImgOrig img_orig;
+    ImgShift img_shift;
+    // a mesh of some cells only (hv_build_mesh_cells): their 0-based ids, sorted; null for all
+    std::shared_ptr<const std::vector<int>> cells;
+};
+
+// The context's working set for writing `bytes`: the current buffer when no mesh refers to
+// it and it is large enough, a new one otherwise (nullptr when out of memory).
+double* hv_points_for_write(hv_ctx* c, size_t bytes);
+// Record on a new mesh what it was built on (points, boundary, periodic state).
+void hv_mesh_snapshot(const hv_ctx* c, hv_mesh* m);
+
+// Binds a context to the state a mesh was built on for the duration of one call (the
+// integrators read points, planes and periodic images from the context) and restores the
+// context's own state afterwards.
+struct hv_bind_mesh {
+    hv_ctx* c;
+    int D;
+    int64_t N, nreal;
+    std::shared_ptr<DevArr> xw;
+    int P;
+    std::vector<double> pbase, pnormal;
+    std::vector<int16_t> pbc;
+    bool extended;
+    double delta, rmax;
+    ImgOrig img_orig;
+    ImgShift img_shift;
+    hv_bind_mesh(hv_ctx* ctx, const hv_mesh* m);
+    ~hv_bind_mesh();
 };
 
 // planes of the context as an axis-aligned box: 0 and the mask of periodic directions
@@ -73,7 +124,7 @@ int hv_box_periodic_mask(hv_ctx* c, int D, int* mask);
 int hv_fail(hv_ctx* c, int code, const std::string& msg);
 
 template <int D>
-int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out);
+int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, const std::vector<int>* cells);
 template <int D>
 int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, const int64_t* ptr,
                       const uint64_t* pkey, unsigned long long* ccnt, unsigned long long* pcnt, int64_t* nused,

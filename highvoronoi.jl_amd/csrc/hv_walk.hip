@@ -451,10 +451,22 @@ __global__ __launch_bounds__(256) void k_rt_build(DevRays R, int nr, const int* 
     hv_wave_count(&ctr[1], full);
 }
 
+// Walked edges: those with a generator below nwalk (all edges when nwalk is the generator
+// count), or, with a cell mask over the ngen generators, those holding a marked generator.
+template <int D>
+__device__ __forceinline__ bool hv_edge_walked(const int* e, int nwalk, const uint8_t* __restrict__ cmask, int ngen) {
+    if (!cmask) return e[0] < nwalk;
+    bool w = false;
+#pragma unroll
+    for (int j = 0; j < D; ++j) w = w || (e[j] < ngen && cmask[e[j]] != 0);
+    return w;
+}
+
 template <int D>
 __global__ __launch_bounds__(256) void k_register(const int* __restrict__ frontier, int nf,
                                                   const int* __restrict__ nf_dev, const int* __restrict__ vkey,
-                                                  int N, DevRays R, unsigned long long* __restrict__ rt, uint64_t rmask,
+                                                  int N, const uint8_t* __restrict__ cmask, int ngen, DevRays R,
+                                                  unsigned long long* __restrict__ rt, uint64_t rmask,
                                                   unsigned long long* __restrict__ nt, uint64_t nmask,
                                                   int64_t* __restrict__ reg, unsigned long long* __restrict__ ctr) {
     // nf_dev (when given): the vertex count in device memory, nf then only bounds the grid
@@ -468,7 +480,7 @@ __global__ __launch_bounds__(256) void k_register(const int* __restrict__ fronti
         int e[D];
         hv_edge_of<D>(key, pos, e);
         int64_t out = -1;   // an edge of planes only is a corner point, not a line
-        if (e[0] < N) {
+        if (hv_edge_walked<D>(e, N, cmask, ngen)) {
             const uint64_t hfull = hv_hash_ints<D>(e);
             const unsigned long long fp = HV_EDGE_FP(hfull);
             // the edge of a ray of this level: its other end point is the ray's source
@@ -2038,8 +2050,16 @@ __global__ void k_iota(int* __restrict__ a, int n) {
 // host driver
 
 template <int D>
-__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int stride, int nwalk, int* __restrict__ vkey,
-                                                     double* __restrict__ vx, int64_t base, int* __restrict__ status);
+__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int stride, int nwalk, const int* __restrict__ cells,
+                                                     int* __restrict__ vkey, double* __restrict__ vx, int64_t base,
+                                                     int* __restrict__ status);
+template <int D>
+__global__ __launch_bounds__(256) void k_seed_exact(DevGeo g, const int* __restrict__ cells, int ncell,
+                                                    int* __restrict__ vkey, double* __restrict__ vx, int64_t base,
+                                                    int* __restrict__ status);
+template <int D>
+__global__ __launch_bounds__(256) void k_mark_cells(const int* __restrict__ ids, int nv, const int* __restrict__ vkey,
+                                                    int nreal, int* __restrict__ mark);
 
 namespace {
 
@@ -2098,7 +2118,8 @@ int build_tree(hv_ctx* c, int L, Tree& T, int tables = HV_PREC_FP64) {
     }
     g.pn = pn;
     g.ph = ph;
-    const double* xin = (const double*)c->pool->get("xin", 0);
+    if (!c->xw) return hv_fail(c, HV_ERR_STATE, "no points set");
+    const double* xin = (const double*)c->xw->p;
     double* xo = pget<double>(c, "xo", (size_t)N * D);
     double* xs = pget<double>(c, "xs", (size_t)N * (D + 1));
     int* sidx = pget<int>(c, "sidx", N);
@@ -2341,9 +2362,13 @@ __global__ void k_chunk(const int* __restrict__ ord, int nr, int c, int K, const
 
 // The walk over the context's working set.  Only edges with a generator below nwalk are
 // followed and only those generators seed it, so the mesh holds the vertices of the cells
-// 0 .. nwalk-1 (all of them when nwalk == N).
+// 0 .. nwalk-1 (all of them when nwalk == N).  With a list of cells (sorted, distinct, each
+// below nwalk), only edges holding one of them are followed and only they seed the walk: the
+// mesh holds exactly the vertices of those cells (each cell's edge graph is connected, and an
+// edge's end points are vertices of every cell the edge holds).  A listed cell that neither a
+// seed nor the walk reached gets an exact descent, and the walk continues from there.
 template <int D>
-int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk) {
+int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const std::vector<int>* cells = nullptr) {
     hv_opts opt{};
     if (o) opt = *o;
     if (opt.precision != HV_PREC_FP64 && opt.precision != HV_PREC_BF16 && opt.precision != HV_PREC_FP32)
@@ -2357,6 +2382,8 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk) {
     int rc = check_box(c, D);
     if (rc) return rc;
     const int N = (int)c->N;
+    const bool subset = cells != nullptr;
+    const int ncell = subset ? (int)cells->size() : 0;
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1], r0 = c->ev[2], r1 = c->ev[3];
     HVCK(hipEventRecord(e0, c->stream));
     Tree T;
@@ -2364,8 +2391,22 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk) {
     if (rc) return rc;
     DevGeo& g = T.g;
     const int nseed = 1 << D;
+    // the listed cells (device copy) and their mask over the generators
+    uint8_t* cmask = nullptr;
+    int* dcells = nullptr;
+    if (subset) {
+        cmask = pget<uint8_t>(c, "cmask", (size_t)N);
+        dcells = pget<int>(c, "cells", (size_t)std::max(1, ncell));
+        if (!cmask || !dcells) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (cells)");
+        std::vector<uint8_t> hm((size_t)N, 0);
+        for (int i : *cells) hm[(size_t)i] = 1;
+        HVCK(hipMemcpyAsync(cmask, hm.data(), (size_t)N, hipMemcpyHostToDevice, c->stream));
+        if (ncell) HVCK(hipMemcpyAsync(dcells, cells->data(), sizeof(int) * ncell, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+    }
     // initial sizes from the expected vertex count (or the caller's hint); all tables grow on demand
     const double est = opt.vertex_capacity > 0 ? (double)opt.vertex_capacity
+                       : subset ? vertices_per_periodic_cell(D) * ncell
                        : (nwalk < N ? vertices_per_periodic_cell(D) : vertices_per_point(D)) * nwalk;
     int64_t vcap = (int64_t)(2.0 * est) + 4 * nseed + 1024;
     auto hpow = [](int64_t n) { uint64_t p = 1024; while ((int64_t)p < n) p <<= 1; return p; };
@@ -2471,7 +2512,7 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk) {
         }
         if (count == 0) return HV_OK;
         k_register<D><<<nblk((int64_t)count * (D + 1), 256), 256, 0, c->stream>>>(
-            ids + first, count, count_dev, vkey, nwalk, R, rtab, rsize - 1, ntab, nsize - 1, reg, ctr + 4);
+            ids + first, count, count_dev, vkey, nwalk, cmask, N, R, rtab, rsize - 1, ntab, nsize - 1, reg, ctr + 4);
         HVCK(hipGetLastError());
         return HV_OK;
     };
@@ -2483,9 +2524,9 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk) {
     // in larger sets, where the descents' own cost outgrows the levels they save (3D N=5M:
     // 91 against 102 ms).  Every generator's cell when only some cells are walked: their
     // union need not be connected through walked edges.
-    const bool part = nwalk < N;
+    const bool part = nwalk < N || subset;
     const int stride = (part || D == 2 || N <= HV_SEED_ALL_MAX) ? 1 : HV_SEED_STRIDE;
-    const int ndesc = N / stride;
+    const int ndesc = subset ? ncell : N / stride;
     const int nstart = nseed + ndesc;
     int64_t* reg_cur = pget<int64_t>(c, "regA", (size_t)nstart * (D + 1));
     if (!reg_cur) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
@@ -2501,7 +2542,7 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk) {
             HVCK(hipGetLastError());
         }
         if (ndesc > 0) {
-            k_seed_descent<D><<<nblk(ndesc, 64), 64, 0, c->stream>>>(g, ndesc, stride, nwalk, vkey, vx,
+            k_seed_descent<D><<<nblk(ndesc, 64), 64, 0, c->stream>>>(g, ndesc, stride, nwalk, dcells, vkey, vx,
                                                                        vnext + nseed, status + nseed);
             HVCK(hipGetLastError());
         }
@@ -2521,6 +2562,7 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk) {
     double ms_ray = 0.0;
     int64_t levels = 0, skipped = 0;
     int ncar = 0;   // rays carried from the previous level (waiting rays whose edge stayed open)
+    for (int phase = 0;; ++phase) {
     while (fcount > 0 || ncar > 0) {
         if (++levels > 1000000) return hv_fail(c, HV_ERR_CAPACITY, "walk did not terminate");
         const int64_t nslots = fcount * (D + 1) + ncar;
@@ -2718,6 +2760,52 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk) {
         fbeg = level_begin;
         fcount = nlive - level_begin;
     }
+    if (!subset || ncell == 0) break;
+    // listed cells without a vertex: an exact descent each (phase 0), or give up (phase 1)
+    std::vector<int> miss;
+    {
+        int* mark = pget<int>(c, "cmark", (size_t)N);
+        if (!mark) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (cells)");
+        HVCK(hipMemsetAsync(mark, 0, sizeof(int) * N, c->stream));
+        if (nlive) {
+            k_mark_cells<D><<<nblk(nlive, 256), 256, 0, c->stream>>>(ids, (int)nlive, vkey, N, mark);
+            HVCK(hipGetLastError());
+        }
+        std::vector<int> hmark((size_t)N);
+        HVCK(hipMemcpyAsync(hmark.data(), mark, sizeof(int) * N, hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        for (int i : *cells)
+            if (!hmark[(size_t)i]) miss.push_back(i);
+    }
+    if (miss.empty()) break;
+    if (phase > 0) return hv_fail(c, HV_ERR_DEGENERATE, "a listed cell has no certified vertex (input not in general position?)");
+    {
+        const int nm = (int)miss.size();
+        rc = ensure_vertices(nm);
+        if (rc) return rc;
+        int* status = pget<int>(c, "status", nm);
+        int* dmiss = pget<int>(c, "cmiss", nm);
+        if (!status || !dmiss) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (cells)");
+        HVCK(hipMemcpyAsync(dmiss, miss.data(), sizeof(int) * nm, hipMemcpyHostToDevice, c->stream));
+        k_seed_exact<D><<<nm, 256, 0, c->stream>>>(g, dmiss, nm, vkey, vx, vnext, status);
+        HVCK(hipGetLastError());
+        int nsel = 0;
+        rc = insert_and_select(status, nm, vnext, nsel);
+        if (rc) return rc;
+        DevRays none{};
+        rc = build_rt(none, 0);
+        if (rc) return rc;
+        nt_fresh = true;
+        reg_cur = pget<int64_t>(c, reg_is_a ? "regA" : "regB", (size_t)std::max(1, nsel) * (D + 1));
+        if (!reg_cur) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+        rc = register_new(none, nlive, nsel, nsel, reg_cur);
+        if (rc) return rc;
+        vnext += nm;
+        fbeg = nlive;
+        fcount = nsel;
+        nlive += nsel;
+    }
+    }
     HVCK(hipEventRecord(e1, c->stream));
     rc = read_ctr();
     if (rc) return rc;
@@ -2752,8 +2840,8 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk) {
     m->x = c->pool->take("vx");
     m->ids = c->pool->take("ids");
     m->stats = st;
-    m->nreal = N;
-    m->gen = c->gen;
+    hv_mesh_snapshot(c, m);
+    if (subset) m->cells = std::make_shared<const std::vector<int>>(*cells);
     *out = m;
     return HV_OK;
 }
@@ -2832,17 +2920,16 @@ __global__ __launch_bounds__(256) void k_count_unmarked(const int* __restrict__ 
 }
 
 template <int D>
-static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
+static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask, const std::vector<int>* cells) {
     const int64_t n = c->nreal;
     const size_t obytes = sizeof(double) * D * n;
     if (!c->extended) {   // keep the originals: the working set is about to be replaced
-        void* dst = c->pool->get("xorig", obytes, false, c->stream);
-        const void* src = c->pool->get("xin", 0);
-        if (!dst || !src) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
-        HVCK(hipMemcpyAsync(dst, src, obytes, hipMemcpyDeviceToDevice, c->stream));
+        if (!c->xw) return hv_fail(c, HV_ERR_STATE, "no points set");
+        c->xorig = c->xw;
     }
+    if (!c->xorig) return hv_fail(c, HV_ERR_STATE, "the originals of the periodic set are gone");
     std::vector<double> hx((size_t)n * D);
-    HVCK(hipMemcpyAsync(hx.data(), c->pool->get("xorig", 0), obytes, hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipMemcpyAsync(hx.data(), c->xorig->p, obytes, hipMemcpyDeviceToHost, c->stream));
     HVCK(hipStreamSynchronize(c->stream));
     HvBox box{};
     double vol = 1.0;
@@ -2870,19 +2957,10 @@ static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
         for (int k = 0; k < D; ++k)
             if ((mask >> k) & 1) grow *= 1.0 + 2.0 * delta / (box.hi[k] - box.lo[k]);
         const size_t expect = (size_t)std::min(1.1 * grow * (double)n + 1024.0, (double)(1ll << 30));
-        std::vector<double> ext;
-        std::vector<int32_t> iorig;
-        std::vector<int8_t> ishift;
-        ext.reserve(expect * D);
-        iorig.reserve(expect);
-        ishift.reserve(expect * D);
-        ext.assign(hx.begin(), hx.end());
-        iorig.resize((size_t)n);
-        ishift.assign((size_t)n * D, 0);
-        for (int64_t i = 0; i < n; ++i) iorig[(size_t)i] = (int32_t)i;
-        int lo_s[6], cnt_s[6];
-        for (int64_t i = 0; i < n; ++i) {
-            const double* x = &hx[(size_t)i * D];
+        // admissible shifts per original and direction form one interval [lo_s, lo_s + cnt_s);
+        // labels are int8 and the point-sample integrator packs them in 4 bits: at most 7 box
+        // lengths each way
+        const auto shifts_of = [&](const double* x, int* lo_s, int* cnt_s) -> int64_t {
             int64_t combos = 1;
             for (int k = 0; k < D; ++k) {
                 lo_s[k] = 0;
@@ -2890,7 +2968,7 @@ static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
                 if (!((mask >> k) & 1)) continue;
                 const double L = box.hi[k] - box.lo[k];
                 const int m = (int)std::ceil(delta / L) + 1;
-                int a = 1, b = -1;   // the admissible shifts form one interval
+                int a = 1, b = -1;
                 for (int s = -m; s <= m; ++s) {
                     const double y = x[k] + s * L;
                     if (y >= box.lo[k] - delta && y <= box.hi[k] + delta) { a = std::min(a, s); b = std::max(b, s); }
@@ -2899,6 +2977,30 @@ static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
                 cnt_s[k] = b - a + 1;
                 combos *= cnt_s[k];
             }
+            return combos;
+        };
+        for (int k = 0; k < D; ++k)
+            if (((mask >> k) & 1) && std::ceil(delta / (box.hi[k] - box.lo[k])) > 7.0)
+                return hv_fail(c, HV_ERR_CAPACITY, "periodic images beyond 7 box lengths (too few points for this box)");
+        int lo_s[6], cnt_s[6];
+        int64_t total = n;
+        for (int64_t i = 0; i < n; ++i) {
+            total += shifts_of(&hx[(size_t)i * D], lo_s, cnt_s) - 1;
+            if (total >= (1ll << 30) - 64) return hv_fail(c, HV_ERR_CAPACITY, "too many periodic images for 32-bit keys");
+        }
+        std::vector<double> ext;
+        std::vector<int32_t> iorig;
+        std::vector<int8_t> ishift;
+        ext.reserve((size_t)total * D);
+        iorig.reserve((size_t)total);
+        ishift.reserve((size_t)total * D);
+        ext.assign(hx.begin(), hx.end());
+        iorig.resize((size_t)n);
+        ishift.assign((size_t)n * D, 0);
+        for (int64_t i = 0; i < n; ++i) iorig[(size_t)i] = (int32_t)i;
+        for (int64_t i = 0; i < n; ++i) {
+            const double* x = &hx[(size_t)i * D];
+            const int64_t combos = shifts_of(x, lo_s, cnt_s);
             for (int64_t q = 0; q < combos; ++q) {
                 int sh[6];
                 int64_t r = q;
@@ -2909,8 +3011,6 @@ static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
                     zero = zero && sh[k] == 0;
                 }
                 if (zero) continue;
-                if ((int64_t)iorig.size() >= (1ll << 30) - 64)
-                    return hv_fail(c, HV_ERR_CAPACITY, "too many periodic images for 32-bit keys");
                 for (int k = 0; k < D; ++k) {
                     ext.push_back(x[k] + sh[k] * (box.hi[k] - box.lo[k]));
                     ishift.push_back((int8_t)sh[k]);
@@ -2919,15 +3019,16 @@ static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
             }
         }
         const int64_t next = (int64_t)iorig.size();
-        double* xin = (double*)c->pool->get("xin", sizeof(double) * D * next, false, c->stream);
+        double* xin = hv_points_for_write(c, sizeof(double) * D * next);
         if (!xin) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (periodic images)");
         HVCK(hipMemcpyAsync(xin, ext.data(), sizeof(double) * D * next, hipMemcpyHostToDevice, c->stream));
         c->N = next;
         c->extended = true;
-        ++c->gen;
         c->delta = delta;
-        c->img_orig = iorig;
-        c->img_shift = ishift;
+        auto img_o = std::make_shared<const std::vector<int32_t>>(std::move(iorig));
+        auto img_s = std::make_shared<const std::vector<int8_t>>(std::move(ishift));
+        c->img_orig = img_o;
+        c->img_shift = img_s;
         // the widened box, Dirichlet, for the walk; the caller's planes come back afterwards
         for (int k = 0; k < D; ++k)
             if ((mask >> k) & 1) {
@@ -2936,7 +3037,7 @@ static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
                 c->pbc[2 * k] = c->pbc[2 * k + 1] = 0;
             }
         hv_mesh* m = nullptr;
-        int rc = hv_walk_mesh<D>(c, o, &m, (int)n);
+        int rc = hv_walk_mesh<D>(c, o, &m, (int)n, cells);
         c->pbase = pbase;
         c->pbc = pbc;
         if (rc) return rc;
@@ -2962,6 +3063,12 @@ static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
         double need = 0.0;
         std::memcpy(&need, &hb[0], sizeof need);
         std::memcpy(&c->rmax, &hb[1], sizeof need);
+        if (need <= delta * (1.0 - 1e-9) && cells) {
+            // listed cells: the walk has given each of them a vertex
+            hv_mesh_snapshot(c, m);
+            *out = m;
+            return HV_OK;
+        }
         if (need <= delta * (1.0 - 1e-9)) {
             // a cell whose own seed was dropped is reached through a walked edge shared with
             // another original's cell; one with no such neighbour would be missing: refuse
@@ -2985,11 +3092,7 @@ static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask) {
                 hv_mesh_free(m);
                 return hv_fail(c, HV_ERR_DEGENERATE, "a periodic cell was not reached by the walk");
             }
-            m->nreal = n;
-            m->gen = c->gen;
-            m->delta = delta;
-            m->img_orig = std::move(iorig);
-            m->img_shift = std::move(ishift);
+            hv_mesh_snapshot(c, m);
             *out = m;
             return HV_OK;
         }
@@ -3018,12 +3121,12 @@ int hv_box_periodic_mask(hv_ctx* c, int D, int* mask) {
 }
 
 template <int D>
-int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
+int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, const std::vector<int>* cells) {
     int mask = 0;
     const int rc = hv_box_periodic_mask(c, D, &mask);
     if (rc) return rc;
-    if (!mask) return hv_walk_mesh<D>(c, o, out, (int)c->N);
-    return periodic_mesh<D>(c, o, out, mask);
+    if (!mask) return hv_walk_mesh<D>(c, o, out, (int)c->N, cells);
+    return periodic_mesh<D>(c, o, out, mask, cells);
 }
 
 template <int D>
@@ -3178,13 +3281,15 @@ __global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const 
 // the walk, reaching the same vertex, gives it the same key; its coordinates are computed
 // from the key as every other vertex's.  status[t] = HV_ST_FOUND kept, HV_ST_DROPPED not.
 template <int D>
-__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int stride, int nwalk, int* __restrict__ vkey,
-                                                     double* __restrict__ vx, int64_t base, int* __restrict__ status) {
+__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int stride, int nwalk, const int* __restrict__ cells,
+                                                     int* __restrict__ vkey, double* __restrict__ vx, int64_t base,
+                                                     int* __restrict__ status) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nseed) return;
     const int N = g.N;
     int S[D + 1];
-    const int js = min(N - 1, t * stride + stride / 2);
+    // from the listed cells, or from every stride-th generator in Morton order
+    const int js = cells ? g.rank[cells[t]] : min(N - 1, t * stride + stride / 2);
     S[0] = g.sidx[js];
     if (S[0] >= nwalk) {   // only cells of generators below nwalk are walked
         status[t] = HV_ST_DROPPED;
@@ -3337,6 +3442,202 @@ __global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int st
         for (int k = 0; k < D; ++k) vx[id * D + k] = xn[k];
     }
     status[t] = ok ? HV_ST_FOUND : HV_ST_DROPPED;
+}
+
+// (t, id) minimum over a workgroup, smaller id first on equal t; every thread gets the result
+__device__ __forceinline__ void hv_block_argmin(double& t, int& id, double* st, int* sid) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double t2 = __shfl_xor(t, off);
+        const int i2 = __shfl_xor(id, off);
+        if (t2 < t || (t2 == t && i2 < id)) { t = t2; id = i2; }
+    }
+    if ((tid & 63) == 0) { st[tid >> 6] = t; sid[tid >> 6] = id; }
+    __syncthreads();
+    const int nw = (int)(blockDim.x >> 6);
+    t = st[0];
+    id = sid[0];
+    for (int w = 1; w < nw; ++w)
+        if (st[w] < t || (st[w] == t && sid[w] < id)) { t = st[w]; id = sid[w]; }
+    __syncthreads();
+}
+
+// Exact descent for a cell that the fast seeds and the walk left without a vertex
+// (k_seed_descent takes its candidates from the 16 nearest neighbours, which can miss the
+// first equidistant generator, and gives up when a boundary plane comes first): the same
+// moves, each step's first equidistant generator found by a scan over every generator (one
+// workgroup per cell), and boundary planes admitted as members, as the reference's descent
+// casts its rays against the mirrors as well (src/raycast.jl:45-109).  The result is certified
+// as in k_seed_descent: inside the box, away from the planes not in its key, and exactly its
+// generators on or within (1 + 1e-9) times its radius (counted over every generator).
+// status[t] = HV_ST_FOUND, or HV_ST_DROPPED when no vertex could be certified.
+template <int D>
+__global__ __launch_bounds__(256) void k_seed_exact(DevGeo g, const int* __restrict__ cells, int ncell,
+                                                    int* __restrict__ vkey, double* __restrict__ vx, int64_t base,
+                                                    int* __restrict__ status) {
+    __shared__ double s_t[4];
+    __shared__ int s_i[4];
+    __shared__ int s_cnt;
+    const int t = blockIdx.x;
+    if (t >= ncell) return;   // uniform over the workgroup
+    const int N = g.N, P = g.P, tid = threadIdx.x;
+    const int cell = cells[t];
+    int S[D + 1];
+#pragma unroll
+    for (int a = 0; a <= D; ++a) S[a] = cell;
+    double xi[D], p[D], q[D][D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) { xi[k] = g.xo[(int64_t)cell * D + k]; p[k] = xi[k]; }
+    // every thread computes the same moves from the same reduced results: `ok` stays uniform
+    bool ok = true;
+#pragma unroll
+    for (int step = 0; step < D; ++step) {
+        if (!ok) break;
+        double u[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            u[k] = (double)(hv_mix64(((uint64_t)cell << 8) + (uint64_t)(step * D + k) + 0xe5acull) >> 11) *
+                       (1.0 / 9007199254740992.0) - 0.5;
+#pragma unroll
+        for (int rep = 0; rep < 2; ++rep)
+#pragma unroll
+            for (int a = 0; a < step; ++a) {
+                double dp = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) dp = fma(u[k], q[a][k], dp);
+#pragma unroll
+                for (int k = 0; k < D; ++k) u[k] = fma(-dp, q[a][k], u[k]);
+            }
+        double nu = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) nu = fma(u[k], u[k], nu);
+        if (!(nu > 1e-12)) { ok = false; break; }
+        nu = 1.0 / sqrt(nu);
+#pragma unroll
+        for (int k = 0; k < D; ++k) u[k] *= nu;
+        double pi2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) pi2 = fma(p[k] - xi[k], p[k] - xi[k], pi2);
+        double best = INFINITY;
+        int bid = INT_MAX;
+        for (int j = tid; j < N; j += (int)blockDim.x) {
+            const int o = g.sidx[j];
+            bool member = false;
+#pragma unroll
+            for (int b = 0; b <= D; ++b) member = member || (b <= step && S[b] == o);
+            if (member) continue;
+            double den = 0.0, px2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const double x = g.xs[(int64_t)j * (D + 1) + k];
+                den = fma(u[k], x - xi[k], den);
+                px2 = fma(p[k] - x, p[k] - x, px2);
+            }
+            if (!(den > 0.0)) continue;
+            const double tt = (px2 - pi2) / (2.0 * den);
+            if (tt < best || (tt == best && o < bid)) { best = tt; bid = o; }
+        }
+        if (tid == 0)
+            for (int pl = 0; pl < P; ++pl) {
+                bool member = false;
+#pragma unroll
+                for (int b = 0; b <= D; ++b) member = member || (b <= step && S[b] == N + pl);
+                if (member) continue;
+                double nu2 = 0.0, np = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) { nu2 = fma(g.pn[pl * D + k], u[k], nu2); np = fma(g.pn[pl * D + k], p[k], np); }
+                if (!(nu2 > 0.0)) continue;
+                const double tt = (g.ph[pl] - np) / nu2;
+                if (tt < best || (tt == best && N + pl < bid)) { best = tt; bid = N + pl; }
+            }
+        hv_block_argmin(best, bid, s_t, s_i);
+        if (bid == INT_MAX || !(best > 0.0) || !(best < INFINITY)) { ok = false; break; }
+#pragma unroll
+        for (int k = 0; k < D; ++k) p[k] = fma(best, u[k], p[k]);
+        S[step + 1] = bid;
+        if (step + 1 < D) {
+            // the new member's normal (bisector or plane), orthonormalised into the face basis
+            double w[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) w[k] = bid < N ? g.xo[(int64_t)bid * D + k] - xi[k] : g.pn[(bid - N) * D + k];
+#pragma unroll
+            for (int rep = 0; rep < 2; ++rep)
+#pragma unroll
+                for (int a = 0; a < step; ++a) {
+                    double dp = 0.0;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) dp = fma(w[k], q[a][k], dp);
+#pragma unroll
+                    for (int k = 0; k < D; ++k) w[k] = fma(-dp, q[a][k], w[k]);
+                }
+            double nw = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) nw = fma(w[k], w[k], nw);
+            if (!(nw > 1e-24)) { ok = false; break; }
+            nw = 1.0 / sqrt(nw);
+#pragma unroll
+            for (int k = 0; k < D; ++k) q[step][k] = w[k] * nw;
+        }
+    }
+    int key[D + 1];
+    double xn[D];
+    int ngm = 0;
+    if (ok) {
+        // sorted key: generators first, planes (ids N + p) after
+#pragma unroll
+        for (int a = 0; a <= D; ++a) key[a] = S[a];
+        for (int a = 1; a <= D; ++a)
+            for (int b = a; b > 0 && key[b - 1] > key[b]; --b) { const int x = key[b]; key[b] = key[b - 1]; key[b - 1] = x; }
+#pragma unroll
+        for (int a = 0; a <= D; ++a) ngm += key[a] < N ? 1 : 0;
+        ok = hv_vertex_coords<D>(g, key, xn);
+    }
+    if (ok) {
+        double r2 = 0.0, ext = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double a = xn[k] - g.xo[(int64_t)key[0] * D + k];
+            r2 = fma(a, a, r2);
+            ext = fmax(ext, g.dhi[k] - g.dlo[k]);
+        }
+        for (int pl = 0; pl < P && ok; ++pl) {
+            bool in_key = false;
+#pragma unroll
+            for (int a = 0; a <= D; ++a) in_key = in_key || key[a] == N + pl;
+            if (in_key) continue;
+            double np = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) np = fma(g.pn[pl * D + k], xn[k], np);
+            if (!(g.ph[pl] - np > 1e-9 * ext + sqrt(r2) * 1e-6)) ok = false;
+        }
+        const double rr = r2 * (1.0 + 2e-9);
+        if (tid == 0) s_cnt = 0;
+        __syncthreads();
+        int cnt = 0;
+        for (int j = tid; j < N; j += (int)blockDim.x) {
+            double d2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const double a = g.xs[(int64_t)j * (D + 1) + k] - xn[k];
+                d2 = fma(a, a, d2);
+            }
+            cnt += (d2 <= rr) ? 1 : 0;
+        }
+        if (cnt) atomicAdd(&s_cnt, cnt);
+        __syncthreads();
+        ok = ok && s_cnt == ngm;
+    }
+    if (tid == 0) {
+        if (ok) {
+            const int64_t id = base + t;
+#pragma unroll
+            for (int a = 0; a <= D; ++a) vkey[id * (D + 1) + a] = key[a];
+#pragma unroll
+            for (int k = 0; k < D; ++k) vx[id * D + k] = xn[k];
+        }
+        status[t] = ok ? HV_ST_FOUND : HV_ST_DROPPED;
+    }
 }
 
 template <int D>
@@ -3503,7 +3804,8 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     int pmask = 0;
     int rc = hv_box_periodic_mask(c, D, &pmask);
     if (rc) return rc;
-    if (pmask && !c->extended) return hv_fail(c, HV_ERR_STATE, "a periodic box needs its mesh built first");
+    if (pmask && (!c->extended || !c->img_orig || !c->img_shift))
+        return hv_fail(c, HV_ERR_STATE, "a periodic box needs its mesh built first");
     HvBox box{};
     double vbox = 1.0;
     for (int k = 0; k < D; ++k) {
@@ -3542,7 +3844,7 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
         for (int64_t i = 0; i < ne; ++i) {
             uint32_t v = 0u;
             for (int k = 0; k < D; ++k) {
-                const int sh = c->img_shift[(size_t)i * D + k];
+                const int sh = (*c->img_shift)[(size_t)i * D + k];
                 if (sh < -8 || sh > 7) return hv_fail(c, HV_ERR_CAPACITY, "periodic image shift beyond 7 box lengths");
                 v |= (uint32_t)(sh + 8) << (4 * k);
             }
@@ -3554,7 +3856,7 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
         std::vector<unsigned long long> hk(hs, ~0ull);
         std::vector<int> hv(hs, -1);
         for (int64_t i = 0; i < ne; ++i) {
-            const unsigned long long key = ((unsigned long long)(unsigned)c->img_orig[(size_t)i] << 32) | code[(size_t)i];
+            const unsigned long long key = ((unsigned long long)(unsigned)(*c->img_orig)[(size_t)i] << 32) | code[(size_t)i];
             uint64_t h = hv_mix64(key) & (hs - 1);
             while (hk[h] != ~0ull) h = (h + 1) & (hs - 1);
             hk[h] = key;
@@ -3566,7 +3868,7 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
         int* dvals = pget<int>(c, "pvals", hs);
         if (!dcode || !dorig || !dkeys || !dvals) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
         HVCK(hipMemcpyAsync(dcode, code.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice, c->stream));
-        HVCK(hipMemcpyAsync(dorig, c->img_orig.data(), sizeof(int) * ne, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipMemcpyAsync(dorig, c->img_orig->data(), sizeof(int) * ne, hipMemcpyHostToDevice, c->stream));
         HVCK(hipMemcpyAsync(dkeys, hk.data(), sizeof(unsigned long long) * hs, hipMemcpyHostToDevice, c->stream));
         HVCK(hipMemcpyAsync(dvals, hv.data(), sizeof(int) * hs, hipMemcpyHostToDevice, c->stream));
         HVCK(hipStreamSynchronize(c->stream));
@@ -3598,7 +3900,7 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     template int hv_mc_points_impl<D>(hv_ctx*, int64_t, uint64_t, double, const int64_t*,         \
                                       const uint64_t*, unsigned long long*, unsigned long long*,   \
                                       int64_t*, double*, double*);                                 \
-    template int hv_build_mesh_impl<D>(hv_ctx*, const hv_opts*, hv_mesh**);                        \
+    template int hv_build_mesh_impl<D>(hv_ctx*, const hv_opts*, hv_mesh**, const std::vector<int>*);                        \
     template int hv_mesh_export_impl<D>(const hv_mesh*, int64_t*, double*);                        \
     template int hv_raycast_batch_impl<D>(hv_ctx*, int, int64_t, const int64_t*, const int32_t*,   \
                                           const double*, int64_t*, double*, double*, int32_t*);    \

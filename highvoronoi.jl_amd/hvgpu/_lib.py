@@ -68,6 +68,7 @@ def lib():
     L.hv_set_points_device.argtypes = [vp, C.c_int, C.c_int64, vp]
     L.hv_set_boundary.argtypes = [vp, C.c_int, P(C.c_double), P(C.c_double), P(C.c_int16)]
     L.hv_build_mesh.argtypes = [vp, P(hv_opts), P(vp)]
+    L.hv_build_mesh_cells.argtypes = [vp, P(hv_opts), C.c_int64, P(C.c_int64), P(vp)]
     L.hv_build_mesh_batch.argtypes = [C.c_int, C.c_int32, P(C.c_int32), P(C.c_int64), P(P(C.c_double)),
                                       P(C.c_int32), P(P(C.c_double)), P(P(C.c_double)), P(P(C.c_int16)),
                                       P(hv_opts), C.c_int32, P(vp), P(C.c_int32)]
@@ -200,6 +201,16 @@ class Context:
         o.vertex_capacity = vertex_capacity
         m = C.c_void_p()
         self._check(lib().hv_build_mesh(self._h, C.byref(o), C.byref(m)))
+        return Mesh(self, m)
+
+    def build_mesh_cells(self, cells, precision: str = "fp64", leaf_size: int = 0) -> Mesh:
+        """The vertices of the cells `cells` (1-based generator ids) only: every vertex whose
+        key holds one of them (voronoi(mesh; Iter=cells) in the reference)."""
+        cells = np.ascontiguousarray(cells, dtype=np.int64).ravel()
+        o = _opts(precision, leaf_size)
+        m = C.c_void_p()
+        self._check(lib().hv_build_mesh_cells(self._h, C.byref(o), len(cells),
+                                              _i64(cells) if len(cells) else None, C.byref(m)))
         return Mesh(self, m)
 
     def raycast_batch(self, sig, drop, r, precision: str = "fp64"):

@@ -18,6 +18,10 @@
  *    src/raycast-types.jl:298-307).  Calls are synchronous on return.
  *  - The caller allocates every host output; variable-size outputs use the two-step
  *    counts -> export protocol.
+ *  - A mesh keeps what it was built on (generators, boundary, periodic images), so a later
+ *    hv_set_points on its context does not change it, and any context on the mesh's device
+ *    can integrate it.  Distinct meshes may be exported, integrated or freed from different
+ *    host threads at once; one mesh must not be freed while another call uses it.
  */
 #ifndef HV_H
 #define HV_H
@@ -107,6 +111,15 @@ int hv_set_boundary(hv_ctx* ctx, int nplanes, const double* base, const double* 
 /* Replaces voronoi(mesh; searcher) with threading dispatch (src/sysvoronoi.jl:21-82):
  * the whole vertex set of the tessellation, computed on the device. */
 int hv_build_mesh(hv_ctx* ctx, const hv_opts* opts, hv_mesh** out);
+/* Replaces voronoi(mesh; Iter=TODO) -> _voronoi(mesh, TODO, ...) for a subset of cells
+ * (src/sysvoronoi.jl:21-39), the call systematic_refine! makes with the new and the affected
+ * cells (src/meshrefine.jl:199,213,215): the vertices of the cells `cells` (ncells 1-based
+ * generator ids, any order, repeats ignored; in a periodic box, original ids 1..n) -- every
+ * vertex whose key holds one of them, each exactly as hv_build_mesh lists it -- found by a
+ * walk over those cells' edges only.  The polytope integrators integrate the listed cells of
+ * such a mesh (the others get volume 0 and no rows: integrate(..., relevant, ...) of
+ * src/polyintegrator.jl:33); the Monte-Carlo integrators refuse it (HV_ERR_UNSUPPORTED). */
+int hv_build_mesh_cells(hv_ctx* ctx, const hv_opts* opts, int64_t ncells, const int64_t* cells, hv_mesh** out);
 int hv_mesh_counts(const hv_mesh* mesh, int64_t* nvert, int32_t* max_sig, int64_t* nrays_unbounded);
 /* sig: max_sig x nvert int64 (1-based, sorted, 0 padded); coords: d x nvert FP64.
  * Vertices are exported sorted lexicographically by key. */
@@ -167,8 +180,9 @@ int hv_nn_halfspace_batch(hv_ctx* ctx, int64_t nq, const double* q, const double
 int hv_inrange_batch(hv_ctx* ctx, int64_t nq, const double* q, const double* radius, int64_t* ptr, int64_t* idx);
 
 /* Integration ------------------------------------------------------------ */
-/* Exact cell volumes and interface areas of a mesh built from ctx's current points and
- * boundary: the integrator=VI_POLYGON path, Polygon_Integrator + _integrate
+/* Exact cell volumes and interface areas of a mesh (of the generators and boundary it was
+ * built on; ctx, any context on the mesh's device, lends its stream and scratch memory and
+ * keeps its own points): the integrator=VI_POLYGON path, Polygon_Integrator + _integrate
  * (src/polyintegrator.jl:33, :80-468; driver src/integrate.jl:252-382; neighbours
  * src/neighbors.jl:214-258).  General position only (HV_ERR_DEGENERATE otherwise). */
 int hv_integrate_polytope(hv_ctx* ctx, const hv_mesh* mesh, hv_integral** out);

@@ -99,3 +99,52 @@ def test_batch_with_periodic_boxes_equals_single_builds():
         assert np.array_equal(o, o1) and np.array_equal(sh, sh1)
         m.free()
         m1.free()
+
+
+def test_batch_meshes_integrate_through_any_context():
+    # a mesh keeps the generators and boundary it was built on: a context that holds other
+    # points (or none) integrates batch meshes exactly as the single builds integrate
+    from hvgpu.geometry import cuboid
+    cases = [(synth.uniform(3, 300, 51), cuboid(3)), (synth.uniform(2, 250, 52), cuboid(2, periodic=[1])),
+             (synth.uniform(4, 150, 53), cuboid(4, periodic=[])), (synth.uniform(3, 200, 54), cuboid(3, periodic=[2, 3]))]
+    pts = [x for x, _ in cases]
+    planes = [b.arrays() for _, b in cases]
+    out, status = build_mesh_batch(pts, planes, precision="bf16", nworkers=4)
+    assert list(status) == [0] * len(cases)
+    other = Context(0)
+    other.set_points(synth.uniform(5, 1000, 55))          # unrelated points, other dimension
+    other.set_boundary(*synth.cube_planes(5))
+    ctx = Context(0)
+    for m, (x, b) in zip(out, cases):
+        base, normal, bc = b.arrays()
+        ctx.set_points(x)
+        ctx.set_boundary(base, normal, bc)
+        m1 = ctx.build_mesh(precision="bf16")
+        for method, kw in (("polytope", {}), ("points", {"nsamples": 200_000, "seed": 3})):
+            i_batch = other.integrate(m, method, **kw)
+            i_single = ctx.integrate(m1, method, **kw)
+            for key in ("volumes", "nbr_ptr", "neighbors", "areas"):
+                assert np.array_equal(i_batch[key], i_single[key]), (method, key)
+        m1.free()
+    # the other context still builds its own points
+    k_other = other.build_mesh()
+    assert k_other.counts()[0] > 0
+    k_other.free()
+    for m in out:
+        m.free()
+
+
+def test_mesh_survives_new_points_on_its_context():
+    xs = synth.uniform(3, 400, 61)
+    ctx = Context(0)
+    b, nrm = synth.cube_planes(3)
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    m = ctx.build_mesh()
+    want = ctx.integrate(m, "polytope")
+    ctx.set_points(synth.uniform(3, 400, 62))           # same n and d, other points
+    ctx.set_boundary(b, nrm)
+    got = ctx.integrate(m, "polytope")
+    assert np.array_equal(got["volumes"], want["volumes"])
+    assert np.array_equal(got["areas"], want["areas"])
+    m.free()
