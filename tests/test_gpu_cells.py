@@ -128,11 +128,14 @@ def test_empty_list_and_bad_ids(ctx):
         assert e.value.code == -1
 
 
-def test_integrators_refuse_a_mesh_of_some_cells(ctx):
+def test_monte_carlo_integrators_refuse_a_mesh_of_some_cells(ctx):
+    # the polytope integrators take the listed cells (tests/test_gpu_c5.py); the Monte-Carlo
+    # ones need every cell's neighbours
     xs = synth.uniform(2, 300, 42)
     setup(ctx, xs)
     m = ctx.build_mesh_cells([1, 2, 3])
-    with pytest.raises(HVError) as e:
-        ctx.integrate(m, "polytope")
-    assert e.value.code == -7
+    for method in ("montecarlo", "points"):
+        with pytest.raises(HVError) as e:
+            ctx.integrate(m, method)
+        assert e.value.code == -7
     m.free()
