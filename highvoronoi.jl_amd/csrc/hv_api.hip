@@ -106,6 +106,7 @@ hv_bind_mesh::hv_bind_mesh(hv_ctx* ctx, const hv_mesh* m)
     : c(ctx), D(ctx->D), N(ctx->N), nreal(ctx->nreal), xw(ctx->xw), P(ctx->P), pbase(ctx->pbase),
       pnormal(ctx->pnormal), pbc(ctx->pbc), extended(ctx->extended), delta(ctx->delta), rmax(ctx->rmax),
       img_orig(ctx->img_orig), img_shift(ctx->img_shift) {
+    c->tree_ok = false;
     c->D = m->D;
     c->N = m->N;
     c->nreal = m->nreal;
@@ -122,6 +123,7 @@ hv_bind_mesh::hv_bind_mesh(hv_ctx* ctx, const hv_mesh* m)
 }
 
 hv_bind_mesh::~hv_bind_mesh() {
+    c->tree_ok = false;   // a tree built meanwhile belongs to the mesh's generators
     c->D = D;
     c->N = N;
     c->nreal = nreal;
@@ -166,6 +168,10 @@ int hv_create(hv_ctx** out, int device_ordinal) {
     c->device = device_ordinal;
     c->pool = std::make_shared<DevPool>();
     c->pool->device = device_ordinal;
+    if (hv_scratch_init(device_ordinal) != HV_OK) {
+        delete c;
+        return HV_ERR_DEVICE;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return HV_ERR_DEVICE;
@@ -215,6 +221,7 @@ static int set_points_common(hv_ctx* c, int d, int64_t n, const double* xs, hipM
     c->N = n;
     c->nreal = n;
     c->extended = false;
+    c->tree_ok = false;
     c->xorig.reset();
     c->img_orig.reset();
     c->img_shift.reset();
@@ -227,6 +234,7 @@ static int restore_originals(hv_ctx* c) {
     if (!c->extended) return HV_OK;
     if (!c->xorig) return hv_fail(c, HV_ERR_STATE, "the originals of the periodic set are gone");
     c->xw = std::move(c->xorig);
+    c->tree_ok = false;
     c->N = c->nreal;
     c->extended = false;
     c->img_orig.reset();
@@ -251,6 +259,7 @@ int hv_set_boundary(hv_ctx* c, int nplanes, const double* base, const double* no
     (void)hipSetDevice(c->device);
     const int rc = restore_originals(c);
     if (rc) return rc;
+    c->tree_ok = false;
     c->P = nplanes;
     c->pbase.assign(base, base + (size_t)nplanes * D);
     c->pnormal.assign(normal, normal + (size_t)nplanes * D);
@@ -382,6 +391,16 @@ int hv_raycast_batch(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig
         return hv_fail(c, HV_ERR_ARG, "null argument");
     (void)hipSetDevice(c->device);
     HV_DISPATCH(c->D, hv_raycast_batch_impl, c, precision, nrays, sig, drop, r, out_sig, out_r, out_t, out_status)
+}
+
+int hv_raycast_batch_noalloc(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
+                             const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status) {
+    if (!c) return HV_ERR_ARG;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    if (nrays < 0 || (nrays > 0 && (!sig || !drop || !r || !out_sig || !out_r || !out_t || !out_status)))
+        return hv_fail(c, HV_ERR_ARG, "null argument");
+    (void)hipSetDevice(c->device);
+    HV_DISPATCH(c->D, hv_raycast_noalloc_impl, c, precision, nrays, sig, drop, r, out_sig, out_r, out_t, out_status)
 }
 
 int hv_nn_halfspace_batch(hv_ctx* c, int64_t nq, const double* q, const double* u, const double* cc,

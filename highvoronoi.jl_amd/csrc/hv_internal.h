@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 #include "../../include/hv.h"
+#include "hv_device.h"
 
 // Device buffer pool: tables survive between builds so a steady-state build does no
 // hipMalloc; a finished mesh takes ownership of its vertex tables and returns them.
@@ -64,6 +65,13 @@ struct hv_ctx {
     double rmax = 0.0;                   // largest circumradius of the periodic mesh
     ImgOrig img_orig;                    // per working-set id: the original (0-based)
     ImgShift img_shift;                  // per working-set id: d shifts in box lengths
+    // the search tree of the last tree build over the context's own points and planes (the
+    // ray-level entry point without allocation runs on it); cleared when they change
+    bool tree_ok = false;
+    int tree_tables = 0;                 // HV_PREC_* tables built beside it
+    int tree_leaf = 0;
+    DevGeo tree_g{};
+    std::shared_ptr<DevArr> tree_xw;
 };
 
 struct hv_mesh {
@@ -122,6 +130,8 @@ struct hv_bind_mesh {
 int hv_box_periodic_mask(hv_ctx* c, int D, int* mask);
 
 int hv_fail(hv_ctx* c, int code, const std::string& msg);
+// per-device address of the scratch array of hv_raycast_batch_noalloc (resolved once per device)
+int hv_scratch_init(int device);
 
 template <int D>
 int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, const std::vector<int>* cells);
@@ -132,6 +142,9 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
 template <int D>
 int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
                           const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status);
+template <int D>
+int hv_raycast_noalloc_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
+                            const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status);
 template <int D>
 int hv_nn_halfspace_impl(hv_ctx* c, int64_t nq, const double* q, const double* u, const double* cc,
                          int64_t* idx, double* dist);

@@ -17,6 +17,7 @@
 #include "hv_device.h"
 #include "hv_internal.h"
 #include <hipcub/hipcub.hpp>
+#include <mutex>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -2210,6 +2211,16 @@ int build_tree(hv_ctx* c, int L, Tree& T, int tables = HV_PREC_FP64) {
 }
 
 // axis-aligned box check: planes 2k (x_k = hi, +e_k) and 2k+1 (x_k = lo, -e_k)
+// the tree just built over the context's own points and planes, for the entry point without
+// allocation (its tables stay in the pool until the next tree build replaces them)
+void record_tree(hv_ctx* c, const DevGeo& g, int tables, int L) {
+    c->tree_ok = true;
+    c->tree_tables = tables;
+    c->tree_leaf = L;
+    c->tree_g = g;
+    c->tree_xw = c->xw;
+}
+
 int check_box(hv_ctx* c, int D) {
     if (c->P != 2 * D) return hv_fail(c, HV_ERR_UNSUPPORTED, "the device walk needs the 2d planes of a cuboid boundary");
     for (int k = 0; k < D; ++k)
@@ -2389,6 +2400,7 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     Tree T;
     rc = build_tree<D>(c, L, T, prec);
     if (rc) return rc;
+    if (!c->extended) record_tree(c, T.g, prec, L);
     DevGeo& g = T.g;
     const int nseed = 1 << D;
     // the listed cells (device copy) and their mask over the generators
@@ -3669,6 +3681,7 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     Tree T;
     rc = build_tree<D>(c, 32, T, precision);
     if (rc) return rc;
+    record_tree(c, T.g, precision, 32);
     DevGeo& g = T.g;
     int* vkey = pget<int>(c, "b_vkey", (size_t)2 * nrays * (D + 1));
     double* vx = pget<double>(c, "b_vx", (size_t)2 * nrays * D);
@@ -3732,6 +3745,167 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
         for (int q = 0; q < D; ++q) out_r[(int64_t)k * D + q] = s > 0 ? hx[rid * D + q] : NAN;
         out_t[k] = s > 0 ? ht[rid] : INFINITY;
     }
+    return HV_OK;
+}
+
+// ---- ray-level entry point without allocation -----------------------------------------------
+// Device scratch of hv_raycast_batch_noalloc: per-ray state for at most HV_NA_MAX_RAYS rays (in
+// any dimension up to 6) and the ray sort's temporary storage.  One copy per GPU, held until the
+// process ends; host code takes its address from hipGetSymbolAddress (per device, once, when a
+// context is created there), never from the symbol.  Calls on one GPU share it: two calls that
+// overlap on one GPU race.
+constexpr int HV_NA_MAX_RAYS = 32768;
+constexpr size_t HV_NA_BYTES = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[HV_NA_BYTES];
+
+static std::mutex hv_na_mu;
+static void* hv_na_ptr[64] = {};   // per device ordinal: the address of `scratch`
+
+int hv_scratch_init(int device) {
+    if (device < 0 || device >= 64) return HV_ERR_ARG;
+    std::lock_guard<std::mutex> lk(hv_na_mu);
+    if (hv_na_ptr[device]) return HV_OK;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return HV_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return HV_ERR_DEVICE;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return HV_ERR_DEVICE;
+    }
+    hv_na_ptr[device] = p;
+    return HV_OK;
+}
+
+// carve the scratch: 256-byte aligned pieces; false when they do not fit
+struct NaCarve {
+    unsigned char* base;
+    size_t off = 0, cap;
+    template <typename T>
+    T* take(size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = (T*)(base + off);
+        off += n * sizeof(T);
+        return p;
+    }
+    bool ok() const { return off <= cap; }
+};
+
+// caller's rays (1-based int64 keys, drop positions, vertices) -> the walk's records, checked
+template <int D>
+__global__ void k_na_rays_in(int n, const int64_t* __restrict__ sig, const int32_t* __restrict__ drop,
+                             const double* __restrict__ r, int N, int P, int* __restrict__ vkey, double* __restrict__ vx,
+                             int* __restrict__ dd, int* __restrict__ bad) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    bool ok = true;
+    int64_t prev = 0;
+#pragma unroll
+    for (int j = 0; j <= D; ++j) {
+        const int64_t e = sig[(int64_t)t * (D + 1) + j];
+        ok = ok && e >= 1 && e <= (int64_t)N + P && e > prev;
+        prev = e;
+        vkey[(int64_t)t * (D + 1) + j] = ok ? (int)(e - 1) : 0;
+    }
+    const int p = drop[t];
+    ok = ok && p >= 0 && p <= D;
+    // the walked edge holds a generator: its smallest element
+    if (ok) ok = (p == 0 ? sig[(int64_t)t * (D + 1) + 1] : sig[(int64_t)t * (D + 1)]) <= N;
+    dd[t] = ok ? p : 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) vx[(int64_t)t * D + k] = r[(int64_t)t * D + k];
+    if (!ok) atomicOr(bad, 1);
+}
+
+// results in the caller's ray order (ord: sorted position -> ray), as hv_raycast_batch returns them
+template <int D>
+__global__ void k_na_out(int n, const int* __restrict__ ord, const int* __restrict__ status, const int* __restrict__ vkey,
+                         const double* __restrict__ vx, const double* __restrict__ tout, int64_t* __restrict__ out_sig,
+                         double* __restrict__ out_r, double* __restrict__ out_t, int32_t* __restrict__ out_status) {
+    const int rid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rid >= n) return;
+    const int k = ord[rid];
+    const int s = status[rid];
+    out_status[k] = s == HV_ST_FOUND ? 0 : (s == HV_ST_TIE ? 2 : (s == HV_ST_NONE ? 1 : -1));
+#pragma unroll
+    for (int j = 0; j <= D; ++j)
+        out_sig[(int64_t)k * (D + 1) + j] = s > 0 ? (int64_t)vkey[((int64_t)n + rid) * (D + 1) + j] + 1 : 0;
+#pragma unroll
+    for (int q = 0; q < D; ++q) out_r[(int64_t)k * D + q] = s > 0 ? vx[((int64_t)n + rid) * D + q] : (double)NAN;
+    out_t[k] = s > 0 ? tout[rid] : (double)INFINITY;
+}
+
+template <int D>
+int hv_raycast_noalloc_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
+                            const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status) {
+    if (precision != HV_PREC_FP64 && precision != HV_PREC_BF16 && precision != HV_PREC_FP32)
+        return hv_fail(c, HV_ERR_UNSUPPORTED, "precision: HV_PREC_FP64, HV_PREC_FP32 or HV_PREC_BF16");
+    if (nrays > HV_NA_MAX_RAYS) return hv_fail(c, HV_ERR_CAPACITY, "at most 32768 rays per call");
+    int rc = check_box(c, D);
+    if (rc) return rc;
+    if (!c->tree_ok || c->tree_xw != c->xw || c->tree_leaf != 32 ||
+        (precision != HV_PREC_FP64 && c->tree_tables != precision))
+        return hv_fail(c, HV_ERR_STATE, "no search tree of these points for this precision: call hv_build_mesh or "
+                                        "hv_raycast_batch with it first");
+    if (nrays == 0) return HV_OK;
+    void* sp = hv_na_ptr[c->device];
+    if (!sp) return hv_fail(c, HV_ERR_STATE, "scratch not resolved on this device");
+    const int n = (int)nrays;
+    const DevGeo& g = c->tree_g;
+    NaCarve cv{(unsigned char*)sp, 0, HV_NA_BYTES};
+    int* vkey = cv.take<int>((size_t)2 * n * (D + 1));
+    double* vx = cv.take<double>((size_t)2 * n * D);
+    int* dd = cv.take<int>(n);
+    DevRays R{};
+    R.v = cv.take<int>(n);
+    R.drop = cv.take<int>(n);
+    R.u = cv.take<double>((size_t)n * D);
+    R.c = cv.take<double>(n);
+    R.du = cv.take<double>(n);
+    R.code = cv.take<uint64_t>(n);
+    R.t1 = cv.take<double>(n);
+    R.t2 = cv.take<double>(n);
+    R.i1 = cv.take<int>(n);
+    R.i2 = cv.take<int>(n);
+    uint64_t* code2 = cv.take<uint64_t>(n);
+    int* ord0 = cv.take<int>(n);
+    int* ord = cv.take<int>(n);
+    int* status = cv.take<int>(n);
+    double* tout = cv.take<double>(n);
+    int* rwin = cv.take<int>(n);
+    unsigned long long* ctr = cv.take<unsigned long long>(16);
+    int* bad = cv.take<int>(1);
+    size_t tb = 0;
+    HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, R.code, code2, ord0, ord, n, HV_RAY_SORT_LO, 60, c->stream));
+    void* tmp = cv.take<unsigned char>(tb);
+    if (!cv.ok()) return hv_fail(c, HV_ERR_CAPACITY, "the rays do not fit the scratch array");
+    HVCK(hipMemsetAsync(ctr, 0, 16 * sizeof(unsigned long long), c->stream));
+    HVCK(hipMemsetAsync(status, 0, sizeof(int) * n, c->stream));
+    HVCK(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+    k_na_rays_in<D><<<nblk(n, 256), 256, 0, c->stream>>>(n, sig, drop, r, g.N, g.P, vkey, vx, dd, bad);
+    HVCK(hipGetLastError());
+    int hbad = 0;
+    HVCK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    if (hbad) return hv_fail(c, HV_ERR_ARG, "a key is unsorted or out of range, a drop position is out of range, or "
+                                           "a walked edge holds no generator");
+    const double plane_tol = 1e-12;
+    k_setup_rays<D><<<nblk(n, 256), 256, 0, c->stream>>>(n, vkey, vx, dd, g, R, plane_tol);
+    HVCK(hipGetLastError());
+    k_ray_init<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, n, vkey, vx, plane_tol);
+    HVCK(hipGetLastError());
+    k_iota<<<nblk(n, 256), 256, 0, c->stream>>>(ord0, n);
+    HVCK(hipGetLastError());
+    HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, n, HV_RAY_SORT_LO, 60, c->stream));
+    launch_scan<D>(precision, g, R, ord, n, nullptr, vkey, vx, status, rwin, tout, plane_tol, ctr, c->stream);
+    HVCK(hipGetLastError());
+    k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, ord, n, nullptr, vkey, vx, nrays, status, rwin);
+    HVCK(hipGetLastError());
+    k_na_out<D><<<nblk(n, 256), 256, 0, c->stream>>>(n, ord, status, vkey, vx, tout, out_sig, out_r, out_t, out_status);
+    HVCK(hipGetLastError());
+    HVCK(hipStreamSynchronize(c->stream));
     return HV_OK;
 }
 
@@ -3904,6 +4078,8 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     template int hv_mesh_export_impl<D>(const hv_mesh*, int64_t*, double*);                        \
     template int hv_raycast_batch_impl<D>(hv_ctx*, int, int64_t, const int64_t*, const int32_t*,   \
                                           const double*, int64_t*, double*, double*, int32_t*);    \
+    template int hv_raycast_noalloc_impl<D>(hv_ctx*, int, int64_t, const int64_t*, const int32_t*, \
+                                            const double*, int64_t*, double*, double*, int32_t*);  \
     template int hv_nn_halfspace_impl<D>(hv_ctx*, int64_t, const double*, const double*,           \
                                          const double*, int64_t*, double*);                        \
     template int hv_inrange_impl<D>(hv_ctx*, int64_t, const double*, const double*, int64_t*, int64_t*);

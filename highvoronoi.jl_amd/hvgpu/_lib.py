@@ -79,6 +79,7 @@ def lib():
     L.hv_mesh_images.argtypes = [vp, P(C.c_int64), P(C.c_int64), P(C.c_int8)]
     L.hv_raycast_batch.argtypes = [vp, C.c_int, C.c_int64, P(C.c_int64), P(C.c_int32), P(C.c_double),
                                    P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_int32)]
+    L.hv_raycast_batch_noalloc.argtypes = [vp, C.c_int, C.c_int64, vp, vp, vp, vp, vp, vp, vp]
     L.hv_nn_halfspace_batch.argtypes = [vp, C.c_int64, P(C.c_double), P(C.c_double), P(C.c_double),
                                         P(C.c_int64), P(C.c_double)]
     L.hv_inrange_batch.argtypes = [vp, C.c_int64, P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_int64)]

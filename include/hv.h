@@ -167,6 +167,18 @@ int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, con
 int hv_raycast_batch(hv_ctx* ctx, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
                      const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).  hv_raycast_batch
+ * with every pointer in device memory on the context's device (sig, drop, r in; out_* out, the
+ * same layouts), allocating nothing: its per-ray state lives in one static device array per GPU
+ * (16 MiB, at most 32768 rays per call: HV_ERR_CAPACITY beyond), and it runs on the search tree
+ * the context's last hv_build_mesh or hv_raycast_batch built over its current points and
+ * boundary (HV_ERR_STATE when there is none, or none with the tables of this precision: a
+ * BF16 or FP32 call needs a build or a ray batch of that precision first).  Synchronous.  Calls
+ * on one GPU share the array: calls that overlap on one GPU (several host threads) race. */
+int hv_raycast_batch_noalloc(hv_ctx* ctx, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
+                             const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status);
+
 /* Half-space nearest neighbour (src/extended.jl:108-142 with the predicate of
  * src/raycast.jl:385-392): for query k, the generator minimising |x - q_k| among
  * those with <x, u_k> > c_k (c_k = -inf disables the filter).  idx is 1-based
