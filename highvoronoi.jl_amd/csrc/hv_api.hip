@@ -66,7 +66,7 @@ void DevPool::give(const std::string& name, DBuf b) {
 }
 
 DevArr::~DevArr() {
-    if (!p) return;
+    if (!p || parent) return;
     int cur = 0;
     const bool had = hipGetDevice(&cur) == hipSuccess;
     (void)hipSetDevice(device);
@@ -294,6 +294,34 @@ int hv_build_mesh_cells(hv_ctx* c, const hv_opts* o, int64_t ncells, const int64
     HV_DISPATCH(c->D, hv_build_mesh_impl, c, o, out, &list)
 }
 
+// A mesh of a batch that the fused walk takes: a cuboid in the reference's plane order without
+// periodic sides, n > d, every point inside the box (else its status is set and it is left out).
+static int fusable(int d, int64_t n, const double* xs, int npl, const double* base, const double* normal,
+                   const int16_t* bc, bool& periodic) {
+    periodic = false;
+    if (d < 2 || d > 6) return HV_ERR_UNSUPPORTED;
+    if (!xs || (npl > 0 && (!base || !normal))) return HV_ERR_ARG;
+    if (n <= d) return HV_ERR_FEW_POINTS;
+    if (n >= (1ll << 30)) return HV_ERR_UNSUPPORTED;
+    if (npl != 2 * d) return HV_ERR_UNSUPPORTED;
+    double lo[6], hi[6];
+    for (int k = 0; k < d; ++k)
+        for (int side = 0; side < 2; ++side) {
+            const int p = 2 * k + side;
+            for (int q = 0; q < d; ++q)
+                if (normal[p * d + q] != ((q == k) ? (side == 0 ? 1.0 : -1.0) : 0.0)) return HV_ERR_UNSUPPORTED;
+            if (bc && bc[p] > 0) periodic = true;
+            (side == 0 ? hi : lo)[k] = base[p * d + k];
+        }
+    if (periodic) return HV_OK;
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < d; ++k) {
+            const double x = xs[i * d + k];
+            if (!(x >= lo[k] && x <= hi[k])) return HV_ERR_DOMAIN;
+        }
+    return HV_OK;
+}
+
 int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, const int64_t* n,
                         const double* const* xs, const int32_t* nplanes, const double* const* base,
                         const double* const* normal, const int16_t* const* bc, const hv_opts* opts,
@@ -302,35 +330,86 @@ int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, con
         return HV_ERR_ARG;
     for (int k = 0; k < nmesh; ++k) {
         out[k] = nullptr;
-        status[k] = HV_ERR_DEVICE;   // overwritten by the worker that builds it
+        status[k] = HV_ERR_DEVICE;   // overwritten by the path that builds it
     }
     if (nmesh == 0) return HV_OK;
-    const int nw = std::max(1, std::min<int>(nworkers > 0 ? nworkers : 8, nmesh));
-    std::atomic<int> next{0};
-    auto work = [&]() {
-        hv_ctx* c = nullptr;
-        if (hv_create(&c, device_ordinal) != HV_OK) return;   // the other workers take its share
-        for (;;) {
-            const int k = next.fetch_add(1);
-            if (k >= nmesh) break;
-            int rc = hv_set_points(c, d[k], n[k], xs[k]);
-            if (rc == HV_OK) rc = hv_set_boundary(c, nplanes[k], base[k], normal[k], bc ? bc[k] : nullptr);
-            if (rc == HV_OK) rc = hv_build_mesh(c, opts, &out[k]);
-            // the mesh owns its vertex tables outright: the worker's pool goes with the worker
-            if (rc == HV_OK) out[k]->pool.reset();
-            status[k] = rc;
-        }
-        hv_destroy(c);
-    };
-    std::vector<std::thread> th;
-    try {
-        th.reserve(nw);
-        for (int w = 0; w < nw; ++w) th.emplace_back(work);
-    } catch (...) {
-        // fewer workers than asked (at least the ones started run every mesh); none: fail
-        if (th.empty()) return HV_ERR_DEVICE;
+    // fused: the cuboid meshes of each dimension in one walk (nworkers >= 0); the others
+    // (periodic sides, other boundaries; every mesh when nworkers < 0) on worker threads
+    std::vector<int> rest;
+    std::vector<std::vector<int>> group(7);
+    for (int k = 0; k < nmesh; ++k) {
+        if (nworkers < 0) { rest.push_back(k); continue; }
+        bool periodic = false;
+        const int rc = fusable(d[k], n[k], xs[k], nplanes[k], base[k], normal[k], bc ? bc[k] : nullptr, periodic);
+        if (rc == HV_OK && !periodic) group[d[k]].push_back(k);
+        else if (rc == HV_OK || rc == HV_ERR_UNSUPPORTED) rest.push_back(k);   // the single path decides
+        else status[k] = rc;
     }
-    for (auto& t : th) t.join();
+    bool any = false;
+    for (int dd = 2; dd <= 6; ++dd) any = any || !group[dd].empty();
+    if (any) {
+        hv_ctx* c = nullptr;
+        const int rc0 = hv_create(&c, device_ordinal);
+        for (int dd = 2; dd <= 6; ++dd) {
+            if (group[dd].empty()) continue;
+            if (rc0 != HV_OK) {
+                for (int k : group[dd]) status[k] = rc0;
+                continue;
+            }
+            Forest fo;
+            fo.nmesh = (int)group[dd].size();
+            for (int k : group[dd]) {
+                fo.xs.push_back(xs[k]);
+                fo.n.push_back((int)n[k]);
+                fo.base.push_back(base[k]);
+                fo.normal.push_back(normal[k]);
+            }
+            c->D = dd;
+            int rc = HV_OK;
+            switch (dd) {
+                case 2: rc = hv_build_forest_meshes<2>(c, opts, fo); break;
+                case 3: rc = hv_build_forest_meshes<3>(c, opts, fo); break;
+                case 4: rc = hv_build_forest_meshes<4>(c, opts, fo); break;
+                case 5: rc = hv_build_forest_meshes<5>(c, opts, fo); break;
+                default: rc = hv_build_forest_meshes<6>(c, opts, fo); break;
+            }
+            for (size_t q = 0; q < group[dd].size(); ++q) {
+                const int k = group[dd][q];
+                out[k] = rc == HV_OK ? fo.out[q] : nullptr;
+                status[k] = rc;
+            }
+        }
+        if (c) hv_destroy(c);
+    }
+    if (!rest.empty()) {
+        const int nw = std::max(1, std::min<int>(nworkers > 0 ? nworkers : (nworkers < 0 ? -nworkers : 8), (int)rest.size()));
+        std::atomic<int> next{0};
+        auto work = [&]() {
+            hv_ctx* c = nullptr;
+            if (hv_create(&c, device_ordinal) != HV_OK) return;   // the other workers take its share
+            for (;;) {
+                const int q = next.fetch_add(1);
+                if (q >= (int)rest.size()) break;
+                const int k = rest[q];
+                int rc = hv_set_points(c, d[k], n[k], xs[k]);
+                if (rc == HV_OK) rc = hv_set_boundary(c, nplanes[k], base[k], normal[k], bc ? bc[k] : nullptr);
+                if (rc == HV_OK) rc = hv_build_mesh(c, opts, &out[k]);
+                // the mesh owns its vertex tables outright: the worker's pool goes with the worker
+                if (rc == HV_OK) out[k]->pool.reset();
+                status[k] = rc;
+            }
+            hv_destroy(c);
+        };
+        std::vector<std::thread> th;
+        try {
+            th.reserve(nw);
+            for (int w = 0; w < nw; ++w) th.emplace_back(work);
+        } catch (...) {
+            // fewer workers than asked (at least the ones started run every mesh); none: fail
+            if (th.empty()) return HV_ERR_DEVICE;
+        }
+        for (auto& t : th) t.join();
+    }
     for (int k = 0; k < nmesh; ++k)
         if (status[k] != HV_OK) return status[k];
     return HV_OK;
@@ -371,7 +450,9 @@ int hv_mesh_stats(const hv_mesh* m, hv_stats_t* out) {
 int hv_mesh_free(hv_mesh* m) {
     if (!m) return HV_ERR_ARG;
     (void)hipSetDevice(m->device);
-    if (m->pool) {
+    if (m->store) {
+        // tables in the batch's shared allocation: released with the last mesh of the batch
+    } else if (m->pool) {
         m->pool->give("vkey", m->key);
         m->pool->give("vx", m->x);
         m->pool->give("ids", m->ids);

@@ -25,7 +25,10 @@
 #define HV_EDGE_FP(hash64) (((hash64) >> 37) << 35)
 
 struct DevGeo {
-    int N, P, L, nleaf, P2, NN;
+    // N: generator ids are below N and boundary plane p is N + p; n: the points of this tree
+    // (sorted positions 0 .. n-1).  Equal for one point set; for a forest of meshes in one walk
+    // each mesh's tree has its own n and its plane ids above every mesh's generators.
+    int N, n, P, L, nleaf, P2, NN;
     const double* __restrict__ xo;   // N x D
     const double* __restrict__ xs;   // N x (D+1), slot D = |x|^2
     const int* __restrict__ sidx;
@@ -53,6 +56,26 @@ struct DevGeo {
 };
 
 #define HV_KNN 16
+
+// A walk over a forest of meshes (batched meshes of one dimension): generator ids of all meshes
+// in one range, each mesh with its own tree and planes (its DevGeo), mesh of a generator by table.
+struct GeoSet {
+    const DevGeo* __restrict__ gs;   // per mesh
+    const int* __restrict__ mof;     // generator id -> mesh
+    int nmesh;
+};
+// The tree and planes a ray or vertex is worked in: the one tree (B false), or the tree of the
+// mesh its generator `gen` belongs to (B true).
+template <bool B>
+__device__ __forceinline__ const DevGeo& hv_geo(const DevGeo& g, const GeoSet& F, int gen) {
+    if constexpr (B) return F.gs[F.mof[gen]];
+    else return g;
+}
+template <bool B>
+__device__ __forceinline__ const DevGeo& hv_geo_mesh(const DevGeo& g, const GeoSet& F, int mesh) {
+    if constexpr (B) return F.gs[mesh];
+    else return g;
+}
 
 typedef unsigned int hv_u32x4 __attribute__((ext_vector_type(4)));
 typedef float hv_f32x4 __attribute__((ext_vector_type(4)));

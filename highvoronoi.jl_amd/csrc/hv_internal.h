@@ -36,6 +36,8 @@ struct DevArr {
     int device = 0;
     void* p = nullptr;
     size_t n = 0;
+    // a view into another allocation (which it keeps alive) instead of an allocation of its own
+    std::shared_ptr<DevArr> parent;
     ~DevArr();
 };
 using ImgOrig = std::shared_ptr<const std::vector<int32_t>>;
@@ -98,6 +100,26 @@ struct hv_mesh {
     ImgShift img_shift;
     // a mesh of some cells only (hv_build_mesh_cells): their 0-based ids, sorted; null for all
     std::shared_ptr<const std::vector<int>> cells;
+    // meshes of one batched walk share one allocation for their vertex tables (key, x, ids
+    // point into it); null when the mesh owns its tables
+    std::shared_ptr<DevArr> store;
+};
+
+// Meshes of one dimension walked together (hv_build_mesh_batch): inputs, the forest of their
+// trees and the resulting meshes.  Generator ids are global over the meshes (mesh k holds ids
+// goff[k] .. goff[k] + n[k] - 1) and mesh k's boundary plane p has id ntot + 2 D k + p.
+struct Forest {
+    int nmesh = 0;
+    std::vector<const double*> xs;          // host points, d x n[k]
+    std::vector<int> n;
+    std::vector<const double*> base, normal;   // host planes, d x 2d each (a cuboid)
+    int64_t ntot = 0;
+    int mbits = 1;                          // bits of a mesh index
+    std::vector<int> goff;
+    std::shared_ptr<DevArr> xo;             // every mesh's points by global id
+    std::vector<DevGeo> geo;                // host copy of each mesh's tree
+    GeoSet F{};                             // device copies (per-mesh trees, generator -> mesh)
+    std::vector<hv_mesh*> out;              // one mesh per input, null until built
 };
 
 // The context's working set for writing `bytes`: the current buffer when no mesh refers to
@@ -135,6 +157,8 @@ int hv_scratch_init(int device);
 
 template <int D>
 int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, const std::vector<int>* cells);
+template <int D>
+int hv_build_forest_meshes(hv_ctx* c, const hv_opts* o, Forest& fo);
 template <int D>
 int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, const int64_t* ptr,
                       const uint64_t* pkey, unsigned long long* ccnt, unsigned long long* pcnt, int64_t* nused,

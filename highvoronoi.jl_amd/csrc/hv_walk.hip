@@ -68,13 +68,14 @@ __global__ void k_points_in(const double* __restrict__ xin, int N, DevGeo g, dou
     idx[i] = i;
 }
 
+// xo and rank: this tree's rows (local index); sidx receives global ids (goff + local)
 template <int D>
-__global__ void k_points_sorted(const double* __restrict__ xo, const int* __restrict__ idx, int N,
+__global__ void k_points_sorted(const double* __restrict__ xo, const int* __restrict__ idx, int N, int goff,
                                 double* __restrict__ xs, int* __restrict__ sidx, int* __restrict__ rank) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
     int gi = idx[j];
-    sidx[j] = gi;
+    sidx[j] = goff + gi;
     rank[gi] = j;
     double s = 0.0;
 #pragma unroll
@@ -91,7 +92,7 @@ __global__ void k_leaf_boxes(DevGeo g, double* __restrict__ blo, double* __restr
     int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= g.P2) return;
     int node = g.P2 + l;
-    int j0 = l * g.L, j1 = min(j0 + g.L, g.N);
+    int j0 = l * g.L, j1 = min(j0 + g.L, g.n);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         double lo = 1e300, hi = -1e300;
@@ -135,7 +136,7 @@ __global__ void k_tree8_level(DevGeo g, int l, int nodes_at_l, float* __restrict
     for (int k = 0; k < D; ++k) { lo[k] = 1e30f; hi[k] = -1e30f; }
     const int child = 8 * i + c;
     if (l + 1 == g.h) {
-        const int j0 = child * g.L, j1 = min(j0 + g.L, g.N);
+        const int j0 = child * g.L, j1 = min(j0 + g.L, g.n);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             double mn = 1e300, mx = -1e300;
@@ -174,7 +175,7 @@ template <int D>
 __global__ void k_leaf_origin(DevGeo g, double* __restrict__ lorg) {
     const int leaf = blockIdx.x * blockDim.x + threadIdx.x;   // group of 8 leaves (256 points)
     if (leaf >= (g.nleaf + 7) / 8) return;
-    const int j0 = leaf * 256, j1 = min(j0 + 256, g.N);
+    const int j0 = leaf * 256, j1 = min(j0 + 256, g.n);
     double o[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -208,7 +209,7 @@ __global__ void k_point_kvec(DevGeo g, const double* __restrict__ lorg, hv_u32x4
     uint32_t sl[32];
 #pragma unroll
     for (int s = 0; s < 32; ++s) sl[s] = 0u;
-    if (j < g.N) {
+    if (j < g.n) {
         const int leaf = j >> 8;
         double xp[D], s2 = 0.0;
 #pragma unroll
@@ -240,7 +241,7 @@ __global__ void k_point_kvec(DevGeo g, const double* __restrict__ lorg, hv_u32x4
 template <int D>
 __global__ void k_point_rows32(DevGeo g, float* __restrict__ xf) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= g.N) return;
+    if (j >= g.n) return;
 #pragma unroll
     for (int k = 0; k < 8; ++k) xf[(int64_t)j * 8 + k] = k < D ? (float)g.xs[(int64_t)j * (D + 1) + k] : 0.f;
 }
@@ -252,7 +253,7 @@ __global__ void k_point_f32(DevGeo g, const double* __restrict__ lorg, float* __
     float row[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) row[k] = 0.f;
-    if (j < g.N) {
+    if (j < g.n) {
         const int grp = j >> 8;
         double s2 = 0.0;
 #pragma unroll
@@ -270,14 +271,16 @@ __global__ void k_point_f32(DevGeo g, const double* __restrict__ lorg, float* __
 }
 
 // ---------------------------------------------------------------------------------
-// seeds: one block per box corner, brute-force nearest generator (ties -> smaller index)
+// seeds: one block per box corner, brute-force nearest generator (ties -> smaller index);
+// over a forest (B), block b is corner b % 2^D of mesh b / 2^D
 
-template <int D>
-__global__ __launch_bounds__(256) void k_seed_corners(DevGeo g, int* __restrict__ vkey, double* __restrict__ vx,
+template <int D, bool B>
+__global__ __launch_bounds__(256) void k_seed_corners(DevGeo g0, GeoSet F, int* __restrict__ vkey, double* __restrict__ vx,
                                                       int64_t base, int* __restrict__ status) {
     __shared__ double sd[256];
     __shared__ int si[256];
-    const int corner = blockIdx.x;
+    const int corner = B ? (int)(blockIdx.x & ((1u << D) - 1u)) : (int)blockIdx.x;
+    const DevGeo& g = B ? F.gs[blockIdx.x >> D] : g0;
     double cp[D];
     int planes[D];
 #pragma unroll
@@ -288,14 +291,15 @@ __global__ __launch_bounds__(256) void k_seed_corners(DevGeo g, int* __restrict_
     }
     double best = 1e300;
     int bi = 0x7fffffff;
-    for (int j = threadIdx.x; j < g.N; j += blockDim.x) {
+    for (int j = threadIdx.x; j < g.n; j += blockDim.x) {
         double s = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            double t = g.xo[(int64_t)(j) * D + k] - cp[k];
+            double t = g.xs[(int64_t)(j) * (D + 1) + k] - cp[k];
             s = fma(t, t, s);
         }
-        if (s < best || (s == best && j < bi)) { best = s; bi = j; }
+        const int id = g.sidx[j];
+        if (s < best || (s == best && id < bi)) { best = s; bi = id; }
     }
     sd[threadIdx.x] = best;
     si[threadIdx.x] = bi;
@@ -312,14 +316,14 @@ __global__ __launch_bounds__(256) void k_seed_corners(DevGeo g, int* __restrict_
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        int64_t id = base + corner;
+        int64_t id = base + blockIdx.x;
         vkey[id * (D + 1)] = si[0];
         // planes are already ascending in k
 #pragma unroll
         for (int k = 0; k < D; ++k) vkey[id * (D + 1) + 1 + k] = planes[k];
 #pragma unroll
         for (int k = 0; k < D; ++k) vx[id * D + k] = cp[k];
-        status[corner] = 1;
+        status[blockIdx.x] = 1;
     }
 }
 
@@ -711,14 +715,15 @@ __global__ __launch_bounds__(256) void k_emit(const int* __restrict__ frontier, 
 
 // Direction, half-space threshold and orthogonality defect of rays [0, n) (dense: one
 // thread per emitted ray).
-template <int D>
-__global__ __launch_bounds__(256) void k_ray_dir(DevGeo g, DevRays R, int n, const int* __restrict__ vkey,
+template <int D, bool B>
+__global__ __launch_bounds__(256) void k_ray_dir(DevGeo g0, GeoSet F, DevRays R, int n, const int* __restrict__ vkey,
                                                  const double* __restrict__ vx, double plane_tol) {
     const int t = hv_xcd_block() * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int v = R.v[t], pos = R.drop[t];
     // key read straight from global memory: a private copy indexed by `pos` would live in scratch
     const int* key = vkey + (int64_t)v * (D + 1);
+    const DevGeo& g = hv_geo<B>(g0, F, key[0]);
     double r[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) r[k] = vx[(int64_t)v * D + k];
@@ -756,6 +761,9 @@ __device__ __forceinline__ void hv_wave_sync() {
 }
 
 constexpr double HV_W = 1e-7;
+// Rays are ordered by the top 32 of the 60 Morton bits of their ball centre (2^(32/d) cells
+// per axis: far finer than a level's rays are spread), half the radix passes of all 60.
+constexpr int HV_RAY_SORT_LO = 28;
 
 
 // read-only tables seen through the constant address space: uniform loads become SMEM
@@ -842,11 +850,13 @@ __device__ __forceinline__ void hv_consider(const RayGeo<D>& G, Top2& T, const d
 // src/raycast.jl:354-375, src/extended.jl:131-141) and the HV_KNN nearest generators of
 // x0 (the winner is a Delaunay neighbour of x0, usually among them).  The Morton code
 // of the resulting certificate-ball centre orders the rays into packets.
-template <int D>
-__global__ __launch_bounds__(256) void k_ray_init(DevGeo g, DevRays R, int nrays, const int* __restrict__ vkey,
+template <int D, bool B>
+__global__ __launch_bounds__(256) void k_ray_init(DevGeo g0, GeoSet F, DevRays R, int nrays, const int* __restrict__ vkey,
                                                   const double* __restrict__ vx, double plane_tol) {
     const int ray = hv_xcd_block() * blockDim.x + threadIdx.x;
     if (ray >= nrays) return;
+    const int gen0 = vkey[(int64_t)R.v[ray] * (D + 1)];   // a generator: planes sort last
+    const DevGeo& g = hv_geo<B>(g0, F, gen0);
     const int N = g.N;
     RayGeo<D> G;
     hv_load_ray<D>(g, R, ray, vkey, vx, G);
@@ -912,8 +922,8 @@ __global__ __launch_bounds__(256) void k_ray_init(DevGeo g, DevRays R, int nrays
             const float4 A = xf4[(int64_t)j * 2];
             float x[8] = {A.x, A.y, A.z, A.w, 0.f, 0.f, 0.f, 0.f};
             if constexpr (D > 4) {
-                const float4 B = xf4[(int64_t)j * 2 + 1];
-                x[4] = B.x; x[5] = B.y; x[6] = B.z; x[7] = B.w;
+                const float4 Bq = xf4[(int64_t)j * 2 + 1];
+                x[4] = Bq.x; x[5] = Bq.y; x[6] = Bq.z; x[7] = Bq.w;
             }
             float vu = 0.f, vv = 0.f, vw = 0.f;
 #pragma unroll
@@ -954,7 +964,9 @@ __global__ __launch_bounds__(256) void k_ray_init(DevGeo g, DevRays R, int nrays
     double bc[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) bc[k] = G.r[k] + (T.t1 + HV_W) * G.u[k];
-    R.code[ray] = hv_morton<D>(g, bc);
+    // over a forest: rays of one mesh together (mesh above the top Morton bits the sort uses)
+    if constexpr (B) R.code[ray] = ((uint64_t)F.mof[gen0] << 32) | (hv_morton<D>(g, bc) >> HV_RAY_SORT_LO);
+    else R.code[ray] = hv_morton<D>(g, bc);
 }
 
 // One wavefront serves 64 rays at once (a packet).  Every generator x with t(x) < tau
@@ -981,7 +993,7 @@ __global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
     __shared__ unsigned long long hms[8];
     __shared__ float px[D][64];
     const int lane = threadIdx.x;
-    const int N = g.N;
+    const int N = g.n;   // sorted positions
     const int base = hv_xcd_block() * 64;
     const int j = base + lane;
     const bool act = j < N;
@@ -1423,8 +1435,11 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
 // (4 wavefronts per SIMD): the few values that spill are the rarely used FP64 ray data.
 // A packet holds `ppw` <= 64 rays (lanes ppw..63 idle): small levels use fewer rays per
 // packet, so that there are enough packets to fill the chip and each one walks fewer leaves.
-template <int D, int MODE>
-__global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
+// Over a forest (B) the rays are ordered by mesh and a packet holds rays of one mesh: mesh m
+// holds the list positions seg[m] .. seg[m+1] and the packets pk_off[m] .. pk_off[m+1].
+template <int D, int MODE, bool B>
+__global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g0, GeoSet F, const int* __restrict__ pk_off,
+                                                  const int* __restrict__ seg, DevRays R, const int* __restrict__ order, int nrays,
                                                   const int* __restrict__ nrays_dev, int ppw,
                                                   const int* __restrict__ vkey, const double* __restrict__ vx,
                                                   int* __restrict__ status, int* __restrict__ win,
@@ -1444,11 +1459,29 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
     const int lane = threadIdx.x & 63;
     // nrays_dev (when given): the ray count in device memory, nrays then only bounds the grid
     if (nrays_dev) nrays = min(nrays, *nrays_dev);
-    const int pk = hv_xcd_block_first((nrays + ppw - 1) / ppw);
-    if (pk < 0) return;   // the whole packet is past the last ray
-    const int rid = pk * ppw + lane;
-    const bool act = lane < ppw && rid < nrays;
-    const int ray = act ? order[rid] : order[0];
+    int start, cnt, mk = 0;
+    if constexpr (B) {
+        const int pk = hv_xcd_block_first(pk_off[F.nmesh]);
+        if (pk < 0) return;   // past the last packet
+        int lo = 0, hi = F.nmesh;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (pk_off[mid] <= pk) lo = mid;
+            else hi = mid;
+        }
+        mk = __builtin_amdgcn_readfirstlane(lo);
+        start = seg[mk] + (pk - pk_off[mk]) * ppw;
+        cnt = min(ppw, seg[mk + 1] - start);
+    } else {
+        const int pk = hv_xcd_block_first((nrays + ppw - 1) / ppw);
+        if (pk < 0) return;   // the whole packet is past the last ray
+        start = pk * ppw;
+        cnt = min(ppw, nrays - start);
+    }
+    const DevGeo& g = hv_geo_mesh<B>(g0, F, mk);
+    const int rid = start + lane;
+    const bool act = lane < cnt;
+    const int ray = act ? order[rid] : order[start];
     const int N = g.N;
     RayGeo<D> G;
     hv_load_ray<D>(g, R, ray, vkey, vx, G);
@@ -1564,7 +1597,7 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
         return rr;
     };
     auto load_leaf = [&](int leaf, double* pf) {
-        const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
+        const int j0 = leaf * g.L, npts = max(0, min(g.L, g.n - j0));
         if (lane < npts) {
 #pragma unroll
             for (int k = 0; k <= D; ++k) pf[k] = g.xs[(int64_t)(j0 + lane) * (D + 1) + k];
@@ -1651,7 +1684,7 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                     const int cn = rr ? (int)((ord >> (4 * __builtin_ctz(rr))) & 7u) : -1;
                     rr &= rr - 1u;
                     const int leaf = 8 * i + cc;
-                    const int j0 = leaf * g.L, npts = max(0, min(g.L, N - j0));
+                    const int j0 = leaf * g.L, npts = max(0, min(g.L, g.n - j0));
                     // balls only shrink: re-test with the current one
                     const bool hit = act && ((mine >> cc) & 1u) && (tball == tnode || child_d2f(cc, bcf) <= R2f);
                     const unsigned long long m = __ballot(hit);
@@ -1752,7 +1785,7 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                     const int cn = rr ? (int)((ord >> (4 * __builtin_ctz(rr))) & 7u) : -1;
                     rr &= rr - 1u;
                     const int leaf = 8 * i + cc;
-                    const int j0 = leaf * 32, npts = max(0, min(32, N - j0));
+                    const int j0 = leaf * 32, npts = max(0, min(32, g.n - j0));
                     const bool hit = act && ((mine >> cc) & 1u) && (tball == tnode || child_d2f(cc, bcf) <= R2f);
                     const unsigned long long m = __ballot(hit);
                     if (m && lane < 32) {
@@ -1827,7 +1860,7 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g, D
                     const int cn = rr ? (int)((ord >> (4 * __builtin_ctz(rr))) & 7u) : -1;
                     rr &= rr - 1u;
                     const int leaf = 8 * i + cc;
-                    const int j0 = leaf * 32, npts = max(0, min(32, N - j0));
+                    const int j0 = leaf * 32, npts = max(0, min(32, g.n - j0));
                     const bool hit = act && ((mine >> cc) & 1u) && (tball == tnode || child_d2f(cc, bcf) <= R2f);
                     const unsigned long long m = __ballot(hit);
                     const hv_u32x4 ca0 = pa0, ca1 = pa1;
@@ -1988,30 +2021,40 @@ constexpr int64_t HV_MIN_PACKETS = 2048;
 // one interior seed vertex per this many generators (Morton order)
 constexpr int HV_SEED_STRIDE = 8;          // large point sets: one descent per 8 generators
 constexpr int HV_SEED_ALL_MAX = 1 << 17;   // up to this many generators, and in 2D, every one
-// Rays are ordered by the top 32 of the 60 Morton bits of their ball centre (2^(32/d) cells
-// per axis: far finer than a level's rays are spread), half the radix passes of all 60.
-constexpr int HV_RAY_SORT_LO = 28;
 static inline int hv_packet_rays(int64_t n) {
     int p = 64;
     while (p > 8 && n < (int64_t)p * HV_MIN_PACKETS) p >>= 1;
     return p;
 }
 
-template <int D>
-static void launch_scan(int precision, const DevGeo& g, const DevRays& R, const int* list, int n, const int* n_dev,
+// ppw rays per packet in nblocks workgroups (over a forest: the packet plan pk_off / seg)
+template <int D, bool B>
+static void launch_scan(int precision, const DevGeo& g, const GeoSet& F, const int* pk_off, const int* seg, int ppw,
+                        unsigned nblocks, const DevRays& R, const int* list, int n, const int* n_dev,
                         const int* vkey, const double* vx, int* status, int* win, double* tout, double plane_tol,
                         unsigned long long* ctr, hipStream_t st) {
-    const int ppw = hv_packet_rays(n);
     if (precision == HV_PREC_BF16)
-        k_raycast64<D, 1><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, n_dev, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
+        k_raycast64<D, 1, B><<<nblocks, 64, 0, st>>>(g, F, pk_off, seg, R, list, n, n_dev, ppw, vkey, vx, status, win, tout,
+                                                    plane_tol, ctr);
     else if (precision == HV_PREC_FP32)
-        k_raycast64<D, 2><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, n_dev, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
+        k_raycast64<D, 2, B><<<nblocks, 64, 0, st>>>(g, F, pk_off, seg, R, list, n, n_dev, ppw, vkey, vx, status, win, tout,
+                                                    plane_tol, ctr);
     else
-        k_raycast64<D, 0><<<nblk(n, ppw), 64, 0, st>>>(g, R, list, n, n_dev, ppw, vkey, vx, status, win, tout, plane_tol, ctr);
+        k_raycast64<D, 0, B><<<nblocks, 64, 0, st>>>(g, F, pk_off, seg, R, list, n, n_dev, ppw, vkey, vx, status, win, tout,
+                                                    plane_tol, ctr);
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_finalize(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
+static void launch_scan1(int precision, const DevGeo& g, const DevRays& R, const int* list, int n, const int* n_dev,
+                         const int* vkey, const double* vx, int* status, int* win, double* tout, double plane_tol,
+                         unsigned long long* ctr, hipStream_t st) {
+    const int ppw = hv_packet_rays(n);
+    launch_scan<D, false>(precision, g, GeoSet{}, nullptr, nullptr, ppw, nblk(n, ppw), R, list, n, n_dev, vkey, vx, status,
+                          win, tout, plane_tol, ctr, st);
+}
+
+template <int D, bool B>
+__global__ __launch_bounds__(256) void k_finalize(DevGeo g0, GeoSet F, DevRays R, const int* __restrict__ order, int nrays,
                            const int* __restrict__ nrays_dev, int* __restrict__ vkey, double* __restrict__ vx,
                            int64_t base, int* __restrict__ status, const int* __restrict__ win) {
     if (nrays_dev) nrays = min(nrays, *nrays_dev);
@@ -2024,6 +2067,7 @@ __global__ __launch_bounds__(256) void k_finalize(DevGeo g, DevRays R, const int
     const int v = R.v[ray], pos = R.drop[ray];
     // key read straight from global memory: a private copy indexed by `pos` would live in scratch
     const int* key = vkey + (int64_t)v * (D + 1);
+    const DevGeo& g = hv_geo<B>(g0, F, key[0]);
     int e[D];
     hv_edge_of<D>(key, pos, e);
     const int w1 = win[rid];
@@ -2050,10 +2094,10 @@ __global__ void k_iota(int* __restrict__ a, int n) {
 // ---------------------------------------------------------------------------------
 // host driver
 
-template <int D>
-__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int stride, int nwalk, const int* __restrict__ cells,
-                                                     int* __restrict__ vkey, double* __restrict__ vx, int64_t base,
-                                                     int* __restrict__ status);
+template <int D, bool B>
+__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int nseed, int stride, int nwalk,
+                                                     const int* __restrict__ cells, int* __restrict__ vkey,
+                                                     double* __restrict__ vx, int64_t base, int* __restrict__ status);
 template <int D>
 __global__ __launch_bounds__(256) void k_seed_exact(DevGeo g, const int* __restrict__ cells, int ncell,
                                                     int* __restrict__ vkey, double* __restrict__ vx, int64_t base,
@@ -2076,25 +2120,76 @@ T* pget(hv_ctx* c, const char* name, size_t count, bool keep = false) {
 // Points -> SoA + Morton order + leaf boxes + complete binary tree of boxes.  tables: the
 // per-point scan tables to build besides the tree (HV_PREC_BF16: BF16 K-vectors,
 // HV_PREC_FP32: FP32 relative rows; both with the leaf-group origins).
+// Where the tables of one tree go: the context's pool tables (one point set), or one mesh's
+// segments of a forest's tables (batched meshes walked together).  Generator ids are global
+// over a forest: the mesh's points are ids goff .. goff + n - 1, row goff + i of xo_base.
+struct TreeOut {
+    const double* xin = nullptr;   // the n points, original order
+    int n = 0, goff = 0;
+    int Nid = 0;                   // generator ids are below Nid; plane p of this tree is Nid + p
+    int P = 0;
+    const double* pnormal = nullptr;   // host planes (P x D)
+    const double* pbase = nullptr;
+    double *pn = nullptr, *ph = nullptr;
+    double* xo_base = nullptr;
+    double* xs = nullptr;
+    int* sidx = nullptr;
+    int* rank_base = nullptr;      // indexed by global id
+    int* knn = nullptr;
+    float* xf = nullptr;
+    double *blo = nullptr, *bhi = nullptr;
+    float* cb = nullptr;
+    double* lorg = nullptr;
+    hv_u32x4* pkv = nullptr;
+    float* pxf = nullptr;
+    int* bad = nullptr;            // points outside the planes are counted here
+    bool planes_ready = false;     // pn / ph already hold the planes
+};
+
+// Table sizes of a tree over n points in leaves of L
+struct TreeDims {
+    int nleaf, P2, NN, h;
+    int64_t ninternal;
+};
+static TreeDims tree_dims(int n, int L) {
+    TreeDims t{};
+    t.nleaf = (n + L - 1) / L;
+    t.P2 = 1;
+    while (t.P2 < t.nleaf) t.P2 <<= 1;
+    t.NN = 2 * t.P2;
+    t.h = 1;
+    int64_t cap = 8;
+    while (cap < t.nleaf) { cap *= 8; ++t.h; }
+    int64_t pw = 1;
+    t.ninternal = 0;
+    for (int l = 0; l < t.h; ++l) { t.ninternal += pw; pw *= 8; }
+    return t;
+}
+
+// Points -> Morton order + leaf boxes + complete binary tree of boxes + 8-ary child-box tree +
+// 16-NN lists; tables: the per-point scan tables to build besides (HV_PREC_BF16: BF16
+// K-vectors, HV_PREC_FP32: FP32 relative rows; both with the leaf-group origins).  Launches
+// only (the domain check is left in *o.bad).
 template <int D>
-int build_tree(hv_ctx* c, int L, Tree& T, int tables = HV_PREC_FP64) {
-    const int N = (int)c->N;
+int build_tree_core(hv_ctx* c, int L, int tables, const TreeOut& o, Tree& T) {
+    const int N = o.n;
     DevGeo& g = T.g;
     memset(&g, 0, sizeof g);
-    g.N = N;
-    g.P = c->P;
+    g.N = o.Nid;
+    g.n = N;
+    g.P = o.P;
     g.L = L;
-    g.nleaf = (N + L - 1) / L;
-    g.P2 = 1;
-    while (g.P2 < g.nleaf) g.P2 <<= 1;
-    g.NN = 2 * g.P2;
+    const TreeDims td = tree_dims(N, L);
+    g.nleaf = td.nleaf;
+    g.P2 = td.P2;
+    g.NN = td.NN;
     // domain box from the axis-aligned planes
     for (int k = 0; k < D; ++k) { g.dlo[k] = 0.0; g.dhi[k] = 1.0; }
-    for (int p = 0; p < c->P; ++p)
+    for (int p = 0; p < o.P; ++p)
         for (int k = 0; k < D; ++k) {
-            double nk = c->pnormal[p * D + k];
-            if (nk > 0.5) g.dhi[k] = c->pbase[p * D + k];
-            if (nk < -0.5) g.dlo[k] = c->pbase[p * D + k];
+            double nk = o.pnormal[p * D + k];
+            if (nk > 0.5) g.dhi[k] = o.pbase[p * D + k];
+            if (nk < -0.5) g.dlo[k] = o.pbase[p * D + k];
         }
     {
         double s = 0.0;
@@ -2104,107 +2199,120 @@ int build_tree(hv_ctx* c, int L, Tree& T, int tables = HV_PREC_FP64) {
         }
         g.xmax = std::sqrt(s);
     }
-    double* pn = pget<double>(c, "pn", std::max(1, c->P * D));
-    double* ph = pget<double>(c, "ph", std::max(1, c->P));
-    if (!pn || !ph) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (planes)");
-    std::vector<double> hph(std::max(1, c->P));
-    for (int p = 0; p < c->P; ++p) {
+    std::vector<double> hph(std::max(1, o.P));
+    for (int p = 0; p < o.P; ++p) {
         double s = 0;
-        for (int k = 0; k < D; ++k) s += c->pnormal[p * D + k] * c->pbase[p * D + k];
+        for (int k = 0; k < D; ++k) s += o.pnormal[p * D + k] * o.pbase[p * D + k];
         hph[p] = s;
     }
-    if (c->P) {
-        HVCK(hipMemcpyAsync(pn, c->pnormal.data(), sizeof(double) * c->P * D, hipMemcpyHostToDevice, c->stream));
-        HVCK(hipMemcpyAsync(ph, hph.data(), sizeof(double) * c->P, hipMemcpyHostToDevice, c->stream));
+    if (o.P && !o.planes_ready) {
+        HVCK(hipMemcpyAsync(o.pn, o.pnormal, sizeof(double) * o.P * D, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipMemcpyAsync(o.ph, hph.data(), sizeof(double) * o.P, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));   // hph is a local
     }
-    g.pn = pn;
-    g.ph = ph;
-    if (!c->xw) return hv_fail(c, HV_ERR_STATE, "no points set");
-    const double* xin = (const double*)c->xw->p;
-    double* xo = pget<double>(c, "xo", (size_t)N * D);
-    double* xs = pget<double>(c, "xs", (size_t)N * (D + 1));
-    int* sidx = pget<int>(c, "sidx", N);
+    g.pn = o.pn;
+    g.ph = o.ph;
     uint64_t* code = pget<uint64_t>(c, "code", N);
     uint64_t* code2 = pget<uint64_t>(c, "code2", N);
     int* idx = pget<int>(c, "idx", N);
     int* idx2 = pget<int>(c, "idx2", N);
-    int* bad = pget<int>(c, "bad", 1);
-    double* blo = pget<double>(c, "blo", (size_t)g.NN * D);
-    double* bhi = pget<double>(c, "bhi", (size_t)g.NN * D);
-    if (!xin || !xo || !xs || !sidx || !code || !code2 || !idx || !idx2 || !bad || !blo || !bhi)
-        return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tree)");
-    HVCK(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
-    k_points_in<D><<<nblk(N, 256), 256, 0, c->stream>>>(xin, N, g, xo, code, idx, bad);
+    if (!code || !code2 || !idx || !idx2) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tree)");
+    double* xo = o.xo_base + (int64_t)o.goff * D;
+    k_points_in<D><<<nblk(N, 256), 256, 0, c->stream>>>(o.xin, N, g, xo, code, idx, o.bad);
     HVCK(hipGetLastError());
     size_t tmpb = 0;
     HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, code, code2, idx, idx2, N, 0, 64, c->stream));
     void* tmp = c->pool->get("cubtmp", tmpb, false, c->stream);
     if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
     HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tmpb, code, code2, idx, idx2, N, 0, 64, c->stream));
-    int* rank = pget<int>(c, "rank", N);
-    int* knn = pget<int>(c, "knn", (size_t)N * HV_KNN);
-    float* xf = pget<float>(c, "xf", (size_t)N * 8);
-    if (!rank || !knn || !xf) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (knn)");
-    k_points_sorted<D><<<nblk(N, 256), 256, 0, c->stream>>>(xo, idx2, N, xs, sidx, rank);
+    k_points_sorted<D><<<nblk(N, 256), 256, 0, c->stream>>>(xo, idx2, N, o.goff, o.xs, o.sidx, o.rank_base + o.goff);
     HVCK(hipGetLastError());
-    g.xo = xo;
-    g.xs = xs;
-    g.sidx = sidx;
-    g.rank = rank;
-    g.knn = knn;
-    k_point_rows32<D><<<nblk(N, 256), 256, 0, c->stream>>>(g, xf);
+    g.xo = o.xo_base;
+    g.xs = o.xs;
+    g.sidx = o.sidx;
+    g.rank = o.rank_base;
+    g.knn = o.knn;
+    k_point_rows32<D><<<nblk(N, 256), 256, 0, c->stream>>>(g, o.xf);
     HVCK(hipGetLastError());
-    g.xf = xf;
-    g.blo = blo;
-    g.bhi = bhi;
-    k_leaf_boxes<D><<<nblk(g.P2, 256), 256, 0, c->stream>>>(g, blo, bhi);
+    g.xf = o.xf;
+    g.blo = o.blo;
+    g.bhi = o.bhi;
+    k_leaf_boxes<D><<<nblk(g.P2, 256), 256, 0, c->stream>>>(g, o.blo, o.bhi);
     HVCK(hipGetLastError());
     for (int first = g.P2 / 2; first >= 1; first /= 2) {
-        k_node_boxes<D><<<nblk(first, 256), 256, 0, c->stream>>>(first, first, g.NN, blo, bhi);
+        k_node_boxes<D><<<nblk(first, 256), 256, 0, c->stream>>>(first, first, g.NN, o.blo, o.bhi);
         HVCK(hipGetLastError());
     }
     // 8-ary tree of child boxes (levels bottom-up)
-    g.h = 1;
-    {
-        int64_t cap = 8;
-        while (cap < g.nleaf) { cap *= 8; ++g.h; }
-        int64_t ninternal = 0, pw = 1;
-        for (int l = 0; l < g.h; ++l) { ninternal += pw; pw *= 8; }
-        float* cb = pget<float>(c, "cb8", (size_t)ninternal * 8 * 2 * D);
-        if (!cb) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tree)");
-        g.cb = cb;
-        for (int l = g.h - 1; l >= 0; --l) {
-            int nodes = 1;
-            for (int q = 0; q < l; ++q) nodes *= 8;
-            k_tree8_level<D><<<nblk((int64_t)nodes * 8, 256), 256, 0, c->stream>>>(g, l, nodes, cb);
-            HVCK(hipGetLastError());
-        }
+    g.h = td.h;
+    g.cb = o.cb;
+    for (int l = g.h - 1; l >= 0; --l) {
+        int nodes = 1;
+        for (int q = 0; q < l; ++q) nodes *= 8;
+        k_tree8_level<D><<<nblk((int64_t)nodes * 8, 256), 256, 0, c->stream>>>(g, l, nodes, o.cb);
+        HVCK(hipGetLastError());
     }
-    k_knn64<D><<<nblk(N, 64), 64, 0, c->stream>>>(g, knn);
+    k_knn64<D><<<nblk(N, 64), 64, 0, c->stream>>>(g, o.knn);
     HVCK(hipGetLastError());
     if (tables != HV_PREC_FP64) {
         if (L != 32) return hv_fail(c, HV_ERR_UNSUPPORTED, "the mixed-precision scans use leaves of 32 points");
-        double* lorg = pget<double>(c, "lorg", (size_t)((g.nleaf + 7) / 8) * 8);
-        if (!lorg) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (mixed-precision tables)");
-        k_leaf_origin<D><<<nblk((g.nleaf + 7) / 8, 256), 256, 0, c->stream>>>(g, lorg);
+        k_leaf_origin<D><<<nblk((g.nleaf + 7) / 8, 256), 256, 0, c->stream>>>(g, o.lorg);
         HVCK(hipGetLastError());
-        g.lorg = lorg;
+        g.lorg = o.lorg;
         if (tables == HV_PREC_BF16) {
-            hv_u32x4* pkv = pget<hv_u32x4>(c, "pkv", (size_t)g.nleaf * 32 * 4);
-            if (!pkv) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (mixed-precision tables)");
-            k_point_kvec<D><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, c->stream>>>(g, lorg, pkv);
+            k_point_kvec<D><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, c->stream>>>(g, o.lorg, o.pkv);
             HVCK(hipGetLastError());
-            g.pkv = (const uint32_t*)pkv;
+            g.pkv = (const uint32_t*)o.pkv;
         } else {
-            float* pxf = pget<float>(c, "pxf", (size_t)g.nleaf * 32 * 8);
-            if (!pxf) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (mixed-precision tables)");
-            k_point_f32<D><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, c->stream>>>(g, lorg, pxf);
+            k_point_f32<D><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, c->stream>>>(g, o.lorg, o.pxf);
             HVCK(hipGetLastError());
-            g.pxf = pxf;
+            g.pxf = o.pxf;
         }
     }
+    return HV_OK;
+}
+
+// The tree over the context's working set, its tables in the context's pool.
+template <int D>
+int build_tree(hv_ctx* c, int L, Tree& T, int tables = HV_PREC_FP64) {
+    const int N = (int)c->N;
+    if (!c->xw) return hv_fail(c, HV_ERR_STATE, "no points set");
+    const TreeDims td = tree_dims(N, L);
+    TreeOut o;
+    o.xin = (const double*)c->xw->p;
+    o.n = N;
+    o.goff = 0;
+    o.Nid = N;
+    o.P = c->P;
+    o.pnormal = c->pnormal.data();
+    o.pbase = c->pbase.data();
+    o.pn = pget<double>(c, "pn", std::max(1, c->P * D));
+    o.ph = pget<double>(c, "ph", std::max(1, c->P));
+    o.xo_base = pget<double>(c, "xo", (size_t)N * D);
+    o.xs = pget<double>(c, "xs", (size_t)N * (D + 1));
+    o.sidx = pget<int>(c, "sidx", N);
+    o.rank_base = pget<int>(c, "rank", N);
+    o.knn = pget<int>(c, "knn", (size_t)N * HV_KNN);
+    o.xf = pget<float>(c, "xf", (size_t)N * 8);
+    o.blo = pget<double>(c, "blo", (size_t)td.NN * D);
+    o.bhi = pget<double>(c, "bhi", (size_t)td.NN * D);
+    o.cb = pget<float>(c, "cb8", (size_t)td.ninternal * 8 * 2 * D);
+    o.bad = pget<int>(c, "bad", 1);
+    if (!o.pn || !o.ph || !o.xo_base || !o.xs || !o.sidx || !o.rank_base || !o.knn || !o.xf || !o.blo || !o.bhi ||
+        !o.cb || !o.bad)
+        return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tree)");
+    if (tables != HV_PREC_FP64) {
+        o.lorg = pget<double>(c, "lorg", (size_t)((td.nleaf + 7) / 8) * 8);
+        if (tables == HV_PREC_BF16) o.pkv = pget<hv_u32x4>(c, "pkv", (size_t)td.nleaf * 32 * 4);
+        else o.pxf = pget<float>(c, "pxf", (size_t)td.nleaf * 32 * 8);
+        if (!o.lorg || (tables == HV_PREC_BF16 ? !o.pkv : !o.pxf))
+            return hv_fail(c, HV_ERR_DEVICE, "out of device memory (mixed-precision tables)");
+    }
+    HVCK(hipMemsetAsync(o.bad, 0, sizeof(int), c->stream));
+    int rc = build_tree_core<D>(c, L, tables, o, T);
+    if (rc) return rc;
     int hbad = 0;
-    HVCK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipMemcpyAsync(&hbad, o.bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HVCK(hipStreamSynchronize(c->stream));
     if (hbad) return hv_fail(c, HV_ERR_DOMAIN, "a node does not lie in the domain given by the boundary");
     return HV_OK;
@@ -2284,12 +2392,13 @@ __device__ __forceinline__ bool hv_tent_key(const DevGeo& g, const DevRays& R, c
     return true;
 }
 
-template <int D>
-__global__ __launch_bounds__(256) void k_tent_group(DevGeo g, DevRays R, int nr, int nfresh,
+template <int D, bool B>
+__global__ __launch_bounds__(256) void k_tent_group(DevGeo g0, GeoSet F, DevRays R, int nr, int nfresh,
                                                     const int* __restrict__ vkey, int* __restrict__ ttab, uint64_t tmask,
                                                     int* __restrict__ prim_of, uint8_t* __restrict__ primary) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nr) return;
+    const DevGeo& g = hv_geo<B>(g0, F, vkey[(int64_t)R.v[t] * (D + 1)]);
     int tk[D + 1];
     // carried rays (t >= nfresh) already waited once: scanned, and they claim no key
     if (t >= nfresh || !hv_tent_key<D>(g, R, vkey, t, tk)) { prim_of[t] = t; primary[t] = 1; return; }
@@ -2371,6 +2480,192 @@ __global__ void k_chunk(const int* __restrict__ ord, int nr, int c, int K, const
     keep[i] = (c == 0) ? 1 : ((rt[slot[ray]] & HV_EDGE_DONE) ? 0 : 1);
 }
 
+// ---- a forest of meshes walked together ----------------------------------------------------
+// Rays of the (mesh-ordered) list per mesh.
+template <int D>
+__global__ __launch_bounds__(256) void k_pkt_hist(const int* __restrict__ list, int n, const int* __restrict__ n_dev,
+                                                  DevRays R, const int* __restrict__ vkey, const int* __restrict__ mof,
+                                                  int* __restrict__ cnt) {
+    if (n_dev) n = min(n, *n_dev);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    atomicAdd(&cnt[mof[vkey[(int64_t)R.v[list[t]] * (D + 1)]]], 1);
+}
+
+// Prefix sums over the meshes (a wavefront, 64 meshes per step): seg = first list position of
+// each mesh's rays, pk_off = first packet of each mesh (packets of ppw rays within a mesh).
+__global__ void k_pkt_plan(const int* __restrict__ cnt, int nmesh, int ppw, int* __restrict__ seg, int* __restrict__ pk_off) {
+    const int lane = threadIdx.x;
+    int s0 = 0, p0 = 0;
+    for (int b = 0; b < nmesh; b += 64) {
+        const int m = b + lane;
+        const int cm = m < nmesh ? cnt[m] : 0;
+        int s = cm, p = (cm + ppw - 1) / ppw;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int s2 = __shfl_up(s, off), p2 = __shfl_up(p, off);
+            if (lane >= off) { s += s2; p += p2; }
+        }
+        if (m < nmesh) {
+            seg[m] = s0 + s - cm;
+            pk_off[m] = p0 + p - (cm + ppw - 1) / ppw;
+        }
+        s0 += __shfl(s, 63);
+        p0 += __shfl(p, 63);
+    }
+    if (lane == 0) {
+        seg[nmesh] = s0;
+        pk_off[nmesh] = p0;
+    }
+}
+
+// Every mesh's tree into segments of one set of tables: sorted positions of mesh k start at a
+// multiple of 256 (whole groups of 8 leaves), generator ids at goff[k].
+template <int D>
+static int build_forest(hv_ctx* c, int L, int tables, Forest& fo) {
+    const int nm = fo.nmesh;
+    const int64_t ntot = fo.ntot;
+    if (ntot + (int64_t)2 * D * nm >= (1ll << 30)) return hv_fail(c, HV_ERR_UNSUPPORTED, "too many generators in one batch");
+    std::vector<int64_t> soff(nm + 1, 0), bo(nm + 1, 0), cbo(nm + 1, 0);
+    std::vector<TreeDims> td(nm);
+    for (int k = 0; k < nm; ++k) {
+        td[k] = tree_dims(fo.n[k], L);
+        soff[k + 1] = soff[k] + ((int64_t)fo.n[k] + 255) / 256 * 256;
+        bo[k + 1] = bo[k] + (int64_t)td[k].NN * D;
+        cbo[k + 1] = cbo[k] + td[k].ninternal * 8 * 2 * D;
+    }
+    const int64_t S = soff[nm];
+    // points: packed on the host, one copy, kept by the meshes (their generator sets)
+    auto xo = std::make_shared<DevArr>();
+    xo->device = c->device;
+    if (hipMalloc(&xo->p, sizeof(double) * (size_t)std::max<int64_t>(ntot, 1) * D) != hipSuccess) {
+        (void)hipGetLastError();
+        xo->p = nullptr;
+        return hv_fail(c, HV_ERR_DEVICE, "out of device memory (batch points)");
+    }
+    xo->n = sizeof(double) * (size_t)ntot * D;
+    fo.xo = xo;
+    double* dxo = (double*)xo->p;
+    {
+        std::vector<double> hx((size_t)ntot * D);
+        for (int k = 0; k < nm; ++k)
+            std::memcpy(&hx[(size_t)fo.goff[k] * D], fo.xs[k], sizeof(double) * (size_t)fo.n[k] * D);
+        HVCK(hipMemcpyAsync(dxo, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+    }
+    const int P = 2 * D;
+    double* pn = pget<double>(c, "f_pn", (size_t)nm * P * D);
+    double* ph = pget<double>(c, "f_ph", (size_t)nm * P);
+    double* xs = pget<double>(c, "f_xs", (size_t)S * (D + 1));
+    int* sidx = pget<int>(c, "f_sidx", (size_t)S);
+    int* rank = pget<int>(c, "f_rank", (size_t)ntot);
+    int* knn = pget<int>(c, "f_knn", (size_t)S * HV_KNN);
+    float* xf = pget<float>(c, "f_xf", (size_t)S * 8);
+    double* blo = pget<double>(c, "f_blo", (size_t)bo[nm]);
+    double* bhi = pget<double>(c, "f_bhi", (size_t)bo[nm]);
+    float* cb = pget<float>(c, "f_cb", (size_t)cbo[nm]);
+    double* lorg = tables != HV_PREC_FP64 ? pget<double>(c, "f_lorg", (size_t)(S / 256) * 8) : nullptr;
+    hv_u32x4* pkv = tables == HV_PREC_BF16 ? pget<hv_u32x4>(c, "f_pkv", (size_t)S * 4) : nullptr;
+    float* pxf = tables == HV_PREC_FP32 ? pget<float>(c, "f_pxf", (size_t)S * 8) : nullptr;
+    int* bad = pget<int>(c, "f_bad", (size_t)nm);
+    int* mof = pget<int>(c, "f_mof", (size_t)ntot);
+    DevGeo* gs = pget<DevGeo>(c, "f_geo", (size_t)nm);
+    if (!pn || !ph || !xs || !sidx || !rank || !knn || !xf || !blo || !bhi || !cb || !bad || !mof || !gs ||
+        (tables != HV_PREC_FP64 && !lorg) || (tables == HV_PREC_BF16 && !pkv) || (tables == HV_PREC_FP32 && !pxf))
+        return hv_fail(c, HV_ERR_DEVICE, "out of device memory (batch trees)");
+    std::vector<double> hpn((size_t)nm * P * D), hph((size_t)nm * P);
+    std::vector<int> hmof((size_t)ntot);
+    for (int k = 0; k < nm; ++k) {
+        for (int p = 0; p < P; ++p) {
+            double h = 0.0;
+            for (int q = 0; q < D; ++q) {
+                hpn[((size_t)k * P + p) * D + q] = fo.normal[k][p * D + q];
+                h += fo.normal[k][p * D + q] * fo.base[k][p * D + q];
+            }
+            hph[(size_t)k * P + p] = h;
+        }
+        std::fill(hmof.begin() + fo.goff[k], hmof.begin() + fo.goff[k] + fo.n[k], k);
+    }
+    HVCK(hipMemcpyAsync(pn, hpn.data(), sizeof(double) * hpn.size(), hipMemcpyHostToDevice, c->stream));
+    HVCK(hipMemcpyAsync(ph, hph.data(), sizeof(double) * hph.size(), hipMemcpyHostToDevice, c->stream));
+    HVCK(hipMemcpyAsync(mof, hmof.data(), sizeof(int) * hmof.size(), hipMemcpyHostToDevice, c->stream));
+    HVCK(hipMemsetAsync(bad, 0, sizeof(int) * nm, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    fo.geo.assign(nm, DevGeo{});
+    for (int k = 0; k < nm; ++k) {
+        TreeOut o;
+        o.xin = dxo + (int64_t)fo.goff[k] * D;
+        o.n = fo.n[k];
+        o.goff = fo.goff[k];
+        o.Nid = (int)(ntot + (int64_t)P * k);
+        o.P = P;
+        o.pnormal = fo.normal[k];
+        o.pbase = fo.base[k];
+        o.pn = pn + (size_t)k * P * D;
+        o.ph = ph + (size_t)k * P;
+        o.planes_ready = true;
+        o.xo_base = dxo;
+        o.xs = xs + soff[k] * (D + 1);
+        o.sidx = sidx + soff[k];
+        o.rank_base = rank;
+        o.knn = knn + soff[k] * HV_KNN;
+        o.xf = xf + soff[k] * 8;
+        o.blo = blo + bo[k];
+        o.bhi = bhi + bo[k];
+        o.cb = cb + cbo[k];
+        o.lorg = lorg ? lorg + (soff[k] / 256) * 8 : nullptr;
+        o.pkv = pkv ? pkv + soff[k] * 4 : nullptr;
+        o.pxf = pxf ? pxf + soff[k] * 8 : nullptr;
+        o.bad = bad + k;
+        Tree T;
+        int rc = build_tree_core<D>(c, L, tables, o, T);
+        if (rc) return rc;
+        fo.geo[k] = T.g;
+    }
+    HVCK(hipMemcpyAsync(gs, fo.geo.data(), sizeof(DevGeo) * nm, hipMemcpyHostToDevice, c->stream));
+    std::vector<int> hbad(nm);
+    HVCK(hipMemcpyAsync(hbad.data(), bad, sizeof(int) * nm, hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < nm; ++k)
+        if (hbad[k]) return hv_fail(c, HV_ERR_DOMAIN, "a node does not lie in the domain given by the boundary");
+    fo.F.gs = gs;
+    fo.F.mof = mof;
+    fo.F.nmesh = nm;
+    return HV_OK;
+}
+
+// Vertices of a forest walk -> the meshes: per mesh, keys in the mesh's own ids (generator
+// goff + i -> i, plane ntot + 2 D k + p -> n_k + p).
+template <int D>
+__global__ __launch_bounds__(256) void k_split_count(const int* __restrict__ ids, int nv, const int* __restrict__ vkey,
+                                                     const int* __restrict__ mof, int* __restrict__ cnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nv) atomicAdd(&cnt[mof[vkey[(int64_t)ids[t] * (D + 1)]]], 1);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_split_scatter(const int* __restrict__ ids, int nv, const int* __restrict__ vkey,
+                                                       const double* __restrict__ vx, const int* __restrict__ mof,
+                                                       const int* __restrict__ goff, const int* __restrict__ nper,
+                                                       int64_t ntot, const int64_t* __restrict__ off,
+                                                       int* __restrict__ fill, int* __restrict__ okey,
+                                                       double* __restrict__ ox) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;
+    const int v = ids[t];
+    const int* key = vkey + (int64_t)v * (D + 1);
+    const int m = mof[key[0]];
+    const int64_t pos = off[m] + atomicAdd(&fill[m], 1);
+    const int64_t pbase = ntot + (int64_t)2 * D * m;
+#pragma unroll
+    for (int j = 0; j <= D; ++j) {
+        const int e = key[j];
+        okey[pos * (D + 1) + j] = e < ntot ? e - goff[m] : (int)(e - pbase) + nper[m];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) ox[pos * D + k] = vx[(int64_t)v * D + k];
+}
+
 // The walk over the context's working set.  Only edges with a generator below nwalk are
 // followed and only those generators seed it, so the mesh holds the vertices of the cells
 // 0 .. nwalk-1 (all of them when nwalk == N).  With a list of cells (sorted, distinct, each
@@ -2378,8 +2673,10 @@ __global__ void k_chunk(const int* __restrict__ ord, int nr, int c, int K, const
 // mesh holds exactly the vertices of those cells (each cell's edge graph is connected, and an
 // edge's end points are vertices of every cell the edge holds).  A listed cell that neither a
 // seed nor the walk reached gets an exact descent, and the walk continues from there.
-template <int D>
-int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const std::vector<int>* cells = nullptr) {
+// B: one walk over a forest of meshes of dimension D (Forest: every mesh's tree, generator ids
+// global over the meshes, plane ids above all of them); its vertices go to the forest's meshes.
+template <int D, bool B>
+int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const std::vector<int>* cells, Forest* fo) {
     hv_opts opt{};
     if (o) opt = *o;
     if (opt.precision != HV_PREC_FP64 && opt.precision != HV_PREC_BF16 && opt.precision != HV_PREC_FP32)
@@ -2390,19 +2687,30 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     const int L = opt.leaf_size > 0 ? opt.leaf_size : 32;
     const double plane_tol = opt.plane_tolerance > 0 ? opt.plane_tolerance : 1e-12;
     const int KCH = opt.reserved0 > 1 ? opt.reserved0 : 1;   // chunks per level (0: tentative-key grouping)
-    int rc = check_box(c, D);
-    if (rc) return rc;
-    const int N = (int)c->N;
+    int rc = HV_OK;
+    if constexpr (!B) {
+        rc = check_box(c, D);
+        if (rc) return rc;
+    }
+    const int N = B ? (int)fo->ntot : (int)c->N;
     const bool subset = cells != nullptr;
     const int ncell = subset ? (int)cells->size() : 0;
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1], r0 = c->ev[2], r1 = c->ev[3];
     HVCK(hipEventRecord(e0, c->stream));
     Tree T;
-    rc = build_tree<D>(c, L, T, prec);
-    if (rc) return rc;
-    if (!c->extended) record_tree(c, T.g, prec, L);
-    DevGeo& g = T.g;
-    const int nseed = 1 << D;
+    GeoSet F{};
+    if constexpr (B) {
+        rc = build_forest<D>(c, L, prec, *fo);
+        if (rc) return rc;
+        F = fo->F;
+    } else {
+        rc = build_tree<D>(c, L, T, prec);
+        if (rc) return rc;
+        if (!c->extended) record_tree(c, T.g, prec, L);
+    }
+    DevGeo& g = T.g;   // over a forest: unused (every kernel takes its mesh's tree from F)
+    const int nmesh = B ? fo->nmesh : 1;
+    const int nseed = nmesh << D;   // box corners
     // the listed cells (device copy) and their mask over the generators
     uint8_t* cmask = nullptr;
     int* dcells = nullptr;
@@ -2418,6 +2726,7 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     }
     // initial sizes from the expected vertex count (or the caller's hint); all tables grow on demand
     const double est = opt.vertex_capacity > 0 ? (double)opt.vertex_capacity
+                       : B ? vertices_per_point(D) * N
                        : subset ? vertices_per_periodic_cell(D) * ncell
                        : (nwalk < N ? vertices_per_periodic_cell(D) : vertices_per_point(D)) * nwalk;
     int64_t vcap = (int64_t)(2.0 * est) + 4 * nseed + 1024;
@@ -2537,7 +2846,8 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     // 91 against 102 ms).  Every generator's cell when only some cells are walked: their
     // union need not be connected through walked edges.
     const bool part = nwalk < N || subset;
-    const int stride = (part || D == 2 || N <= HV_SEED_ALL_MAX) ? 1 : HV_SEED_STRIDE;
+    // over a forest every generator seeds (meshes of a batch are small)
+    const int stride = (B || part || D == 2 || N <= HV_SEED_ALL_MAX) ? 1 : HV_SEED_STRIDE;
     const int ndesc = subset ? ncell : N / stride;
     const int nstart = nseed + ndesc;
     int64_t* reg_cur = pget<int64_t>(c, "regA", (size_t)nstart * (D + 1));
@@ -2550,12 +2860,12 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
         if (part) {   // a corner belongs to a cell that is not walked: status HV_ST_DROPPED (-1)
             HVCK(hipMemsetAsync(status, 0xff, sizeof(int) * nseed, c->stream));
         } else {
-            k_seed_corners<D><<<nseed, 256, 0, c->stream>>>(g, vkey, vx, vnext, status);
+            k_seed_corners<D, B><<<nseed, 256, 0, c->stream>>>(g, F, vkey, vx, vnext, status);
             HVCK(hipGetLastError());
         }
         if (ndesc > 0) {
-            k_seed_descent<D><<<nblk(ndesc, 64), 64, 0, c->stream>>>(g, ndesc, stride, nwalk, dcells, vkey, vx,
-                                                                       vnext + nseed, status + nseed);
+            k_seed_descent<D, B><<<nblk(ndesc, 64), 64, 0, c->stream>>>(g, F, ndesc, stride, nwalk, dcells, vkey, vx,
+                                                                          vnext + nseed, status + nseed);
             HVCK(hipGetLastError());
         }
         int nsel = 0;
@@ -2619,7 +2929,7 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
         if (hctr[5]) return hv_fail(c, HV_ERR_CAPACITY, "edge hash overflow");
         const int nfresh = nr;
         if (nfresh > 0) {
-            k_ray_dir<D><<<nblk(nfresh, 256), 256, 0, c->stream>>>(g, R, nfresh, vkey, vx, plane_tol);
+            k_ray_dir<D, B><<<nblk(nfresh, 256), 256, 0, c->stream>>>(g, F, R, nfresh, vkey, vx, plane_tol);
             HVCK(hipGetLastError());
         }
         if (ncar > 0) {
@@ -2636,16 +2946,18 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
         if (rc) return rc;
         nt_fresh = true;
         // start phase, then sort rays by the Morton code of their certificate-ball centre
-        k_ray_init<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, nr, vkey, vx, plane_tol);
+        k_ray_init<D, B><<<nblk(nr, 256), 256, 0, c->stream>>>(g, F, R, nr, vkey, vx, plane_tol);
         HVCK(hipGetLastError());
         {
+            // over a forest the code holds the mesh above 32 Morton bits
+            const int sb0 = B ? 0 : HV_RAY_SORT_LO, sb1 = B ? 32 + (B ? fo->mbits : 0) : 60;
             size_t tb = 0;
-            HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, R.code, code2, ord0, ord, nr, HV_RAY_SORT_LO, 60, c->stream));
+            HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, R.code, code2, ord0, ord, nr, sb0, sb1, c->stream));
             void* tmp = c->pool->get("cubtmp", tb, false, c->stream);
             if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
             k_iota<<<nblk(nr, 256), 256, 0, c->stream>>>(ord0, nr);
             HVCK(hipGetLastError());
-            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, nr, HV_RAY_SORT_LO, 60, c->stream));
+            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, nr, sb0, sb1, c->stream));
         }
         const int64_t level_begin = nlive;
         // scan a list of rays, then new vertices -> hash -> registered edges
@@ -2655,11 +2967,28 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
         auto process = [&](const int* list, int n, bool defer, const int* n_dev = nullptr) -> int {
             int r_ = ensure_vertices(n);
             if (r_) return r_;
-            HVCK(hipEventRecord(r0, c->stream));
-            launch_scan<D>(prec, g, R, list, n, n_dev, vkey, vx, status, rwin, rt, plane_tol, ctr + 0, c->stream);
+            if constexpr (B) {
+                // packets of one mesh each: rays and packets per mesh of the (mesh-ordered) list
+                const int ppw = hv_packet_rays(n);
+                int* pcnt = pget<int>(c, "pk_cnt", (size_t)nmesh);
+                int* seg = pget<int>(c, "pk_seg", (size_t)nmesh + 1);
+                int* pko = pget<int>(c, "pk_off", (size_t)nmesh + 1);
+                if (!pcnt || !seg || !pko) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (packets)");
+                HVCK(hipMemsetAsync(pcnt, 0, sizeof(int) * nmesh, c->stream));
+                k_pkt_hist<D><<<nblk(n, 256), 256, 0, c->stream>>>(list, n, n_dev, R, vkey, F.mof, pcnt);
+                HVCK(hipGetLastError());
+                k_pkt_plan<<<1, 64, 0, c->stream>>>(pcnt, nmesh, ppw, seg, pko);
+                HVCK(hipGetLastError());
+                HVCK(hipEventRecord(r0, c->stream));
+                launch_scan<D, true>(prec, g, F, pko, seg, ppw, nblk(n, ppw) + (unsigned)nmesh, R, list, n, n_dev, vkey, vx,
+                                     status, rwin, rt, plane_tol, ctr + 0, c->stream);
+            } else {
+                HVCK(hipEventRecord(r0, c->stream));
+                launch_scan1<D>(prec, g, R, list, n, n_dev, vkey, vx, status, rwin, rt, plane_tol, ctr + 0, c->stream);
+            }
             HVCK(hipGetLastError());
             HVCK(hipEventRecord(r1, c->stream));
-            k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, list, n, n_dev, vkey, vx, vnext, status, rwin);
+            k_finalize<D, B><<<nblk(n, 256), 256, 0, c->stream>>>(g, F, R, list, n, n_dev, vkey, vx, vnext, status, rwin);
             HVCK(hipGetLastError());
             int nsel = 0;
             if (defer) {
@@ -2712,7 +3041,7 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
             if (!ttab || !prim_of || !pflag || !cs || !cv || !cd || !cu || !cc || !cdu)
                 return hv_fail(c, HV_ERR_DEVICE, "out of device memory (groups)");
             HVCK(hipMemsetAsync(ttab, 0xff, tsz * sizeof(int), c->stream));
-            k_tent_group<D><<<nblk(nr, 256), 256, 0, c->stream>>>(g, R, nr, nfresh, vkey, ttab, tsz - 1, prim_of, pflag);
+            k_tent_group<D, B><<<nblk(nr, 256), 256, 0, c->stream>>>(g, F, R, nr, nfresh, vkey, ttab, tsz - 1, prim_of, pflag);
             HVCK(hipGetLastError());
             k_gather_flags<<<nblk(nr, 256), 256, 0, c->stream>>>(ord, nr, pflag, keep);
             HVCK(hipGetLastError());
@@ -2772,7 +3101,7 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
         fbeg = level_begin;
         fcount = nlive - level_begin;
     }
-    if (!subset || ncell == 0) break;
+    if (B || !subset || ncell == 0) break;
     // listed cells without a vertex: an exact descent each (phase 0), or give up (phase 1)
     std::vector<int> miss;
     {
@@ -2799,7 +3128,7 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
         int* dmiss = pget<int>(c, "cmiss", nm);
         if (!status || !dmiss) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (cells)");
         HVCK(hipMemcpyAsync(dmiss, miss.data(), sizeof(int) * nm, hipMemcpyHostToDevice, c->stream));
-        k_seed_exact<D><<<nm, 256, 0, c->stream>>>(g, dmiss, nm, vkey, vx, vnext, status);
+        if constexpr (!B) k_seed_exact<D><<<nm, 256, 0, c->stream>>>(g, dmiss, nm, vkey, vx, vnext, status);
         HVCK(hipGetLastError());
         int nsel = 0;
         rc = insert_and_select(status, nm, vnext, nsel);
@@ -2842,6 +3171,83 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     st.max_t_relerr = 0.0;
     st.ms_total = ms_all;
     st.ms_raycast = ms_ray;
+    if constexpr (B) {
+        // the forest's meshes: vertices by mesh, keys in each mesh's own ids, one allocation
+        const int nm = fo->nmesh;
+        int* cnt = pget<int>(c, "s_cnt", (size_t)nm);
+        int* fill = pget<int>(c, "s_fill", (size_t)nm);
+        int64_t* doff = pget<int64_t>(c, "s_off", (size_t)nm + 1);
+        int* dgoff = pget<int>(c, "s_goff", (size_t)nm);
+        int* dn = pget<int>(c, "s_n", (size_t)nm);
+        if (!cnt || !fill || !doff || !dgoff || !dn) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (split)");
+        HVCK(hipMemsetAsync(cnt, 0, sizeof(int) * nm, c->stream));
+        HVCK(hipMemsetAsync(fill, 0, sizeof(int) * nm, c->stream));
+        if (nlive) {
+            k_split_count<D><<<nblk(nlive, 256), 256, 0, c->stream>>>(ids, (int)nlive, vkey, F.mof, cnt);
+            HVCK(hipGetLastError());
+        }
+        std::vector<int> hc(nm);
+        HVCK(hipMemcpyAsync(hc.data(), cnt, sizeof(int) * nm, hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        std::vector<int64_t> hoff(nm + 1, 0);
+        int maxc = 1;
+        for (int k = 0; k < nm; ++k) {
+            hoff[k + 1] = hoff[k] + hc[k];
+            maxc = std::max(maxc, hc[k]);
+        }
+        const size_t kb = ((size_t)nlive * (D + 1) * sizeof(int) + 255) / 256 * 256;
+        const size_t xb = ((size_t)nlive * D * sizeof(double) + 255) / 256 * 256;
+        auto store = std::make_shared<DevArr>();
+        store->device = c->device;
+        if (hipMalloc(&store->p, kb + xb + sizeof(int) * (size_t)maxc) != hipSuccess) {
+            (void)hipGetLastError();
+            store->p = nullptr;
+            return hv_fail(c, HV_ERR_DEVICE, "out of device memory (batch meshes)");
+        }
+        store->n = kb + xb + sizeof(int) * (size_t)maxc;
+        int* okey = (int*)store->p;
+        double* ox = (double*)((char*)store->p + kb);
+        int* oid = (int*)((char*)store->p + kb + xb);
+        HVCK(hipMemcpyAsync(doff, hoff.data(), sizeof(int64_t) * (nm + 1), hipMemcpyHostToDevice, c->stream));
+        HVCK(hipMemcpyAsync(dgoff, fo->goff.data(), sizeof(int) * nm, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipMemcpyAsync(dn, fo->n.data(), sizeof(int) * nm, hipMemcpyHostToDevice, c->stream));
+        if (nlive) {
+            k_split_scatter<D><<<nblk(nlive, 256), 256, 0, c->stream>>>(ids, (int)nlive, vkey, vx, F.mof, dgoff, dn,
+                                                                        fo->ntot, doff, fill, okey, ox);
+            HVCK(hipGetLastError());
+        }
+        k_iota<<<nblk(maxc, 256), 256, 0, c->stream>>>(oid, maxc);
+        HVCK(hipGetLastError());
+        HVCK(hipStreamSynchronize(c->stream));
+        for (int k = 0; k < nm; ++k) {
+            hv_mesh* mk = new hv_mesh();
+            mk->pool = nullptr;
+            mk->store = store;
+            mk->device = c->device;
+            mk->D = D;
+            mk->N = fo->n[k];
+            mk->nv = hc[k];
+            mk->key = DBuf{okey + hoff[k] * (D + 1), sizeof(int) * (size_t)hc[k] * (D + 1)};
+            mk->x = DBuf{ox + hoff[k] * D, sizeof(double) * (size_t)hc[k] * D};
+            mk->ids = DBuf{oid, sizeof(int) * (size_t)hc[k]};
+            mk->stats = st;
+            mk->stats.vertices = hc[k];
+            auto xw = std::make_shared<DevArr>();
+            xw->device = c->device;
+            xw->p = (double*)fo->xo->p + (int64_t)fo->goff[k] * D;
+            xw->n = sizeof(double) * (size_t)fo->n[k] * D;
+            xw->parent = fo->xo;
+            mk->xw = xw;
+            mk->P = 2 * D;
+            mk->pbase.assign(fo->base[k], fo->base[k] + 2 * D * D);
+            mk->pnormal.assign(fo->normal[k], fo->normal[k] + 2 * D * D);
+            mk->pbc.assign(2 * D, 0);
+            mk->nreal = fo->n[k];
+            mk->extended = false;
+            fo->out[k] = mk;
+        }
+        return HV_OK;
+    }
     hv_mesh* m = new hv_mesh();
     m->pool = c->pool;
     m->device = c->device;
@@ -2856,6 +3262,34 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     if (subset) m->cells = std::make_shared<const std::vector<int>>(*cells);
     *out = m;
     return HV_OK;
+}
+
+template <int D>
+int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const std::vector<int>* cells = nullptr) {
+    return hv_walk_impl<D, false>(c, o, out, nwalk, cells, nullptr);
+}
+
+// Meshes of dimension D walked together: one forest of trees, one breadth-first walk whose
+// levels hold the rays of every mesh, packets of one mesh each.
+template <int D>
+int hv_build_forest_meshes(hv_ctx* c, const hv_opts* o, Forest& fo) {
+    fo.goff.assign(fo.nmesh, 0);
+    fo.ntot = 0;
+    for (int k = 0; k < fo.nmesh; ++k) {
+        fo.goff[k] = (int)fo.ntot;
+        fo.ntot += fo.n[k];
+        if (fo.ntot >= (1ll << 30)) return hv_fail(c, HV_ERR_UNSUPPORTED, "too many generators in one batch");
+    }
+    fo.mbits = 1;
+    while ((1 << fo.mbits) < fo.nmesh) ++fo.mbits;
+    fo.out.assign(fo.nmesh, nullptr);
+    if (fo.nmesh == 0) return HV_OK;
+    hv_mesh* none = nullptr;
+    const int rc = hv_walk_impl<D, true>(c, o, &none, (int)fo.ntot, nullptr, &fo);
+    if (rc)
+        for (auto& m : fo.out)
+            if (m) { hv_mesh_free(m); m = nullptr; }
+    return rc;
 }
 
 // ---- periodic boundaries ------------------------------------------------------------------
@@ -3228,7 +3662,7 @@ __global__ void k_nn_halfspace(DevGeo g, int nq, const double* __restrict__ q, c
         if (hv_box_d2<D>(g, node, p) <= best) {
             if (node < g.P2) { node = 2 * node; continue; }
             const int leaf = node - g.P2;
-            const int j0 = leaf * g.L, j1 = min(j0 + g.L, g.N);
+            const int j0 = leaf * g.L, j1 = min(j0 + g.L, g.n);
             for (int j = j0; j < j1; ++j) {
                 double d2 = 0.0, ux = 0.0;
 #pragma unroll
@@ -3265,7 +3699,7 @@ __global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const 
         if (hv_box_d2<D>(g, node, p) <= r2) {
             if (node < g.P2) { node = 2 * node; continue; }
             const int leaf = node - g.P2;
-            const int j0 = leaf * g.L, j1 = min(j0 + g.L, g.N);
+            const int j0 = leaf * g.L, j1 = min(j0 + g.L, g.n);
             for (int j = j0; j < j1; ++j) {
                 double d2 = 0.0;
 #pragma unroll
@@ -3292,16 +3726,18 @@ __global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const 
 // plane, and exactly its d + 1 generators within (1 + 1e-9) times its radius (tree count), so
 // the walk, reaching the same vertex, gives it the same key; its coordinates are computed
 // from the key as every other vertex's.  status[t] = HV_ST_FOUND kept, HV_ST_DROPPED not.
-template <int D>
-__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g, int nseed, int stride, int nwalk, const int* __restrict__ cells,
-                                                     int* __restrict__ vkey, double* __restrict__ vx, int64_t base,
-                                                     int* __restrict__ status) {
+template <int D, bool B>
+__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int nseed, int stride, int nwalk,
+                                                     const int* __restrict__ cells, int* __restrict__ vkey,
+                                                     double* __restrict__ vx, int64_t base, int* __restrict__ status) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nseed) return;
+    // over a forest (B): from every generator t, in its mesh's tree
+    const DevGeo& g = hv_geo<B>(g0, F, t);
     const int N = g.N;
     int S[D + 1];
     // from the listed cells, or from every stride-th generator in Morton order
-    const int js = cells ? g.rank[cells[t]] : min(N - 1, t * stride + stride / 2);
+    const int js = B ? g.rank[t] : (cells ? g.rank[cells[t]] : min(g.n - 1, t * stride + stride / 2));
     S[0] = g.sidx[js];
     if (S[0] >= nwalk) {   // only cells of generators below nwalk are walked
         status[t] = HV_ST_DROPPED;
@@ -3533,7 +3969,7 @@ __global__ __launch_bounds__(256) void k_seed_exact(DevGeo g, const int* __restr
         for (int k = 0; k < D; ++k) pi2 = fma(p[k] - xi[k], p[k] - xi[k], pi2);
         double best = INFINITY;
         int bid = INT_MAX;
-        for (int j = tid; j < N; j += (int)blockDim.x) {
+        for (int j = tid; j < g.n; j += (int)blockDim.x) {
             const int o = g.sidx[j];
             bool member = false;
 #pragma unroll
@@ -3627,7 +4063,7 @@ __global__ __launch_bounds__(256) void k_seed_exact(DevGeo g, const int* __restr
         if (tid == 0) s_cnt = 0;
         __syncthreads();
         int cnt = 0;
-        for (int j = tid; j < N; j += (int)blockDim.x) {
+        for (int j = tid; j < g.n; j += (int)blockDim.x) {
             double d2 = 0.0;
 #pragma unroll
             for (int k = 0; k < D; ++k) {
@@ -3716,7 +4152,7 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     const double plane_tol = 1e-12;
     k_setup_rays<D><<<nblk(nrays, 256), 256, 0, c->stream>>>((int)nrays, vkey, vx, dd, g, R, plane_tol);
     HVCK(hipGetLastError());
-    k_ray_init<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, (int)nrays, vkey, vx, plane_tol);
+    k_ray_init<D, false><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, GeoSet{}, R, (int)nrays, vkey, vx, plane_tol);
     HVCK(hipGetLastError());
     size_t tb = 0;
     HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, R.code, code2, ord0, ord, (int)nrays, HV_RAY_SORT_LO, 60, c->stream));
@@ -3725,9 +4161,10 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     k_iota<<<nblk(nrays, 256), 256, 0, c->stream>>>(ord0, (int)nrays);
     HVCK(hipGetLastError());
     HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, (int)nrays, HV_RAY_SORT_LO, 60, c->stream));
-    launch_scan<D>(precision, g, R, ord, (int)nrays, nullptr, vkey, vx, status, rwin, tout, plane_tol, ctr, c->stream);
+    launch_scan1<D>(precision, g, R, ord, (int)nrays, nullptr, vkey, vx, status, rwin, tout, plane_tol, ctr, c->stream);
     HVCK(hipGetLastError());
-    k_finalize<D><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, R, ord, (int)nrays, nullptr, vkey, vx, nrays, status, rwin);
+    k_finalize<D, false><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, GeoSet{}, R, ord, (int)nrays, nullptr, vkey, vx, nrays,
+                                                                  status, rwin);
     HVCK(hipGetLastError());
     std::vector<int> hord(nrays), hst(nrays), hk((size_t)nrays * (D + 1));
     std::vector<double> hx((size_t)nrays * D), ht(nrays);
@@ -3894,14 +4331,14 @@ int hv_raycast_noalloc_impl(hv_ctx* c, int precision, int64_t nrays, const int64
     const double plane_tol = 1e-12;
     k_setup_rays<D><<<nblk(n, 256), 256, 0, c->stream>>>(n, vkey, vx, dd, g, R, plane_tol);
     HVCK(hipGetLastError());
-    k_ray_init<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, n, vkey, vx, plane_tol);
+    k_ray_init<D, false><<<nblk(n, 256), 256, 0, c->stream>>>(g, GeoSet{}, R, n, vkey, vx, plane_tol);
     HVCK(hipGetLastError());
     k_iota<<<nblk(n, 256), 256, 0, c->stream>>>(ord0, n);
     HVCK(hipGetLastError());
     HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, R.code, code2, ord0, ord, n, HV_RAY_SORT_LO, 60, c->stream));
-    launch_scan<D>(precision, g, R, ord, n, nullptr, vkey, vx, status, rwin, tout, plane_tol, ctr, c->stream);
+    launch_scan1<D>(precision, g, R, ord, n, nullptr, vkey, vx, status, rwin, tout, plane_tol, ctr, c->stream);
     HVCK(hipGetLastError());
-    k_finalize<D><<<nblk(n, 256), 256, 0, c->stream>>>(g, R, ord, n, nullptr, vkey, vx, nrays, status, rwin);
+    k_finalize<D, false><<<nblk(n, 256), 256, 0, c->stream>>>(g, GeoSet{}, R, ord, n, nullptr, vkey, vx, nrays, status, rwin);
     HVCK(hipGetLastError());
     k_na_out<D><<<nblk(n, 256), 256, 0, c->stream>>>(n, ord, status, vkey, vx, tout, out_sig, out_r, out_t, out_status);
     HVCK(hipGetLastError());
@@ -4074,7 +4511,8 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     template int hv_mc_points_impl<D>(hv_ctx*, int64_t, uint64_t, double, const int64_t*,         \
                                       const uint64_t*, unsigned long long*, unsigned long long*,   \
                                       int64_t*, double*, double*);                                 \
-    template int hv_build_mesh_impl<D>(hv_ctx*, const hv_opts*, hv_mesh**, const std::vector<int>*);                        \
+    template int hv_build_mesh_impl<D>(hv_ctx*, const hv_opts*, hv_mesh**, const std::vector<int>*);   \
+    template int hv_build_forest_meshes<D>(hv_ctx*, const hv_opts*, Forest&);                        \
     template int hv_mesh_export_impl<D>(const hv_mesh*, int64_t*, double*);                        \
     template int hv_raycast_batch_impl<D>(hv_ctx*, int, int64_t, const int64_t*, const int32_t*,   \
                                           const double*, int64_t*, double*, double*, int32_t*);    \

@@ -25,7 +25,7 @@ def single(ctx, xs, precision):
     return out
 
 
-@pytest.mark.parametrize("precision,nworkers", [("fp64", 4), ("bf16", 8), ("bf16", 1)])
+@pytest.mark.parametrize("precision,nworkers", [("fp64", 4), ("bf16", 8), ("bf16", 1), ("fp32", 0), ("bf16", -4)])
 def test_batch_equals_single_builds(precision, nworkers):
     meshes = synth.batched_meshes(count=24, seed=3, nmax=1500)
     meshes.append((2, synth.uniform(2, 500, 11)))
@@ -69,6 +69,42 @@ def test_batch_reports_failures_per_mesh():
     assert np.array_equal(k0, k3) and np.array_equal(c0, c3)
     out[0].free()
     out[3].free()
+
+
+def test_c4_batch_of_512_meshes():
+    # the C4 workload: 512 meshes, d uniform in {3..6}, n log-uniform in [200, 5000], unit cubes;
+    # one fused walk per dimension.  Every mesh has the vertex count of its single build, 64 of
+    # them (16 per dimension) equal their single builds bit for bit, and a cell subset of the
+    # largest mesh of each dimension equals the oracle's vertices of those cells.
+    meshes = synth.batched_meshes(count=512, seed=3)
+    pts = [x for _, x in meshes]
+    planes = [synth.cube_planes(x.shape[1]) for x in pts]
+    out, status = build_mesh_batch(pts, planes, precision="bf16")
+    assert list(status) == [0] * len(pts)
+    ctx = Context(0)
+    per_d = {}
+    for k, (m, x) in enumerate(zip(out, pts)):
+        per_d.setdefault(x.shape[1], []).append(k)
+    check = set()
+    for dd, ks in per_d.items():
+        check.update(ks[:16])
+    for k in sorted(check):
+        kk, cc = out[k].export()
+        k1, c1 = single(ctx, pts[k], "bf16")
+        assert np.array_equal(kk, k1) and np.array_equal(cc, c1), k
+    counts = [m.counts()[0] for m in out]
+    assert sum(counts) > 100_000_000
+    for dd, ks in per_d.items():
+        big = max(ks, key=lambda k: len(pts[k]))
+        x = pts[big]
+        b, nrm = synth.cube_planes(dd)
+        kk, _ = out[big].export()
+        cells = (0, 20)
+        ok, _, _ = Oracle(x, b, nrm).voronoi(cells=cells)
+        sel = np.isin(kk, np.arange(1, 21)).any(axis=1)
+        assert np.array_equal(rows_view(kk[sel]), rows_view(ok))
+    for m in out:
+        m.free()
 
 
 def test_empty_batch():
