@@ -102,9 +102,10 @@ def measured_traffic(precision):
 
 
 def bench_batched(args):
-    """512 independent meshes: sequential hv_build_mesh calls on one context against one
-    hv_build_mesh_batch call (args.workers host threads, one stream each); every mesh of the
-    batch is compared with its sequential build (keys)."""
+    """512 independent meshes: sequential hv_build_mesh calls on one context, one
+    hv_build_mesh_batch call on host workers (per-mesh walks on args.workers threads, one
+    stream each) and one hv_build_mesh_batch call that walks the meshes of each dimension
+    together (the headline); every mesh of the batches is compared with its sequential build."""
     from hvgpu import synth
     from hvgpu._lib import Context, build_mesh_batch
 
@@ -122,11 +123,15 @@ def bench_batched(args):
             out.append(ctx.build_mesh(precision=prec))
         return out
 
-    def batched():
-        out, status = build_mesh_batch(pts, planes, precision=prec, nworkers=args.workers)
-        if any(status):
-            raise RuntimeError(f"batched build failed: {sorted(set(status.tolist()))}")
-        return out
+    def batched_with(nworkers):
+        def run():
+            out, status = build_mesh_batch(pts, planes, precision=prec, nworkers=nworkers)
+            if any(status):
+                raise RuntimeError(f"batched build failed: {sorted(set(status.tolist()))}")
+            return out
+        return run
+
+    batched = batched_with(0)
 
     def timed(fn):
         for _ in range(args.warmup):
@@ -145,18 +150,30 @@ def bench_batched(args):
             best = dt if best is None else min(best, dt)
         return best, res
 
+    legs = set(args.legs.split(","))
+    if legs != {"sequential", "workers", "fused"}:
+        # a subset of the legs (profiling runs): timings only
+        for name, fn in (("sequential", sequential), ("workers", batched_with(-args.workers)), ("fused", batched)):
+            if name in legs:
+                t, ms = timed(fn)
+                log(f"[batched {prec}] {name}: {t * 1e3:.1f} ms, {sum(m.stats()['vertices'] for m in ms)} vertices")
+                for m in ms:
+                    m.free()
+        return
     tseq, mseq = timed(sequential)
+    twrk, mwrk = timed(batched_with(-args.workers))
     tbat, mbat = timed(batched)
     nv = sum(m.stats()["vertices"] for m in mbat)
-    same = all(m.stats()["vertices"] == q.stats()["vertices"] for m, q in zip(mbat, mseq))
+    same = all(m.stats()["vertices"] == q.stats()["vertices"] == w.stats()["vertices"]
+               for m, q, w in zip(mbat, mseq, mwrk))
     same = same and all(np.array_equal(m.export()[0], q.export()[0]) for m, q in zip(mbat[:64], mseq[:64]))
-    for m in mbat + mseq:
+    for m in mbat + mseq + mwrk:
         m.free()
     dims = {d: sum(1 for dd, _ in meshes if dd == d) for d in (3, 4, 5, 6)}
     log(f"[batched {prec}] {nv} vertices in 512 meshes: sequential {tseq * 1e3:.1f} ms, "
-        f"batched ({args.workers} workers) {tbat * 1e3:.1f} ms")
+        f"per-mesh walks on {args.workers} workers {twrk * 1e3:.1f} ms, one walk per dimension {tbat * 1e3:.1f} ms")
     out = {
-        "metric": "Voronoi vertices/s, 512 independent meshes in one batched call",
+        "metric": "Voronoi vertices/s, 512 independent meshes in one batched call (one walk per dimension)",
         "value": nv / tbat,
         "unit": "vertices/s",
         "n_gpus": 1,
@@ -173,7 +190,9 @@ def bench_batched(args):
                    "meshes_per_dimension": dims, "vertices": nv, "precision": prec, "workers": args.workers},
         "sequential_vertices_per_s": nv / tseq,
         "sequential_ms": tseq * 1e3,
+        "workers_ms": twrk * 1e3,
         "speedup_vs_sequential": tseq / tbat,
+        "speedup_vs_workers": twrk / tbat,
         "identical_to_sequential": bool(same),
         "timing": "best of steps, host wall clock around the call",
     }
@@ -371,6 +390,8 @@ def main():
                     help="the 512-mesh workload instead (d in 3..6, N log-uniform in 200..5000): one "
                          "hv_build_mesh_batch call against 512 sequential hv_build_mesh calls")
     ap.add_argument("--workers", type=int, default=8, help="host workers (streams) of the batched call")
+    ap.add_argument("--legs", default="sequential,workers,fused",
+                    help="--batched: legs to run (a subset prints timings only, for profiling)")
     ap.add_argument("--mc-points", action="store_true",
                     help="the point-sample Monte-Carlo integrator instead: 6D N=50k uniform, cuboid(6) periodic, "
                          "1e8 samples; against the exact integrator at 6D N=10k periodic")

@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <atomic>
 #include <thread>
+#include <mutex>
+#include <map>
 #include <cstring>
 #include <string>
 
@@ -38,6 +40,13 @@ void* DevPool::get(const std::string& name, size_t bytes, bool keep_contents, hi
     b.p = p;
     b.n = want;
     return p;
+}
+
+size_t DevPool::total() {
+    std::lock_guard<std::mutex> lk(mu);
+    size_t t = 0;
+    for (auto& kv : bufs) t += kv.second.n;
+    return t;
 }
 
 DBuf DevPool::take(const std::string& name) {
@@ -190,6 +199,11 @@ int hv_destroy(hv_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < 4; ++i) (void)hipEventDestroy(c->ev[i]);
+    for (auto s : c->side) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+    for (auto e : c->side_ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     c->xw.reset();
     c->xorig.reset();
@@ -348,8 +362,18 @@ int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, con
     bool any = false;
     for (int dd = 2; dd <= 6; ++dd) any = any || !group[dd].empty();
     if (any) {
-        hv_ctx* c = nullptr;
-        const int rc0 = hv_create(&c, device_ordinal);
+        // one context per device for the fused walks, kept for the process (streams, events,
+        // and its device tables up to HV_BATCH_KEEP bytes, reused by the next call; the meshes
+        // own what they keep); batch calls on one device take turns
+        static std::mutex batch_mu;
+        static std::map<int, hv_ctx*> batch_ctx;
+        std::lock_guard<std::mutex> lk(batch_mu);
+        hv_ctx*& c = batch_ctx[device_ordinal];
+        int rc0 = HV_OK;
+        if (!c) {
+            rc0 = hv_create(&c, device_ordinal);
+            if (rc0 != HV_OK) c = nullptr;
+        }
         for (int dd = 2; dd <= 6; ++dd) {
             if (group[dd].empty()) continue;
             if (rc0 != HV_OK) {
@@ -379,7 +403,16 @@ int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, con
                 status[k] = rc;
             }
         }
-        if (c) hv_destroy(c);
+        if (c) {
+            (void)hipStreamSynchronize(c->stream);
+            c->tree_ok = false;
+            constexpr size_t HV_BATCH_KEEP = (size_t)24 << 30;
+            if (c->pool->total() > HV_BATCH_KEEP) {
+                auto fresh = std::make_shared<DevPool>();
+                fresh->device = c->device;
+                c->pool = fresh;   // the previous pool frees its tables here
+            }
+        }
     }
     if (!rest.empty()) {
         const int nw = std::max(1, std::min<int>(nworkers > 0 ? nworkers : (nworkers < 0 ? -nworkers : 8), (int)rest.size()));

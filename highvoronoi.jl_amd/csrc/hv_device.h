@@ -196,6 +196,27 @@ __device__ __forceinline__ int hv_wave_alloc(int* counter, bool want) {
     return base + __popcll(below);
 }
 
+// Lanes of a wavefront add 1 each to counters[key] with one atomic per distinct key (lanes of a
+// wavefront mostly share a key: rays and vertices come grouped by mesh); returns each active
+// lane's slot, the counter's old value plus its rank among the lanes of its key.  Every lane of
+// the wavefront must reach the call.
+__device__ __forceinline__ int hv_wave_add_by_key(int* counters, int key, bool active) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long todo = __ballot(active);
+    int slot = -1;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int k = __shfl(key, leader);
+        const unsigned long long same = __ballot(active && key == k) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counters[k], __popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull) slot = base + __popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return slot;
+}
+
 __device__ __forceinline__ void hv_wave_count(unsigned long long* ctr, bool cond) {
     const unsigned long long m = __ballot(cond);
     if (m == 0ull) return;

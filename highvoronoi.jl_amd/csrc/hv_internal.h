@@ -28,6 +28,7 @@ struct DevPool {
     void* get(const std::string& name, size_t bytes, bool keep_contents = false, hipStream_t s = nullptr);
     DBuf take(const std::string& name);
     void give(const std::string& name, DBuf b);
+    size_t total();   // bytes held
 };
 
 // A device allocation shared by a context and the meshes built from it (the generator
@@ -57,6 +58,9 @@ struct hv_ctx {
     std::vector<int16_t> pbc;
     std::string err;
     hipEvent_t ev[4];
+    // streams beside `stream` (created on first use): the trees of a forest are built on them
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_ev;
     // periodic boundaries: a build on planes with periodic partners replaces the working point
     // set (N points) by the nreal originals followed by their images ("ghosts"); the originals
     // stay in xorig.

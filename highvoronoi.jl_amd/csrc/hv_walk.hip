@@ -42,6 +42,17 @@ static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / 
 // ---------------------------------------------------------------------------------
 // points and tree
 
+// Over a forest: the mesh of item t, for items numbered mesh after mesh (off[m] .. off[m+1])
+__device__ __forceinline__ int hv_mesh_of_item(const int* __restrict__ off, int nmesh, int t) {
+    int lo = 0, hi = nmesh;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 template <int D>
 __global__ void k_points_in(const double* __restrict__ xin, int N, DevGeo g, double* __restrict__ xo,
                             uint64_t* __restrict__ code, int* __restrict__ idx, int* __restrict__ bad) {
@@ -171,9 +182,17 @@ __global__ void k_tree8_level(DevGeo g, int l, int nodes_at_l, float* __restrict
 // radius of the group about it, then one BF16 K-vector per sorted point relative to its
 // group origin.  Records past N are padding whose form is
 // +1e30 for every ray.
-template <int D>
-__global__ void k_leaf_origin(DevGeo g, double* __restrict__ lorg) {
-    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;   // group of 8 leaves (256 points)
+template <int D, bool B>
+__global__ void k_leaf_origin(DevGeo g0, GeoSet F, const int* __restrict__ off, double* __restrict__ lorg0) {
+    int leaf = blockIdx.x * blockDim.x + threadIdx.x;   // group of 8 leaves (256 points)
+    int mk = 0;
+    if constexpr (B) {
+        if (leaf >= off[F.nmesh]) return;
+        mk = hv_mesh_of_item(off, F.nmesh, leaf);
+        leaf -= off[mk];
+    }
+    const DevGeo& g = hv_geo_mesh<B>(g0, F, mk);
+    double* lorg = B ? const_cast<double*>(g.lorg) : lorg0;
     if (leaf >= (g.nleaf + 7) / 8) return;
     const int j0 = leaf * 256, j1 = min(j0 + 256, g.n);
     double o[D];
@@ -202,9 +221,19 @@ __global__ void k_leaf_origin(DevGeo g, double* __restrict__ lorg) {
     lorg[(int64_t)leaf * 8 + 7] = sqrt(r2) * (1.0 + 1e-12) + 1e-300;
 }
 
-template <int D>
-__global__ void k_point_kvec(DevGeo g, const double* __restrict__ lorg, hv_u32x4* __restrict__ pkv) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+template <int D, bool B>
+__global__ void k_point_kvec(DevGeo g0, GeoSet F, const int* __restrict__ off, const double* __restrict__ lorg0,
+                             hv_u32x4* __restrict__ pkv0) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int mk = 0;
+    if constexpr (B) {
+        if (j >= off[F.nmesh]) return;
+        mk = hv_mesh_of_item(off, F.nmesh, j);
+        j -= off[mk];
+    }
+    const DevGeo& g = hv_geo_mesh<B>(g0, F, mk);
+    const double* lorg = B ? g.lorg : lorg0;
+    hv_u32x4* pkv = B ? (hv_u32x4*)const_cast<uint32_t*>(g.pkv) : pkv0;
     if (j >= g.nleaf * 32) return;
     uint32_t sl[32];
 #pragma unroll
@@ -246,9 +275,19 @@ __global__ void k_point_rows32(DevGeo g, float* __restrict__ xf) {
     for (int k = 0; k < 8; ++k) xf[(int64_t)j * 8 + k] = k < D ? (float)g.xs[(int64_t)j * (D + 1) + k] : 0.f;
 }
 
-template <int D>
-__global__ void k_point_f32(DevGeo g, const double* __restrict__ lorg, float* __restrict__ pxf) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+template <int D, bool B>
+__global__ void k_point_f32(DevGeo g0, GeoSet F, const int* __restrict__ off, const double* __restrict__ lorg0,
+                            float* __restrict__ pxf0) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int mk = 0;
+    if constexpr (B) {
+        if (j >= off[F.nmesh]) return;
+        mk = hv_mesh_of_item(off, F.nmesh, j);
+        j -= off[mk];
+    }
+    const DevGeo& g = hv_geo_mesh<B>(g0, F, mk);
+    const double* lorg = B ? g.lorg : lorg0;
+    float* pxf = B ? const_cast<float*>(g.pxf) : pxf0;
     if (j >= g.nleaf * 32) return;
     float row[8];
 #pragma unroll
@@ -986,15 +1025,22 @@ __global__ __launch_bounds__(256) void k_ray_init(DevGeo g0, GeoSet F, DevRays R
 // wavefront: the packet's own 64 points first, then the 8-ary tree nearest-first, entering a
 // child when its box is within the current 16th distance of some query (FP32 test, widened);
 // each visited leaf is staged in LDS and scanned by every lane into a sorted top-16.
-template <int D>
-__global__ __launch_bounds__(64) void k_knn64(DevGeo g, int* __restrict__ knn) {
+template <int D, bool B>
+__global__ __launch_bounds__(64) void k_knn64(DevGeo g0, GeoSet F, const int* __restrict__ boff, int* __restrict__ knn0) {
     __shared__ int stk[128], stkl[128];
     __shared__ float cbx[16 * D];
     __shared__ unsigned long long hms[8];
     __shared__ float px[D][64];
     const int lane = threadIdx.x;
+    int blk = hv_xcd_block(), mk = 0;
+    if constexpr (B) {   // block of the whole grid -> (mesh, block of the mesh)
+        mk = __builtin_amdgcn_readfirstlane(hv_mesh_of_item(boff, F.nmesh, blk));
+        blk -= boff[mk];
+    }
+    const DevGeo& g = hv_geo_mesh<B>(g0, F, mk);
+    int* knn = B ? const_cast<int*>(g.knn) : knn0;
     const int N = g.n;   // sorted positions
-    const int base = hv_xcd_block() * 64;
+    const int base = blk * 64;
     const int j = base + lane;
     const bool act = j < N;
     const int jj = act ? j : N - 1;
@@ -2144,6 +2190,7 @@ struct TreeOut {
     float* pxf = nullptr;
     int* bad = nullptr;            // points outside the planes are counted here
     bool planes_ready = false;     // pn / ph already hold the planes
+    bool defer_tables = false;     // leave the 16-NN lists and the scan tables to a forest-wide pass
 };
 
 // Table sizes of a tree over n points in leaves of L
@@ -2171,7 +2218,11 @@ static TreeDims tree_dims(int n, int L) {
 // K-vectors, HV_PREC_FP32: FP32 relative rows; both with the leaf-group origins).  Launches
 // only (the domain check is left in *o.bad).
 template <int D>
-int build_tree_core(hv_ctx* c, int L, int tables, const TreeOut& o, Tree& T) {
+int build_tree_core(hv_ctx* c, int L, int tables, const TreeOut& o, Tree& T, hipStream_t st = nullptr, int slot = -1) {
+    // st: the stream to build on (the context's by default); slot >= 0: temporaries of that
+    // slot, so that trees of a forest can be built on several streams at once
+    if (!st) st = c->stream;
+    const std::string sfx = slot >= 0 ? "#" + std::to_string(slot) : std::string();
     const int N = o.n;
     DevGeo& g = T.g;
     memset(&g, 0, sizeof g);
@@ -2206,41 +2257,41 @@ int build_tree_core(hv_ctx* c, int L, int tables, const TreeOut& o, Tree& T) {
         hph[p] = s;
     }
     if (o.P && !o.planes_ready) {
-        HVCK(hipMemcpyAsync(o.pn, o.pnormal, sizeof(double) * o.P * D, hipMemcpyHostToDevice, c->stream));
-        HVCK(hipMemcpyAsync(o.ph, hph.data(), sizeof(double) * o.P, hipMemcpyHostToDevice, c->stream));
-        HVCK(hipStreamSynchronize(c->stream));   // hph is a local
+        HVCK(hipMemcpyAsync(o.pn, o.pnormal, sizeof(double) * o.P * D, hipMemcpyHostToDevice, st));
+        HVCK(hipMemcpyAsync(o.ph, hph.data(), sizeof(double) * o.P, hipMemcpyHostToDevice, st));
+        HVCK(hipStreamSynchronize(st));   // hph is a local
     }
     g.pn = o.pn;
     g.ph = o.ph;
-    uint64_t* code = pget<uint64_t>(c, "code", N);
-    uint64_t* code2 = pget<uint64_t>(c, "code2", N);
-    int* idx = pget<int>(c, "idx", N);
-    int* idx2 = pget<int>(c, "idx2", N);
+    uint64_t* code = pget<uint64_t>(c, ("code" + sfx).c_str(), N);
+    uint64_t* code2 = pget<uint64_t>(c, ("code2" + sfx).c_str(), N);
+    int* idx = pget<int>(c, ("idx" + sfx).c_str(), N);
+    int* idx2 = pget<int>(c, ("idx2" + sfx).c_str(), N);
     if (!code || !code2 || !idx || !idx2) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tree)");
     double* xo = o.xo_base + (int64_t)o.goff * D;
-    k_points_in<D><<<nblk(N, 256), 256, 0, c->stream>>>(o.xin, N, g, xo, code, idx, o.bad);
+    k_points_in<D><<<nblk(N, 256), 256, 0, st>>>(o.xin, N, g, xo, code, idx, o.bad);
     HVCK(hipGetLastError());
     size_t tmpb = 0;
-    HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, code, code2, idx, idx2, N, 0, 64, c->stream));
-    void* tmp = c->pool->get("cubtmp", tmpb, false, c->stream);
+    HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, code, code2, idx, idx2, N, 0, 64, st));
+    void* tmp = c->pool->get("cubtmp" + sfx, tmpb, false, st);
     if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
-    HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tmpb, code, code2, idx, idx2, N, 0, 64, c->stream));
-    k_points_sorted<D><<<nblk(N, 256), 256, 0, c->stream>>>(xo, idx2, N, o.goff, o.xs, o.sidx, o.rank_base + o.goff);
+    HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tmpb, code, code2, idx, idx2, N, 0, 64, st));
+    k_points_sorted<D><<<nblk(N, 256), 256, 0, st>>>(xo, idx2, N, o.goff, o.xs, o.sidx, o.rank_base + o.goff);
     HVCK(hipGetLastError());
     g.xo = o.xo_base;
     g.xs = o.xs;
     g.sidx = o.sidx;
     g.rank = o.rank_base;
     g.knn = o.knn;
-    k_point_rows32<D><<<nblk(N, 256), 256, 0, c->stream>>>(g, o.xf);
+    k_point_rows32<D><<<nblk(N, 256), 256, 0, st>>>(g, o.xf);
     HVCK(hipGetLastError());
     g.xf = o.xf;
     g.blo = o.blo;
     g.bhi = o.bhi;
-    k_leaf_boxes<D><<<nblk(g.P2, 256), 256, 0, c->stream>>>(g, o.blo, o.bhi);
+    k_leaf_boxes<D><<<nblk(g.P2, 256), 256, 0, st>>>(g, o.blo, o.bhi);
     HVCK(hipGetLastError());
     for (int first = g.P2 / 2; first >= 1; first /= 2) {
-        k_node_boxes<D><<<nblk(first, 256), 256, 0, c->stream>>>(first, first, g.NN, o.blo, o.bhi);
+        k_node_boxes<D><<<nblk(first, 256), 256, 0, st>>>(first, first, g.NN, o.blo, o.bhi);
         HVCK(hipGetLastError());
     }
     // 8-ary tree of child boxes (levels bottom-up)
@@ -2249,24 +2300,25 @@ int build_tree_core(hv_ctx* c, int L, int tables, const TreeOut& o, Tree& T) {
     for (int l = g.h - 1; l >= 0; --l) {
         int nodes = 1;
         for (int q = 0; q < l; ++q) nodes *= 8;
-        k_tree8_level<D><<<nblk((int64_t)nodes * 8, 256), 256, 0, c->stream>>>(g, l, nodes, o.cb);
+        k_tree8_level<D><<<nblk((int64_t)nodes * 8, 256), 256, 0, st>>>(g, l, nodes, o.cb);
         HVCK(hipGetLastError());
     }
-    k_knn64<D><<<nblk(N, 64), 64, 0, c->stream>>>(g, o.knn);
+    if (tables != HV_PREC_FP64 && L != 32) return hv_fail(c, HV_ERR_UNSUPPORTED, "the mixed-precision scans use leaves of 32 points");
+    if (tables != HV_PREC_FP64) g.lorg = o.lorg;
+    if (tables == HV_PREC_BF16) g.pkv = (const uint32_t*)o.pkv;
+    if (tables == HV_PREC_FP32) g.pxf = o.pxf;
+    if (o.defer_tables) return HV_OK;
+    k_knn64<D, false><<<nblk(N, 64), 64, 0, st>>>(g, GeoSet{}, nullptr, o.knn);
     HVCK(hipGetLastError());
     if (tables != HV_PREC_FP64) {
-        if (L != 32) return hv_fail(c, HV_ERR_UNSUPPORTED, "the mixed-precision scans use leaves of 32 points");
-        k_leaf_origin<D><<<nblk((g.nleaf + 7) / 8, 256), 256, 0, c->stream>>>(g, o.lorg);
+        k_leaf_origin<D, false><<<nblk((g.nleaf + 7) / 8, 256), 256, 0, st>>>(g, GeoSet{}, nullptr, o.lorg);
         HVCK(hipGetLastError());
-        g.lorg = o.lorg;
         if (tables == HV_PREC_BF16) {
-            k_point_kvec<D><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, c->stream>>>(g, o.lorg, o.pkv);
+            k_point_kvec<D, false><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, st>>>(g, GeoSet{}, nullptr, o.lorg, o.pkv);
             HVCK(hipGetLastError());
-            g.pkv = (const uint32_t*)o.pkv;
         } else {
-            k_point_f32<D><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, c->stream>>>(g, o.lorg, o.pxf);
+            k_point_f32<D, false><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, st>>>(g, GeoSet{}, nullptr, o.lorg, o.pxf);
             HVCK(hipGetLastError());
-            g.pxf = o.pxf;
         }
     }
     return HV_OK;
@@ -2488,8 +2540,9 @@ __global__ __launch_bounds__(256) void k_pkt_hist(const int* __restrict__ list, 
                                                   int* __restrict__ cnt) {
     if (n_dev) n = min(n, *n_dev);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    atomicAdd(&cnt[mof[vkey[(int64_t)R.v[list[t]] * (D + 1)]]], 1);
+    const bool act = t < n;
+    const int m = act ? mof[vkey[(int64_t)R.v[list[t]] * (D + 1)]] : 0;
+    hv_wave_add_by_key(cnt, m, act);
 }
 
 // Prefix sums over the meshes (a wavefront, 64 meshes per step): seg = first list position of
@@ -2518,6 +2571,8 @@ __global__ void k_pkt_plan(const int* __restrict__ cnt, int nmesh, int ppw, int*
         pk_off[nmesh] = p0;
     }
 }
+
+constexpr int HV_FOREST_STREAMS = 4;   // the box's hardware queues per process
 
 // Every mesh's tree into segments of one set of tables: sorted positions of mesh k start at a
 // multiple of 256 (whole groups of 8 leaves), generator ids at goff[k].
@@ -2592,6 +2647,36 @@ static int build_forest(hv_ctx* c, int L, int tables, Forest& fo) {
     HVCK(hipMemsetAsync(bad, 0, sizeof(int) * nm, c->stream));
     HVCK(hipStreamSynchronize(c->stream));
     fo.geo.assign(nm, DevGeo{});
+    // the meshes' trees on HV_FOREST_STREAMS streams at once (small meshes leave the chip
+    // idle one at a time), each stream with its own temporaries, sized up front for the
+    // largest mesh so that nothing is reallocated while another stream works
+    constexpr int NS = HV_FOREST_STREAMS;
+    while ((int)c->side.size() < NS) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return hv_fail(c, HV_ERR_DEVICE, "stream");
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipStreamDestroy(st);
+            return hv_fail(c, HV_ERR_DEVICE, "event");
+        }
+        c->side.push_back(st);
+        c->side_ev.push_back(ev);
+    }
+    {
+        int nmax = 1;
+        for (int k = 0; k < nm; ++k) nmax = std::max(nmax, fo.n[k]);
+        size_t tmpb = 0;
+        HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpb, (uint64_t*)nullptr, (uint64_t*)nullptr, (int*)nullptr,
+                                                (int*)nullptr, nmax, 0, 64, c->stream));
+        for (int q = 0; q < NS; ++q) {
+            const std::string sfx = "#" + std::to_string(q);
+            if (!pget<uint64_t>(c, ("code" + sfx).c_str(), nmax) || !pget<uint64_t>(c, ("code2" + sfx).c_str(), nmax) ||
+                !pget<int>(c, ("idx" + sfx).c_str(), nmax) || !pget<int>(c, ("idx2" + sfx).c_str(), nmax) ||
+                !c->pool->get("cubtmp" + sfx, tmpb, false, c->stream))
+                return hv_fail(c, HV_ERR_DEVICE, "out of device memory (batch trees)");
+        }
+        HVCK(hipStreamSynchronize(c->stream));
+    }
     for (int k = 0; k < nm; ++k) {
         TreeOut o;
         o.xin = dxo + (int64_t)fo.goff[k] * D;
@@ -2617,12 +2702,48 @@ static int build_forest(hv_ctx* c, int L, int tables, Forest& fo) {
         o.pkv = pkv ? pkv + soff[k] * 4 : nullptr;
         o.pxf = pxf ? pxf + soff[k] * 8 : nullptr;
         o.bad = bad + k;
+        o.defer_tables = true;
         Tree T;
-        int rc = build_tree_core<D>(c, L, tables, o, T);
+        int rc = build_tree_core<D>(c, L, tables, o, T, c->side[k % NS], k % NS);
         if (rc) return rc;
         fo.geo[k] = T.g;
     }
+    // the walk (on the context's stream) starts when every tree is built
+    for (int q = 0; q < NS; ++q) {
+        HVCK(hipEventRecord(c->side_ev[q], c->side[q]));
+        HVCK(hipStreamWaitEvent(c->stream, c->side_ev[q], 0));
+    }
     HVCK(hipMemcpyAsync(gs, fo.geo.data(), sizeof(DevGeo) * nm, hipMemcpyHostToDevice, c->stream));
+    // 16-NN lists and scan tables of every mesh in one launch each (items numbered mesh after mesh)
+    {
+        std::vector<int> hoff(3 * (nm + 1), 0);
+        int* ob = hoff.data();             // blocks of 64 points (16-NN)
+        int* og = ob + (nm + 1);           // groups of 8 leaves (origins)
+        int* op = og + (nm + 1);           // padded points of whole leaves (K-vectors / FP32 rows)
+        for (int k = 0; k < nm; ++k) {
+            ob[k + 1] = ob[k] + (fo.n[k] + 63) / 64;
+            og[k + 1] = og[k] + (fo.geo[k].nleaf + 7) / 8;
+            op[k + 1] = op[k] + fo.geo[k].nleaf * 32;
+        }
+        int* doff = pget<int>(c, "f_off", hoff.size());
+        if (!doff) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (batch trees)");
+        HVCK(hipMemcpyAsync(doff, hoff.data(), sizeof(int) * hoff.size(), hipMemcpyHostToDevice, c->stream));
+        GeoSet Ft{gs, mof, nm};
+        k_knn64<D, true><<<ob[nm], 64, 0, c->stream>>>(DevGeo{}, Ft, doff, nullptr);
+        HVCK(hipGetLastError());
+        if (tables != HV_PREC_FP64) {
+            k_leaf_origin<D, true><<<nblk(og[nm], 256), 256, 0, c->stream>>>(DevGeo{}, Ft, doff + (nm + 1), nullptr);
+            HVCK(hipGetLastError());
+            if (tables == HV_PREC_BF16)
+                k_point_kvec<D, true><<<nblk(op[nm], 256), 256, 0, c->stream>>>(DevGeo{}, Ft, doff + 2 * (nm + 1), nullptr,
+                                                                               nullptr);
+            else
+                k_point_f32<D, true><<<nblk(op[nm], 256), 256, 0, c->stream>>>(DevGeo{}, Ft, doff + 2 * (nm + 1), nullptr,
+                                                                              nullptr);
+            HVCK(hipGetLastError());
+        }
+        HVCK(hipStreamSynchronize(c->stream));   // hoff is a local
+    }
     std::vector<int> hbad(nm);
     HVCK(hipMemcpyAsync(hbad.data(), bad, sizeof(int) * nm, hipMemcpyDeviceToHost, c->stream));
     HVCK(hipStreamSynchronize(c->stream));
@@ -2640,7 +2761,9 @@ template <int D>
 __global__ __launch_bounds__(256) void k_split_count(const int* __restrict__ ids, int nv, const int* __restrict__ vkey,
                                                      const int* __restrict__ mof, int* __restrict__ cnt) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nv) atomicAdd(&cnt[mof[vkey[(int64_t)ids[t] * (D + 1)]]], 1);
+    const bool act = t < nv;
+    const int m = act ? mof[vkey[(int64_t)ids[t] * (D + 1)]] : 0;
+    hv_wave_add_by_key(cnt, m, act);
 }
 
 template <int D>
@@ -2651,11 +2774,13 @@ __global__ __launch_bounds__(256) void k_split_scatter(const int* __restrict__ i
                                                        int* __restrict__ fill, int* __restrict__ okey,
                                                        double* __restrict__ ox) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nv) return;
-    const int v = ids[t];
+    const bool act = t < nv;
+    const int v = act ? ids[t] : 0;
     const int* key = vkey + (int64_t)v * (D + 1);
-    const int m = mof[key[0]];
-    const int64_t pos = off[m] + atomicAdd(&fill[m], 1);
+    const int m = act ? mof[key[0]] : 0;
+    const int slot = hv_wave_add_by_key(fill, m, act);
+    if (!act) return;
+    const int64_t pos = off[m] + slot;
     const int64_t pbase = ntot + (int64_t)2 * D * m;
 #pragma unroll
     for (int j = 0; j <= D; ++j) {
@@ -3270,24 +3395,45 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
 }
 
 // Meshes of dimension D walked together: one forest of trees, one breadth-first walk whose
-// levels hold the rays of every mesh, packets of one mesh each.
+// levels hold the rays of every mesh, packets of one mesh each.  The meshes are taken in runs of
+// at most HV_FOREST_VERTICES expected vertices (one walk each): a level's ray tables grow with
+// the vertices of the walk, and a bounded walk keeps the context's tables reusable.
+constexpr double HV_FOREST_VERTICES = 16e6;
+
 template <int D>
-int hv_build_forest_meshes(hv_ctx* c, const hv_opts* o, Forest& fo) {
-    fo.goff.assign(fo.nmesh, 0);
-    fo.ntot = 0;
-    for (int k = 0; k < fo.nmesh; ++k) {
-        fo.goff[k] = (int)fo.ntot;
-        fo.ntot += fo.n[k];
-        if (fo.ntot >= (1ll << 30)) return hv_fail(c, HV_ERR_UNSUPPORTED, "too many generators in one batch");
+int hv_build_forest_meshes(hv_ctx* c, const hv_opts* o, Forest& all) {
+    all.out.assign(all.nmesh, nullptr);
+    int rc = HV_OK;
+    for (int k0 = 0; k0 < all.nmesh && rc == HV_OK;) {
+        Forest fo;
+        double est = 0.0;
+        int k1 = k0;
+        while (k1 < all.nmesh && (k1 == k0 || est + vertices_per_point(D) * all.n[k1] <= HV_FOREST_VERTICES)) {
+            est += vertices_per_point(D) * all.n[k1];
+            fo.xs.push_back(all.xs[k1]);
+            fo.n.push_back(all.n[k1]);
+            fo.base.push_back(all.base[k1]);
+            fo.normal.push_back(all.normal[k1]);
+            ++k1;
+        }
+        fo.nmesh = k1 - k0;
+        fo.goff.assign(fo.nmesh, 0);
+        fo.ntot = 0;
+        for (int k = 0; k < fo.nmesh; ++k) {
+            fo.goff[k] = (int)fo.ntot;
+            fo.ntot += fo.n[k];
+        }
+        if (fo.ntot >= (1ll << 30)) { rc = hv_fail(c, HV_ERR_UNSUPPORTED, "too many generators in one batch"); break; }
+        fo.mbits = 1;
+        while ((1 << fo.mbits) < fo.nmesh) ++fo.mbits;
+        fo.out.assign(fo.nmesh, nullptr);
+        hv_mesh* none = nullptr;
+        rc = hv_walk_impl<D, true>(c, o, &none, (int)fo.ntot, nullptr, &fo);
+        for (int k = 0; k < fo.nmesh; ++k) all.out[k0 + k] = fo.out[k];
+        k0 = k1;
     }
-    fo.mbits = 1;
-    while ((1 << fo.mbits) < fo.nmesh) ++fo.mbits;
-    fo.out.assign(fo.nmesh, nullptr);
-    if (fo.nmesh == 0) return HV_OK;
-    hv_mesh* none = nullptr;
-    const int rc = hv_walk_impl<D, true>(c, o, &none, (int)fo.ntot, nullptr, &fo);
     if (rc)
-        for (auto& m : fo.out)
+        for (auto& m : all.out)
             if (m) { hv_mesh_free(m); m = nullptr; }
     return rc;
 }
