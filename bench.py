@@ -383,7 +383,10 @@ def main():
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp64"],
                     help="headline leg (the other precision, and the FP32 filter, run beside it)")
     ap.add_argument("--cpu-cells", type=int, default=2000, help="cells per thread walked by the CPU baseline (about 15-20 s on 16 threads)")
+    ap.add_argument("--cpu-cells1", type=int, default=2500, help="cells walked by the 1-thread CPU baseline (about 20 s)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the autotuner (hv_tune) on the workload first and time with its scan tiles")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 leg and key-table comparison")
     ap.add_argument("--batched", action="store_true",
@@ -438,6 +441,10 @@ def main():
     ctx.set_points(xs)
     ctx.set_boundary(base, normal)
 
+    tiles = None
+    if args.tune:
+        tiles = ctx.tune(args.precision)   # the autotuner's scan tiles for this workload
+        log(f"[tune] rays_per_block {tiles[0]}, packet_floor {tiles[1]}: {tiles[2]:.2f} ms")
     sec, stats = run_leg(ctx, args.precision, args.steps, args.warmup)
     st = stats[-1]
     nv = st["vertices"]
@@ -477,6 +484,9 @@ def main():
     peak = PEAK_TFLOPS[args.precision]
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": measured_traffic(args.precision),
+                "traffic_source": "profiles/k_raycast64_5d_100k_mixture_traffic.txt: rocprofv3 --pmc FETCH_SIZE and "
+                                  "WRITE_SIZE runs of this workload, scaled by a copy of known size (not measured "
+                                  "in this run)",
                 "kernel": f"k_raycast64<5,{1 if args.precision == 'bf16' else 0}>", "launches": launches,
                 "avg_launch_ms": st["ms_raycast"] / max(1, launches),
                 "flops_per_launch": flops / max(1, launches),
@@ -484,13 +494,19 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        # all host cores of the box's share (at most 16 per GPU), and one thread
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         vps, ncpu, tcpu = cpu_baseline(xs, args.cpu_cells, threads)
+        vps1, ncpu1, tcpu1 = cpu_baseline(xs, args.cpu_cells1, 1)
         cpu = {"value": vps, "unit": "vertices/s", "cores": threads, "kind": "port",
                "sample": f"oracle (C restatement of the reference walk, FP64) over cells 1..{args.cpu_cells * threads} "
                          f"of the same point set, {threads} threads on contiguous cell ranges: "
-                         f"{ncpu} unique vertices in {tcpu:.1f} s"}
-        log(f"[cpu] {ncpu} vertices in {tcpu:.2f} s on {threads} threads -> {vps:.0f} vertices/s")
+                         f"{ncpu} unique vertices in {tcpu:.1f} s",
+               "value_1_thread": vps1,
+               "sample_1_thread": f"the same, 1 thread over cells 1..{args.cpu_cells1}: {ncpu1} vertices in {tcpu1:.1f} s",
+               "host_cpus": os.cpu_count()}
+        log(f"[cpu] {ncpu} vertices in {tcpu:.2f} s on {threads} threads -> {vps:.0f} vertices/s; "
+            f"1 thread {vps1:.0f} vertices/s")
 
     out = {
         "metric": "Voronoi vertices/s at 5D N=100k: mixed-precision vs FP64 GPU, identical vertex set",
@@ -507,7 +523,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "n": int(xs.shape[0]), "d": d, "precision": args.precision,
                    "vertices": nv, "rays": st["rays"], "levels": st["levels"],
-                   "duplicate_rays": st["duplicates"], "near_ties": st["near_ties"]},
+                   "duplicate_rays": st["duplicates"], "near_ties": st["near_ties"],
+                   "tiles": {"rays_per_block": tiles[0], "packet_floor": tiles[1], "source": "hv_tune"} if tiles else
+                            {"rays_per_block": 64, "packet_floor": 2048, "source": "defaults"}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

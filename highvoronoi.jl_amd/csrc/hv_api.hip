@@ -1,8 +1,13 @@
 // hv_api.hip -- the extern "C" surface of libhv.so (declared in include/hv.h).
 #include "hv_internal.h"
 #include <algorithm>
+#include <cstring>
 #include <atomic>
 #include <thread>
+#include <cstdio>
+#include <cstdlib>
+#include <tuple>
+#include <fstream>
 #include <mutex>
 #include <map>
 #include <cstring>
@@ -153,6 +158,59 @@ int hv_fail(hv_ctx* c, int code, const std::string& msg) {
     return code;
 }
 
+// ---------------------------------------------------------------- scan tiles (autotuner)
+// Process-wide table (dimension, precision, size class) -> tiles, filled by hv_tune and from the
+// JSON lines file named by HV_TUNE_CACHE (one object per line, the keys hv_tune writes).
+static std::mutex tune_mu;
+static std::map<std::tuple<int, int, int>, HvTiles> tune_table;
+static bool tune_loaded = false;
+
+static int size_class(int64_t n) {
+    int b = 0;
+    while (b < 62 && (1ll << (b + 1)) <= n) ++b;
+    return b;
+}
+
+static bool json_int(const std::string& line, const char* key, long long& v) {
+    const std::string k = std::string("\"") + key + "\":";
+    const size_t p = line.find(k);
+    if (p == std::string::npos) return false;
+    return std::sscanf(line.c_str() + p + k.size(), "%lld", &v) == 1;
+}
+
+static void tune_load_locked() {
+    if (tune_loaded) return;
+    tune_loaded = true;
+    const char* path = std::getenv("HV_TUNE_CACHE");
+    if (!path || !*path) return;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line)) {
+        long long d, prec, cls, ppw, floor_;
+        if (!json_int(line, "d", d) || !json_int(line, "precision", prec) || !json_int(line, "size_class", cls) ||
+            !json_int(line, "rays_per_block", ppw) || !json_int(line, "packet_floor", floor_))
+            continue;
+        if (ppw != 8 && ppw != 16 && ppw != 32 && ppw != 64) continue;
+        if (floor_ < 1 || floor_ > (1 << 20)) continue;
+        tune_table[{(int)d, (int)prec, (int)cls}] = HvTiles{(int)ppw, (int)floor_};
+    }
+}
+
+HvTiles hv_tiles_for(const hv_ctx* c, const hv_opts& o, int D, int precision, int64_t n) {
+    (void)c;
+    HvTiles t;
+    {
+        std::lock_guard<std::mutex> lk(tune_mu);
+        tune_load_locked();
+        auto it = tune_table.find({D, precision, size_class(n)});
+        if (it != tune_table.end()) t = it->second;
+    }
+    if (o.rays_per_block == 8 || o.rays_per_block == 16 || o.rays_per_block == 32 || o.rays_per_block == 64)
+        t.ppw_max = o.rays_per_block;
+    if (o.packet_floor > 0) t.min_packets = std::min(o.packet_floor, 1 << 20);
+    return t;
+}
+
 // ---------------------------------------------------------------- dispatch helpers
 #define HV_DISPATCH(D, fn, ...)                               \
     switch (D) {                                              \
@@ -163,6 +221,19 @@ int hv_fail(hv_ctx* c, int code, const std::string& msg) {
         case 6: return fn<6>(__VA_ARGS__);                    \
         default: return HV_ERR_UNSUPPORTED;                   \
     }
+
+// the refined mesh of hv_refine_mesh: a walk of the new cells on the union, merged with the old mesh
+template <int D>
+static int refine_d(hv_ctx* c, const hv_opts* o, const hv_mesh* old, hv_mesh** out, int64_t* removed) {
+    std::vector<int> list;
+    for (int64_t i = old->N; i < c->N; ++i) list.push_back((int)i);
+    hv_mesh* S = nullptr;
+    int rc = hv_build_mesh_impl<D>(c, o, &S, &list);
+    if (rc) return rc;
+    rc = hv_refine_merge<D>(c, old, S, out, removed);
+    hv_mesh_free(S);
+    return rc;
+}
 
 extern "C" {
 
@@ -291,6 +362,58 @@ int hv_build_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out) {
     HV_DISPATCH(c->D, hv_build_mesh_impl, c, o, out, nullptr)
 }
 
+int hv_tune(hv_ctx* c, int precision, const char* cache_path, int32_t* rays_per_block, int32_t* packet_floor,
+            double* ms) {
+    if (!c) return HV_ERR_ARG;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    if (precision != HV_PREC_FP64 && precision != HV_PREC_FP32 && precision != HV_PREC_BF16)
+        return hv_fail(c, HV_ERR_UNSUPPORTED, "precision: HV_PREC_FP64, HV_PREC_FP32 or HV_PREC_BF16");
+    int mask = 0;
+    (void)hipSetDevice(c->device);
+    int rc = hv_box_periodic_mask(c, c->D, &mask);
+    if (rc) return rc;
+    if (mask) return hv_fail(c, HV_ERR_UNSUPPORTED, "the autotuner takes a non-periodic box");
+    const int ppws[2] = {32, 64};
+    const int floors[3] = {1024, 2048, 4096};
+    HvTiles best;
+    double best_ms = 1e300;
+    hv_opts o{};
+    o.precision = precision;
+    for (int warm = 0; warm < 2; ++warm)
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 3; ++b) {
+                o.rays_per_block = ppws[a];
+                o.packet_floor = floors[b];
+                hv_mesh* m = nullptr;
+                rc = hv_build_mesh(c, &o, &m);
+                if (rc) return rc;
+                const double t = m->stats.ms_total;
+                hv_mesh_free(m);
+                if (warm == 0) continue;   // the first pass pages the tables in
+                if (t < best_ms) {
+                    best_ms = t;
+                    best = HvTiles{ppws[a], floors[b]};
+                }
+            }
+    const int cls = size_class(c->N);
+    {
+        std::lock_guard<std::mutex> lk(tune_mu);
+        tune_load_locked();
+        tune_table[{c->D, precision, cls}] = best;
+    }
+    if (cache_path && *cache_path) {
+        std::ofstream f(cache_path, std::ios::app);
+        if (!f) return hv_fail(c, HV_ERR_ARG, "cannot write the tune cache");
+        f << "{\"d\":" << c->D << ",\"precision\":" << precision << ",\"size_class\":" << cls
+          << ",\"rays_per_block\":" << best.ppw_max << ",\"packet_floor\":" << best.min_packets
+          << ",\"ms\":" << best_ms << "}\n";
+    }
+    if (rays_per_block) *rays_per_block = best.ppw_max;
+    if (packet_floor) *packet_floor = best.min_packets;
+    if (ms) *ms = best_ms;
+    return HV_OK;
+}
+
 int hv_build_mesh_cells(hv_ctx* c, const hv_opts* o, int64_t ncells, const int64_t* cells, hv_mesh** out) {
     if (!c || !out || ncells < 0 || (ncells > 0 && !cells)) return hv_fail(c, HV_ERR_ARG, "null argument");
     *out = nullptr;
@@ -306,6 +429,37 @@ int hv_build_mesh_cells(hv_ctx* c, const hv_opts* o, int64_t ncells, const int64
     list.erase(std::unique(list.begin(), list.end()), list.end());
     (void)hipSetDevice(c->device);
     HV_DISPATCH(c->D, hv_build_mesh_impl, c, o, out, &list)
+}
+
+int hv_refine_mesh(hv_ctx* c, const hv_opts* o, const hv_mesh* old, int64_t m, const double* xs_new, hv_mesh** out,
+                   int64_t* removed) {
+    if (!c || !old || !out || m < 1 || !xs_new) return hv_fail(c, HV_ERR_ARG, "null argument or no new points");
+    *out = nullptr;
+    if (old->device != c->device) return hv_fail(c, HV_ERR_ARG, "the mesh lives on another device");
+    if (old->extended || old->img_orig || old->cells)
+        return hv_fail(c, HV_ERR_UNSUPPORTED, "refinement of periodic meshes or of meshes of some cells");
+    if (!old->xw) return hv_fail(c, HV_ERR_STATE, "the mesh holds no generators");
+    const int D = old->D;
+    const int64_t n = old->N, nu = n + m;
+    if (nu >= (1ll << 30)) return hv_fail(c, HV_ERR_UNSUPPORTED, "too many generators for 32-bit keys");
+    (void)hipSetDevice(c->device);
+    // the union in device memory: the mesh's generators, then the new points
+    double* u = nullptr;
+    if (hipMalloc(&u, sizeof(double) * D * nu) != hipSuccess) {
+        (void)hipGetLastError();
+        return hv_fail(c, HV_ERR_DEVICE, "out of device memory (refine)");
+    }
+    int rc = HV_OK;
+    if (hipMemcpyAsync(u, old->xw->p, sizeof(double) * D * n, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(u + D * n, xs_new, sizeof(double) * D * m, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = hv_fail(c, HV_ERR_DEVICE, "copy of the points failed");
+    if (!rc) rc = set_points_common(c, D, nu, u, hipMemcpyDeviceToDevice);
+    (void)hipFree(u);
+    if (rc) return rc;
+    rc = hv_set_boundary(c, old->P, old->P ? old->pbase.data() : nullptr, old->P ? old->pnormal.data() : nullptr,
+                         old->pbc.empty() ? nullptr : old->pbc.data());
+    if (rc) return rc;
+    HV_DISPATCH(D, refine_d, c, o, old, out, removed)
 }
 
 // A mesh of a batch that the fused walk takes: a cuboid in the reference's plane order without
@@ -460,6 +614,46 @@ int hv_mesh_export(const hv_mesh* m, int64_t* sig, double* coords) {
     if (!m || !sig || !coords) return HV_ERR_ARG;
     (void)hipSetDevice(m->device);
     HV_DISPATCH(m->D, hv_mesh_export_impl, m, sig, coords)
+}
+
+int hv_mesh_export_db(const hv_mesh* m, int64_t* heap, int64_t* cell_ptr, int64_t* refs) {
+    if (!m || !cell_ptr) return HV_ERR_ARG;
+    if (m->D < 2 || m->D > 6) return HV_ERR_UNSUPPORTED;
+    const int D = m->D;
+    const int64_t nv = m->nv, N = m->N;
+    std::vector<int64_t> sig((size_t)std::max<int64_t>(nv, 1) * (D + 1));
+    std::vector<double> x((size_t)std::max<int64_t>(nv, 1) * D);
+    if (nv > 0) {
+        const int rc = hv_mesh_export(m, sig.data(), x.data());
+        if (rc) return rc;
+    }
+    // per generator the number of vertices holding it
+    std::fill(cell_ptr, cell_ptr + N + 1, 0);
+    for (int64_t v = 0; v < nv; ++v)
+        for (int j = 0; j <= D; ++j) {
+            const int64_t g = sig[(size_t)v * (D + 1) + j];
+            if (g <= N) ++cell_ptr[g];
+        }
+    for (int64_t i = 0; i < N; ++i) cell_ptr[i + 1] += cell_ptr[i];
+    const int64_t W = 2 * D + 2;   // words per vertex: length, d + 1 ids, d coordinates
+    if (heap)
+        for (int64_t v = 0; v < nv; ++v) {
+            int64_t* h = heap + v * W;
+            h[0] = D + 1;
+            for (int j = 0; j <= D; ++j) h[1 + j] = sig[(size_t)v * (D + 1) + j];
+            std::memcpy(h + D + 2, &x[(size_t)v * D], sizeof(double) * D);
+        }
+    if (refs) {
+        std::vector<int64_t> fill(cell_ptr, cell_ptr + N);
+        for (int64_t v = 0; v < nv; ++v) {
+            const int64_t pos = v * W + 1;   // 1-based position of the vertex's length word
+            for (int j = 0; j <= D; ++j) {
+                const int64_t g = sig[(size_t)v * (D + 1) + j];
+                if (g <= N) refs[fill[g - 1]++] = j == 0 ? -pos : pos;
+            }
+        }
+    }
+    return HV_OK;
 }
 
 int hv_mesh_images(const hv_mesh* m, int64_t* n_ext, int64_t* orig, int8_t* shift) {

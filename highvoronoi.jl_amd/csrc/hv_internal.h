@@ -156,6 +156,16 @@ struct hv_bind_mesh {
 int hv_box_periodic_mask(hv_ctx* c, int D, int* mask);
 
 int hv_fail(hv_ctx* c, int code, const std::string& msg);
+
+// Tile sizes of the candidate scan: at most ppw_max rays per packet (8 .. 64), halved while a
+// level would give fewer than min_packets packets.
+struct HvTiles {
+    int ppw_max = 64;
+    int min_packets = 2048;
+};
+// The tiles a build uses: the caller's (hv_opts.rays_per_block / packet_floor when set), else
+// the autotuner's for (dimension, precision, size class of n), else the defaults.
+HvTiles hv_tiles_for(const hv_ctx* c, const hv_opts& o, int D, int precision, int64_t n);
 // per-device address of the scratch array of hv_raycast_batch_noalloc (resolved once per device)
 int hv_scratch_init(int device);
 
@@ -180,6 +190,9 @@ template <int D>
 int hv_inrange_impl(hv_ctx* c, int64_t nq, const double* q, const double* radius, int64_t* ptr, int64_t* idx);
 template <int D>
 int hv_mesh_export_impl(const hv_mesh* m, int64_t* sig, double* coords);
+// the refined mesh from an old mesh and the mesh S of the new cells on the union (hv_refine_mesh)
+template <int D>
+int hv_refine_merge(hv_ctx* c, const hv_mesh* old, const hv_mesh* S, hv_mesh** out, int64_t* removed);
 
 // cell volumes and interface areas (hv_integrate.hip)
 struct hv_integral;

@@ -2061,15 +2061,15 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g0, 
 }
 
 // New key = edge + winner (sorted) and its circumcentre, written to the provisional slot.
-// Packets of 64 rays, halved (down to 8) while the level would give fewer than
-// HV_MIN_PACKETS packets: a packet's walk is serial, and a few long walks leave the chip idle.
-constexpr int64_t HV_MIN_PACKETS = 2048;
+// Packets of at most ppw_max rays (64 = a wavefront), halved (down to 8) while the level would
+// give fewer than min_packets packets: a packet's walk is serial, and a few long walks leave
+// the chip idle.  Both are tile sizes of the autotuner (hv_tune; defaults 64 and 2048).
 // one interior seed vertex per this many generators (Morton order)
 constexpr int HV_SEED_STRIDE = 8;          // large point sets: one descent per 8 generators
 constexpr int HV_SEED_ALL_MAX = 1 << 17;   // up to this many generators, and in 2D, every one
-static inline int hv_packet_rays(int64_t n) {
-    int p = 64;
-    while (p > 8 && n < (int64_t)p * HV_MIN_PACKETS) p >>= 1;
+static inline int hv_packet_rays(int64_t n, const HvTiles& t = HvTiles{}) {
+    int p = t.ppw_max;
+    while (p > 8 && n < (int64_t)p * t.min_packets) p >>= 1;
     return p;
 }
 
@@ -2093,8 +2093,8 @@ static void launch_scan(int precision, const DevGeo& g, const GeoSet& F, const i
 template <int D>
 static void launch_scan1(int precision, const DevGeo& g, const DevRays& R, const int* list, int n, const int* n_dev,
                          const int* vkey, const double* vx, int* status, int* win, double* tout, double plane_tol,
-                         unsigned long long* ctr, hipStream_t st) {
-    const int ppw = hv_packet_rays(n);
+                         unsigned long long* ctr, hipStream_t st, const HvTiles& tiles = HvTiles{}) {
+    const int ppw = hv_packet_rays(n, tiles);
     launch_scan<D, false>(precision, g, GeoSet{}, nullptr, nullptr, ppw, nblk(n, ppw), R, list, n, n_dev, vkey, vx, status,
                           win, tout, plane_tol, ctr, st);
 }
@@ -2143,7 +2143,8 @@ __global__ void k_iota(int* __restrict__ a, int n) {
 template <int D, bool B>
 __global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int nseed, int stride, int nwalk,
                                                      const int* __restrict__ cells, int* __restrict__ vkey,
-                                                     double* __restrict__ vx, int64_t base, int* __restrict__ status);
+                                                     double* __restrict__ vx, int64_t base, int* __restrict__ status,
+                                                     uint64_t seed);
 template <int D>
 __global__ __launch_bounds__(256) void k_seed_exact(DevGeo g, const int* __restrict__ cells, int ncell,
                                                     int* __restrict__ vkey, double* __restrict__ vx, int64_t base,
@@ -2812,6 +2813,8 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     const int L = opt.leaf_size > 0 ? opt.leaf_size : 32;
     const double plane_tol = opt.plane_tolerance > 0 ? opt.plane_tolerance : 1e-12;
     const int KCH = opt.reserved0 > 1 ? opt.reserved0 : 1;   // chunks per level (0: tentative-key grouping)
+    // scan tiles: the caller's, else the autotuner's for this dimension, precision and size
+    const HvTiles tiles = hv_tiles_for(c, opt, D, prec, B ? fo->ntot : c->N);
     int rc = HV_OK;
     if constexpr (!B) {
         rc = check_box(c, D);
@@ -2990,7 +2993,8 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
         }
         if (ndesc > 0) {
             k_seed_descent<D, B><<<nblk(ndesc, 64), 64, 0, c->stream>>>(g, F, ndesc, stride, nwalk, dcells, vkey, vx,
-                                                                          vnext + nseed, status + nseed);
+                                                                          vnext + nseed, status + nseed,
+                                                                          o ? o->seed : 0);
             HVCK(hipGetLastError());
         }
         int nsel = 0;
@@ -3094,7 +3098,7 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
             if (r_) return r_;
             if constexpr (B) {
                 // packets of one mesh each: rays and packets per mesh of the (mesh-ordered) list
-                const int ppw = hv_packet_rays(n);
+                const int ppw = hv_packet_rays(n, tiles);
                 int* pcnt = pget<int>(c, "pk_cnt", (size_t)nmesh);
                 int* seg = pget<int>(c, "pk_seg", (size_t)nmesh + 1);
                 int* pko = pget<int>(c, "pk_off", (size_t)nmesh + 1);
@@ -3109,7 +3113,7 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
                                      status, rwin, rt, plane_tol, ctr + 0, c->stream);
             } else {
                 HVCK(hipEventRecord(r0, c->stream));
-                launch_scan1<D>(prec, g, R, list, n, n_dev, vkey, vx, status, rwin, rt, plane_tol, ctr + 0, c->stream);
+                launch_scan1<D>(prec, g, R, list, n, n_dev, vkey, vx, status, rwin, rt, plane_tol, ctr + 0, c->stream, tiles);
             }
             HVCK(hipGetLastError());
             HVCK(hipEventRecord(r1, c->stream));
@@ -3875,7 +3879,8 @@ __global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const 
 template <int D, bool B>
 __global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int nseed, int stride, int nwalk,
                                                      const int* __restrict__ cells, int* __restrict__ vkey,
-                                                     double* __restrict__ vx, int64_t base, int* __restrict__ status) {
+                                                     double* __restrict__ vx, int64_t base, int* __restrict__ status,
+                                                     uint64_t seed) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nseed) return;
     // over a forest (B): from every generator t, in its mesh's tree
@@ -3895,11 +3900,13 @@ __global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int ns
     bool ok = true;
 #pragma unroll
     for (int step = 0; step < D; ++step) {
-        // direction: a fixed pseudo-random vector, orthogonalised against the face's normals
+        // direction: a pseudo-random vector drawn from (seed, t, step), orthogonalised against
+        // the face's normals
         double u[D];
 #pragma unroll
         for (int k = 0; k < D; ++k)
-            u[k] = (double)(hv_mix64(((uint64_t)t << 8) + (uint64_t)(step * D + k) + 0x5eedull) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+            u[k] = (double)(hv_mix64(((uint64_t)t << 8) + (uint64_t)(step * D + k) + 0x5eedull +
+                                     seed * 0x9e3779b97f4a7c15ull) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
 #pragma unroll
         for (int rep = 0; rep < 2; ++rep) {
 #pragma unroll
@@ -4653,6 +4660,147 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     return HV_OK;
 }
 
+// ---------------------------------------------------------------------------------
+// Refinement: the mesh of a point set X with new points Y appended (refine!,
+// src/geometry.jl:404-438 -> systematic_refine!, src/meshrefine.jl:183-217).  A vertex of X's
+// mesh stays when no new point lies inside its circumsphere, by the reference's rule: kept iff
+// |x_sig[1] - r| <= (1 + 1e-7) * (distance from r to the nearest new point) (clean_affected!,
+// src/meshrefine.jl:126-150, which tests only vertices whose generators are all affected
+// cells; in general position a new point inside a circumsphere makes every generator of that
+// vertex a neighbour of the new cell, so the tested set is the same).  Every other vertex of
+// the refined mesh holds a new point and comes from the walk of the new cells.
+
+// keep[t] = 1 when vertex ids[t] of the old mesh has no new point (sorted position j with
+// sidx[j] >= nold) strictly inside its circumsphere shrunk by 1 + 1e-7
+template <int D>
+__global__ __launch_bounds__(128) void k_refine_keep(DevGeo g, int nv, const int* __restrict__ ids,
+                                                     const int* __restrict__ vkey, const double* __restrict__ vx,
+                                                     const double* __restrict__ xo, int nold,
+                                                     uint8_t* __restrict__ keep) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;
+    const int64_t v = ids[t];
+    const int64_t g0 = vkey[v * (D + 1)];   // the smallest id of a key is a generator
+    double r[D], R2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        r[k] = vx[v * D + k];
+        const double a = xo[g0 * D + k] - r[k];
+        R2 = fma(a, a, R2);
+    }
+    const double f = 1.0 + 1e-7;
+    const double lim = R2 / (f * f);
+    bool hit = false;
+    int node = 1;
+    while (node && !hit) {
+        if (hv_box_d2<D>(g, node, r) < lim) {
+            if (node < g.P2) { node = 2 * node; continue; }
+            const int leaf = node - g.P2;
+            const int j0 = leaf * g.L, j1 = min(j0 + g.L, g.n);
+            for (int j = j0; j < j1; ++j) {
+                if (g.sidx[j] < nold) continue;
+                double d2 = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const double a = g.xs[(int64_t)j * (D + 1) + k] - r[k];
+                    d2 = fma(a, a, d2);
+                }
+                if (d2 < lim) { hit = true; break; }
+            }
+        }
+        node = hv_next_node(node);
+    }
+    keep[t] = hit ? 0 : 1;
+}
+
+// rows base .. base + n of the refined mesh from vertices sel[0 .. n) of a source table; ids at
+// or above `from` (the old boundary planes) move up by `shift`
+template <int D>
+__global__ __launch_bounds__(256) void k_refine_copy(int n, const int* __restrict__ sel, const int* __restrict__ vkey,
+                                                     const double* __restrict__ vx, int from, int shift, int base,
+                                                     int* __restrict__ okey, double* __restrict__ ox,
+                                                     int* __restrict__ oids) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t v = sel[t], o = (int64_t)base + t;
+#pragma unroll
+    for (int j = 0; j <= D; ++j) {
+        const int k = vkey[v * (D + 1) + j];
+        okey[o * (D + 1) + j] = k >= from ? k + shift : k;
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) ox[o * D + k] = vx[v * D + k];
+    oids[o] = (int)o;
+}
+
+// The context holds the union (the old mesh's nold generators, then the new ones) and S is
+// the mesh of the new cells on it.  Out: the old mesh's kept vertices (plane ids shifted by the
+// number of new points) followed by S's vertices, in tables of its own.
+template <int D>
+int hv_refine_merge(hv_ctx* c, const hv_mesh* old, const hv_mesh* S, hv_mesh** out, int64_t* removed) {
+    const int nold = (int)old->N;
+    const int shift = (int)(c->N - old->N);
+    Tree T;
+    int rc = build_tree<D>(c, 32, T);
+    if (rc) return rc;
+    const int nv0 = (int)old->nv;
+    uint8_t* keep = pget<uint8_t>(c, "rf_keep", std::max(nv0, 1));
+    int* sel = pget<int>(c, "rf_sel", std::max(nv0, 1));
+    int* dcnt = pget<int>(c, "rf_cnt", 1);
+    if (!keep || !sel || !dcnt) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (refine)");
+    int nkeep = 0;
+    if (nv0 > 0) {
+        k_refine_keep<D><<<nblk(nv0, 128), 128, 0, c->stream>>>(T.g, nv0, (const int*)old->ids.p, (const int*)old->key.p,
+                                                               (const double*)old->x.p, (const double*)c->xw->p, nold,
+                                                               keep);
+        HVCK(hipGetLastError());
+        size_t tb = 0;
+        HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb, (const int*)old->ids.p, keep, sel, dcnt, nv0, c->stream));
+        void* tmp = c->pool->get("cubtmp2", tb, false, c->stream);
+        if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (select)");
+        HVCK(hipcub::DeviceSelect::Flagged(tmp, tb, (const int*)old->ids.p, keep, sel, dcnt, nv0, c->stream));
+        HVCK(hipMemcpyAsync(&nkeep, dcnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+    }
+    const int64_t nvs = S->nv, tot = (int64_t)nkeep + nvs;
+    if (tot >= (1ll << 31)) return hv_fail(c, HV_ERR_CAPACITY, "refined mesh exceeds 2^31 vertices");
+    hv_mesh* M = new hv_mesh;
+    M->device = c->device;
+    M->D = D;
+    M->N = c->N;
+    M->nv = tot;
+    const size_t nk = (size_t)std::max<int64_t>(tot, 1);
+    M->key.n = nk * (D + 1) * sizeof(int);
+    M->x.n = nk * D * sizeof(double);
+    M->ids.n = nk * sizeof(int);
+    if (hipMalloc(&M->key.p, M->key.n) != hipSuccess || hipMalloc(&M->x.p, M->x.n) != hipSuccess ||
+        hipMalloc(&M->ids.p, M->ids.n) != hipSuccess) {
+        for (DBuf* b : {&M->key, &M->x, &M->ids})
+            if (b->p) (void)hipFree(b->p);
+        delete M;
+        (void)hipGetLastError();
+        return hv_fail(c, HV_ERR_DEVICE, "out of device memory (refined mesh)");
+    }
+    if (nkeep > 0)
+        k_refine_copy<D><<<nblk(nkeep, 256), 256, 0, c->stream>>>(nkeep, sel, (const int*)old->key.p,
+                                                                  (const double*)old->x.p, nold, shift, 0,
+                                                                  (int*)M->key.p, (double*)M->x.p, (int*)M->ids.p);
+    if (nvs > 0)
+        k_refine_copy<D><<<nblk(nvs, 256), 256, 0, c->stream>>>((int)nvs, (const int*)S->ids.p, (const int*)S->key.p,
+                                                                (const double*)S->x.p, INT_MAX, 0, nkeep,
+                                                                (int*)M->key.p, (double*)M->x.p, (int*)M->ids.p);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+        hv_mesh_free(M);
+        return hv_fail(c, HV_ERR_DEVICE, "refine copy failed");
+    }
+    M->stats = S->stats;
+    M->stats.vertices = tot;
+    hv_mesh_snapshot(c, M);
+    if (removed) *removed = nv0 - nkeep;
+    *out = M;
+    return HV_OK;
+}
+
 #define HV_INST(D)                                                                                 \
     template int hv_mc_points_impl<D>(hv_ctx*, int64_t, uint64_t, double, const int64_t*,         \
                                       const uint64_t*, unsigned long long*, unsigned long long*,   \
@@ -4660,6 +4808,7 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     template int hv_build_mesh_impl<D>(hv_ctx*, const hv_opts*, hv_mesh**, const std::vector<int>*);   \
     template int hv_build_forest_meshes<D>(hv_ctx*, const hv_opts*, Forest&);                        \
     template int hv_mesh_export_impl<D>(const hv_mesh*, int64_t*, double*);                        \
+    template int hv_refine_merge<D>(hv_ctx*, const hv_mesh*, const hv_mesh*, hv_mesh**, int64_t*);   \
     template int hv_raycast_batch_impl<D>(hv_ctx*, int, int64_t, const int64_t*, const int32_t*,   \
                                           const double*, int64_t*, double*, double*, int32_t*);    \
     template int hv_raycast_noalloc_impl<D>(hv_ctx*, int, int64_t, const int64_t*, const int32_t*, \

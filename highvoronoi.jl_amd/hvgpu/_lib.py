@@ -30,7 +30,8 @@ class HVError(RuntimeError):
 class hv_opts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("leaf_size", C.c_int32), ("rays_per_block", C.c_int32),
                 ("reserved0", C.c_int32), ("plane_tolerance", C.c_double), ("ray_tol", C.c_double),
-                ("vertex_capacity", C.c_int64), ("seed", C.c_uint64)]
+                ("vertex_capacity", C.c_int64), ("seed", C.c_uint64), ("packet_floor", C.c_int32),
+                ("reserved1", C.c_int32)]
 
 
 class hv_stats_t(C.Structure):
@@ -69,6 +70,7 @@ def lib():
     L.hv_set_boundary.argtypes = [vp, C.c_int, P(C.c_double), P(C.c_double), P(C.c_int16)]
     L.hv_build_mesh.argtypes = [vp, P(hv_opts), P(vp)]
     L.hv_build_mesh_cells.argtypes = [vp, P(hv_opts), C.c_int64, P(C.c_int64), P(vp)]
+    L.hv_tune.argtypes = [vp, C.c_int, C.c_char_p, P(C.c_int32), P(C.c_int32), P(C.c_double)]
     L.hv_build_mesh_batch.argtypes = [C.c_int, C.c_int32, P(C.c_int32), P(C.c_int64), P(P(C.c_double)),
                                       P(C.c_int32), P(P(C.c_double)), P(P(C.c_double)), P(P(C.c_int16)),
                                       P(hv_opts), C.c_int32, P(vp), P(C.c_int32)]
@@ -77,6 +79,8 @@ def lib():
     L.hv_mesh_stats.argtypes = [vp, P(hv_stats_t)]
     L.hv_mesh_free.argtypes = [vp]
     L.hv_mesh_images.argtypes = [vp, P(C.c_int64), P(C.c_int64), P(C.c_int8)]
+    L.hv_refine_mesh.argtypes = [vp, P(hv_opts), vp, C.c_int64, P(C.c_double), P(vp), P(C.c_int64)]
+    L.hv_mesh_export_db.argtypes = [vp, P(C.c_int64), P(C.c_int64), P(C.c_int64)]
     L.hv_raycast_batch.argtypes = [vp, C.c_int, C.c_int64, P(C.c_int64), P(C.c_int32), P(C.c_double),
                                    P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_int32)]
     L.hv_raycast_batch_noalloc.argtypes = [vp, C.c_int, C.c_int64, vp, vp, vp, vp, vp, vp, vp]
@@ -133,6 +137,25 @@ class Mesh:
         if rc != HV_OK:
             raise HVError(rc, "export failed")
         return keys, coords
+
+    def export_db(self):
+        """DatabaseVertexStorage layout (hv_mesh_export_db): heap[V * (2d + 2)] int64 words
+        [d + 1, sig..., r... as FP64 bits] per vertex in export order, cell_ptr[n_ext + 1] and
+        refs[cell_ptr[-1]]: per generator the 1-based heap positions of its vertices, negative
+        where it is the key's first entry."""
+        nv, _, _ = self.counts()
+        n = C.c_int64(0)
+        lib().hv_mesh_images(self._h, C.byref(n), None, None)
+        ptr = np.zeros(n.value + 1, dtype=np.int64)
+        rc = lib().hv_mesh_export_db(self._h, None, _i64(ptr), None)
+        if rc != HV_OK:
+            raise HVError(rc, "export failed")
+        heap = np.zeros(nv * (2 * self.d + 2), dtype=np.int64)
+        refs = np.zeros(max(int(ptr[-1]), 1), dtype=np.int64)
+        rc = lib().hv_mesh_export_db(self._h, _i64(heap), _i64(ptr), _i64(refs))
+        if rc != HV_OK:
+            raise HVError(rc, "export failed")
+        return heap, ptr, refs[:int(ptr[-1])]
 
     def stats(self) -> dict:
         s = hv_stats_t()
@@ -194,15 +217,43 @@ class Context:
         self._check(lib().hv_set_boundary(self._h, npl, _dp(base), _dp(normal), bcp))
 
     def build_mesh(self, precision: str = "fp64", leaf_size: int = 0, vertex_capacity: int = 0,
-                   chunks: int = 0) -> Mesh:
+                   chunks: int = 0, rays_per_block: int = 0, packet_floor: int = 0, seed: int = 0) -> Mesh:
         o = hv_opts()
         o.precision = PRECISIONS[precision]
         o.leaf_size = leaf_size
         o.reserved0 = chunks
         o.vertex_capacity = vertex_capacity
+        o.rays_per_block = rays_per_block
+        o.packet_floor = packet_floor
+        o.seed = seed
         m = C.c_void_p()
         self._check(lib().hv_build_mesh(self._h, C.byref(o), C.byref(m)))
         return Mesh(self, m)
+
+    def tune(self, precision: str = "bf16", cache_path: str | None = None):
+        """Autotune the scan tiles on this context's points (hv_tune): (rays_per_block,
+        packet_floor, ms) chosen for this dimension, precision and size class."""
+        a = C.c_int32(0); b = C.c_int32(0); t = C.c_double(0.0)
+        path = cache_path.encode() if cache_path else None
+        self._check(lib().hv_tune(self._h, PRECISIONS[precision], path, C.byref(a), C.byref(b), C.byref(t)))
+        return a.value, b.value, t.value
+
+    def refine(self, mesh: "Mesh", new_points, precision: str = "fp64", leaf_size: int = 0):
+        """refine!(VG, xs) (hv_refine_mesh): the mesh of mesh's generators followed by
+        new_points[m, d] (ids n+1..n+m); the context's points become the union.  Returns
+        (mesh, number of old vertices dropped)."""
+        x = np.ascontiguousarray(new_points, dtype=np.float64)
+        if x.ndim != 2 or x.shape[1] != mesh.d:
+            raise ValueError("new_points must be [m, d]")
+        o = _opts(precision, leaf_size)
+        m = C.c_void_p()
+        rm = C.c_int64(0)
+        self._check(lib().hv_refine_mesh(self._h, C.byref(o), mesh._h, x.shape[0], _dp(x), C.byref(m), C.byref(rm)))
+        out = Mesh(self, m, mesh.d)
+        n = C.c_int64(0)
+        lib().hv_mesh_images(out._h, C.byref(n), None, None)
+        self.d, self.n = mesh.d, n.value
+        return out, rm.value
 
     def build_mesh_cells(self, cells, precision: str = "fp64", leaf_size: int = 0) -> Mesh:
         """The vertices of the cells `cells` (1-based generator ids) only: every vertex whose

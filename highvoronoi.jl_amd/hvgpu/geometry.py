@@ -114,19 +114,39 @@ class VoronoiGeometry:
         self._ctx.set_points(self.nodes)
         base, normal, bc = self.boundary.arrays()
         self._ctx.set_boundary(base.reshape(-1, d), normal.reshape(-1, d), bc if len(bc) else None)
-        self._mesh = self._ctx.build_mesh(precision=self.precision, leaf_size=int(s.get("leaf_size", 0)))
+        self.leaf_size = int(s.get("leaf_size", 0))
+        self.integrand, self.mc_accurate, self.seed = integrand, mc_accurate, seed
+        self._mesh = self._ctx.build_mesh(precision=self.precision, leaf_size=self.leaf_size)
         self.stats = self._mesh.stats()
+        self._integrate()
+
+    def _integrate(self):
         self.integral = None
-        if integrator in (VI_POLYGON, VI_FAST_POLYGON):
-            if integrand is None:
+        if self.integrator in (VI_POLYGON, VI_FAST_POLYGON):
+            if self.integrand is None:
                 self.integral = self._ctx.integrate(self._mesh, "polytope")
             else:
-                k = np.asarray(integrand(self.nodes[:1]), dtype=np.float64).reshape(1, -1).shape[1]
-                self.integral = self._ctx.integrate(self._mesh, "polytope", integrand=integrand, nf=k)
-        elif integrator == VI_MONTECARLO:
-            self.integral = self._ctx.integrate(self._mesh, "montecarlo", ndir=int(mc_accurate[0]), seed=seed)
+                k = np.asarray(self.integrand(self.nodes[:1]), dtype=np.float64).reshape(1, -1).shape[1]
+                self.integral = self._ctx.integrate(self._mesh, "polytope", integrand=self.integrand, nf=k)
+        elif self.integrator == VI_MONTECARLO:
+            self.integral = self._ctx.integrate(self._mesh, "montecarlo", ndir=int(self.mc_accurate[0]),
+                                                seed=self.seed)
         self._keys = None
         self._coords = None
+
+    def refine(self, xs):
+        """refine!(VG, xs) (src/geometry.jl:404-438): appends the points xs[m, d] to the nodes
+        (ids n+1..n+m), refines the mesh on the device (hv_refine_mesh: the old vertices whose
+        circumsphere holds no new point, plus a walk of the new cells) and integrates again
+        with the same integrator.  Non-periodic boxes only.  Returns self."""
+        xs = np.ascontiguousarray(np.asarray(xs, dtype=np.float64)).reshape(-1, self.nodes.shape[1])
+        mesh, self.removed = self._ctx.refine(self._mesh, xs, precision=self.precision, leaf_size=self.leaf_size)
+        self._mesh.free()
+        self._mesh = mesh
+        self.nodes = np.vstack([self.nodes, xs])
+        self.stats = self._mesh.stats()
+        self._integrate()
+        return self
 
     def vertex_table(self):
         """(keys[V, d+1] sorted 1-based, coords[V, d]) in lexicographic key order."""

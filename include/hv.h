@@ -57,7 +57,8 @@ typedef struct hv_opts {
     int32_t precision;        /* HV_PREC_* */
     int32_t leaf_size;        /* generators per tree leaf (0 = default 32; HV_PREC_BF16 and
                                  HV_PREC_FP32 need 32) */
-    int32_t rays_per_block;   /* ignored: the scan works on packets of 64 rays, one wavefront each */
+    int32_t rays_per_block;   /* scan tile: at most this many rays per packet (8, 16, 32 or 64; a
+                                 packet is one wavefront); 0 = the autotuner's choice (hv_tune) or 64 */
     int32_t reserved0;        /* walk-level schedule: <= 0 (default) = one scan per tentative end
                                  point, the other rays of that end point wait and are carried to
                                  the next level if their edge stays open; 1 = every ray of a level
@@ -66,8 +67,12 @@ typedef struct hv_opts {
     double ray_tol;           /* src/raycast-types.jl:230; ignored (it acts on non-general vertices,
                                  which the device walk reports as HV_ERR_DEGENERATE) */
     int64_t vertex_capacity;  /* device vertex-table capacity hint (0 = auto) */
-    uint64_t seed;            /* ignored: the walk starts from the 2^d box corners, not from the
-                                 reference's random descent (src/raycast.jl:45-109, 211-239) */
+    uint64_t seed;            /* draws the directions of the interior seeds' descents (the
+                                 reference's random descent, src/raycast.jl:45-109, 211-239); the
+                                 vertex set does not depend on it, the traversal does */
+    int32_t packet_floor;     /* scan tile: packets are halved (down to 8 rays) while a level would
+                                 give fewer packets than this; 0 = the autotuner's choice or 2048 */
+    int32_t reserved1;        /* 0 */
 } hv_opts;
 
 typedef struct hv_stats_t {
@@ -107,6 +112,16 @@ int hv_set_points_device(hv_ctx* ctx, int d, int64_t n, const double* xs_dev);
  * bc as the reference's Plane.BC (0 Dirichlet, -1 Neumann, >0 periodic partner). */
 int hv_set_boundary(hv_ctx* ctx, int nplanes, const double* base, const double* normal, const int16_t* bc);
 
+/* No counterpart in the reference (tile sizes are a device matter): the autotuner.  Builds the
+ * mesh of the context's current points and boundary (a non-periodic cuboid) with each candidate
+ * scan tile (rays_per_block 32 or 64, packet_floor 1024, 2048 or 4096), keeps the fastest for
+ * (dimension, precision, size class floor(log2 n)) in a process-wide table that later builds and
+ * batched walks of that class use unless their hv_opts set the tiles, and appends it as one
+ * JSON line to cache_path when that is not NULL (hv_create loads the file named by the
+ * environment variable HV_TUNE_CACHE).  Returns the chosen tiles and their build time (ms). */
+int hv_tune(hv_ctx* ctx, int precision, const char* cache_path, int32_t* rays_per_block, int32_t* packet_floor,
+            double* ms);
+
 /* Mesh -------------------------------------------------------------------- */
 /* Replaces voronoi(mesh; searcher) with threading dispatch (src/sysvoronoi.jl:21-82):
  * the whole vertex set of the tessellation, computed on the device. */
@@ -126,6 +141,30 @@ int hv_mesh_counts(const hv_mesh* mesh, int64_t* nvert, int32_t* max_sig, int64_
 int hv_mesh_export(const hv_mesh* mesh, int64_t* sig, double* coords);
 int hv_mesh_stats(const hv_mesh* mesh, hv_stats_t* out);
 int hv_mesh_free(hv_mesh* mesh);
+/* Replaces refine!(VG, xs) -> systematic_refine!(domain, xs) (src/geometry.jl:404-438,
+ * src/meshrefine.jl:183-217) for a non-periodic mesh of all cells: the mesh of the mesh's n
+ * generators followed by the m new points xs_new (d x m host FP64), which get the ids
+ * n+1..n+m as refine! numbers them publicly; boundary plane p becomes n+m+p.  A vertex of the
+ * old mesh is kept when no new point lies in its circumsphere by the reference's rule
+ * (clean_affected!, src/meshrefine.jl:126-150: kept iff |x_sig[1] - r| <= (1 + 1e-7) times the
+ * distance from r to the nearest new point); every vertex holding a new point comes from a walk
+ * of the new cells (voronoi(mesh; Iter = new cells), src/meshrefine.jl:199).  In general
+ * position the result equals hv_build_mesh of the union.  Afterwards the context holds the
+ * union and the old mesh's boundary; the old mesh is unchanged and stays valid.  removed (may
+ * be NULL): the number of old vertices dropped.  Periodic meshes and meshes of some cells:
+ * HV_ERR_UNSUPPORTED. */
+int hv_refine_mesh(hv_ctx* ctx, const hv_opts* opts, const hv_mesh* mesh, int64_t m, const double* xs_new,
+                   hv_mesh** out, int64_t* removed);
+/* The mesh in DatabaseVertexStorage's layout (HeapDataBase, src/hvdatabase.jl:34-110, under
+ * VDBVertexCentral, src/vdbdatabaseref.jl:88-145): every vertex once, in hv_mesh_export order,
+ * as int64 words [d+1, sig..., r...] (r as FP64 bit patterns), 2d+2 words per vertex in heap;
+ * and per generator i = 1..n_ext (hv_mesh_images) the heap positions (1-based, of the vertex's
+ * first word) of the vertices whose key holds i, in CSR form: refs[cell_ptr[i-1] ..
+ * cell_ptr[i]), -pos where i is the key's first entry (the cell that stores the vertex, a
+ * negative entry of its index list), +pos for the other generators (push_ref!).  cell_ptr has
+ * n_ext+1 entries and is always written; heap and refs are written when not NULL (refs needs
+ * cell_ptr[n_ext] entries, at most (d+1) nvert). */
+int hv_mesh_export_db(const hv_mesh* mesh, int64_t* heap, int64_t* cell_ptr, int64_t* refs);
 
 /* Periodic boundaries (cuboid(d; periodic=...), src/boundary.jl:510-534, where the reference
  * adds periodic copies of the nodes near the boundary, src/domain.jl:139-390).  A box whose

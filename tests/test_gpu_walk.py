@@ -197,3 +197,54 @@ def test_interior_seeds_keep_the_walk_shallow(ctx):
     assert st["vertices"] == len(k) == 2 * 200_000 + 2
     assert st["levels"] <= 20
     assert st["unbounded"] == 0 and st["degenerate"] == 0
+
+
+@pytest.mark.parametrize("ppw,floor", [(8, 64), (16, 1024), (32, 4096), (64, 2048), (64, 100000)])
+def test_scan_tiles_do_not_change_the_mesh(ctx, ppw, floor):
+    xs = synth.uniform(5, 2500, 93)
+    b, nrm = synth.cube_planes(5)
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    ref = ctx.build_mesh(precision="bf16")
+    k0, c0 = ref.export()
+    ref.free()
+    m = ctx.build_mesh(precision="bf16", rays_per_block=ppw, packet_floor=floor)
+    k1, c1 = m.export()
+    m.free()
+    assert np.array_equal(k0, k1) and np.array_equal(c0, c1)
+
+
+@pytest.mark.parametrize("d,n", [(3, 60000), (5, 2500)])
+def test_descent_seed_changes_the_traversal_not_the_mesh(ctx, d, n):
+    """hv_opts.seed draws the interior seeds' descent directions (the reference's random
+    descent): every seed gives the same vertex set and coordinates."""
+    xs = synth.uniform(d, n, 95)
+    b, nrm = synth.cube_planes(d)
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    ref = ctx.build_mesh(precision="bf16")
+    k0, c0 = ref.export()
+    ref.free()
+    for seed in (1, 12345, 2**63 + 7):
+        m = ctx.build_mesh(precision="bf16", seed=seed)
+        k1, c1 = m.export()
+        m.free()
+        assert np.array_equal(k0, k1) and np.array_equal(c0, c1)
+
+
+def test_autotuner_picks_candidate_tiles_and_writes_its_cache(ctx, tmp_path):
+    import json
+    xs = synth.uniform(4, 4000, 94)
+    b, nrm = synth.cube_planes(4)
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    path = tmp_path / "tune.jsonl"
+    ppw, floor, ms = ctx.tune("bf16", str(path))
+    assert ppw in (32, 64) and floor in (1024, 2048, 4096) and ms > 0
+    rec = json.loads(path.read_text().splitlines()[-1])
+    assert rec["d"] == 4 and rec["precision"] == 2 and rec["size_class"] == 11
+    assert rec["rays_per_block"] == ppw and rec["packet_floor"] == floor
+    # builds after tuning use the tuned tiles and give the same mesh
+    k, c, _ = build(ctx, xs, "bf16")
+    ok, _, _ = Oracle(xs, b, nrm).voronoi()
+    assert np.array_equal(rows_view(k), rows_view(ok))
