@@ -866,10 +866,15 @@ struct Top2 {
     }
 };
 
-// exact t of generator j (sorted index) with coordinates xk, |x|^2 = xxj (src/raycast.jl:424-431)
+// exact t of generator j (sorted index) with coordinates xk, |x|^2 = xxj (src/raycast.jl:424-431).
+// Every candidate ahead of the edge competes, also one closer than plane_tol: the reference
+// drops such a candidate, but its window (upper_t, from the nearest generator ahead) then holds
+// nothing else and the ray ends without a vertex (src/raycast.jl:883-894), which the winner
+// rule decides once the nearest candidate is known.
 template <int D>
 __device__ __forceinline__ void hv_consider(const RayGeo<D>& G, Top2& T, const double* xk, double xxj, int j,
                                             double plane_tol) {
+    (void)plane_tol;
     double ux = 0.0, rxj = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -879,10 +884,7 @@ __device__ __forceinline__ void hv_consider(const RayGeo<D>& G, Top2& T, const d
     if (!(ux > G.cth)) return;
     const double den2 = 2.0 * (ux - G.ux0);
     const double num = fma(-2.0, rxj, xxj + G.K);
-    if (num < den2 * (T.t1 + HV_W)) {
-        const double t = num / den2;
-        if (t >= plane_tol) T.offer(t, j);
-    }
+    if (num < den2 * (T.t1 + HV_W)) T.offer(num / den2, j);
 }
 
 // Start phase, one thread per ray: boundary planes (mirror pseudo-generators,
@@ -915,8 +917,7 @@ __global__ __launch_bounds__(256) void k_ray_init(DevGeo g0, GeoSet F, DevRays R
             nr = fma(nk, G.r[k], nr);
         }
         if (!(G.ux0 + 2.0 * dist * nu > G.cth)) continue;
-        const double tp = (g.ph[p] - nr) / nu;
-        if (tp >= plane_tol) T.offer(tp, id);
+        T.offer((g.ph[p] - nr) / nu, id);   // one closer than plane_tol ends the ray (hv_consider)
     }
     // the winner is a Delaunay neighbour of every generator of the edge: scan their lists.
     // FP32 pre-ranking on the difference form t = (|v|^2 + 2<v, x0 - r>) / (2<u, v>), v = x - x0
@@ -2036,7 +2037,10 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g0, 
             double s1, c1;
             double tw = t_ref(w1, T.t1, s1, c1);
             int st = HV_ST_FOUND;
-            if (T.i2 >= 0) {
+            // the nearest candidate closer than plane_tol: the reference's window is empty, no
+            // vertex (src/raycast.jl:883-894 drops it and upper_t excludes every other)
+            if (!(tw >= plane_tol)) st = HV_ST_NONE;
+            else if (T.i2 >= 0) {
                 const int w2 = orig(T.i2);
                 double s2, c2;
                 const double tw2 = t_ref(w2, T.t2, s2, c2);
@@ -2045,12 +2049,25 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g0, 
                 for (int k = 0; k < D; ++k) rn = fma(G.r[k], G.r[k], rn);
                 const double full_error = (tw * R.du[ray] + sqrt(rn) * 1e-15) / c1;
                 const double window = fmin(1e-7, (10.0 + D) * full_error);
+                double to = tw2, so = s2;   // the candidate that does not win
                 if (tw2 < tw || tw2 <= tw + window) {
                     // near tie: the larger <u, x - x0> wins (src/raycast.jl:903-916)
                     st = HV_ST_TIE;
                     tie = true;
-                    if (s2 > s1) { w1 = w2; tw = tw2; }
+                    if (s2 > s1) { to = tw; so = s1; w1 = w2; tw = tw2; }
                 }
+                // The reference's key takes every candidate whose distance from the new vertex is
+                // within rounding of the key's own (src/raycast.jl:926-950): the other candidate
+                // inside the chosen sphere (to < tw), or on it -- |x_o - r|^2 - R^2 = 2 (to - tw)
+                // <u, x_o - x0> within a few ulps of R^2 -- makes a key longer than d + 1, a
+                // vertex out of general position, which the device walk reports instead.
+                double R2 = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const double a = x0t[k] - fma(tw, G.u[k], G.r[k]);
+                    R2 = fma(a, a, R2);
+                }
+                if (to < tw || (to - tw) * so <= 16.0 * 2.220446049250313e-16 * R2) st = HV_ST_SINGULAR;
             }
             win[rid] = w1;
             status[rid] = st;
@@ -3284,7 +3301,8 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     if (hctr[9]) return hv_fail(c, HV_ERR_CAPACITY, "vertex hash overflow");
     if (hctr[5]) return hv_fail(c, HV_ERR_CAPACITY, "edge hash overflow");
     if (hctr[11])
-        return hv_fail(c, HV_ERR_DEGENERATE, "a new vertex has a singular circumcentre system (input not in general position)");
+        return hv_fail(c, HV_ERR_DEGENERATE, "a new vertex is not in general position (more than d + 1 generators on its "
+                                             "sphere within rounding, or a singular circumcentre system)");
     st.levels = levels;
     st.vertices = nlive;
     st.pairs_scanned = (int64_t)hctr[0];

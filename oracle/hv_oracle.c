@@ -26,6 +26,7 @@
 
 #define MAXD 8
 #define MAXSIG 64
+#define ORC_NSTATS 12   /* the counters of oracle_py.STAT_NAMES */
 
 static __thread char g_err[512];
 static void set_err(const char* msg) { snprintf(g_err, sizeof g_err, "%s", msg); }
@@ -72,7 +73,7 @@ struct orc_mesh {
     int64_t* htab; int64_t hmask;
     /* per-cell vertex lists */
     int64_t** cell_v; int64_t* cell_n; int64_t* cell_cap;
-    int64_t stats[10];
+    int64_t stats[ORC_NSTATS];
 };
 
 struct orc_ctx {
@@ -93,7 +94,7 @@ struct orc_ctx {
     int64_t emask, ecount; uint32_t gen;
     uint64_t rng;
     orc_mesh* mesh;   /* mesh being built (stats) */
-    int64_t stats[10];
+    int64_t stats[ORC_NSTATS];
 };
 
 static void* xmalloc(size_t n) { void* p = malloc(n ? n : 1); if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); } return p; }
@@ -517,15 +518,19 @@ static int64_t raycast_hp(orc_ctx* c, const int64_t* sig, int lsig, const double
     }
     upper_t += fmin(10e-8, (10 + d) * full_error);
     double max_dist = 0.0; int64_t generator = -1;
+    int in_window = 0;
     for (int64_t k = 0; k < lidss; ++k) {
         if (ts[k] > upper_t) ts[k] = 0.0;
         else if (ts[k] > 0.0) {
+            ++in_window;
             double s = 0; const double* xk = c->x + idss[k] * d;
             for (int q = 0; q < d; ++q) s += u[q] * (xk[q] - x0[q]);
             ts[k] = s;
             if (s > max_dist) { max_dist = s; generator = idss[k]; }
         }
     }
+    /* a near tie: more than one candidate within the window, the winner by <u, x - x0> */
+    if (walkray_mode && in_window > 1) c->stats[10]++;
     if (generator < 0) { *t_out = INFINITY; return -1; }
     t = get_t_hp(c, r, u, x0, c->x + generator * d);
     double r2[MAXD]; for (int k = 0; k < d; ++k) r2[k] = r[k] + t * u[k];
@@ -981,7 +986,7 @@ int orc_voronoi(orc_ctx* c, int64_t cell_begin, int64_t cell_end, uint64_t seed,
         if (rc < 0) break;
     }
     free(queue);
-    for (int i = 0; i < 10; ++i) m->stats[i] += c->stats[i];
+    for (int i = 0; i < ORC_NSTATS; ++i) m->stats[i] += c->stats[i];
     if (rc < 0) { orc_mesh_free(m); return rc; }
     *out = m;
     return 0;
@@ -998,6 +1003,8 @@ int orc_mesh_export(const orc_mesh* m, int64_t* keys, double* coords) {
     return 0;
 }
 int orc_mesh_stats(const orc_mesh* m, int64_t* out) { memcpy(out, m->stats, sizeof m->stats); return 0; }
+/* the counters of the context's last orc_walkray */
+int orc_ctx_stats(const orc_ctx* c, int64_t* out) { memcpy(out, c->stats, sizeof c->stats); return 0; }
 void orc_mesh_free(orc_mesh* m) {
     if (!m) return;
     free(m->koff); free(m->klen); free(m->kpool); free(m->coords); free(m->htab);
@@ -1011,6 +1018,7 @@ int orc_walkray(orc_ctx* c, const int64_t* sig1, int drop, const double* r, int6
     int d = c->d;
     int64_t sig[MAXSIG];
     for (int j = 0; j <= d; ++j) sig[j] = sig1[j] - 1;
+    memset(c->stats, 0, sizeof c->stats);
     int64_t cell = cell1 - 1;
     if (cell < 0 || cell >= c->n) { set_err("bad cell"); return -1; }
     memset(c->active, 0, c->np);

@@ -36,6 +36,7 @@ int orc_mesh_export(const orc_mesh* m, int64_t* keys, double* coords);
 /* out[0..9]: walkrays, nn calls, inrange calls, descents, corrections (tolerance breach),
  * irreparable, duplicate pushes, fraud rejections, unbounded rays, hp full_mode casts */
 int orc_mesh_stats(const orc_mesh* m, int64_t* out);
+int orc_ctx_stats(const orc_ctx* c, int64_t* out);
 void orc_mesh_free(orc_mesh* m);
 
 /* One walkray from vertex (sig, r) along the edge that drops sig[drop] (0-based position),

@@ -45,6 +45,7 @@ def lib():
         L.orc_mesh_maxsig.argtypes = [C.c_void_p]
         L.orc_mesh_export.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_double)]
         L.orc_mesh_stats.argtypes = [C.c_void_p, P(C.c_int64)]
+        L.orc_ctx_stats.argtypes = [C.c_void_p, P(C.c_int64)]
         L.orc_mesh_free.argtypes = [C.c_void_p]
         L.orc_walkray.argtypes = [C.c_void_p, P(C.c_int64), C.c_int, P(C.c_double), C.c_int64,
                                   P(C.c_int64), P(C.c_int), P(C.c_double), P(C.c_double)]
@@ -68,7 +69,7 @@ def _ip(a):
 
 
 STAT_NAMES = ["walkrays", "nn_calls", "inrange_calls", "descents", "tolerance_breach",
-              "irreparable", "duplicates", "fraud", "unbounded", "hp_full_mode"]
+              "irreparable", "duplicates", "fraud", "unbounded", "hp_full_mode", "near_ties", "unused"]
 
 
 class Oracle:
@@ -108,11 +109,17 @@ class Oracle:
             keys = np.zeros((nv, ms), dtype=np.int64)
             coords = np.zeros((nv, self.d), dtype=np.float64)
             L.orc_mesh_export(m, _ip(keys), _dp(coords))
-            st = np.zeros(10, dtype=np.int64)
+            st = np.zeros(len(STAT_NAMES), dtype=np.int64)
             L.orc_mesh_stats(m, _ip(st))
         finally:
             L.orc_mesh_free(m)
         return keys, coords, dict(zip(STAT_NAMES, st.tolist()))
+
+    def walkray_stats(self):
+        """Counters (STAT_NAMES) of the last walkray call."""
+        st = np.zeros(len(STAT_NAMES), dtype=np.int64)
+        lib().orc_ctx_stats(self._h, _ip(st))
+        return dict(zip(STAT_NAMES, st.tolist()))
 
     def walkray(self, sig, drop: int, r, cell: int):
         """One reference walkray; returns (new_key, coords, u) or None if unbounded."""

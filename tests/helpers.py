@@ -138,3 +138,26 @@ def periodic_mesh_labels(mesh):
     orig, shift = mesh.images()
     labs = vertex_labels(keys, len(orig), orig - 1, shift)
     return dict(zip(labs, coords))
+
+
+def near_tie_points(delta: float, swap: bool = False, theta: float = np.pi / 6):
+    """A 2D point set whose ray from the vertex of generators 1, 2, 3 (centre (0.5, 0.425)),
+    walking away from generator 3 along +y, meets two candidates whose end points differ by
+    `delta` in t: generator 4 (ids 1-based) at t = 0.175 and generator 5 at 0.175 + delta.
+    Without swap, 4 lies at the top of its circle (the larger <u, x - x0> as well as the
+    smaller t); with swap, 5 does.  Eight more points keep the rest of the box away."""
+    a, b, c = np.array([0.4, 0.5]), np.array([0.6, 0.5]), np.array([0.5, 0.3])
+    r4, r5 = np.hypot(0.1, 0.1), np.hypot(0.1, 0.1 + delta)
+    top = lambda cy, rr: np.array([0.5, cy + rr])
+    side = lambda cy, rr: np.array([0.5 + rr * np.cos(theta), cy + rr * np.sin(theta)])
+    p, q = (side(0.6, r4), top(0.6 + delta, r5)) if swap else (top(0.6, r4), side(0.6 + delta, r5))
+    fill = np.array([[0.1, 0.1], [0.9, 0.1], [0.1, 0.9], [0.9, 0.9], [0.2, 0.5], [0.8, 0.5], [0.5, 0.05], [0.5, 0.97]])
+    return np.vstack([a, b, c, p, q, fill]), np.array([0.5, 0.425])
+
+
+def lattice(d: int, k: int, eps: float = 0.0, seed: int = 1):
+    """k^d cell centres of the unit cube, each moved by U(-eps, eps) per coordinate."""
+    g = (np.stack(np.meshgrid(*[np.arange(k)] * d, indexing="ij"), -1).reshape(-1, d) + 0.5) / k
+    if eps:
+        g = g + eps * np.random.default_rng(seed).uniform(-1, 1, g.shape)
+    return g

@@ -1,0 +1,95 @@
+"""GPU parity at the edge of general position (the reference's winner rule,
+src/raycast.jl:877-950).
+
+Bars: on a constructed near tie the device resolves the ray as the oracle's walkray does
+(same key, status 2 = near tie resolved, coordinates within 1e-10), and reports no tie where
+the oracle has none.  With the candidates swapped (the farther one has the larger <u, x - x0>)
+the reference's result depends on the order of its range query: its single pass over the
+candidates (src/raycast.jl:883-894) lowers the window's upper end to each t it meets, so a
+farther candidate met after the nearer one is dropped (the oracle's order: no tie), and one
+met before it stays and wins, making the nearer one, inside the winner's sphere, a further
+member of the key (a vertex out of general position).  The device takes every candidate
+within the window whatever the order and reports that vertex as out of general position
+(status < 0).  Perturbed lattices that the oracle walks in general position give the
+oracle's key set bit for bit in every precision; an exact lattice, where the reference's keys
+grow past d + 1, fails with HV_ERR_DEGENERATE (-4) after every kernel has exited, and the
+context builds the next mesh normally.
+"""
+import numpy as np
+import pytest
+
+from helpers import lattice, near_tie_points, rows_view
+from hvgpu import synth
+from hvgpu._lib import Context, HVError
+from oracle_py import Oracle
+
+pytestmark = pytest.mark.gpu
+
+PRECISIONS = ["fp64", "fp32", "bf16"]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return Context(0)
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("delta,tie", [(1e-13, False), (3e-15, True), (1e-15, True)])
+def test_constructed_near_tie(ctx, prec, delta, tie):
+    xs, r = near_tie_points(delta, False)
+    b, nrm = synth.cube_planes(2)
+    o = Oracle(xs, b, nrm)
+    key, r2, _ = o.walkray(np.array([1, 2, 3]), 2, r, 1)
+    assert (o.walkray_stats()["near_ties"] == 1) == tie
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    out_sig, out_r, _, out_st = ctx.raycast_batch(np.array([[1, 2, 3]]), np.array([2]), r[None, :], precision=prec)
+    assert out_sig[0].tolist() == key.tolist()
+    assert int(out_st[0]) == (2 if tie else 0)
+    assert np.max(np.abs(out_r[0] - r2)) < 1e-10
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_swapped_near_tie_is_not_general(ctx, prec):
+    xs, r = near_tie_points(3e-15, True)
+    b, nrm = synth.cube_planes(2)
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    _, _, _, out_st = ctx.raycast_batch(np.array([[1, 2, 3]]), np.array([2]), r[None, :], precision=prec)
+    assert int(out_st[0]) < 0
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("d,k,eps", [(2, 30, 1e-9), (2, 30, 1e-10), (2, 30, 5e-11), (2, 30, 2e-11), (3, 8, 1e-5)])
+def test_perturbed_lattice_equals_oracle(ctx, prec, d, k, eps):
+    xs = lattice(d, k, eps)
+    b, nrm = synth.cube_planes(d)
+    ok, oc, _ = Oracle(xs, b, nrm).voronoi()
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    m = ctx.build_mesh(precision=prec)
+    keys, coords = m.export()
+    st = m.stats()
+    m.free()
+    assert np.array_equal(rows_view(keys), rows_view(ok))
+    assert st["degenerate"] == 0
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("d,k", [(2, 30), (3, 8)])
+def test_exact_lattice_is_degenerate(ctx, prec, d, k):
+    xs = lattice(d, k)
+    b, nrm = synth.cube_planes(d)
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    with pytest.raises(HVError) as e:
+        ctx.build_mesh(precision=prec)
+    assert e.value.code == -4
+    # the context is usable afterwards
+    ys = synth.uniform(d, 500, 7)
+    ctx.set_points(ys)
+    m = ctx.build_mesh(precision=prec)
+    keys, _ = m.export()
+    m.free()
+    ok, _, _ = Oracle(ys, b, nrm).voronoi()
+    assert np.array_equal(rows_view(keys), rows_view(ok))

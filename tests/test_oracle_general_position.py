@@ -1,0 +1,69 @@
+"""The oracle at the edge of general position (the reference's winner rule,
+src/raycast.jl:877-950): a constructed near tie resolved by the larger <u, x - x0>, perturbed
+lattices that stay in general position, and an exact lattice, which the oracle reports as a
+non-general vertex (its FastEdgeIterator path, src/edgeiterate.jl:82-295, is not restated).
+
+The perturbation scan pins where the rule's two bands sit: the near-tie window
+(10 + d) * full_error is narrower than the band within which the reference puts a further
+generator into the key, so a perturbed lattice goes from no near ties straight to keys longer
+than d + 1 (2D: general down to a perturbation of 2e-11, 3D: at 1e-5, not at 1e-6).
+"""
+import numpy as np
+import pytest
+
+from helpers import lattice, near_tie_points
+from hvgpu import synth
+from oracle_py import Oracle
+
+
+def walk(delta, swap):
+    xs, r = near_tie_points(delta, swap)
+    b, nrm = synth.cube_planes(2)
+    o = Oracle(xs, b, nrm)
+    res = o.walkray(np.array([1, 2, 3]), 2, r, 1)
+    return res, o.walkray_stats()
+
+
+@pytest.mark.parametrize("delta,ties", [(1e-12, 0), (1e-13, 0), (3e-15, 1), (1e-15, 1)])
+def test_constructed_near_tie(delta, ties):
+    (key, r2, _), st = walk(delta, False)
+    assert key.tolist() == [1, 2, 4]
+    assert st["near_ties"] == ties
+    # the new vertex is the centre of generator 4's circle, (0.5, 0.6)
+    assert np.max(np.abs(r2 - [0.5, 0.6])) < 1e-13
+
+
+@pytest.mark.parametrize("delta", [1e-13, 3e-15, 1e-15])
+def test_swapped_candidates_are_no_tie(delta):
+    # generator 5 at the top of its circle: it is not within the window of 4 (the first found)
+    (key, _, _), st = walk(delta, True)
+    assert key.tolist() == [1, 2, 4] and st["near_ties"] == 0
+
+
+@pytest.mark.parametrize("d,k,eps", [(2, 30, 1e-9), (2, 30, 2e-11), (3, 8, 1e-5)])
+def test_perturbed_lattice_stays_general(d, k, eps):
+    xs = lattice(d, k, eps)
+    b, nrm = synth.cube_planes(d)
+    keys, _, st = Oracle(xs, b, nrm).voronoi()
+    assert keys.shape[1] == d + 1 and st["near_ties"] == 0 and len(keys) > k ** d
+
+
+@pytest.mark.parametrize("d,k,eps", [(2, 30, 0.0), (2, 30, 1e-11), (3, 8, 0.0), (3, 8, 1e-6)])
+def test_lattice_is_not_general(d, k, eps):
+    xs = lattice(d, k, eps)
+    b, nrm = synth.cube_planes(d)
+    with pytest.raises(RuntimeError, match="non-general"):
+        Oracle(xs, b, nrm).voronoi()
+
+
+@pytest.mark.parametrize("eps", [1e-10, 5e-11])
+def test_rays_along_edges_shorter_than_plane_tolerance_end_without_a_vertex(eps):
+    """A near-co-circular square has a Voronoi edge of length ~eps: a ray along it whose
+    nearest candidate comes within t < plane_tolerance (1e-12) finds no vertex (its window is
+    empty, src/raycast.jl:883-894); both end points are reached through other edges, and the
+    mesh is the 2 (k - 1)^2 + 4 k vertices of a triangulated lattice."""
+    xs = lattice(2, 30, eps)
+    b, nrm = synth.cube_planes(2)
+    keys, _, st = Oracle(xs, b, nrm).voronoi()
+    assert st["unbounded"] > 0
+    assert len(keys) == 2 * 29 ** 2 + 4 * 30
