@@ -201,9 +201,8 @@ def bench_batched(args):
 
 def bench_mc_points(args):
     """Point-sample Monte-Carlo volumes and areas (1e8 stratified samples) at 6D N=50k in the
-    periodic unit cube cuboid(6) (the mesh of the originals' cells with their periodic images).
-    The exact polytope integrator packs generator ids in 21 bits, which the 3M generators of the
-    N=50k extended set exceed, so the estimator is compared with it at 6D N=10k periodic."""
+    periodic unit cube cuboid(6) (the mesh of the originals' cells with their periodic images),
+    against the exact polytope integrator on the same mesh."""
     from hvgpu import synth
     from hvgpu.geometry import cuboid
     from hvgpu._lib import Context
@@ -219,32 +218,38 @@ def bench_mc_points(args):
     mesh_ms = (time.perf_counter() - t0) * 1e3
     nv = m.stats()["vertices"]
     n_ext = len(m.images()[0])
-    times = []
+    times, kms, pairs = [], [], []
     mc = None
     for k in range(args.warmup + args.steps):
         t0 = time.perf_counter()
         mc = ctx.integrate(m, "points", nsamples=ns, seed=17 + k)
         if k >= args.warmup:
             times.append(time.perf_counter() - t0)
+            kms.append(mc["ms_kernel"])
+            pairs.append(mc["pairs"])
+    t0 = time.perf_counter()
+    ex = ctx.integrate(m, "polytope")
+    exact_ms = (time.perf_counter() - t0) * 1e3
     m.free()
     sec = min(times)
     vsum = float(np.sum(mc["volumes"]))
-    # the estimator against the exact integrator on the same mesh, 6D N=10k periodic
-    xs2 = synth.uniform(d, 10_000, 5)
-    ctx.set_points(xs2)
-    ctx.set_boundary(base, normal, bc)
-    m2 = ctx.build_mesh(precision=args.precision)
-    ex = ctx.integrate(m2, "polytope")
-    mc2 = ctx.integrate(m2, "points", nsamples=ns, seed=17)
-    m2.free()
-    A, Am = ex["areas"], mc2["areas"]
-    vol_l1 = float(np.sum(np.abs(mc2["volumes"] - ex["volumes"])) / np.sum(ex["volumes"]))
-    area_l1 = float(np.sum(np.abs(Am - A)) / np.sum(A))
+    vol, A, Am = ex["volumes"], ex["areas"], mc["areas"]
+    same_rows = bool(np.array_equal(mc["nbr_ptr"], ex["nbr_ptr"]) and np.array_equal(mc["neighbors"], ex["neighbors"]))
+    vol_l1 = float(np.sum(np.abs(mc["volumes"] - vol)) / np.sum(vol))
+    sigma = np.sqrt(vol * (1.0 - vol) / ns)
+    vol_z = float(np.max(np.abs(mc["volumes"] - vol) / sigma))
+    area_l1 = float(np.sum(np.abs(Am - A)) / np.sum(A)) if same_rows else None
     area_bias = float((np.sum(Am) - np.sum(A)) / np.sum(A))
+    # roofline of the sampling kernel: one (sample, generator) distance = 2d FP32 flops
+    avg_ms = float(np.mean(kms))
+    avg_pairs = float(np.mean(pairs))
+    achieved = avg_pairs * 2 * d / (avg_ms * 1e-3) / 1e12
+    peak = 157.3
     log(f"[mc-points] 6D N=50k periodic: mesh {mesh_ms:.0f} ms ({nv} vertices, {n_ext} generators with images); "
-        f"{ns:.0e} samples: {sec * 1e3:.0f} ms (device {mc['ms']:.0f} ms), volume sum {vsum:.15f}")
-    log(f"[mc-points] 6D N=10k periodic: exact integrator {ex['ms']:.0f} ms, point MC {mc2['ms']:.0f} ms; "
-        f"volume L1 {vol_l1:.4f}, area L1 {area_l1:.4f}, area bias {area_bias:+.4f}")
+        f"{ns:.0e} samples: {sec * 1e3:.0f} ms (device {mc['ms']:.0f} ms, sampling kernel {avg_ms:.0f} ms, "
+        f"{avg_pairs / ns:.0f} distances per sample), volume sum {vsum:.15f}")
+    log(f"[mc-points] exact integrator on the same mesh {exact_ms:.0f} ms; volume L1 {vol_l1:.4f} (max {vol_z:.2f} sigma), "
+        f"area L1 {area_l1}, area bias {area_bias:+.4f}")
     out = {
         "metric": "Monte-Carlo point samples/s (nearest and second-nearest generator per sample), 6D N=50k periodic",
         "value": ns / sec,
@@ -256,14 +261,20 @@ def bench_mc_points(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 distances, f64 decisions",
         "data": "synthetic",
         "config": {"workload": "6D N=50000 uniform in [0,1]^6 (seed 4), cuboid(6) periodic in every direction, "
                                "1e8 stratified samples, slab 0.03", "vertices": nv, "generators_with_images": n_ext,
                    "mesh_ms": mesh_ms},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None, "kernel": f"k_mc_points<{d}>", "avg_launch_ms": avg_ms,
+                     "pairs_per_launch": avg_pairs, "flops_per_pair": 2 * d,
+                     "note": "peak = published FP32 vector rate; this build disables packed FP32, which halves the "
+                             "reachable VALU rate to 78.6 TFLOP/s"},
         "volume_sum": vsum,
-        "vs_exact_at_6d_10k_periodic": {"exact_integrator_ms": ex["ms"], "points_mc_ms": mc2["ms"],
-                                        "volume_rel_l1": vol_l1, "area_rel_l1": area_l1, "area_total_bias": area_bias},
+        "vs_exact_on_the_same_mesh": {"exact_integrator_ms": exact_ms, "same_neighbour_rows": same_rows,
+                                      "volume_rel_l1": vol_l1, "volume_max_sigma": vol_z,
+                                      "area_rel_l1": area_l1, "area_total_bias": area_bias},
     }
     print(json.dumps(out), flush=True)
 
@@ -335,6 +346,42 @@ def bench_integrators(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_clipped(args):
+    """The 5D N=100k mixture with out-of-cube cluster samples clipped onto the cube's faces
+    instead of drawn again: generators on a boundary plane.  Such a generator coincides with
+    its mirror image, the reference's walk of its cell reaches a vertex with more than d + 1
+    generators (its non-general path), and the device build reports HV_ERR_DEGENERATE; the
+    line gives that outcome per precision (value null when no mesh is built)."""
+    from hvgpu import synth
+    from hvgpu._lib import Context, HVError
+
+    xs = synth.gaussian_mixture_5d(clip=True)
+    on = int(np.sum(((xs == 0.0) | (xs == 1.0)).any(axis=1)))
+    b, nrm = synth.cube_planes(5)
+    ctx = Context(0)
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    res = {}
+    for prec in ("bf16", "fp64"):
+        t0 = time.perf_counter()
+        try:
+            m = ctx.build_mesh(precision=prec)
+            res[prec] = {"status": "ok", "vertices": m.stats()["vertices"], "ms": (time.perf_counter() - t0) * 1e3}
+            m.free()
+        except HVError as e:
+            res[prec] = {"status": f"error {e.code}", "message": str(e), "ms": (time.perf_counter() - t0) * 1e3}
+        log(f"[clipped] {prec}: {res[prec]}")
+    ok = res["bf16"]["status"] == "ok"
+    out = {"metric": "Voronoi vertices/s at 5D N=100k, mixture clipped onto the cube (generators on boundary planes)",
+           "value": res["bf16"]["vertices"] / (res["bf16"]["ms"] * 1e-3) if ok else None, "unit": "vertices/s",
+           "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16+f64", "data": "synthetic",
+           "config": {"workload": "5D N=100000 Gaussian mixture (as the headline workload, seed 2) with out-of-cube "
+                                  "samples clipped to [0,1]^5", "generators_on_a_plane": on},
+           "outcome": res}
+    print(json.dumps(out), flush=True)
+
+
 def bench_scale(args):
     """Large uniform inputs in the unit cube: best-of-steps build time for BF16 and FP64 after a
     warm-up build, levels, and the key tables of the two precisions compared."""
@@ -395,6 +442,8 @@ def main():
     ap.add_argument("--workers", type=int, default=8, help="host workers (streams) of the batched call")
     ap.add_argument("--legs", default="sequential,workers,fused",
                     help="--batched: legs to run (a subset prints timings only, for profiling)")
+    ap.add_argument("--clipped", action="store_true",
+                    help="the headline mixture with out-of-cube samples clipped onto the faces (its own line)")
     ap.add_argument("--mc-points", action="store_true",
                     help="the point-sample Monte-Carlo integrator instead: 6D N=50k uniform, cuboid(6) periodic, "
                          "1e8 samples; against the exact integrator at 6D N=10k periodic")
@@ -425,6 +474,8 @@ def main():
     from hvgpu import synth
     from hvgpu._lib import Context
 
+    if args.clipped:
+        return bench_clipped(args)
     if args.batched:
         return bench_batched(args)
     if args.mc_points:

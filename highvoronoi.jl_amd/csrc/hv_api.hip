@@ -486,6 +486,9 @@ static int fusable(int d, int64_t n, const double* xs, int npl, const double* ba
         for (int k = 0; k < d; ++k) {
             const double x = xs[i * d + k];
             if (!(x >= lo[k] && x <= hi[k])) return HV_ERR_DOMAIN;
+            // on a face (the walk's rule for a generator on a plane): out of general position
+            if (hi[k] - x <= 1e-12 * (1.0 + std::fabs(hi[k])) || x - lo[k] <= 1e-12 * (1.0 + std::fabs(lo[k])))
+                return HV_ERR_DEGENERATE;
         }
     return HV_OK;
 }
@@ -775,6 +778,10 @@ int hv_integrate_mc_points(hv_ctx* c, const hv_mesh* mesh, int64_t nsamples, uin
 
 int hv_integral_counts(const hv_integral* I, int64_t* ncells, int64_t* nnbr) {
     return hv_integral_counts_impl(I, ncells, nnbr);
+}
+
+int hv_integral_work(const hv_integral* I, int64_t* work, double* ms_kernel) {
+    return hv_integral_work_impl(I, work, ms_kernel);
 }
 
 int hv_integral_export(const hv_integral* I, double* vol, int64_t* nbr_ptr, int64_t* nbr, double* area, double* ms) {

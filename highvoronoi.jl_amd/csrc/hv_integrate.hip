@@ -915,6 +915,8 @@ struct hv_integral {
     std::vector<double> vol, area;
     std::vector<int64_t> ptr, nbr;
     double ms = 0.0;
+    int64_t work = 0;                  // point samples: (sample, generator) distances evaluated
+    double ms_kernel = 0.0;            // point samples: device time of the sampling kernel
     int nf = 0;                        // integrand components (0: none)
     std::vector<double> bulk, iface;   // nf x N, nf x rows
 };
@@ -1735,7 +1737,9 @@ int hv_integrate_mc_points_impl(hv_ctx* c, const hv_mesh* mesh, int64_t nsamples
     HVCK(hipMemsetAsync(pcnt, 0, sizeof(unsigned long long) * 2 * std::max<int64_t>(npair, 1), c->stream));
     int64_t n = 0;
     double eps = 0.0, vbox = 0.0;
-    rc = hv_mc_points_impl<D>(c, nsamples, seed, slab, ptr, pkey, ccnt, pcnt, &n, &eps, &vbox);
+    int64_t work = 0;
+    double kms = 0.0;
+    rc = hv_mc_points_impl<D>(c, nsamples, seed, slab, ptr, pkey, ccnt, pcnt, &n, &eps, &vbox, &work, &kms);
     if (rc) return rc;
     k_mcp_finish<<<nblk(std::max<int64_t>(c->N, npair), 256), 256, 0, c->stream>>>(
         g.nreal, ptr, pkey, npair, ccnt, pcnt, (double)n, eps, vbox, vol, area);
@@ -1744,7 +1748,19 @@ int hv_integrate_mc_points_impl(hv_ctx* c, const hv_mesh* mesh, int64_t nsamples
     HVCK(hipEventSynchronize(c->ev[1]));
     float ms = 0.f;
     HVCK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    return finish<D>(c, g.nreal, pkey, area, vol, ptr, npair, ms, out);
+    rc = finish<D>(c, g.nreal, pkey, area, vol, ptr, npair, ms, out);
+    if (rc == HV_OK) {
+        (*out)->work = work;
+        (*out)->ms_kernel = kms;
+    }
+    return rc;
+}
+
+int hv_integral_work_impl(const hv_integral* I, int64_t* work, double* ms_kernel) {
+    if (!I) return HV_ERR_ARG;
+    if (work) *work = I->work;
+    if (ms_kernel) *ms_kernel = I->ms_kernel;
+    return HV_OK;
 }
 
 int hv_integral_counts_impl(const hv_integral* I, int64_t* ncells, int64_t* nnbr) {

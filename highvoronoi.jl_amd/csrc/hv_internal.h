@@ -74,6 +74,7 @@ struct hv_ctx {
     // the search tree of the last tree build over the context's own points and planes (the
     // ray-level entry point without allocation runs on it); cleared when they change
     bool tree_ok = false;
+    bool tree_on_plane = false;   // the last tree build found a generator on a boundary plane
     int tree_tables = 0;                 // HV_PREC_* tables built beside it
     int tree_leaf = 0;
     DevGeo tree_g{};
@@ -176,7 +177,8 @@ int hv_build_forest_meshes(hv_ctx* c, const hv_opts* o, Forest& fo);
 template <int D>
 int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, const int64_t* ptr,
                       const uint64_t* pkey, unsigned long long* ccnt, unsigned long long* pcnt, int64_t* nused,
-                      double* eps_out, double* vbox_out);
+                      double* eps_out, double* vbox_out, int64_t* pairs_out,
+                      double* kernel_ms);
 template <int D>
 int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
                           const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status);
@@ -208,6 +210,7 @@ template <int D>
 int hv_integrate_mc_points_impl(hv_ctx* c, const hv_mesh* mesh, int64_t nsamples, uint64_t seed, double slab,
                                 hv_integral** out);
 int hv_integral_counts_impl(const hv_integral* I, int64_t* ncells, int64_t* nnbr);
+int hv_integral_work_impl(const hv_integral* I, int64_t* work, double* ms_kernel);
 int hv_integral_export_impl(const hv_integral* I, double* vol, int64_t* nbr_ptr, int64_t* nbr, double* area,
                             double* ms);
 void hv_integral_free_impl(hv_integral* I);

@@ -64,17 +64,22 @@ __global__ void k_points_in(const double* __restrict__ xin, int N, DevGeo g, dou
         p[k] = xin[(int64_t)i * D + k];
         xo[(int64_t)i * D + k] = p[k];
     }
-    // check_boundary (src/boundary.jl:427-441): <b_p - x, n_p> >= 0 for every plane
-    bool ok = true;
+    // check_boundary (src/boundary.jl:427-441): <b_p - x, n_p> >= 0 for every plane (bit 0 of
+    // *bad when not).  Bit 1: a generator on a plane (within 1e-12 relative), which coincides
+    // with its own mirror image: the walk of its cell meets vertices with more than d + 1
+    // generators on their sphere (the reference's non-general path), reported as degenerate.
+    bool ok = true, on = false;
     for (int q = 0; q < g.P; ++q) {
         double s = g.ph[q];
 #pragma unroll
         for (int k = 0; k < D; ++k) s -= g.pn[q * D + k] * p[k];
         ok = ok && (s >= 0.0);
+        on = on || (s <= 1e-12 * (1.0 + fabs(g.ph[q])));
     }
 #pragma unroll
     for (int k = 0; k < D; ++k) ok = ok && isfinite(p[k]);
-    if (!ok) atomicAdd(bad, 1);
+    if (!ok) atomicOr(bad, 1);
+    else if (on) atomicOr(bad, 2);
     code[i] = hv_morton<D>(g, p);
     idx[i] = i;
 }
@@ -1245,7 +1250,8 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
                                                   int nreal, const int* __restrict__ porig,
                                                   const uint32_t* __restrict__ pshift,
                                                   const unsigned long long* __restrict__ pkeys,
-                                                  const int* __restrict__ pvals, uint64_t pmaskh) {
+                                                  const int* __restrict__ pvals, uint64_t pmaskh,
+                                                  unsigned long long* __restrict__ npairs) {
     // samples are drawn in `box`.  Periodic directions (pmask): the working set holds images
     // of the nreal originals (porig / pshift per id: the original and the shift, 4 bits per
     // direction, shift + 8).  A sample whose nearest generator a is an image lies, moved by
@@ -1320,6 +1326,7 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
     int sp = 1;
     stk[0] = 0;
     stkl[0] = 0;
+    unsigned work = 0u;   // (sample, generator) distances this lane evaluated
     while (sp > 0) {
         --sp;
         hv_wave_sync();
@@ -1374,6 +1381,7 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
                     }
                     hv_wave_sync();
                     if (hit) {
+                        work += (unsigned)cnt;
                         for (int q = 0; q < cnt; ++q) {
                             float d2 = 0.f;
 #pragma unroll
@@ -1399,6 +1407,12 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
                 ++sp;
             }
         }
+    }
+    {
+        unsigned long long w = work;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
+        if (lane == 0) atomicAdd(npairs, w);
     }
     // the two nearest in FP64 (the FP32 scan may order a near tie either way)
     auto d2_64 = [&](int j) {
@@ -2384,7 +2398,8 @@ int build_tree(hv_ctx* c, int L, Tree& T, int tables = HV_PREC_FP64) {
     int hbad = 0;
     HVCK(hipMemcpyAsync(&hbad, o.bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HVCK(hipStreamSynchronize(c->stream));
-    if (hbad) return hv_fail(c, HV_ERR_DOMAIN, "a node does not lie in the domain given by the boundary");
+    if (hbad & 1) return hv_fail(c, HV_ERR_DOMAIN, "a node does not lie in the domain given by the boundary");
+    c->tree_on_plane = hbad != 0;   // the walk refuses it; the queries do not mind
     return HV_OK;
 }
 
@@ -2765,8 +2780,10 @@ static int build_forest(hv_ctx* c, int L, int tables, Forest& fo) {
     std::vector<int> hbad(nm);
     HVCK(hipMemcpyAsync(hbad.data(), bad, sizeof(int) * nm, hipMemcpyDeviceToHost, c->stream));
     HVCK(hipStreamSynchronize(c->stream));
-    for (int k = 0; k < nm; ++k)
-        if (hbad[k]) return hv_fail(c, HV_ERR_DOMAIN, "a node does not lie in the domain given by the boundary");
+    for (int k = 0; k < nm; ++k) {
+        if (hbad[k] & 1) return hv_fail(c, HV_ERR_DOMAIN, "a node does not lie in the domain given by the boundary");
+        if (hbad[k]) return hv_fail(c, HV_ERR_DEGENERATE, "a node lies on a boundary plane (it coincides with its mirror image)");
+    }
     fo.F.gs = gs;
     fo.F.mof = mof;
     fo.F.nmesh = nm;
@@ -2851,6 +2868,8 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     } else {
         rc = build_tree<D>(c, L, T, prec);
         if (rc) return rc;
+        if (c->tree_on_plane)
+            return hv_fail(c, HV_ERR_DEGENERATE, "a node lies on a boundary plane (it coincides with its mirror image)");
         if (!c->extended) record_tree(c, T.g, prec, L);
     }
     DevGeo& g = T.g;   // over a forest: unused (every kernel takes its mesh's tree from F)
@@ -4582,7 +4601,8 @@ int hv_inrange_impl(hv_ctx* c, int64_t nq, const double* q, const double* radius
 template <int D>
 int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, const int64_t* ptr,
                       const uint64_t* pkey, unsigned long long* ccnt, unsigned long long* pcnt, int64_t* nused,
-                      double* eps_out, double* vbox_out) {
+                      double* eps_out, double* vbox_out, int64_t* pairs_out,
+                      double* kernel_ms) {
     int pmask = 0;
     int rc = hv_box_periodic_mask(c, D, &pmask);
     if (rc) return rc;
@@ -4669,9 +4689,21 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     const int64_t n = B * S;
     // one work-item per sample, and a dispatch holds at most 2^32 work-items
     if (n > (1ll << 32) - 64) return hv_fail(c, HV_ERR_CAPACITY, "too many samples for one call (at most 2^32)");
+    unsigned long long* npairs = pget<unsigned long long>(c, "mcp_pairs", 1);
+    if (!npairs) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
+    HVCK(hipMemsetAsync(npairs, 0, sizeof(unsigned long long), c->stream));
+    HVCK(hipEventRecord(c->ev[2], c->stream));
     k_mc_points<D><<<(unsigned)(n / 64), 64, 0, c->stream>>>(g, G, S, seed, eps, ptr, pkey, ccnt, pcnt, box, pmask,
-                                                             (int)c->nreal, porig, pshift, pkeys, pvals, hmask);
+                                                             (int)c->nreal, porig, pshift, pkeys, pvals, hmask, npairs);
     HVCK(hipGetLastError());
+    HVCK(hipEventRecord(c->ev[3], c->stream));
+    unsigned long long hp = 0;
+    HVCK(hipMemcpyAsync(&hp, npairs, sizeof(hp), hipMemcpyDeviceToHost, c->stream));
+    HVCK(hipStreamSynchronize(c->stream));
+    float kms = 0.f;
+    HVCK(hipEventElapsedTime(&kms, c->ev[2], c->ev[3]));
+    *pairs_out = (int64_t)hp;
+    *kernel_ms = kms;
     *nused = n;
     *eps_out = eps;
     *vbox_out = vbox;
@@ -4822,7 +4854,7 @@ int hv_refine_merge(hv_ctx* c, const hv_mesh* old, const hv_mesh* S, hv_mesh** o
 #define HV_INST(D)                                                                                 \
     template int hv_mc_points_impl<D>(hv_ctx*, int64_t, uint64_t, double, const int64_t*,         \
                                       const uint64_t*, unsigned long long*, unsigned long long*,   \
-                                      int64_t*, double*, double*);                                 \
+                                      int64_t*, double*, double*, int64_t*, double*);                                 \
     template int hv_build_mesh_impl<D>(hv_ctx*, const hv_opts*, hv_mesh**, const std::vector<int>*);   \
     template int hv_build_forest_meshes<D>(hv_ctx*, const hv_opts*, Forest&);                        \
     template int hv_mesh_export_impl<D>(const hv_mesh*, int64_t*, double*);                        \

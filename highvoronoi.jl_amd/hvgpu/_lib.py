@@ -91,6 +91,7 @@ def lib():
     L.hv_integrate_mc.argtypes = [vp, vp, C.c_int64, C.c_uint64, P(vp)]
     L.hv_integrate_mc_points.argtypes = [vp, vp, C.c_int64, C.c_uint64, C.c_double, P(vp)]
     L.hv_integral_counts.argtypes = [vp, P(C.c_int64), P(C.c_int64)]
+    L.hv_integral_work.argtypes = [vp, P(C.c_int64), P(C.c_double)]
     L.hv_integral_export.argtypes = [vp, P(C.c_double), P(C.c_int64), P(C.c_int64), P(C.c_double), P(C.c_double)]
     L.hv_integral_free.argtypes = [vp]
     L.hv_integrate_polytope_f.argtypes = [vp, vp, C.c_int32, INTEGRAND, vp, P(vp)]
@@ -348,6 +349,10 @@ class Context:
                 iface = np.zeros((max(1, nn.value), k.value))
                 L.hv_integral_export_f(h, C.byref(k), _dp(bulk), _dp(iface))
                 extra = {"bulk": bulk, "interface": iface[: nn.value]}
+            work, kms = C.c_int64(0), C.c_double(0.0)
+            L.hv_integral_work(h, C.byref(work), C.byref(kms))
+            if work.value:
+                extra.update(pairs=work.value, ms_kernel=kms.value)
         finally:
             L.hv_integral_free(h)
         return {"volumes": vol, "nbr_ptr": ptr, "neighbors": nbr[: nn.value], "areas": area[: nn.value],

@@ -57,7 +57,11 @@ def normals(seed: int, count: int, start: int = 0) -> np.ndarray:
 
 
 def gaussian_mixture_5d(n: int = 100_000, seed: int = 2, clusters: int = 8, sigma: float = 0.05,
-                        background: float = 0.10, d: int = 5) -> np.ndarray:
+                        background: float = 0.10, d: int = 5, clip: bool = False) -> np.ndarray:
+    """n points in [0, 1]^d: `clusters` Gaussian clusters (centres U[0.2, 0.8]^d, standard
+    deviation sigma) and a uniform background fraction.  Cluster samples outside the open cube
+    are drawn again, or with clip=True clipped onto the cube's faces (generators on a boundary
+    plane)."""
     nb = int(round(n * background))
     nc = n - nb
     per = [nc // clusters + (1 if i < nc % clusters else 0) for i in range(clusters)]
@@ -72,8 +76,11 @@ def gaussian_mixture_5d(n: int = 100_000, seed: int = 2, clusters: int = 8, sigm
             z = normals(seed, batch * d, start=stream).reshape(batch, d)
             stream += batch * d
             cand = centres[ci] + sigma * z
-            ok = np.all((cand > 0.0) & (cand < 1.0), axis=1)
-            cand = cand[ok][:need]
+            if clip:
+                cand = np.clip(cand[:need], 0.0, 1.0)
+            else:
+                ok = np.all((cand > 0.0) & (cand < 1.0), axis=1)
+                cand = cand[ok][:need]
             got.append(cand)
             need -= len(cand)
         pts.append(np.concatenate(got))

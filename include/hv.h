@@ -269,6 +269,10 @@ int hv_integrate_mc_points(hv_ctx* ctx, const hv_mesh* mesh, int64_t nsamples, u
  * its neighbours nbr[nbr_ptr[i] .. nbr_ptr[i+1]) sorted, 1-based (planes n+p), and the
  * interface areas aligned with them.  ms (optional) receives the device time. */
 int hv_integral_counts(const hv_integral* I, int64_t* ncells, int64_t* nnbr);
+/* No counterpart in the reference (a measurement): for hv_integrate_mc_points, the (sample,
+ * generator) distances its sampling kernel evaluated and that kernel's device time in ms
+ * (0 and 0 for the other integrators). */
+int hv_integral_work(const hv_integral* I, int64_t* work, double* ms_kernel);
 int hv_integral_export(const hv_integral* I, double* vol, int64_t* nbr_ptr, int64_t* nbr, double* area, double* ms);
 /* Integrand results of hv_integrate_polytope_f (Voronoi_Integral's bulk_integral and
  * interface_integral, src/integral.jl:11-21): nf, bulk (nf x ncells) and iface (nf x nnbr,

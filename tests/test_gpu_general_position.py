@@ -93,3 +93,32 @@ def test_exact_lattice_is_degenerate(ctx, prec, d, k):
     m.free()
     ok, _, _ = Oracle(ys, b, nrm).voronoi()
     assert np.array_equal(rows_view(keys), rows_view(ok))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16"])
+def test_clipped_mixture_is_degenerate(ctx, prec):
+    # generators on a boundary plane (the clipped 5D N=100k mixture): not in general position
+    xs = synth.gaussian_mixture_5d(clip=True)
+    b, nrm = synth.cube_planes(5)
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    with pytest.raises(HVError) as e:
+        ctx.build_mesh(precision=prec)
+    assert e.value.code == -4
+
+
+def test_batch_mesh_with_a_generator_on_a_face_is_degenerate():
+    from hvgpu._lib import build_mesh_batch
+    pts = [synth.uniform(3, 400, 71), synth.uniform(3, 300, 72), synth.uniform(4, 500, 73)]
+    pts[1][5, 2] = 0.0   # on the face x_3 = 0
+    planes = [synth.cube_planes(x.shape[1]) for x in pts]
+    out, status = build_mesh_batch(pts, planes, precision="bf16")
+    assert list(status) == [0, -4, 0]
+    for k in (0, 2):
+        b, nrm = planes[k]
+        ok, _, _ = Oracle(pts[k], b, nrm).voronoi()
+        keys, _ = out[k].export()
+        assert np.array_equal(rows_view(keys), rows_view(ok))
+    for m in out:
+        if m is not None:
+            m.free()

@@ -77,6 +77,56 @@ def test_mixed_identical_vertex_set_at_5d_100k_mixture(ctx, prec):
     assert np.array_equal(c, c64)
 
 
+def edge_closure_defects_hashed(keys, n):
+    """Walkable edges (d-subsets of a key whose smallest id is a generator) not in exactly two
+    keys, over a large table: every edge packed into 17-bit ids (ids < 2^17 here) and hashed to
+    64 bits (a collision among the ~5e7 edges has probability ~1e-4 and can only merge two
+    counts)."""
+    keys = np.asarray(keys, dtype=np.uint64)
+    V, m = keys.shape
+    assert int(keys.max()) < (1 << 17)
+    hs = []
+    for p in range(m):
+        e = np.delete(keys, p, axis=1)
+        e = e[e[:, 0] <= n]
+        hi = (e[:, 0] << np.uint64(34)) | (e[:, 1] << np.uint64(17)) | e[:, 2]
+        lo = np.zeros(len(e), dtype=np.uint64)
+        for j in range(3, m - 1):
+            lo = (lo << np.uint64(17)) | e[:, j]
+        h = hi * np.uint64(0x9E3779B97F4A7C15) ^ (lo + np.uint64(0x632BE59BD9B4E019))
+        h ^= h >> np.uint64(29)
+        h *= np.uint64(0xBF58476D1CE4E5B9)
+        hs.append(h ^ (h >> np.uint64(32)))
+    h = np.sort(np.concatenate(hs))
+    starts = np.flatnonzero(np.r_[True, h[1:] != h[:-1]])
+    counts = np.diff(np.r_[starts, len(h)])
+    return int(np.sum(counts != 2))
+
+
+def test_5d_100k_mixture_cells_equal_oracle(ctx):
+    """The benchmark workload against the oracle: the vertices of cells 1..2000 (the oracle's
+    walk over Iter = 1:2000) equal the FP64 build's vertices that hold one of them and the
+    BF16 walk of those cells alone (hv_build_mesh_cells); the whole FP64 table closes every
+    edge (each walkable edge in exactly two keys)."""
+    xs = synth.gaussian_mixture_5d()
+    n, b = len(xs), 2000
+    lo, nrm = synth.cube_planes(5)
+    ok, oc, _ = Oracle(xs, lo, nrm).voronoi(cells=(0, b))
+    k64, c64, st = build(ctx, xs, "fp64")
+    assert st["vertices"] > 10_000_000
+    held = (k64 <= b).any(axis=1)
+    assert np.array_equal(rows_view(k64[held]), rows_view(ok))
+    order = np.lexsort(ok.T[::-1])
+    assert float(np.max(np.abs(c64[held] - oc[order]))) < COORD_TOL
+    assert edge_closure_defects_hashed(k64, n) == 0
+    del k64, c64
+    m = ctx.build_mesh_cells(np.arange(1, b + 1), precision="bf16")
+    kc, cc = m.export()
+    m.free()
+    assert np.array_equal(rows_view(kc), rows_view(ok))
+    assert float(np.max(np.abs(cc - oc[order]))) < COORD_TOL
+
+
 @pytest.mark.parametrize("prec", MIXED)
 def test_mixed_rejects_other_leaf_sizes(ctx, prec):
     xs = synth.uniform(3, 200, 1)

@@ -67,3 +67,15 @@ def test_rays_along_edges_shorter_than_plane_tolerance_end_without_a_vertex(eps)
     keys, _, st = Oracle(xs, b, nrm).voronoi()
     assert st["unbounded"] > 0
     assert len(keys) == 2 * 29 ** 2 + 4 * 30
+
+
+def test_clipped_mixture_is_not_general():
+    """The 5D N=100k mixture with out-of-cube samples clipped onto the faces (instead of drawn
+    again) holds generators on a boundary plane; such a generator coincides with its own
+    mirror image and the walk of its cell reaches a vertex with more than d + 1 generators."""
+    xs = synth.gaussian_mixture_5d(clip=True)
+    on = np.nonzero(((xs == 0.0) | (xs == 1.0)).any(axis=1))[0]
+    assert len(on) > 0
+    b, nrm = synth.cube_planes(5)
+    with pytest.raises(RuntimeError, match="non-general"):
+        Oracle(xs, b, nrm).voronoi(cells=(int(on[0]), int(on[0]) + 1))
