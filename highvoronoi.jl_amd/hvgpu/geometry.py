@@ -29,6 +29,23 @@ VI_FAST_POLYGON = "fast_polygon"
 DIRICHLET = 0
 NEUMANN = -1
 
+# raycast methods (src/raycast-types.jl:245-310).  Every method computes the same tessellation
+# (test/rcmethods.jl checks exactly that through the volume integral); on the device they are
+# one walk, the FP64 decision rule of Raycast_Non_General_HP behind the filter the precision
+# names, so the method is accepted and recorded, and changes nothing.
+RCOriginal = "RCOriginal"
+RCOriginalSafety = "RCOriginalSafety"
+RCOriginalHP = "RCOriginalHP"
+RCNonGeneralFast = "RCNonGeneralFast"
+RCNonGeneralSkip = "RCNonGeneralSkip"
+RCCombined = "RCCombined"
+RCNonGeneralHP = "RCNonGeneralHP"
+RCNonGeneralCutoff = "RCNonGeneralCutoff"
+RCNonGeneral = RCNonGeneralHP
+RCStandard = RCNonGeneral
+METHODS = (RCOriginal, RCOriginalSafety, RCOriginalHP, RCNonGeneralFast, RCNonGeneralSkip, RCCombined,
+           RCNonGeneralHP, RCNonGeneralCutoff)
+
 
 @dataclass
 class Plane:
@@ -90,7 +107,8 @@ class VoronoiGeometry:
     """Mesh (and, with an integrator, volumes and interface areas) of the generators xs
     in `boundary`, computed on one MI355X.
 
-    search_settings: {"precision": "bf16" | "fp64", "leaf_size": int, "device": int}.
+    search_settings: {"precision": "bf16" | "fp32" | "fp64", "leaf_size": int, "device": int,
+    "method": one of METHODS (RaycastParameter.method; all give the same mesh)}.
     mc_accurate: (directions per cell, unused, unused) as the reference's tuple
     (src/geometry.jl:139); seed drives the Monte-Carlo directions.
     integrand (VI_POLYGON / VI_FAST_POLYGON): f(points[npts, d]) -> values[npts, k], the
@@ -107,6 +125,9 @@ class VoronoiGeometry:
         self.boundary = boundary if boundary is not None else Boundary([])
         s = dict(search_settings or {})
         self.precision = s.get("precision", "bf16")
+        self.method = s.get("method", RCStandard)
+        if self.method not in METHODS:
+            raise ValueError(f"unknown raycast method {self.method!r} (one of {', '.join(METHODS)})")
         self.integrator = integrator
         if integrand is not None and integrator not in (VI_POLYGON, VI_FAST_POLYGON):
             raise NotImplementedError("integrands are integrated by VI_POLYGON / VI_FAST_POLYGON on the device path")

@@ -187,11 +187,18 @@ int hv_mesh_images(const hv_mesh* mesh, int64_t* n_ext, int64_t* orig, int8_t* s
  * host memory) and the nplanes[k] boundary planes base[k], normal[k] (d[k] x nplanes[k] each)
  * with the conditions bc[k] (nplanes[k] values as in hv_set_boundary; bc == NULL or
  * bc[k] == NULL: Dirichlet), all built with *opts.  Replaces a host loop of VoronoiGeometry(xs_k, boundary_k)
- * calls (src/geometry.jl:156 -> voronoi, src/sysvoronoi.jl:21-82), one per mesh.  nworkers host
- * threads (0 = 8), each with its own context and HIP stream on device_ordinal, take meshes in
- * index order, so the level loops of different meshes overlap on the device.  out[k] receives
- * the mesh (free each with hv_mesh_free) or NULL, status[k] its HV_* code; the return value is
- * HV_OK or the first failing mesh's code. */
+ * calls (src/geometry.jl:156 -> voronoi, src/sysvoronoi.jl:21-82), one per mesh.
+ * Fused walk (nworkers >= 0): the meshes in a cuboid of the reference's plane order without
+ * periodic sides are walked together, one walk per dimension: a forest of per-mesh trees in
+ * shared tables, global generator ids, every level one launch per kernel over the rays of all
+ * meshes (packets of one mesh each), taken in runs of at most 16M expected vertices.  A mesh
+ * with a generator outside its box or on a face gets HV_ERR_DOMAIN / HV_ERR_DEGENERATE alone.
+ * Every other mesh (periodic sides, other planes; every mesh when nworkers < 0) is built on
+ * |nworkers| host threads (0 = 8), each with its own context and HIP stream, taking meshes in
+ * index order.  Each mesh equals its own hv_build_mesh bit for bit (keys and coordinates); its
+ * hv_mesh_stats describe the walk it was part of.  out[k] receives the mesh (free each with
+ * hv_mesh_free; fused meshes share one allocation, released with the last of them) or NULL,
+ * status[k] its HV_* code; the return value is HV_OK or the first failing mesh's code. */
 int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, const int64_t* n,
                         const double* const* xs, const int32_t* nplanes, const double* const* base,
                         const double* const* normal, const int16_t* const* bc, const hv_opts* opts,

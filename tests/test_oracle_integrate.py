@@ -2,9 +2,10 @@
 
 Exact polytope integrator against: the Qhull cell volumes stored with the golden
 fixtures (exact vertex sets); exact rational polygon areas (shoelace) in 2D; the unit-box
-tiling (sum of volumes = 1); per-cell Qhull volumes on random meshes (d <= 4); the two
-measurements of each interface (one from each cell) agreeing; and a planted mistake (a
-wrong cone height) that these checks catch.  Monte-Carlo integrator: statistical agreement
+tiling (sum of volumes = 1); per-cell Qhull volumes on random meshes (d <= 4); per-facet
+Qhull (d-1)-volumes of every interface (the facet's vertices projected into its hyperplane,
+d = 3 and 4); the two measurements of each interface (one from each cell) agreeing; and a
+planted mistake (a wrong cone height) that these checks catch.  Monte-Carlo integrator: statistical agreement
 with the exact volumes and areas, and the fixed direction stream (directions differ).
 """
 from fractions import Fraction
@@ -72,6 +73,43 @@ def test_random_meshes_match_qhull_and_tile_the_box(d, n, seed):
             if k <= n:
                 tk = ptr[k - 1] + np.searchsorted(nbr[ptr[k - 1]:ptr[k]], i + 1)
                 assert abs(area[t] - area[tk]) <= 1e-10 * max(area[t], 1e-6 * area.max())
+
+
+@pytest.mark.parametrize("d,n,seed", [(3, 250, 4), (4, 120, 5)])
+def test_facet_areas_match_qhull(d, n, seed):
+    """Every interface area of the oracle's integrator equals the (d-1)-volume of the convex
+    hull of the facet's vertices (those whose keys hold both ids) in an orthonormal basis of
+    the facet's hyperplane: the bisector of two generators, or the boundary plane."""
+    from scipy.spatial import ConvexHull
+    xs = synth.uniform(d, n, seed)
+    b, nrm = synth.cube_planes(d)
+    keys, coords, _ = Oracle(xs, b, nrm).voronoi()
+    vol, ptr, nbr, area = polytope(xs, b, nrm, keys, coords)
+    rows_of = {}
+    for r, key in enumerate(keys.tolist()):
+        for g in key:
+            rows_of.setdefault(g, set()).add(r)
+    checked = 0
+    for i in range(n):
+        for t in range(ptr[i], ptr[i + 1]):
+            k = int(nbr[t])
+            rows = sorted(rows_of[i + 1] & rows_of[k])
+            normal = xs[k - 1] - xs[i] if k <= n else nrm[k - n - 1]
+            normal = normal / np.linalg.norm(normal)
+            # orthonormal basis of the hyperplane: the last d - 1 columns of a QR of [normal | I]
+            q, _ = np.linalg.qr(np.column_stack([normal, np.eye(d)]))
+            basis = q[:, 1:d]
+            pts = (coords[rows] - coords[rows[0]]) @ basis
+            if len(rows) < d:
+                assert area[t] < 1e-12
+                continue
+            try:
+                a = ConvexHull(pts).volume
+            except Exception:   # flat facet (all vertices in a (d-2)-plane): no (d-1)-volume
+                a = 0.0
+            assert abs(area[t] - a) <= 1e-9 * max(a, 1e-6 * area.max())
+            checked += 1
+    assert checked > n * d
 
 
 def test_planted_height_mistake_is_caught():
