@@ -153,8 +153,12 @@ hv_bind_mesh::~hv_bind_mesh() {
     c->img_shift = std::move(img_shift);
 }
 
+// HV_LOG_ERRORS set: every failure is also written to stderr (the contexts of a batch call are
+// internal, so their messages are not otherwise visible)
 int hv_fail(hv_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
+    static const bool log = std::getenv("HV_LOG_ERRORS") != nullptr;
+    if (log) std::fprintf(stderr, "libhv: error %d: %s\n", code, msg.c_str());
     return code;
 }
 

@@ -2495,8 +2495,11 @@ __global__ __launch_bounds__(256) void k_tent_group(DevGeo g0, GeoSet F, DevRays
             if (old == -1) { prim_of[t] = t; primary[t] = 1; return; }
             cur = old;
         }
+        // a claimer is a fresh ray with a key; over a forest it may belong to another mesh,
+        // whose tree maps its start candidate (a sorted position or a plane id of that mesh)
         int ok[D + 1];
-        hv_tent_key<D>(g, R, vkey, cur, ok);   // a claimer is a fresh ray with a key
+        const DevGeo& gc = hv_geo<B>(g0, F, vkey[(int64_t)R.v[cur] * (D + 1)]);
+        hv_tent_key<D>(gc, R, vkey, cur, ok);
         if (keys_equal<D>(tk, ok)) { prim_of[t] = cur; primary[t] = 0; return; }
         h = (h + 1) & tmask;
     }
@@ -2566,15 +2569,13 @@ __global__ void k_chunk(const int* __restrict__ ord, int nr, int c, int K, const
 }
 
 // ---- a forest of meshes walked together ----------------------------------------------------
-// Rays of the (mesh-ordered) list per mesh.
-template <int D>
+// Rays of the (mesh-ordered) list per mesh: a ray's sort code holds its mesh above 32 bits.
 __global__ __launch_bounds__(256) void k_pkt_hist(const int* __restrict__ list, int n, const int* __restrict__ n_dev,
-                                                  DevRays R, const int* __restrict__ vkey, const int* __restrict__ mof,
-                                                  int* __restrict__ cnt) {
+                                                  const uint64_t* __restrict__ code, int* __restrict__ cnt) {
     if (n_dev) n = min(n, *n_dev);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = t < n;
-    const int m = act ? mof[vkey[(int64_t)R.v[list[t]] * (D + 1)]] : 0;
+    const int m = act ? (int)(code[list[t]] >> 32) : 0;
     hv_wave_add_by_key(cnt, m, act);
 }
 
@@ -2790,40 +2791,47 @@ static int build_forest(hv_ctx* c, int L, int tables, Forest& fo) {
     return HV_OK;
 }
 
-// Vertices of a forest walk -> the meshes: per mesh, keys in the mesh's own ids (generator
-// goff + i -> i, plane ntot + 2 D k + p -> n_k + p).
+// Vertices of a forest walk -> the meshes: the live vertices sorted by mesh (stable: each
+// mesh keeps the walk's order), then copied with keys in the mesh's own ids (generator
+// goff + i -> i, plane ntot + 2 D k + p -> n_k + p).  A key's first entry is a generator of
+// its mesh (planes sort last).
 template <int D>
-__global__ __launch_bounds__(256) void k_split_count(const int* __restrict__ ids, int nv, const int* __restrict__ vkey,
-                                                     const int* __restrict__ mof, int* __restrict__ cnt) {
+__global__ __launch_bounds__(256) void k_split_key(const int* __restrict__ ids, int nv, const int* __restrict__ vkey,
+                                                   const int* __restrict__ mof, int* __restrict__ mkey) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = t < nv;
-    const int m = act ? mof[vkey[(int64_t)ids[t] * (D + 1)]] : 0;
-    hv_wave_add_by_key(cnt, m, act);
+    if (t < nv) mkey[t] = mof[vkey[(int64_t)ids[t] * (D + 1)]];
+}
+
+// off[m] = first sorted position of mesh m (binary search in the sorted mesh keys), off[nm] = nv
+__global__ void k_split_offsets(const int* __restrict__ mkey, int nv, int nm, int64_t* __restrict__ off) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > nm) return;
+    int lo = 0, hi = nv;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (mkey[mid] < m) lo = mid + 1;
+        else hi = mid;
+    }
+    off[m] = lo;
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_split_scatter(const int* __restrict__ ids, int nv, const int* __restrict__ vkey,
-                                                       const double* __restrict__ vx, const int* __restrict__ mof,
-                                                       const int* __restrict__ goff, const int* __restrict__ nper,
-                                                       int64_t ntot, const int64_t* __restrict__ off,
-                                                       int* __restrict__ fill, int* __restrict__ okey,
-                                                       double* __restrict__ ox) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = t < nv;
-    const int v = act ? ids[t] : 0;
+__global__ __launch_bounds__(256) void k_split_copy(const int* __restrict__ ids, const int* __restrict__ mkey, int nv,
+                                                    const int* __restrict__ vkey, const double* __restrict__ vx,
+                                                    const int* __restrict__ goff, const int* __restrict__ nper,
+                                                    int64_t ntot, int* __restrict__ okey, double* __restrict__ ox) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;
+    const int v = ids[t], m = mkey[t];
     const int* key = vkey + (int64_t)v * (D + 1);
-    const int m = act ? mof[key[0]] : 0;
-    const int slot = hv_wave_add_by_key(fill, m, act);
-    if (!act) return;
-    const int64_t pos = off[m] + slot;
     const int64_t pbase = ntot + (int64_t)2 * D * m;
 #pragma unroll
     for (int j = 0; j <= D; ++j) {
         const int e = key[j];
-        okey[pos * (D + 1) + j] = e < ntot ? e - goff[m] : (int)(e - pbase) + nper[m];
+        okey[t * (D + 1) + j] = e < ntot ? e - goff[m] : (int)(e - pbase) + nper[m];
     }
 #pragma unroll
-    for (int k = 0; k < D; ++k) ox[pos * D + k] = vx[(int64_t)v * D + k];
+    for (int k = 0; k < D; ++k) ox[t * D + k] = vx[(int64_t)v * D + k];
 }
 
 // The walk over the context's working set.  Only edges with a generator below nwalk are
@@ -3140,7 +3148,7 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
                 int* pko = pget<int>(c, "pk_off", (size_t)nmesh + 1);
                 if (!pcnt || !seg || !pko) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (packets)");
                 HVCK(hipMemsetAsync(pcnt, 0, sizeof(int) * nmesh, c->stream));
-                k_pkt_hist<D><<<nblk(n, 256), 256, 0, c->stream>>>(list, n, n_dev, R, vkey, F.mof, pcnt);
+                k_pkt_hist<<<nblk(n, 256), 256, 0, c->stream>>>(list, n, n_dev, R.code, pcnt);
                 HVCK(hipGetLastError());
                 k_pkt_plan<<<1, 64, 0, c->stream>>>(pcnt, nmesh, ppw, seg, pko);
                 HVCK(hipGetLastError());
@@ -3340,25 +3348,32 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     if constexpr (B) {
         // the forest's meshes: vertices by mesh, keys in each mesh's own ids, one allocation
         const int nm = fo->nmesh;
-        int* cnt = pget<int>(c, "s_cnt", (size_t)nm);
-        int* fill = pget<int>(c, "s_fill", (size_t)nm);
+        const int nl = (int)std::max<int64_t>(nlive, 1);
+        int* mk0 = pget<int>(c, "s_mkey0", (size_t)nl);
+        int* mk1 = pget<int>(c, "s_mkey1", (size_t)nl);
+        int* sid = pget<int>(c, "s_ids", (size_t)nl);
         int64_t* doff = pget<int64_t>(c, "s_off", (size_t)nm + 1);
         int* dgoff = pget<int>(c, "s_goff", (size_t)nm);
         int* dn = pget<int>(c, "s_n", (size_t)nm);
-        if (!cnt || !fill || !doff || !dgoff || !dn) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (split)");
-        HVCK(hipMemsetAsync(cnt, 0, sizeof(int) * nm, c->stream));
-        HVCK(hipMemsetAsync(fill, 0, sizeof(int) * nm, c->stream));
+        if (!mk0 || !mk1 || !sid || !doff || !dgoff || !dn) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (split)");
         if (nlive) {
-            k_split_count<D><<<nblk(nlive, 256), 256, 0, c->stream>>>(ids, (int)nlive, vkey, F.mof, cnt);
+            k_split_key<D><<<nblk(nlive, 256), 256, 0, c->stream>>>(ids, (int)nlive, vkey, F.mof, mk0);
             HVCK(hipGetLastError());
+            size_t tb = 0;
+            HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, mk0, mk1, ids, sid, (int)nlive, 0, fo->mbits, c->stream));
+            void* tmp = c->pool->get("cubtmp", tb, false, c->stream);
+            if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (split sort)");
+            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, mk0, mk1, ids, sid, (int)nlive, 0, fo->mbits, c->stream));
         }
-        std::vector<int> hc(nm);
-        HVCK(hipMemcpyAsync(hc.data(), cnt, sizeof(int) * nm, hipMemcpyDeviceToHost, c->stream));
-        HVCK(hipStreamSynchronize(c->stream));
+        k_split_offsets<<<nblk(nm + 1, 256), 256, 0, c->stream>>>(mk1, (int)nlive, nm, doff);
+        HVCK(hipGetLastError());
         std::vector<int64_t> hoff(nm + 1, 0);
+        HVCK(hipMemcpyAsync(hoff.data(), doff, sizeof(int64_t) * (nm + 1), hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        std::vector<int> hc(nm);
         int maxc = 1;
         for (int k = 0; k < nm; ++k) {
-            hoff[k + 1] = hoff[k] + hc[k];
+            hc[k] = (int)(hoff[k + 1] - hoff[k]);
             maxc = std::max(maxc, hc[k]);
         }
         const size_t kb = ((size_t)nlive * (D + 1) * sizeof(int) + 255) / 256 * 256;
@@ -3374,12 +3389,11 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
         int* okey = (int*)store->p;
         double* ox = (double*)((char*)store->p + kb);
         int* oid = (int*)((char*)store->p + kb + xb);
-        HVCK(hipMemcpyAsync(doff, hoff.data(), sizeof(int64_t) * (nm + 1), hipMemcpyHostToDevice, c->stream));
         HVCK(hipMemcpyAsync(dgoff, fo->goff.data(), sizeof(int) * nm, hipMemcpyHostToDevice, c->stream));
         HVCK(hipMemcpyAsync(dn, fo->n.data(), sizeof(int) * nm, hipMemcpyHostToDevice, c->stream));
         if (nlive) {
-            k_split_scatter<D><<<nblk(nlive, 256), 256, 0, c->stream>>>(ids, (int)nlive, vkey, vx, F.mof, dgoff, dn,
-                                                                        fo->ntot, doff, fill, okey, ox);
+            k_split_copy<D><<<nblk(nlive, 256), 256, 0, c->stream>>>(sid, mk1, (int)nlive, vkey, vx, dgoff, dn, fo->ntot,
+                                                                     okey, ox);
             HVCK(hipGetLastError());
         }
         k_iota<<<nblk(maxc, 256), 256, 0, c->stream>>>(oid, maxc);
@@ -4056,7 +4070,7 @@ __global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int ns
             if (hv_box_d2<D>(g, node, xn) <= rr) {
                 if (node < g.P2) { node = 2 * node; continue; }
                 const int leaf = node - g.P2;
-                const int j0 = leaf * g.L, j1 = min(j0 + g.L, N);
+                const int j0 = leaf * g.L, j1 = min(j0 + g.L, g.n);   // sorted positions of this tree
                 for (int j = j0; j < j1; ++j) {
                     double d2 = 0.0;
 #pragma unroll
