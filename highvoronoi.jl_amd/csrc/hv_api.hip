@@ -212,6 +212,8 @@ HvTiles hv_tiles_for(const hv_ctx* c, const hv_opts& o, int D, int precision, in
     if (o.rays_per_block == 8 || o.rays_per_block == 16 || o.rays_per_block == 32 || o.rays_per_block == 64)
         t.ppw_max = o.rays_per_block;
     if (o.packet_floor > 0) t.min_packets = std::min(o.packet_floor, 1 << 20);
+    if (o.packet_min == 1 || o.packet_min == 2 || o.packet_min == 4 || o.packet_min == 8) t.ppw_min = o.packet_min;
+    t.ppw_min = std::min(t.ppw_min, t.ppw_max);
     return t;
 }
 

@@ -163,6 +163,7 @@ int hv_fail(hv_ctx* c, int code, const std::string& msg);
 struct HvTiles {
     int ppw_max = 64;
     int min_packets = 2048;
+    int ppw_min = 8;   // packets are not halved below this many rays
 };
 // The tiles a build uses: the caller's (hv_opts.rays_per_block / packet_floor when set), else
 // the autotuner's for (dimension, precision, size class of n), else the defaults.

@@ -2100,7 +2100,7 @@ constexpr int HV_SEED_STRIDE = 8;          // large point sets: one descent per 
 constexpr int HV_SEED_ALL_MAX = 1 << 17;   // up to this many generators, and in 2D, every one
 static inline int hv_packet_rays(int64_t n, const HvTiles& t = HvTiles{}) {
     int p = t.ppw_max;
-    while (p > 8 && n < (int64_t)p * t.min_packets) p >>= 1;
+    while (p > t.ppw_min && n < (int64_t)p * t.min_packets) p >>= 1;
     return p;
 }
 

@@ -31,7 +31,7 @@ class hv_opts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("leaf_size", C.c_int32), ("rays_per_block", C.c_int32),
                 ("reserved0", C.c_int32), ("plane_tolerance", C.c_double), ("ray_tol", C.c_double),
                 ("vertex_capacity", C.c_int64), ("seed", C.c_uint64), ("packet_floor", C.c_int32),
-                ("reserved1", C.c_int32)]
+                ("packet_min", C.c_int32)]
 
 
 class hv_stats_t(C.Structure):
@@ -218,7 +218,8 @@ class Context:
         self._check(lib().hv_set_boundary(self._h, npl, _dp(base), _dp(normal), bcp))
 
     def build_mesh(self, precision: str = "fp64", leaf_size: int = 0, vertex_capacity: int = 0,
-                   chunks: int = 0, rays_per_block: int = 0, packet_floor: int = 0, seed: int = 0) -> Mesh:
+                   chunks: int = 0, rays_per_block: int = 0, packet_floor: int = 0, seed: int = 0,
+                   packet_min: int = 0) -> Mesh:
         o = hv_opts()
         o.precision = PRECISIONS[precision]
         o.leaf_size = leaf_size
@@ -226,6 +227,7 @@ class Context:
         o.vertex_capacity = vertex_capacity
         o.rays_per_block = rays_per_block
         o.packet_floor = packet_floor
+        o.packet_min = packet_min
         o.seed = seed
         m = C.c_void_p()
         self._check(lib().hv_build_mesh(self._h, C.byref(o), C.byref(m)))

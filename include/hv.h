@@ -72,7 +72,8 @@ typedef struct hv_opts {
                                  vertex set does not depend on it, the traversal does */
     int32_t packet_floor;     /* scan tile: packets are halved (down to 8 rays) while a level would
                                  give fewer packets than this; 0 = the autotuner's choice or 2048 */
-    int32_t reserved1;        /* 0 */
+    int32_t packet_min;       /* scan tile: packets are not halved below this many rays (1, 2, 4 or
+                                 8); 0 = the autotuner's choice or 8 */
 } hv_opts;
 
 typedef struct hv_stats_t {

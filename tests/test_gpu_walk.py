@@ -199,8 +199,9 @@ def test_interior_seeds_keep_the_walk_shallow(ctx):
     assert st["unbounded"] == 0 and st["degenerate"] == 0
 
 
-@pytest.mark.parametrize("ppw,floor", [(8, 64), (16, 1024), (32, 4096), (64, 2048), (64, 100000)])
-def test_scan_tiles_do_not_change_the_mesh(ctx, ppw, floor):
+@pytest.mark.parametrize("ppw,floor,pmin", [(8, 64, 0), (16, 1024, 0), (32, 4096, 0), (64, 2048, 0), (64, 100000, 0),
+                                            (64, 100000, 1), (64, 4096, 2)])
+def test_scan_tiles_do_not_change_the_mesh(ctx, ppw, floor, pmin):
     xs = synth.uniform(5, 2500, 93)
     b, nrm = synth.cube_planes(5)
     ctx.set_points(xs)
@@ -208,7 +209,7 @@ def test_scan_tiles_do_not_change_the_mesh(ctx, ppw, floor):
     ref = ctx.build_mesh(precision="bf16")
     k0, c0 = ref.export()
     ref.free()
-    m = ctx.build_mesh(precision="bf16", rays_per_block=ppw, packet_floor=floor)
+    m = ctx.build_mesh(precision="bf16", rays_per_block=ppw, packet_floor=floor, packet_min=pmin)
     k1, c1 = m.export()
     m.free()
     assert np.array_equal(k0, k1) and np.array_equal(c0, c1)
