@@ -710,6 +710,21 @@ int hv_raycast_batch(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig
     HV_DISPATCH(c->D, hv_raycast_batch_impl, c, precision, nrays, sig, drop, r, out_sig, out_r, out_t, out_status)
 }
 
+int hv_raycast_batch_keys(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
+                          const double* r, int32_t max_sig, int64_t* out_sig, int32_t* out_len, double* out_r,
+                          double* out_t, int32_t* out_status) {
+    if (!c) return HV_ERR_ARG;
+    if (c->D == 0) return hv_fail(c, HV_ERR_STATE, "no points set");
+    if (max_sig < c->D + 1 || max_sig > 1024) return hv_fail(c, HV_ERR_ARG, "max_sig must be in d+1..1024");
+    if (nrays < 0 ||
+        (nrays > 0 && (!sig || !drop || !r || !out_sig || !out_len || !out_r || !out_t || !out_status)))
+        return hv_fail(c, HV_ERR_ARG, "null argument");
+    (void)hipSetDevice(c->device);
+    std::vector<int64_t> key((size_t)std::max<int64_t>(nrays, 1) * (c->D + 1));
+    HV_DISPATCH(c->D, hv_raycast_batch_impl, c, precision, nrays, sig, drop, r, key.data(), out_r, out_t, out_status,
+                max_sig, out_sig, out_len)
+}
+
 int hv_raycast_batch_noalloc(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
                              const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status) {
     if (!c) return HV_ERR_ARG;

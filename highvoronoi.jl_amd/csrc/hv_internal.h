@@ -181,8 +181,11 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
                       double* eps_out, double* vbox_out, int64_t* pairs_out,
                       double* kernel_ms);
 template <int D>
+// full_sig / full_len (hv_raycast_batch_keys; NULL otherwise): max_sig ids per ray, every generator
+// and plane on the new vertex's sphere for a ray out of general position
 int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
-                          const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status);
+                          const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status,
+                          int32_t max_sig = 0, int64_t* full_sig = nullptr, int32_t* full_len = nullptr);
 template <int D>
 int hv_raycast_noalloc_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
                             const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status);

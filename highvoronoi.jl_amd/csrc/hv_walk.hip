@@ -4292,9 +4292,81 @@ __global__ __launch_bounds__(256) void k_seed_exact(DevGeo g, const int* __restr
     }
 }
 
+// Degenerate inclusion (src/raycast.jl:926-950): for a ray out of general position (status
+// HV_ST_SINGULAR), every generator and boundary plane on the sphere of its new vertex
+// r2 = r + t u, the reference's key of more than d + 1 entries.  A generator belongs when its
+// distance from r2 is within 1e-10 (relative) of the edge's, a plane when r2 lies on it within
+// 1e-10 of that radius.  Per ray, up to `maxk` sorted ids (key, 0-based) and their count
+// (-1: more than maxk).
+template <int D>
+__global__ __launch_bounds__(128) void k_full_key(DevGeo g, DevRays R, const int* __restrict__ order, int nrays,
+                                                  const int* __restrict__ vkey, const double* __restrict__ vx,
+                                                  const int* __restrict__ status, const double* __restrict__ tout,
+                                                  int maxk, int* __restrict__ okey, int* __restrict__ olen,
+                                                  double* __restrict__ ox) {
+    const int rid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rid >= nrays) return;
+    if (status[rid] != HV_ST_SINGULAR) { olen[rid] = 0; return; }
+    const int ray = order[rid];
+    const int v = R.v[ray];
+    const int e0 = vkey[(int64_t)v * (D + 1) + (R.drop[ray] == 0 ? 1 : 0)];   // the edge's first generator
+    const double t = tout[rid];
+    double r2[D], R2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        r2[k] = fma(t, R.u[(int64_t)ray * D + k], vx[(int64_t)v * D + k]);
+        const double a = g.xo[(int64_t)e0 * D + k] - r2[k];
+        R2 = fma(a, a, R2);
+        ox[(int64_t)rid * D + k] = r2[k];
+    }
+    const double lim = R2 * (1.0 + 2e-10);
+    int* key = okey + (int64_t)rid * maxk;
+    int n = 0;
+    bool over = false;
+    int node = 1;
+    while (node) {
+        if (hv_box_d2<D>(g, node, r2) <= lim) {
+            if (node < g.P2) { node = 2 * node; continue; }
+            const int leaf = node - g.P2;
+            const int j0 = leaf * g.L, j1 = min(j0 + g.L, g.n);
+            for (int j = j0; j < j1; ++j) {
+                double d2 = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const double a = g.xs[(int64_t)j * (D + 1) + k] - r2[k];
+                    d2 = fma(a, a, d2);
+                }
+                if (d2 <= lim) {
+                    if (n < maxk) key[n] = g.sidx[j];
+                    else over = true;
+                    ++n;
+                }
+            }
+        }
+        node = hv_next_node(node);
+    }
+    const double rad = sqrt(R2);
+    for (int p = 0; p < g.P; ++p) {
+        double dist = g.ph[p];
+#pragma unroll
+        for (int k = 0; k < D; ++k) dist = fma(-g.pn[p * D + k], r2[k], dist);
+        if (fabs(dist) <= 1e-10 * rad) {
+            if (n < maxk) key[n] = g.N + p;
+            else over = true;
+            ++n;
+        }
+    }
+    // insertion sort (a key holds a few tens of ids at most)
+    const int m = min(n, maxk);
+    for (int a = 1; a < m; ++a)
+        for (int b = a; b > 0 && key[b - 1] > key[b]; --b) { const int x = key[b]; key[b] = key[b - 1]; key[b - 1] = x; }
+    olen[rid] = over ? -1 : n;
+}
+
 template <int D>
 int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
-                          const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status) {
+                          const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status,
+                          int32_t max_sig, int64_t* full_sig, int32_t* full_len) {
     if (precision != HV_PREC_FP64 && precision != HV_PREC_BF16 && precision != HV_PREC_FP32)
         return hv_fail(c, HV_ERR_UNSUPPORTED, "precision: HV_PREC_FP64, HV_PREC_FP32 or HV_PREC_BF16");
     if (nrays == 0) return HV_OK;
@@ -4370,6 +4442,24 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
     k_finalize<D, false><<<nblk(nrays, 256), 256, 0, c->stream>>>(g, GeoSet{}, R, ord, (int)nrays, nullptr, vkey, vx, nrays,
                                                                   status, rwin);
     HVCK(hipGetLastError());
+    // full keys of the rays out of general position (hv_raycast_batch_keys)
+    std::vector<int> hfk, hfl;
+    std::vector<double> hfx;
+    if (full_sig) {
+        int* fk = pget<int>(c, "b_fkey", (size_t)nrays * max_sig);
+        int* fl = pget<int>(c, "b_flen", nrays);
+        double* fx = pget<double>(c, "b_fx", (size_t)nrays * D);
+        if (!fk || !fl || !fx) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (full keys)");
+        k_full_key<D><<<nblk(nrays, 128), 128, 0, c->stream>>>(g, R, ord, (int)nrays, vkey, vx, status, tout, max_sig,
+                                                               fk, fl, fx);
+        HVCK(hipGetLastError());
+        hfk.resize((size_t)nrays * max_sig);
+        hfl.resize(nrays);
+        hfx.resize((size_t)nrays * D);
+        HVCK(hipMemcpyAsync(hfk.data(), fk, sizeof(int) * hfk.size(), hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipMemcpyAsync(hfl.data(), fl, sizeof(int) * nrays, hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipMemcpyAsync(hfx.data(), fx, sizeof(double) * hfx.size(), hipMemcpyDeviceToHost, c->stream));
+    }
     std::vector<int> hord(nrays), hst(nrays), hk((size_t)nrays * (D + 1));
     std::vector<double> hx((size_t)nrays * D), ht(nrays);
     HVCK(hipMemcpyAsync(hord.data(), ord, sizeof(int) * nrays, hipMemcpyDeviceToHost, c->stream));
@@ -4385,6 +4475,20 @@ int hv_raycast_batch_impl(hv_ctx* c, int precision, int64_t nrays, const int64_t
         for (int j = 0; j <= D; ++j) out_sig[(int64_t)k * (D + 1) + j] = s > 0 ? (int64_t)hk[rid * (D + 1) + j] + 1 : 0;
         for (int q = 0; q < D; ++q) out_r[(int64_t)k * D + q] = s > 0 ? hx[rid * D + q] : NAN;
         out_t[k] = s > 0 ? ht[rid] : INFINITY;
+        if (full_sig) {
+            int64_t* fs = full_sig + (int64_t)k * max_sig;
+            const int fl = hfl[rid];
+            full_len[k] = s > 0 ? D + 1 : (s == HV_ST_SINGULAR ? fl : 0);
+            for (int j = 0; j < max_sig; ++j) fs[j] = 0;
+            if (s > 0) {
+                for (int j = 0; j <= D && j < max_sig; ++j) fs[j] = out_sig[(int64_t)k * (D + 1) + j];
+            } else if (s == HV_ST_SINGULAR && fl > 0) {
+                for (int j = 0; j < fl; ++j) fs[j] = (int64_t)hfk[(size_t)rid * max_sig + j] + 1;
+                for (int q = 0; q < D; ++q) out_r[(int64_t)k * D + q] = hfx[(size_t)rid * D + q];
+                out_t[k] = ht[rid];
+                out_status[k] = 3;
+            }
+        }
     }
     return HV_OK;
 }
@@ -4874,7 +4978,8 @@ int hv_refine_merge(hv_ctx* c, const hv_mesh* old, const hv_mesh* S, hv_mesh** o
     template int hv_mesh_export_impl<D>(const hv_mesh*, int64_t*, double*);                        \
     template int hv_refine_merge<D>(hv_ctx*, const hv_mesh*, const hv_mesh*, hv_mesh**, int64_t*);   \
     template int hv_raycast_batch_impl<D>(hv_ctx*, int, int64_t, const int64_t*, const int32_t*,   \
-                                          const double*, int64_t*, double*, double*, int32_t*);    \
+                                          const double*, int64_t*, double*, double*, int32_t*,     \
+                                          int32_t, int64_t*, int32_t*);                            \
     template int hv_raycast_noalloc_impl<D>(hv_ctx*, int, int64_t, const int64_t*, const int32_t*, \
                                             const double*, int64_t*, double*, double*, int32_t*);  \
     template int hv_nn_halfspace_impl<D>(hv_ctx*, int64_t, const double*, const double*,           \

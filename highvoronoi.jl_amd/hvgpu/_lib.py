@@ -81,6 +81,9 @@ def lib():
     L.hv_mesh_images.argtypes = [vp, P(C.c_int64), P(C.c_int64), P(C.c_int8)]
     L.hv_refine_mesh.argtypes = [vp, P(hv_opts), vp, C.c_int64, P(C.c_double), P(vp), P(C.c_int64)]
     L.hv_mesh_export_db.argtypes = [vp, P(C.c_int64), P(C.c_int64), P(C.c_int64)]
+    L.hv_raycast_batch_keys.argtypes = [vp, C.c_int, C.c_int64, P(C.c_int64), P(C.c_int32), P(C.c_double),
+                                        C.c_int32, P(C.c_int64), P(C.c_int32), P(C.c_double), P(C.c_double),
+                                        P(C.c_int32)]
     L.hv_raycast_batch.argtypes = [vp, C.c_int, C.c_int64, P(C.c_int64), P(C.c_int32), P(C.c_double),
                                    P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_int32)]
     L.hv_raycast_batch_noalloc.argtypes = [vp, C.c_int, C.c_int64, vp, vp, vp, vp, vp, vp, vp]
@@ -267,6 +270,25 @@ class Context:
         self._check(lib().hv_build_mesh_cells(self._h, C.byref(o), len(cells),
                                               _i64(cells) if len(cells) else None, C.byref(m)))
         return Mesh(self, m)
+
+    def raycast_batch_keys(self, sig, drop, r, max_sig: int = 32, precision: str = "fp64"):
+        """hv_raycast_batch_keys: walkray with the reference's degenerate inclusion.  Returns
+        (keys[n, max_sig] 0-padded, lengths[n], coords[n, d], t[n], status[n]); status 3 marks a
+        key of more than d + 1 ids."""
+        sig = np.ascontiguousarray(sig, dtype=np.int64)
+        drop = np.ascontiguousarray(drop, dtype=np.int32)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        n, d = sig.shape[0], self.d
+        keys = np.zeros((n, max_sig), dtype=np.int64)
+        lens = np.zeros(n, dtype=np.int32)
+        out_r = np.zeros((n, d))
+        out_t = np.zeros(n)
+        st = np.zeros(n, dtype=np.int32)
+        self._check(lib().hv_raycast_batch_keys(self._h, PRECISIONS[precision], n, _i64(sig),
+                                                drop.ctypes.data_as(C.POINTER(C.c_int32)), _dp(r), max_sig,
+                                                _i64(keys), lens.ctypes.data_as(C.POINTER(C.c_int32)), _dp(out_r),
+                                                _dp(out_t), st.ctypes.data_as(C.POINTER(C.c_int32))))
+        return keys, lens, out_r, out_t, st
 
     def raycast_batch(self, sig, drop, r, precision: str = "fp64"):
         sig = np.ascontiguousarray(sig, dtype=np.int64)

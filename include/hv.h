@@ -214,6 +214,16 @@ int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, con
 int hv_raycast_batch(hv_ctx* ctx, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
                      const double* r, int64_t* out_sig, double* out_r, double* out_t, int32_t* out_status);
 
+/* The same with the reference's degenerate inclusion (src/raycast.jl:926-950): out_sig holds
+ * max_sig (>= d+1) ids per ray, 0 padded, and out_len the key's length.  A ray whose new vertex
+ * has more than d + 1 generators and planes on its sphere (within 1e-10 relative; the reference
+ * keeps every candidate within the rounding of its key) gets the whole key, status 3 and its
+ * vertex r + t u in out_r; a longer key than max_sig leaves status < 0 and out_len -1.  The
+ * edges of such a vertex (FastEdgeIterator, src/edgeiterate.jl:82-295) are not enumerated. */
+int hv_raycast_batch_keys(hv_ctx* ctx, int precision, int64_t nrays, const int64_t* sig, const int32_t* drop,
+                          const double* r, int32_t max_sig, int64_t* out_sig, int32_t* out_len, double* out_r,
+                          double* out_t, int32_t* out_status);
+
 /* No counterpart in the reference: for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid allocation).  hv_raycast_batch
  * with every pointer in device memory on the context's device (sig, drop, r in; out_* out, the

@@ -161,3 +161,25 @@ def lattice(d: int, k: int, eps: float = 0.0, seed: int = 1):
     if eps:
         g = g + eps * np.random.default_rng(seed).uniform(-1, 1, g.shape)
     return g
+
+
+def lattice_boundary_rays(d: int, k: int):
+    """Rays of the exact lattice(d, k) (k a power of two: coordinates exact in binary) that
+    start at a general boundary vertex and end at a vertex out of general position, with the
+    cell the reference walks them from.  2D: the vertex between cells (0, j) and (0, j+1) on
+    the face x_1 = 0, walking into the box (its next vertex is a square's centre, 4 generators
+    on its circle).  3D: the vertex between cells (0, 0, l) and (0, 0, l+1) on the box edge
+    x_1 = x_2 = 0, walking along the face x_1 = 0 (its next vertex has 4 generators and the
+    face on its sphere).  Returns (sig 1-based sorted, drop, r, cell) lists."""
+    n = k ** d
+    idx = lambda *c: int(np.ravel_multi_index(c, (k,) * d)) + 1
+    sig, drop, r, cell = [], [], [], []
+    if d == 2:
+        for j in range(k - 1):
+            key = sorted([idx(0, j), idx(0, j + 1), n + 2])   # plane 2: x_1 = 0
+            sig.append(key); drop.append(key.index(n + 2)); r.append([0.0, (j + 1) / k]); cell.append(idx(0, j))
+    else:
+        for l in range(k - 1):
+            key = sorted([idx(0, 0, l), idx(0, 0, l + 1), n + 2, n + 4])   # planes x_1 = 0, x_2 = 0
+            sig.append(key); drop.append(key.index(n + 4)); r.append([0.0, 0.0, (l + 1) / k]); cell.append(idx(0, 0, l))
+    return np.array(sig), np.array(drop), np.array(r), cell

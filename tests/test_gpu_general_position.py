@@ -122,3 +122,39 @@ def test_batch_mesh_with_a_generator_on_a_face_is_degenerate():
     for m in out:
         if m is not None:
             m.free()
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("d,k", [(2, 8), (3, 4)])
+def test_degenerate_inclusion_matches_oracle(ctx, prec, d, k):
+    """hv_raycast_batch_keys on rays of an exact lattice that end at vertices with more than
+    d + 1 generators (and planes) on their sphere: the oracle's walkray keys, status 3, the
+    same vertex; the rays that stay general give hv_raycast_batch's answer."""
+    from helpers import lattice_boundary_rays
+    xs = lattice(d, k)
+    b, nrm = synth.cube_planes(d)
+    o = Oracle(xs, b, nrm)
+    sig, drop, r, cell = lattice_boundary_rays(d, k)
+    want = [o.walkray(sig[q], int(drop[q]), r[q], cell[q]) for q in range(len(sig))]
+    ctx.set_points(xs)
+    ctx.set_boundary(b, nrm)
+    keys, lens, out_r, _, st = ctx.raycast_batch_keys(sig, drop, r, max_sig=16, precision=prec)
+    for q in range(len(sig)):
+        wk, wr, _ = want[q]
+        assert int(st[q]) == 3 and int(lens[q]) == len(wk) > d + 1
+        assert keys[q, :lens[q]].tolist() == wk.tolist() and not keys[q, lens[q]:].any()
+        assert np.max(np.abs(out_r[q] - wr)) < 1e-12
+    # general rays (from random points) are unchanged
+    ys = synth.uniform(d, 300, 5)
+    ctx.set_points(ys)
+    m = ctx.build_mesh(precision=prec)
+    kk, cc = m.export()
+    m.free()
+    rows = np.nonzero(kk[:, 1] <= 300)[0][::7][:60]   # edges (drop position 0) holding a generator
+    gs = kk[rows]
+    gd = np.zeros(len(rows), dtype=np.int32)
+    a = ctx.raycast_batch(gs, gd, cc[rows], precision=prec)
+    keys2, lens2, out_r2, _, st2 = ctx.raycast_batch_keys(gs, gd, cc[rows], max_sig=d + 1, precision=prec)
+    assert np.array_equal(a[3], st2)
+    ok = st2 != 1
+    assert np.array_equal(a[0][ok], keys2[ok]) and np.all(lens2[ok] == d + 1)

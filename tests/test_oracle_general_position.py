@@ -79,3 +79,29 @@ def test_clipped_mixture_is_not_general():
     b, nrm = synth.cube_planes(5)
     with pytest.raises(RuntimeError, match="non-general"):
         Oracle(xs, b, nrm).voronoi(cells=(int(on[0]), int(on[0]) + 1))
+
+
+@pytest.mark.parametrize("d,k", [(2, 8), (3, 4)])
+def test_degenerate_inclusion_on_exact_lattices(d, k):
+    """The oracle's walkray keeps every generator and plane on the new vertex's sphere
+    (src/raycast.jl:926-950): from a boundary vertex of an exact lattice the ray ends at the
+    centre of a lattice square (2D: its 4 generators) or on a face at the centre of 4 cells
+    (3D: 4 generators and the face), the closed-form keys."""
+    from helpers import lattice_boundary_rays
+    xs = lattice(d, k)
+    b, nrm = synth.cube_planes(d)
+    o = Oracle(xs, b, nrm)
+    sig, drop, r, cell = lattice_boundary_rays(d, k)
+    n = k ** d
+    for q in range(len(sig)):
+        key, r2, _ = o.walkray(sig[q], int(drop[q]), r[q], cell[q])
+        if d == 2:
+            j = q
+            want = sorted([j + 1, j + 2, k + j + 1, k + j + 2])
+            centre = [1.0 / k, (j + 1) / k]
+        else:
+            l = q
+            want = sorted([l + 1, l + 2, k + l + 1, k + l + 2, n + 2])
+            centre = [0.0, 1.0 / k, (l + 1) / k]
+        assert key.tolist() == want
+        assert np.max(np.abs(r2 - centre)) < 1e-12
