@@ -2569,14 +2569,24 @@ __global__ void k_chunk(const int* __restrict__ ord, int nr, int c, int K, const
 }
 
 // ---- a forest of meshes walked together ----------------------------------------------------
-// Rays of the (mesh-ordered) list per mesh: a ray's sort code holds its mesh above 32 bits.
-__global__ __launch_bounds__(256) void k_pkt_hist(const int* __restrict__ list, int n, const int* __restrict__ n_dev,
-                                                  const uint64_t* __restrict__ code, int* __restrict__ cnt) {
+// Rays of the list per mesh.  The list is a subsequence of the rays sorted by their codes,
+// which hold the mesh above 32 bits, so each mesh's rays form one segment: its bounds by binary
+// search, one thread per mesh.
+__global__ void k_pkt_count(const int* __restrict__ list, int n, const int* __restrict__ n_dev,
+                            const uint64_t* __restrict__ code, int nmesh, int* __restrict__ cnt) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmesh) return;
     if (n_dev) n = min(n, *n_dev);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = t < n;
-    const int m = act ? (int)(code[list[t]] >> 32) : 0;
-    hv_wave_add_by_key(cnt, m, act);
+    auto first_of = [&](int key) {   // first list position whose mesh is >= key
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((int)(code[list[mid]] >> 32) < key) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    cnt[m] = first_of(m + 1) - first_of(m);
 }
 
 // Prefix sums over the meshes (a wavefront, 64 meshes per step): seg = first list position of
@@ -3147,8 +3157,7 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
                 int* seg = pget<int>(c, "pk_seg", (size_t)nmesh + 1);
                 int* pko = pget<int>(c, "pk_off", (size_t)nmesh + 1);
                 if (!pcnt || !seg || !pko) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (packets)");
-                HVCK(hipMemsetAsync(pcnt, 0, sizeof(int) * nmesh, c->stream));
-                k_pkt_hist<<<nblk(n, 256), 256, 0, c->stream>>>(list, n, n_dev, R.code, pcnt);
+                k_pkt_count<<<nblk(nmesh, 64), 64, 0, c->stream>>>(list, n, n_dev, R.code, nmesh, pcnt);
                 HVCK(hipGetLastError());
                 k_pkt_plan<<<1, 64, 0, c->stream>>>(pcnt, nmesh, ppw, seg, pko);
                 HVCK(hipGetLastError());

@@ -184,3 +184,38 @@ def test_mesh_survives_new_points_on_its_context():
     assert np.array_equal(got["volumes"], want["volumes"])
     assert np.array_equal(got["areas"], want["areas"])
     m.free()
+
+
+def test_batch_meshes_exported_and_freed_from_many_threads():
+    # meshes of one batch share one allocation; distinct meshes may be exported and freed from
+    # different host threads at once (include/hv.h), the allocation going with the last of them
+    import threading
+    meshes = synth.batched_meshes(count=64, seed=9)
+    pts = [x for _, x in meshes]
+    planes = [synth.cube_planes(x.shape[1]) for x in pts]
+    out, status = build_mesh_batch(pts, planes, precision="bf16")
+    assert list(status) == [0] * len(pts)
+    counts = [m.counts()[0] for m in out]
+    got = [None] * len(out)
+    errors = []
+
+    def work(w):
+        try:
+            for k in range(w, len(out), 8):
+                keys, _ = out[k].export()
+                got[k] = len(keys)
+                out[k].free()
+        except Exception as e:   # reported below
+            errors.append(e)
+
+    th = [threading.Thread(target=work, args=(w,)) for w in range(8)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors and got == counts
+    # the device is fine afterwards: the same batch again
+    out2, status2 = build_mesh_batch(pts, planes, precision="bf16")
+    assert list(status2) == [0] * len(pts) and [m.counts()[0] for m in out2] == counts
+    for m in out2:
+        m.free()
