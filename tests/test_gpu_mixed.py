@@ -79,24 +79,23 @@ def test_mixed_identical_vertex_set_at_5d_100k_mixture(ctx, prec):
 
 def edge_closure_defects_hashed(keys, n):
     """Walkable edges (d-subsets of a key whose smallest id is a generator) not in exactly two
-    keys, over a large table: every edge packed into 17-bit ids (ids < 2^17 here) and hashed to
-    64 bits (a collision among the ~5e7 edges has probability ~1e-4 and can only merge two
-    counts)."""
+    keys, over a large table: every edge hashed to 64 bits id by id (a collision among ~1e8
+    edges has probability ~1e-3 and could only merge two counts)."""
     keys = np.asarray(keys, dtype=np.uint64)
     V, m = keys.shape
-    assert int(keys.max()) < (1 << 17)
+    M1, M2 = np.uint64(0xBF58476D1CE4E5B9), np.uint64(0x94D049BB133111EB)
     hs = []
-    for p in range(m):
-        e = np.delete(keys, p, axis=1)
-        e = e[e[:, 0] <= n]
-        hi = (e[:, 0] << np.uint64(34)) | (e[:, 1] << np.uint64(17)) | e[:, 2]
-        lo = np.zeros(len(e), dtype=np.uint64)
-        for j in range(3, m - 1):
-            lo = (lo << np.uint64(17)) | e[:, j]
-        h = hi * np.uint64(0x9E3779B97F4A7C15) ^ (lo + np.uint64(0x632BE59BD9B4E019))
-        h ^= h >> np.uint64(29)
-        h *= np.uint64(0xBF58476D1CE4E5B9)
-        hs.append(h ^ (h >> np.uint64(32)))
+    with np.errstate(over="ignore"):
+        for p in range(m):
+            e = np.delete(keys, p, axis=1)
+            e = e[e[:, 0] <= n]
+            h = np.full(len(e), 0x9E3779B97F4A7C15, dtype=np.uint64)
+            for j in range(m - 1):
+                h = (h ^ e[:, j]) * M1
+                h ^= h >> np.uint64(31)
+                h *= M2
+                h ^= h >> np.uint64(29)
+            hs.append(h)
     h = np.sort(np.concatenate(hs))
     starts = np.flatnonzero(np.r_[True, h[1:] != h[:-1]])
     counts = np.diff(np.r_[starts, len(h)])
