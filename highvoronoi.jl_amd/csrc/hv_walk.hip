@@ -2171,6 +2171,7 @@ __global__ void k_iota(int* __restrict__ a, int n) {
 // ---------------------------------------------------------------------------------
 // host driver
 
+constexpr int HV_DESC_LANES = 4;   // lanes per interior seed (k_seed_descent)
 template <int D, bool B>
 __global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int nseed, int stride, int nwalk,
                                                      const int* __restrict__ cells, int* __restrict__ vkey,
@@ -3046,7 +3047,7 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
             HVCK(hipGetLastError());
         }
         if (ndesc > 0) {
-            k_seed_descent<D, B><<<nblk(ndesc, 64), 64, 0, c->stream>>>(g, F, ndesc, stride, nwalk, dcells, vkey, vx,
+            k_seed_descent<D, B><<<nblk((int64_t)ndesc * HV_DESC_LANES, 64), 64, 0, c->stream>>>(g, F, ndesc, stride, nwalk, dcells, vkey, vx,
                                                                           vnext + nseed, status + nseed,
                                                                           o ? o->seed : 0);
             HVCK(hipGetLastError());
@@ -3938,13 +3939,17 @@ __global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const 
 // plane, and exactly its d + 1 generators within (1 + 1e-9) times its radius (tree count), so
 // the walk, reaching the same vertex, gives it the same key; its coordinates are computed
 // from the key as every other vertex's.  status[t] = HV_ST_FOUND kept, HV_ST_DROPPED not.
+// HV_DESC_LANES lanes work on one seed: each takes every HV_DESC_LANES-th candidate of a list
+// and every HV_DESC_LANES-th point of a leaf, and the group combines them with shuffles; the
+// rest of a seed's state is held alike by its lanes, so the group's control flow is uniform.
 template <int D, bool B>
 __global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int nseed, int stride, int nwalk,
                                                      const int* __restrict__ cells, int* __restrict__ vkey,
                                                      double* __restrict__ vx, int64_t base, int* __restrict__ status,
                                                      uint64_t seed) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nseed) return;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gt / HV_DESC_LANES, sub = gt % HV_DESC_LANES;
+    if (t >= nseed) return;   // a seed's lanes leave together
     // over a forest (B): from every generator t, in its mesh's tree
     const DevGeo& g = hv_geo<B>(g0, F, t);
     const int N = g.N;
@@ -3953,7 +3958,7 @@ __global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int ns
     const int js = B ? g.rank[t] : (cells ? g.rank[cells[t]] : min(g.n - 1, t * stride + stride / 2));
     S[0] = g.sidx[js];
     if (S[0] >= nwalk) {   // only cells of generators below nwalk are walked
-        status[t] = HV_ST_DROPPED;
+        if (sub == 0) status[t] = HV_ST_DROPPED;
         return;
     }
     double xi[D], p[D], q[D][D];
@@ -3994,7 +3999,7 @@ __global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int ns
         int bo = -1, bj = -1;
         for (int a = 0; a <= step; ++a) {
             const int* nb = g.knn + (int64_t)g.rank[S[a]] * HV_KNN;
-            for (int m = 0; m < HV_KNN; ++m) {
+            for (int m = sub; m < HV_KNN; m += HV_DESC_LANES) {
                 const int j = nb[m];
                 const int o = g.sidx[j];
                 bool member = false;
@@ -4011,6 +4016,14 @@ __global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int ns
                 const double tt = (px2 - pi2) / (2.0 * den);
                 if (tt < best || (tt == best && o < bo)) { best = tt; bo = o; bj = j; }
             }
+        }
+        // the group's (t, id) minimum: the order of the candidates does not change it
+#pragma unroll
+        for (int off = 1; off < HV_DESC_LANES; off <<= 1) {
+            const double b2 = __shfl_xor(best, off, HV_DESC_LANES);
+            const int o2 = __shfl_xor(bo, off, HV_DESC_LANES);
+            const int j2 = __shfl_xor(bj, off, HV_DESC_LANES);
+            if (b2 < best || (b2 == best && o2 >= 0 && (bo < 0 || o2 < bo))) { best = b2; bo = o2; bj = j2; }
         }
         if (bo < 0 || !(best > 0.0) || !(best < INFINITY)) { ok = false; break; }
         // a boundary plane reached first: not an interior seed
@@ -4082,21 +4095,26 @@ __global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int ns
                 if (node < g.P2) { node = 2 * node; continue; }
                 const int leaf = node - g.P2;
                 const int j0 = leaf * g.L, j1 = min(j0 + g.L, g.n);   // sorted positions of this tree
-                for (int j = j0; j < j1; ++j) {
+                int c = 0;
+                for (int j = j0 + sub; j < j1; j += HV_DESC_LANES) {
                     double d2 = 0.0;
 #pragma unroll
                     for (int k = 0; k < D; ++k) {
                         const double a = g.xs[(int64_t)j * (D + 1) + k] - xn[k];
                         d2 = fma(a, a, d2);
                     }
-                    cnt += (d2 <= rr) ? 1 : 0;
+                    c += (d2 <= rr) ? 1 : 0;
                 }
+#pragma unroll
+                for (int off = 1; off < HV_DESC_LANES; off <<= 1) c += __shfl_xor(c, off, HV_DESC_LANES);
+                cnt += c;
                 if (cnt > D + 1) ok = false;
             }
             node = hv_next_node(node);
         }
         ok = ok && cnt == D + 1;
     }
+    if (sub != 0) return;
     if (ok) {
         const int64_t id = base + t;
 #pragma unroll
