@@ -187,9 +187,13 @@ __global__ void k_tree8_level(DevGeo g, int l, int nodes_at_l, float* __restrict
 // radius of the group about it, then one BF16 K-vector per sorted point relative to its
 // group origin.  Records past N are padding whose form is
 // +1e30 for every ray.
+// One wavefront per group of 8 leaves (256 points), the lanes over its points (min, max and
+// the largest distance do not depend on the order they are combined in).
 template <int D, bool B>
-__global__ void k_leaf_origin(DevGeo g0, GeoSet F, const int* __restrict__ off, double* __restrict__ lorg0) {
-    int leaf = blockIdx.x * blockDim.x + threadIdx.x;   // group of 8 leaves (256 points)
+__global__ __launch_bounds__(64) void k_leaf_origin(DevGeo g0, GeoSet F, const int* __restrict__ off,
+                                                    double* __restrict__ lorg0) {
+    int leaf = blockIdx.x;
+    const int lane = threadIdx.x;
     int mk = 0;
     if constexpr (B) {
         if (leaf >= off[F.nmesh]) return;
@@ -204,15 +208,20 @@ __global__ void k_leaf_origin(DevGeo g0, GeoSet F, const int* __restrict__ off, 
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         double mn = 1e300, mx = -1e300;
-        for (int j = j0; j < j1; ++j) {
+        for (int j = j0 + lane; j < j1; j += 64) {
             const double v = g.xs[(int64_t)j * (D + 1) + k];
             mn = fmin(mn, v);
             mx = fmax(mx, v);
         }
+#pragma unroll
+        for (int s2 = 32; s2 > 0; s2 >>= 1) {
+            mn = fmin(mn, __shfl_xor(mn, s2));
+            mx = fmax(mx, __shfl_xor(mx, s2));
+        }
         o[k] = 0.5 * (mn + mx);
     }
     double r2 = 0.0;
-    for (int j = j0; j < j1; ++j) {
+    for (int j = j0 + lane; j < j1; j += 64) {
         double s = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
@@ -222,8 +231,14 @@ __global__ void k_leaf_origin(DevGeo g0, GeoSet F, const int* __restrict__ off, 
         r2 = fmax(r2, s);
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) lorg[(int64_t)leaf * 8 + k] = k < D ? o[k] : 0.0;
-    lorg[(int64_t)leaf * 8 + 7] = sqrt(r2) * (1.0 + 1e-12) + 1e-300;
+    for (int s2 = 32; s2 > 0; s2 >>= 1) r2 = fmax(r2, __shfl_xor(r2, s2));
+    if (lane < 8) {   // the origin in slots 0..D-1, the radius in slot 7 (D <= 6)
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) v = lane == k ? o[k] : v;
+        if (lane == 7) v = sqrt(r2) * (1.0 + 1e-12) + 1e-300;
+        lorg[(int64_t)leaf * 8 + lane] = v;
+    }
 }
 
 template <int D, bool B>
@@ -2344,7 +2359,7 @@ int build_tree_core(hv_ctx* c, int L, int tables, const TreeOut& o, Tree& T, hip
     k_knn64<D, false><<<nblk(N, 64), 64, 0, st>>>(g, GeoSet{}, nullptr, o.knn);
     HVCK(hipGetLastError());
     if (tables != HV_PREC_FP64) {
-        k_leaf_origin<D, false><<<nblk((g.nleaf + 7) / 8, 256), 256, 0, st>>>(g, GeoSet{}, nullptr, o.lorg);
+        k_leaf_origin<D, false><<<(g.nleaf + 7) / 8, 64, 0, st>>>(g, GeoSet{}, nullptr, o.lorg);
         HVCK(hipGetLastError());
         if (tables == HV_PREC_BF16) {
             k_point_kvec<D, false><<<nblk((int64_t)g.nleaf * 32, 256), 256, 0, st>>>(g, GeoSet{}, nullptr, o.lorg, o.pkv);
@@ -2777,7 +2792,7 @@ static int build_forest(hv_ctx* c, int L, int tables, Forest& fo) {
         k_knn64<D, true><<<ob[nm], 64, 0, c->stream>>>(DevGeo{}, Ft, doff, nullptr);
         HVCK(hipGetLastError());
         if (tables != HV_PREC_FP64) {
-            k_leaf_origin<D, true><<<nblk(og[nm], 256), 256, 0, c->stream>>>(DevGeo{}, Ft, doff + (nm + 1), nullptr);
+            k_leaf_origin<D, true><<<og[nm], 64, 0, c->stream>>>(DevGeo{}, Ft, doff + (nm + 1), nullptr);
             HVCK(hipGetLastError());
             if (tables == HV_PREC_BF16)
                 k_point_kvec<D, true><<<nblk(op[nm], 256), 256, 0, c->stream>>>(DevGeo{}, Ft, doff + 2 * (nm + 1), nullptr,
