@@ -47,6 +47,26 @@ void* DevPool::get(const std::string& name, size_t bytes, bool keep_contents, hi
     return p;
 }
 
+void* DevPool::reserve(const std::string& name, size_t bytes, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(mu);
+    DBuf& b = bufs[name];
+    if (bytes <= b.n && b.p) return b.p;
+    if (b.p) {   // the old one first: the pair may not fit together
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    b.p = p;
+    b.n = std::max<size_t>(bytes, 256);
+    return p;
+}
+
 size_t DevPool::total() {
     std::lock_guard<std::mutex> lk(mu);
     size_t t = 0;

@@ -12,9 +12,12 @@
 //   * edges (m = d) measure |v_a - v_b|; a face of dimension k > 1 is the union of cones
 //     from one of its vertices p over its (k-1)-faces T = S + {j} that miss p:
 //     |F_S| = (1/k) sum_T h(p, aff F_T) |F_T|, h measured inside aff F_S;
+//   * a sub-face T is found from a vertex v of S that holds j: T is the subset of v's key
+//     with S's positions and j's, whose incidence (v, subset) the pass m + 1 maps to its face
+//     (one load; the map also says whether v is T's first vertex, which takes T once);
 //   * facets (m = 2) are the interfaces A_ik; volumes take the generator as apex.
-// One thread per face reads its children in a fixed order: no atomics, the result does not
-// depend on scheduling.
+// 1, 16 or 64 lanes per face read its children in a fixed order: no atomics, the result does
+// not depend on scheduling.
 //
 // Monte-Carlo integrator (src/mcintegrator.jl:98-204): one workgroup per cell casts ndir
 // rays from the generator (mc_raycast, :207-226) against the cell's neighbours (boundary
@@ -169,6 +172,33 @@ __global__ void k_face_table(const int* __restrict__ fstart, int nface, const ui
     fhi[f] = hi[fstart[f]];
 }
 
+// k_face_emit numbers the m-subsets of a key by their position masks in increasing order, so
+// the subset with mask M is the rank of M among the masks of its popcount
+struct HvMaskRank {
+    unsigned char r[256];
+    constexpr HvMaskRank() : r{} {
+        int cnt[9] = {};
+        for (int mask = 0; mask < 256; ++mask) {
+            int p = 0;
+            for (int b = 0; b < 8; ++b) p += (mask >> b) & 1;
+            r[mask] = (unsigned char)cnt[p]++;
+        }
+    }
+};
+__constant__ HvMaskRank hv_mask_rank = HvMaskRank();
+
+// face of each incidence of a pass (incidence o = vertex * C + subset): f for the face's first
+// incidence (its smallest vertex), ~f for the others; the parent pass finds a sub-face
+// S + {j} of a face S from one of its vertices with one load
+__global__ void k_face_inc(const int* __restrict__ fstart, int nface, const int* __restrict__ perm,
+                           int* __restrict__ foi) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    const int b = fstart[f], e = fstart[f + 1];
+    foi[perm[b]] = f;
+    for (int i = b + 1; i < e; ++i) foi[perm[i]] = ~f;
+}
+
 template <int D>
 __global__ void k_edge_len(const int* __restrict__ fstart, int nface, const int* __restrict__ vid,
                            const double* __restrict__ vc, double* __restrict__ meas, int* __restrict__ bad) {
@@ -206,17 +236,16 @@ __device__ __forceinline__ void hv_face_grad(const IntGeo& g, int a, int b, doub
     }
 }
 
-// Measure of each face of pass m from the faces of pass m + 1.  WAVE = false: one thread per
-// face; WAVE = true: one wavefront per face, its lanes splitting the face's vertex list (faces
-// of many vertices: the facets and ridges of cells in 4D and up), the partial sums reduced.
-// (A dispatch holds at most 2^32 work-items, so the WAVE form walks the faces grid-stride.)
-template <int D, bool WAVE>
+// Measure of each face of pass m from the faces of pass m + 1.  L lanes per face (1, 16 or 64),
+// splitting the face's vertex list (faces of many vertices: the facets and ridges of cells in
+// 5D and up), the partial sums reduced in a fixed order.  (A dispatch holds at most 2^32
+// work-items, so the L > 1 forms walk the faces grid-stride.)
+template <int D, int L>
 __device__ __forceinline__ void hv_face_measure_one(int f, int lane, int step, const IntGeo& g, int m,
                                                     const int* __restrict__ fstart, const int* __restrict__ vid,
                                                     const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi,
-                                                    int nface1, const int* __restrict__ fstart1,
-                                                    const int* __restrict__ vid1, const uint64_t* __restrict__ flo1,
-                                                    const uint64_t* __restrict__ fhi1, const double* __restrict__ meas1,
+                                                    int nface1, int C1, const int* __restrict__ foi1,
+                                                    const double* __restrict__ meas1,
                                                     const int* __restrict__ vk, const double* __restrict__ vc,
                                                     double* __restrict__ meas, int* __restrict__ bad) {
     const uint64_t lo = flo[f], hi = fhi[f];
@@ -266,31 +295,28 @@ __device__ __forceinline__ void hv_face_measure_one(int f, int lane, int step, c
     for (int i = b0 + lane; i < e0; i += step) {
         const int v = vid[i];
         const int* kv = vk + (int64_t)v * (D + 1);
+        int kr[D + 1], smask = 0;
+#pragma unroll
         for (int e = 0; e <= D; ++e) {
-            const int j = kv[e];
+            kr[e] = kv[e];
+            bool in = false;
+#pragma unroll
+            for (int s = 0; s <= D; ++s) in = in || (S[s] == kr[e]);
+            smask |= in ? 1 << e : 0;
+        }
+        for (int e = 0; e <= D; ++e) {
+            if ((smask >> e) & 1) continue;   // j in S
+            const int j = kr[e];
             bool skip = false;
 #pragma unroll
-            for (int s = 0; s <= D; ++s) skip = skip || (S[s] == j) || (kp[s] == j);
-            if (skip) continue;   // j in S, or the sub-face holds the apex (zero height)
-            // T = S + {j}, sorted
-            uint64_t tl = ~0ull, th = ~0ull;
-            int s2 = 0;
-            bool put = false;
-            for (int s = 0; s < m; ++s) {
-                if (!put && j < S[s]) { hv_fkey_put(tl, th, s2++, (uint64_t)j); put = true; }
-                hv_fkey_put(tl, th, s2++, (uint64_t)S[s]);
-            }
-            if (!put) hv_fkey_put(tl, th, s2++, (uint64_t)j);
-            // lower bound in the pass-(m+1) face table
-            int lo_i = 0, hi_i = nface1;
-            while (lo_i < hi_i) {
-                const int mid = (lo_i + hi_i) >> 1;
-                if (hv_fkey_less(fhi1[mid], flo1[mid], th, tl)) lo_i = mid + 1;
-                else hi_i = mid;
-            }
-            if (lo_i >= nface1 || flo1[lo_i] != tl || fhi1[lo_i] != th) { atomicOr(bad, 4); continue; }
-            // each sub-face once: from the first of its vertices in this face's list
-            if (vid1[fstart1[lo_i]] != v) continue;
+            for (int s = 0; s <= D; ++s) skip = skip || (kp[s] == j);
+            if (skip) continue;   // the sub-face holds the apex (zero height)
+            // T = S + {j}: the subset smask + e of this vertex's key in pass m + 1
+            const int x = foi1[(int64_t)v * C1 + hv_mask_rank.r[smask | (1 << e)]];
+            const int t1 = x >= 0 ? x : ~x;
+            if (t1 >= nface1) { atomicOr(bad, 4); continue; }
+            // each sub-face once: from the first of its vertices (lists ascend by vertex)
+            if (x < 0) continue;
             double w[D];
             hv_face_grad<D>(g, a, j, w);
             for (int pass = 0; pass < 2; ++pass)
@@ -320,33 +346,32 @@ __device__ __forceinline__ void hv_face_measure_one(int f, int lane, int step, c
                 }
                 h = fabs(dj - da) / (2.0 * nr);
             }
-            sum = fma(h, meas1[lo_i], sum);
+            sum = fma(h, meas1[t1], sum);
         }
     }
-    if constexpr (WAVE) {
+    if constexpr (L > 1) {
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+        for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
         if (lane != 0) return;
     }
     meas[f] = sum / (double)(D + 1 - m);
 }
 
-template <int D, bool WAVE>
-__global__ void k_face_measure(IntGeo g, int m, int nface, const int* __restrict__ fstart, const int* __restrict__ vid,
-                               const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi, int nface1,
-                               const int* __restrict__ fstart1, const int* __restrict__ vid1,
-                               const uint64_t* __restrict__ flo1, const uint64_t* __restrict__ fhi1,
-                               const double* __restrict__ meas1, const int* __restrict__ vk,
-                               const double* __restrict__ vc, double* __restrict__ meas, int* __restrict__ bad) {
-    if constexpr (WAVE) {
-        for (int f = blockIdx.x; f < nface; f += gridDim.x)
-            hv_face_measure_one<D, true>(f, (int)(threadIdx.x & 63), 64, g, m, fstart, vid, flo, fhi, nface1, fstart1,
-                                         vid1, flo1, fhi1, meas1, vk, vc, meas, bad);
+template <int D, int L>
+__global__ __launch_bounds__(128) void k_face_measure(IntGeo g, int m, int nface, const int* __restrict__ fstart, const int* __restrict__ vid,
+                               const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi, int nface1, int C1,
+                               const int* __restrict__ foi1, const double* __restrict__ meas1,
+                               const int* __restrict__ vk, const double* __restrict__ vc, double* __restrict__ meas,
+                               int* __restrict__ bad) {
+    if constexpr (L > 1) {   // one wavefront per workgroup, 64 / L faces at a time
+        for (int64_t f = (int64_t)blockIdx.x * (64 / L) + threadIdx.x / L; f < nface; f += (int64_t)gridDim.x * (64 / L))
+            hv_face_measure_one<D, L>((int)f, (int)(threadIdx.x % L), L, g, m, fstart, vid, flo, fhi, nface1, C1,
+                                      foi1, meas1, vk, vc, meas, bad);
     } else {
         const int f = blockIdx.x * blockDim.x + threadIdx.x;
         if (f < nface)
-            hv_face_measure_one<D, false>(f, 0, 1, g, m, fstart, vid, flo, fhi, nface1, fstart1, vid1, flo1, fhi1,
-                                          meas1, vk, vc, meas, bad);
+            hv_face_measure_one<D, 1>(f, 0, 1, g, m, fstart, vid, flo, fhi, nface1, C1, foi1, meas1, vk, vc, meas,
+                                      bad);
     }
 }
 
@@ -394,15 +419,14 @@ __global__ void k_edge_int(const int* __restrict__ fstart, int nface, const int*
         J[(int64_t)f * nf + c] = meas[f] * 0.5 * (fv[(int64_t)va * nf + c] + fv[(int64_t)vb * nf + c]);
 }
 
-// J of each face of pass m from the faces of pass m + 1, apex at the face's centre (WAVE: one
-// wavefront per face, as k_face_measure)
-template <int D, bool WAVE>
+// J of each face of pass m from the faces of pass m + 1, apex at the face's centre (L lanes per
+// face, as k_face_measure)
+template <int D, int L>
 __device__ __forceinline__ void hv_face_int_one(int f, int lane, int step, const IntGeo& g, int m,
                                                 const int* __restrict__ fstart, const int* __restrict__ vid,
                                                 const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi,
-                                                const double* __restrict__ cen, int nface1,
-                                                const int* __restrict__ fstart1, const int* __restrict__ vid1,
-                                                const uint64_t* __restrict__ flo1, const uint64_t* __restrict__ fhi1,
+                                                const double* __restrict__ cen, int nface1, int C1,
+                                                const int* __restrict__ foi1,
                                                 const double* __restrict__ J1, int nf, const int* __restrict__ vk,
                                                 double* __restrict__ J, int* __restrict__ bad) {
     const uint64_t lo = flo[f], hi = fhi[f];
@@ -450,28 +474,22 @@ __device__ __forceinline__ void hv_face_int_one(int f, int lane, int step, const
     for (int i = b0 + lane; i < e0; i += step) {
         const int v = vid[i];
         const int* kv = vk + (int64_t)v * (D + 1);
-        for (int e = 0; e <= D; ++e) {
-            const int j = kv[e];
-            bool skip = false;
+        int kr[D + 1], smask = 0;
 #pragma unroll
-            for (int s = 0; s <= D; ++s) skip = skip || (S[s] == j);
-            if (skip) continue;
-            uint64_t tl = ~0ull, th = ~0ull;
-            int s2 = 0;
-            bool put = false;
-            for (int s = 0; s < m; ++s) {
-                if (!put && j < S[s]) { hv_fkey_put(tl, th, s2++, (uint64_t)j); put = true; }
-                hv_fkey_put(tl, th, s2++, (uint64_t)S[s]);
-            }
-            if (!put) hv_fkey_put(tl, th, s2++, (uint64_t)j);
-            int lo_i = 0, hi_i = nface1;
-            while (lo_i < hi_i) {
-                const int mid = (lo_i + hi_i) >> 1;
-                if (hv_fkey_less(fhi1[mid], flo1[mid], th, tl)) lo_i = mid + 1;
-                else hi_i = mid;
-            }
-            if (lo_i >= nface1 || flo1[lo_i] != tl || fhi1[lo_i] != th) { atomicOr(bad, 4); continue; }
-            if (vid1[fstart1[lo_i]] != v) continue;   // each sub-face once
+        for (int e = 0; e <= D; ++e) {
+            kr[e] = kv[e];
+            bool in = false;
+#pragma unroll
+            for (int s = 0; s <= D; ++s) in = in || (S[s] == kr[e]);
+            smask |= in ? 1 << e : 0;
+        }
+        for (int e = 0; e <= D; ++e) {
+            if ((smask >> e) & 1) continue;
+            const int j = kr[e];
+            const int x = foi1[(int64_t)v * C1 + hv_mask_rank.r[smask | (1 << e)]];
+            const int t1 = x >= 0 ? x : ~x;
+            if (t1 >= nface1) { atomicOr(bad, 4); continue; }
+            if (x < 0) continue;   // each sub-face once
             double w[D];
             hv_face_grad<D>(g, a, j, w);
             for (int pass = 0; pass < 2; ++pass)
@@ -501,41 +519,53 @@ __device__ __forceinline__ void hv_face_int_one(int f, int lane, int step, const
                 }
                 h = fabs(dj - da) / (2.0 * nr);
             }
-            for (int c = 0; c < nf; ++c) sum[c] = fma(h, J1[(int64_t)lo_i * nf + c], sum[c]);
+            for (int c = 0; c < nf; ++c) sum[c] = fma(h, J1[(int64_t)t1 * nf + c], sum[c]);
         }
     }
-    if constexpr (WAVE) {
+    if constexpr (L > 1) {
         for (int c = 0; c < nf; ++c) {
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) sum[c] += __shfl_xor(sum[c], off);
+            for (int off = L / 2; off > 0; off >>= 1) sum[c] += __shfl_xor(sum[c], off);
         }
         if (lane != 0) return;
     }
     for (int c = 0; c < nf; ++c) J[(int64_t)f * nf + c] = sum[c] / (double)(D + 1 - m);
 }
 
-template <int D, bool WAVE>
-__global__ void k_face_int(IntGeo g, int m, int nface, const int* __restrict__ fstart, const int* __restrict__ vid,
+template <int D, int L>
+__global__ __launch_bounds__(128) void k_face_int(IntGeo g, int m, int nface, const int* __restrict__ fstart, const int* __restrict__ vid,
                            const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi,
-                           const double* __restrict__ cen, int nface1, const int* __restrict__ fstart1,
-                           const int* __restrict__ vid1, const uint64_t* __restrict__ flo1,
-                           const uint64_t* __restrict__ fhi1, const double* __restrict__ J1, int nf,
-                           const int* __restrict__ vk, double* __restrict__ J, int* __restrict__ bad) {
-    if constexpr (WAVE) {
-        for (int f = blockIdx.x; f < nface; f += gridDim.x)
-            hv_face_int_one<D, true>(f, (int)(threadIdx.x & 63), 64, g, m, fstart, vid, flo, fhi, cen, nface1, fstart1,
-                                     vid1, flo1, fhi1, J1, nf, vk, J, bad);
+                           const double* __restrict__ cen, int nface1, int C1, const int* __restrict__ foi1,
+                           const double* __restrict__ J1, int nf, const int* __restrict__ vk, double* __restrict__ J,
+                           int* __restrict__ bad) {
+    if constexpr (L > 1) {
+        for (int64_t f = (int64_t)blockIdx.x * (64 / L) + threadIdx.x / L; f < nface; f += (int64_t)gridDim.x * (64 / L))
+            hv_face_int_one<D, L>((int)f, (int)(threadIdx.x % L), L, g, m, fstart, vid, flo, fhi, cen, nface1, C1,
+                                  foi1, J1, nf, vk, J, bad);
     } else {
         const int f = blockIdx.x * blockDim.x + threadIdx.x;
         if (f < nface)
-            hv_face_int_one<D, false>(f, 0, 1, g, m, fstart, vid, flo, fhi, cen, nface1, fstart1, vid1, flo1, fhi1,
-                                      J1, nf, vk, J, bad);
+            hv_face_int_one<D, 1>(f, 0, 1, g, m, fstart, vid, flo, fhi, cen, nface1, C1, foi1, J1, nf, vk, J, bad);
     }
 }
 
-// faces of many vertices get a wavefront each: the facets and ridges (2 and 3 generators)
-// from 5D on (6D N=50k: 5.4 -> 3.3 s; a thread per face is faster in 4D, and for 6D 3-faces)
-__host__ inline bool hv_face_wave(int D, int m) { return D >= 5 && m <= 3; }
+// lanes per face of pass m, by the face dimension k = d + 1 - m (polygons and 3-faces hold
+// tens of vertices, 4-faces hundreds, 5-faces more).  Measured per pass at 6D N=50k periodic
+// (ms per chunk, L = 1 / 16 / 64): k = 2: 130 / 455 / 1270, k = 3: 231 / 304 / 686,
+// k = 4: 272 / 188 / 206, k = 5: 160 / 87 / 55; at 5D N=100k: k = 3: 56 / 59 / 126,
+// k = 4: 49 / 29 / 31.  A function of (d, m) only, so that every mesh and chunk sums a face's
+// cones in the same order.
+__host__ inline int hv_face_lanes(int D, int m) {
+    const int k = D + 1 - m;
+    return k >= 5 ? 64 : (k == 4 ? 16 : 1);
+}
+
+// launches k_face_measure or k_face_int (K) with L lanes per face
+template <int L, typename K, typename... A>
+static void hv_face_launch(K kern, int nface, hipStream_t s, A... args) {
+    if (L == 1) kern<<<nblk(std::max(nface, 1), 128), 128, 0, s>>>(args...);
+    else kern<<<(unsigned)std::min<int64_t>(std::max<int64_t>(((int64_t)nface * L + 63) / 64, 1), 1 << 20), 64, 0, s>>>(args...);
+}
 
 // facets: I(F) = (d-1)/d J(F) + f(c_F) |F| / d
 template <int D>
@@ -1120,6 +1150,7 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
         }
         return HV_OK;
     };
+    int64_t nvc_max = nv;   // vertices of the largest chunk
     if (!whole && nchunk > 1) {
         // a vertex of cells in several chunks is in each of them: more chunks until every
         // chunk's passes fit (at most 2^31 - 1 incidences)
@@ -1138,10 +1169,37 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
                 HVCK(hipStreamSynchronize(c->stream));
                 worst = std::max<int64_t>(worst, (int64_t)h);
             }
-            if (worst * cmax < (1ll << 31)) break;
+            if (worst * cmax < (1ll << 31)) {
+                nvc_max = worst;
+                break;
+            }
             if (nchunk >= ncell) return hv_fail(c, HV_ERR_CAPACITY, "too many face incidences for one cell");
             nchunk = std::min(ncell, 2 * nchunk);
         }
+    }
+    {
+        // the per-incidence buffers of the face passes at their largest size, once: growing them
+        // between passes and chunks frees and maps tens of GB while the device idles
+        const int64_t tmax = nvc_max * cmax;
+        if (tmax >= (1ll << 31)) return hv_fail(c, HV_ERR_CAPACITY, "too many face incidences for one pass");
+        const size_t T = (size_t)std::max<int64_t>(tmax, 1);
+        bool ok = true;
+        for (const char* nmv : {"i_lo", "i_hi", "i_lo2", "i_hi2"}) ok = ok && c->pool->reserve(nmv, 8 * T, c->stream);
+        for (const char* nmv : {"i_pa", "i_pb", "i_iota", "i_vida", "i_vidb", "i_foia", "i_foib"})
+            ok = ok && c->pool->reserve(nmv, 4 * T, c->stream);
+        ok = ok && c->pool->reserve("i_head", T, c->stream);
+        ok = ok && c->pool->reserve("i_fsta", 4 * (T + 1), c->stream) && c->pool->reserve("i_fstb", 4 * (T + 1), c->stream);
+        if (!whole) {
+            ok = ok && c->pool->reserve("i_vkc", sizeof(int) * (size_t)std::max<int64_t>(nvc_max, 1) * (D + 1), c->stream);
+            ok = ok && c->pool->reserve("i_vcc", sizeof(double) * (size_t)std::max<int64_t>(nvc_max, 1) * D, c->stream);
+        }
+        if (!ok) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
+        size_t tb = 0, tb2 = 0;
+        uint64_t* k0 = (uint64_t*)c->pool->get("i_lo", 0);
+        HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k0, (int*)nullptr, (int*)nullptr, (int)tmax, 0, 64, c->stream));
+        HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb2, (int*)nullptr, (uint8_t*)nullptr, (int*)nullptr, cnt + 1, (int)tmax, c->stream));
+        if (!c->pool->reserve("i_tmp", tb, c->stream) || !c->pool->reserve("i_tmp2", tb2, c->stream))
+            return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
     }
     int64_t ntot = 0;                                  // rows of the chunks so far
     for (int q = 0; q < nchunk; ++q) {
@@ -1204,17 +1262,22 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
             // faces with a cell of the chunk (the others may be incomplete here)
             k_face_emit<D><<<nblk(nvc, 256), 256, 0, c->stream>>>(vkc, nvc, m, g.N, cmask, C, lo, hi, pa, cnt);
             HVCK(hipGetLastError());
-            // two stable radix passes: by lo, then by hi -> (hi, lo) order
-            size_t tb = 0;
-            HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, lo, lo2, pa, pb, (int)tot, 0, 64, c->stream));
-            void* tmp = c->pool->get("i_tmp", tb, false, c->stream);
+            // two stable radix passes: by lo, then by hi -> (hi, lo) order.  Only the id fields a
+            // pass uses: the empty ones are all ones in every real key, and a real key's last
+            // used field is below all ones, so the incidences sent past the end stay there
+            const int lo_bits = 21 * std::min(m, 3), hi_bits = 21 * std::max(m - 3, 0);
+            size_t tb = 0, tbh = 0;
+            HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, lo, lo2, pa, pb, (int)tot, 0, lo_bits, c->stream));
+            if (m > 3)
+                HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbh, hi2, hi, pb, pa, (int)tot, 0, hi_bits, c->stream));
+            void* tmp = c->pool->get("i_tmp", std::max(tb, tbh), false, c->stream);
             if (!tmp) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (sort)");
-            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, lo, lo2, pa, pb, (int)tot, 0, 64, c->stream));
+            HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, lo, lo2, pa, pb, (int)tot, 0, lo_bits, c->stream));
             int* perm = pb;
             if (m > 3) {
                 k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(hi, pb, tot, hi2);
                 HVCK(hipGetLastError());
-                HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, hi2, hi, pb, pa, (int)tot, 0, 64, c->stream));
+                HVCK(hipcub::DeviceRadixSort::SortPairs(tmp, tbh, hi2, hi, pb, pa, (int)tot, 0, hi_bits, c->stream));
                 perm = pa;
                 k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(lo, perm, tot, lo2);
                 HVCK(hipGetLastError());
@@ -1253,6 +1316,13 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
             if (!flo || !fhi || !meas) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
             k_face_table<<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, lo2, hi, flo, fhi);
             HVCK(hipGetLastError());
+            if (m > 2) {   // the next pass finds its sub-faces here
+                int* foi = iget<int>(c, nm("i_foi").c_str(), tot);
+                if (!foi) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
+                HVCK(hipMemsetAsync(foi, 0x80, sizeof(int) * tot, c->stream));   // no face: decodes past the end
+                k_face_inc<<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, perm, foi);
+                HVCK(hipGetLastError());
+            }
             double* J = fi ? iget<double>(c, nm("i_J").c_str(), (size_t)nface * nf) : nullptr;
             if (fi && !J) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
             if (m == D) {
@@ -1264,19 +1334,17 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
             } else {
                 const char* s1 = (m & 1) ? "b" : "a";
                 auto n1 = [&](const char* s) { return std::string(s) + s1; };
-                const int* fstart1 = (const int*)c->pool->get(n1("i_fst"), 0);
-                const int* vid1 = (const int*)c->pool->get(n1("i_vid"), 0);
-                const uint64_t* flo1 = (const uint64_t*)c->pool->get(n1("i_flo"), 0);
-                const uint64_t* fhi1 = (const uint64_t*)c->pool->get(n1("i_fhi"), 0);
+                const int C1 = (int)binom(D + 1, m + 1);
+                const int* foi1 = (const int*)c->pool->get(n1("i_foi"), 0);
                 const double* meas1 = (const double*)c->pool->get(n1("i_meas"), 0);
-                if (hv_face_wave(D, m))
-                    k_face_measure<D, true><<<(unsigned)std::min(std::max(nface, 1), 1 << 20), 64, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi,
-                                                                                   nface1, fstart1, vid1, flo1, fhi1,
-                                                                                   meas1, vkc, vcc, meas, bad);
-                else
-                    k_face_measure<D, false><<<nblk(nface, 128), 128, 0, c->stream>>>(g, m, nface, fstart, vid, flo, fhi,
-                                                                                      nface1, fstart1, vid1, flo1, fhi1,
-                                                                                      meas1, vkc, vcc, meas, bad);
+                if (!foi1 || !meas1) return hv_fail(c, HV_ERR_DEVICE, "face tables of the previous pass missing");
+                const int L = hv_face_lanes(D, m);
+#define HV_FM(LL) hv_face_launch<LL>(k_face_measure<D, LL>, nface, c->stream, g, m, nface, fstart, vid, flo, fhi, \
+                                     nface1, C1, foi1, meas1, (const int*)vkc, (const double*)vcc, meas, bad)
+                if (L == 64) HV_FM(64);
+                else if (L == 16) HV_FM(16);
+                else HV_FM(1);
+#undef HV_FM
                 if (fi) {
                     HVCK(hipGetLastError());
                     double* cen = iget<double>(c, "i_cen", (size_t)nface * D);
@@ -1284,12 +1352,12 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
                     if (!cen || !J1) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (integrand)");
                     k_face_center<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vcc, cen);
                     HVCK(hipGetLastError());
-                    if (hv_face_wave(D, m))
-                        k_face_int<D, true><<<(unsigned)std::min(std::max(nface, 1), 1 << 20), 64, 0, c->stream>>>(
-                            g, m, nface, fstart, vid, flo, fhi, cen, nface1, fstart1, vid1, flo1, fhi1, J1, nf, vkc, J, bad);
-                    else
-                        k_face_int<D, false><<<nblk(nface, 128), 128, 0, c->stream>>>(
-                            g, m, nface, fstart, vid, flo, fhi, cen, nface1, fstart1, vid1, flo1, fhi1, J1, nf, vkc, J, bad);
+#define HV_FI(LL) hv_face_launch<LL>(k_face_int<D, LL>, nface, c->stream, g, m, nface, fstart, vid, flo, fhi, \
+                                     (const double*)cen, nface1, C1, foi1, J1, nf, (const int*)vkc, J, bad)
+                    if (L == 64) HV_FI(64);
+                    else if (L == 16) HV_FI(16);
+                    else HV_FI(1);
+#undef HV_FI
                 }
             }
             HVCK(hipGetLastError());

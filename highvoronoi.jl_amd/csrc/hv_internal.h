@@ -26,6 +26,8 @@ struct DevPool {
     ~DevPool();
     // returns nullptr (and keeps the old buffer) when hipMalloc fails
     void* get(const std::string& name, size_t bytes, bool keep_contents = false, hipStream_t s = nullptr);
+    // at least `bytes`, allocated exactly (no growth margin), contents not kept
+    void* reserve(const std::string& name, size_t bytes, hipStream_t s);
     DBuf take(const std::string& name);
     void give(const std::string& name, DBuf b);
     size_t total();   // bytes held

@@ -343,7 +343,9 @@ class Context:
             if err:
                 if rc == HV_OK:
                     L.hv_integral_free(h)
-                raise err[0]
+                # popped, not indexed: the list would tie the exception to this frame (and the
+                # context) in a cycle that only the collector frees
+                raise err.pop()
             self._check(rc)
         elif method == "polytope":
             self._check(L.hv_integrate_polytope(self._h, mesh._h, C.byref(h)))

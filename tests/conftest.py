@@ -1,3 +1,4 @@
+import gc
 import os
 import sys
 
@@ -11,3 +12,11 @@ for p in (os.path.join(ROOT, "highvoronoi.jl_amd"), os.path.join(ROOT, "oracle")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libhv.so)")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _release_module_contexts():
+    # a module's device contexts (and their memory pools) go with the module, also where a
+    # reference cycle holds one, before the next module allocates its own
+    yield
+    gc.collect()
