@@ -545,57 +545,100 @@ int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, con
     bool any = false;
     for (int dd = 2; dd <= 6; ++dd) any = any || !group[dd].empty();
     if (any) {
-        // one context per device for the fused walks, kept for the process (streams, events,
-        // and its device tables up to HV_BATCH_KEEP bytes, reused by the next call; the meshes
-        // own what they keep); batch calls on one device take turns
+        // Fused walks: the meshes of each dimension in runs of at most HV_FOREST_VERTICES
+        // expected vertices (one walk each; the runs do not depend on how they are scheduled),
+        // taken largest first by a few walkers, each with its own stream and context, so that
+        // one walk's small levels and host waits overlap another's work.  The walkers'
+        // contexts stay for the process (streams, events, and device tables up to
+        // HV_BATCH_KEEP bytes each, reused by the next call; the meshes own what they keep);
+        // batch calls on one device take turns.
         static std::mutex batch_mu;
-        static std::map<int, hv_ctx*> batch_ctx;
+        static std::map<std::pair<int, int>, hv_ctx*> batch_ctx;
         std::lock_guard<std::mutex> lk(batch_mu);
-        hv_ctx*& c = batch_ctx[device_ordinal];
-        int rc0 = HV_OK;
-        if (!c) {
-            rc0 = hv_create(&c, device_ordinal);
-            if (rc0 != HV_OK) c = nullptr;
+        struct Job {
+            int dd;
+            std::vector<int> ks;
+            double est;
+        };
+        std::vector<Job> jobs;
+        for (int dd = 2; dd <= 6; ++dd)
+            for (size_t k0 = 0; k0 < group[dd].size();) {
+                Job j{dd, {}, 0.0};
+                size_t k1 = k0;
+                while (k1 < group[dd].size() &&
+                       (k1 == k0 || j.est + vertices_per_point(dd) * n[group[dd][k1]] <= HV_FOREST_VERTICES)) {
+                    j.est += vertices_per_point(dd) * n[group[dd][k1]];
+                    j.ks.push_back(group[dd][k1]);
+                    ++k1;
+                }
+                jobs.push_back(std::move(j));
+                k0 = k1;
+            }
+        std::stable_sort(jobs.begin(), jobs.end(), [](const Job& a, const Job& b) { return a.est > b.est; });
+        const int nwk = std::max(1, std::min<int>(HV_BATCH_WALKERS, (int)jobs.size()));
+        std::vector<hv_ctx*> ctxs(nwk, nullptr);
+        std::vector<int> crc(nwk, HV_OK);
+        for (int w = 0; w < nwk; ++w) {
+            hv_ctx*& c = batch_ctx[{device_ordinal, w}];
+            if (!c && (crc[w] = hv_create(&c, device_ordinal)) != HV_OK) c = nullptr;
+            ctxs[w] = c;
         }
-        for (int dd = 2; dd <= 6; ++dd) {
-            if (group[dd].empty()) continue;
-            if (rc0 != HV_OK) {
-                for (int k : group[dd]) status[k] = rc0;
-                continue;
+        std::atomic<int> next{0};
+        auto walker = [&](int w) {
+            hv_ctx* c = ctxs[w];
+            if (c) (void)hipSetDevice(c->device);
+            for (;;) {
+                const int q = next.fetch_add(1);
+                if (q >= (int)jobs.size()) break;
+                const Job& j = jobs[q];
+                if (!c) {
+                    for (int k : j.ks) status[k] = crc[w];
+                    continue;
+                }
+                Forest fo;
+                fo.nmesh = (int)j.ks.size();
+                for (int k : j.ks) {
+                    fo.xs.push_back(xs[k]);
+                    fo.n.push_back((int)n[k]);
+                    fo.base.push_back(base[k]);
+                    fo.normal.push_back(normal[k]);
+                }
+                c->D = j.dd;
+                int rc = HV_OK;
+                switch (j.dd) {
+                    case 2: rc = hv_build_forest_meshes<2>(c, opts, fo); break;
+                    case 3: rc = hv_build_forest_meshes<3>(c, opts, fo); break;
+                    case 4: rc = hv_build_forest_meshes<4>(c, opts, fo); break;
+                    case 5: rc = hv_build_forest_meshes<5>(c, opts, fo); break;
+                    default: rc = hv_build_forest_meshes<6>(c, opts, fo); break;
+                }
+                for (size_t q2 = 0; q2 < j.ks.size(); ++q2) {
+                    const int k = j.ks[q2];
+                    out[k] = rc == HV_OK ? fo.out[q2] : nullptr;
+                    status[k] = rc;
+                }
             }
-            Forest fo;
-            fo.nmesh = (int)group[dd].size();
-            for (int k : group[dd]) {
-                fo.xs.push_back(xs[k]);
-                fo.n.push_back((int)n[k]);
-                fo.base.push_back(base[k]);
-                fo.normal.push_back(normal[k]);
+            if (c) {
+                (void)hipStreamSynchronize(c->stream);
+                c->tree_ok = false;
+                constexpr size_t HV_BATCH_KEEP = (size_t)24 << 30;
+                if (c->pool->total() > HV_BATCH_KEEP) {
+                    auto fresh = std::make_shared<DevPool>();
+                    fresh->device = c->device;
+                    c->pool = fresh;   // the previous pool frees its tables here
+                }
             }
-            c->D = dd;
-            int rc = HV_OK;
-            switch (dd) {
-                case 2: rc = hv_build_forest_meshes<2>(c, opts, fo); break;
-                case 3: rc = hv_build_forest_meshes<3>(c, opts, fo); break;
-                case 4: rc = hv_build_forest_meshes<4>(c, opts, fo); break;
-                case 5: rc = hv_build_forest_meshes<5>(c, opts, fo); break;
-                default: rc = hv_build_forest_meshes<6>(c, opts, fo); break;
-            }
-            for (size_t q = 0; q < group[dd].size(); ++q) {
-                const int k = group[dd][q];
-                out[k] = rc == HV_OK ? fo.out[q] : nullptr;
-                status[k] = rc;
+        };
+        std::vector<std::thread> th;
+        for (int w = 1; w < nwk; ++w) {
+            try {
+                th.emplace_back(walker, w);
+            } catch (...) {
+                break;   // fewer walkers: the ones running take every run
             }
         }
-        if (c) {
-            (void)hipStreamSynchronize(c->stream);
-            c->tree_ok = false;
-            constexpr size_t HV_BATCH_KEEP = (size_t)24 << 30;
-            if (c->pool->total() > HV_BATCH_KEEP) {
-                auto fresh = std::make_shared<DevPool>();
-                fresh->device = c->device;
-                c->pool = fresh;   // the previous pool frees its tables here
-            }
-        }
+        walker(0);
+        for (auto& t : th) t.join();
     }
     if (!rest.empty()) {
         const int nw = std::max(1, std::min<int>(nworkers > 0 ? nworkers : (nworkers < 0 ? -nworkers : 8), (int)rest.size()));

@@ -6,6 +6,7 @@
 #include <mutex>
 #include <memory>
 #include <string>
+#include <cstdlib>
 #include <vector>
 #include "../../include/hv.h"
 #include "hv_device.h"
@@ -177,6 +178,17 @@ template <int D>
 int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, const std::vector<int>* cells);
 template <int D>
 int hv_build_forest_meshes(hv_ctx* c, const hv_opts* o, Forest& fo);
+// expected vertices per generator of a mesh in the unit cube (sizes the walk's tables)
+double vertices_per_point(int D);
+// A batch walks the meshes of a dimension in runs of at most HV_FOREST_VERTICES expected
+// vertices (one walk each: a level's ray tables grow with the vertices of the walk, and a
+// bounded walk keeps a context's tables reusable), HV_BATCH_WALKERS runs at once, each on its
+// own context and stream.  Measured on 512 meshes of 3..6D (ms per batch, tables kept between
+// calls): one walker with runs of 16M / 24M / 32M / 40M vertices 541 / 514 / 499 / 583; runs
+// of 32M with 1 / 2 / 3 / 4 walkers 475 / 408 / 393 / 386 (26 GB of tables per walker); 24M
+// with 3 / 4 walkers 398 / 402 (20 GB each); 16M with 4 / 6 walkers 409 / 413 (13 GB each).
+constexpr double HV_FOREST_VERTICES = 24e6;
+constexpr int HV_BATCH_WALKERS = 3;
 template <int D>
 int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, const int64_t* ptr,
                       const uint64_t* pkey, unsigned long long* ccnt, unsigned long long* pcnt, int64_t* nused,

@@ -2458,6 +2458,9 @@ double vertices_per_periodic_cell(int D) {
     }
 }
 
+
+}  // namespace
+
 // vertex count per generator in the unit cube, measured with the CPU oracle (d = 2..6)
 double vertices_per_point(int D) {
     switch (D) {
@@ -2468,8 +2471,6 @@ double vertices_per_point(int D) {
         default: return 700.0;
     }
 }
-
-}  // namespace
 
 // Tentative keys: after the start phase each ray knows a candidate vertex
 // key = edge + nearest-neighbour winner.  Rays of one level with the same tentative key
@@ -3475,12 +3476,8 @@ int hv_walk_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
 }
 
 // Meshes of dimension D walked together: one forest of trees, one breadth-first walk whose
-// levels hold the rays of every mesh, packets of one mesh each.  The meshes are taken in runs of
-// at most HV_FOREST_VERTICES expected vertices (one walk each): a level's ray tables grow with
-// the vertices of the walk, and a bounded walk keeps the context's tables reusable.  Measured
-// on 512 meshes of 3..6D: runs of 16M / 24M / 32M / 40M vertices take 541 / 514 / 499 / 583 ms
-// (fewer levels in all, against tables that no longer stay in the kept pool).
-constexpr double HV_FOREST_VERTICES = 32e6;
+// levels hold the rays of every mesh, packets of one mesh each, in runs of at most
+// HV_FOREST_VERTICES expected vertices.
 
 template <int D>
 int hv_build_forest_meshes(hv_ctx* c, const hv_opts* o, Forest& all) {

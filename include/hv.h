@@ -192,7 +192,9 @@ int hv_mesh_images(const hv_mesh* mesh, int64_t* n_ext, int64_t* orig, int8_t* s
  * Fused walk (nworkers >= 0): the meshes in a cuboid of the reference's plane order without
  * periodic sides are walked together, one walk per dimension: a forest of per-mesh trees in
  * shared tables, global generator ids, every level one launch per kernel over the rays of all
- * meshes (packets of one mesh each), taken in runs of at most 32M expected vertices.  A mesh
+ * meshes (packets of one mesh each), taken in runs of at most 24M expected vertices, three
+ * runs at once, each on its own stream and context (those contexts, with up to 24 GiB of device
+ * tables each, stay for the process and serve later batch calls on the device).  A mesh
  * with a generator outside its box or on a face gets HV_ERR_DOMAIN / HV_ERR_DEGENERATE alone.
  * Every other mesh (periodic sides, other planes; every mesh when nworkers < 0) is built on
  * |nworkers| host threads (0 = 8), each with its own context and HIP stream, taking meshes in
