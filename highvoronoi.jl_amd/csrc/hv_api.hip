@@ -1,6 +1,7 @@
 // hv_api.hip -- the extern "C" surface of libhv.so (declared in include/hv.h).
 #include "hv_internal.h"
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <atomic>
 #include <thread>
@@ -370,6 +371,13 @@ int hv_set_boundary(hv_ctx* c, int nplanes, const double* base, const double* no
     (void)hipSetDevice(c->device);
     const int rc = restore_originals(c);
     if (rc) return rc;
+    // unit normals: a plane's mirror image x + 2<n, b - x> n (src/boundary.jl:206-211) and its
+    // ray intersections assume them
+    for (int i = 0; i < nplanes; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < D; ++k) s += normal[(size_t)i * D + k] * normal[(size_t)i * D + k];
+        if (!(std::fabs(s - 1.0) <= 1e-12)) return hv_fail(c, HV_ERR_ARG, "boundary normals must have unit length");
+    }
     c->tree_ok = false;
     c->P = nplanes;
     c->pbase.assign(base, base + (size_t)nplanes * D);
@@ -519,6 +527,23 @@ static int fusable(int d, int64_t n, const double* xs, int npl, const double* ba
     return HV_OK;
 }
 
+// the fused batch's walker contexts per (device, walker), kept between calls
+static std::mutex batch_mu;
+static std::map<std::pair<int, int>, hv_ctx*> batch_ctx;
+
+int hv_batch_release(int device_ordinal) {
+    std::lock_guard<std::mutex> lk(batch_mu);
+    for (auto it = batch_ctx.begin(); it != batch_ctx.end();) {
+        if (it->first.first == device_ordinal || device_ordinal < 0) {
+            if (it->second) hv_destroy(it->second);
+            it = batch_ctx.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    return HV_OK;
+}
+
 int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, const int64_t* n,
                         const double* const* xs, const int32_t* nplanes, const double* const* base,
                         const double* const* normal, const int16_t* const* bc, const hv_opts* opts,
@@ -552,8 +577,6 @@ int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, con
         // contexts stay for the process (streams, events, and device tables up to
         // HV_BATCH_KEEP bytes each, reused by the next call; the meshes own what they keep);
         // batch calls on one device take turns.
-        static std::mutex batch_mu;
-        static std::map<std::pair<int, int>, hv_ctx*> batch_ctx;
         std::lock_guard<std::mutex> lk(batch_mu);
         struct Job {
             int dd;

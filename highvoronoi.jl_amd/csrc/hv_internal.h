@@ -59,6 +59,14 @@ struct hv_ctx {
     int P = 0;
     std::vector<double> pbase, pnormal;
     std::vector<int16_t> pbc;
+    // the domain the planes bound (set by hv_domain from the current planes): the cuboid of
+    // the reference's plane order, or a general bounded convex polytope with its bounding box
+    // (the Morton frame and the error scales) and its simple vertices (d planes meeting there:
+    // the walk's corner seeds; point ncorner x D, plane indices ncorner x D ascending)
+    bool dom_general = false;
+    double dom_lo[HV_MAXD] = {}, dom_hi[HV_MAXD] = {};
+    std::vector<double> dom_corner;
+    std::vector<int> dom_corner_pl;
     std::string err;
     hipEvent_t ev[4];
     // streams beside `stream` (created on first use): the trees of a forest are built on them
@@ -178,6 +186,8 @@ template <int D>
 int hv_build_mesh_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, const std::vector<int>* cells);
 template <int D>
 int hv_build_forest_meshes(hv_ctx* c, const hv_opts* o, Forest& fo);
+// the domain of the context's planes (fills dom_*); HV_OK, or an HV_* code with *why set
+int hv_domain(hv_ctx* c, int D, const char** why);
 // expected vertices per generator of a mesh in the unit cube (sizes the walk's tables)
 double vertices_per_point(int D);
 // A batch walks the meshes of a dimension in runs of at most HV_FOREST_VERTICES expected

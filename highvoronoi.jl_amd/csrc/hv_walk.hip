@@ -330,12 +330,16 @@ __global__ void k_point_f32(DevGeo g0, GeoSet F, const int* __restrict__ off, co
 }
 
 // ---------------------------------------------------------------------------------
-// seeds: one block per box corner, brute-force nearest generator (ties -> smaller index);
-// over a forest (B), block b is corner b % 2^D of mesh b / 2^D
+// seeds: one block per corner of the domain, brute-force nearest generator (ties -> smaller
+// index): the 2^D box corners, or (cx, cpl: point and ascending plane indices per corner) the
+// vertices of a general polytope where exactly D planes meet.  Over a forest (B), block b is
+// box corner b % 2^D of mesh b / 2^D.
 
 template <int D, bool B>
 __global__ __launch_bounds__(256) void k_seed_corners(DevGeo g0, GeoSet F, int* __restrict__ vkey, double* __restrict__ vx,
-                                                      int64_t base, int* __restrict__ status) {
+                                                      int64_t base, int* __restrict__ status,
+                                                      const double* __restrict__ cx = nullptr,
+                                                      const int* __restrict__ cpl = nullptr) {
     __shared__ double sd[256];
     __shared__ int si[256];
     const int corner = B ? (int)(blockIdx.x & ((1u << D) - 1u)) : (int)blockIdx.x;
@@ -344,9 +348,14 @@ __global__ __launch_bounds__(256) void k_seed_corners(DevGeo g0, GeoSet F, int* 
     int planes[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        bool hi = (corner >> k) & 1;
-        cp[k] = hi ? g.dhi[k] : g.dlo[k];
-        planes[k] = g.N + 2 * k + (hi ? 0 : 1);   // plane 2k: x_k = hi, 2k+1: x_k = lo
+        if (cx) {
+            cp[k] = cx[(int64_t)corner * D + k];
+            planes[k] = g.N + cpl[(int64_t)corner * D + k];
+        } else {
+            bool hi = (corner >> k) & 1;
+            cp[k] = hi ? g.dhi[k] : g.dlo[k];
+            planes[k] = g.N + 2 * k + (hi ? 0 : 1);   // plane 2k: x_k = hi, 2k+1: x_k = lo
+        }
     }
     double best = 1e300;
     int bi = 0x7fffffff;
@@ -2238,6 +2247,8 @@ struct TreeOut {
     float* pxf = nullptr;
     int* bad = nullptr;            // points outside the planes are counted here
     bool planes_ready = false;     // pn / ph already hold the planes
+    const double* dom_lo = nullptr;    // a general polytope's bounding box (null: axis planes)
+    const double* dom_hi = nullptr;
     bool defer_tables = false;     // leave the 16-NN lists and the scan tables to a forest-wide pass
 };
 
@@ -2282,14 +2293,18 @@ int build_tree_core(hv_ctx* c, int L, int tables, const TreeOut& o, Tree& T, hip
     g.nleaf = td.nleaf;
     g.P2 = td.P2;
     g.NN = td.NN;
-    // domain box from the axis-aligned planes
+    // domain box: a general polytope's bounding box, else from the axis-aligned planes
     for (int k = 0; k < D; ++k) { g.dlo[k] = 0.0; g.dhi[k] = 1.0; }
-    for (int p = 0; p < o.P; ++p)
-        for (int k = 0; k < D; ++k) {
-            double nk = o.pnormal[p * D + k];
-            if (nk > 0.5) g.dhi[k] = o.pbase[p * D + k];
-            if (nk < -0.5) g.dlo[k] = o.pbase[p * D + k];
-        }
+    if (o.dom_lo) {
+        for (int k = 0; k < D; ++k) { g.dlo[k] = o.dom_lo[k]; g.dhi[k] = o.dom_hi[k]; }
+    } else {
+        for (int p = 0; p < o.P; ++p)
+            for (int k = 0; k < D; ++k) {
+                double nk = o.pnormal[p * D + k];
+                if (nk > 0.5) g.dhi[k] = o.pbase[p * D + k];
+                if (nk < -0.5) g.dlo[k] = o.pbase[p * D + k];
+            }
+    }
     {
         double s = 0.0;
         for (int k = 0; k < D; ++k) {
@@ -2386,6 +2401,13 @@ int build_tree(hv_ctx* c, int L, Tree& T, int tables = HV_PREC_FP64) {
     o.P = c->P;
     o.pnormal = c->pnormal.data();
     o.pbase = c->pbase.data();
+    {
+        const char* why = nullptr;   // planes that bound no domain (ray-level queries): axis planes
+        if (hv_domain(c, D, &why) == HV_OK && c->dom_general) {
+            o.dom_lo = c->dom_lo;
+            o.dom_hi = c->dom_hi;
+        }
+    }
     o.pn = pget<double>(c, "pn", std::max(1, c->P * D));
     o.ph = pget<double>(c, "ph", std::max(1, c->P));
     o.xo_base = pget<double>(c, "xo", (size_t)N * D);
@@ -2430,18 +2452,13 @@ void record_tree(hv_ctx* c, const DevGeo& g, int tables, int L) {
     c->tree_xw = c->xw;
 }
 
+// the walk's domain: a cuboid or a bounded convex polytope, Dirichlet / Neumann planes
 int check_box(hv_ctx* c, int D) {
-    if (c->P != 2 * D) return hv_fail(c, HV_ERR_UNSUPPORTED, "the device walk needs the 2d planes of a cuboid boundary");
-    for (int k = 0; k < D; ++k)
-        for (int side = 0; side < 2; ++side) {
-            const int p = 2 * k + side;
-            for (int q = 0; q < D; ++q) {
-                double want = (q == k) ? (side == 0 ? 1.0 : -1.0) : 0.0;
-                if (c->pnormal[p * D + q] != want)
-                    return hv_fail(c, HV_ERR_UNSUPPORTED, "boundary planes must be those of cuboid(d, periodic=[]) in the reference's order");
-            }
-            if (c->pbc[p] > 0) return hv_fail(c, HV_ERR_UNSUPPORTED, "periodic planes are not handled by the device walk");
-        }
+    const char* why = nullptr;
+    const int rc = hv_domain(c, D, &why);
+    if (rc) return hv_fail(c, rc, why);
+    for (int p = 0; p < c->P; ++p)
+        if (c->pbc[p] > 0) return hv_fail(c, HV_ERR_UNSUPPORTED, "periodic planes are not handled by the device walk");
     return HV_OK;
 }
 
@@ -2460,6 +2477,208 @@ double vertices_per_periodic_cell(int D) {
 
 
 }  // namespace
+
+// The domain {x : <n_p, x> <= <n_p, b_p> for every plane p} (src/boundary.jl:427-441).
+// The cuboid of the reference's plane order (cuboid(d; ...), src/boundary.jl:510-534): plane
+// 2k is x_k = hi_k with normal +e_k, plane 2k+1 is x_k = lo_k with normal -e_k.  Any other set
+// of planes must bound a convex polytope: it is unbounded when the normals do not span R^d or
+// when some direction y != 0 has <n_p, y> <= 0 for every plane, and such a direction, if any,
+// is the null vector of d - 1 independent normals (an extreme ray of the cone {y : N y <= 0}).
+// The polytope's vertices (d independent planes meeting, no plane violated) give its bounding
+// box; those where exactly d planes meet are the walk's corner seeds.
+static bool hv_solve_small(int D, double* A, double* b) {   // A x = b in place (x -> b), pivoting
+    for (int col = 0; col < D; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < D; ++r)
+            if (std::fabs(A[r * D + col]) > std::fabs(A[piv * D + col])) piv = r;
+        if (!(std::fabs(A[piv * D + col]) > 1e-10)) return false;   // unit normals: well scaled
+        if (piv != col) {
+            for (int k = 0; k < D; ++k) std::swap(A[col * D + k], A[piv * D + k]);
+            std::swap(b[col], b[piv]);
+        }
+        for (int r = col + 1; r < D; ++r) {
+            const double f = A[r * D + col] / A[col * D + col];
+            for (int k = col; k < D; ++k) A[r * D + k] -= f * A[col * D + k];
+            b[r] -= f * b[col];
+        }
+    }
+    for (int r = D - 1; r >= 0; --r) {
+        double s = b[r];
+        for (int k = r + 1; k < D; ++k) s -= A[r * D + k] * b[k];
+        b[r] = s / A[r * D + r];
+    }
+    return true;
+}
+
+// next combination of `m` indices out of `n` (ascending); false after the last
+static bool hv_next_comb(std::vector<int>& a, int n) {
+    const int m = (int)a.size();
+    int i = m - 1;
+    while (i >= 0 && a[i] == n - m + i) --i;
+    if (i < 0) return false;
+    ++a[i];
+    for (int j = i + 1; j < m; ++j) a[j] = a[j - 1] + 1;
+    return true;
+}
+
+int hv_domain(hv_ctx* c, int D, const char** why) {
+    const int P = c->P;
+    const double* nrm = c->pnormal.data();
+    const double* bas = c->pbase.data();
+    c->dom_general = false;
+    c->dom_corner.clear();
+    c->dom_corner_pl.clear();
+    bool cub = P == 2 * D;
+    for (int p = 0; cub && p < P; ++p)
+        for (int q = 0; q < D; ++q)
+            cub = cub && nrm[p * D + q] == ((q == p / 2) ? ((p & 1) ? -1.0 : 1.0) : 0.0);
+    if (cub) {
+        for (int k = 0; k < D; ++k) {
+            c->dom_hi[k] = bas[(2 * k) * D + k];
+            c->dom_lo[k] = bas[(2 * k + 1) * D + k];
+        }
+        return HV_OK;
+    }
+    c->dom_general = true;
+    std::vector<double> h(P);
+    for (int p = 0; p < P; ++p) {
+        double s = 0.0;
+        for (int k = 0; k < D; ++k) s += nrm[p * D + k] * bas[p * D + k];
+        h[p] = s;
+    }
+    double ncomb = 1.0;
+    for (int i = 0; i < D; ++i) ncomb = ncomb * (double)(P - i) / (double)(i + 1);
+    if (P <= D) {
+        *why = "the boundary planes do not enclose a bounded domain (at least d + 1 planes needed)";
+        return HV_ERR_DOMAIN;
+    }
+    if (ncomb > 2e6) {
+        *why = "too many boundary planes for a general polytope (more than 2e6 d-subsets)";
+        return HV_ERR_UNSUPPORTED;
+    }
+    // the normals span R^d?
+    {
+        std::vector<double> Q;
+        int r = 0;
+        for (int p = 0; p < P && r < D; ++p) {
+            std::vector<double> w(nrm + p * D, nrm + p * D + D);
+            for (int pass = 0; pass < 2; ++pass)
+                for (int a = 0; a < r; ++a) {
+                    double dp = 0.0;
+                    for (int k = 0; k < D; ++k) dp += w[k] * Q[a * D + k];
+                    for (int k = 0; k < D; ++k) w[k] -= dp * Q[a * D + k];
+                }
+            double nw = 0.0;
+            for (int k = 0; k < D; ++k) nw += w[k] * w[k];
+            if (nw > 1e-18) {
+                nw = std::sqrt(nw);
+                for (int k = 0; k < D; ++k) Q.push_back(w[k] / nw);
+                ++r;
+            }
+        }
+        if (r < D) {
+            *why = "the boundary planes do not enclose a bounded domain (their normals do not span R^d)";
+            return HV_ERR_DOMAIN;
+        }
+    }
+    // no direction of recession: the null vector of every d - 1 independent normals has a
+    // plane facing it on both sides
+    {
+        std::vector<int> S(D - 1);
+        for (int i = 0; i < D - 1; ++i) S[i] = i;
+        do {
+            std::vector<double> Q;
+            bool indep = true;
+            for (int i = 0; i < D - 1 && indep; ++i) {
+                std::vector<double> w(nrm + S[i] * D, nrm + S[i] * D + D);
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int a = 0; a < i; ++a) {
+                        double dp = 0.0;
+                        for (int k = 0; k < D; ++k) dp += w[k] * Q[a * D + k];
+                        for (int k = 0; k < D; ++k) w[k] -= dp * Q[a * D + k];
+                    }
+                double nw = 0.0;
+                for (int k = 0; k < D; ++k) nw += w[k] * w[k];
+                if (!(nw > 1e-18)) { indep = false; break; }
+                nw = std::sqrt(nw);
+                for (int k = 0; k < D; ++k) Q.push_back(w[k] / nw);
+            }
+            if (!indep) continue;
+            // y: the coordinate axis with the largest part orthogonal to the d - 1 normals
+            double y[HV_MAXD] = {}, best = -1.0;
+            for (int e = 0; e < D; ++e) {
+                double w[HV_MAXD] = {};
+                w[e] = 1.0;
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int a = 0; a < D - 1; ++a) {
+                        double dp = 0.0;
+                        for (int k = 0; k < D; ++k) dp += w[k] * Q[a * D + k];
+                        for (int k = 0; k < D; ++k) w[k] -= dp * Q[a * D + k];
+                    }
+                double nw = 0.0;
+                for (int k = 0; k < D; ++k) nw += w[k] * w[k];
+                if (nw > best) {
+                    best = nw;
+                    for (int k = 0; k < D; ++k) y[k] = w[k];
+                }
+            }
+            best = std::sqrt(best);
+            for (int k = 0; k < D; ++k) y[k] /= best;
+            for (int sgn = -1; sgn <= 1; sgn += 2) {
+                bool recedes = true;
+                for (int q = 0; q < P && recedes; ++q) {
+                    double dp = 0.0;
+                    for (int k = 0; k < D; ++k) dp += nrm[q * D + k] * y[k];
+                    recedes = sgn * dp <= 1e-12;
+                }
+                if (recedes) {
+                    *why = "the boundary planes do not enclose a bounded domain";
+                    return HV_ERR_DOMAIN;
+                }
+            }
+        } while (hv_next_comb(S, P));
+    }
+    // vertices: bounding box, and the simple ones as corner seeds
+    for (int k = 0; k < D; ++k) {
+        c->dom_lo[k] = INFINITY;
+        c->dom_hi[k] = -INFINITY;
+    }
+    std::vector<int> S(D);
+    for (int i = 0; i < D; ++i) S[i] = i;
+    int nvert = 0;
+    do {
+        double A[HV_MAXD * HV_MAXD], x[HV_MAXD];
+        for (int i = 0; i < D; ++i) {
+            for (int k = 0; k < D; ++k) A[i * D + k] = nrm[S[i] * D + k];
+            x[i] = h[S[i]];
+        }
+        if (!hv_solve_small(D, A, x)) continue;
+        bool inside = true;
+        int active = 0;
+        for (int q = 0; q < P && inside; ++q) {
+            double s = -h[q];
+            for (int k = 0; k < D; ++k) s += nrm[q * D + k] * x[k];
+            const double tol = 1e-10 * (1.0 + std::fabs(h[q]));
+            if (s > tol) inside = false;
+            else if (s >= -tol) ++active;
+        }
+        if (!inside) continue;
+        ++nvert;
+        for (int k = 0; k < D; ++k) {
+            c->dom_lo[k] = std::min(c->dom_lo[k], x[k]);
+            c->dom_hi[k] = std::max(c->dom_hi[k], x[k]);
+        }
+        if (active == D) {
+            for (int k = 0; k < D; ++k) c->dom_corner.push_back(x[k]);
+            for (int i = 0; i < D; ++i) c->dom_corner_pl.push_back(S[i]);
+        }
+    } while (hv_next_comb(S, P));
+    if (nvert == 0) {
+        *why = "the boundary planes do not enclose a domain";
+        return HV_ERR_DOMAIN;
+    }
+    return HV_OK;
+}
 
 // vertex count per generator in the unit cube, measured with the CPU oracle (d = 2..6)
 double vertices_per_point(int D) {
@@ -2909,7 +3128,19 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     }
     DevGeo& g = T.g;   // over a forest: unused (every kernel takes its mesh's tree from F)
     const int nmesh = B ? fo->nmesh : 1;
-    const int nseed = nmesh << D;   // box corners
+    // box corners, or the simple vertices of a general polytope (single meshes only)
+    const bool poly = !B && c->dom_general;
+    const int nseed = poly ? (int)(c->dom_corner_pl.size() / D) : nmesh << D;
+    double* ccx = nullptr;
+    int* ccpl = nullptr;
+    if (poly && nseed > 0) {
+        ccx = pget<double>(c, "corner_x", (size_t)nseed * D);
+        ccpl = pget<int>(c, "corner_pl", (size_t)nseed * D);
+        if (!ccx || !ccpl) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (corners)");
+        HVCK(hipMemcpyAsync(ccx, c->dom_corner.data(), sizeof(double) * nseed * D, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipMemcpyAsync(ccpl, c->dom_corner_pl.data(), sizeof(int) * nseed * D, hipMemcpyHostToDevice, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));   // the host vectors may change with the planes
+    }
     // the listed cells (device copy) and their mask over the generators
     uint8_t* cmask = nullptr;
     int* dcells = nullptr;
@@ -3057,9 +3288,9 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
         int* status = pget<int>(c, "status", nstart);
         if (!status) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
         if (part) {   // a corner belongs to a cell that is not walked: status HV_ST_DROPPED (-1)
-            HVCK(hipMemsetAsync(status, 0xff, sizeof(int) * nseed, c->stream));
-        } else {
-            k_seed_corners<D, B><<<nseed, 256, 0, c->stream>>>(g, F, vkey, vx, vnext, status);
+            if (nseed > 0) HVCK(hipMemsetAsync(status, 0xff, sizeof(int) * nseed, c->stream));
+        } else if (nseed > 0) {
+            k_seed_corners<D, B><<<nseed, 256, 0, c->stream>>>(g, F, vkey, vx, vnext, status, ccx, ccpl);
             HVCK(hipGetLastError());
         }
         if (ndesc > 0) {
@@ -3775,6 +4006,9 @@ static int periodic_mesh(hv_ctx* c, const hv_opts* o, hv_mesh** out, int mask, c
 
 int hv_box_periodic_mask(hv_ctx* c, int D, int* mask) {
     *mask = 0;
+    const char* why = nullptr;
+    const int drc = hv_domain(c, D, &why);
+    if (drc) return hv_fail(c, drc, why);
     for (int k = 0; k < D && k < c->P / 2; ++k) {
         const int a = c->pbc[2 * k], b = c->pbc[2 * k + 1];
         if (a <= 0 && b <= 0) continue;
@@ -3783,6 +4017,8 @@ int hv_box_periodic_mask(hv_ctx* c, int D, int* mask) {
             return hv_fail(c, HV_ERR_UNSUPPORTED, "periodic planes must be the pairs of cuboid(d; periodic=...)");
         *mask |= 1 << k;
     }
+    if (*mask && c->dom_general)
+        return hv_fail(c, HV_ERR_UNSUPPORTED, "periodic planes must be the pairs of cuboid(d; periodic=...)");
     if (!*mask) return check_box(c, D);
     std::vector<int16_t> keep = c->pbc;
     std::fill(c->pbc.begin(), c->pbc.end(), (int16_t)0);
@@ -4765,6 +5001,8 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     int pmask = 0;
     int rc = hv_box_periodic_mask(c, D, &pmask);
     if (rc) return rc;
+    if (c->dom_general)
+        return hv_fail(c, HV_ERR_UNSUPPORTED, "point samples are drawn in a box: the planes must be those of cuboid(d; ...)");
     if (pmask && (!c->extended || !c->img_orig || !c->img_shift))
         return hv_fail(c, HV_ERR_STATE, "a periodic box needs its mesh built first");
     HvBox box{};

@@ -71,6 +71,8 @@ def lib():
     L.hv_build_mesh.argtypes = [vp, P(hv_opts), P(vp)]
     L.hv_build_mesh_cells.argtypes = [vp, P(hv_opts), C.c_int64, P(C.c_int64), P(vp)]
     L.hv_tune.argtypes = [vp, C.c_int, C.c_char_p, P(C.c_int32), P(C.c_int32), P(C.c_double)]
+    L.hv_batch_release.argtypes = [C.c_int]
+    L.hv_batch_release.restype = C.c_int
     L.hv_build_mesh_batch.argtypes = [C.c_int, C.c_int32, P(C.c_int32), P(C.c_int64), P(P(C.c_double)),
                                       P(C.c_int32), P(P(C.c_double)), P(P(C.c_double)), P(P(C.c_int16)),
                                       P(hv_opts), C.c_int32, P(vp), P(C.c_int32)]
@@ -435,3 +437,9 @@ def build_mesh_batch(point_sets, planes, precision: str = "fp64", nworkers: int 
                           status.ctypes.data_as(C.POINTER(C.c_int32)))
     meshes = [Mesh(None, C.c_void_p(out[k]), int(d[k])) if out[k] else None for k in range(m)]
     return meshes, status[:m].copy()
+
+
+def release_batch(device: int = 0):
+    """Frees the contexts and device tables that build_mesh_batch's fused walks keep between
+    calls on `device` (all devices when negative); meshes already built stay valid."""
+    lib().hv_batch_release(int(device))

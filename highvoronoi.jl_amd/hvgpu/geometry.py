@@ -71,6 +71,43 @@ class Boundary:
         return base, normal, bc
 
 
+def BC_Dirichlet(base, normal) -> Plane:
+    """A Dirichlet plane with base point `base` and outer unit normal `normal`
+    (src/boundary.jl:37-39)."""
+    return Plane(np.asarray(base, dtype=np.float64), np.asarray(normal, dtype=np.float64), DIRICHLET)
+
+
+def BC_Neumann(base, normal) -> Plane:
+    """A Neumann plane (src/boundary.jl:41-43)."""
+    return Plane(np.asarray(base, dtype=np.float64), np.asarray(normal, dtype=np.float64), NEUMANN)
+
+
+@dataclass
+class BC_Periodic:
+    """Two periodic planes: `base1` with outer normal `normal1`, `base2` with -normal1
+    (src/boundary.jl:45-54)."""
+    base1: np.ndarray
+    base2: np.ndarray
+    normal1: np.ndarray
+
+
+def boundary(*planes) -> Boundary:
+    """Boundary(planes...) (src/boundary.jl:109-131): planes from BC_Dirichlet / BC_Neumann, and
+    BC_Periodic pairs whose two planes name each other (1-based) as partners.  Normals must have
+    unit length (the mirror image x + 2<n, b - x> n assumes it).  The device walk takes the planes
+    of cuboid(d; ...) or any planes of Dirichlet / Neumann type that bound a convex polytope."""
+    out = []
+    for p in planes:
+        if isinstance(p, BC_Periodic):
+            k = len(out) + 1
+            n1 = np.asarray(p.normal1, dtype=np.float64)
+            out.append(Plane(np.asarray(p.base1, dtype=np.float64), n1, k + 1))
+            out.append(Plane(np.asarray(p.base2, dtype=np.float64), -n1, k))
+        else:
+            out.append(p)
+    return Boundary(out)
+
+
 def cuboid(dim: int, dimensions=None, periodic=None, neumann=(), offset=None) -> Boundary:
     """cuboid(dim; dimensions=ones, periodic=1:dim, neumann=[], offset=zeros)
     (src/boundary.jl:510-534).  As in the reference the default is periodic in every

@@ -109,8 +109,14 @@ const char* hv_last_error(const hv_ctx* ctx);
 int hv_set_points(hv_ctx* ctx, int d, int64_t n, const double* xs);
 /* Same, with xs already in device memory (HBM) on the context's device. */
 int hv_set_points_device(hv_ctx* ctx, int d, int64_t n, const double* xs_dev);
-/* Boundary planes (src/boundary.jl:22-29): base and outer normal, d x nplanes each;
- * bc as the reference's Plane.BC (0 Dirichlet, -1 Neumann, >0 periodic partner). */
+/* Boundary planes (src/boundary.jl:22-29, Boundary(planes...) :109-131): base and outer normal,
+ * d x nplanes each, normals of unit length (HV_ERR_ARG otherwise: the mirror image
+ * x + 2<n, b - x> n of src/boundary.jl:206-211 assumes it); bc as the reference's Plane.BC
+ * (0 Dirichlet, -1 Neumann, >0 periodic partner).  The builds take the planes of
+ * cuboid(d; ...) (periodic pairs included), or Dirichlet / Neumann planes of any direction that
+ * bound a convex polytope: the walk then seeds from the polytope's vertices where exactly d
+ * planes meet, and planes that bound no domain (a direction with no plane in front) give
+ * HV_ERR_DOMAIN at build time. */
 int hv_set_boundary(hv_ctx* ctx, int nplanes, const double* base, const double* normal, const int16_t* bc);
 
 /* No counterpart in the reference (tile sizes are a device matter): the autotuner.  Builds the
@@ -206,6 +212,12 @@ int hv_build_mesh_batch(int device_ordinal, int32_t nmesh, const int32_t* d, con
                         const double* const* xs, const int32_t* nplanes, const double* const* base,
                         const double* const* normal, const int16_t* const* bc, const hv_opts* opts,
                         int32_t nworkers, hv_mesh** out, int32_t* status);
+
+/* No counterpart in the reference: frees the contexts (streams and kept device tables) that
+ * hv_build_mesh_batch's fused walks keep on the device between calls (device_ordinal < 0: on
+ * every device), for callers that need the memory back.  Meshes already built stay valid; the
+ * next batch call creates its contexts again.  Waits for a batch call running meanwhile. */
+int hv_batch_release(int device_ordinal);
 
 /* Ray-level entry point: one walkray per input (src/raycast.jl:125-164 via
  * raycast_des2, src/raycast.jl:794-970).  For ray k: sig[k*(d+1) ...] is the sorted

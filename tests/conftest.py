@@ -17,6 +17,10 @@ def pytest_configure(config):
 @pytest.fixture(scope="module", autouse=True)
 def _release_module_contexts():
     # a module's device contexts (and their memory pools) go with the module, also where a
-    # reference cycle holds one, before the next module allocates its own
+    # reference cycle holds one, before the next module allocates its own; so do the contexts
+    # a batched build keeps for the process
     yield
     gc.collect()
+    lib = sys.modules.get("hvgpu._lib")
+    if lib is not None and getattr(lib, "_lib", None) is not None:
+        lib.release_batch(-1)
