@@ -108,18 +108,33 @@ __global__ void k_int_permute(const int* __restrict__ perm, int nv, const int* _
     for (int k = 0; k < D; ++k) vc2[(int64_t)v * D + k] = vc[(int64_t)s * D + k];
 }
 
-// (face, vertex) incidences of pass m: the m-subsets of each key holding a cell of the chunk
-// (cmask over the ids; ids below N are generators); the others are sent past the end
+// k_face_emit numbers the m-subsets of a key by their position masks in increasing order: the
+// c-th mask of popcount m (those below 2^(d+1) come first, so one list serves every d)
+struct HvMaskList {
+    unsigned char m[8][36];
+    constexpr HvMaskList() : m{} {
+        int cnt[8] = {};
+        for (int mask = 0; mask < 128; ++mask) {
+            int p = 0;
+            for (int b = 0; b < 7; ++b) p += (mask >> b) & 1;
+            m[p][cnt[p]++] = (unsigned char)mask;
+        }
+    }
+};
+__constant__ HvMaskList hv_mask_list = HvMaskList();
+
+// (face, vertex) incidences of pass m, one thread each (contiguous stores): incidence
+// o = v * C + c is the c-th m-subset of vertex v's key; one holding no cell of the chunk (cmask
+// over the ids; ids below N are generators) gets the all-ones key, which sorts past the end
+// (k_first_sent counts them after the sort: no same-address atomics here)
 template <int D>
 __global__ void k_face_emit(const int* __restrict__ vk, int nv, int m, int N, const uint8_t* __restrict__ cmask, int C,
-                            uint64_t* __restrict__ lo, uint64_t* __restrict__ hi, int* __restrict__ idx,
-                            unsigned long long* __restrict__ nsent) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nv) return;
-    const int* k = vk + (int64_t)v * (D + 1);
-    int c = 0, sent = 0;
-    for (int mask = 0; mask < (1 << (D + 1)); ++mask) {
-        if (__popc(mask) != m) continue;
+                            uint64_t* __restrict__ lo, uint64_t* __restrict__ hi, int* __restrict__ idx) {
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o < (int64_t)nv * C) {
+        const int v = (int)(o / C), c = (int)(o - (int64_t)v * C);
+        const int mask = hv_mask_list.m[m][c];
+        const int* k = vk + (int64_t)v * (D + 1);
         uint64_t l = ~0ull, h = ~0ull;
         int s = 0;
         bool real = false;
@@ -132,14 +147,24 @@ __global__ void k_face_emit(const int* __restrict__ vk, int nv, int m, int N, co
                 ++s;
             }
         }
-        if (!real) { l = ~0ull; h = ~0ull; ++sent; }
-        const int64_t o = (int64_t)v * C + c;
+        if (!real) { l = ~0ull; h = ~0ull; }
         lo[o] = l;
         hi[o] = h;
         idx[o] = (int)o;
-        ++c;
     }
-    if (sent) atomicAdd(nsent, (unsigned long long)sent);
+}
+
+// incidences sent past the end of a sorted pass (lo in sorted order: all ones exactly for them,
+// a real key's first id is below all ones): n minus the first such position, by bisection
+__global__ void k_first_sent(const uint64_t* __restrict__ lo, int64_t n, unsigned long long* __restrict__ nsent) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int64_t a = 0, b = n;
+    while (a < b) {
+        const int64_t mid = (a + b) >> 1;
+        if (lo[mid] == ~0ull) b = mid;
+        else a = mid + 1;
+    }
+    *nsent = (unsigned long long)(n - a);
 }
 
 __global__ void k_gather_u64(const uint64_t* __restrict__ src, const int* __restrict__ perm, int64_t n,
@@ -1258,9 +1283,8 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
             int* pb = iget<int>(c, "i_pb", tot);
             uint8_t* head = iget<uint8_t>(c, "i_head", tot);
             if (!lo || !hi || !lo2 || !hi2 || !pa || !pb || !head) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
-            HVCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
             // faces with a cell of the chunk (the others may be incomplete here)
-            k_face_emit<D><<<nblk(nvc, 256), 256, 0, c->stream>>>(vkc, nvc, m, g.N, cmask, C, lo, hi, pa, cnt);
+            k_face_emit<D><<<nblk(std::max<int64_t>(tot, 1), 256), 256, 0, c->stream>>>(vkc, nvc, m, g.N, cmask, C, lo, hi, pa);
             HVCK(hipGetLastError());
             // two stable radix passes: by lo, then by hi -> (hi, lo) order.  Only the id fields a
             // pass uses: the empty ones are all ones in every real key, and a real key's last
@@ -1287,6 +1311,8 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
                 HVCK(hipMemcpyAsync(hi, hi2, sizeof(uint64_t) * tot, hipMemcpyDeviceToDevice, c->stream));
             }
             // lo2 / hi: sorted keys; perm: sorted incidence numbers
+            k_first_sent<<<1, 64, 0, c->stream>>>(lo2, tot, cnt);
+            HVCK(hipGetLastError());
             unsigned long long hsent = 0;
             HVCK(hipMemcpyAsync(&hsent, cnt, sizeof(hsent), hipMemcpyDeviceToHost, c->stream));
             HVCK(hipStreamSynchronize(c->stream));

@@ -192,3 +192,34 @@ def test_periodic_flags_need_a_cuboid(ctx):
     with pytest.raises(HVError) as e:
         ctx.build_mesh()
     assert e.value.code == -7
+
+
+def test_batch_and_voronoi_geometry_on_polytopes(ctx):
+    # non-cuboid planes go to the batch's host workers; each mesh equals its single build
+    from hvgpu import BC_Dirichlet, BC_Neumann, VI_POLYGON, VoronoiGeometry, boundary
+    from hvgpu._lib import build_mesh_batch
+    cases = []
+    for d, n, seed in ((2, 300, 61), (3, 400, 62), (4, 200, 63)):
+        base, normal = cut_cube(d)
+        cases.append((points(d, n, seed, base, normal), base, normal))
+    pb, pn = pentagon()
+    cases.append((points(2, 250, 64, pb, pn), pb, pn))
+    out, status = build_mesh_batch([x for x, _, _ in cases], [(b, nr) for _, b, nr in cases], precision="bf16")
+    assert list(status) == [0] * len(cases)
+    for m, (x, b, nr) in zip(out, cases):
+        single = build(ctx, x, b, nr, "bf16")
+        k1, c1 = single.export()
+        k, c = m.export()
+        assert np.array_equal(k, k1) and np.array_equal(c, c1)
+        single.free()
+        m.free()
+    # VoronoiGeometry with Boundary(planes...): Dirichlet and Neumann planes act alike on the mesh
+    d = 3
+    base, normal = cut_cube(d)
+    xs = points(d, 500, 65, base, normal)
+    planes = [BC_Dirichlet(b, nr) if i % 2 else BC_Neumann(b, nr) for i, (b, nr) in enumerate(zip(base, normal))]
+    VG = VoronoiGeometry(xs, boundary(*planes), integrator=VI_POLYGON)
+    assert abs(VG.integral["volumes"].sum() - (1.0 - 0.5 ** d / 6.0)) < 1e-12
+    keys, _ = VG.vertex_table()
+    ok, _, _ = Oracle(xs, base, normal).voronoi()
+    assert np.array_equal(rows_view(keys), rows_view(ok))
