@@ -1529,9 +1529,13 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g0, 
                                                   const int* __restrict__ vkey, const double* __restrict__ vx,
                                                   int* __restrict__ status, int* __restrict__ win,
                                                   double* __restrict__ tout, double plane_tol,
-                                                  unsigned long long* __restrict__ ctr) {
+                                                  unsigned long long* __restrict__ ctr0, int cstride) {
     __shared__ int stk[128], stkl[128];
     __shared__ float cbx[16 * D];
+    // statistics counter k of this workgroup: ctr[k * cstride]; with cstride > 1 the
+    // workgroups spread over cstride copies (same-address atomics from every packet, one per
+    // wavefront, cost about 1 ms per build at 5D N=100k)
+    unsigned long long* __restrict__ ctr = ctr0 + (cstride > 1 ? blockIdx.x % (unsigned)cstride : 0u);
     // MODE 0: the staged leaf (FP64).  MODE 1: ray K-vectors of the current node (MFMA B
     // operand source, 80-byte rows).  MODE 2: the staged leaf (FP32 rows relative to the
     // group origin).  MODE 1 and 2: the sorted ids of each ray's edge generators (the
@@ -2026,14 +2030,14 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g0, 
     if (lane == 0) {
         // BF16: every MFMA tile forms 16 x 16 (ray, point) pairs
         atomicAdd(&ctr[0], MODE == 1 ? 256ull * tiles : pairs);
-        atomicAdd(&ctr[1], (unsigned long long)leaves);
+        atomicAdd(&ctr[cstride], (unsigned long long)leaves);
     }
     if constexpr (MODE >= 1) {
         // FP64 re-verifications and the rays that needed any
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) rechecks += __shfl_xor(rechecks, off);
-        if (lane == 0 && rechecks) atomicAdd(&ctr[6], rechecks);
-        hv_wave_count(&ctr[7], flagged && act);
+        if (lane == 0 && rechecks) atomicAdd(&ctr[6 * cstride], rechecks);
+        hv_wave_count(&ctr[7 * cstride], flagged && act);
     }
     bool tie = false;
     if (act) {
@@ -2112,7 +2116,7 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g0, 
             tout[rid] = tw;
         }
     }
-    hv_wave_count(&ctr[2], tie);
+    hv_wave_count(&ctr[2 * cstride], tie);
 }
 
 // New key = edge + winner (sorted) and its circumcentre, written to the provisional slot.
@@ -2133,25 +2137,25 @@ template <int D, bool B>
 static void launch_scan(int precision, const DevGeo& g, const GeoSet& F, const int* pk_off, const int* seg, int ppw,
                         unsigned nblocks, const DevRays& R, const int* list, int n, const int* n_dev,
                         const int* vkey, const double* vx, int* status, int* win, double* tout, double plane_tol,
-                        unsigned long long* ctr, hipStream_t st) {
+                        unsigned long long* ctr, hipStream_t st, int cstride = 1) {
     if (precision == HV_PREC_BF16)
         k_raycast64<D, 1, B><<<nblocks, 64, 0, st>>>(g, F, pk_off, seg, R, list, n, n_dev, ppw, vkey, vx, status, win, tout,
-                                                    plane_tol, ctr);
+                                                    plane_tol, ctr, cstride);
     else if (precision == HV_PREC_FP32)
         k_raycast64<D, 2, B><<<nblocks, 64, 0, st>>>(g, F, pk_off, seg, R, list, n, n_dev, ppw, vkey, vx, status, win, tout,
-                                                    plane_tol, ctr);
+                                                    plane_tol, ctr, cstride);
     else
         k_raycast64<D, 0, B><<<nblocks, 64, 0, st>>>(g, F, pk_off, seg, R, list, n, n_dev, ppw, vkey, vx, status, win, tout,
-                                                    plane_tol, ctr);
+                                                    plane_tol, ctr, cstride);
 }
 
 template <int D>
 static void launch_scan1(int precision, const DevGeo& g, const DevRays& R, const int* list, int n, const int* n_dev,
                          const int* vkey, const double* vx, int* status, int* win, double* tout, double plane_tol,
-                         unsigned long long* ctr, hipStream_t st, const HvTiles& tiles = HvTiles{}) {
+                         unsigned long long* ctr, hipStream_t st, const HvTiles& tiles = HvTiles{}, int cstride = 1) {
     const int ppw = hv_packet_rays(n, tiles);
     launch_scan<D, false>(precision, g, GeoSet{}, nullptr, nullptr, ppw, nblk(n, ppw), R, list, n, n_dev, vkey, vx, status,
-                          win, tout, plane_tol, ctr, st);
+                          win, tout, plane_tol, ctr, st, cstride);
 }
 
 template <int D, bool B>
@@ -3167,11 +3171,16 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     int* ids = pget<int>(c, "ids", (size_t)vcap);
     int* htab = pget<int>(c, "htab", hsize);
     unsigned long long* ctr = pget<unsigned long long>(c, "ctr", 16);
+    // the scan's statistics (0 pairs, 1 leaves, 2 ties, 6 FP64 rechecks, 7 rays rechecked), each
+    // in HV_SCAN_SPREAD copies that the packets share out
+    constexpr int HV_SCAN_SPREAD = 256;
+    unsigned long long* sctr = pget<unsigned long long>(c, "sctr", 8 * HV_SCAN_SPREAD);
     int* dcount = pget<int>(c, "dcount", 4);
-    if (!vkey || !vx || !ids || !htab || !ctr || !dcount)
+    if (!vkey || !vx || !ids || !htab || !ctr || !sctr || !dcount)
         return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tables)");
     HVCK(hipMemsetAsync(htab, 0xff, hsize * sizeof(int), c->stream));
     HVCK(hipMemsetAsync(ctr, 0, 16 * sizeof(unsigned long long), c->stream));
+    HVCK(hipMemsetAsync(sctr, 0, 8 * HV_SCAN_SPREAD * sizeof(unsigned long long), c->stream));
     hv_stats_t st{};
     unsigned long long hctr[16];
     int64_t vnext = 0;   // next provisional vertex id
@@ -3411,10 +3420,11 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
                 HVCK(hipGetLastError());
                 HVCK(hipEventRecord(r0, c->stream));
                 launch_scan<D, true>(prec, g, F, pko, seg, ppw, nblk(n, ppw) + (unsigned)nmesh, R, list, n, n_dev, vkey, vx,
-                                     status, rwin, rt, plane_tol, ctr + 0, c->stream);
+                                     status, rwin, rt, plane_tol, sctr, c->stream, HV_SCAN_SPREAD);
             } else {
                 HVCK(hipEventRecord(r0, c->stream));
-                launch_scan1<D>(prec, g, R, list, n, n_dev, vkey, vx, status, rwin, rt, plane_tol, ctr + 0, c->stream, tiles);
+                launch_scan1<D>(prec, g, R, list, n, n_dev, vkey, vx, status, rwin, rt, plane_tol, sctr, c->stream, tiles,
+                                HV_SCAN_SPREAD);
             }
             HVCK(hipGetLastError());
             HVCK(hipEventRecord(r1, c->stream));
@@ -3580,6 +3590,15 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     HVCK(hipEventRecord(e1, c->stream));
     rc = read_ctr();
     if (rc) return rc;
+    {
+        std::vector<unsigned long long> hs(8 * HV_SCAN_SPREAD);
+        HVCK(hipMemcpyAsync(hs.data(), sctr, sizeof(unsigned long long) * hs.size(), hipMemcpyDeviceToHost, c->stream));
+        HVCK(hipStreamSynchronize(c->stream));
+        for (int k : {0, 1, 2, 6, 7}) {
+            hctr[k] = 0;
+            for (int q = 0; q < HV_SCAN_SPREAD; ++q) hctr[k] += hs[(size_t)k * HV_SCAN_SPREAD + q];
+        }
+    }
     float ms_all = 0.f;
     HVCK(hipEventElapsedTime(&ms_all, e0, e1));
     if (hctr[9]) return hv_fail(c, HV_ERR_CAPACITY, "vertex hash overflow");
