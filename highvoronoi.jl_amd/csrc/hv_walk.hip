@@ -406,10 +406,12 @@ __device__ __forceinline__ bool keys_equal(const int* __restrict__ a, const int*
     return eq;
 }
 
-// Adds the workgroup's counts of four conditions to ctr[0..3]: partial sums in LDS, then at
-// most one global atomic per counter and workgroup (same-address atomics serialise).  Every
-// thread of the workgroup calls it.
-__device__ __forceinline__ void hv_block_count4(unsigned long long* ctr, bool c0, bool c1, bool c2, bool c3) {
+// Adds the workgroup's counts of four conditions to counters 0..3 (counter k at
+// ctr[k * stride + the workgroup's copy]: with stride > 1 the workgroups spread over that many
+// copies): partial sums in LDS, then at most one global atomic per counter and workgroup
+// (same-address atomics serialise).  Every thread of the workgroup calls it.
+__device__ __forceinline__ void hv_block_count4(unsigned long long* ctr, bool c0, bool c1, bool c2, bool c3,
+                                                int stride = 1) {
     __shared__ unsigned int part[4];
     if (threadIdx.x < 4) part[threadIdx.x] = 0u;
     __syncthreads();
@@ -422,14 +424,16 @@ __device__ __forceinline__ void hv_block_count4(unsigned long long* ctr, bool c0
         if (n3) atomicAdd(&part[3], n3);
     }
     __syncthreads();
-    if (threadIdx.x < 4 && part[threadIdx.x]) atomicAdd(&ctr[threadIdx.x], (unsigned long long)part[threadIdx.x]);
+    if (threadIdx.x < 4 && part[threadIdx.x])
+        atomicAdd(&ctr[threadIdx.x * stride + (stride > 1 ? blockIdx.x % (unsigned)stride : 0u)],
+                  (unsigned long long)part[threadIdx.x]);
 }
 
 template <int D>
 __global__ __launch_bounds__(256) void k_insert(const int* __restrict__ status, int n, const int* __restrict__ n_dev,
                                                 const int* __restrict__ vkey, int64_t base, int* __restrict__ htab,
                                                 uint64_t hmask, uint8_t* __restrict__ win,
-                                                unsigned long long* __restrict__ ctr) {
+                                                unsigned long long* __restrict__ ctr, int cstride) {
     // n_dev (when given): the candidate count in device memory; flags up to n are written
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nv = n_dev ? min(n, *n_dev) : n;
@@ -457,7 +461,7 @@ __global__ __launch_bounds__(256) void k_insert(const int* __restrict__ status, 
         }
         win[t] = won;
     }
-    hv_block_count4(ctr, dup, full, none, singular);
+    hv_block_count4(ctr, dup, full, none, singular, cstride);
 }
 
 template <int D>
@@ -1266,6 +1270,8 @@ __device__ __forceinline__ void hv_row_add(const int64_t* __restrict__ ptr, cons
     }
 }
 
+constexpr int HV_MCP_SPREAD = 256;   // copies of the sample kernel's work counter
+
 template <int D>
 __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, uint64_t seed, double eps,
                                                   const int64_t* __restrict__ ptr, const uint64_t* __restrict__ pkey,
@@ -1436,7 +1442,8 @@ __global__ __launch_bounds__(64) void k_mc_points(DevGeo g, int G, int64_t S, ui
         unsigned long long w = work;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
-        if (lane == 0) atomicAdd(npairs, w);
+        // spread over HV_MCP_SPREAD copies (one same-address atomic per wavefront serialises)
+        if (lane == 0) atomicAdd(&npairs[blockIdx.x % HV_MCP_SPREAD], w);
     }
     // the two nearest in FP64 (the FP32 scan may order a near tie either way)
     auto d2_64 = [&](int j) {
@@ -3174,13 +3181,16 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     // the scan's statistics (0 pairs, 1 leaves, 2 ties, 6 FP64 rechecks, 7 rays rechecked), each
     // in HV_SCAN_SPREAD copies that the packets share out
     constexpr int HV_SCAN_SPREAD = 256;
-    unsigned long long* sctr = pget<unsigned long long>(c, "sctr", 8 * HV_SCAN_SPREAD);
+    unsigned long long* sctr = pget<unsigned long long>(c, "sctr", 12 * HV_SCAN_SPREAD);
+    // k_insert's counters (8 duplicates, 9 vertex-hash overflow, 10 rays without a vertex,
+    // 11 singular vertices) likewise, after the scan's
+    unsigned long long* ictr = sctr + 8 * HV_SCAN_SPREAD;
     int* dcount = pget<int>(c, "dcount", 4);
     if (!vkey || !vx || !ids || !htab || !ctr || !sctr || !dcount)
         return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tables)");
     HVCK(hipMemsetAsync(htab, 0xff, hsize * sizeof(int), c->stream));
     HVCK(hipMemsetAsync(ctr, 0, 16 * sizeof(unsigned long long), c->stream));
-    HVCK(hipMemsetAsync(sctr, 0, 8 * HV_SCAN_SPREAD * sizeof(unsigned long long), c->stream));
+    HVCK(hipMemsetAsync(sctr, 0, 12 * HV_SCAN_SPREAD * sizeof(unsigned long long), c->stream));
     hv_stats_t st{};
     unsigned long long hctr[16];
     int64_t vnext = 0;   // next provisional vertex id
@@ -3244,7 +3254,8 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
                                  const int* n_dev = nullptr) -> int {
         uint8_t* win = pget<uint8_t>(c, "win", n);
         if (!win) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (win)");
-        k_insert<D><<<nblk(n, 256), 256, 0, c->stream>>>(status, n, n_dev, vkey, base, htab, hsize - 1, win, ctr + 8);
+        k_insert<D><<<nblk(n, 256), 256, 0, c->stream>>>(status, n, n_dev, vkey, base, htab, hsize - 1, win, ictr,
+                                                         HV_SCAN_SPREAD);
         HVCK(hipGetLastError());
         size_t tb = 0;
         hipcub::CountingInputIterator<int> it((int)base);
@@ -3591,10 +3602,10 @@ int hv_walk_impl(hv_ctx* c, const hv_opts* o, hv_mesh** out, int nwalk, const st
     rc = read_ctr();
     if (rc) return rc;
     {
-        std::vector<unsigned long long> hs(8 * HV_SCAN_SPREAD);
+        std::vector<unsigned long long> hs(12 * HV_SCAN_SPREAD);
         HVCK(hipMemcpyAsync(hs.data(), sctr, sizeof(unsigned long long) * hs.size(), hipMemcpyDeviceToHost, c->stream));
         HVCK(hipStreamSynchronize(c->stream));
-        for (int k : {0, 1, 2, 6, 7}) {
+        for (int k : {0, 1, 2, 6, 7, 8, 9, 10, 11}) {
             hctr[k] = 0;
             for (int q = 0; q < HV_SCAN_SPREAD; ++q) hctr[k] += hs[(size_t)k * HV_SCAN_SPREAD + q];
         }
@@ -5105,17 +5116,19 @@ int hv_mc_points_impl(hv_ctx* c, int64_t nsamples, uint64_t seed, double slab, c
     const int64_t n = B * S;
     // one work-item per sample, and a dispatch holds at most 2^32 work-items
     if (n > (1ll << 32) - 64) return hv_fail(c, HV_ERR_CAPACITY, "too many samples for one call (at most 2^32)");
-    unsigned long long* npairs = pget<unsigned long long>(c, "mcp_pairs", 1);
+    unsigned long long* npairs = pget<unsigned long long>(c, "mcp_pairs", HV_MCP_SPREAD);
     if (!npairs) return hv_fail(c, HV_ERR_DEVICE, "out of device memory");
-    HVCK(hipMemsetAsync(npairs, 0, sizeof(unsigned long long), c->stream));
+    HVCK(hipMemsetAsync(npairs, 0, sizeof(unsigned long long) * HV_MCP_SPREAD, c->stream));
     HVCK(hipEventRecord(c->ev[2], c->stream));
     k_mc_points<D><<<(unsigned)(n / 64), 64, 0, c->stream>>>(g, G, S, seed, eps, ptr, pkey, ccnt, pcnt, box, pmask,
                                                              (int)c->nreal, porig, pshift, pkeys, pvals, hmask, npairs);
     HVCK(hipGetLastError());
     HVCK(hipEventRecord(c->ev[3], c->stream));
-    unsigned long long hp = 0;
-    HVCK(hipMemcpyAsync(&hp, npairs, sizeof(hp), hipMemcpyDeviceToHost, c->stream));
+    unsigned long long hpv[HV_MCP_SPREAD];
+    HVCK(hipMemcpyAsync(hpv, npairs, sizeof(hpv), hipMemcpyDeviceToHost, c->stream));
     HVCK(hipStreamSynchronize(c->stream));
+    unsigned long long hp = 0;
+    for (int q = 0; q < HV_MCP_SPREAD; ++q) hp += hpv[q];
     float kms = 0.f;
     HVCK(hipEventElapsedTime(&kms, c->ev[2], c->ev[3]));
     *pairs_out = (int64_t)hp;
