@@ -42,6 +42,7 @@ struct DevGeo {
     const int* __restrict__ knn;     // HV_KNN nearest generators of each sorted position (sorted indices)
     double dlo[HV_MAXD], dhi[HV_MAXD];  // domain box (Morton frame)
     double xmax;                        // bound on |x| over the domain (rounding bounds)
+    int unbounded;                      // no bounded domain: a ray may have no plane ahead
     // mixed-precision scan (leaves of 32): per sorted point a 32-slot BF16 K-vector relative
     // to the origin of its group of 8 leaves (16 dwords); per group the origin (slots
     // 0..D-1) and the radius of the group about it (slot 7)

@@ -1636,6 +1636,12 @@ static int make_geo(hv_ctx* c, const hv_mesh* mesh, IntGeo& g) {
     // the context is bound to the mesh's own working set and boundary (hv_bind_mesh)
     if (!mesh || mesh->D != D || mesh->N != c->N || !c->xw || c->xw != mesh->xw)
         return hv_fail(c, HV_ERR_STATE, "the context is not bound to the mesh's generators");
+    {
+        const char* why = nullptr;
+        const int rc = hv_domain(c, D, &why);
+        if (rc) return hv_fail(c, rc, why);
+        if (c->dom_unbounded) return hv_fail(c, HV_ERR_UNSUPPORTED, "the integrators need a bounded domain");
+    }
     g.N = (int)c->N;
     g.nreal = (int)mesh->nreal;
     g.P = c->P;

@@ -64,6 +64,8 @@ struct hv_ctx {
     // (the Morton frame and the error scales) and its simple vertices (d planes meeting there:
     // the walk's corner seeds; point ncorner x D, plane indices ncorner x D ascending)
     bool dom_general = false;
+    bool dom_unbounded = false;   // no bounded domain (Boundary() or too few planes): rays may
+                                  // run to infinity; the frame is the generators' bounding box
     double dom_lo[HV_MAXD] = {}, dom_hi[HV_MAXD] = {};
     std::vector<double> dom_corner;
     std::vector<int> dom_corner_pl;

@@ -53,6 +53,54 @@ __device__ __forceinline__ int hv_mesh_of_item(const int* __restrict__ off, int 
     return lo;
 }
 
+// doubles as unsigned keys in the same order (min / max by integer atomics)
+__host__ __device__ inline unsigned long long hv_double_key(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double hv_key_double(unsigned long long k) {
+    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __builtin_bit_cast(double, b);
+}
+
+// bounding box of n points (n x D): key[k] min, key[D + k] max; per workgroup in LDS, then one
+// atomic per coordinate and workgroup
+template <int D>
+__global__ __launch_bounds__(256) void k_bbox(const double* __restrict__ x, int n, unsigned long long* __restrict__ key) {
+    __shared__ unsigned long long smin[D], smax[D];
+    if (threadIdx.x < D) { smin[threadIdx.x] = ~0ull; smax[threadIdx.x] = 0ull; }
+    __syncthreads();
+    double lo[D], hi[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double v = x[(int64_t)i * D + k];
+            lo[k] = fmin(lo[k], v);
+            hi[k] = fmax(hi[k], v);
+        }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[k] = fmin(lo[k], __shfl_xor(lo[k], off));
+            hi[k] = fmax(hi[k], __shfl_xor(hi[k], off));
+        }
+    }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            atomicMin(&smin[k], hv_double_key(lo[k]));
+            atomicMax(&smax[k], hv_double_key(hi[k]));
+        }
+    __syncthreads();
+    if (threadIdx.x < D) {
+        atomicMin(&key[threadIdx.x], smin[threadIdx.x]);
+        atomicMax(&key[D + threadIdx.x], smax[threadIdx.x]);
+    }
+}
+
 template <int D>
 __global__ void k_points_in(const double* __restrict__ xin, int N, DevGeo g, double* __restrict__ xo,
                             uint64_t* __restrict__ code, int* __restrict__ idx, int* __restrict__ bad) {
@@ -920,6 +968,67 @@ __device__ __forceinline__ void hv_consider(const RayGeo<D>& G, Top2& T, const d
     if (num < den2 * (T.t1 + HV_W)) T.offer(num / den2, j);
 }
 
+// box distance helpers for the per-thread (stackless, heap-order) tree walks
+template <int D>
+__device__ __forceinline__ double hv_box_d2(const DevGeo& g, int node, const double* q) {
+    double d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double lo = g.blo[(int64_t)k * g.NN + node], hi = g.bhi[(int64_t)k * g.NN + node];
+        const double a = fmax(fmax(lo - q[k], q[k] - hi), 0.0);
+        d2 = fma(a, a, d2);
+    }
+    return d2;
+}
+
+__device__ __forceinline__ int hv_next_node(int i) {
+    while (i & 1) i >>= 1;
+    return i == 0 ? 0 : i + 1;
+}
+
+// A ray whose start candidates all lie behind it in a domain without a bounded boundary: the
+// best candidate ahead of it from the binary tree (boxes with a point ahead, <u, x> > c, and
+// within the ball of the best t so far); none: the ray runs to infinity (the reference's
+// unbounded ray, t = Inf, src/sysvoronoi.jl:504-511)
+template <int D>
+__device__ void hv_probe(const DevGeo& g, const RayGeo<D>& G, Top2& T, double plane_tol) {
+    int node = 1;
+    while (node) {
+        double umax = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double lo = g.blo[(int64_t)k * g.NN + node], hi = g.bhi[(int64_t)k * g.NN + node];
+            umax = fma(G.u[k], G.u[k] > 0.0 ? hi : lo, umax);
+        }
+        bool enter = umax > G.cth;
+        if (enter && T.i1 >= 0) {
+            const double tau = T.t1 + HV_W;
+            double q[D], R2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                q[k] = fma(tau, G.u[k], G.r[k]);
+                const double a = q[k] - G.x0[k];
+                R2 = fma(a, a, R2);
+            }
+            enter = hv_box_d2<D>(g, node, q) <= R2 * (1.0 + 1e-9);
+        }
+        if (enter) {
+            if (node < g.P2) {
+                node = 2 * node;
+                continue;
+            }
+            const int j0 = (node - g.P2) * g.L, j1 = min(j0 + g.L, g.n);
+            for (int j = j0; j < j1; ++j) {
+                double xk[D];
+#pragma unroll
+                for (int k = 0; k < D; ++k) xk[k] = g.xs[(int64_t)j * (D + 1) + k];
+                hv_consider<D>(G, T, xk, g.xs[(int64_t)j * (D + 1) + D], j, plane_tol);
+            }
+        }
+        node = hv_next_node(node);
+    }
+}
+
 // Start phase, one thread per ray: boundary planes (mirror pseudo-generators,
 // src/raycast.jl:354-375, src/extended.jl:131-141) and the HV_KNN nearest generators of
 // x0 (the winner is a Delaunay neighbour of x0, usually among them).  The Morton code
@@ -1030,6 +1139,7 @@ __global__ __launch_bounds__(256) void k_ray_init(DevGeo g0, GeoSet F, DevRays R
             hv_consider<D>(G, T, xk, g.xs[(int64_t)(j) * (D + 1) + D], j, plane_tol);
         }
     }
+    if (g.unbounded && T.i1 < 0) hv_probe<D>(g, G, T, plane_tol);
     R.t1[ray] = T.t1;
     R.t2[ray] = T.t2;
     R.i1[ray] = T.i1;
@@ -1615,7 +1725,9 @@ __global__ __launch_bounds__(64, MODE == 1 ? 4 : 1) void k_raycast64(DevGeo g0, 
         const double rr = sqrt(R2) + slackf;
         R2f = (float)(rr * rr * (1.0 + 1e-5));
     };
-    set_ball(T.t1 + HV_W);
+    // a ray with nothing ahead (unbounded domains: its start searched the whole half-space)
+    // keeps an empty ball at its own vertex and ends without one
+    set_ball(T.t1 < 1e299 ? T.t1 + HV_W : 0.0);
     // packet centre: mean ball centre over the active lanes (orders the children)
     float pcf[D];
     {
@@ -2260,6 +2372,7 @@ struct TreeOut {
     bool planes_ready = false;     // pn / ph already hold the planes
     const double* dom_lo = nullptr;    // a general polytope's bounding box (null: axis planes)
     const double* dom_hi = nullptr;
+    bool unbounded = false;        // no bounded domain (dom_lo / dom_hi: the generators' box)
     bool defer_tables = false;     // leave the 16-NN lists and the scan tables to a forest-wide pass
 };
 
@@ -2316,6 +2429,7 @@ int build_tree_core(hv_ctx* c, int L, int tables, const TreeOut& o, Tree& T, hip
                 if (nk < -0.5) g.dlo[k] = o.pbase[p * D + k];
             }
     }
+    g.unbounded = o.unbounded ? 1 : 0;
     {
         double s = 0.0;
         for (int k = 0; k < D; ++k) {
@@ -2413,8 +2527,27 @@ int build_tree(hv_ctx* c, int L, Tree& T, int tables = HV_PREC_FP64) {
     o.pnormal = c->pnormal.data();
     o.pbase = c->pbase.data();
     {
-        const char* why = nullptr;   // planes that bound no domain (ray-level queries): axis planes
+        const char* why = nullptr;
         if (hv_domain(c, D, &why) == HV_OK && c->dom_general) {
+            if (c->dom_unbounded) {   // the frame: the generators' bounding box
+                unsigned long long* kb = pget<unsigned long long>(c, "bbox", 2 * D);
+                if (!kb) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (tree)");
+                std::vector<unsigned long long> init(2 * D);
+                for (int k = 0; k < D; ++k) { init[k] = ~0ull; init[D + k] = 0ull; }
+                HVCK(hipMemcpyAsync(kb, init.data(), sizeof(unsigned long long) * 2 * D, hipMemcpyHostToDevice, c->stream));
+                if (N > 0) {
+                    k_bbox<D><<<std::min<unsigned>(nblk(N, 256), 1024u), 256, 0, c->stream>>>((const double*)c->xw->p, N, kb);
+                    HVCK(hipGetLastError());
+                }
+                HVCK(hipMemcpyAsync(init.data(), kb, sizeof(unsigned long long) * 2 * D, hipMemcpyDeviceToHost, c->stream));
+                HVCK(hipStreamSynchronize(c->stream));
+                for (int k = 0; k < D; ++k) {
+                    c->dom_lo[k] = hv_key_double(init[k]);
+                    c->dom_hi[k] = hv_key_double(init[D + k]);
+                    if (!(c->dom_hi[k] >= c->dom_lo[k])) { c->dom_lo[k] = 0.0; c->dom_hi[k] = 1.0; }
+                }
+                o.unbounded = true;
+            }
             o.dom_lo = c->dom_lo;
             o.dom_hi = c->dom_hi;
         }
@@ -2492,11 +2625,14 @@ double vertices_per_periodic_cell(int D) {
 // The domain {x : <n_p, x> <= <n_p, b_p> for every plane p} (src/boundary.jl:427-441).
 // The cuboid of the reference's plane order (cuboid(d; ...), src/boundary.jl:510-534): plane
 // 2k is x_k = hi_k with normal +e_k, plane 2k+1 is x_k = lo_k with normal -e_k.  Any other set
-// of planes must bound a convex polytope: it is unbounded when the normals do not span R^d or
-// when some direction y != 0 has <n_p, y> <= 0 for every plane, and such a direction, if any,
-// is the null vector of d - 1 independent normals (an extreme ray of the cone {y : N y <= 0}).
-// The polytope's vertices (d independent planes meeting, no plane violated) give its bounding
-// box; those where exactly d planes meet are the walk's corner seeds.
+// of planes bounds a convex polytope or nothing bounded: it is unbounded when there are at
+// most d planes, when the normals do not span R^d, or when some direction y != 0 has
+// <n_p, y> <= 0 for every plane (such a direction, if any, is the null vector of d - 1
+// independent normals: an extreme ray of the cone {y : N y <= 0}).  A bounded polytope's
+// vertices (d independent planes meeting, no plane violated) give its bounding box; those
+// where exactly d planes meet are the walk's corner seeds.  Unbounded (Boundary() without
+// planes among them, src/boundary.jl:133-135): rays may leave every plane behind, and the
+// frame is the generators' bounding box (set by the tree build).
 static bool hv_solve_small(int D, double* A, double* b) {   // A x = b in place (x -> b), pivoting
     for (int col = 0; col < D; ++col) {
         int piv = col;
@@ -2537,6 +2673,7 @@ int hv_domain(hv_ctx* c, int D, const char** why) {
     const double* nrm = c->pnormal.data();
     const double* bas = c->pbase.data();
     c->dom_general = false;
+    c->dom_unbounded = false;
     c->dom_corner.clear();
     c->dom_corner_pl.clear();
     bool cub = P == 2 * D;
@@ -2560,8 +2697,8 @@ int hv_domain(hv_ctx* c, int D, const char** why) {
     double ncomb = 1.0;
     for (int i = 0; i < D; ++i) ncomb = ncomb * (double)(P - i) / (double)(i + 1);
     if (P <= D) {
-        *why = "the boundary planes do not enclose a bounded domain (at least d + 1 planes needed)";
-        return HV_ERR_DOMAIN;
+        c->dom_unbounded = true;
+        return HV_OK;
     }
     if (ncomb > 2e6) {
         *why = "too many boundary planes for a general polytope (more than 2e6 d-subsets)";
@@ -2588,8 +2725,8 @@ int hv_domain(hv_ctx* c, int D, const char** why) {
             }
         }
         if (r < D) {
-            *why = "the boundary planes do not enclose a bounded domain (their normals do not span R^d)";
-            return HV_ERR_DOMAIN;
+            c->dom_unbounded = true;
+            return HV_OK;
         }
     }
     // no direction of recession: the null vector of every d - 1 independent normals has a
@@ -2643,8 +2780,8 @@ int hv_domain(hv_ctx* c, int D, const char** why) {
                     recedes = sgn * dp <= 1e-12;
                 }
                 if (recedes) {
-                    *why = "the boundary planes do not enclose a bounded domain";
-                    return HV_ERR_DOMAIN;
+                    c->dom_unbounded = true;
+                    return HV_OK;
                 }
             }
         } while (hv_next_comb(S, P));
@@ -4119,23 +4256,6 @@ __global__ __launch_bounds__(256) void k_setup_rays(int n, const int* __restrict
     R.code[t] = hv_morton<D>(g, pe);
 }
 
-// box distance helpers for the per-thread (stackless, heap-order) tree walks
-template <int D>
-__device__ __forceinline__ double hv_box_d2(const DevGeo& g, int node, const double* q) {
-    double d2 = 0.0;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        const double lo = g.blo[(int64_t)k * g.NN + node], hi = g.bhi[(int64_t)k * g.NN + node];
-        const double a = fmax(fmax(lo - q[k], q[k] - hi), 0.0);
-        d2 = fma(a, a, d2);
-    }
-    return d2;
-}
-
-__device__ __forceinline__ int hv_next_node(int i) {
-    while (i & 1) i >>= 1;
-    return i == 0 ? 0 : i + 1;
-}
 
 template <int D>
 __global__ void k_nn_halfspace(DevGeo g, int nq, const double* __restrict__ q, const double* __restrict__ u,

@@ -113,10 +113,11 @@ int hv_set_points_device(hv_ctx* ctx, int d, int64_t n, const double* xs_dev);
  * d x nplanes each, normals of unit length (HV_ERR_ARG otherwise: the mirror image
  * x + 2<n, b - x> n of src/boundary.jl:206-211 assumes it); bc as the reference's Plane.BC
  * (0 Dirichlet, -1 Neumann, >0 periodic partner).  The builds take the planes of
- * cuboid(d; ...) (periodic pairs included), or Dirichlet / Neumann planes of any direction that
- * bound a convex polytope: the walk then seeds from the polytope's vertices where exactly d
- * planes meet, and planes that bound no domain (a direction with no plane in front) give
- * HV_ERR_DOMAIN at build time. */
+ * cuboid(d; ...) (periodic pairs included), or Dirichlet / Neumann planes of any direction:
+ * when they bound a convex polytope the walk also seeds from its vertices where exactly d
+ * planes meet; when they bound nothing (no planes at all: Boundary(), src/boundary.jl:133-135;
+ * or a direction with no plane in front) a ray may run to infinity: it ends without a vertex
+ * (hv_stats_t.unbounded counts them) and the integrators refuse the mesh. */
 int hv_set_boundary(hv_ctx* ctx, int nplanes, const double* base, const double* normal, const int16_t* bc);
 
 /* No counterpart in the reference (tile sizes are a device matter): the autotuner.  Builds the

@@ -10,9 +10,9 @@ Bars: the vertex keys equal the oracle's walk (src/sysvoronoi.jl) bit for bit, c
 within 1e-10; the BF16 and FP32 filters give the FP64 build's keys and coordinates; the exact
 integrals equal the oracle's polytope restatement (volumes 1e-10 relative, the same
 neighbour rows) and tile the domain (sum of volumes = the polytope's volume to 1e-12); a mesh
-of some cells and a refined mesh equal their full and fresh builds.  Planes that bound no
-domain (a wedge, parallel planes only) are refused with HV_ERR_DOMAIN, normals of other than
-unit length with HV_ERR_ARG, point samples (drawn in a box) with HV_ERR_UNSUPPORTED.
+of some cells and a refined mesh equal their full and fresh builds.  Normals of other than
+unit length are refused with HV_ERR_ARG, point samples (drawn in a box) with
+HV_ERR_UNSUPPORTED.  (Planes that bound no domain: tests/test_gpu_unbounded.py.)
 """
 import math
 
@@ -144,33 +144,15 @@ def test_cells_and_refine_on_a_cut_cube(ctx):
         mm.free()
 
 
-def test_planes_that_bound_no_domain_are_refused(ctx):
-    xs = synth.uniform(2, 50, 41) * 0.5 + 0.25
-    # a wedge: x >= 0, y >= 0, x + y <= 2 is bounded; drop the last plane: unbounded
-    wedge_b = np.array([[0.0, 0.0], [0.0, 0.0]])
-    wedge_n = np.array([[-1.0, 0.0], [0.0, -1.0]])
-    ctx.set_points(xs)
-    ctx.set_boundary(wedge_b, wedge_n)
-    with pytest.raises(HVError) as e:
-        ctx.build_mesh()
-    assert e.value.code == -2
-    # three planes whose normals do not surround the origin: x >= 0, y >= 0, x + y >= 0.2
-    b3 = np.array([[0.0, 0.0], [0.0, 0.0], [0.1, 0.1]])
-    n3 = np.array([[-1.0, 0.0], [0.0, -1.0], [-1.0, -1.0]]) / np.array([[1.0], [1.0], [np.sqrt(2)]])
-    ctx.set_boundary(b3, n3)
-    with pytest.raises(HVError) as e:
-        ctx.build_mesh()
-    assert e.value.code == -2
-    # parallel planes only (a slab)
-    ctx.set_boundary(np.array([[0.0, 1.0], [0.0, 0.0]]), np.array([[0.0, 1.0], [0.0, -1.0]]))
-    with pytest.raises(HVError) as e:
-        ctx.build_mesh()
-    assert e.value.code == -2
-    # a normal of length 2
+def test_normals_of_other_than_unit_length_are_refused(ctx):
+    ctx.set_points(synth.uniform(2, 50, 41) * 0.5 + 0.25)
     with pytest.raises(HVError) as e:
         ctx.set_boundary(np.array([[1.0, 0.0]]), np.array([[2.0, 0.0]]))
     assert e.value.code == -1
-    # the bounded triangle x >= 0, y >= 0, x + y <= 1 walks
+
+
+def test_triangle_matches_oracle(ctx):
+    # x >= 0, y >= 0, x + y <= 1
     tb = np.array([[0.0, 0.0], [0.0, 0.0], [0.5, 0.5]])
     tn = np.array([[-1.0, 0.0], [0.0, -1.0], [1.0 / np.sqrt(2), 1.0 / np.sqrt(2)]])
     xt = points(2, 300, 42, tb, tn)
