@@ -262,6 +262,42 @@ static int refine_d(hv_ctx* c, const hv_opts* o, const hv_mesh* old, hv_mesh** o
     return rc;
 }
 
+// Vertex labels of a mesh: per vertex its key with every generator id replaced by
+// (original, shift) and every plane id by its plane, sorted: the same physical vertex in the
+// same cell frame has the same label in meshes of different working sets.
+static std::vector<std::vector<uint64_t>> mesh_labels(const hv_mesh* m, int* rc) {
+    std::vector<std::vector<uint64_t>> out;
+    const int D = m->D;
+    const int64_t nv = m->nv;
+    std::vector<int64_t> sig((size_t)std::max<int64_t>(nv, 1) * (D + 1));
+    std::vector<double> x((size_t)std::max<int64_t>(nv, 1) * D);
+    *rc = nv ? hv_mesh_export(m, sig.data(), x.data()) : HV_OK;
+    if (*rc) return out;
+    const bool per = m->img_orig && m->img_shift;
+    out.resize((size_t)nv);
+    for (int64_t v = 0; v < nv; ++v) {
+        auto& row = out[(size_t)v];
+        row.resize(D + 1);
+        for (int a = 0; a <= D; ++a) {
+            const int64_t id = sig[(size_t)v * (D + 1) + a] - 1;   // 0-based
+            uint64_t code;
+            if (id >= m->N) {
+                code = (1ull << 62) | (uint64_t)(id - m->N);   // plane
+            } else {
+                const int64_t o = per ? (*m->img_orig)[(size_t)id] : id;
+                uint64_t sh = 0;
+                for (int k = 0; k < D; ++k)
+                    sh |= (uint64_t)((per ? (*m->img_shift)[(size_t)id * D + k] : 0) + 8) << (4 * k);
+                code = ((uint64_t)o << 24) | sh;
+            }
+            row[a] = code;
+        }
+        std::sort(row.begin(), row.end());
+    }
+    std::sort(out.begin(), out.end());
+    return out;
+}
+
 extern "C" {
 
 int hv_create(hv_ctx** out, int device_ordinal) {
@@ -465,13 +501,62 @@ int hv_build_mesh_cells(hv_ctx* c, const hv_opts* o, int64_t ncells, const int64
     HV_DISPATCH(c->D, hv_build_mesh_impl, c, o, out, &list)
 }
 
+// refine! of a periodic mesh (src/meshrefine.jl:183-217 on the reference's ghost nodes): the
+// originals and the new points are meshed afresh (the new points' images change the working
+// set); `removed` counts the old vertices, by label, that the new mesh does not hold
+static int refine_periodic(hv_ctx* c, const hv_opts* o, const hv_mesh* old, int64_t m, const double* xs_new,
+                           hv_mesh** out, int64_t* removed) {
+    const int D = old->D;
+    const int64_t n = old->nreal, nu = n + m;
+    if (nu >= (1ll << 30)) return hv_fail(c, HV_ERR_UNSUPPORTED, "too many generators for 32-bit keys");
+    double* u = nullptr;
+    if (hipMalloc(&u, sizeof(double) * D * nu) != hipSuccess) {
+        (void)hipGetLastError();
+        return hv_fail(c, HV_ERR_DEVICE, "out of device memory (refine)");
+    }
+    int rc = HV_OK;
+    // the originals lead the mesh's working set
+    if (hipMemcpyAsync(u, old->xw->p, sizeof(double) * D * n, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(u + D * n, xs_new, sizeof(double) * D * m, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = hv_fail(c, HV_ERR_DEVICE, "copy of the points failed");
+    if (!rc) rc = set_points_common(c, D, nu, u, hipMemcpyDeviceToDevice);
+    (void)hipFree(u);
+    if (rc) return rc;
+    rc = hv_set_boundary(c, old->P, old->P ? old->pbase.data() : nullptr, old->P ? old->pnormal.data() : nullptr,
+                         old->pbc.empty() ? nullptr : old->pbc.data());
+    if (rc) return rc;
+    hv_mesh* M = nullptr;
+    rc = hv_build_mesh(c, o, &M);
+    if (rc) return rc;
+    if (removed) {
+        int lrc = HV_OK;
+        const auto a = mesh_labels(old, &lrc);
+        const auto b = lrc ? decltype(a)() : mesh_labels(M, &lrc);
+        if (lrc) {
+            hv_mesh_free(M);
+            return hv_fail(c, lrc, "export of the meshes failed");
+        }
+        int64_t gone = 0;
+        size_t j = 0;
+        for (const auto& row : a) {
+            while (j < b.size() && b[j] < row) ++j;
+            if (j == b.size() || b[j] != row) ++gone;
+        }
+        *removed = gone;
+    }
+    *out = M;
+    return HV_OK;
+}
+
 int hv_refine_mesh(hv_ctx* c, const hv_opts* o, const hv_mesh* old, int64_t m, const double* xs_new, hv_mesh** out,
                    int64_t* removed) {
     if (!c || !old || !out || m < 1 || !xs_new) return hv_fail(c, HV_ERR_ARG, "null argument or no new points");
     *out = nullptr;
     if (old->device != c->device) return hv_fail(c, HV_ERR_ARG, "the mesh lives on another device");
-    if (old->extended || old->img_orig || old->cells)
-        return hv_fail(c, HV_ERR_UNSUPPORTED, "refinement of periodic meshes or of meshes of some cells");
+    if (old->cells) return hv_fail(c, HV_ERR_UNSUPPORTED, "refinement of a mesh of some cells");
+    if (!old->xw) return hv_fail(c, HV_ERR_STATE, "the mesh holds no generators");
+    (void)hipSetDevice(c->device);
+    if (old->extended || old->img_orig) return refine_periodic(c, o, old, m, xs_new, out, removed);
     if (!old->xw) return hv_fail(c, HV_ERR_STATE, "the mesh holds no generators");
     const int D = old->D;
     const int64_t n = old->N, nu = n + m;

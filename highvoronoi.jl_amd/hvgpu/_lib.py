@@ -247,9 +247,9 @@ class Context:
         return a.value, b.value, t.value
 
     def refine(self, mesh: "Mesh", new_points, precision: str = "fp64", leaf_size: int = 0):
-        """refine!(VG, xs) (hv_refine_mesh): the mesh of mesh's generators followed by
-        new_points[m, d] (ids n+1..n+m); the context's points become the union.  Returns
-        (mesh, number of old vertices dropped)."""
+        """refine!(VG, xs) (hv_refine_mesh): the mesh of mesh's generators (of a periodic mesh:
+        its originals) followed by new_points[m, d] (ids n+1..n+m); the context's points become
+        the union.  Returns (mesh, number of old vertices dropped)."""
         x = np.ascontiguousarray(new_points, dtype=np.float64)
         if x.ndim != 2 or x.shape[1] != mesh.d:
             raise ValueError("new_points must be [m, d]")
@@ -258,9 +258,8 @@ class Context:
         rm = C.c_int64(0)
         self._check(lib().hv_refine_mesh(self._h, C.byref(o), mesh._h, x.shape[0], _dp(x), C.byref(m), C.byref(rm)))
         out = Mesh(self, m, mesh.d)
-        n = C.c_int64(0)
-        lib().hv_mesh_images(out._h, C.byref(n), None, None)
-        self.d, self.n = mesh.d, n.value
+        orig, _ = out.images()   # periodic meshes: originals first, then images
+        self.d, self.n = mesh.d, int(orig.max()) if len(orig) else 0
         return out, rm.value
 
     def build_mesh_cells(self, cells, precision: str = "fp64", leaf_size: int = 0) -> Mesh:

@@ -196,7 +196,8 @@ class VoronoiGeometry:
         """refine!(VG, xs) (src/geometry.jl:404-438): appends the points xs[m, d] to the nodes
         (ids n+1..n+m), refines the mesh on the device (hv_refine_mesh: the old vertices whose
         circumsphere holds no new point, plus a walk of the new cells) and integrates again
-        with the same integrator.  Non-periodic boxes only.  Returns self."""
+        with the same integrator; in a periodic box the mesh of the union is built afresh (the new
+        points bring their own images).  Returns self."""
         xs = np.ascontiguousarray(np.asarray(xs, dtype=np.float64)).reshape(-1, self.nodes.shape[1])
         mesh, self.removed = self._ctx.refine(self._mesh, xs, precision=self.precision, leaf_size=self.leaf_size)
         self._mesh.free()

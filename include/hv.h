@@ -159,8 +159,11 @@ int hv_mesh_free(hv_mesh* mesh);
  * of the new cells (voronoi(mesh; Iter = new cells), src/meshrefine.jl:199).  In general
  * position the result equals hv_build_mesh of the union.  Afterwards the context holds the
  * union and the old mesh's boundary; the old mesh is unchanged and stays valid.  removed (may
- * be NULL): the number of old vertices dropped.  Periodic meshes and meshes of some cells:
- * HV_ERR_UNSUPPORTED. */
+ * be NULL): the number of old vertices dropped.  A periodic mesh (src/meshrefine.jl on the
+ * reference's ghost nodes): its n originals and the new points are meshed afresh, since the new
+ * points bring their own images; removed then counts the old vertices, by their (original,
+ * shift) labels, that the new mesh no longer holds (the same vertices the rule above drops,
+ * new points' images included).  Meshes of some cells: HV_ERR_UNSUPPORTED. */
 int hv_refine_mesh(hv_ctx* ctx, const hv_opts* opts, const hv_mesh* mesh, int64_t m, const double* xs_new,
                    hv_mesh** out, int64_t* removed);
 /* The mesh in DatabaseVertexStorage's layout (HeapDataBase, src/hvdatabase.jl:34-110, under

@@ -5,7 +5,9 @@ systematic_refine! (src/meshrefine.jl:183-217), and of the DatabaseVertexStorage
 Bars: a refined mesh has exactly the keys of a fresh build of the union (old generators, then
 the new ones) and of the oracle's walk over the union, coordinates within 1e-12 of the fresh
 build (every coordinate comes from its key) and 1e-10 of the oracle; refining twice equals one
-fresh build; the old mesh is untouched; the exact integrals of the refined mesh equal those of
+fresh build; the old mesh is untouched; a periodic mesh refined equals the periodic mesh of the union (by
+generator labels) and drops exactly the old vertices whose circumsphere holds a new point or an
+image of one; the exact integrals of the refined mesh equal those of
 the fresh build (volumes and areas to 1e-12 relative, the same neighbour rows); the database
 layout decodes back to the export and lists every (generator, vertex) incidence once, negative
 for the key's first generator.
@@ -122,18 +124,52 @@ def test_voronoi_geometry_refine(ctx):
     assert VG.nodes.shape == (n + m, d)
 
 
+@pytest.mark.parametrize("d,n,m,per,seed", [(2, 600, 40, None, 71), (3, 400, 25, [1], 72), (4, 200, 10, [1, 3], 73)])
+def test_refine_periodic_equals_fresh_build(ctx, d, n, m, per, seed):
+    # a periodic mesh refined equals the periodic mesh of the union (labels and coordinates),
+    # and `removed` counts the old vertices (by label) the refined mesh no longer holds
+    from helpers import periodic_mesh_labels
+    xs = synth.uniform(d, n + m, seed)
+    box = cuboid(d) if per is None else cuboid(d, periodic=per)
+    base, normal, bc = box.arrays()
+    ctx.set_points(xs[:n])
+    ctx.set_boundary(base, normal, bc)
+    old = ctx.build_mesh(precision="bf16")
+    lab_old = periodic_mesh_labels(old)
+    ref, removed = ctx.refine(old, xs[n:], precision="bf16")
+    assert ctx.n == n + m
+    lab_ref = periodic_mesh_labels(ref)
+    c2 = Context(0)
+    c2.set_points(xs)
+    c2.set_boundary(base, normal, bc)
+    fresh = c2.build_mesh(precision="bf16")
+    lab_fresh = periodic_mesh_labels(fresh)
+    assert set(lab_ref) == set(lab_fresh)
+    assert max(float(np.max(np.abs(lab_ref[k] - lab_fresh[k]))) for k in lab_ref) == 0.0
+    assert removed == len(set(lab_old) - set(lab_ref)) > 0
+    # the kept old vertices are those whose circumsphere holds no new point nor an image of one
+    # (clean_affected!, src/meshrefine.jl:126-150): count them from the old coordinates
+    L = 1.0
+    shifts = np.array(np.meshgrid(*[[-1, 0, 1] if (per is None or k + 1 in per) else [0] for k in range(d)],
+                                  indexing="ij")).reshape(d, -1).T * L
+    new_imgs = (xs[n:][:, None, :] + shifts[None, :, :]).reshape(-1, d)
+    keys_old = list(lab_old)
+    kept = 0
+    for k in keys_old:
+        r = lab_old[k]
+        g = k[0]   # a generator label (0, orig, *shift) sorts first
+        x0 = xs[g[1]] + np.array(g[2:]) * L
+        R = np.linalg.norm(r - x0)
+        dmin = np.min(np.linalg.norm(new_imgs - r, axis=1))
+        kept += R <= (1 + 1e-7) * dmin
+    assert kept == len(keys_old) - removed
+    for mm in (old, ref, fresh):
+        mm.free()
+
+
 def test_refine_refusals(ctx):
     d = 3
     xs = synth.uniform(d, 400, 51)
-    b = cuboid(d, periodic=[1])
-    base, normal, bc = b.arrays()
-    ctx.set_points(xs)
-    ctx.set_boundary(base, normal, bc)
-    per = ctx.build_mesh()
-    with pytest.raises(HVError) as e:
-        ctx.refine(per, synth.uniform(d, 5, 52))
-    assert e.value.code == -7
-    per.free()
     setup(ctx, xs)
     sub = ctx.build_mesh_cells([1, 2, 3])
     with pytest.raises(HVError) as e:
