@@ -2319,8 +2319,12 @@ __global__ void k_iota(int* __restrict__ a, int n) {
 // host driver
 
 constexpr int HV_DESC_LANES = 4;   // lanes per interior seed (k_seed_descent)
+// The descent and its vertex solve take about 200 VGPRs in 5D (2 wavefronts per SIMD, 6 250
+// wavefronts for 100k seeds): held to 128 registers (4 per SIMD) with some spills it runs
+// faster (5D N=100k mixture 56.9 -> 56.2 ms per build, 512 meshes 356 -> 350 ms).
+#define HV_SEED_OCC __attribute__((amdgpu_waves_per_eu(4, 8)))
 template <int D, bool B>
-__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int nseed, int stride, int nwalk,
+__global__ __launch_bounds__(64) HV_SEED_OCC void k_seed_descent(DevGeo g0, GeoSet F, int nseed, int stride, int nwalk,
                                                      const int* __restrict__ cells, int* __restrict__ vkey,
                                                      double* __restrict__ vx, int64_t base, int* __restrict__ status,
                                                      uint64_t seed);
@@ -4341,7 +4345,7 @@ __global__ void k_inrange(DevGeo g, int nq, const double* __restrict__ q, const 
 // and every HV_DESC_LANES-th point of a leaf, and the group combines them with shuffles; the
 // rest of a seed's state is held alike by its lanes, so the group's control flow is uniform.
 template <int D, bool B>
-__global__ __launch_bounds__(64) void k_seed_descent(DevGeo g0, GeoSet F, int nseed, int stride, int nwalk,
+__global__ __launch_bounds__(64) HV_SEED_OCC void k_seed_descent(DevGeo g0, GeoSet F, int nseed, int stride, int nwalk,
                                                      const int* __restrict__ cells, int* __restrict__ vkey,
                                                      double* __restrict__ vx, int64_t base, int* __restrict__ status,
                                                      uint64_t seed) {
