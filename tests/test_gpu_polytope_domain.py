@@ -3,8 +3,12 @@ Dirichlet / Neumann planes of any unit normal that bound a convex polytope.  The
 the reference's reflect (src/boundary.jl:206-211) for every plane, as for the cuboid.
 
 Domains: the unit cube with the corner at (1, ..., 1) cut off by sum(x) <= d - 1/2 (volume
-1 - 2^-d / d!), and a regular pentagon in 2D.  Every vertex of these polytopes has exactly d
-planes, so the tessellations are in general position.
+1 - 2^-d / d!), a regular pentagon and a triangle in 2D, the unit cube turned by a random
+rotation (3D, 4D: no plane normal along an axis) and the 6D simplex x >= 0, sum(x) <= 1
+(volume 1/720).  Every vertex of these polytopes has exactly d planes, so the tessellations
+are in general position.  The apex of a square pyramid has four planes: the vertex of the cell
+there has d + 2 elements, which the oracle reports as a non-general vertex and the device as
+HV_ERR_DEGENERATE.
 
 Bars: the vertex keys equal the oracle's walk (src/sysvoronoi.jl) bit for bit, coordinates
 within 1e-10; the BF16 and FP32 filters give the FP64 build's keys and coordinates; the exact
@@ -205,3 +209,60 @@ def test_batch_and_voronoi_geometry_on_polytopes(ctx):
     keys, _ = VG.vertex_table()
     ok, _, _ = Oracle(xs, base, normal).voronoi()
     assert np.array_equal(rows_view(keys), rows_view(ok))
+
+
+def rotation(d, seed):
+    q, r = np.linalg.qr(np.random.default_rng(seed).normal(size=(d, d)))
+    return q * np.sign(np.diag(r))
+
+
+@pytest.mark.parametrize("d,n,seed", [(3, 400, 81), (4, 200, 82)])
+def test_rotated_cube_matches_oracle(ctx, d, n, seed):
+    # the unit cube about (1/2, ..., 1/2) turned by a random rotation: every plane has a
+    # non-axis normal; keys equal to the oracle, volumes summing to 1
+    Q = rotation(d, seed)
+    c = np.full(d, 0.5)
+    b0, n0 = synth.cube_planes(d)
+    normal = n0 @ Q.T
+    base = (b0 - c) @ Q.T + c
+    xs = points(d, n, seed, base, normal)
+    m = build(ctx, xs, base, normal, "bf16")
+    keys, coords = m.export()
+    ok, oc, _ = Oracle(xs, base, normal).voronoi()
+    check_against_oracle(keys, coords, ok, oc)
+    assert abs(ctx.integrate(m, "polytope")["volumes"].sum() - 1.0) < 1e-12
+    m.free()
+
+
+def test_simplex_6d_matches_oracle(ctx):
+    # x_k >= 0 and sum x <= 1: volume 1/6!
+    d = 6
+    base = np.zeros((d + 1, d))
+    normal = -np.eye(d + 1, d)
+    normal[d] = np.ones(d) / np.sqrt(d)
+    base[d] = np.full(d, 1.0 / d)
+    e = np.random.default_rng(91).exponential(size=(60, d + 1))
+    xs = np.ascontiguousarray((e / e.sum(axis=1, keepdims=True))[:, :d])
+    assert inside(xs, base, normal).all()
+    m = build(ctx, xs, base, normal)
+    keys, coords = m.export()
+    ok, oc, _ = Oracle(xs, base, normal).voronoi()
+    check_against_oracle(keys, coords, ok, oc)
+    assert abs(ctx.integrate(m, "polytope")["volumes"].sum() - 1.0 / 720.0) < 1e-13
+    m.free()
+
+
+def test_pyramid_apex_is_out_of_general_position(ctx):
+    # four slanted planes meet at the apex of a square pyramid: the vertex there holds the
+    # apex's cell and four planes, d + 2 elements (the reference's non-general path); the
+    # device reports it as HV_ERR_DEGENERATE
+    s = 1.0 / np.sqrt(2.0)
+    base = np.array([[0.5, 0.5, 0.0], [0.5, 0.5, 1.0], [0.5, 0.5, 1.0], [0.5, 0.5, 1.0], [0.5, 0.5, 1.0]])
+    normal = np.array([[0, 0, -1.0], [s, 0, s], [-s, 0, s], [0, s, s], [0, -s, s]])
+    base[1:] = np.array([0.5, 0.5, 1.0])
+    xs = points(3, 300, 92, base, normal)
+    ctx.set_points(xs)
+    ctx.set_boundary(base, normal)
+    with pytest.raises(HVError) as e:
+        ctx.build_mesh()
+    assert e.value.code == -4
