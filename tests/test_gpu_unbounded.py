@@ -8,7 +8,9 @@ Bars: the vertex keys equal the oracle's walk bit for bit, coordinates within 1e
 to max(1, |x|) (far vertices reach |x| ~ 1e4); the unbounded rays are the oracle's, one per
 facet of the convex hull (scipy's Qhull); the BF16 and FP32 filters give the FP64 build's keys
 and coordinates; in 2D the vertex count is 2n - 2 - h (Euler, h hull points) at the size of
-the reference's test/fraud.jl (N = 300 000); the integrators refuse such meshes.
+the reference's test/fraud.jl (N = 300 000); the integrators refuse such meshes.  Cell subsets,
+refinement and batches work on such domains as on bounded ones (a subset equals the full
+build's vertices of its cells, a refined mesh a fresh build, a batch mesh its single build).
 """
 import numpy as np
 import pytest
@@ -116,3 +118,40 @@ def test_voronoi_geometry_without_planes(ctx):
     keys, _ = VG.vertex_table()
     ok, _, _ = Oracle(xs).voronoi()
     assert np.array_equal(rows_view(keys), rows_view(ok))
+
+
+def test_cells_refine_and_batch_without_planes(ctx):
+    # a cell subset is the full build's vertices that hold a listed cell; a refined mesh equals
+    # a fresh build of all points; a batch of unbounded meshes equals their single builds
+    from hvgpu._lib import build_mesh_batch
+    xs = synth.uniform(3, 900, 41)
+    n0 = 800
+    m = build(ctx, xs[:n0])
+    kf, _ = m.export()
+    cells = [1, 5, 17, 400]
+    sub = ctx.build_mesh_cells(cells)
+    ks, _ = sub.export()
+    assert np.array_equal(rows_view(ks), rows_view(kf[np.isin(kf, cells).any(axis=1)]))
+    sub.free()
+    ref, removed = ctx.refine(m, xs[n0:])
+    kr, cr = ref.export()
+    assert removed > 0
+    fresh = build(ctx, xs)
+    kx, cx = fresh.export()
+    assert np.array_equal(rows_view(kr), rows_view(kx))
+    row = {k: i for i, k in enumerate(map(tuple, kx.tolist()))}
+    j = np.array([row[k] for k in map(tuple, kr.tolist())])
+    assert float(np.max(np.abs(cr - cx[j]) / np.maximum(1.0, np.abs(cx[j])))) < 1e-10
+    for mm in (m, ref, fresh):
+        mm.free()
+    cases = [synth.uniform(d, n, 50 + d) for d, n in ((2, 400), (3, 300), (4, 150))]
+    out, status = build_mesh_batch(cases, [(np.zeros((0, x.shape[1])), np.zeros((0, x.shape[1]))) for x in cases],
+                                   precision="bf16")
+    assert list(status) == [0] * len(cases)
+    for mb, x in zip(out, cases):
+        single = build(ctx, x, prec="bf16")
+        k1, c1 = single.export()
+        k, c = mb.export()
+        assert np.array_equal(k, k1) and np.array_equal(c, c1)
+        single.free()
+        mb.free()
