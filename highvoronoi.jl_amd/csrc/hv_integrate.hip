@@ -261,6 +261,37 @@ __device__ __forceinline__ void hv_face_grad(const IntGeo& g, int a, int b, doub
     }
 }
 
+// Sub-face references of the incidences of a pass, written in incidence order (vertex-major:
+// a vertex's key and its row of the previous pass's incidence-to-face map are read once for
+// all its subsets, contiguously), so that the measure of a face reads one record per vertex
+// instead of the vertex's key and that row.  Entry q of incidence o (the q-th key position e
+// not in the subset, ascending): the id j = key[e] in the low word, the pass-(m + 1) map entry
+// of the subset + {e} in the high word.  Up to four entries (faces of m >= d - 3 ids).
+struct HvRec {
+    uint64_t* p[4];
+};
+
+template <int D>
+__global__ void k_face_rec(const int* __restrict__ vk, int64_t tot, int m, int C, int C1, const int* __restrict__ foi1,
+                           HvRec rec) {
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= tot) return;
+    const int64_t v = o / C;
+    const int mask = hv_mask_list.m[m][(int)(o - v * C)];
+    const int* kv = vk + v * (D + 1);
+    int q = 0;
+#pragma unroll
+    for (int e = 0; e <= D; ++e) {
+        if ((mask >> e) & 1) continue;
+        const int x = foi1[v * C1 + hv_mask_rank.r[mask | (1 << e)]];
+        const uint64_t w = (uint64_t)(uint32_t)kv[e] | ((uint64_t)(uint32_t)x << 32);
+#pragma unroll
+        for (int z = 0; z < 4; ++z)
+            if (z == q) rec.p[z][o] = w;
+        ++q;
+    }
+}
+
 // Measure of each face of pass m from the faces of pass m + 1.  L lanes per face (1, 16 or 64),
 // splitting the face's vertex list (faces of many vertices: the facets and ridges of cells in
 // 5D and up), the partial sums reduced in a fixed order.  (A dispatch holds at most 2^32
@@ -272,6 +303,7 @@ __device__ __forceinline__ void hv_face_measure_one(int f, int lane, int step, c
                                                     int nface1, int C1, const int* __restrict__ foi1,
                                                     const double* __restrict__ meas1,
                                                     const int* __restrict__ vk, const double* __restrict__ vc,
+                                                    const int* __restrict__ perm, const HvRec& rec,
                                                     double* __restrict__ meas, int* __restrict__ bad) {
     const uint64_t lo = flo[f], hi = fhi[f];
     int S[D + 1];
@@ -287,7 +319,9 @@ __device__ __forceinline__ void hv_face_measure_one(int f, int lane, int step, c
         double w[D];
         hv_face_grad<D>(g, a, S[s], w);
         for (int pass = 0; pass < 2; ++pass)
-            for (int r = 0; r < q; ++r) {
+#pragma unroll
+            for (int r = 0; r < D - 1; ++r) {   // r < q, unrolled: Q stays in registers
+                if (r >= q) break;
                 double cdot = 0.0;
 #pragma unroll
                 for (int k = 0; k < D; ++k) cdot = fma(w[k], Q[r][k], cdot);
@@ -303,7 +337,7 @@ __device__ __forceinline__ void hv_face_measure_one(int f, int lane, int step, c
             return;
         }
 #pragma unroll
-        for (int k = 0; k < D; ++k) Q[q][k] = w[k] / nw;
+        for (int k = 0; k < D; ++k) Q[s - 1][k] = w[k] / nw;   // q == s - 1
         ++q;
     }
     const int b0 = fstart[f], e0 = fstart[f + 1];
@@ -317,61 +351,76 @@ __device__ __forceinline__ void hv_face_measure_one(int f, int lane, int step, c
         da = fma(t, t, da);
     }
     double sum = 0.0;
-    for (int i = b0 + lane; i < e0; i += step) {
-        const int v = vid[i];
-        const int* kv = vk + (int64_t)v * (D + 1);
-        int kr[D + 1], smask = 0;
+    // the sub-face T = S + {j} of the face, met at one of its vertices; x: T's entry in the
+    // incidence-to-face map of pass m + 1
+    auto subface = [&](int j, int x) {
+        bool skip = false;
 #pragma unroll
-        for (int e = 0; e <= D; ++e) {
-            kr[e] = kv[e];
-            bool in = false;
+        for (int s = 0; s <= D; ++s) skip = skip || (kp[s] == j);
+        if (skip) return;   // the sub-face holds the apex (zero height)
+        const int t1 = x >= 0 ? x : ~x;
+        if (t1 >= nface1) { atomicOr(bad, 4); return; }
+        // each sub-face once: from the first of its vertices (lists ascend by vertex)
+        if (x < 0) return;
+        double w[D];
+        hv_face_grad<D>(g, a, j, w);
+        for (int pass = 0; pass < 2; ++pass)
 #pragma unroll
-            for (int s = 0; s <= D; ++s) in = in || (S[s] == kr[e]);
-            smask |= in ? 1 << e : 0;
-        }
-        for (int e = 0; e <= D; ++e) {
-            if ((smask >> e) & 1) continue;   // j in S
-            const int j = kr[e];
-            bool skip = false;
+            for (int r = 0; r < D - 1; ++r) {   // r < q, unrolled: Q stays in registers
+                if (r >= q) break;
+                double cdot = 0.0;
 #pragma unroll
-            for (int s = 0; s <= D; ++s) skip = skip || (kp[s] == j);
-            if (skip) continue;   // the sub-face holds the apex (zero height)
-            // T = S + {j}: the subset smask + e of this vertex's key in pass m + 1
-            const int x = foi1[(int64_t)v * C1 + hv_mask_rank.r[smask | (1 << e)]];
-            const int t1 = x >= 0 ? x : ~x;
-            if (t1 >= nface1) { atomicOr(bad, 4); continue; }
-            // each sub-face once: from the first of its vertices (lists ascend by vertex)
-            if (x < 0) continue;
-            double w[D];
-            hv_face_grad<D>(g, a, j, w);
-            for (int pass = 0; pass < 2; ++pass)
-                for (int r = 0; r < q; ++r) {
-                    double cdot = 0.0;
+                for (int k = 0; k < D; ++k) cdot = fma(w[k], Q[r][k], cdot);
 #pragma unroll
-                    for (int k = 0; k < D; ++k) cdot = fma(w[k], Q[r][k], cdot);
-#pragma unroll
-                    for (int k = 0; k < D; ++k) w[k] = fma(-cdot, Q[r][k], w[k]);
-                }
-            double nr = 0.0;
-#pragma unroll
-            for (int k = 0; k < D; ++k) nr = fma(w[k], w[k], nr);
-            nr = sqrt(nr);
-            double h;
-            if (j >= g.N) {
-                double np_ = 0.0;
-#pragma unroll
-                for (int k = 0; k < D; ++k) np_ = fma(g.pn[(j - g.N) * D + k], p[k], np_);
-                h = fabs(g.ph[j - g.N] - np_) / nr;
-            } else {
-                double dj = 0.0;
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    const double t = p[k] - g.xo[(int64_t)j * D + k];
-                    dj = fma(t, t, dj);
-                }
-                h = fabs(dj - da) / (2.0 * nr);
+                for (int k = 0; k < D; ++k) w[k] = fma(-cdot, Q[r][k], w[k]);
             }
-            sum = fma(h, meas1[t1], sum);
+        double nr = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) nr = fma(w[k], w[k], nr);
+        nr = sqrt(nr);
+        double h;
+        if (j >= g.N) {
+            double np_ = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) np_ = fma(g.pn[(j - g.N) * D + k], p[k], np_);
+            h = fabs(g.ph[j - g.N] - np_) / nr;
+        } else {
+            double dj = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const double t = p[k] - g.xo[(int64_t)j * D + k];
+                dj = fma(t, t, dj);
+            }
+            h = fabs(dj - da) / (2.0 * nr);
+        }
+        sum = fma(h, meas1[t1], sum);
+    };
+    if (rec.p[0]) {
+        for (int i = b0 + lane; i < e0; i += step) {
+            const int64_t o = perm[i];
+#pragma unroll
+            for (int z = 0; z < 4; ++z) {
+                if (z > D - m) break;
+                const uint64_t rw = rec.p[z][o];
+                subface((int)(uint32_t)rw, (int)(uint32_t)(rw >> 32));
+            }
+        }
+    } else {
+        for (int i = b0 + lane; i < e0; i += step) {
+            const int v = vid[i];
+            const int* kv = vk + (int64_t)v * (D + 1);
+            int kr[D + 1], smask = 0;
+#pragma unroll
+            for (int e = 0; e <= D; ++e) {
+                kr[e] = kv[e];
+                bool in = false;
+#pragma unroll
+                for (int s = 0; s <= D; ++s) in = in || (S[s] == kr[e]);
+                smask |= in ? 1 << e : 0;
+            }
+            for (int e = 0; e <= D; ++e)
+                if (!((smask >> e) & 1))   // j not in S: T is the subset smask + e in pass m + 1
+                    subface(kr[e], foi1[(int64_t)v * C1 + hv_mask_rank.r[smask | (1 << e)]]);
         }
     }
     if constexpr (L > 1) {
@@ -386,17 +435,18 @@ template <int D, int L>
 __global__ __launch_bounds__(128) void k_face_measure(IntGeo g, int m, int nface, const int* __restrict__ fstart, const int* __restrict__ vid,
                                const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi, int nface1, int C1,
                                const int* __restrict__ foi1, const double* __restrict__ meas1,
-                               const int* __restrict__ vk, const double* __restrict__ vc, double* __restrict__ meas,
+                               const int* __restrict__ vk, const double* __restrict__ vc,
+                               const int* __restrict__ perm, HvRec rec, double* __restrict__ meas,
                                int* __restrict__ bad) {
     if constexpr (L > 1) {   // one wavefront per workgroup, 64 / L faces at a time
         for (int64_t f = (int64_t)blockIdx.x * (64 / L) + threadIdx.x / L; f < nface; f += (int64_t)gridDim.x * (64 / L))
             hv_face_measure_one<D, L>((int)f, (int)(threadIdx.x % L), L, g, m, fstart, vid, flo, fhi, nface1, C1,
-                                      foi1, meas1, vk, vc, meas, bad);
+                                      foi1, meas1, vk, vc, perm, rec, meas, bad);
     } else {
         const int f = blockIdx.x * blockDim.x + threadIdx.x;
         if (f < nface)
-            hv_face_measure_one<D, 1>(f, 0, 1, g, m, fstart, vid, flo, fhi, nface1, C1, foi1, meas1, vk, vc, meas,
-                                      bad);
+            hv_face_measure_one<D, 1>(f, 0, 1, g, m, fstart, vid, flo, fhi, nface1, C1, foi1, meas1, vk, vc, perm,
+                                      rec, meas, bad);
     }
 }
 
@@ -467,7 +517,9 @@ __device__ __forceinline__ void hv_face_int_one(int f, int lane, int step, const
         double w[D];
         hv_face_grad<D>(g, a, S[s], w);
         for (int pass = 0; pass < 2; ++pass)
-            for (int r = 0; r < q; ++r) {
+#pragma unroll
+            for (int r = 0; r < D - 1; ++r) {   // r < q, unrolled: Q stays in registers
+                if (r >= q) break;
                 double cdot = 0.0;
 #pragma unroll
                 for (int k = 0; k < D; ++k) cdot = fma(w[k], Q[r][k], cdot);
@@ -483,7 +535,7 @@ __device__ __forceinline__ void hv_face_int_one(int f, int lane, int step, const
             return;
         }
 #pragma unroll
-        for (int k = 0; k < D; ++k) Q[q][k] = w[k] / nw;
+        for (int k = 0; k < D; ++k) Q[s - 1][k] = w[k] / nw;   // q == s - 1
         ++q;
     }
     double p[D], da = 0.0;
@@ -518,7 +570,9 @@ __device__ __forceinline__ void hv_face_int_one(int f, int lane, int step, const
             double w[D];
             hv_face_grad<D>(g, a, j, w);
             for (int pass = 0; pass < 2; ++pass)
-                for (int r = 0; r < q; ++r) {
+#pragma unroll
+                for (int r = 0; r < D - 1; ++r) {   // r < q, unrolled: Q stays in registers
+                    if (r >= q) break;
                     double cdot = 0.0;
 #pragma unroll
                     for (int k = 0; k < D; ++k) cdot = fma(w[k], Q[r][k], cdot);
@@ -1365,8 +1419,18 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
                 const double* meas1 = (const double*)c->pool->get(n1("i_meas"), 0);
                 if (!foi1 || !meas1) return hv_fail(c, HV_ERR_DEVICE, "face tables of the previous pass missing");
                 const int L = hv_face_lanes(D, m);
+                // sub-face records in the sort's key buffers, free once the face table is out
+                HvRec rec{{nullptr, nullptr, nullptr, nullptr}};
+                if (D - m <= 3) {   // facets in 6D (five entries) read the keys directly
+                    uint64_t* kb[4] = {lo, hi2, lo2, hi};
+                    for (int z = 0; z <= D - m; ++z) rec.p[z] = kb[z];
+                    k_face_rec<D><<<nblk(std::max<int64_t>(tot, 1), 256), 256, 0, c->stream>>>(
+                        (const int*)vkc, tot, m, C, C1, foi1, rec);
+                    HVCK(hipGetLastError());
+                }
 #define HV_FM(LL) hv_face_launch<LL>(k_face_measure<D, LL>, nface, c->stream, g, m, nface, fstart, vid, flo, fhi, \
-                                     nface1, C1, foi1, meas1, (const int*)vkc, (const double*)vcc, meas, bad)
+                                     nface1, C1, foi1, meas1, (const int*)vkc, (const double*)vcc, (const int*)perm, \
+                                     rec, meas, bad)
                 if (L == 64) HV_FM(64);
                 else if (L == 16) HV_FM(16);
                 else HV_FM(1);
