@@ -1359,11 +1359,9 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
                 perm = pa;
                 k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(lo, perm, tot, lo2);
                 HVCK(hipGetLastError());
-            } else {
-                k_gather_u64<<<nblk(tot, 256), 256, 0, c->stream>>>(hi, perm, tot, hi2);
-                HVCK(hipGetLastError());
-                HVCK(hipMemcpyAsync(hi, hi2, sizeof(uint64_t) * tot, hipMemcpyDeviceToDevice, c->stream));
             }
+            // (m <= 3: every key's hi word is all ones, no id past the third field, so hi needs
+            // no reordering)
             // lo2 / hi: sorted keys; perm: sorted incidence numbers
             k_first_sent<<<1, 64, 0, c->stream>>>(lo2, tot, cnt);
             HVCK(hipGetLastError());
