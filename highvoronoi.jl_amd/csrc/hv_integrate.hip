@@ -503,6 +503,7 @@ __device__ __forceinline__ void hv_face_int_one(int f, int lane, int step, const
                                                 const double* __restrict__ cen, int nface1, int C1,
                                                 const int* __restrict__ foi1,
                                                 const double* __restrict__ J1, int nf, const int* __restrict__ vk,
+                                                const int* __restrict__ perm, const HvRec& rec,
                                                 double* __restrict__ J, int* __restrict__ bad) {
     const uint64_t lo = flo[f], hi = fhi[f];
     int S[D + 1];
@@ -548,57 +549,70 @@ __device__ __forceinline__ void hv_face_int_one(int f, int lane, int step, const
     double sum[HV_MAXF];
     for (int c = 0; c < nf; ++c) sum[c] = 0.0;
     const int b0 = fstart[f], e0 = fstart[f + 1];
-    for (int i = b0 + lane; i < e0; i += step) {
-        const int v = vid[i];
-        const int* kv = vk + (int64_t)v * (D + 1);
-        int kr[D + 1], smask = 0;
+    // the sub-face T = S + {j}; x: T's entry in the incidence-to-face map of pass m + 1
+    auto subface = [&](int j, int x) {
+        const int t1 = x >= 0 ? x : ~x;
+        if (t1 >= nface1) { atomicOr(bad, 4); return; }
+        if (x < 0) return;   // each sub-face once
+        double w[D];
+        hv_face_grad<D>(g, a, j, w);
+        for (int pass = 0; pass < 2; ++pass)
 #pragma unroll
-        for (int e = 0; e <= D; ++e) {
-            kr[e] = kv[e];
-            bool in = false;
+            for (int r = 0; r < D - 1; ++r) {   // r < q, unrolled: Q stays in registers
+                if (r >= q) break;
+                double cdot = 0.0;
 #pragma unroll
-            for (int s = 0; s <= D; ++s) in = in || (S[s] == kr[e]);
-            smask |= in ? 1 << e : 0;
-        }
-        for (int e = 0; e <= D; ++e) {
-            if ((smask >> e) & 1) continue;
-            const int j = kr[e];
-            const int x = foi1[(int64_t)v * C1 + hv_mask_rank.r[smask | (1 << e)]];
-            const int t1 = x >= 0 ? x : ~x;
-            if (t1 >= nface1) { atomicOr(bad, 4); continue; }
-            if (x < 0) continue;   // each sub-face once
-            double w[D];
-            hv_face_grad<D>(g, a, j, w);
-            for (int pass = 0; pass < 2; ++pass)
+                for (int k = 0; k < D; ++k) cdot = fma(w[k], Q[r][k], cdot);
 #pragma unroll
-                for (int r = 0; r < D - 1; ++r) {   // r < q, unrolled: Q stays in registers
-                    if (r >= q) break;
-                    double cdot = 0.0;
-#pragma unroll
-                    for (int k = 0; k < D; ++k) cdot = fma(w[k], Q[r][k], cdot);
-#pragma unroll
-                    for (int k = 0; k < D; ++k) w[k] = fma(-cdot, Q[r][k], w[k]);
-                }
-            double nr = 0.0;
-#pragma unroll
-            for (int k = 0; k < D; ++k) nr = fma(w[k], w[k], nr);
-            nr = sqrt(nr);
-            double h;
-            if (j >= g.N) {
-                double np_ = 0.0;
-#pragma unroll
-                for (int k = 0; k < D; ++k) np_ = fma(g.pn[(j - g.N) * D + k], p[k], np_);
-                h = fabs(g.ph[j - g.N] - np_) / nr;
-            } else {
-                double dj = 0.0;
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    const double t = p[k] - g.xo[(int64_t)j * D + k];
-                    dj = fma(t, t, dj);
-                }
-                h = fabs(dj - da) / (2.0 * nr);
+                for (int k = 0; k < D; ++k) w[k] = fma(-cdot, Q[r][k], w[k]);
             }
-            for (int c = 0; c < nf; ++c) sum[c] = fma(h, J1[(int64_t)t1 * nf + c], sum[c]);
+        double nr = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) nr = fma(w[k], w[k], nr);
+        nr = sqrt(nr);
+        double h;
+        if (j >= g.N) {
+            double np_ = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) np_ = fma(g.pn[(j - g.N) * D + k], p[k], np_);
+            h = fabs(g.ph[j - g.N] - np_) / nr;
+        } else {
+            double dj = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const double t = p[k] - g.xo[(int64_t)j * D + k];
+                dj = fma(t, t, dj);
+            }
+            h = fabs(dj - da) / (2.0 * nr);
+        }
+        for (int c = 0; c < nf; ++c) sum[c] = fma(h, J1[(int64_t)t1 * nf + c], sum[c]);
+    };
+    if (rec.p[0]) {
+        for (int i = b0 + lane; i < e0; i += step) {
+            const int64_t o = perm[i];
+#pragma unroll
+            for (int z = 0; z < 4; ++z) {
+                if (z > D - m) break;
+                const uint64_t rw = rec.p[z][o];
+                subface((int)(uint32_t)rw, (int)(uint32_t)(rw >> 32));
+            }
+        }
+    } else {
+        for (int i = b0 + lane; i < e0; i += step) {
+            const int v = vid[i];
+            const int* kv = vk + (int64_t)v * (D + 1);
+            int kr[D + 1], smask = 0;
+#pragma unroll
+            for (int e = 0; e <= D; ++e) {
+                kr[e] = kv[e];
+                bool in = false;
+#pragma unroll
+                for (int s = 0; s <= D; ++s) in = in || (S[s] == kr[e]);
+                smask |= in ? 1 << e : 0;
+            }
+            for (int e = 0; e <= D; ++e)
+                if (!((smask >> e) & 1))
+                    subface(kr[e], foi1[(int64_t)v * C1 + hv_mask_rank.r[smask | (1 << e)]]);
         }
     }
     if constexpr (L > 1) {
@@ -615,16 +629,17 @@ template <int D, int L>
 __global__ __launch_bounds__(128) void k_face_int(IntGeo g, int m, int nface, const int* __restrict__ fstart, const int* __restrict__ vid,
                            const uint64_t* __restrict__ flo, const uint64_t* __restrict__ fhi,
                            const double* __restrict__ cen, int nface1, int C1, const int* __restrict__ foi1,
-                           const double* __restrict__ J1, int nf, const int* __restrict__ vk, double* __restrict__ J,
-                           int* __restrict__ bad) {
+                           const double* __restrict__ J1, int nf, const int* __restrict__ vk,
+                           const int* __restrict__ perm, HvRec rec, double* __restrict__ J, int* __restrict__ bad) {
     if constexpr (L > 1) {
         for (int64_t f = (int64_t)blockIdx.x * (64 / L) + threadIdx.x / L; f < nface; f += (int64_t)gridDim.x * (64 / L))
             hv_face_int_one<D, L>((int)f, (int)(threadIdx.x % L), L, g, m, fstart, vid, flo, fhi, cen, nface1, C1,
-                                  foi1, J1, nf, vk, J, bad);
+                                  foi1, J1, nf, vk, perm, rec, J, bad);
     } else {
         const int f = blockIdx.x * blockDim.x + threadIdx.x;
         if (f < nface)
-            hv_face_int_one<D, 1>(f, 0, 1, g, m, fstart, vid, flo, fhi, cen, nface1, C1, foi1, J1, nf, vk, J, bad);
+            hv_face_int_one<D, 1>(f, 0, 1, g, m, fstart, vid, flo, fhi, cen, nface1, C1, foi1, J1, nf, vk, perm, rec,
+                                  J, bad);
     }
 }
 
@@ -1441,7 +1456,8 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
                     k_face_center<D><<<nblk(nface, 256), 256, 0, c->stream>>>(fstart, nface, vid, vcc, cen);
                     HVCK(hipGetLastError());
 #define HV_FI(LL) hv_face_launch<LL>(k_face_int<D, LL>, nface, c->stream, g, m, nface, fstart, vid, flo, fhi, \
-                                     (const double*)cen, nface1, C1, foi1, J1, nf, (const int*)vkc, J, bad)
+                                     (const double*)cen, nface1, C1, foi1, J1, nf, (const int*)vkc, \
+                                     (const int*)perm, rec, J, bad)
                     if (L == 64) HV_FI(64);
                     else if (L == 16) HV_FI(16);
                     else HV_FI(1);
