@@ -266,9 +266,11 @@ __device__ __forceinline__ void hv_face_grad(const IntGeo& g, int a, int b, doub
 // all its subsets, contiguously), so that the measure of a face reads one record per vertex
 // instead of the vertex's key and that row.  Entry q of incidence o (the q-th key position e
 // not in the subset, ascending): the id j = key[e] in the low word, the pass-(m + 1) map entry
-// of the subset + {e} in the high word.  Up to four entries (faces of m >= d - 3 ids).
+// of the subset + {e} in the high word; the ne = d + 1 - m entries of an incidence lie together
+// at p[o * ne] (faces of m >= d - 3 ids: at most four entries, in the four key arrays' space).
 struct HvRec {
-    uint64_t* p[4];
+    uint64_t* p;
+    int ne;
 };
 
 template <int D>
@@ -284,10 +286,7 @@ __global__ void k_face_rec(const int* __restrict__ vk, int64_t tot, int m, int C
     for (int e = 0; e <= D; ++e) {
         if ((mask >> e) & 1) continue;
         const int x = foi1[v * C1 + hv_mask_rank.r[mask | (1 << e)]];
-        const uint64_t w = (uint64_t)(uint32_t)kv[e] | ((uint64_t)(uint32_t)x << 32);
-#pragma unroll
-        for (int z = 0; z < 4; ++z)
-            if (z == q) rec.p[z][o] = w;
+        rec.p[o * rec.ne + q] = (uint64_t)(uint32_t)kv[e] | ((uint64_t)(uint32_t)x << 32);
         ++q;
     }
 }
@@ -395,13 +394,13 @@ __device__ __forceinline__ void hv_face_measure_one(int f, int lane, int step, c
         }
         sum = fma(h, meas1[t1], sum);
     };
-    if (rec.p[0]) {
+    if (rec.p) {
         for (int i = b0 + lane; i < e0; i += step) {
-            const int64_t o = perm[i];
+            const uint64_t* ro = rec.p + (int64_t)perm[i] * rec.ne;
 #pragma unroll
             for (int z = 0; z < 4; ++z) {
-                if (z > D - m) break;
-                const uint64_t rw = rec.p[z][o];
+                if (z >= rec.ne) break;
+                const uint64_t rw = ro[z];
                 subface((int)(uint32_t)rw, (int)(uint32_t)(rw >> 32));
             }
         }
@@ -587,13 +586,13 @@ __device__ __forceinline__ void hv_face_int_one(int f, int lane, int step, const
         }
         for (int c = 0; c < nf; ++c) sum[c] = fma(h, J1[(int64_t)t1 * nf + c], sum[c]);
     };
-    if (rec.p[0]) {
+    if (rec.p) {
         for (int i = b0 + lane; i < e0; i += step) {
-            const int64_t o = perm[i];
+            const uint64_t* ro = rec.p + (int64_t)perm[i] * rec.ne;
 #pragma unroll
             for (int z = 0; z < 4; ++z) {
-                if (z > D - m) break;
-                const uint64_t rw = rec.p[z][o];
+                if (z >= rec.ne) break;
+                const uint64_t rw = ro[z];
                 subface((int)(uint32_t)rw, (int)(uint32_t)(rw >> 32));
             }
         }
@@ -1278,7 +1277,7 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
         if (tmax >= (1ll << 31)) return hv_fail(c, HV_ERR_CAPACITY, "too many face incidences for one pass");
         const size_t T = (size_t)std::max<int64_t>(tmax, 1);
         bool ok = true;
-        for (const char* nmv : {"i_lo", "i_hi", "i_lo2", "i_hi2"}) ok = ok && c->pool->reserve(nmv, 8 * T, c->stream);
+        ok = ok && c->pool->reserve("i_keys", 32 * T, c->stream);   // lo, hi, lo2, hi2 of a pass
         for (const char* nmv : {"i_pa", "i_pb", "i_iota", "i_vida", "i_vidb", "i_foia", "i_foib"})
             ok = ok && c->pool->reserve(nmv, 4 * T, c->stream);
         ok = ok && c->pool->reserve("i_head", T, c->stream);
@@ -1289,7 +1288,7 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
         }
         if (!ok) return hv_fail(c, HV_ERR_DEVICE, "out of device memory (faces)");
         size_t tb = 0, tb2 = 0;
-        uint64_t* k0 = (uint64_t*)c->pool->get("i_lo", 0);
+        uint64_t* k0 = (uint64_t*)c->pool->get("i_keys", 0);
         HVCK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k0, (int*)nullptr, (int*)nullptr, (int)tmax, 0, 64, c->stream));
         HVCK(hipcub::DeviceSelect::Flagged(nullptr, tb2, (int*)nullptr, (uint8_t*)nullptr, (int*)nullptr, cnt + 1, (int)tmax, c->stream));
         if (!c->pool->reserve("i_tmp", tb, c->stream) || !c->pool->reserve("i_tmp2", tb2, c->stream))
@@ -1344,10 +1343,13 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
             if (tot >= (1ll << 31)) return hv_fail(c, HV_ERR_CAPACITY, "too many face incidences for one pass");
             const char* sfx = (m & 1) ? "a" : "b";
             auto nm = [&](const char* s) { return std::string(s) + sfx; };
-            uint64_t* lo = iget<uint64_t>(c, "i_lo", tot);
-            uint64_t* hi = iget<uint64_t>(c, "i_hi", tot);
-            uint64_t* lo2 = iget<uint64_t>(c, "i_lo2", tot);
-            uint64_t* hi2 = iget<uint64_t>(c, "i_hi2", tot);
+            // the pass's four key arrays in one allocation (their space holds the sub-face
+            // records once the face table is out)
+            uint64_t* keys = iget<uint64_t>(c, "i_keys", 4 * (size_t)tot);
+            uint64_t* lo = keys;
+            uint64_t* hi = keys ? keys + tot : nullptr;
+            uint64_t* lo2 = keys ? keys + 2 * tot : nullptr;
+            uint64_t* hi2 = keys ? keys + 3 * tot : nullptr;
             int* pa = iget<int>(c, "i_pa", tot);
             int* pb = iget<int>(c, "i_pb", tot);
             uint8_t* head = iget<uint8_t>(c, "i_head", tot);
@@ -1432,11 +1434,11 @@ static int facet_rows(hv_ctx* c, const hv_mesh* mesh, const IntGeo& g0, uint64_t
                 const double* meas1 = (const double*)c->pool->get(n1("i_meas"), 0);
                 if (!foi1 || !meas1) return hv_fail(c, HV_ERR_DEVICE, "face tables of the previous pass missing");
                 const int L = hv_face_lanes(D, m);
-                // sub-face records in the sort's key buffers, free once the face table is out
-                HvRec rec{{nullptr, nullptr, nullptr, nullptr}};
+                // sub-face records in the key arrays' space, free once the face table is out
+                HvRec rec{nullptr, 0};
                 if (D - m <= 3) {   // facets in 6D (five entries) read the keys directly
-                    uint64_t* kb[4] = {lo, hi2, lo2, hi};
-                    for (int z = 0; z <= D - m; ++z) rec.p[z] = kb[z];
+                    rec.p = keys;
+                    rec.ne = D + 1 - m;
                     k_face_rec<D><<<nblk(std::max<int64_t>(tot, 1), 256), 256, 0, c->stream>>>(
                         (const int*)vkc, tot, m, C, C1, foi1, rec);
                     HVCK(hipGetLastError());
