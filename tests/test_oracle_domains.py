@@ -111,3 +111,15 @@ def test_planted_missing_plane_is_caught():
     keys, cen = circumcentres(xs)
     inner = np.all(ok <= len(xs), axis=1)
     assert key_set(ok[inner]) != key_set(keys[inside(cen, base, normal)])
+
+
+def test_pyramid_apex_is_out_of_general_position():
+    # four slanted planes through the apex of a square pyramid: the vertex of the apex's cell
+    # there is equidistant from the generator and its four mirrors (d + 2 members), which the
+    # oracle's walk reports instead of continuing (its FastEdgeIterator path is not restated)
+    s = 1.0 / np.sqrt(2.0)
+    base = np.array([[0.5, 0.5, 0.0]] + [[0.5, 0.5, 1.0]] * 4)
+    normal = np.array([[0, 0, -1.0], [s, 0, s], [-s, 0, s], [0, s, s], [0, -s, s]])
+    xs = sample(3, 300, 92, base, normal)
+    with pytest.raises(RuntimeError, match="non-general"):
+        Oracle(xs, base, normal).voronoi()
