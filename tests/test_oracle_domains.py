@@ -123,3 +123,20 @@ def test_pyramid_apex_is_out_of_general_position():
     xs = sample(3, 300, 92, base, normal)
     with pytest.raises(RuntimeError, match="non-general"):
         Oracle(xs, base, normal).voronoi()
+
+
+@pytest.mark.parametrize("d,n,seed", [(2, 40, 21), (3, 25, 22)])
+def test_integrand_rule_exact_for_affine_f_on_the_cut_cube(d, n, seed):
+    # the reference's quadrature is exact for affine f up to d = 3: the cells' integrals of
+    # x_k add up to the cut cube's, 1/2 - (c - c / (2 (d + 1))) with c = 2^-d / d! the volume
+    # of the corner cut off (y = 1 - x ranges over a simplex of edge 1/2, centroid 1/(2(d+1)))
+    from oracle_py import polytope_integrand
+    base, normal = cut_cube(d)
+    xs = sample(d, n, seed, base, normal)
+    keys, coords, _ = Oracle(xs, base, normal).voronoi()
+    vol, _, _, _ = polytope(xs, base, normal, keys, coords)
+    f = lambda p: np.concatenate([np.ones((len(p), 1)), p], axis=1)
+    bulk, _ = polytope_integrand(xs, base, normal, keys, coords, f, d + 1)
+    assert np.allclose(bulk[:, 0], vol, rtol=1e-11, atol=0)
+    c = 0.5 ** d / math.factorial(d)
+    assert np.allclose(bulk[:, 1:].sum(axis=0), 0.5 - (c - c / (2 * (d + 1))), atol=1e-12)
