@@ -14,7 +14,8 @@ Bars: the vertex keys equal the oracle's walk (src/sysvoronoi.jl) bit for bit, c
 within 1e-10; the BF16 and FP32 filters give the FP64 build's keys and coordinates; the exact
 integrals equal the oracle's polytope restatement (volumes 1e-10 relative, the same
 neighbour rows) and tile the domain (sum of volumes = the polytope's volume to 1e-12); a mesh
-of some cells and a refined mesh equal their full and fresh builds.  Normals of other than
+of some cells and a refined mesh equal their full and fresh builds; integrands (f = 1, x)
+equal the oracle's rule to 1e-10 and the cut cube's closed-form first moments to 1e-12.  Normals of other than
 unit length are refused with HV_ERR_ARG, point samples (drawn in a box) with
 HV_ERR_UNSUPPORTED.  (Planes that bound no domain: tests/test_gpu_unbounded.py.)
 """
@@ -266,3 +267,21 @@ def test_pyramid_apex_is_out_of_general_position(ctx):
     with pytest.raises(HVError) as e:
         ctx.build_mesh()
     assert e.value.code == -4
+
+
+@pytest.mark.parametrize("d,n,seed", [(2, 40, 21), (3, 25, 22)])
+def test_integrand_on_a_cut_cube_matches_oracle_and_closed_form(ctx, d, n, seed):
+    # f = (1, x): the device's bulk integrals equal the oracle's restatement of the reference
+    # rule (1e-10 relative), and the first moments add up to the cut cube's closed form
+    from oracle_py import polytope_integrand
+    base, normal = cut_cube(d)
+    xs = points(d, n, seed, base, normal)
+    m = build(ctx, xs, base, normal)
+    f = lambda p: np.concatenate([np.ones((len(p), 1)), p], axis=1)
+    r = ctx.integrate(m, "polytope", integrand=f, nf=d + 1)
+    m.free()
+    keys, coords, _ = Oracle(xs, base, normal).voronoi()
+    ob, _ = polytope_integrand(xs, base, normal, keys, coords, f, d + 1)
+    assert np.allclose(r["bulk"], ob, rtol=1e-10, atol=1e-14)
+    c = 0.5 ** d / math.factorial(d)
+    assert np.allclose(r["bulk"][:, 1:].sum(axis=0), 0.5 - (c - c / (2 * (d + 1))), atol=1e-12)
